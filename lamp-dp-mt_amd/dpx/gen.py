@@ -1,0 +1,75 @@
+"""Seeded synthetic inputs for the DP workloads.
+
+A self-contained splitmix64 stream, so that every sequence set is the same
+on any machine and numpy version.  Generators:
+  rna_uniform(n, seed)        i.i.d. uniform over ACGU
+  rna_gc(n, seed, gc=0.5)     i.i.d. with P(G)=P(C)=gc/2, P(A)=P(U)=(1-gc)/2
+  dna_uniform(n, seed)        i.i.d. uniform over ACGT
+  log_uniform_lengths(k, lo, hi, seed)
+  matrix_chain(n, seed, lo=1, hi=1000)  consistent (rows, cols) pairs
+"""
+import math
+
+MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.state = (seed * 0x9E3779B97F4A7C15 + 0x1234567) & MASK
+
+    def next(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / (1 << 53))
+
+    def below(self, m):
+        return self.next() % m
+
+
+def rna_uniform(n, seed):
+    r = SplitMix64(seed)
+    return "".join("ACGU"[r.below(4)] for _ in range(n))
+
+
+def rna_gc(n, seed, gc=0.5):
+    r = SplitMix64(seed)
+    out = []
+    for _ in range(n):
+        u = r.uniform()
+        if u < gc / 2:
+            out.append("G")
+        elif u < gc:
+            out.append("C")
+        elif u < gc + (1 - gc) / 2:
+            out.append("A")
+        else:
+            out.append("U")
+    return "".join(out)
+
+
+def dna_uniform(n, seed):
+    r = SplitMix64(seed)
+    return "".join("ACGT"[r.below(4)] for _ in range(n))
+
+
+def log_uniform_lengths(k, lo, hi, seed):
+    r = SplitMix64(seed)
+    a, b = math.log(lo), math.log(hi)
+    return [min(hi, max(lo, int(round(math.exp(a + (b - a) * r.uniform()))))) for _ in range(k)]
+
+
+def rna_batch(k, lo, hi, seed, gc=0.5):
+    """k sequences with log-uniform lengths in [lo,hi] (the batched-fold set)."""
+    lens = log_uniform_lengths(k, lo, hi, seed)
+    return [rna_gc(n, seed * 1000003 + q + 1, gc) for q, n in enumerate(lens)]
+
+
+def matrix_chain(n, seed, lo=1, hi=1000):
+    r = SplitMix64(seed)
+    dims = [lo + r.below(hi - lo + 1) for _ in range(n + 1)]
+    return dims[:-1], dims[1:]
