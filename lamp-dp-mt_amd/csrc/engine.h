@@ -1,0 +1,98 @@
+// engine.h -- data layout shared by the host launcher and the GPU kernels.
+//
+// Triangular grammars (Nussinov, Zuker, matrix chain) fill the cells (i,j),
+// 0 <= i <= j <= n, of one or many instances.  The (n+1)x(n+1) upper
+// triangle is cut into T x T tiles; tile (I,J), I <= J, holds cells
+// i in [I*T, I*T+T), j in [J*T, J*T+T).  Each tile is one record of NT tables
+// of T*T int32 (row-major [i%T][j%T]) stored contiguously in HBM, tiles of an
+// instance numbered row-major over the upper triangle of tiles.
+//
+// Scheduling: tile (I,J) with I<J depends on (I,J-1) and (I+1,J) (and
+// through them on every tile inside [I..J]); diagonal tiles have no
+// dependency.  One persistent kernel pulls tiles from a queue in HBM; a tile
+// is appended to the queue when its second predecessor completes (device-scope
+// acq/rel counters), so tiles of many instances interleave in order of
+// readiness.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dpx {
+
+constexpr int T = 32;           // tile edge
+constexpr int NTHR = 256;       // threads per workgroup of the fill kernels
+constexpr int NWAVE = NTHR / 64;
+constexpr int INV = 1 << 28;    // "no valid candidate" for min-plus tables
+constexpr int INV_THR = 1 << 27;
+constexpr int NEG = -(1 << 28); // "no candidate" for max-plus tables
+constexpr int ZU_EMPTY = 999999;  // the reference's empty value (Zuker.scala:154)
+
+enum Grammar { G_NUSSINOV = 0, G_ZUKER = 1, G_MATCHAIN = 2 };
+// tables per tile record
+constexpr int NT_NU = 1;  // s
+constexpr int NT_MC = 1;  // M
+constexpr int NT_ZU = 5;  // cl, U (= dangle-closed cl + ML_intern), ml, ml1, R11 (row prefix-min of U)
+enum { ZT_CL = 0, ZT_U = 1, ZT_ML = 2, ZT_ML1 = 3, ZT_R11 = 4 };
+
+struct TriInst {
+  int32_t n;         // sequence length / chain length; cells 0..n
+  int32_t nb;        // tiles per side = ceil((n+1)/T)
+  int64_t seq_off;   // first base in codes[] (matrix chain: first dim in dims[])
+  int64_t tile0;     // record index of tile (0,0)
+  int64_t trace_off; // first trace record of this instance
+  int32_t trace_cap; // trace records available
+  int32_t out;       // result slot
+};
+
+struct TriArgs {
+  const TriInst* inst;
+  int32_t n_inst;
+  int32_t total_tiles;
+  unsigned long long* queue;   // total_tiles slots, ~0 = empty
+  int32_t* ctrl;               // [0] next slot to claim, [1] next slot to fill
+  int32_t* dep;                // completed predecessors per tile
+  int32_t* tab;                // tile records
+  const uint8_t* codes;        // bases, codes 1..4
+  const int32_t* dims;         // matrix chain: rows then cols, n each
+  const void* rna;             // DevRna (Zuker)
+  const int32_t* hl_ent;       // hairpin entropy by loop size (Zuker)
+  int32_t min_loop;            // Nussinov
+  // finalize
+  int32_t* results;            // per instance
+  void* traces;                // dpx_trace_t records
+  int32_t* trace_lens;
+  int32_t* st;                 // Zuker st column scratch: n+1 per instance at seq_off+q
+  int32_t want_trace;
+};
+
+__host__ __device__ inline int tile_index(int nb, int I, int J) { return I * nb - (I * (I - 1)) / 2 + (J - I); }
+__host__ __device__ inline int64_t tiles_of(int nb) { return (int64_t)nb * (nb + 1) / 2; }
+__host__ __device__ inline unsigned long long pack_item(int inst, int I, int J) {
+  return ((unsigned long long)(unsigned)inst << 32) | ((unsigned long long)(unsigned)I << 16) | (unsigned)J;
+}
+
+// Smith-Waterman/Gotoh (swat.hip)
+struct SwArgs {
+  const uint8_t* a;     // n1 bytes
+  const uint8_t* b;     // n2 bytes
+  int n1, n2;
+  int match, mismatch, open, ext;
+  int nbands, nwords;   // nwords = words per lane per band
+  int2* hrow;           // (nbands+1) x (n2+1): (M, g1) of rows 64*band
+  int* prog;            // nbands+1: columns of hrow[band] published
+  int* ctrl;            // [0] band claim counter
+  uint32_t* bt;         // nbands x nwords x 64
+  unsigned long long* best;  // packed (M << 32) | ~(i*(n2+1)+j)
+  int* out;             // [0] = M(n1,n2)
+};
+int sw_launch(const SwArgs& A, const int2* row0, int grid, void* stream);
+int sw_trace_launch(const SwArgs& A, const uint8_t* r0m, const uint8_t* r0g2, int i0, int j0, void* tr, int cap,
+                    int* len, void* stream);
+int sw_occupancy();
+
+// launchers (tri.hip, finalize.hip); stream is a hipStream_t
+int fill_occupancy(int grammar);
+int launch_fill(int grammar, const TriArgs& a, const unsigned long long* seeds, int nseed, int grid, void* stream);
+int launch_finalize(int grammar, const TriArgs& a, int count, int n_max, void* stream);
+
+}  // namespace dpx

@@ -1,0 +1,443 @@
+// finalize.hip -- per-instance answer and backtrace for the triangular grammars.
+//
+// One workgroup per instance.  The fill kernels store values only; here the
+// backtrace walks the reference's BFS (bench2/zu.c:49-93, nu.c:38-59,
+// CodeGen.scala:233-252) and, for every visited cell, re-evaluates the
+// candidates of its table in the reference's evaluation order, all 256
+// threads in parallel, taking the first candidate equal to the stored
+// optimum.  Because the reference keeps a candidate only on strict
+// improvement, that first candidate is exactly its stored back-pointer.
+// Zuker's exterior table st (only column j = n exists, zu.c:37 fun14) is a
+// 1-D O(n^2) pass computed here before the backtrace.
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+#include "rna_model.h"
+
+namespace dpx {
+
+typedef struct { int32_t i, j, rule, pos[2]; } TraceRec;
+
+__device__ __forceinline__ int fmin_(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int fmax_(int a, int b) { return a > b ? a : b; }
+
+constexpr unsigned long long fbp_word(int lo) {
+  unsigned long long w = 0;
+  for (int q = 0; q < 16; ++q) {
+    int idx = lo + q, x = idx / 5, y = idx % 5, v = 7;
+    if (x == 1 && y == 4) v = 5;
+    if (x == 2 && y == 3) v = 1;
+    if (x == 3 && y == 2) v = 2;
+    if (x == 3 && y == 4) v = 3;
+    if (x == 4 && y == 3) v = 4;
+    if (x == 4 && y == 1) v = 6;
+    if (idx < 25) w |= (unsigned long long)v << (4 * q);
+  }
+  return w;
+}
+__device__ __forceinline__ int fbpc(int x, int y) {
+  const int idx = x * 5 + y;
+  const unsigned long long w = idx < 16 ? fbp_word(0) : fbp_word(16);
+  return (int)((w >> (4 * (idx & 15))) & 15);
+}
+
+struct Cells {
+  const int32_t* tab;
+  int nb, NTv;
+  int64_t tile0;
+  __device__ int get(int t, int i, int j) const {
+    const int I = i / T, J = j / T;
+    return tab[(size_t)(tile0 + tile_index(nb, I, J)) * (size_t)(NTv * T * T) + t * T * T + (i % T) * T + (j % T)];
+  }
+};
+
+// Shared BFS driver.  G provides: ncand(e), value(e,c,&v) -> bool, target(e),
+// decode(e,c,&rule,&p0,&p1), push children via the lambda.
+struct Bfs {
+  int rd, wr, first, overflow;
+  TraceRec cur;
+};
+
+template <class Fn>
+__device__ void first_match(int ncand, int target, Fn&& cand, int* s_first) {
+  // smallest c in [0,ncand) with cand(c) == target, 0x7fffffff if none
+  if (threadIdx.x == 0) *s_first = 0x7fffffff;
+  __syncthreads();
+  for (int base = 0; base < ncand; base += NTHR) {
+    const int c = base + threadIdx.x;
+    int v;
+    if (c < ncand && cand(c, &v) && v == target) atomicMin(s_first, c);
+    __syncthreads();
+    const bool found = *s_first != 0x7fffffff;
+    __syncthreads();
+    if (found) return;
+  }
+}
+
+// ----------------------------------------------------------------- Nussinov
+__global__ __launch_bounds__(NTHR) void nu_finalize(TriArgs a) {
+  const TriInst in = a.inst[blockIdx.x];
+  const Cells C{a.tab, in.nb, NT_NU, in.tile0};
+  const uint8_t* s = a.codes + in.seq_off;
+  const int n = in.n;
+  __shared__ Bfs b;
+  if (threadIdx.x == 0) a.results[in.out] = C.get(0, 0, n);
+  if (!a.want_trace) return;
+  TraceRec* tr = (TraceRec*)a.traces + in.trace_off;
+  const int cap = in.trace_cap;
+  if (threadIdx.x == 0) {
+    b.rd = 0; b.wr = 0; b.overflow = 0;
+    if (cap > 0) { tr[0] = TraceRec{0, n, 0, {0, 0}}; b.wr = 1; } else b.overflow = 1;
+  }
+  __syncthreads();
+  for (;;) {
+    if (threadIdx.x == 0 && b.rd < b.wr) b.cur = tr[b.rd];
+    __syncthreads();
+    if (b.rd >= b.wr || b.overflow) break;
+    const int i = b.cur.i, j = b.cur.j;
+    const int target = C.get(0, i, j);
+    // candidates in nu_kern.h order: nil, left, right, pair, split(k asc)
+    auto cand = [&](int c, int* v) -> bool {
+      if (c == 0) { *v = 0; return i == j; }
+      if (c == 1) { if (i + 1 > j) return false; *v = C.get(0, i + 1, j); return true; }
+      if (c == 2) { if (i + 1 > j) return false; *v = C.get(0, i, j - 1); return true; }
+      if (c == 3) {
+        if (i + 2 + a.min_loop > j || fbpc(s[i], s[j - 1]) == 7) return false;
+        *v = C.get(0, i + 1, j - 1) + 1;
+        return true;
+      }
+      const int k = i + 1 + (c - 4);
+      if (k > j - 1) return false;
+      *v = C.get(0, i, k) + C.get(0, k, j);
+      return true;
+    };
+    first_match(4 + fmax_(0, j - i - 1), target, cand, &b.first);
+    if (threadIdx.x == 0) {
+      const int c = b.first;
+      if (c == 0x7fffffff) b.overflow = 1;
+      TraceRec& e = tr[b.rd];
+      e.rule = c < 4 ? c : 4;
+      e.pos[0] = c >= 4 ? i + 1 + (c - 4) : 0;
+      e.pos[1] = 0;
+      auto push = [&](int pi, int pj) {
+        if (b.wr >= cap) { b.overflow = 1; return; }
+        tr[b.wr++] = TraceRec{pi, pj, 0, {0, 0}};
+      };
+      switch (e.rule) {
+        case 1: push(i + 1, j); break;
+        case 2: push(i, fmax_(i, j - 1)); break;
+        case 3: push(i + 1, fmax_(i + 1, j - 1)); break;
+        case 4: push(i, e.pos[0]); push(e.pos[0], j); break;
+        default: break;
+      }
+      b.rd++;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.trace_lens[in.out] = b.overflow ? -1 : b.wr;
+}
+
+// ------------------------------------------------------------- matrix chain
+__global__ __launch_bounds__(NTHR) void mc_finalize(TriArgs a) {
+  const TriInst in = a.inst[blockIdx.x];
+  const Cells C{a.tab, in.nb, NT_MC, in.tile0};
+  const int32_t* rows = a.dims + in.seq_off;
+  const int32_t* cols = rows + in.n;
+  const int n = in.n;
+  __shared__ Bfs b;
+  if (threadIdx.x == 0) a.results[in.out] = C.get(0, 0, n);
+  if (!a.want_trace) return;
+  TraceRec* tr = (TraceRec*)a.traces + in.trace_off;
+  const int cap = in.trace_cap;
+  if (threadIdx.x == 0) {
+    b.rd = 0; b.wr = 0; b.overflow = 0;
+    if (cap > 0) { tr[0] = TraceRec{0, n, 0, {0, 0}}; b.wr = 1; } else b.overflow = 1;
+  }
+  __syncthreads();
+  for (;;) {
+    if (threadIdx.x == 0 && b.rd < b.wr) b.cur = tr[b.rd];
+    __syncthreads();
+    if (b.rd >= b.wr || b.overflow) break;
+    const int i = b.cur.i, j = b.cur.j;
+    const int target = C.get(0, i, j);
+    auto cand = [&](int c, int* v) -> bool {
+      if (c == 0) { *v = 0; return i + 1 == j; }
+      const int k = i + c;
+      if (k > j - 1) return false;
+      *v = (int)((uint32_t)C.get(0, i, k) + (uint32_t)C.get(0, k, j) +
+                 (uint32_t)rows[i] * (uint32_t)rows[k] * (uint32_t)cols[j - 1]);
+      return true;
+    };
+    first_match(1 + fmax_(0, j - i - 1), target, cand, &b.first);
+    if (threadIdx.x == 0) {
+      const int c = b.first;
+      if (c == 0x7fffffff) b.overflow = 1;
+      TraceRec& e = tr[b.rd];
+      e.rule = c == 0 ? 0 : 1;
+      e.pos[0] = c == 0 ? 0 : i + c;
+      e.pos[1] = 0;
+      if (e.rule == 1) {
+        if (b.wr + 2 > cap) b.overflow = 1;
+        else { tr[b.wr++] = TraceRec{i, e.pos[0], 0, {0, 0}}; tr[b.wr++] = TraceRec{e.pos[0], j, 0, {0, 0}}; }
+      }
+      b.rd++;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.trace_lens[in.out] = b.overflow ? -1 : b.wr;
+}
+
+// -------------------------------------------------------------------- Zuker
+struct ZuF {
+  const uint8_t* s;
+  const DevRna* P;
+  const int32_t* hl;
+  int n;
+  __device__ int S(int k) const { return s[k]; }
+  __device__ int bp(int x, int y) const { return fbpc(S(x), S(y)); }
+  __device__ int termau(int x, int y) const {
+    const int p = S(x), q = S(y);
+    return ((p == 3 && q == 2) || (p == 2 && q == 3)) ? 0 : P->term_au;
+  }
+  __device__ int sr(int i, int j) const { return P->stack[bp(i, j)][bp(j - 1, i + 1)]; }
+  __device__ int hairpin(int i, int j) const {
+    const int size = j - i - 1;
+    if (size < 3) return 600;
+    const int cls = size == 3 ? 0 : size == 4 ? 1 : size == 6 ? 2 : -1;
+    if (cls >= 0) {
+      int key = 0;
+      for (int m = 0; m <= size + 1; ++m) key = key * 8 + S(i + m);
+      for (int e = 0; e < P->n_loop[cls]; ++e)
+        if (P->loop_key[cls][e] == key) return P->loop_e[cls][e];
+    }
+    const int ent = hl[size];
+    if (size == 3) return ent + termau(i, j);
+    return ent + P->mmH[bp(i, j)][S(i + 1)][S(j - 1)];
+  }
+  __device__ int il(int i, int k, int l, int j) const {
+    const int sl = k - i - 1, sr_ = j - l - 1;
+    const int ocb = bp(i, j), icb = bp(l, k);
+    if (sl == 1) {
+      if (sr_ == 1) return P->int11[ocb][bp(j - 2, i + 2)][S(i + 1)][S(j - 1)];
+      if (sr_ == 2) return P->int21[ocb][bp(j - 3, i + 2)][S(i + 1)][S(j - 2)][S(j - 1)];
+      return P->F[sl][sr_] + P->mm1nI[ocb][S(i + 1)][S(j - 1)] + P->mm1nI[icb][S(l + 1)][S(k - 1)];
+    }
+    if (sl == 2) {
+      if (sr_ == 1) return P->int21[bp(j - 2, i + 3)][ocb][S(j - 1)][S(i + 1)][S(i + 2)];
+      if (sr_ == 2) return P->int22[ocb][bp(j - 3, i + 3)][S(i + 1)][S(i + 2)][S(j - 2)][S(j - 1)];
+      if (sr_ == 3) return P->il5 + P->ninio0 + P->mm23I[ocb][S(i + 1)][S(j - 1)] + P->mm23I[icb][S(l + 1)][S(k - 1)];
+    } else if (sl == 3 && sr_ == 2) {
+      return P->il5 + P->ninio0 + P->mm23I[ocb][S(i + 1)][S(j - 1)] + P->mm23I[icb][S(l + 1)][S(k - 1)];
+    } else if (sr_ == 1) {
+      return P->F[sl][sr_] + P->mm1nI[ocb][S(i + 1)][S(j - 1)] + P->mm1nI[icb][S(l + 1)][S(k - 1)];
+    }
+    return P->F[sl][sr_] + P->mmI[ocb][S(i + 1)][S(j - 1)] + P->mmI[icb][S(l + 1)][S(k - 1)];
+  }
+  __device__ int bulge_l(int i, int j, int k) const {
+    const int size = k - i - 2;
+    if (size == 1) return P->bulge[1] + P->stack[bp(i + 1, j - 2)][bp(j - 3, k)];
+    return P->bulge[size] + termau(i + 1, j - 2) + termau(j - 3, k);
+  }
+  __device__ int bulge_r(int i, int j, int k) const {
+    const int size = j - k - 2;
+    if (size == 1) return P->bulge[1] + P->stack[bp(i + 1, j - 2)][bp(k - 1, i + 2)];
+    return P->bulge[size] + termau(i + 1, j - 2) + termau(k - 1, i + 2);
+  }
+};
+
+__global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
+  extern __shared__ int st[];  // st(i,n), i = 0..n
+  const TriInst in = a.inst[blockIdx.x];
+  const Cells C{a.tab, in.nb, NT_ZU, in.tile0};
+  const int n = in.n;
+  const ZuF z{a.codes + in.seq_off, (const DevRna*)a.rna, a.hl_ent, n};
+  const int mlint = z.P->ml_intern;
+  __shared__ int red;
+  __shared__ Bfs b;
+  // ---- st column (zu_kern.h:166-182): rule 12 st(i+1), rule 13 D(i,k)+st(k), rule 14 i==n
+  if (threadIdx.x == 0) st[n] = 0;
+  for (int i = n - 1; i >= 0; --i) {
+    if (threadIdx.x == 0) red = st[i + 1];
+    __syncthreads();
+    int m = INV;
+    for (int k = i + 7 + (int)threadIdx.x; k <= n; k += NTHR) {
+      const int U = C.get(ZT_U, i, k);
+      if (U < INV_THR) m = fmin_(m, U - mlint + st[k]);
+    }
+    for (int off = 32; off >= 1; off >>= 1) m = fmin_(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0 && m < red) atomicMin(&red, m);
+    __syncthreads();
+    if (threadIdx.x == 0) st[i] = red;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.results[in.out] = st[0];
+  if (!a.want_trace) return;
+  TraceRec* tr = (TraceRec*)a.traces + in.trace_off;
+  const int cap = in.trace_cap;
+  if (threadIdx.x == 0) {
+    b.rd = 0; b.wr = 0; b.overflow = 0;
+    if (cap > 0) { tr[0] = TraceRec{0, n, 12, {0, 0}}; b.wr = 1; } else b.overflow = 1;
+  }
+  __syncthreads();
+  for (;;) {
+    if (threadIdx.x == 0 && b.rd < b.wr) b.cur = tr[b.rd];
+    __syncthreads();
+    if (b.rd >= b.wr || b.overflow) break;
+    const int i = b.cur.i, j = b.cur.j, tabrule = b.cur.rule, d = j - i;
+    int ncand, target;
+    if (tabrule == 12) { target = st[i]; ncand = 2 + fmax_(0, n - (i + 7) + 1); }
+    else if (tabrule == 0) { target = C.get(ZT_CL, i, j); ncand = 963; }
+    else if (tabrule == 6) { target = C.get(ZT_ML, i, j); ncand = 1 + fmax_(0, j - 7 - (i + 7) + 1); }
+    else { target = C.get(ZT_ML1, i, j); ncand = 2 + fmax_(0, j - 7 - (i + 7) + 1) + fmax_(0, j - 1 - (i + 7) + 1); }
+    const int nk9 = fmax_(0, j - 7 - (i + 7) + 1);
+    auto cand = [&](int c, int* v) -> bool {
+      if (tabrule == 12) {  // st
+        if (c == 0) { if (i + 1 > n) return false; *v = st[i + 1]; return true; }
+        const int k = i + 7 + (c - 1);
+        if (c - 1 < fmax_(0, n - (i + 7) + 1)) {
+          const int U = C.get(ZT_U, i, k);
+          if (U >= INV_THR) return false;
+          *v = U - mlint + st[k];
+          return true;
+        }
+        *v = 0;
+        return i == n;
+      }
+      if (tabrule == 0) {  // cl: rules 0..5
+        const int sr = z.sr(i, j - 1);
+        if (c == 0) {
+          if (d < 9) return false;
+          const int x = C.get(ZT_CL, i + 1, j - 1);
+          if (x >= INV_THR) return false;
+          *v = x + sr; return true;
+        }
+        if (c == 1) { if (d < 7) return false; *v = z.hairpin(i + 1, j - 2) + sr; return true; }
+        if (c < 32) {
+          const int k = i + 3 + (c - 2);
+          if (d < 12 || k > fmin_(j - 9, i + 32)) return false;
+          const int x = C.get(ZT_CL, k, j - 2);
+          if (x >= INV_THR) return false;
+          *v = x + z.bulge_l(i, j, k) + sr; return true;
+        }
+        if (c < 62) {
+          const int k = j - 32 + (c - 32);
+          if (d < 12 || k < i + 9) return false;
+          const int x = C.get(ZT_CL, i + 2, k);
+          if (x >= INV_THR) return false;
+          *v = x + z.bulge_r(i, j, k) + sr; return true;
+        }
+        if (c < 962) {
+          const int kk = (c - 62) / 30, ll = (c - 62) % 30;
+          const int k = j - 32 + kk, l = i + 3 + ll;
+          if (d < 13 || k < i + 10 || l > k - 7) return false;
+          const int x = C.get(ZT_CL, l, k);
+          if (x >= INV_THR) return false;
+          *v = x + z.il(i + 1, l, k - 1, j - 2) + sr; return true;
+        }
+        if (d < 18) return false;
+        const int x = C.get(ZT_ML, i + 2, j - 2);
+        if (x >= INV_THR) return false;
+        *v = z.P->ml_closing + mlint + x + z.termau(i + 1, j - 2) + sr +
+             z.P->mmM[z.bp(j - 2, i + 1)][z.S(j - 3)][z.S(i + 2)];
+        return true;
+      }
+      if (tabrule == 6) {  // ml: rules 6, 7
+        if (c == 0) {
+          if (d < 15) return false;
+          const int x = C.get(ZT_ML, i + 1, j);
+          if (x >= INV_THR) return false;
+          *v = x; return true;
+        }
+        const int k = i + 7 + (c - 1);
+        const int U = C.get(ZT_U, i, k), M1 = C.get(ZT_ML1, k, j);
+        if (U >= INV_THR || M1 >= INV_THR) return false;
+        *v = U + M1; return true;
+      }
+      // ml1: rules 8, 9, 10, 11
+      if (c == 0) {
+        if (d < 8) return false;
+        const int x = C.get(ZT_ML1, i + 1, j);
+        if (x >= INV_THR) return false;
+        *v = x; return true;
+      }
+      if (c <= nk9) {
+        const int k = i + 7 + (c - 1);
+        const int U = C.get(ZT_U, i, k), M1 = C.get(ZT_ML1, k, j);
+        if (U >= INV_THR || M1 >= INV_THR) return false;
+        *v = U + M1; return true;
+      }
+      if (c == nk9 + 1) {
+        if (d < 7) return false;
+        const int U = C.get(ZT_U, i, j);
+        if (U >= INV_THR) return false;
+        *v = U; return true;
+      }
+      const int k = i + 7 + (c - nk9 - 2);
+      if (k > j - 1) return false;
+      const int U = C.get(ZT_U, i, k);
+      if (U >= INV_THR) return false;
+      *v = U; return true;
+    };
+    first_match(ncand, target, cand, &b.first);
+    if (threadIdx.x == 0) {
+      const int c = b.first;
+      TraceRec& e = tr[b.rd];
+      int rule = -1, p0 = 0, p1 = 0;
+      if (c == 0x7fffffff) { b.overflow = 1; }
+      else if (tabrule == 12) {
+        if (c == 0) rule = 12;
+        else if (c - 1 < fmax_(0, n - (i + 7) + 1)) { rule = 13; p0 = i + 7 + (c - 1); }
+        else rule = 14;
+      } else if (tabrule == 0) {
+        if (c == 0) rule = 0;
+        else if (c == 1) rule = 1;
+        else if (c < 32) { rule = 2; p0 = i + 3 + (c - 2); }
+        else if (c < 62) { rule = 3; p0 = j - 32 + (c - 32); }
+        else if (c < 962) { rule = 4; p0 = i + 3 + (c - 62) % 30; p1 = j - 32 + (c - 62) / 30; }
+        else rule = 5;
+      } else if (tabrule == 6) {
+        if (c == 0) rule = 6; else { rule = 7; p0 = i + 7 + (c - 1); }
+      } else {
+        if (c == 0) rule = 8;
+        else if (c <= nk9) { rule = 9; p0 = i + 7 + (c - 1); }
+        else if (c == nk9 + 1) rule = 10;
+        else { rule = 11; p0 = i + 7 + (c - nk9 - 2); }
+      }
+      e.rule = rule; e.pos[0] = p0; e.pos[1] = p1;
+      auto push = [&](int pi, int pj, int pr) {
+        if (b.wr >= cap) { b.overflow = 1; return; }
+        tr[b.wr++] = TraceRec{pi, pj, pr, {0, 0}};
+      };
+      switch (rule) {  // children as zu.c:77-91
+        case 0: push(i + 1, fmax_(i + 8, j - 1), 0); break;
+        case 2: push(p0, fmax_(i + 10, fmax_(i + 11, j - 1) - 1), 0); break;
+        case 3: push(i + 2, p0, 0); break;
+        case 4: push(p0, p1, 0); break;
+        case 5: push(i + 2, fmax_(i + 16, fmax_(i + 17, j - 1) - 1), 6); break;
+        case 6: push(i + 1, j, 6); break;
+        case 7: case 9: case 13:
+          push(i, fmax_(i + 7, p0), 0); push(p0, j, rule == 13 ? 12 : 8); break;
+        case 8: push(i + 1, j, 8); break;
+        case 10: push(i, fmax_(i + 7, j), 0); break;
+        case 11: push(i, fmax_(i + 7, p0), 0); break;
+        case 12: push(i + 1, j, 12); break;
+        default: break;
+      }
+      b.rd++;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.trace_lens[in.out] = b.overflow ? -1 : b.wr;
+}
+
+// ------------------------------------------------------------------ launch
+int launch_finalize(int grammar, const TriArgs& a, int count, int n_max, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (count <= 0) return 0;
+  if (grammar == G_NUSSINOV) hipLaunchKernelGGL(nu_finalize, dim3(count), dim3(NTHR), 0, st, a);
+  else if (grammar == G_MATCHAIN) hipLaunchKernelGGL(mc_finalize, dim3(count), dim3(NTHR), 0, st, a);
+  else hipLaunchKernelGGL(zu_finalize, dim3(count), dim3(NTHR), (size_t)(n_max + 1) * sizeof(int), st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace dpx

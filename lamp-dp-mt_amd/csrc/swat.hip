@@ -1,0 +1,214 @@
+// swat.hip -- Smith-Waterman/Gotoh two-track grammar on gfx950.
+//
+// Recurrences (bench/dp_swat/comp1.cu:50-97, SWatAffine.scala:22-69) with
+// parametric scores; per cell the reference evaluates g2, g1, then M and keeps
+// the first candidate, later ones only when strictly greater:
+//   g2(i,j) = j==0 ? 0 [7] : max{ max(0,M(i,j-1)+open) [8], max(0,g2(i,j-1)+ext) [9] }
+//   g1(i,j) = i==0 ? 0 [4] : max{ max(0,M(i-1,j)+open) [5], max(0,g1(i-1,j)+ext) [6] }
+//   M(i,j)  = max{ 0 [0, i=j=0]; M(i-1,j-1)+(a==b?match:mismatch) [1]; g1 [2]; g2 [3] }
+//
+// Layout: rows are cut into bands of 64; one wave64 owns a band, lane r owns
+// row i = 64*band + r + 1 and sweeps the columns skewed by r (anti-diagonal
+// wavefront in registers): at step t lane r computes column j = t - r, takes
+// M(i-1,j), g1(i-1,j) from lane r-1 through a DPP wave shift and keeps its own
+// M(i,j-1), g2(i,j-1) and the diagonal M(i-1,j-1) in registers.  The band's
+// last row goes to HBM and is published to the next band in chunks through a
+// device-scope progress counter (persistent kernel, bands claimed in order).
+// The traceback keeps one 4-bit code per cell (M rule, g1 ext?, g2 ext?),
+// 8 steps of a lane per 32-bit word, words of a band stored lane-major so each
+// store is a coalesced 256 B row.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "engine.h"
+
+namespace dpx {
+
+typedef struct { int32_t i, j, rule, pos[2]; } SwTrace;
+
+
+
+__device__ __forceinline__ int sw_shr1(int lane0_val, int v) {
+  // lanes 1..63 receive lane-1's v; lane 0 keeps lane0_val (DPP wave_shr:1)
+  return __builtin_amdgcn_update_dpp(lane0_val, v, 0x138, 0xf, 0xf, false);
+}
+
+constexpr int SW_CH = 64;  // publish granularity (columns)
+
+__global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
+  const int lane = threadIdx.x;
+  for (;;) {
+    // one wave per workgroup: the band claim and every wait are wave-uniform
+    int band = 0;
+    if (lane == 0) band = atomicAdd(&A.ctrl[0], 1);
+    band = __builtin_amdgcn_readfirstlane(band);
+    if (band >= A.nbands) break;
+    const int i = band * 64 + lane + 1;
+    const bool row_ok = i <= A.n1;
+    const int ai = row_ok ? A.a[i - 1] : -1;
+    const int W = A.n2 + 1;
+    const int2* above = A.hrow + (size_t)band * W;
+    int2* below = A.hrow + (size_t)(band + 1) * W;
+    uint32_t* btb = A.bt + (size_t)band * A.nwords * 64;
+    int Mc = 0, G1c = 0, G2c = 0, upPrev = 0;
+    int best = -1, bestj = 0;
+    uint32_t word = 0;
+    int known = 0;  // columns of `above` known to be published
+    int chM = 0, chG = 0;
+    const int tend = A.n2 + 64;
+    for (int t = 0; t < tend; ++t) {
+      const int tc = t & (SW_CH - 1);
+      if (tc == 0) {
+        // make the next chunk of the row above available, then load it
+        const int need = min(t + SW_CH, W);
+        if (known < need) {
+          int k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&A.prog[band], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          for (int spin = 0; k < need; ++spin) {
+            if (spin > (1 << 24)) {  // bounded wait: flag the error, stop the band
+              if (lane == 0) A.ctrl[1] = 1;
+              k = need;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&A.prog[band], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          known = k;
+        }
+        const int col = t + lane;
+        if (col < W) { const int2 v = above[col]; chM = v.x; chG = v.y; }
+      }
+      const int hM = __shfl(chM, tc), hG = __shfl(chG, tc);
+      const int upM = sw_shr1(hM, Mc);
+      const int upG = sw_shr1(hG, G1c);
+      const int j = t - lane;
+      const bool act = row_ok && j >= 0 && j <= A.n2;
+      // g2
+      int g2, r2;
+      if (j == 0) { g2 = 0; r2 = 0; }
+      else {
+        const int o = max(0, Mc + A.open), e = max(0, G2c + A.ext);
+        r2 = e > o; g2 = r2 ? e : o;
+      }
+      // g1
+      const int o1 = max(0, upM + A.open), e1 = max(0, upG + A.ext);
+      const int r1 = e1 > o1;
+      const int g1 = r1 ? e1 : o1;
+      // M
+      int M, rm;
+      if (j == 0) { M = g1; rm = 2; }
+      else {
+        const int bj = A.b[j - 1 < A.n2 ? (j - 1 >= 0 ? j - 1 : 0) : 0];
+        M = upPrev + (ai == bj ? A.match : A.mismatch); rm = 1;
+        if (g1 > M) { M = g1; rm = 2; }
+      }
+      if (g2 > M) { M = g2; rm = 3; }
+      upPrev = upM;
+      if (act) {
+        Mc = M; G1c = g1; G2c = g2;
+        if (M > best) { best = M; bestj = j; }
+        if (i == A.n1 && j == A.n2) A.out[0] = M;
+        if (lane == 63) below[j] = make_int2(M, g1);
+      }
+      word |= (uint32_t)(rm | (r1 << 2) | (r2 << 3)) << (4 * (t & 7));
+      if ((t & 7) == 7 || t == tend - 1) {
+        btb[(size_t)(t >> 3) * 64 + lane] = word;
+        word = 0;
+      }
+      // publish the finished columns of this band's last row
+      if (tc == SW_CH - 1 || t == tend - 1) {
+        const int done = min(t - 63 + 1, W);  // columns 0..t-63 of row 64*band+64
+        if (done > 0) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          if (lane == 63) __hip_atomic_store(&A.prog[band + 1], done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+    // best cell of this band: largest M, then smallest (i, j)
+    if (row_ok && best >= 0) {
+      const unsigned long long key = ((unsigned long long)(unsigned)best << 32) |
+                                     (unsigned long long)(0xffffffffu - (uint32_t)((uint64_t)i * (uint64_t)W + (uint64_t)bestj));
+      atomicMax(A.best, key);
+    }
+  }
+}
+
+__global__ void sw_reset(SwArgs A, const int2* row0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int W = A.n2 + 1;
+  if (t < W) A.hrow[t] = row0[t];
+  if (t <= A.nbands) A.prog[t] = t == 0 ? W : 0;
+  if (t == 0) { A.ctrl[0] = 0; A.ctrl[1] = 0; *A.best = 0; A.out[0] = 0; }
+}
+
+// Traceback: the reference BFS from (i0,j0) (comp1.cu:100-132) is a single
+// chain here; one wave walks it, caching a 64-word window of the band's
+// traceback words in LDS (the walk only moves to smaller steps within a band).
+__global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mrule, const uint8_t* row0_g2rule,
+                                               int i0, int j0, SwTrace* tr, int cap, int* len) {
+  constexpr int NW = 64;
+  __shared__ uint32_t win[NW][64];
+  const int lane = threadIdx.x;
+  int wb = -1, w0 = 0;
+  int i = i0, j = j0, tab = 0, n = 0;
+  bool ok = true;
+  for (;;) {
+    int rule;
+    if (tab <= 3) {
+      if (i == 0 && j == 0) rule = 0;
+      else if (i == 0) rule = row0_mrule[j];
+      else rule = -1;
+    } else if (tab <= 6) {
+      rule = i == 0 ? 4 : -1;
+    } else {
+      rule = j == 0 ? 7 : (i == 0 ? row0_g2rule[j] : -1);
+    }
+    if (rule < 0) {
+      const int b = (i - 1) / 64, r = (i - 1) % 64, t = j + r, w = t >> 3;
+      if (b != wb || w < w0 || w >= w0 + NW) {
+        wb = b;
+        w0 = max(0, w - NW + 1);
+        for (int k = 0; k < NW; ++k)
+          if (w0 + k < A.nwords) win[k][lane] = A.bt[((size_t)b * A.nwords + w0 + k) * 64 + lane];
+        __syncthreads();
+      }
+      const uint32_t nib = (win[w - w0][r] >> (4 * (t & 7))) & 15u;
+      rule = tab <= 3 ? (int)(nib & 3u) : tab <= 6 ? 5 + (int)((nib >> 2) & 1u) : 8 + (int)((nib >> 3) & 1u);
+    }
+    if (n >= cap) { ok = false; break; }
+    if (lane == 0) tr[n] = SwTrace{i, j, rule, {0, 0}};
+    ++n;
+    if (rule == 1) { --i; --j; tab = 0; }
+    else if (rule == 2) { tab = 4; }
+    else if (rule == 3) { tab = 7; }
+    else if (rule == 5) { --i; tab = 0; }
+    else if (rule == 6) { --i; tab = 4; }
+    else if (rule == 8) { --j; tab = 0; }
+    else if (rule == 9) { --j; tab = 7; }
+    else break;
+  }
+  if (lane == 0) *len = ok ? n : -1;
+}
+
+int sw_launch(const SwArgs& A, const int2* row0, int grid, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const int W = A.n2 + 1;
+  const int nthr = max(W, A.nbands + 1);
+  hipLaunchKernelGGL(sw_reset, dim3((nthr + 255) / 256), dim3(256), 0, s, A, row0);
+  hipLaunchKernelGGL(sw_fill, dim3(grid), dim3(64), 0, s, A);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sw_trace_launch(const SwArgs& A, const uint8_t* r0m, const uint8_t* r0g2, int i0, int j0, void* tr, int cap,
+                    int* len, void* stream) {
+  hipLaunchKernelGGL(sw_trace, dim3(1), dim3(64), 0, (hipStream_t)stream, A, r0m, r0g2, i0, j0, (SwTrace*)tr, cap, len);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sw_occupancy() {
+  int blocks = 0;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sw_fill, 64, 0) == hipSuccess ? blocks : 0;
+}
+
+}  // namespace dpx

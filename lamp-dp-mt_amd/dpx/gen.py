@@ -13,16 +13,20 @@ import math
 MASK = (1 << 64) - 1
 
 
+def _mix(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+    return z ^ (z >> 31)
+
+
 class SplitMix64:
     def __init__(self, seed):
-        self.state = (seed * 0x9E3779B97F4A7C15 + 0x1234567) & MASK
+        # the seed is hashed, so nearby seeds start unrelated streams
+        self.state = _mix((seed + 0x632BE59BD9B4E019) & MASK)
 
     def next(self):
         self.state = (self.state + 0x9E3779B97F4A7C15) & MASK
-        z = self.state
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
-        return z ^ (z >> 31)
+        return _mix(self.state)
 
     def uniform(self):
         return (self.next() >> 11) * (1.0 / (1 << 53))
@@ -55,6 +59,12 @@ def rna_gc(n, seed, gc=0.5):
 def dna_uniform(n, seed):
     r = SplitMix64(seed)
     return "".join("ACGT"[r.below(4)] for _ in range(n))
+
+
+def dna_pair(n1, n2, seed):
+    """Two sequences drawn back to back from one seeded stream."""
+    s = dna_uniform(n1 + n2, seed)
+    return s[:n1], s[n1:]
 
 
 def log_uniform_lengths(k, lo, hi, seed):
