@@ -157,6 +157,7 @@ template <class G>
 __global__ __launch_bounds__(NTHR) void tri_fill(TriArgs a) {
   __shared__ typename G::Smem sm;
   __shared__ unsigned long long s_item;
+  G::init(a, sm);
   bool live = true;
   while (live) {
     if (threadIdx.x < 64) {
@@ -216,6 +217,7 @@ __device__ __forceinline__ bool nu_pair(int x, int y) {
 }
 
 struct GNu {
+  template <class S> __device__ static void init(const TriArgs&, S&) {}
   struct Smem {
     union {
       int stage[NWAVE * (T * (T + 1) + T * T + T)];
@@ -316,6 +318,7 @@ struct GNu {
 // M(i,j) = min_{i<k<j} M(i,k)+M(k,j)+rows[i]*rows[k]*cols[j-1]  (int32 wrap),
 // M(i,i+1) = 0   (Benchmarks.scala:18-27)
 struct GMc {
+  template <class S> __device__ static void init(const TriArgs&, S&) {}
   struct Smem {
     union {
       int stage[NWAVE * (T * (T + 1) + T * T + T)];
@@ -410,128 +413,232 @@ struct GMc {
 //   ml1(i,j) = min{ ml1(i+1,j) [j-i>=8]; X; U(i,j); R11(i,j) }
 //   R11(i,j) = min_{i+7<=k<j} U(i,k) = min{R11(i,j-1), U(i,j-1)}     (rule 11)
 // INV marks "no valid candidate", i.e. the reference's 999999.
-struct ZuCtx {
-  const uint8_t* s;
+// Energy tables read by the Zuker fill, staged once per workgroup in LDS
+// (int16, the width of the reference's paramT, vienna.h:24-67).
+struct ZuTab {
+  int16_t stack[8][8];
+  int16_t mmH[8][5][5], mmI[8][5][5], mm1nI[8][5][5], mm23I[8][5][5], mmM[8][5][5], mmExt[8][5][5];
+  int16_t d5[8][5], d3[8][5];
+  int16_t bulge[32], hairpin[32];
+  int16_t F[31][32];
+  int16_t loop_e[3][40];
+  int32_t loop_key[3][40];
+  int32_t n_loop[3];
+};
+
+// Per-cell terms that depend on the sequence only, precomputed one wavefront
+// step ahead by the waves that are idle during the cell-finalize phase.
+struct ZuPre {
+  int16_t valid;  // cl(i,j) exists: j-i >= 7 and both stacked pairs canonical (fun6)
+  int16_t sr;     // sr_energy(i,j-1)
+  int16_t r1;     // rule 1: hairpin(i+1,j-2) + sr
+  int16_t r5c;    // rule 5 without ml: ML_closing + ML_intern + termau + ml_mismatch + sr
+  int16_t uoff;   // U - cl = ext_mismatch(i,j-1) + termau(i,j-1) + ML_intern
+  int16_t boff;   // B - cl = interior mismatch of (i,j) as an inner pair (kNoB if unused)
+  int16_t outI;   // generic interior-loop outer mismatch + sr
+  int16_t in0;    // interior mismatch of (i+1,j-1): recovers cl(i+1,j-1) from B (rule 0)
+  int16_t out1n;  // 1xn loops: outer mismatch1nI + sr
+  int16_t out23;  // 2x3 loops: il(5) + ninio + outer mismatch23I + sr
+  int16_t ocb;    // pair code of (i+1, j-2)
+  int16_t tauo;   // termau(i+1, j-2)
+};
+constexpr int kNoB = -32768;
+
+// A tile's view of the sequence: row side [IT-1, IT+66), column side [JT-34, JT+34).
+struct ZuView {
+  const ZuTab& E;
   const DevRna* P;
   const int32_t* hl;
-  int n;
-  __device__ int S(int k) const { return s[k]; }
-  __device__ int bp(int x, int y) const { return bpc(S(x), S(y)); }
-  __device__ int termau(int x, int y) const {
-    const int a = S(x), b = S(y);
-    return ((a == 3 && b == 2) || (a == 2 && b == 3)) ? 0 : P->term_au;
-  }
-  __device__ int sr(int i, int j) const { return P->stack[bp(i, j)][bp(j - 1, i + 1)]; }
-  // inner mismatch of cell (l,k) (pair l..k-1) as an interior-loop closing pair
-  __device__ int inner_I(int l, int k) const { return P->mmI[bpc(S(k - 1), S(l))][S(k)][S(l - 1)]; }
-  __device__ int ext_mm(int i, int j) const {
-    const int b = bp(i, j);
-    if (i > 0 && j + 1 < n) return P->mmExt[b][S(i - 1)][S(j + 1)];
-    if (i > 0) return P->d5[b][S(i - 1)];
-    if (j + 1 < n) return P->d3[b][S(j + 1)];
-    return 0;
-  }
-  // hairpin closed by (i,j) (librna_impl.h:181-193)
-  __device__ int hairpin(int i, int j) const {
-    const int size = j - i - 1;
-    if (size < 3) return 600;
-    int cls = size == 3 ? 0 : size == 4 ? 1 : size == 6 ? 2 : -1;
-    if (cls >= 0) {
-      int key = 0;
-      for (int m = 0; m <= size + 1; ++m) key = key * 8 + S(i + m);
-      for (int e = 0; e < P->n_loop[cls]; ++e)
-        if (P->loop_key[cls][e] == key) return P->loop_e[cls][e];
-    }
-    const int ent = hl[size];
-    if (size == 3) return ent + termau(i, j);
-    return ent + P->mmH[bp(i, j)][S(i + 1)][S(j - 1)];
-  }
-  // interior loop (i,j) closing (k,l), (librna_impl.h:218-265) for sl,sr >= 1
-  __device__ int il(int i, int k, int l, int j) const {
-    const int sl = k - i - 1, sr_ = j - l - 1;
-    const int ocb = bp(i, j), icb = bp(l, k);
-    if (sl == 1) {
-      if (sr_ == 1) return P->int11[ocb][bp(j - 2, i + 2)][S(i + 1)][S(j - 1)];
-      if (sr_ == 2) return P->int21[ocb][bp(j - 3, i + 2)][S(i + 1)][S(j - 2)][S(j - 1)];
-      return P->F[sl][sr_] + P->mm1nI[ocb][S(i + 1)][S(j - 1)] + P->mm1nI[icb][S(l + 1)][S(k - 1)];
-    }
-    if (sl == 2) {
-      if (sr_ == 1) return P->int21[bp(j - 2, i + 3)][ocb][S(j - 1)][S(i + 1)][S(i + 2)];
-      if (sr_ == 2) return P->int22[ocb][bp(j - 3, i + 3)][S(i + 1)][S(i + 2)][S(j - 2)][S(j - 1)];
-      if (sr_ == 3) return P->il5 + P->ninio0 + P->mm23I[ocb][S(i + 1)][S(j - 1)] + P->mm23I[icb][S(l + 1)][S(k - 1)];
-    } else if (sl == 3 && sr_ == 2) {
-      return P->il5 + P->ninio0 + P->mm23I[ocb][S(i + 1)][S(j - 1)] + P->mm23I[icb][S(l + 1)][S(k - 1)];
-    } else if (sr_ == 1) {
-      return P->F[sl][sr_] + P->mm1nI[ocb][S(i + 1)][S(j - 1)] + P->mm1nI[icb][S(l + 1)][S(k - 1)];
-    }
-    return P->F[sl][sr_] + P->mmI[ocb][S(i + 1)][S(j - 1)] + P->mmI[icb][S(l + 1)][S(k - 1)];
-  }
-  // bulge terms of rules 2 and 3 (bl_energy/br_energy, librna_impl.h:199-215)
-  __device__ int bulge_l(int i, int j, int k) const {  // cl(k,j-2) inside, size k-i-2
-    const int size = k - i - 2;
-    if (size == 1) return P->bulge[1] + P->stack[bp(i + 1, j - 2)][bp(j - 3, k)];
-    return P->bulge[size] + termau(i + 1, j - 2) + termau(j - 3, k);
-  }
-  __device__ int bulge_r(int i, int j, int k) const {  // cl(i+2,k) inside, size j-k-2
-    const int size = j - k - 2;
-    if (size == 1) return P->bulge[1] + P->stack[bp(i + 1, j - 2)][bp(k - 1, i + 2)];
-    return P->bulge[size] + termau(i + 1, j - 2) + termau(k - 1, i + 2);
+  const uint8_t* rs;
+  const uint8_t* cs;
+  int IT1, JT34, n;
+  __device__ int R(int x) const { return rs[x - IT1]; }
+  __device__ int C(int x) const { return cs[x - JT34]; }
+  __device__ int tau(int x, int y) const {  // termau_energy on bases (librna_impl.h:152)
+    return ((x == 3 && y == 2) || (x == 2 && y == 3)) ? 0 : P->term_au;
   }
   __device__ bool stackpair(int i, int j) const {  // fun6 (zu.c:35)
-    return i + 4 <= j && bp(i, j - 1) != 7 && bp(i + 1, j - 2) != 7;
+    return i + 4 <= j && bpc(R(i), C(j - 1)) != 7 && bpc(R(i + 1), C(j - 2)) != 7;
   }
-  __device__ int rule5(int i, int j, int mlv) const {
-    return P->ml_closing + P->ml_intern + mlv + termau(i + 1, j - 2) + P->mmM[bp(j - 2, i + 1)][S(j - 3)][S(i + 2)];
+  // interior mismatch of the pair closing cell (l,k) (pair l..k-1), seen from outside
+  __device__ int inner(int l, int k) const { return E.mmI[bpc(C(k - 1), R(l))][C(k)][R(l - 1)]; }
+  __device__ int sr(int i, int j) const { return E.stack[bpc(R(i), C(j - 1))][bpc(C(j - 2), R(i + 1))]; }
+  __device__ int hairpin(int i, int j) const {  // hl_energy(i+1, j-2) (librna_impl.h:181-193)
+    const int a = i + 1, b = j - 2, size = b - a - 1;
+    const int cls = size == 3 ? 0 : size == 4 ? 1 : size == 6 ? 2 : -1;
+    if (cls >= 0) {  // special loops: the window lies on the row side (size <= 6)
+      int key = 0;
+      for (int m = 0; m <= size + 1; ++m) key = key * 8 + R(a + m);
+      for (int e = 0; e < E.n_loop[cls]; ++e)
+        if (E.loop_key[cls][e] == key) return E.loop_e[cls][e];
+    }
+    const int ent = size <= 30 ? E.hairpin[size] : hl[size];
+    if (size == 3) return ent + tau(R(a), C(b));
+    return ent + E.mmH[bpc(R(a), C(b))][R(a + 1)][C(b - 1)];
+  }
+  __device__ int ext(int i, int j) const {  // ext_mismatch_energy(i, j) with j = cell column - 1
+    const int bb = bpc(R(i), C(j));
+    if (i > 0 && j + 1 < n) return E.mmExt[bb][R(i - 1)][C(j + 1)];
+    if (i > 0) return E.d5[bb][R(i - 1)];
+    if (j + 1 < n) return E.d3[bb][C(j + 1)];
+    return 0;
+  }
+  // il_energy(i+1, l, k-1, j-2) (librna_impl.h:230-265), sizes sl = l-i-2, sr = j-k-2 >= 1
+  __device__ int il(int i, int j, int l, int k) const {
+    const int sl = l - i - 2, srr = j - k - 2;
+    const int ocb = bpc(R(i + 1), C(j - 2)), icb = bpc(C(k - 1), R(l));
+    const int ol = R(i + 2), orr = C(j - 3), il_ = C(k), ir = R(l - 1);
+    if (sl == 1) {
+      if (srr == 1) return P->int11[ocb][bpc(C(j - 4), R(i + 3))][ol][orr];
+      if (srr == 2) return P->int21[ocb][bpc(C(j - 5), R(i + 3))][ol][C(j - 4)][orr];
+      return E.F[sl][srr] + E.mm1nI[ocb][ol][orr] + E.mm1nI[icb][il_][ir];
+    }
+    if (sl == 2) {
+      if (srr == 1) return P->int21[bpc(C(j - 4), R(i + 4))][ocb][orr][ol][R(i + 3)];
+      if (srr == 2) return P->int22[ocb][bpc(C(j - 5), R(i + 4))][ol][R(i + 3)][C(j - 4)][orr];
+      if (srr == 3) return P->il5 + P->ninio0 + E.mm23I[ocb][ol][orr] + E.mm23I[icb][il_][ir];
+    } else if (sl == 3 && srr == 2) {
+      return P->il5 + P->ninio0 + E.mm23I[ocb][ol][orr] + E.mm23I[icb][il_][ir];
+    } else if (srr == 1) {
+      return E.F[sl][srr] + E.mm1nI[ocb][ol][orr] + E.mm1nI[icb][il_][ir];
+    }
+    return E.F[sl][srr] + E.mmI[ocb][ol][orr] + E.mmI[icb][il_][ir];
+  }
+  // bl_energy / br_energy terms of rules 2 and 3 (librna_impl.h:199-215)
+  __device__ int bulge_l(int i, int j, int k) const {  // inner cl(k, j-2), size k-i-2
+    const int size = k - i - 2;
+    if (size == 1) return E.bulge[1] + E.stack[bpc(R(i + 1), C(j - 2))][bpc(C(j - 3), R(k))];
+    return E.bulge[size] + tau(R(i + 1), C(j - 2)) + tau(C(j - 3), R(k));
+  }
+  __device__ int bulge_r(int i, int j, int k) const {  // inner cl(i+2, k), size j-k-2
+    const int size = j - k - 2;
+    if (size == 1) return E.bulge[1] + E.stack[bpc(R(i + 1), C(j - 2))][bpc(C(k - 1), R(i + 2))];
+    return E.bulge[size] + tau(R(i + 1), C(j - 2)) + tau(C(k - 1), R(i + 2));
+  }
+  __device__ void pre(int i, int j, ZuPre* o) const {
+    const int d = j - i;
+    o->valid = 0;
+    if (j > n || d < 7 || !stackpair(i, j)) return;
+    o->valid = 1;
+    const int s_ = sr(i, j);
+    o->sr = s_;
+    o->r1 = hairpin(i, j) + s_;
+    o->r5c = P->ml_closing + P->ml_intern + tau(R(i + 1), C(j - 2)) + E.mmM[bpc(C(j - 2), R(i + 1))][C(j - 3)][R(i + 2)] + s_;
+    o->uoff = ext(i, j - 1) + tau(R(i), C(j - 1)) + P->ml_intern;
+    o->boff = (i >= 1 && j <= n - 1) ? inner(i, j) : kNoB;
+    const int ocb = bpc(R(i + 1), C(j - 2)), ol = R(i + 2), orr = C(j - 3);
+    o->outI = E.mmI[ocb][ol][orr] + s_;
+    o->in0 = d >= 9 ? inner(i + 1, j - 1) : 0;
+    o->out1n = E.mm1nI[ocb][ol][orr] + s_;
+    o->out23 = P->il5 + P->ninio0 + E.mm23I[ocb][ol][orr] + s_;
+    o->ocb = ocb;
+    o->tauo = tau(R(i + 1), C(j - 2));
   }
 };
 
 struct GZu {
   static constexpr int BR = 64, BC = 64;   // B region: rows IT..IT+63, cols JT-32..JT+31
+  // The interior-loop window of a cell on diagonal d reads diagonals <= d-6,
+  // its bulges <= d-5 and its split points inside blocks I, J <= d-7; so the
+  // heavy terms of G consecutive diagonals are computed in one wide phase, and
+  // only the O(1) cell finalize (rules 0, 1, 5, 6, 8, 10 and the prefix mins)
+  // runs diagonal by diagonal.
+  static constexpr int G = 5;
   struct Smem {
     union {
       int stage[NWAVE * (T * (T + 1) + T * T + T)];
       struct {
-        int B[BR][BC + 1];     // cl + inner_I, origin (IT, JT-32)
-        int U[T][T + 2];       // origin (IT, JT-1)
-        int ML[T + 2][T + 3];  // origin (IT, JT-2)
-        int ML1[T + 1][T + 1]; // origin (IT, JT)
-        int R11[T][T + 2];     // origin (IT, JT-1)
-        int LU[T][T + 1];      // U tile (I,I)
-        int RM1[T][T + 1];     // ml1 tile (J,J)
+        int B[BR][BC];         // cl + inner mismatch, origin (IT, JT-32)
+        int U[T][T + 1];       // origin (IT, JT-1)
+        int ML[T + 2][T + 2];  // origin (IT, JT-2)
+        int ML1[T + 1][T];     // origin (IT, JT)
+        int R11[T][T + 1];     // origin (IT, JT-1)
+        int RM1[T][T];         // ml1 tile (J,J)
       } w;
     } u;
     int X[T][T + 1];
-    int16_t F[31][32];
-    int acc[T];
-    int list[2][T];
-    int nlist[2];
+    ZuTab E;
+    ZuPre pre[G][T];
+    uint8_t rs[72], cs[72];
+    int acc[G][T];
+    int16_t list[G * T];
+    int nlist;
   };
 
-  __device__ static int cell_valid_ml(int d) { return d >= 12; }
-
-  __device__ static void build_list(const ZuCtx& z, int IT, int JT, int ld, int* list, int* nlist, int lane) {
-    // cells of local diagonal ld whose cl can use bulges / interior loops
-    const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
-    bool ok = false;
-    if (lane < ncell) {
-      const int r = rmin + lane, i = IT + r, j = JT + r + ld;
-      ok = j <= z.n && j - i >= 12 && z.stackpair(i, j);
+  __device__ static void init(const TriArgs& a, Smem& sm) {
+    const DevRna* P = (const DevRna*)a.rna;
+    ZuTab& E = sm.E;
+    for (int e = threadIdx.x; e < 64; e += NTHR) (&E.stack[0][0])[e] = (int16_t)(&P->stack[0][0])[e];
+    for (int e = threadIdx.x; e < 200; e += NTHR) {
+      (&E.mmH[0][0][0])[e] = (int16_t)(&P->mmH[0][0][0])[e];
+      (&E.mmI[0][0][0])[e] = (int16_t)(&P->mmI[0][0][0])[e];
+      (&E.mm1nI[0][0][0])[e] = (int16_t)(&P->mm1nI[0][0][0])[e];
+      (&E.mm23I[0][0][0])[e] = (int16_t)(&P->mm23I[0][0][0])[e];
+      (&E.mmM[0][0][0])[e] = (int16_t)(&P->mmM[0][0][0])[e];
+      (&E.mmExt[0][0][0])[e] = (int16_t)(&P->mmExt[0][0][0])[e];
     }
-    const unsigned long long m = __ballot(ok);
-    if (ok) list[__popcll(m & ((1ull << lane) - 1))] = lane;
-    if (lane == 0) *nlist = __popcll(m);
+    for (int e = threadIdx.x; e < 40; e += NTHR) {
+      (&E.d5[0][0])[e] = (int16_t)(&P->d5[0][0])[e];
+      (&E.d3[0][0])[e] = (int16_t)(&P->d3[0][0])[e];
+    }
+    for (int e = threadIdx.x; e < 31; e += NTHR) { E.bulge[e] = (int16_t)P->bulge[e]; E.hairpin[e] = (int16_t)P->hairpin[e]; }
+    for (int e = threadIdx.x; e < 31 * 31; e += NTHR) E.F[e / 31][e % 31] = P->F[e / 31][e % 31];
+    for (int e = threadIdx.x; e < 120; e += NTHR) {
+      (&E.loop_key[0][0])[e] = (&P->loop_key[0][0])[e];
+      (&E.loop_e[0][0])[e] = (int16_t)(&P->loop_e[0][0])[e];
+    }
+    if (threadIdx.x < 3) E.n_loop[threadIdx.x] = P->n_loop[threadIdx.x];
+  }
+
+  // sequence-only terms of the cells of band [ld0, ld0+G) and the compacted
+  // list of cells whose cl can close bulges / interior loops; all threads of
+  // `nthr` starting at thread index `t0` take part, wave `lw` builds the list
+  __device__ static void prep_band(const ZuView& z, int IT, int JT, int ld0, Smem& sm, int t) {
+    // t in [0, 192): t < 64 -> list (one wave), else per-cell terms
+    if (t < 64) {
+      int base = 0;
+      for (int q = 0; q < G; ++q) {
+        const int ld = ld0 + q;
+        bool ok = false;
+        if (ld <= T - 1) {
+          const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
+          if (t < ncell) {
+            const int r = rmin + t, i = IT + r, j = JT + r + ld;
+            ok = j <= z.n && j - i >= 12 && z.stackpair(i, j);
+          }
+        }
+        const unsigned long long m = __ballot(ok);
+        if (ok) sm.list[base + __popcll(m & ((1ull << t) - 1))] = (int16_t)(q * T + t);
+        base += __popcll(m);
+      }
+      if (t == 0) sm.nlist = base;
+    } else {
+      for (int e = t - 64; e < G * T; e += 128) {
+        const int q = e / T, p = e % T, ld = ld0 + q;
+        if (ld > T - 1) continue;
+        const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
+        if (p >= ncell) continue;
+        const int r = rmin + p;
+        z.pre(IT + r, JT + r + ld, &sm.pre[q][p]);
+      }
+    }
   }
 
   __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm) {
     const int tid = threadIdx.x, IT = I * T, JT = J * T, n = in.n, nb = in.nb;
-    ZuCtx z{a.codes + in.seq_off, (const DevRna*)a.rna, a.hl_ent, n};
-    const DevRna* P = z.P;
+    const uint8_t* seq = a.codes + in.seq_off;
+    const DevRna* P = (const DevRna*)a.rna;
+    const ZuView z{sm.E, P, a.hl_ent, sm.rs, sm.cs, IT - 1, JT - 34, n};
     int acc[4][4];
     for (int t = 0; t < 4; ++t)
       for (int u = 0; u < 4; ++u) acc[t][u] = INV;
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = INV;
-    for (int e = tid; e < 31 * 31; e += NTHR) sm.F[e / 31][e % 31] = P->F[e / 31][e % 31];
-    if (tid < T) sm.acc[tid] = INV;
+    for (int e = tid; e < G * T; e += NTHR) (&sm.acc[0][0])[e] = INV;
+    if (tid < 72) {
+      const int x = IT - 1 + tid, y = JT - 34 + tid;
+      sm.rs[tid] = (x >= 0 && x < n) ? seq[x] : 0;
+      sm.cs[tid] = (y >= 0 && y < n) ? seq[y] : 0;
+    }
     if (J - I >= 2) middle_product<false, false>(a, in, NT_ZU, ZT_U, ZT_ML1, I, J, sm.u.stage, acc);
     __syncthreads();
     {
@@ -553,7 +660,7 @@ struct GZu {
       int v = INV;
       if (!(rr < T && cc >= 32)) {
         const int cl = fetch(ZT_CL, l, k);
-        if (cl < INV_THR && l >= 1 && k <= n - 1) v = cl + z.inner_I(l, k);
+        if (cl < INV_THR && l >= 1 && k <= n - 1) v = cl + z.inner(l, k);
       }
       sm.u.w.B[rr][cc] = v;
     }
@@ -570,28 +677,34 @@ struct GZu {
       const int rr = e / T, cc = e % T;
       sm.u.w.ML1[rr][cc] = rr == T ? fetch(ZT_ML1, IT + T, JT + cc) : INV;
     }
+    const int* gLU = rec_ptr(a, in, NT_ZU, I, I) + ZT_U * T * T;  // U tile (I,I), read in place
     if (I < J) {
-      const int* gL = rec_ptr(a, in, NT_ZU, I, I) + ZT_U * T * T;
       const int* gR = rec_ptr(a, in, NT_ZU, J, J) + ZT_ML1 * T * T;
-      for (int e = tid; e < T * T; e += NTHR) {
-        sm.u.w.LU[e / T][e % T] = gL[e];
-        sm.u.w.RM1[e / T][e % T] = gR[e];
-      }
+      for (int e = tid; e < T * T; e += NTHR) sm.u.w.RM1[e / T][e % T] = gR[e];
     }
     const int dlo = I == J ? 0 : -(T - 1);
-    if (tid >= 64 && tid < 128) build_list(z, IT, JT, dlo, sm.list[0], &sm.nlist[0], tid - 64);
+    __syncthreads();  // sequence segments visible to prep_band
+    if (tid >= 64) prep_band(z, IT, JT, dlo, sm, tid - 64);
     __syncthreads();
     int32_t* gcl = rec_ptr(a, in, NT_ZU, I, J) + ZT_CL * T * T;
-    int buf = 0;
-    for (int ld = dlo; ld <= T - 1; ++ld, buf ^= 1) {
-      const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
-      const int nvalid = sm.nlist[buf];
-      const int total = ncell * 4 + nvalid * 34;
-      // ---------------- phase A ----------------
+    for (int ld0 = dlo; ld0 <= T - 1; ld0 += G) {
+      // ---------------- heavy phase: band [ld0, ld0+G) ----------------
+      const int qn = imin(G, T - ld0);
+      int nedge[G + 1];
+      nedge[0] = 0;
+      for (int q = 0; q < G; ++q) {
+        const int ld = ld0 + q;
+        nedge[q + 1] = nedge[q] + (q < qn ? 4 * (T - (ld < 0 ? -ld : ld)) : 0);
+      }
+      const int ne = nedge[G], nv = sm.nlist, nw = nv * 30;
+      const int total = ne + nw + nv * 4;
       for (int w = tid; w < total; w += NTHR) {
-        if (w < ncell * 4) {
+        if (w < ne) {
           // split points of X inside blocks I and J
-          const int p = w >> 2, part = w & 3, r = rmin + p, c = r + ld, i = IT + r, j = JT + c;
+          int q = 0;
+          while (w >= nedge[q + 1]) ++q;
+          const int ld = ld0 + q, rmin = ld < 0 ? -ld : 0;
+          const int we = w - nedge[q], p = we >> 2, part = we & 3, r = rmin + p, c = r + ld, i = IT + r, j = JT + c;
           if (j > n || j - i < 14) continue;
           int k1lo, k1hi, k2lo, k2hi;
           if (I == J) { k1lo = i + 7; k1hi = j - 7; k2lo = 1; k2hi = 0; }
@@ -603,7 +716,7 @@ struct GZu {
             int v;
             if (x < n1) {
               const int k = k1lo + x;
-              const int left = I == J ? sm.u.w.U[r][k - JT + 1] : sm.u.w.LU[r][k - IT];
+              const int left = I == J ? sm.u.w.U[r][k - JT + 1] : gLU[r * T + (k - IT)];
               v = left + sm.u.w.ML1[k - IT][c];
             } else {
               const int k = k2lo + (x - n1);
@@ -612,81 +725,113 @@ struct GZu {
             best = imin(best, v);
           }
           if (best < INV_THR) atomicMin(&sm.X[r][c], best);
-        } else {
-          const int w2 = w - ncell * 4, v = w2 / 34, slot = w2 % 34;
-          const int p = sm.list[buf][v], r = rmin + p, c = r + ld, i = IT + r, j = JT + c, d = j - i;
-          const int srv = z.sr(i, j - 1);
-          int best = INV;
-          if (slot < 30) {
-            // interior loops with left size a = slot+1 (rule 4)
-            const int aa = slot + 1, bmax = imin(30, d - 11 - aa);
-            if (bmax >= 1) {
-              const int lrow = r + 2 + aa, l = i + 2 + aa;
+          continue;
+        }
+        const bool win = w < ne + nw;
+        const int w2 = win ? w - ne : w - ne - nw;
+        const int v = win ? w2 / 30 : w2 >> 2, slot = win ? w2 % 30 : (w2 & 3);
+        const int qp = sm.list[v], q = qp / T, p = qp % T, ld = ld0 + q;
+        const int r = (ld < 0 ? -ld : 0) + p, c = r + ld, i = IT + r, j = JT + c, d = j - i;
+        const ZuPre& pc = sm.pre[q][p];
+        int best = INV;
+        if (win) {
+          // interior loops with left size a = slot+1 (rule 4): l = i+2+a fixed
+          const int aa = slot + 1, bmax = imin(30, d - 11 - aa);
+          if (bmax >= 1) {
+            const int l = i + 2 + aa;
+            const int* Brow = &sm.u.w.B[r + 2 + aa][c + 30];  // Brow[-b] = B(l, j-2-b)
+            if (aa >= 2) {
               int gen = INV;
-              if (aa >= 2) {
-                const int bst = aa == 2 ? 4 : aa == 3 ? 3 : 2;
-                for (int b = bst; b <= bmax; ++b) gen = imin(gen, sm.u.w.B[lrow][c + 30 - b] + sm.F[aa][b]);
-                if (gen < INV_THR) gen += P->mmI[z.bp(i + 1, j - 2)][z.S(i + 2)][z.S(j - 3)];
-              }
-              best = gen;
-              // the special sizes: a==1 row, b==1 column, (2,2) (2,3) (3,2)
-              const int bsp = aa == 1 ? bmax : aa == 2 ? imin(3, bmax) : aa == 3 ? imin(2, bmax) : 1;
-              for (int b = 1; b <= bsp; ++b) {
-                const int k = j - 2 - b;
-                const int Bv = sm.u.w.B[lrow][c + 30 - b];
-                if (Bv >= INV_THR) continue;
-                const int cl = Bv - z.inner_I(l, k);
-                best = imin(best, cl + z.il(i + 1, l, k - 1, j - 2));
-              }
+              const int bst = aa == 2 ? 4 : aa == 3 ? 3 : 2;
+              for (int b = bst; b <= bmax; ++b) gen = imin(gen, Brow[-b] + sm.E.F[aa][b]);
+              if (gen < INV_THR) best = gen + pc.outI;
             }
-          } else if (slot < 32) {
-            // left bulges (rule 2), half of the k range each
-            if (d >= 12) {
-              const int klo = i + 3, khi = imin(j - 9, i + 32), len = khi - klo + 1;
-              const int h = slot - 30, x0 = klo + h * len / 2, x1 = klo + (h + 1) * len / 2;
-              for (int k = x0; k < x1; ++k) {
-                const int Bv = sm.u.w.B[k - IT][c + 30];
-                if (Bv >= INV_THR) continue;
-                best = imin(best, Bv - z.inner_I(k, j - 2) + z.bulge_l(i, j, k));
+            // special sizes: a==1 row, b==1 column, (2,2) (2,3) (3,2); cl = B - inner
+            const int Rl = z.R(l), Rl1 = z.R(l - 1);
+            const int bsp = aa == 1 ? bmax : aa == 2 ? imin(3, bmax) : aa == 3 ? imin(2, bmax) : 1;
+            int Ck = z.C(j - 3);
+            for (int b = 1; b <= bsp; ++b) {
+              const int k = j - 2 - b, Ck1 = z.C(k - 1);
+              const int Bv = Brow[-b];
+              if (Bv < INV_THR) {
+                const int icb = bpc(Ck1, Rl);
+                const int cl = Bv - sm.E.mmI[icb][Ck][Rl1];
+                int e;
+                if ((aa == 1 && b >= 3) || (b == 1 && aa >= 3))
+                  e = sm.E.F[aa][b] + sm.E.mm1nI[icb][Ck][Rl1] + pc.out1n;
+                else if ((aa == 2 && b == 3) || (aa == 3 && b == 2))
+                  e = sm.E.mm23I[icb][Ck][Rl1] + pc.out23;
+                else
+                  e = z.il(i, j, l, k) + pc.sr;  // 1x1, 1x2, 2x1, 2x2 tables
+                best = imin(best, cl + e);
               }
-            }
-          } else {
-            // right bulges (rule 3)
-            if (d >= 12) {
-              const int klo = imax(i + 9, j - 32), khi = j - 3, len = khi - klo + 1;
-              const int h = slot - 32, x0 = klo + h * len / 2, x1 = klo + (h + 1) * len / 2;
-              for (int k = x0; k < x1; ++k) {
-                const int Bv = sm.u.w.B[r + 2][k - JT + 32];
-                if (Bv >= INV_THR) continue;
-                best = imin(best, Bv - z.inner_I(i + 2, k) + z.bulge_r(i, j, k));
-              }
+              Ck = Ck1;
             }
           }
-          if (best < INV_THR) atomicMin(&sm.acc[p], best + srv);
+        } else if (slot < 2) {
+          // left bulges (rule 2): inner cl(k, j-2), size k-i-2; half of the k range each
+          const int klo = i + 3, khi = imin(j - 9, i + 32), len = khi - klo + 1;
+          const int x0 = klo + slot * len / 2, x1 = klo + (slot + 1) * len / 2;
+          const int Cj3 = z.C(j - 3), Cj2 = z.C(j - 2);
+          const int* Bcol = &sm.u.w.B[0][c + 30];
+          int Rk1 = x0 < x1 ? z.R(x0 - 1) : 0;
+          for (int k = x0; k < x1; ++k) {
+            const int Rk = z.R(k);
+            const int Bv = Bcol[(k - IT) * BC];
+            if (Bv < INV_THR) {
+              const int icb = bpc(Cj3, Rk);
+              const int cl = Bv - sm.E.mmI[icb][Cj2][Rk1];
+              const int size = k - i - 2;
+              const int e = size == 1 ? sm.E.bulge[1] + sm.E.stack[pc.ocb][icb]
+                                      : sm.E.bulge[size] + pc.tauo + z.tau(Cj3, Rk);
+              best = imin(best, cl + e + pc.sr);
+            }
+            Rk1 = Rk;
+          }
+        } else {
+          // right bulges (rule 3): inner cl(i+2, k), size j-k-2
+          const int klo = imax(i + 9, j - 32), khi = j - 3, len = khi - klo + 1;
+          const int h = slot - 2, x0 = klo + h * len / 2, x1 = klo + (h + 1) * len / 2;
+          const int Ri2 = z.R(i + 2), Ri1 = z.R(i + 1);
+          const int* Brow = &sm.u.w.B[r + 2][32 - JT];
+          int Ck1 = x0 < x1 ? z.C(x0 - 1) : 0;
+          for (int k = x0; k < x1; ++k) {
+            const int Ck = z.C(k);
+            const int Bv = Brow[k];
+            if (Bv < INV_THR) {
+              const int icb = bpc(Ck1, Ri2);
+              const int cl = Bv - sm.E.mmI[icb][Ck][Ri1];
+              const int size = j - k - 2;
+              const int e = size == 1 ? sm.E.bulge[1] + sm.E.stack[pc.ocb][icb]
+                                      : sm.E.bulge[size] + pc.tauo + z.tau(Ck1, Ri2);
+              best = imin(best, cl + e + pc.sr);
+            }
+            Ck1 = Ck;
+          }
         }
+        if (best < INV_THR) atomicMin(&sm.acc[q][p], best);
       }
       __syncthreads();
-      // ---------------- phase B: one thread per cell ----------------
-      if (tid < T) {
-        const int p = tid;
-        if (p < ncell) {
-          const int r = rmin + p, c = r + ld, i = IT + r, j = JT + c, d = j - i;
+      // ---------------- light steps: finalize the band diagonal by diagonal ----------------
+      for (int q = 0; q < qn; ++q) {
+        const int ld = ld0 + q, rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
+        if (tid < ncell) {
+          const int p = tid, r = rmin + p, c = r + ld, i = IT + r, j = JT + c, d = j - i;
           int cl = INV, U = INV, ml = INV, ml1 = INV, R11 = INV, Bv = INV;
           if (i <= j && j <= n) {
-            if (d >= 7 && z.stackpair(i, j)) {
-              const int srv = z.sr(i, j - 1);
-              cl = sm.acc[p];
+            const ZuPre& pc = sm.pre[q][p];
+            if (pc.valid) {
+              cl = imin(sm.acc[q][p], pc.r1);
               if (d >= 9) {
                 const int Bi = sm.u.w.B[r + 1][c + 31];
-                if (Bi < INV_THR) cl = imin(cl, Bi - z.inner_I(i + 1, j - 1) + srv);
+                if (Bi < INV_THR) cl = imin(cl, Bi - pc.in0 + pc.sr);
               }
-              cl = imin(cl, z.hairpin(i + 1, j - 2) + srv);
               if (d >= 18) {
                 const int mlv = sm.u.w.ML[r + 2][c];
-                if (mlv < INV_THR) cl = imin(cl, z.rule5(i, j, mlv) + srv);
+                if (mlv < INV_THR) cl = imin(cl, mlv + pc.r5c);
               }
-              U = cl + z.ext_mm(i, j - 1) + z.termau(i, j - 1) + P->ml_intern;
-              if (i >= 1 && j <= n - 1) Bv = cl + z.inner_I(i, j);
+              U = cl + pc.uoff;
+              if (pc.boff != kNoB) Bv = cl + pc.boff;
             }
             int X = sm.X[r][c];
             if (X >= INV_THR) X = INV;
@@ -708,11 +853,12 @@ struct GZu {
           sm.u.w.ML1[r][c] = ml1;
           sm.u.w.R11[r][c + 1] = R11;
           gcl[r * T + c] = cl;
+          sm.acc[q][p] = INV;
         }
-        sm.acc[p] = INV;
-      } else if (tid >= 64 && tid < 128 && ld + 1 <= T - 1) {
-        build_list(z, IT, JT, ld + 1, sm.list[buf ^ 1], &sm.nlist[buf ^ 1], tid - 64);
+        __syncthreads();
       }
+      // terms of the next band (reads only the sequence: safe to overwrite now)
+      if (ld0 + G <= T - 1 && tid >= 64) prep_band(z, IT, JT, ld0 + G, sm, tid - 64);
       __syncthreads();
     }
     int32_t* g = rec_ptr(a, in, NT_ZU, I, J);
