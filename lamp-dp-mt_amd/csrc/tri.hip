@@ -420,7 +420,7 @@ struct ZuTab {
   int16_t mmH[8][5][5], mmI[8][5][5], mm1nI[8][5][5], mm23I[8][5][5], mmM[8][5][5], mmExt[8][5][5];
   int16_t d5[8][5], d3[8][5];
   int16_t bulge[32], hairpin[32];
-  int16_t F[31][32];
+  int16_t F[31][34];  // row stride of 17 words: lanes with different a hit distinct banks
   int16_t loop_e[3][40];
   int32_t loop_key[3][40];
   int32_t n_loop[3];
@@ -538,6 +538,7 @@ struct ZuView {
 
 struct GZu {
   static constexpr int BR = 64, BC = 64;   // B region: rows IT..IT+63, cols JT-32..JT+31
+  static constexpr int BS = BC + 1;        // odd row stride: lanes walking a column hit distinct banks
   // The interior-loop window of a cell on diagonal d reads diagonals <= d-6,
   // its bulges <= d-5 and its split points inside blocks I, J <= d-7; so the
   // heavy terms of G consecutive diagonals are computed in one wide phase, and
@@ -548,12 +549,12 @@ struct GZu {
     union {
       int stage[NWAVE * (T * (T + 1) + T * T + T)];
       struct {
-        int B[BR][BC];         // cl + inner mismatch, origin (IT, JT-32)
+        int B[BR][BS];         // cl + inner mismatch, origin (IT, JT-32)
         int U[T][T + 1];       // origin (IT, JT-1)
         int ML[T + 2][T + 2];  // origin (IT, JT-2)
-        int ML1[T + 1][T];     // origin (IT, JT)
+        int ML1[T + 1][T + 1]; // origin (IT, JT)
         int R11[T][T + 1];     // origin (IT, JT-1)
-        int RM1[T][T];         // ml1 tile (J,J)
+        int RM1[T][T + 1];     // ml1 tile (J,J)
       } w;
     } u;
     int X[T][T + 1];
@@ -773,11 +774,11 @@ struct GZu {
           const int klo = i + 3, khi = imin(j - 9, i + 32), len = khi - klo + 1;
           const int x0 = klo + slot * len / 2, x1 = klo + (slot + 1) * len / 2;
           const int Cj3 = z.C(j - 3), Cj2 = z.C(j - 2);
-          const int* Bcol = &sm.u.w.B[0][c + 30];
+          const int* Bcol = &sm.u.w.B[0][c + 30];  // Bcol[x * BS] = B(IT + x, j-2)
           int Rk1 = x0 < x1 ? z.R(x0 - 1) : 0;
           for (int k = x0; k < x1; ++k) {
             const int Rk = z.R(k);
-            const int Bv = Bcol[(k - IT) * BC];
+            const int Bv = Bcol[(k - IT) * BS];
             if (Bv < INV_THR) {
               const int icb = bpc(Cj3, Rk);
               const int cl = Bv - sm.E.mmI[icb][Cj2][Rk1];
