@@ -106,9 +106,81 @@ int dpx_zuker_batch(dpx_ctx* ctx, const char* seqs, const int64_t* offs, int cou
 int dpx_nussinov_batch(dpx_ctx* ctx, const char* seqs, const int64_t* offs, int count, int min_loop,
                        int32_t* results, dpx_trace_t* traces, const int64_t* trace_offs, int32_t* trace_lens);
 
-/* Batch timing: the last batch call's device time (ms) for the fill
- * (persistent tile kernel) and the finalize (st + backtrace) kernels. */
+/* Timing of the last synchronous call on this context: device time (ms) of
+ * the fill kernel (persistent tile queue / alignment bands) and of the
+ * finalize kernels (answer, st column, backtrace). */
 int dpx_last_timing(const dpx_ctx* ctx, float* fill_ms, float* finalize_ms);
+
+/* Seconds on the host's monotonic clock, the time base of dpx_result. */
+double dpx_now(void);
+
+/* ---- serving: continuous batching -----------------------------------------
+ * No reference counterpart (the reference solves one problem per loaded
+ * library, CodeGen.scala:287-325).  A server owns `slots` streams and one
+ * device arena of hbm_budget bytes.  Submitted requests wait in arrival
+ * order; whenever a slot is free the oldest waiting request starts a launch,
+ * joined by later requests of the same kind, options and neighbouring size
+ * class (sizes within a factor of about 4), up to max_batch, while their DP
+ * footprint fits the arena.  Launches on different slots run concurrently on
+ * the device, so requests arriving while others run are admitted as soon as
+ * a slot and memory free up.  Requests whose footprint alone exceeds the
+ * budget are refused at submission with DPX_ENOMEM.  Results are identical
+ * to the synchronous entry points for the same input.
+ * The context must outlive its servers, and dpx_set_rna_params returns
+ * DPX_EBUSY while a server exists. */
+typedef struct dpx_server dpx_server;
+
+enum { DPX_REQ_NUSSINOV = 1, DPX_REQ_ZUKER = 2, DPX_REQ_SWAT = 3, DPX_REQ_MATCHAIN = 4 };
+enum { DPX_SUBMIT_TRACE = 1, DPX_SUBMIT_TRACE_FROM_BEST = 2 };
+
+typedef struct {
+  int64_t hbm_budget;   /* bytes of device memory for the DP matrices in flight (>= 1 MiB) */
+  int32_t slots;        /* launches in flight at once, one stream each (1..8) */
+  int32_t max_batch;    /* most requests in one triangular launch */
+  int32_t max_pending;  /* waiting requests beyond this are refused with DPX_EBUSY */
+  int32_t reserved;
+} dpx_server_config;
+
+typedef struct {
+  int32_t kind;             /* DPX_REQ_* */
+  int32_t flags;            /* DPX_SUBMIT_* */
+  const char* a; int32_t n1;  /* RNA sequence; SW first sequence; matrix chain: n1 = chain length */
+  const char* b; int32_t n2;  /* SW second sequence */
+  const int32_t* rows; const int32_t* cols;  /* matrix chain dimensions, n1 each */
+  dpx_swat_scores sc;       /* SW scores */
+  int32_t min_loop;         /* Nussinov */
+} dpx_request;
+
+typedef struct {
+  uint64_t id;
+  int32_t kind, status;             /* status: DPX_OK or the error of the launch */
+  int32_t score;                    /* MFE / Nussinov score / chain cost / SW axiom M(n1,n2) */
+  int32_t best, best_i, best_j;     /* SW only (see dpx_swat) */
+  int32_t trace_len, batch_size;    /* records kept for dpx_server_trace; requests in the launch */
+  int64_t cells;                    /* DP cells of this request */
+  int64_t footprint;                /* device bytes admitted for it */
+  double t_submit, t_launch, t_done;  /* dpx_now() at submission, launch and completion */
+  float device_ms;                  /* device time of the launch that carried it */
+  int32_t reserved;
+} dpx_result;
+
+typedef struct {
+  int64_t submitted, completed, failed, rejected, launches, cells_done;
+  int64_t bytes_in_flight, peak_bytes_in_flight, max_batch_seen, pending;
+} dpx_server_stats;
+
+int dpx_server_create(dpx_ctx* ctx, const dpx_server_config* cfg, dpx_server** out);
+/* Finishes the launches in flight, drops waiting requests, frees the arena. */
+void dpx_server_destroy(dpx_server* srv);
+/* Copies the request's inputs; *id identifies it in dpx_result. */
+int dpx_server_submit(dpx_server* srv, const dpx_request* req, uint64_t* id);
+/* Up to max finished requests, waiting up to wait_us for the first one;
+ * DPX_EAGAIN when none finished. */
+int dpx_server_poll(dpx_server* srv, dpx_result* out, int max, int wait_us, int* got);
+/* The backtrace of a polled request that asked for one.  buf == NULL drops
+ * it; cap smaller than the trace gives DPX_ETRACE and keeps it. */
+int dpx_server_trace(dpx_server* srv, uint64_t id, dpx_trace_t* buf, int cap, int* len);
+int dpx_server_get_stats(dpx_server* srv, dpx_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -1,15 +1,28 @@
 // api.cpp -- C ABI of libdpx.so (declared in include/dpx.h).
 //
-// Host side of the backend: owns the device, a stream, the resident Turner
-// tables and grow-only DP scratch, turns the reference-style entry points
-// (one instance or many) into one persistent fill launch plus one finalize
-// launch, and copies answers and traces back.
+// Host side of the backend.  A context owns the device, the resident Turner
+// tables and one workspace (stream, events, pinned staging) for the
+// synchronous entry points.  Every computation is a Job: prepared on the
+// host, given one contiguous block of device memory laid out by the job
+// itself, launched asynchronously on a workspace (copies in, fill kernel,
+// finalize kernel, copies out into pinned memory, one completion event) and
+// harvested once that event has completed.  The server (continuous batching)
+// runs jobs on several workspaces at once, carving their device blocks out of
+// one arena whose size is the HBM budget.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/dpx.h"
@@ -19,6 +32,12 @@
 using namespace dpx;
 
 namespace {
+
+constexpr int kMaxZuker = 15999;      // finalize keeps the st column (n+1 int32) in 64 KiB of LDS
+constexpr int kMaxTri = 30000;        // Nussinov / matrix chain
+constexpr int kHairpinTable = 32768;  // hairpin entropy entries kept on the device
+
+size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct DevBuf {
   void* p = nullptr;
@@ -34,189 +53,760 @@ struct DevBuf {
   void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
 };
 
+struct HostBuf {  // pinned host memory, so that the copies are asynchronous
+  void* p = nullptr;
+  size_t cap = 0;
+  bool ensure(size_t bytes) {
+    if (bytes <= cap && p) return true;
+    if (p) { (void)hipHostFree(p); p = nullptr; cap = 0; }
+    size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 16);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) { p = nullptr; return false; }
+    cap = want;
+    return true;
+  }
+  void release() { if (p) (void)hipHostFree(p); p = nullptr; cap = 0; }
+};
+
+struct Workspace {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  HostBuf h_in, h_out;
+  bool init() {
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return false;
+    for (auto& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) return false;
+    return true;
+  }
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    h_in.release();
+    h_out.release();
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+enum { JOB_TRI = 0, JOB_SW = 1 };
+// device block of a triangular job
+enum { TL_TAB, TL_QUEUE, TL_DEP, TL_CTRL, TL_INST, TL_SEEDS, TL_DATA, TL_RES, TL_LENS, TL_TRACES, TL_N };
+// device block of an alignment job
+enum { SL_A, SL_B, SL_ROWS, SL_FLAGS, SL_BEST, SL_BT, SL_TR, SL_N };
+
+struct Job {
+  int kind = JOB_TRI;
+  size_t off[TL_N] = {0};
+  size_t dev_bytes = 0;  // size of the job's device block
+  // triangular grammars
+  int g = G_ZUKER, count = 0, n_max = 0, min_loop = 0;
+  bool want_trace = false;
+  int64_t tiles = 0, trace_total = 0;
+  std::vector<TriInst> inst;
+  std::vector<unsigned long long> seeds;
+  std::vector<uint8_t> data;  // codes (RNA) or rows|cols (int32, matrix chain)
+  // Smith-Waterman
+  std::string a, b;
+  dpx_swat_scores sc{};
+  int trace_from = 0, sw_cap = 0;
+  std::vector<int2> row0;
+  std::vector<uint8_t> r0m, r0g2;
+  int row0_best = 0, row0_bestj = 0, row0_axiom = 0;
+  // outputs: views into the workspace's pinned buffer, valid after the event
+  const int32_t* res = nullptr;
+  const int32_t* lens = nullptr;
+  const int32_t* flags = nullptr;
+  const dpx_trace_t* tr = nullptr;
+  float t_fill = 0, t_fin = 0;
+  int64_t cells = 0;
+};
+
 }  // namespace
 
 struct dpx_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
   std::string err;
   std::mutex mu;
   int cu_count = 0;
-  int occ[3] = {0, 0, 0};
-  // energy model
+  int occ[3] = {1, 1, 1};
+  int sw_occ = 1;
   bool has_rna = false;
   HostRna* rna = nullptr;
   DevBuf d_rna, d_hl;
-  int hl_n = -1;
-  // scratch
-  DevBuf tab, queue, dep, ctrl, inst, seeds, codes, dims, results, traces, lens;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  Workspace ws;
+  DevBuf block;  // device block of the synchronous entry points (grow-only)
   float t_fill = 0, t_fin = 0;
-  // SW scratch
-  DevBuf sw_a, sw_b, sw_bt, sw_row, sw_flag, sw_best, sw_out, sw_tr;
+  std::atomic<int> servers{0};
 };
 
 namespace {
 
-int fail(dpx_ctx* c, int code, const std::string& msg) {
-  c->err = msg;
+int fail(std::string& err, int code, const std::string& msg) {
+  err = msg;
   return code;
 }
 
-int hip_fail(dpx_ctx* c, hipError_t e, const char* what) {
-  return fail(c, DPX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+int hip_fail(std::string& err, hipError_t e, const char* what) {
+  return fail(err, DPX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIPCHK(call)                                   \
-  do {                                                 \
-    hipError_t _e = (call);                            \
-    if (_e != hipSuccess) return hip_fail(c, _e, #call); \
+#define HIPCHK(call)                                       \
+  do {                                                     \
+    hipError_t _e = (call);                                \
+    if (_e != hipSuccess) return hip_fail(err, _e, #call); \
   } while (0)
 
-int encode_rna(dpx_ctx* c, const char* seq, int n, uint8_t* out) {
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int encode_rna(std::string& err, const char* seq, int n, uint8_t* out) {
   for (int q = 0; q < n; ++q) {
     switch (seq[q]) {
       case 'A': case 'a': out[q] = 1; break;
       case 'C': case 'c': out[q] = 2; break;
       case 'G': case 'g': out[q] = 3; break;
       case 'U': case 'u': case 'T': case 't': out[q] = 4; break;
-      default: return fail(c, DPX_EINVAL, std::string("symbol outside ACGU at position ") + std::to_string(q));
+      default: return fail(err, DPX_EINVAL, std::string("symbol outside ACGU at position ") + std::to_string(q));
     }
   }
-  return DPX_OK;
-}
-
-int ensure_hl(dpx_ctx* c, int n_max) {
-  if (n_max <= c->hl_n) return DPX_OK;
-  const int len = std::max(n_max, 64) + 1;
-  std::vector<int32_t> hl(len);
-  for (int l = 0; l < len; ++l) hl[l] = l < 3 ? 0 : c->rna->hl_ent(l);
-  if (!c->d_hl.ensure(sizeof(int32_t) * len)) return fail(c, DPX_ENOMEM, "hairpin table");
-  HIPCHK(hipMemcpy(c->d_hl.p, hl.data(), sizeof(int32_t) * len, hipMemcpyHostToDevice));
-  c->hl_n = len - 1;
   return DPX_OK;
 }
 
 int nt_of(int g) { return g == G_ZUKER ? NT_ZU : g == G_NUSSINOV ? NT_NU : NT_MC; }
+int64_t tri_cells(int n) { return (int64_t)(n + 1) * (n + 2) / 2; }
+int tri_trace_cap(int n) { return 4 * n + 8; }
+int max_len(int g) { return g == G_ZUKER ? kMaxZuker : kMaxTri; }
 
-// One fill + finalize launch over `count` instances of grammar g.
-// lens[q]: instance size; data: codes (RNA) or rows|cols (chain) per instance.
-int run_tri(dpx_ctx* c, int g, int count, const std::vector<int>& lens, const std::vector<int64_t>& data_off,
-            const void* data, size_t data_bytes, int min_loop, int32_t* results, dpx_trace_t* traces,
-            const int64_t* trace_offs, int32_t* trace_lens) {
-  if (count <= 0) return DPX_OK;
-  const int NTv = nt_of(g);
-  std::vector<TriInst> inst(count);
-  std::vector<int> order(count);
-  std::iota(order.begin(), order.end(), 0);
-  // seed the biggest instances first: their critical path is the longest
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return lens[x] > lens[y]; });
-  int64_t tiles = 0, trace_total = 0;
-  int n_max = 0;
-  const bool want_trace = traces != nullptr;
+// ---------------------------------------------------------------- layouts
+
+// Device block of a triangular launch over instances with the given totals.
+size_t tri_layout(int g, int64_t tiles, int count, size_t data_bytes, int64_t trace_total, size_t* off) {
+  size_t sz[TL_N];
+  sz[TL_TAB] = (size_t)tiles * nt_of(g) * T * T * sizeof(int32_t);
+  sz[TL_QUEUE] = (size_t)tiles * 8;
+  sz[TL_DEP] = (size_t)tiles * 4;
+  sz[TL_CTRL] = 64;
+  sz[TL_INST] = sizeof(TriInst) * (size_t)count;
+  sz[TL_SEEDS] = (size_t)tiles * 8;
+  sz[TL_DATA] = data_bytes + 64;
+  sz[TL_RES] = 4 * (size_t)count;
+  sz[TL_LENS] = 4 * (size_t)count;
+  sz[TL_TRACES] = sizeof(dpx_trace_t) * (size_t)trace_total;
+  size_t at = 0;
+  for (int k = 0; k < TL_N; ++k) {
+    if (off) off[k] = at;
+    at += a256(sz[k]);
+  }
+  return at;
+}
+
+// footprint of one instance launched alone; a batch needs at most the sum
+size_t tri_footprint(int g, int n, bool trace) {
+  const int nb = (n + 1 + T - 1) / T;
+  const size_t data = g == G_MATCHAIN ? 8 * (size_t)n : (size_t)n + 1;
+  return tri_layout(g, tiles_of(nb), 1, data, trace ? tri_trace_cap(n) : 0, nullptr);
+}
+
+size_t sw_layout(int n1, int n2, int cap, size_t* off) {
+  const size_t nbands = (n1 + 63) / 64, nwords = (n2 + 64 + 7) / 8, W = (size_t)n2 + 1;
+  size_t sz[SL_N];
+  sz[SL_A] = (size_t)n1 + 64;
+  sz[SL_B] = (size_t)n2 + 64;
+  sz[SL_ROWS] = (nbands + 2) * W * 8 + 2 * W + 64;  // boundary rows, then row 0 and its rule bytes
+  sz[SL_FLAGS] = 4 * (nbands + 2) + 64;
+  sz[SL_BEST] = 64;
+  sz[SL_BT] = nbands * nwords * 64 * 4;
+  sz[SL_TR] = sizeof(dpx_trace_t) * (size_t)(cap + 1);
+  size_t at = 0;
+  for (int k = 0; k < SL_N; ++k) {
+    if (off) off[k] = at;
+    at += a256(sz[k]);
+  }
+  return at;
+}
+
+int sw_trace_cap(int n1, int n2) { return 2 * (n1 + n2) + 8; }
+
+// ---------------------------------------------------------------- jobs
+
+// Triangular job over `count` instances: lens[q] and the per-instance data
+// (codes for RNA, rows|cols for the chain) at data_off[q] within `data`.
+int tri_prepare(std::string& err, Job& j, int g, const std::vector<int>& lens, const std::vector<int64_t>& data_off,
+                std::vector<uint8_t>&& data, int min_loop, bool want_trace) {
+  const int count = (int)lens.size();
+  j.kind = JOB_TRI;
+  j.g = g;
+  j.count = count;
+  j.min_loop = min_loop;
+  j.want_trace = want_trace;
+  j.data = std::move(data);
+  j.inst.assign(count, TriInst{});
+  j.tiles = j.trace_total = 0;
+  j.n_max = 0;
+  j.cells = 0;
   for (int q = 0; q < count; ++q) {
-    TriInst& t = inst[q];
+    TriInst& t = j.inst[q];
+    if (lens[q] < 0 || lens[q] > max_len(g)) return fail(err, DPX_EINVAL, "instance size out of range");
     t.n = lens[q];
     t.nb = (t.n + 1 + T - 1) / T;
     t.seq_off = data_off[q];
-    t.tile0 = tiles;
-    tiles += tiles_of(t.nb);
-    t.trace_off = trace_total;
-    t.trace_cap = want_trace ? 4 * t.n + 8 : 0;
-    trace_total += t.trace_cap;
+    t.tile0 = j.tiles;
+    j.tiles += tiles_of(t.nb);
+    t.trace_off = j.trace_total;
+    t.trace_cap = want_trace ? tri_trace_cap(t.n) : 0;
+    j.trace_total += t.trace_cap;
     t.out = q;
-    n_max = std::max(n_max, t.n);
+    j.n_max = std::max(j.n_max, t.n);
+    j.cells += tri_cells(t.n);
   }
-  if (tiles >= (int64_t)1 << 31) return fail(c, DPX_ENOMEM, "too many tiles in one launch");
-  std::vector<unsigned long long> seeds;
-  seeds.reserve(tiles);
+  if (j.tiles >= ((int64_t)1 << 31)) return fail(err, DPX_ENOMEM, "too many tiles in one launch");
+  // seed the longest instances first: their chain of tile diagonals is the longest
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return lens[x] > lens[y]; });
+  j.seeds.clear();
+  j.seeds.reserve(j.tiles);
   for (int q : order)
-    for (int I = 0; I < inst[q].nb; ++I) seeds.push_back(pack_item(q, I, I));
-  const size_t tab_bytes = (size_t)tiles * NTv * T * T * sizeof(int32_t);
-  if (!c->tab.ensure(tab_bytes) || !c->queue.ensure(tiles * 8) || !c->dep.ensure(tiles * 4) || !c->ctrl.ensure(64) ||
-      !c->inst.ensure(sizeof(TriInst) * count) || !c->seeds.ensure(seeds.size() * 8) || !c->codes.ensure(data_bytes + 64) ||
-      !c->results.ensure(sizeof(int32_t) * count) || !c->lens.ensure(sizeof(int32_t) * count) ||
-      (want_trace && !c->traces.ensure(sizeof(dpx_trace_t) * std::max<int64_t>(trace_total, 1))))
-    return fail(c, DPX_ENOMEM, "device allocation for the DP tables failed (" + std::to_string(tab_bytes >> 20) + " MiB)");
-  if (g == G_ZUKER) {
-    int r = ensure_hl(c, n_max);
-    if (r) return r;
-    if (n_max + 1 > 16000) return fail(c, DPX_EINVAL, "Zuker sequences are limited to 15999 bases");
-  }
-  hipStream_t s = c->stream;
-  HIPCHK(hipMemcpyAsync(c->inst.p, inst.data(), sizeof(TriInst) * count, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(c->seeds.p, seeds.data(), seeds.size() * 8, hipMemcpyHostToDevice, s));
-  if (data_bytes) HIPCHK(hipMemcpyAsync(c->codes.p, data, data_bytes, hipMemcpyHostToDevice, s));
+    for (int I = 0; I < j.inst[q].nb; ++I) j.seeds.push_back(pack_item(q, I, I));
+  j.dev_bytes = tri_layout(g, j.tiles, count, j.data.size(), j.trace_total, j.off);
+  return DPX_OK;
+}
+
+int tri_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
+  const int g = j.g, count = j.count;
+  const size_t in_bytes = sizeof(TriInst) * count + j.seeds.size() * 8 + j.data.size() + 64;
+  const size_t out_bytes = 8 * (size_t)count + 64 + (j.want_trace ? sizeof(dpx_trace_t) * (size_t)j.trace_total : 0);
+  if (!w.h_in.ensure(in_bytes) || !w.h_out.ensure(out_bytes)) return fail(err, DPX_ENOMEM, "pinned host staging");
+  // stage the inputs in pinned memory, then copy them asynchronously
+  char* p_inst = (char*)w.h_in.p;
+  std::memcpy(p_inst, j.inst.data(), sizeof(TriInst) * count);
+  char* p_seeds = p_inst + sizeof(TriInst) * count;
+  std::memcpy(p_seeds, j.seeds.data(), j.seeds.size() * 8);
+  char* p_data = p_seeds + j.seeds.size() * 8;
+  if (!j.data.empty()) std::memcpy(p_data, j.data.data(), j.data.size());
+  hipStream_t s = w.stream;
+  HIPCHK(hipMemcpyAsync(dev + j.off[TL_INST], p_inst, sizeof(TriInst) * count, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dev + j.off[TL_SEEDS], p_seeds, j.seeds.size() * 8, hipMemcpyHostToDevice, s));
+  if (!j.data.empty()) HIPCHK(hipMemcpyAsync(dev + j.off[TL_DATA], p_data, j.data.size(), hipMemcpyHostToDevice, s));
   TriArgs a{};
-  a.inst = (const TriInst*)c->inst.p;
+  a.inst = (const TriInst*)(dev + j.off[TL_INST]);
   a.n_inst = count;
-  a.total_tiles = (int32_t)tiles;
-  a.queue = (unsigned long long*)c->queue.p;
-  a.ctrl = (int32_t*)c->ctrl.p;
-  a.dep = (int32_t*)c->dep.p;
-  a.tab = (int32_t*)c->tab.p;
-  a.codes = (const uint8_t*)c->codes.p;
-  a.dims = (const int32_t*)c->codes.p;
+  a.total_tiles = (int32_t)j.tiles;
+  a.queue = (unsigned long long*)(dev + j.off[TL_QUEUE]);
+  a.ctrl = (int32_t*)(dev + j.off[TL_CTRL]);
+  a.dep = (int32_t*)(dev + j.off[TL_DEP]);
+  a.tab = (int32_t*)(dev + j.off[TL_TAB]);
+  a.codes = (const uint8_t*)(dev + j.off[TL_DATA]);
+  a.dims = (const int32_t*)(dev + j.off[TL_DATA]);
   a.rna = c->d_rna.p;
   a.hl_ent = (const int32_t*)c->d_hl.p;
-  a.min_loop = min_loop;
-  a.results = (int32_t*)c->results.p;
-  a.traces = c->traces.p;
-  a.trace_lens = (int32_t*)c->lens.p;
-  a.want_trace = want_trace ? 1 : 0;
-  if (!c->occ[g]) c->occ[g] = fill_occupancy(g);
-  const int per_cu = std::max(1, c->occ[g]);
-  const int grid = (int)std::min<int64_t>(tiles, (int64_t)per_cu * c->cu_count);
-  HIPCHK(hipEventRecord(c->ev[0], s));
-  if (launch_fill(g, a, (const unsigned long long*)c->seeds.p, (int)seeds.size(), grid, s)) return hip_fail(c, hipGetLastError(), "fill launch");
-  HIPCHK(hipEventRecord(c->ev[1], s));
-  if (launch_finalize(g, a, count, n_max, s)) return hip_fail(c, hipGetLastError(), "finalize launch");
-  HIPCHK(hipEventRecord(c->ev[2], s));
-  std::vector<int32_t> res(count), tl(count);
-  int32_t ctrl[4] = {0, 0, 0, 0};
-  HIPCHK(hipMemcpyAsync(ctrl, c->ctrl.p, sizeof ctrl, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(res.data(), c->results.p, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s));
-  if (want_trace) HIPCHK(hipMemcpyAsync(tl.data(), c->lens.p, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  (void)hipEventElapsedTime(&c->t_fill, c->ev[0], c->ev[1]);
-  (void)hipEventElapsedTime(&c->t_fin, c->ev[1], c->ev[2]);
-  if (ctrl[2]) return fail(c, DPX_EHIP, "tile queue wait timed out (scheduler error)");
-  int rc = DPX_OK;
-  for (int q = 0; q < count; ++q) results[q] = res[q];
-  if (want_trace) {
-    for (int q = 0; q < count; ++q) {
-      const int64_t cap = trace_offs[q + 1] - trace_offs[q];
-      if (tl[q] < 0) { trace_lens[q] = -1; rc = fail(c, DPX_ETRACE, "backtrace failed"); continue; }
-      trace_lens[q] = tl[q];
-      if (tl[q] > cap) { rc = fail(c, DPX_ETRACE, "trace buffer too small"); continue; }
-      HIPCHK(hipMemcpyAsync(traces + trace_offs[q], (const dpx_trace_t*)c->traces.p + inst[q].trace_off,
-                            sizeof(dpx_trace_t) * tl[q], hipMemcpyDeviceToHost, s));
+  a.min_loop = j.min_loop;
+  a.results = (int32_t*)(dev + j.off[TL_RES]);
+  a.traces = dev + j.off[TL_TRACES];
+  a.trace_lens = (int32_t*)(dev + j.off[TL_LENS]);
+  a.want_trace = j.want_trace ? 1 : 0;
+  const int grid = (int)std::min<int64_t>(j.tiles, (int64_t)c->occ[g] * c->cu_count);
+  HIPCHK(hipEventRecord(w.ev[0], s));
+  if (launch_fill(g, a, (const unsigned long long*)(dev + j.off[TL_SEEDS]), (int)j.seeds.size(), grid, s))
+    return fail(err, DPX_EHIP, "fill launch failed");
+  HIPCHK(hipEventRecord(w.ev[1], s));
+  if (launch_finalize(g, a, count, j.n_max, s)) return fail(err, DPX_EHIP, "finalize launch failed");
+  HIPCHK(hipEventRecord(w.ev[2], s));
+  char* hout = (char*)w.h_out.p;
+  int32_t* o_res = (int32_t*)hout;
+  int32_t* o_lens = o_res + count;
+  int32_t* o_flags = o_lens + count;
+  dpx_trace_t* o_tr = (dpx_trace_t*)(((uintptr_t)(o_flags + 4) + 15) & ~(uintptr_t)15);
+  HIPCHK(hipMemcpyAsync(o_res, a.results, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(o_flags, a.ctrl, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
+  if (j.want_trace) {
+    HIPCHK(hipMemcpyAsync(o_lens, a.trace_lens, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(o_tr, a.traces, sizeof(dpx_trace_t) * j.trace_total, hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipEventRecord(w.ev[3], s));
+  j.res = o_res;
+  j.lens = o_lens;
+  j.flags = o_flags;
+  j.tr = o_tr;
+  return DPX_OK;
+}
+
+int sw_prepare(std::string& err, Job& j, const char* a, int n1, const char* b, int n2, const dpx_swat_scores& sc,
+               int trace_from, int trace_cap) {
+  if ((int64_t)(n1 + 1) * (n2 + 1) >= ((int64_t)1 << 32)) return fail(err, DPX_EINVAL, "matrix too large");
+  j.kind = JOB_SW;
+  j.a.assign(a, n1);
+  j.b.assign(b, n2);
+  j.sc = sc;
+  j.trace_from = trace_from;
+  j.sw_cap = trace_cap;
+  const int W = n2 + 1;
+  // row 0 (i == 0): g1 = 0 [4]; g2 [7/8/9]; M = g1 [2], g2 when strictly greater [3]
+  j.row0.resize(W);
+  j.r0m.resize(W);
+  j.r0g2.resize(W);
+  int Mprev = 0, G2prev = 0;
+  j.row0_best = -1;
+  j.row0_bestj = 0;
+  for (int x = 0; x < W; ++x) {
+    int g2, rg2;
+    if (x == 0) {
+      g2 = 0;
+      rg2 = 7;
+    } else {
+      const int o = std::max(0, Mprev + sc.gap_open), e = std::max(0, G2prev + sc.gap_ext);
+      rg2 = e > o ? 9 : 8;
+      g2 = e > o ? e : o;
     }
-    HIPCHK(hipStreamSynchronize(s));
+    int M = 0, rm = x == 0 ? 0 : 2;
+    if (g2 > M) { M = g2; rm = 3; }
+    j.row0[x] = make_int2(M, 0);
+    j.r0m[x] = (uint8_t)rm;
+    j.r0g2[x] = (uint8_t)rg2;
+    if (M > j.row0_best) { j.row0_best = M; j.row0_bestj = x; }
+    Mprev = M;
+    G2prev = g2;
+  }
+  j.row0_axiom = j.row0[n2].x;
+  j.cells = (int64_t)(n1 + 1) * (n2 + 1);
+  j.dev_bytes = sw_layout(n1, n2, trace_cap, j.off);
+  return DPX_OK;
+}
+
+int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
+  const int n1 = (int)j.a.size(), n2 = (int)j.b.size(), W = n2 + 1;
+  SwArgs A{};
+  A.n1 = n1;
+  A.n2 = n2;
+  A.match = j.sc.match;
+  A.mismatch = j.sc.mismatch;
+  A.open = j.sc.gap_open;
+  A.ext = j.sc.gap_ext;
+  A.nbands = (n1 + 63) / 64;
+  A.nwords = (n2 + 64 + 7) / 8;
+  const bool want_trace = j.sw_cap > 0;
+  const size_t in_bytes = (size_t)n1 + n2 + sizeof(int2) * W + 2 * (size_t)W + 64;
+  const size_t out_bytes = 64 + (want_trace ? sizeof(dpx_trace_t) * (size_t)j.sw_cap : 0);
+  if (!w.h_in.ensure(in_bytes) || !w.h_out.ensure(out_bytes)) return fail(err, DPX_ENOMEM, "pinned host staging");
+  hipStream_t s = w.stream;
+  char* rows = dev + j.off[SL_ROWS];
+  A.hrow = (int2*)rows;
+  int2* d_row0 = (int2*)(rows + (size_t)(A.nbands + 1) * W * 8);
+  uint8_t* d_r0m = (uint8_t*)(d_row0 + W);
+  uint8_t* d_r0g2 = d_r0m + W;
+  A.a = (const uint8_t*)(dev + j.off[SL_A]);
+  A.b = (const uint8_t*)(dev + j.off[SL_B]);
+  A.prog = (int*)(dev + j.off[SL_FLAGS]);
+  A.ctrl = A.prog + A.nbands + 1;
+  A.bt = (uint32_t*)(dev + j.off[SL_BT]);
+  char* bestblk = dev + j.off[SL_BEST];
+  A.best = (unsigned long long*)bestblk;
+  A.out = (int*)(bestblk + 8);
+  int* d_pick = (int*)(bestblk + 16);  // 6 ints, see sw_pick
+  int* d_len = (int*)(bestblk + 48);
+  char* h = (char*)w.h_in.p;
+  std::memcpy(h, j.a.data(), n1);
+  std::memcpy(h + n1, j.b.data(), n2);
+  char* h_row0 = h + ((n1 + n2 + 15) & ~15);
+  std::memcpy(h_row0, j.row0.data(), sizeof(int2) * W);
+  std::memcpy(h_row0 + sizeof(int2) * W, j.r0m.data(), W);
+  std::memcpy(h_row0 + sizeof(int2) * W + W, j.r0g2.data(), W);
+  if (n1) HIPCHK(hipMemcpyAsync((void*)A.a, h, n1, hipMemcpyHostToDevice, s));
+  if (n2) HIPCHK(hipMemcpyAsync((void*)A.b, h + n1, n2, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_row0, h_row0, sizeof(int2) * W + 2 * (size_t)W, hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(w.ev[0], s));
+  if (A.nbands > 0) {
+    const int grid = std::min(A.nbands, c->sw_occ * c->cu_count);
+    if (sw_launch(A, d_row0, grid, s)) return fail(err, DPX_EHIP, "alignment launch failed");
+  }
+  HIPCHK(hipEventRecord(w.ev[1], s));
+  void* d_tr = want_trace ? dev + j.off[SL_TR] : nullptr;
+  if (sw_finish_launch(A, j.row0_best, j.row0_bestj, j.row0_axiom, j.trace_from, d_pick, d_r0m, d_r0g2, d_tr, j.sw_cap,
+                       d_len, s))
+    return fail(err, DPX_EHIP, "traceback launch failed");
+  HIPCHK(hipEventRecord(w.ev[2], s));
+  char* ho = (char*)w.h_out.p;
+  // [0..5] sw_pick output, [6] trace length, [7] band-wait time-out flag, trace records at +64
+  HIPCHK(hipMemcpyAsync(ho, d_pick, 6 * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (want_trace) HIPCHK(hipMemcpyAsync(ho + 24, d_len, sizeof(int), hipMemcpyDeviceToHost, s));
+  else std::memset(ho + 24, 0, 4);
+  if (A.nbands > 0) HIPCHK(hipMemcpyAsync(ho + 28, A.ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  else std::memset(ho + 28, 0, 4);
+  if (want_trace) HIPCHK(hipMemcpyAsync(ho + 64, d_tr, sizeof(dpx_trace_t) * j.sw_cap, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(w.ev[3], s));
+  j.res = (const int32_t*)ho;
+  j.tr = (const dpx_trace_t*)(ho + 64);
+  return DPX_OK;
+}
+
+int job_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
+  return j.kind == JOB_TRI ? tri_launch(c, err, w, dev, j) : sw_launch_job(c, err, w, dev, j);
+}
+
+// after ev[3] has completed: timings and the kernels' time-out flags
+int job_harvest(std::string& err, Workspace& w, Job& j) {
+  (void)hipEventElapsedTime(&j.t_fill, w.ev[0], w.ev[1]);
+  (void)hipEventElapsedTime(&j.t_fin, w.ev[1], w.ev[2]);
+  if (j.kind == JOB_TRI && j.flags[2]) return fail(err, DPX_EHIP, "tile queue wait timed out (scheduler error)");
+  if (j.kind == JOB_SW && j.res[7]) return fail(err, DPX_EHIP, "band wait timed out (scheduler error)");
+  return DPX_OK;
+}
+
+// synchronous run on the context's workspace and device block
+int run_sync(dpx_ctx* c, Job& j) {
+  std::string& err = c->err;
+  if (!c->block.ensure(j.dev_bytes))
+    return fail(err, DPX_ENOMEM, "device allocation for the DP tables failed (" + std::to_string(j.dev_bytes >> 20) + " MiB)");
+  int r = job_launch(c, err, c->ws, (char*)c->block.p, j);
+  if (r) return r;
+  HIPCHK(hipEventSynchronize(c->ws.ev[3]));
+  r = job_harvest(err, c->ws, j);
+  c->t_fill = j.t_fill;
+  c->t_fin = j.t_fin;
+  return r;
+}
+
+// results and traces of a finished triangular job into the caller's buffers
+int tri_copy_out(std::string& err, const Job& j, int32_t* results, dpx_trace_t* traces, const int64_t* trace_offs,
+                 int32_t* trace_lens) {
+  int rc = DPX_OK;
+  for (int q = 0; q < j.count; ++q) results[q] = j.res[q];
+  if (!j.want_trace || !traces) return rc;
+  for (int q = 0; q < j.count; ++q) {
+    const int64_t cap = trace_offs[q + 1] - trace_offs[q];
+    const int tl = j.lens[q];
+    trace_lens[q] = tl;
+    if (tl < 0) { rc = fail(err, DPX_ETRACE, "backtrace failed"); continue; }
+    if (tl > cap) { rc = fail(err, DPX_ETRACE, "trace buffer too small"); continue; }
+    std::memcpy(traces + trace_offs[q], j.tr + j.inst[q].trace_off, sizeof(dpx_trace_t) * tl);
   }
   return rc;
 }
 
-int rna_batch(dpx_ctx* c, int g, const char* seqs, const int64_t* offs, int count, int min_loop, int32_t* results,
-              dpx_trace_t* traces, const int64_t* trace_offs, int32_t* trace_lens) {
-  if (count < 0 || (count > 0 && (!seqs || !offs || !results))) return fail(c, DPX_EINVAL, "null argument");
-  if (traces && (!trace_offs || !trace_lens)) return fail(c, DPX_EINVAL, "trace offsets missing");
-  if (g == G_ZUKER && !c->has_rna) return fail(c, DPX_ENOPARAMS, "dpx_set_rna_params was not called");
+int rna_job(std::string& err, Job& j, int g, const char* seqs, const int64_t* offs, int count, int min_loop, bool trace) {
   std::vector<int> lens(count);
   std::vector<int64_t> off(count);
   const int64_t total = count ? offs[count] - offs[0] : 0;
-  std::vector<uint8_t> codes(total + count + 64);
+  std::vector<uint8_t> codes((size_t)std::max<int64_t>(total, 0) + count + 64);
   int64_t pos = 0;
   for (int q = 0; q < count; ++q) {
     const int64_t n = offs[q + 1] - offs[q];
-    if (n < 0 || n > 30000) return fail(c, DPX_EINVAL, "sequence length out of range");
+    if (n < 0 || n > max_len(g)) return fail(err, DPX_EINVAL, "sequence length out of range");
     lens[q] = (int)n;
     off[q] = pos;
-    int r = encode_rna(c, seqs + offs[q], (int)n, codes.data() + pos);
+    int r = encode_rna(err, seqs + offs[q], (int)n, codes.data() + pos);
     if (r) return r;
-    pos += n + 1;  // one spare slot per instance (st scratch shares the offsets)
+    pos += n + 1;
   }
-  return run_tri(c, g, count, lens, off, codes.data(), (size_t)pos, min_loop, results, traces, trace_offs, trace_lens);
+  codes.resize(pos);
+  return tri_prepare(err, j, g, lens, off, std::move(codes), min_loop, trace);
+}
+
+int rna_batch(dpx_ctx* c, int g, const char* seqs, const int64_t* offs, int count, int min_loop, int32_t* results,
+              dpx_trace_t* traces, const int64_t* trace_offs, int32_t* trace_lens) {
+  std::string& err = c->err;
+  if (count < 0 || (count > 0 && (!seqs || !offs || !results))) return fail(err, DPX_EINVAL, "null argument");
+  if (traces && (!trace_offs || !trace_lens)) return fail(err, DPX_EINVAL, "trace offsets missing");
+  if (g == G_ZUKER && !c->has_rna) return fail(err, DPX_ENOPARAMS, "dpx_set_rna_params was not called");
+  if (count == 0) return DPX_OK;
+  Job j;
+  int r = rna_job(err, j, g, seqs, offs, count, min_loop, traces != nullptr);
+  if (!r) r = run_sync(c, j);
+  if (r) return r;
+  return tri_copy_out(err, j, results, traces, trace_offs, trace_lens);
+}
+
+std::vector<uint8_t> chain_data(const int32_t* rows, const int32_t* cols, int n) {
+  std::vector<uint8_t> d(2 * (size_t)n * sizeof(int32_t));
+  std::memcpy(d.data(), rows, sizeof(int32_t) * n);
+  std::memcpy(d.data() + sizeof(int32_t) * n, cols, sizeof(int32_t) * n);
+  return d;
+}
+
+}  // namespace
+
+// ================================================================== server
+
+namespace {
+
+struct Request {
+  uint64_t id = 0;
+  int kind = 0;
+  int trace = 0, trace_from = 0, min_loop = 0;
+  std::string s1, s2;
+  std::vector<int32_t> rows, cols;
+  dpx_swat_scores sc{};
+  double t_submit = 0;
+  size_t footprint = 0;
+  int64_t cells = 0;
+  int size_class = 0;
+};
+
+struct Finished {
+  dpx_result r{};
+  std::vector<dpx_trace_t> trace;
+};
+
+struct Slot {
+  Workspace ws;
+  Job job;
+  std::vector<Request> reqs;
+  bool busy = false;
+  size_t block_off = 0;
+  double t_launch = 0;
+  std::string err;
+};
+
+// First-fit allocator of 256-byte aligned ranges of the arena.
+struct Arena {
+  DevBuf mem;
+  std::map<size_t, size_t> free_;  // offset -> size
+  size_t largest() const {
+    size_t m = 0;
+    for (auto& f : free_) m = std::max(m, f.second);
+    return m;
+  }
+  bool alloc(size_t bytes, size_t* off) {
+    bytes = a256(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      *off = it->first;
+      const size_t rest = it->second - bytes, at = it->first + bytes;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = a256(bytes);
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+}  // namespace
+
+struct dpx_server {
+  dpx_ctx* ctx = nullptr;
+  dpx_server_config cfg{};
+  Arena arena;
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request> pending;
+  std::deque<uint64_t> ready;
+  std::unordered_map<uint64_t, Finished> done;
+  std::unordered_map<uint64_t, std::vector<dpx_trace_t>> traces;
+  std::thread th;
+  std::atomic<bool> stop{false};
+  uint64_t next_id = 1;
+  dpx_server_stats st{};
+};
+
+namespace {
+
+int size_class(int n) {
+  int c = 0;
+  while ((32 << c) < n) ++c;
+  return c;
+}
+
+void publish(dpx_server* S, Finished&& f) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (f.r.status == DPX_OK) {
+    S->st.completed++;
+    S->st.cells_done += f.r.cells;
+  } else {
+    S->st.failed++;
+  }
+  const uint64_t id = f.r.id;
+  S->done[id] = std::move(f);
+  S->ready.push_back(id);
+  S->cv_done.notify_all();
+}
+
+Finished base_result(const Request& q, const Slot& sl, int status) {
+  Finished f;
+  f.r.id = q.id;
+  f.r.kind = q.kind;
+  f.r.status = status;
+  f.r.t_submit = q.t_submit;
+  f.r.t_launch = sl.t_launch;
+  f.r.batch_size = (int32_t)sl.reqs.size();
+  f.r.cells = q.cells;
+  f.r.footprint = (int64_t)q.footprint;
+  return f;
+}
+
+// build the job of the requests in `sl` (server thread)
+int prepare_slot(dpx_server* S, Slot& sl) {
+  Job& j = sl.job;
+  j = Job();
+  const Request& r0 = sl.reqs.front();
+  if (r0.kind == DPX_REQ_SWAT)
+    return sw_prepare(sl.err, j, r0.s1.data(), (int)r0.s1.size(), r0.s2.data(), (int)r0.s2.size(), r0.sc, r0.trace_from,
+                      r0.trace ? sw_trace_cap((int)r0.s1.size(), (int)r0.s2.size()) : 0);
+  const int count = (int)sl.reqs.size();
+  std::vector<int> lens(count);
+  std::vector<int64_t> off(count);
+  std::vector<uint8_t> data;
+  if (r0.kind == DPX_REQ_MATCHAIN) {
+    for (int q = 0; q < count; ++q) {
+      const Request& r = sl.reqs[q];
+      lens[q] = (int)r.rows.size();
+      off[q] = (int64_t)(data.size() / 4);  // matrix chain offsets count int32 dims
+      std::vector<uint8_t> d = chain_data(r.rows.data(), r.cols.data(), lens[q]);
+      data.insert(data.end(), d.begin(), d.end());
+    }
+    return tri_prepare(sl.err, j, G_MATCHAIN, lens, off, std::move(data), 0, r0.trace != 0);
+  }
+  for (int q = 0; q < count; ++q) {
+    const Request& r = sl.reqs[q];
+    lens[q] = (int)r.s1.size();
+    off[q] = (int64_t)data.size();
+    const size_t at = data.size();
+    data.resize(at + r.s1.size() + 1);
+    int rc = encode_rna(sl.err, r.s1.data(), lens[q], data.data() + at);
+    if (rc) return rc;
+  }
+  const int g = r0.kind == DPX_REQ_NUSSINOV ? G_NUSSINOV : G_ZUKER;
+  return tri_prepare(sl.err, j, g, lens, off, std::move(data), r0.min_loop, r0.trace != 0);
+}
+
+void harvest_slot(dpx_server* S, Slot& sl, int status) {
+  Job& j = sl.job;
+  const int rc = status ? status : job_harvest(sl.err, sl.ws, j);
+  const int batch = (int)sl.reqs.size();
+  for (int q = 0; q < batch; ++q) {
+    const Request& rq = sl.reqs[q];
+    Finished f = base_result(rq, sl, rc);
+    f.r.device_ms = j.t_fill + j.t_fin;
+    if (!rc) {
+      int tl = 0;
+      const dpx_trace_t* src = nullptr;
+      if (j.kind == JOB_SW) {
+        f.r.score = j.res[0];
+        f.r.best = j.res[1];
+        f.r.best_i = j.res[2];
+        f.r.best_j = j.res[3];
+        tl = j.sw_cap > 0 ? j.res[6] : 0;
+        src = j.tr;
+      } else {
+        f.r.score = j.res[q];
+        tl = j.want_trace ? j.lens[q] : 0;
+        src = j.tr + j.inst[q].trace_off;
+      }
+      f.r.trace_len = tl;
+      if (tl < 0) f.r.status = DPX_ETRACE;
+      else if (tl > 0) f.trace.assign(src, src + tl);
+    }
+    f.r.t_done = now_s();
+    publish(S, std::move(f));
+  }
+  S->arena.release(sl.block_off, j.dev_bytes);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->st.bytes_in_flight -= (int64_t)a256(j.dev_bytes);
+  }
+  sl.busy = false;
+  sl.reqs.clear();
+}
+
+// Take the next batch for a free slot: the oldest waiting request decides
+// kind, options and size class; later requests of the same kind, options and
+// a neighbouring size class join in arrival order while the batch fits the
+// largest free range of the arena.  Returns false when nothing can start.
+bool take_batch(dpx_server* S, Slot& sl) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->pending.empty()) return false;
+  const size_t room = S->arena.largest();
+  const Request& h = S->pending.front();
+  if (h.footprint > room) return false;  // wait until launches in flight release memory
+  const int cap = h.kind == DPX_REQ_SWAT ? 1 : S->cfg.max_batch;
+  size_t used = 0;
+  const int kind = h.kind, trace = h.trace, ml = h.min_loop, cls = h.size_class;
+  for (auto it = S->pending.begin(); it != S->pending.end() && (int)sl.reqs.size() < cap;) {
+    const bool same = it->kind == kind && it->trace == trace && it->min_loop == ml && std::abs(it->size_class - cls) <= 1;
+    if (same && used + it->footprint <= room) {
+      used += it->footprint;
+      sl.reqs.push_back(std::move(*it));
+      it = S->pending.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  S->st.pending = (int64_t)S->pending.size();
+  return true;
+}
+
+void server_loop(dpx_server* S) {
+  (void)hipSetDevice(S->ctx->device);
+  while (!S->stop.load()) {
+    bool progress = false;
+    for (Slot& sl : S->slots) {
+      if (!sl.busy) continue;
+      const hipError_t q = hipEventQuery(sl.ws.ev[3]);
+      if (q == hipErrorNotReady) continue;
+      harvest_slot(S, sl, q == hipSuccess ? DPX_OK : hip_fail(sl.err, q, "launch in flight"));
+      progress = true;
+    }
+    for (Slot& sl : S->slots) {
+      if (sl.busy) continue;
+      if (!take_batch(S, sl)) break;
+      progress = true;
+      sl.t_launch = now_s();
+      int rc = prepare_slot(S, sl);
+      if (!rc && !S->arena.alloc(sl.job.dev_bytes, &sl.block_off)) rc = fail(sl.err, DPX_ENOMEM, "arena exhausted");
+      if (!rc) {
+        rc = job_launch(S->ctx, sl.err, sl.ws, (char*)S->arena.mem.p + sl.block_off, sl.job);
+        if (rc) {
+          // work queued before the failure may still run: let the stream drain first
+          (void)hipStreamSynchronize(sl.ws.stream);
+          S->arena.release(sl.block_off, sl.job.dev_bytes);
+        }
+      }
+      if (!rc) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->st.launches++;
+        S->st.bytes_in_flight += (int64_t)a256(sl.job.dev_bytes);
+        S->st.peak_bytes_in_flight = std::max(S->st.peak_bytes_in_flight, S->st.bytes_in_flight);
+        S->st.max_batch_seen = std::max<int64_t>(S->st.max_batch_seen, (int64_t)sl.reqs.size());
+        sl.busy = true;
+      } else {
+        for (const Request& q : sl.reqs) {
+          Finished f = base_result(q, sl, rc);
+          f.r.t_done = now_s();
+          publish(S, std::move(f));
+        }
+        sl.reqs.clear();
+      }
+    }
+    if (!progress) {
+      bool any_busy = false;
+      for (Slot& sl : S->slots) any_busy |= sl.busy;
+      std::unique_lock<std::mutex> lk(S->mu);
+      // with launches in flight, poll their events every 50 us; otherwise sleep until a submission
+      S->cv_work.wait_for(lk, std::chrono::microseconds(any_busy ? 50 : 20000));
+    }
+  }
+  for (Slot& sl : S->slots)
+    if (sl.busy) {
+      const hipError_t q = hipEventSynchronize(sl.ws.ev[3]);
+      harvest_slot(S, sl, q == hipSuccess ? DPX_OK : hip_fail(sl.err, q, "launch in flight"));
+    }
 }
 
 }  // namespace
@@ -231,12 +821,14 @@ int dpx_create(dpx_ctx** out, int device) {
   if (device < 0 || device >= ndev) return DPX_EINVAL;
   dpx_ctx* c = new dpx_ctx();
   c->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || !c->ws.init()) {
+    c->ws.release();
     delete c;
     return DPX_EHIP;
   }
   (void)hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, device);
-  for (auto& e : c->ev) (void)hipEventCreate(&e);
+  for (int g = 0; g < 3; ++g) c->occ[g] = std::max(1, fill_occupancy(g));
+  c->sw_occ = std::max(1, sw_occupancy());
   *out = c;
   return DPX_OK;
 }
@@ -244,13 +836,10 @@ int dpx_create(dpx_ctx** out, int device) {
 void dpx_destroy(dpx_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->d_rna, &c->d_hl, &c->tab, &c->queue, &c->dep, &c->ctrl, &c->inst, &c->seeds, &c->codes,
-                    &c->dims, &c->results, &c->traces, &c->lens, &c->sw_a, &c->sw_b, &c->sw_bt, &c->sw_row,
-                    &c->sw_flag, &c->sw_best, &c->sw_out, &c->sw_tr};
-  for (DevBuf* b : bufs) b->release();
-  for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  c->ws.release();
+  c->block.release();
+  c->d_rna.release();
+  c->d_hl.release();
   delete c->rna;
   delete c;
 }
@@ -260,20 +849,26 @@ const char* dpx_last_error(const dpx_ctx* c) { return c ? c->err.c_str() : "null
 int dpx_set_rna_params(dpx_ctx* c, const char* path) {
   if (!c || !path) return DPX_EINVAL;
   std::lock_guard<std::mutex> lk(c->mu);
+  std::string& err = c->err;
+  if (c->servers.load() > 0) return fail(err, DPX_EBUSY, "energy parameters cannot change while a server runs");
   (void)hipSetDevice(c->device);
   HostRna* h = new HostRna();
   char msg[256] = {0};
   if (load_par_file(path, h, msg, sizeof msg)) {
     delete h;
-    return fail(c, DPX_EPARAMS, msg);
+    return fail(err, DPX_EPARAMS, msg);
   }
   delete c->rna;
   c->rna = h;
-  if (!c->d_rna.ensure(sizeof(DevRna))) return fail(c, DPX_ENOMEM, "energy tables");
+  c->has_rna = false;
+  std::vector<int32_t> hl(kHairpinTable);
+  for (int l = 0; l < kHairpinTable; ++l) hl[l] = l < 3 ? 0 : h->hl_ent(l);
+  if (!c->d_rna.ensure(sizeof(DevRna)) || !c->d_hl.ensure(sizeof(int32_t) * kHairpinTable))
+    return fail(err, DPX_ENOMEM, "energy tables");
   HIPCHK(hipMemcpy(c->d_rna.p, &h->dev, sizeof(DevRna), hipMemcpyHostToDevice));
-  c->hl_n = -1;
+  HIPCHK(hipMemcpy(c->d_hl.p, hl.data(), sizeof(int32_t) * kHairpinTable, hipMemcpyHostToDevice));
   c->has_rna = true;
-  return ensure_hl(c, 4096);
+  return DPX_OK;
 }
 
 int dpx_zuker_batch(dpx_ctx* c, const char* seqs, const int64_t* offs, int count, int32_t* results, dpx_trace_t* traces,
@@ -287,8 +882,8 @@ int dpx_zuker_batch(dpx_ctx* c, const char* seqs, const int64_t* offs, int count
 int dpx_nussinov_batch(dpx_ctx* c, const char* seqs, const int64_t* offs, int count, int min_loop, int32_t* results,
                        dpx_trace_t* traces, const int64_t* trace_offs, int32_t* trace_lens) {
   if (!c) return DPX_EINVAL;
-  if (min_loop < 0) return fail(c, DPX_EINVAL, "min_loop < 0");
   std::lock_guard<std::mutex> lk(c->mu);
+  if (min_loop < 0) return fail(c->err, DPX_EINVAL, "min_loop < 0");
   (void)hipSetDevice(c->device);
   return rna_batch(c, G_NUSSINOV, seqs, offs, count, min_loop, results, traces, trace_offs, trace_lens);
 }
@@ -296,11 +891,13 @@ int dpx_nussinov_batch(dpx_ctx* c, const char* seqs, const int64_t* offs, int co
 static int single_rna(dpx_ctx* c, int g, const char* seq, int n, int min_loop, int32_t* score, dpx_trace_t* trace,
                       int trace_cap, int* trace_len) {
   if (!c) return DPX_EINVAL;
-  if (!seq || n < 0 || !score || (trace && (!trace_len || trace_cap < 0))) return fail(c, DPX_EINVAL, "bad argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!seq || n < 0 || !score || (trace && (!trace_len || trace_cap < 0)))
+    return fail(c->err, DPX_EINVAL, "bad argument");
+  if (min_loop < 0) return fail(c->err, DPX_EINVAL, "min_loop < 0");
   int64_t offs[2] = {0, n};
   int64_t toffs[2] = {0, trace_cap};
   int32_t tl = 0, res = 0;
-  std::lock_guard<std::mutex> lk(c->mu);
   (void)hipSetDevice(c->device);
   int r = rna_batch(c, g, seq, offs, 1, min_loop, &res, trace, toffs, &tl);
   *score = res;
@@ -310,7 +907,6 @@ static int single_rna(dpx_ctx* c, int g, const char* seq, int n, int min_loop, i
 
 int dpx_nussinov(dpx_ctx* c, const char* seq, int n, int min_loop, int32_t* score, dpx_trace_t* trace, int trace_cap,
                  int* trace_len) {
-  if (c && min_loop < 0) return fail(c, DPX_EINVAL, "min_loop < 0");
   return single_rna(c, G_NUSSINOV, seq, n, min_loop, score, trace, trace_cap, trace_len);
 }
 
@@ -321,18 +917,20 @@ int dpx_zuker(dpx_ctx* c, const char* seq, int n, int32_t* mfe, dpx_trace_t* tra
 int dpx_matchain(dpx_ctx* c, const int32_t* rows, const int32_t* cols, int n, int32_t* cost, dpx_trace_t* trace,
                  int trace_cap, int* trace_len) {
   if (!c) return DPX_EINVAL;
-  if (!rows || !cols || n < 1 || !cost || (trace && (!trace_len || trace_cap < 0))) return fail(c, DPX_EINVAL, "bad argument");
   std::lock_guard<std::mutex> lk(c->mu);
+  std::string& err = c->err;
+  if (!rows || !cols || n < 1 || n > kMaxTri || !cost || (trace && (!trace_len || trace_cap < 0)))
+    return fail(err, DPX_EINVAL, "bad argument");
   (void)hipSetDevice(c->device);
-  std::vector<int32_t> dims(2 * (size_t)n);
-  std::copy(rows, rows + n, dims.begin());
-  std::copy(cols, cols + n, dims.begin() + n);
+  Job j;
   std::vector<int> lens{n};
   std::vector<int64_t> off{0};
+  int r = tri_prepare(err, j, G_MATCHAIN, lens, off, chain_data(rows, cols, n), 0, trace != nullptr);
+  if (!r) r = run_sync(c, j);
+  if (r) return r;
   int64_t toffs[2] = {0, trace_cap};
-  int32_t tl = 0, res = 0;
-  int r = run_tri(c, G_MATCHAIN, 1, lens, off, dims.data(), dims.size() * sizeof(int32_t), 0, &res, trace, toffs, &tl);
-  *cost = res;
+  int32_t tl = 0;
+  r = tri_copy_out(err, j, cost, trace, toffs, &tl);
   if (trace) *trace_len = tl;
   return r;
 }
@@ -341,100 +939,26 @@ int dpx_swat(dpx_ctx* c, const char* a, int n1, const char* b, int n2, const dpx
              int32_t* best, int32_t* best_i, int32_t* best_j, int trace_from, dpx_trace_t* trace, int trace_cap,
              int* trace_len) {
   if (!c) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::string& err = c->err;
   if (!a || !b || n1 < 0 || n2 < 0 || !sc || !score || (trace && (!trace_len || trace_cap < 0)) ||
       (trace_from != 0 && trace_from != 1))
-    return fail(c, DPX_EINVAL, "bad argument");
-  if ((int64_t)(n1 + 1) * (n2 + 1) >= ((int64_t)1 << 32)) return fail(c, DPX_EINVAL, "matrix too large");
-  std::lock_guard<std::mutex> lk(c->mu);
+    return fail(err, DPX_EINVAL, "bad argument");
   (void)hipSetDevice(c->device);
-  const int W = n2 + 1;
-  // row 0 (i == 0): g1 = 0 [4]; g2 chain [7/8/9]; M = first of g1 [2], g2 if strictly greater [3]
-  std::vector<int2> row0(W);
-  std::vector<uint8_t> r0m(W), r0g2(W);
-  int Mprev = 0, G2prev = 0, best0 = -1, best0j = 0;
-  for (int j = 0; j < W; ++j) {
-    int g2, rg2;
-    if (j == 0) { g2 = 0; rg2 = 7; }
-    else {
-      const int o = std::max(0, Mprev + sc->gap_open), e = std::max(0, G2prev + sc->gap_ext);
-      rg2 = e > o ? 9 : 8; g2 = e > o ? e : o;
-    }
-    int M = 0, rm = j == 0 ? 0 : 2;
-    if (g2 > M) { M = g2; rm = 3; }
-    row0[j] = make_int2(M, 0);
-    r0m[j] = (uint8_t)rm; r0g2[j] = (uint8_t)rg2;
-    if (M > best0) { best0 = M; best0j = j; }
-    Mprev = M; G2prev = g2;
-  }
-  SwArgs A{};
-  A.n1 = n1; A.n2 = n2;
-  A.match = sc->match; A.mismatch = sc->mismatch; A.open = sc->gap_open; A.ext = sc->gap_ext;
-  A.nbands = (n1 + 63) / 64;
-  A.nwords = (n2 + 64 + 7) / 8;
-  const size_t bt_bytes = (size_t)A.nbands * A.nwords * 64 * 4;
-  if (!c->sw_a.ensure(n1 + 64) || !c->sw_b.ensure(n2 + 64) || !c->sw_row.ensure((size_t)(A.nbands + 2) * W * 8 + 2 * (size_t)W + 64) ||
-      !c->sw_flag.ensure(4 * (A.nbands + 2) + 64) || !c->sw_best.ensure(64) || !c->sw_bt.ensure(bt_bytes) ||
-      (trace && !c->sw_tr.ensure(sizeof(dpx_trace_t) * (size_t)(trace_cap + 1))))
-    return fail(c, DPX_ENOMEM, "device allocation for the alignment failed");
-  hipStream_t s = c->stream;
-  char* rowbase = (char*)c->sw_row.p;
-  A.hrow = (int2*)rowbase;
-  int2* d_row0 = (int2*)(rowbase + (size_t)(A.nbands + 1) * W * 8);  // scratch after the rows
-  uint8_t* d_r0m = (uint8_t*)(d_row0 + W);
-  uint8_t* d_r0g2 = d_r0m + W;
-  A.a = (const uint8_t*)c->sw_a.p;
-  A.b = (const uint8_t*)c->sw_b.p;
-  A.prog = (int*)c->sw_flag.p;
-  A.ctrl = A.prog + A.nbands + 1;
-  A.bt = (uint32_t*)c->sw_bt.p;
-  A.best = (unsigned long long*)c->sw_best.p;
-  A.out = (int*)((char*)c->sw_best.p + 8);
-  if (n1) HIPCHK(hipMemcpyAsync(c->sw_a.p, a, n1, hipMemcpyHostToDevice, s));
-  if (n2) HIPCHK(hipMemcpyAsync(c->sw_b.p, b, n2, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(d_row0, row0.data(), sizeof(int2) * W, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(d_r0m, r0m.data(), W, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(d_r0g2, r0g2.data(), W, hipMemcpyHostToDevice, s));
-  HIPCHK(hipEventRecord(c->ev[0], s));
-  if (A.nbands > 0) {
-    static int occ = 0;
-    if (!occ) occ = std::max(1, sw_occupancy());
-    const int grid = std::min(A.nbands, occ * c->cu_count);
-    if (sw_launch(A, d_row0, grid, s)) return hip_fail(c, hipGetLastError(), "sw launch");
-  }
-  HIPCHK(hipEventRecord(c->ev[1], s));
-  unsigned long long key = 0;
-  int mlast = 0, flags[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(&key, A.best, 8, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(&mlast, A.out, 4, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(flags, A.ctrl, 8, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  if (A.nbands > 0 && flags[1]) return fail(c, DPX_EHIP, "band wait timed out (scheduler error)");
-  *score = n1 == 0 ? row0[n2].x : mlast;
-  int bv = best0, bi = 0, bj = best0j;
-  if (A.nbands > 0) {
-    const int dv = (int)(key >> 32);
-    const uint32_t lin = 0xffffffffu - (uint32_t)(key & 0xffffffffu);
-    if (dv > bv) { bv = dv; bi = (int)(lin / (uint32_t)W); bj = (int)(lin % (uint32_t)W); }
-  }
-  if (best) *best = bv;
-  if (best_i) *best_i = bi;
-  if (best_j) *best_j = bj;
+  Job j;
+  int r = sw_prepare(err, j, a, n1, b, n2, *sc, trace_from, trace ? trace_cap : 0);
+  if (!r) r = run_sync(c, j);
+  if (r) return r;
+  *score = j.res[0];
+  if (best) *best = j.res[1];
+  if (best_i) *best_i = j.res[2];
+  if (best_j) *best_j = j.res[3];
   if (trace) {
-    const int i0 = trace_from ? bi : n1, j0 = trace_from ? bj : n2;
-    int* d_len = (int*)((char*)c->sw_best.p + 16);
-    if (sw_trace_launch(A, d_r0m, d_r0g2, i0, j0, c->sw_tr.p, trace_cap, d_len, s)) return hip_fail(c, hipGetLastError(), "sw trace");
-    HIPCHK(hipEventRecord(c->ev[2], s));
-    int tl = 0;
-    HIPCHK(hipMemcpyAsync(&tl, d_len, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (tl < 0) { *trace_len = -1; return fail(c, DPX_ETRACE, "trace buffer too small"); }
-    HIPCHK(hipMemcpy(trace, c->sw_tr.p, sizeof(dpx_trace_t) * tl, hipMemcpyDeviceToHost));
+    const int tl = j.res[6];
     *trace_len = tl;
-    (void)hipEventElapsedTime(&c->t_fin, c->ev[1], c->ev[2]);
-  } else {
-    c->t_fin = 0;
+    if (tl < 0) return fail(err, DPX_ETRACE, "trace buffer too small");
+    std::memcpy(trace, j.tr, sizeof(dpx_trace_t) * tl);
   }
-  (void)hipEventElapsedTime(&c->t_fill, c->ev[0], c->ev[1]);
   return DPX_OK;
 }
 
@@ -442,6 +966,151 @@ int dpx_last_timing(const dpx_ctx* c, float* fill_ms, float* finalize_ms) {
   if (!c) return DPX_EINVAL;
   if (fill_ms) *fill_ms = c->t_fill;
   if (finalize_ms) *finalize_ms = c->t_fin;
+  return DPX_OK;
+}
+
+double dpx_now(void) { return now_s(); }
+
+// ---------------------------------------------------------------- server
+
+int dpx_server_create(dpx_ctx* c, const dpx_server_config* cfg, dpx_server** out) {
+  if (!c) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::string& err = c->err;
+  if (!cfg || !out || cfg->slots < 1 || cfg->slots > 8 || cfg->max_batch < 1 || cfg->hbm_budget < (1 << 20) ||
+      cfg->max_pending < 1)
+    return fail(err, DPX_EINVAL, "bad server configuration");
+  *out = nullptr;
+  (void)hipSetDevice(c->device);
+  dpx_server* S = new dpx_server();
+  S->ctx = c;
+  S->cfg = *cfg;
+  const size_t budget = a256((size_t)cfg->hbm_budget);
+  if (!S->arena.mem.ensure(budget)) {
+    delete S;
+    return fail(err, DPX_ENOMEM, "cannot reserve the HBM budget");
+  }
+  S->arena.free_[0] = budget;
+  S->slots.resize(cfg->slots);
+  for (Slot& sl : S->slots)
+    if (!sl.ws.init()) {
+      for (Slot& x : S->slots) x.ws.release();
+      S->arena.mem.release();
+      delete S;
+      return fail(err, DPX_EHIP, "stream creation failed");
+    }
+  c->servers++;
+  S->th = std::thread(server_loop, S);
+  *out = S;
+  return DPX_OK;
+}
+
+void dpx_server_destroy(dpx_server* S) {
+  if (!S) return;
+  S->stop.store(true);
+  S->cv_work.notify_all();
+  if (S->th.joinable()) S->th.join();
+  (void)hipSetDevice(S->ctx->device);
+  for (Slot& sl : S->slots) sl.ws.release();
+  S->arena.mem.release();
+  S->ctx->servers--;
+  delete S;
+}
+
+int dpx_server_submit(dpx_server* S, const dpx_request* req, uint64_t* id) {
+  if (!S || !req || !id) return DPX_EINVAL;
+  Request q;
+  q.kind = req->kind;
+  q.trace = (req->flags & DPX_SUBMIT_TRACE) ? 1 : 0;
+  q.trace_from = (req->flags & DPX_SUBMIT_TRACE_FROM_BEST) ? 1 : 0;
+  const int n1 = req->n1, n2 = req->n2;
+  if (q.kind == DPX_REQ_ZUKER || q.kind == DPX_REQ_NUSSINOV) {
+    const int g = q.kind == DPX_REQ_ZUKER ? G_ZUKER : G_NUSSINOV;
+    if (!req->a || n1 < 0 || n1 > max_len(g) || req->min_loop < 0) return DPX_EINVAL;
+    if (g == G_ZUKER && !S->ctx->has_rna) return DPX_ENOPARAMS;
+    for (int x = 0; x < n1; ++x)
+      if (!req->a[x] || !std::strchr("ACGUTacgut", req->a[x])) return DPX_EINVAL;
+    q.s1.assign(req->a, n1);
+    q.min_loop = g == G_NUSSINOV ? req->min_loop : 0;
+    q.footprint = tri_footprint(g, n1, q.trace);
+    q.cells = tri_cells(n1);
+    q.size_class = size_class(n1);
+  } else if (q.kind == DPX_REQ_MATCHAIN) {
+    if (!req->rows || !req->cols || n1 < 1 || n1 > kMaxTri) return DPX_EINVAL;
+    q.rows.assign(req->rows, req->rows + n1);
+    q.cols.assign(req->cols, req->cols + n1);
+    q.footprint = tri_footprint(G_MATCHAIN, n1, q.trace);
+    q.cells = tri_cells(n1);
+    q.size_class = size_class(n1);
+  } else if (q.kind == DPX_REQ_SWAT) {
+    if (!req->a || !req->b || n1 < 0 || n2 < 0 || (int64_t)(n1 + 1) * (n2 + 1) >= ((int64_t)1 << 32)) return DPX_EINVAL;
+    q.s1.assign(req->a, n1);
+    q.s2.assign(req->b, n2);
+    q.sc = req->sc;
+    q.footprint = sw_layout(n1, n2, q.trace ? sw_trace_cap(n1, n2) : 0, nullptr);
+    q.cells = (int64_t)(n1 + 1) * (n2 + 1);
+  } else {
+    return DPX_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (q.footprint > a256((size_t)S->cfg.hbm_budget)) {
+    S->st.rejected++;
+    return DPX_ENOMEM;
+  }
+  if ((int64_t)S->pending.size() >= S->cfg.max_pending) {
+    S->st.rejected++;
+    return DPX_EBUSY;
+  }
+  q.t_submit = now_s();
+  q.id = S->next_id++;
+  *id = q.id;
+  S->st.submitted++;
+  S->pending.push_back(std::move(q));
+  S->st.pending = (int64_t)S->pending.size();
+  S->cv_work.notify_all();
+  return DPX_OK;
+}
+
+int dpx_server_poll(dpx_server* S, dpx_result* out, int max, int wait_us, int* got) {
+  if (!S || !got || max < 0 || (max > 0 && !out)) return DPX_EINVAL;
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (S->ready.empty() && wait_us > 0)
+    S->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [&] { return !S->ready.empty(); });
+  int n = 0;
+  while (n < max && !S->ready.empty()) {
+    const uint64_t id = S->ready.front();
+    S->ready.pop_front();
+    auto it = S->done.find(id);
+    if (it == S->done.end()) continue;
+    out[n++] = it->second.r;
+    if (!it->second.trace.empty()) S->traces[id] = std::move(it->second.trace);
+    S->done.erase(it);
+  }
+  *got = n;
+  return n ? DPX_OK : DPX_EAGAIN;
+}
+
+int dpx_server_trace(dpx_server* S, uint64_t id, dpx_trace_t* buf, int cap, int* len) {
+  if (!S || !len) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->traces.find(id);
+  if (it == S->traces.end()) {
+    *len = 0;
+    return DPX_EAGAIN;
+  }
+  *len = (int)it->second.size();
+  if (buf) {
+    if (cap < *len) return DPX_ETRACE;
+    std::memcpy(buf, it->second.data(), sizeof(dpx_trace_t) * it->second.size());
+  }
+  S->traces.erase(it);
+  return DPX_OK;
+}
+
+int dpx_server_get_stats(dpx_server* S, dpx_server_stats* out) {
+  if (!S || !out) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(S->mu);
+  *out = S->st;
   return DPX_OK;
 }
 

@@ -86,8 +86,8 @@ struct SwArgs {
   int* out;             // [0] = M(n1,n2)
 };
 int sw_launch(const SwArgs& A, const int2* row0, int grid, void* stream);
-int sw_trace_launch(const SwArgs& A, const uint8_t* r0m, const uint8_t* r0g2, int i0, int j0, void* tr, int cap,
-                    int* len, void* stream);
+int sw_finish_launch(const SwArgs& A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out,
+                     const uint8_t* r0m, const uint8_t* r0g2, void* tr, int cap, int* len, void* stream);
 int sw_occupancy();
 
 // launchers (tri.hip, finalize.hip); stream is a hipStream_t

@@ -145,13 +145,31 @@ __global__ void sw_reset(SwArgs A, const int2* row0) {
 // Traceback: the reference BFS from (i0,j0) (comp1.cu:100-132) is a single
 // chain here; one wave walks it, caching a 64-word window of the band's
 // traceback words in LDS (the walk only moves to smaller steps within a band).
+// Best cell over row 0 (host-computed: value, column) and the device rows
+// (largest M, then smallest (i, j)); start of the trace.  out: [0]=axiom
+// M(n1,n2), [1]=best, [2]=best_i, [3]=best_j, [4]=trace i0, [5]=trace j0.
+__global__ void sw_pick(SwArgs A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out) {
+  const unsigned long long key = *A.best;
+  const int W = A.n2 + 1;
+  int bv = row0_best, bi = 0, bj = row0_bestj;
+  if (A.nbands > 0) {
+    const int dv = (int)(key >> 32);
+    const uint32_t lin = 0xffffffffu - (uint32_t)(key & 0xffffffffu);
+    if (dv > bv) { bv = dv; bi = (int)(lin / (uint32_t)W); bj = (int)(lin % (uint32_t)W); }
+  }
+  out[0] = A.n1 == 0 ? row0_axiom : A.out[0];
+  out[1] = bv; out[2] = bi; out[3] = bj;
+  out[4] = trace_from ? bi : A.n1;
+  out[5] = trace_from ? bj : A.n2;
+}
+
 __global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mrule, const uint8_t* row0_g2rule,
-                                               int i0, int j0, SwTrace* tr, int cap, int* len) {
+                                               const int* start, SwTrace* tr, int cap, int* len) {
   constexpr int NW = 64;
   __shared__ uint32_t win[NW][64];
   const int lane = threadIdx.x;
   int wb = -1, w0 = 0;
-  int i = i0, j = j0, tab = 0, n = 0;
+  int i = start[4], j = start[5], tab = 0, n = 0;
   bool ok = true;
   for (;;) {
     int rule;
@@ -200,9 +218,11 @@ int sw_launch(const SwArgs& A, const int2* row0, int grid, void* stream) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int sw_trace_launch(const SwArgs& A, const uint8_t* r0m, const uint8_t* r0g2, int i0, int j0, void* tr, int cap,
-                    int* len, void* stream) {
-  hipLaunchKernelGGL(sw_trace, dim3(1), dim3(64), 0, (hipStream_t)stream, A, r0m, r0g2, i0, j0, (SwTrace*)tr, cap, len);
+int sw_finish_launch(const SwArgs& A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out,
+                     const uint8_t* r0m, const uint8_t* r0g2, void* tr, int cap, int* len, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(sw_pick, dim3(1), dim3(1), 0, s, A, row0_best, row0_bestj, row0_axiom, trace_from, out);
+  if (tr) hipLaunchKernelGGL(sw_trace, dim3(1), dim3(64), 0, s, A, r0m, r0g2, (const int*)out, (SwTrace*)tr, cap, len);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -16,8 +16,13 @@ OK, EINVAL, ENOMEM, EHIP, ENOPARAMS, EPARAMS, ETRACE, EBUSY, EAGAIN = range(9)
 EXPORTS = [
     "dpx_create", "dpx_destroy", "dpx_last_error", "dpx_set_rna_params",
     "dpx_nussinov", "dpx_zuker", "dpx_swat", "dpx_matchain",
-    "dpx_zuker_batch", "dpx_nussinov_batch", "dpx_last_timing",
+    "dpx_zuker_batch", "dpx_nussinov_batch", "dpx_last_timing", "dpx_now",
+    "dpx_server_create", "dpx_server_destroy", "dpx_server_submit", "dpx_server_poll",
+    "dpx_server_trace", "dpx_server_get_stats",
 ]
+
+REQ_NUSSINOV, REQ_ZUKER, REQ_SWAT, REQ_MATCHAIN = 1, 2, 3, 4
+SUBMIT_TRACE, SUBMIT_TRACE_FROM_BEST = 1, 2
 
 
 class DpxError(RuntimeError):
@@ -36,6 +41,34 @@ class SwatScores(ctypes.Structure):
                 ("gap_open", ctypes.c_int32), ("gap_ext", ctypes.c_int32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_int64), ("slots", ctypes.c_int32), ("max_batch", ctypes.c_int32),
+                ("max_pending", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("a", ctypes.c_char_p), ("n1", ctypes.c_int32),
+                ("b", ctypes.c_char_p), ("n2", ctypes.c_int32),
+                ("rows", ctypes.POINTER(ctypes.c_int32)), ("cols", ctypes.POINTER(ctypes.c_int32)),
+                ("sc", SwatScores), ("min_loop", ctypes.c_int32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("kind", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("score", ctypes.c_int32), ("best", ctypes.c_int32), ("best_i", ctypes.c_int32),
+                ("best_j", ctypes.c_int32), ("trace_len", ctypes.c_int32), ("batch_size", ctypes.c_int32),
+                ("cells", ctypes.c_int64), ("footprint", ctypes.c_int64),
+                ("t_submit", ctypes.c_double), ("t_launch", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("device_ms", ctypes.c_float), ("reserved", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in
+                ("submitted", "completed", "failed", "rejected", "launches", "cells_done",
+                 "bytes_in_flight", "peak_bytes_in_flight", "max_batch_seen", "pending")]
+
+
 _lib = None
 
 
@@ -49,6 +82,15 @@ def lib():
         L.dpx_last_error.argtypes = [ctypes.c_void_p]
         L.dpx_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
         L.dpx_destroy.argtypes = [ctypes.c_void_p]
+        L.dpx_now.restype = ctypes.c_double
+        L.dpx_server_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerConfig), ctypes.POINTER(ctypes.c_void_p)]
+        L.dpx_server_destroy.argtypes = [ctypes.c_void_p]
+        L.dpx_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+        L.dpx_server_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(Result), ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int)]
+        L.dpx_server_trace.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(Trace), ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int)]
+        L.dpx_server_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
         _lib = L
     return _lib
 
@@ -161,3 +203,106 @@ class Context:
         f, z = ctypes.c_float(), ctypes.c_float()
         self._check(self._L.dpx_last_timing(self.h, ctypes.byref(f), ctypes.byref(z)))
         return f.value, z.value
+
+
+def now():
+    return lib().dpx_now()
+
+
+class Server:
+    """Continuous-batching request loop on one context (dpx_server_* in include/dpx.h)."""
+
+    def __init__(self, ctx, hbm_budget=8 << 30, slots=3, max_batch=256, max_pending=1 << 20):
+        self._L = ctx._L
+        self.ctx = ctx
+        cfg = ServerConfig(hbm_budget, slots, max_batch, max_pending, 0)
+        h = ctypes.c_void_p()
+        rc = self._L.dpx_server_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h))
+        if rc != OK:
+            raise DpxError(rc, self._L.dpx_last_error(ctx.h).decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self._L.dpx_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _submit(self, req):
+        rid = ctypes.c_uint64()
+        rc = self._L.dpx_server_submit(self.h, ctypes.byref(req), ctypes.byref(rid))
+        if rc != OK:
+            raise DpxError(rc, "submit refused")
+        return rid.value
+
+    def submit_zuker(self, seq, trace=True):
+        return self._submit(Request(kind=REQ_ZUKER, flags=SUBMIT_TRACE if trace else 0, a=seq.encode(), n1=len(seq)))
+
+    def submit_nussinov(self, seq, min_loop=0, trace=True):
+        return self._submit(Request(kind=REQ_NUSSINOV, flags=SUBMIT_TRACE if trace else 0, a=seq.encode(),
+                                    n1=len(seq), min_loop=min_loop))
+
+    def submit_matchain(self, rows, cols, trace=True):
+        n = len(rows)
+        r = (ctypes.c_int32 * n)(*rows)
+        c = (ctypes.c_int32 * n)(*cols)
+        return self._submit(Request(kind=REQ_MATCHAIN, flags=SUBMIT_TRACE if trace else 0, n1=n,
+                                    rows=ctypes.cast(r, ctypes.POINTER(ctypes.c_int32)),
+                                    cols=ctypes.cast(c, ctypes.POINTER(ctypes.c_int32))))
+
+    def submit_swat(self, a, b, match, mismatch, gap_open, gap_ext, trace_from=0, trace=True):
+        flags = (SUBMIT_TRACE if trace else 0) | (SUBMIT_TRACE_FROM_BEST if trace_from else 0)
+        return self._submit(Request(kind=REQ_SWAT, flags=flags, a=a.encode(), n1=len(a), b=b.encode(), n2=len(b),
+                                    sc=SwatScores(match, mismatch, gap_open, gap_ext)))
+
+    def poll(self, max_results=256, wait_us=0):
+        out = (Result * max_results)()
+        got = ctypes.c_int()
+        rc = self._L.dpx_server_poll(self.h, out, max_results, wait_us, ctypes.byref(got))
+        if rc not in (OK, EAGAIN):
+            raise DpxError(rc, "poll failed")
+        return [out[q] for q in range(got.value)]
+
+    def trace(self, rid, length):
+        buf = (Trace * max(length, 1))()
+        tl = ctypes.c_int()
+        rc = self._L.dpx_server_trace(self.h, rid, buf, max(length, 1), ctypes.byref(tl))
+        if rc == EAGAIN:
+            return []
+        if rc != OK:
+            raise DpxError(rc, "trace fetch failed")
+        return _tuples(buf, tl.value)
+
+    def drop_trace(self, rid):
+        tl = ctypes.c_int()
+        self._L.dpx_server_trace(self.h, rid, None, 0, ctypes.byref(tl))
+
+    def stats(self):
+        st = ServerStats()
+        self._L.dpx_server_get_stats(self.h, ctypes.byref(st))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def drain(self, ids, timeout_s=120.0, keep_trace=None):
+        """Wait for every id in `ids`; returns {id: (Result, trace)}.  Traces
+        are fetched for the ids in keep_trace (all when None), dropped otherwise."""
+        want = set(ids)
+        got = {}
+        t_end = now() + timeout_s
+        while want:
+            if now() > t_end:
+                raise DpxError(EAGAIN, "server did not finish %d requests in %.0f s" % (len(want), timeout_s))
+            for r in self.poll(wait_us=20000):
+                tr = []
+                if r.trace_len > 0:
+                    if keep_trace is None or r.id in keep_trace:
+                        tr = self.trace(r.id, r.trace_len)
+                    else:
+                        self.drop_trace(r.id)
+                got[r.id] = (r, tr)
+                want.discard(r.id)
+        return got

@@ -10,7 +10,10 @@ on any machine and numpy version.  Generators:
 """
 import math
 
+import numpy as np
+
 MASK = (1 << 64) - 1
+GAMMA = 0x9E3779B97F4A7C15
 
 
 def _mix(z):
@@ -35,30 +38,34 @@ class SplitMix64:
         return self.next() % m
 
 
+def _stream(seed, n):
+    """The first n outputs of SplitMix64(seed).next(), vectorised (uint64
+    arithmetic wraps modulo 2**64 exactly like the scalar class)."""
+    s0 = np.uint64(_mix((seed + 0x632BE59BD9B4E019) & MASK))
+    with np.errstate(over="ignore"):
+        z = s0 + np.arange(1, n + 1, dtype=np.uint64) * np.uint64(GAMMA)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _letters(alphabet, idx):
+    return np.frombuffer(alphabet.encode(), dtype=np.uint8)[idx].tobytes().decode()
+
+
 def rna_uniform(n, seed):
-    r = SplitMix64(seed)
-    return "".join("ACGU"[r.below(4)] for _ in range(n))
+    return _letters("ACGU", (_stream(seed, n) % np.uint64(4)).astype(np.int64))
 
 
 def rna_gc(n, seed, gc=0.5):
-    r = SplitMix64(seed)
-    out = []
-    for _ in range(n):
-        u = r.uniform()
-        if u < gc / 2:
-            out.append("G")
-        elif u < gc:
-            out.append("C")
-        elif u < gc + (1 - gc) / 2:
-            out.append("A")
-        else:
-            out.append("U")
-    return "".join(out)
+    """P(G)=P(C)=gc/2, P(A)=P(U)=(1-gc)/2 from one uniform draw per base."""
+    u = (_stream(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+    idx = np.where(u < gc / 2, 0, np.where(u < gc, 1, np.where(u < gc + (1 - gc) / 2, 2, 3)))
+    return _letters("GCAU", idx)
 
 
 def dna_uniform(n, seed):
-    r = SplitMix64(seed)
-    return "".join("ACGT"[r.below(4)] for _ in range(n))
+    return _letters("ACGT", (_stream(seed, n) % np.uint64(4)).astype(np.int64))
 
 
 def dna_pair(n1, n2, seed):

@@ -28,3 +28,26 @@ def test_library_exports_all_declared():
 
 def test_python_binding_lists_exports():
     assert set(dpx.EXPORTS) <= set(_declared())
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of the header's structs have the C sizes and offsets."""
+    import subprocess
+    structs = {"dpx_trace_t": dpx.Trace, "dpx_swat_scores": dpx.SwatScores, "dpx_server_config": dpx.ServerConfig,
+               "dpx_request": dpx.Request, "dpx_result": dpx.Result, "dpx_server_stats": dpx.ServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dpx.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
+        for f in py._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (cname, f[0]))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = []
+    for py in structs.values():
+        want.append(ctypes.sizeof(py))
+        want += [getattr(py, f[0]).offset for f in py._fields_]
+    assert got == want
