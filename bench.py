@@ -5,6 +5,15 @@ Workload (one "step"): batched Zuker MFE + backtrace of 4096 RNA sequences,
 lengths log-uniform in [50, 2000], i.i.d. 50% GC (seed 3), one persistent
 tile-queue launch plus one finalize launch.  Prints one JSON line.
 
+Served stream (reported beside it under "served"): Poisson arrivals at a fixed
+rate of Zuker fold requests (lengths log-uniform 30-3000, 50% GC) and Gotoh SW
+requests (each sequence log-uniform 500-20000 bp, ACGT, +2/-3/-5/-2, traced
+from the best cell) in a 4:1 mix, plus matrix-chain requests (64-512
+matrices, dimensions uniform in [1,1000]) as 10% of all requests, through the
+continuous-batching server under a fixed HBM budget: p50/p99 latency, GCUPS,
+admission rejects, and a bit-exact check of a sample against the same
+requests solved alone.
+
 value = cell updates per second of the batched step, with inputs resident in
 HBM: the device time of the fill and finalize kernels (HIP events on the
 library's stream).  Also reported: the same sequences as plain per-instance
@@ -14,8 +23,10 @@ a bounded sample of the same sequences).
 """
 import argparse
 import json
+import math
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -37,6 +48,125 @@ def x_candidates(n):
     return m * (m + 1) * (m + 2) // 6 if m > 0 else 0
 
 
+def pct(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(math.ceil(p / 100.0 * len(xs))) - 1))
+    return xs[k]
+
+
+def served_requests(rate, seconds, seed=5):
+    """Seeded Poisson stream: (arrival s, kind, payload)."""
+    from dpx import gen
+    r = gen.SplitMix64(seed)
+
+    def log_uniform(lo, hi):
+        return int(round(math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * r.uniform())))
+
+    out, t, q = [], 0.0, 0
+    while True:
+        t += -math.log(1.0 - r.uniform()) / rate
+        if t >= seconds:
+            return out
+        u = r.uniform()
+        if u < 0.1:
+            rows, cols = gen.matrix_chain(64 + r.below(449), 500000 + q)
+            out.append((t, "matchain", (rows, cols)))
+        elif u < 0.1 + 0.9 * 0.8:
+            out.append((t, "zuker", gen.rna_gc(log_uniform(30, 3000), 600000 + q, 0.5)))
+        else:
+            a, b = gen.dna_pair(log_uniform(500, 20000), log_uniform(500, 20000), 700000 + q)
+            out.append((t, "swat", (a, b)))
+        q += 1
+
+
+SW_SCORES = (2, -3, -5, -2)
+
+
+def serve(ctx, rate, seconds, budget, slots, max_batch, check_every):
+    import dpx
+    reqs = served_requests(rate, seconds)
+    srv = dpx.Server(ctx, hbm_budget=budget, slots=slots, max_batch=max_batch)
+    # load every kernel once before the clock starts
+    w = [srv.submit_zuker("GGGGAAAACCCC"), srv.submit_swat("ACGT" * 20, "AGCT" * 20, *SW_SCORES, trace_from=1),
+         srv.submit_matchain([3, 4, 5], [4, 5, 6])]
+    srv.drain(w)
+    ids = [None] * len(reqs)
+    rejected = []
+
+    def produce():
+        t0 = dpx.now()
+        for q, (at, kind, x) in enumerate(reqs):
+            d = t0 + at - dpx.now()
+            if d > 0:
+                time.sleep(d)
+            try:
+                if kind == "zuker":
+                    ids[q] = srv.submit_zuker(x)
+                elif kind == "swat":
+                    ids[q] = srv.submit_swat(x[0], x[1], *SW_SCORES, trace_from=1)
+                else:
+                    ids[q] = srv.submit_matchain(x[0], x[1])
+            except dpx.DpxError as e:
+                rejected.append((q, e.code))
+
+    th = threading.Thread(target=produce)
+    t_start = dpx.now()
+    th.start()
+    check = set(range(0, len(reqs), check_every))
+    got = {}
+    while th.is_alive() or len(got) < sum(1 for x in ids if x is not None):
+        for res in srv.poll(wait_us=5000):
+            tr = None
+            if res.trace_len > 0:
+                tr = srv.trace(res.id, res.trace_len)
+            got[res.id] = (res, tr)
+    th.join()
+    st = srv.stats()
+    srv.close()
+    done = [got[i][0] for i in ids if i is not None]
+    t_end = max(r.t_done for r in done)
+    cells = sum(r.cells for r in done)
+    lat = {}
+    for k in ("zuker", "swat", "matchain"):
+        lat[k] = [(got[ids[q]][0].t_done - got[ids[q]][0].t_submit) * 1e3
+                  for q in range(len(reqs)) if reqs[q][1] == k and ids[q] is not None]
+    every = [x for v in lat.values() for x in v]
+    # bit-exact sample: the same requests solved alone
+    exact, checked = True, 0
+    for q in sorted(check):
+        if ids[q] is None:
+            continue
+        res, tr = got[ids[q]]
+        kind, x = reqs[q][1], reqs[q][2]
+        if kind == "zuker":
+            want = ctx.zuker(x)
+            ok = (res.score, tr) == want
+        elif kind == "swat":
+            s, best, wtr = ctx.swat(x[0], x[1], *SW_SCORES, trace_from=1)
+            ok = (res.score, (res.best, res.best_i, res.best_j), tr) == (s, best, wtr)
+        else:
+            ok = (res.score, tr) == ctx.matchain(x[0], x[1])
+        exact &= bool(ok) and res.status == 0
+        checked += 1
+    failed = sum(1 for r in done if r.status != 0)
+    return {
+        "gcups": round(cells / (t_end - t_start) / 1e9, 4),
+        "arrival_rate_per_s": rate, "seconds": seconds, "requests": len(reqs),
+        "completed": len(done) - failed, "failed": failed, "rejected": len(rejected),
+        "latency_ms": {k: {"p50": round(pct(v, 50), 3), "p99": round(pct(v, 99), 3), "n": len(v)}
+                       for k, v in lat.items() if v},
+        "latency_ms_all": {"p50": round(pct(every, 50), 3), "p99": round(pct(every, 99), 3)},
+        "cells": cells, "span_s": round(t_end - t_start, 3),
+        "hbm_budget_bytes": budget, "slots": slots, "max_batch": max_batch,
+        "launches": st["launches"], "mean_batch": round(len(done) / max(1, st["launches"] - 3), 2),
+        "peak_bytes_in_flight": st["peak_bytes_in_flight"],
+        "bitexact_sample": exact, "checked": checked,
+        "mix": "Zuker:SW 4:1 (72%:18%), matrix chain 10% of requests",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +176,12 @@ def main():
     ap.add_argument("--plain-stride", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=300.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=16.0)
+    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-check-every", type=int, default=25)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -78,15 +214,17 @@ def main():
     wall_ms = sum(walls) / args.steps * 1e3
 
     # plain per-instance launches over a fixed sample of the same sequences
-    sample = seqs[::args.plain_stride]
-    p_cells, p_ms = 0, 0.0
-    for s in sample:
-        r, _ = ctx.zuker(s)
-        f, z = ctx.last_timing()
-        p_cells += cells(len(s))
-        p_ms += f + z
-    plain = p_cells / (p_ms * 1e-3) / 1e9
-    print("bench: plain per-instance %.3f GCUPS over %d sequences" % (plain, len(sample)), file=sys.stderr, flush=True)
+    plain = None
+    if args.plain_stride > 0:
+        sample = seqs[::args.plain_stride]
+        p_cells, p_ms = 0, 0.0
+        for s in sample:
+            r, _ = ctx.zuker(s)
+            f, z = ctx.last_timing()
+            p_cells += cells(len(s))
+            p_ms += f + z
+        plain = p_cells / (p_ms * 1e-3) / 1e9
+        print("bench: plain per-instance %.3f GCUPS over %d sequences" % (plain, len(sample)), file=sys.stderr, flush=True)
 
     # bit-exact spot check against the oracle on the shortest sequences of the batch
     import oracle_check
@@ -97,6 +235,12 @@ def main():
     roofline = {"bound": "valu", "achieved": round(achieved, 4), "peak": round(PEAK_INT32_TOPS, 2), "unit": "TOP/s",
                 "frac": round(achieved / PEAK_INT32_TOPS, 5), "traffic": None,
                 "kernel": "tri_fill<GZu>", "ops": "2 int32 ops (add, min) per ml/ml1 split candidate"}
+
+    served = None
+    if not args.no_serve:
+        served = serve(ctx, args.serve_rate, args.serve_seconds, int(args.serve_budget_gib * (1 << 30)),
+                       args.serve_slots, 256, args.serve_check_every)
+        print("bench: served %s" % json.dumps(served), file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -109,10 +253,11 @@ def main():
         "config": {"workload": "batched Zuker MFE + backtrace: %d RNA sequences, lengths log-uniform [50,2000], "
                                "50%% GC, seed 3, Turner 2004 parameters" % len(seqs),
                    "instances": len(seqs), "cells": total_cells, "parallelism": "single GPU, persistent tile queue"},
-        "plain_per_instance_gcups": round(plain, 4), "speedup_vs_plain": round(value / plain, 2),
+        "plain_per_instance_gcups": round(plain, 4) if plain else None,
+        "speedup_vs_plain": round(value / plain, 2) if plain else None,
         "wall_ms_per_step_pcie": round(wall_ms, 3), "fill_ms": round(fill_ms, 3),
         "finalize_ms": round(sum(fins) / args.steps, 3), "bitexact_sample": exact,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "served": served,
     }
     print(json.dumps(line))
 

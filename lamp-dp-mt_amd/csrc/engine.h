@@ -31,8 +31,10 @@ enum Grammar { G_NUSSINOV = 0, G_ZUKER = 1, G_MATCHAIN = 2 };
 // tables per tile record
 constexpr int NT_NU = 1;  // s
 constexpr int NT_MC = 1;  // M
-constexpr int NT_ZU = 5;  // cl, U (= dangle-closed cl + ML_intern), ml, ml1, R11 (row prefix-min of U)
-enum { ZT_CL = 0, ZT_U = 1, ZT_ML = 2, ZT_ML1 = 3, ZT_R11 = 4 };
+// cl, U (= dangle-closed cl + ML_intern), ml, ml1, R11 (row prefix-min of U),
+// B (= cl + interior mismatch of the pair as the inner pair of a loop)
+constexpr int NT_ZU = 6;
+enum { ZT_CL = 0, ZT_U = 1, ZT_ML = 2, ZT_ML1 = 3, ZT_R11 = 4, ZT_B = 5 };
 
 struct TriInst {
   int32_t n;         // sequence length / chain length; cells 0..n

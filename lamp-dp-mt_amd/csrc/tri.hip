@@ -625,11 +625,92 @@ struct GZu {
     }
   }
 
+  static constexpr int kMask = 1 << 28;  // masked loop energy: candidates through it stay >= INV_THR
+
+  // Rules 2, 3 and 4 of listed cell (i,j) (tile coordinates r, c): lane ap of
+  // the cell's 32 lanes evaluates its share, the result is the minimum over
+  // the 32 lanes.  Lane ap < 30 (left size a = ap+1): the generic interior
+  // loops (a, b = 2..30), the loops (a,1) and (1,a) for a >= 2, and the left
+  // and right bulges of size a.  Lane 30: (1,1) and (2,2); lane 31: (2,3) and
+  // (3,2).  B(l,k) = cl(l,k) + interior mismatch of (l,k), so
+  // cl(l,k) = B(l,k) - inner(l,k) (librna_impl.h:199-265).
+  __device__ static int window_lane(const ZuView& z, Smem& sm, const ZuPre& pc, int ap, int i, int j, int r, int c,
+                                    const int* fg, int f_a1, int f_1a) {
+    const ZuTab& E = sm.E;
+    int best = INV;
+    if (ap < 30) {
+      const int aa = ap + 1;
+      const int* Brow = &sm.u.w.B[r + 2 + aa][c + 30];  // Brow[-b] = B(i+2+aa, j-2-b)
+      int g = INV;
+#pragma unroll
+      for (int b = 2; b <= 30; ++b) g = imin(g, Brow[-b] + fg[b - 2]);
+      best = g + pc.outI;
+      const int Rl = z.R(i + 2 + aa), Rl1 = z.R(i + 1 + aa), Cj3 = z.C(j - 3);
+      if (aa >= 2) {
+        // (aa, 1): inner pair (i+2+aa, j-3); 2x1 from int21, else 1xn
+        const int Bv = Brow[-1], icb = bpc(z.C(j - 4), Rl);
+        const int cl = Bv - E.mmI[icb][Cj3][Rl1];
+        const int e = aa == 2 ? z.il(i, j, i + 4, j - 3) + pc.sr : f_a1 + E.mm1nI[icb][Cj3][Rl1] + pc.out1n;
+        if (Bv < INV_THR) best = imin(best, cl + e);
+        // (1, aa): inner pair (i+3, j-2-aa); 1x2 from int21, else 1xn
+        const int k = j - 2 - aa, Bw = sm.u.w.B[r + 3][c + 30 - aa];
+        const int Ri2 = z.R(i + 2), Ck = z.C(k), icb2 = bpc(z.C(k - 1), z.R(i + 3));
+        const int cl2 = Bw - E.mmI[icb2][Ck][Ri2];
+        const int e2 = aa == 2 ? z.il(i, j, i + 3, k) + pc.sr : f_1a + E.mm1nI[icb2][Ck][Ri2] + pc.out1n;
+        if (Bw < INV_THR) best = imin(best, cl2 + e2);
+      }
+      {
+        // left bulge of size aa: inner pair (i+2+aa, j-2)
+        const int Bv = Brow[0], icb = bpc(Cj3, Rl);
+        const int cl = Bv - E.mmI[icb][z.C(j - 2)][Rl1];
+        const int e = aa == 1 ? E.bulge[1] + E.stack[pc.ocb][icb] : E.bulge[aa] + pc.tauo + z.tau(Cj3, Rl);
+        if (Bv < INV_THR) best = imin(best, cl + e + pc.sr);
+      }
+      {
+        // right bulge of size aa: inner pair (i+2, j-2-aa)
+        const int k = j - 2 - aa, Bv = sm.u.w.B[r + 2][c + 30 - aa];
+        const int Ri2 = z.R(i + 2), Ck1 = z.C(k - 1), icb = bpc(Ck1, Ri2);
+        const int cl = Bv - E.mmI[icb][z.C(k)][z.R(i + 1)];
+        const int e = aa == 1 ? E.bulge[1] + E.stack[pc.ocb][icb] : E.bulge[aa] + pc.tauo + z.tau(Ck1, Ri2);
+        if (Bv < INV_THR) best = imin(best, cl + e + pc.sr);
+      }
+    } else if (ap == 30) {
+      // 1x1 and 2x2 loops (int11, int22)
+      const int B11 = sm.u.w.B[r + 3][c + 29], B22 = sm.u.w.B[r + 4][c + 28];
+      if (B11 < INV_THR) best = imin(best, B11 - z.inner(i + 3, j - 3) + z.il(i, j, i + 3, j - 3) + pc.sr);
+      if (B22 < INV_THR) best = imin(best, B22 - z.inner(i + 4, j - 4) + z.il(i, j, i + 4, j - 4) + pc.sr);
+    } else {
+      // 2x3 and 3x2 loops
+      const int B23 = sm.u.w.B[r + 4][c + 27], B32 = sm.u.w.B[r + 5][c + 28];
+      if (B23 < INV_THR) {
+        const int l = i + 4, k = j - 5, icb = bpc(z.C(k - 1), z.R(l)), Ck = z.C(k), Rl1 = z.R(l - 1);
+        best = imin(best, B23 - E.mmI[icb][Ck][Rl1] + E.mm23I[icb][Ck][Rl1] + pc.out23);
+      }
+      if (B32 < INV_THR) {
+        const int l = i + 5, k = j - 4, icb = bpc(z.C(k - 1), z.R(l)), Ck = z.C(k), Rl1 = z.R(l - 1);
+        best = imin(best, B32 - E.mmI[icb][Ck][Rl1] + E.mm23I[icb][Ck][Rl1] + pc.out23);
+      }
+    }
+    for (int m = 16; m >= 1; m >>= 1) best = imin(best, __shfl_xor(best, m));
+    return best;
+  }
+
   __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm) {
     const int tid = threadIdx.x, IT = I * T, JT = J * T, n = in.n, nb = in.nb;
     const uint8_t* seq = a.codes + in.seq_off;
     const DevRna* P = (const DevRna*)a.rna;
     const ZuView z{sm.E, P, a.hl_ent, sm.rs, sm.cs, IT - 1, JT - 34, n};
+    // interior-loop lane of this thread: left size a = ap + 1 (ap < 30); its
+    // row of generic loop energies F[a][2..30] in registers, the special
+    // positions (all of a == 1, (2,2), (2,3), (3,2)) masked with kMask
+    const int ap = tid & 31, aa = ap + 1;
+    int fg[29];
+#pragma unroll
+    for (int b = 2; b <= 30; ++b) {
+      const bool special = ap >= 30 || aa == 1 || (aa == 2 && b <= 3) || (aa == 3 && b == 2);
+      fg[b - 2] = special ? kMask : sm.E.F[ap < 30 ? aa : 1][b];
+    }
+    const int f_a1 = sm.E.F[ap < 30 ? aa : 1][1], f_1a = sm.E.F[1][ap < 30 ? aa : 1];
     int acc[4][4];
     for (int t = 0; t < 4; ++t)
       for (int u = 0; u < 4; ++u) acc[t][u] = INV;
@@ -656,14 +737,21 @@ struct GZu {
       if (I2 >= nb || J2 >= nb) return INV;
       return rec_ptr(a, in, NT_ZU, I2, J2)[tab * T * T + (i % T) * T + (j % T)];
     };
-    for (int e = tid; e < BR * BC; e += NTHR) {
-      const int rr = e / BC, cc = e % BC, l = IT + rr, k = JT - 32 + cc;
-      int v = INV;
-      if (!(rr < T && cc >= 32)) {
-        const int cl = fetch(ZT_CL, l, k);
-        if (cl < INV_THR && l >= 1 && k <= n - 1) v = cl + z.inner(l, k);
+    {
+      // B quadrants: tiles (I,J-1) | current tile / (I+1,J-1) | (I+1,J); absent tiles are INV
+      const int* qb[3] = {J - 1 >= I ? rec_ptr(a, in, NT_ZU, I, J - 1) + ZT_B * T * T : nullptr,
+                          I + 1 <= J - 1 ? rec_ptr(a, in, NT_ZU, I + 1, J - 1) + ZT_B * T * T : nullptr,
+                          I + 1 <= J ? rec_ptr(a, in, NT_ZU, I + 1, J) + ZT_B * T * T : nullptr};
+      for (int quad = 0; quad < 3; ++quad) {
+        const int x = tid * 4, rr = x / T + (quad ? T : 0), cc = x % T + (quad == 2 ? T : 0);
+        const int4 v = qb[quad] ? *(const int4*)(qb[quad] + x) : make_int4(INV, INV, INV, INV);
+        int* dst = &sm.u.w.B[rr][cc];
+        dst[0] = v.x;
+        dst[1] = v.y;
+        dst[2] = v.z;
+        dst[3] = v.w;
       }
-      sm.u.w.B[rr][cc] = v;
+      for (int e = tid; e < T * T; e += NTHR) sm.u.w.B[e / T][T + e % T] = INV;
     }
     for (int e = tid; e < T * (T + 1); e += NTHR) {
       const int rr = e / (T + 1), cc = e % (T + 1);
@@ -690,127 +778,51 @@ struct GZu {
     int32_t* gcl = rec_ptr(a, in, NT_ZU, I, J) + ZT_CL * T * T;
     for (int ld0 = dlo; ld0 <= T - 1; ld0 += G) {
       // ---------------- heavy phase: band [ld0, ld0+G) ----------------
+      // Lanes [0, wl): 32 per listed cell (its interior loops and bulges),
+      // padded to whole waves; lanes [wl, wl + 2*ncell): two per cell of the
+      // band (split points of X inside block I, inside block J).  Every wave
+      // runs one kind of lane with fixed trip counts: the operands outside a
+      // rule's index range are INV in the tables, so no bounds are tested.
       const int qn = imin(G, T - ld0);
-      int nedge[G + 1];
-      nedge[0] = 0;
-      for (int q = 0; q < G; ++q) {
-        const int ld = ld0 + q;
-        nedge[q + 1] = nedge[q] + (q < qn ? 4 * (T - (ld < 0 ? -ld : ld)) : 0);
-      }
-      const int ne = nedge[G], nv = sm.nlist, nw = nv * 30;
-      const int total = ne + nw + nv * 4;
+      int cpre[G + 1];
+      cpre[0] = 0;
+      for (int q = 0; q < G; ++q) cpre[q + 1] = cpre[q] + (q < qn ? T - abs(ld0 + q) : 0);
+      const int nv = sm.nlist, wl = ((nv + 1) >> 1) * 64, total = wl + 2 * cpre[G];
       for (int w = tid; w < total; w += NTHR) {
-        if (w < ne) {
-          // split points of X inside blocks I and J
-          int q = 0;
-          while (w >= nedge[q + 1]) ++q;
-          const int ld = ld0 + q, rmin = ld < 0 ? -ld : 0;
-          const int we = w - nedge[q], p = we >> 2, part = we & 3, r = rmin + p, c = r + ld, i = IT + r, j = JT + c;
-          if (j > n || j - i < 14) continue;
-          int k1lo, k1hi, k2lo, k2hi;
-          if (I == J) { k1lo = i + 7; k1hi = j - 7; k2lo = 1; k2hi = 0; }
-          else { k1lo = i + 7; k1hi = imin(IT + T - 1, j - 7); k2lo = imax(JT, i + 7); k2hi = j - 7; }
-          const int n1 = imax(0, k1hi - k1lo + 1), n2 = imax(0, k2hi - k2lo + 1), len = n1 + n2;
-          const int b0 = part * len / 4, b1 = (part + 1) * len / 4;
-          int best = INV;
-          for (int x = b0; x < b1; ++x) {
-            int v;
-            if (x < n1) {
-              const int k = k1lo + x;
-              const int left = I == J ? sm.u.w.U[r][k - JT + 1] : gLU[r * T + (k - IT)];
-              v = left + sm.u.w.ML1[k - IT][c];
-            } else {
-              const int k = k2lo + (x - n1);
-              v = sm.u.w.U[r][k - JT + 1] + sm.u.w.RM1[k - JT][c];
-            }
-            best = imin(best, v);
-          }
-          if (best < INV_THR) atomicMin(&sm.X[r][c], best);
-          continue;
-        }
-        const bool win = w < ne + nw;
-        const int w2 = win ? w - ne : w - ne - nw;
-        const int v = win ? w2 / 30 : w2 >> 2, slot = win ? w2 % 30 : (w2 & 3);
-        const int qp = sm.list[v], q = qp / T, p = qp % T, ld = ld0 + q;
-        const int r = (ld < 0 ? -ld : 0) + p, c = r + ld, i = IT + r, j = JT + c, d = j - i;
-        const ZuPre& pc = sm.pre[q][p];
-        int best = INV;
-        if (win) {
-          // interior loops with left size a = slot+1 (rule 4): l = i+2+a fixed
-          const int aa = slot + 1, bmax = imin(30, d - 11 - aa);
-          if (bmax >= 1) {
-            const int l = i + 2 + aa;
-            const int* Brow = &sm.u.w.B[r + 2 + aa][c + 30];  // Brow[-b] = B(l, j-2-b)
-            if (aa >= 2) {
-              int gen = INV;
-              const int bst = aa == 2 ? 4 : aa == 3 ? 3 : 2;
-              for (int b = bst; b <= bmax; ++b) gen = imin(gen, Brow[-b] + sm.E.F[aa][b]);
-              if (gen < INV_THR) best = gen + pc.outI;
-            }
-            // special sizes: a==1 row, b==1 column, (2,2) (2,3) (3,2); cl = B - inner
-            const int Rl = z.R(l), Rl1 = z.R(l - 1);
-            const int bsp = aa == 1 ? bmax : aa == 2 ? imin(3, bmax) : aa == 3 ? imin(2, bmax) : 1;
-            int Ck = z.C(j - 3);
-            for (int b = 1; b <= bsp; ++b) {
-              const int k = j - 2 - b, Ck1 = z.C(k - 1);
-              const int Bv = Brow[-b];
-              if (Bv < INV_THR) {
-                const int icb = bpc(Ck1, Rl);
-                const int cl = Bv - sm.E.mmI[icb][Ck][Rl1];
-                int e;
-                if ((aa == 1 && b >= 3) || (b == 1 && aa >= 3))
-                  e = sm.E.F[aa][b] + sm.E.mm1nI[icb][Ck][Rl1] + pc.out1n;
-                else if ((aa == 2 && b == 3) || (aa == 3 && b == 2))
-                  e = sm.E.mm23I[icb][Ck][Rl1] + pc.out23;
-                else
-                  e = z.il(i, j, l, k) + pc.sr;  // 1x1, 1x2, 2x1, 2x2 tables
-                best = imin(best, cl + e);
-              }
-              Ck = Ck1;
-            }
-          }
-        } else if (slot < 2) {
-          // left bulges (rule 2): inner cl(k, j-2), size k-i-2; half of the k range each
-          const int klo = i + 3, khi = imin(j - 9, i + 32), len = khi - klo + 1;
-          const int x0 = klo + slot * len / 2, x1 = klo + (slot + 1) * len / 2;
-          const int Cj3 = z.C(j - 3), Cj2 = z.C(j - 2);
-          const int* Bcol = &sm.u.w.B[0][c + 30];  // Bcol[x * BS] = B(IT + x, j-2)
-          int Rk1 = x0 < x1 ? z.R(x0 - 1) : 0;
-          for (int k = x0; k < x1; ++k) {
-            const int Rk = z.R(k);
-            const int Bv = Bcol[(k - IT) * BS];
-            if (Bv < INV_THR) {
-              const int icb = bpc(Cj3, Rk);
-              const int cl = Bv - sm.E.mmI[icb][Cj2][Rk1];
-              const int size = k - i - 2;
-              const int e = size == 1 ? sm.E.bulge[1] + sm.E.stack[pc.ocb][icb]
-                                      : sm.E.bulge[size] + pc.tauo + z.tau(Cj3, Rk);
-              best = imin(best, cl + e + pc.sr);
-            }
-            Rk1 = Rk;
-          }
+        if (w < wl) {
+          const int v = w >> 5;
+          if (v >= nv) continue;
+          const int qp = sm.list[v], q = qp / T, p = qp % T, ld = ld0 + q;
+          const int r = (ld < 0 ? -ld : 0) + p, c = r + ld, i = IT + r, j = JT + c;
+          const int best = window_lane(z, sm, sm.pre[q][p], ap, i, j, r, c, fg, f_a1, f_1a);
+          if (ap == 0) sm.acc[q][p] = best;
         } else {
-          // right bulges (rule 3): inner cl(i+2, k), size j-k-2
-          const int klo = imax(i + 9, j - 32), khi = j - 3, len = khi - klo + 1;
-          const int h = slot - 2, x0 = klo + h * len / 2, x1 = klo + (h + 1) * len / 2;
-          const int Ri2 = z.R(i + 2), Ri1 = z.R(i + 1);
-          const int* Brow = &sm.u.w.B[r + 2][32 - JT];
-          int Ck1 = x0 < x1 ? z.C(x0 - 1) : 0;
-          for (int k = x0; k < x1; ++k) {
-            const int Ck = z.C(k);
-            const int Bv = Brow[k];
-            if (Bv < INV_THR) {
-              const int icb = bpc(Ck1, Ri2);
-              const int cl = Bv - sm.E.mmI[icb][Ck][Ri1];
-              const int size = j - k - 2;
-              const int e = size == 1 ? sm.E.bulge[1] + sm.E.stack[pc.ocb][icb]
-                                      : sm.E.bulge[size] + pc.tauo + z.tau(Ck1, Ri2);
-              best = imin(best, cl + e + pc.sr);
+          const int e = (w - wl) >> 1, part = w & 1;
+          int q = 0;
+          while (e >= cpre[q + 1]) ++q;
+          const int ld = ld0 + q, r = (ld < 0 ? -ld : 0) + (e - cpre[q]), c = r + ld;
+          int best = INV;
+          if (part == 0) {
+            if (I == J) {
+#pragma unroll 8
+              for (int x = 0; x < T; ++x) best = imin(best, sm.u.w.U[r][x + 1] + sm.u.w.ML1[x][c]);
+            } else {
+              const int4* g4 = (const int4*)(gLU + r * T);
+#pragma unroll 2
+              for (int x4 = 0; x4 < T / 4; ++x4) {
+                const int4 u = g4[x4];
+                const int x = x4 * 4;
+                best = imin(imin(best, u.x + sm.u.w.ML1[x][c]), u.y + sm.u.w.ML1[x + 1][c]);
+                best = imin(imin(best, u.z + sm.u.w.ML1[x + 2][c]), u.w + sm.u.w.ML1[x + 3][c]);
+              }
             }
-            Ck1 = Ck;
+          } else if (I < J) {
+#pragma unroll 8
+            for (int x = 0; x < T; ++x) best = imin(best, sm.u.w.U[r][x + 1] + sm.u.w.RM1[x][c]);
           }
+          best = imin(best, __shfl_xor(best, 1));
+          if (part == 0 && best < INV_THR) sm.X[r][c] = imin(sm.X[r][c], best);
         }
-        if (best < INV_THR) atomicMin(&sm.acc[q][p], best);
       }
       __syncthreads();
       // ---------------- light steps: finalize the band diagonal by diagonal ----------------
@@ -869,6 +881,7 @@ struct GZu {
       g[ZT_ML * T * T + e] = sm.u.w.ML[r][c + 2];
       g[ZT_ML1 * T * T + e] = sm.u.w.ML1[r][c];
       g[ZT_R11 * T * T + e] = sm.u.w.R11[r][c + 1];
+      g[ZT_B * T * T + e] = sm.u.w.B[r][T + c];
     }
   }
 };
