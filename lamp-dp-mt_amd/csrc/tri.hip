@@ -154,7 +154,7 @@ __device__ __forceinline__ unsigned long long claim_item(const TriArgs& a) {
 }
 
 template <class G>
-__global__ __launch_bounds__(NTHR) void tri_fill(TriArgs a) {
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) void tri_fill(TriArgs a) {
   __shared__ typename G::Smem sm;
   __shared__ unsigned long long s_item;
   G::init(a, sm);
@@ -420,10 +420,6 @@ struct ZuTab {
   int16_t mmH[8][5][5], mmI[8][5][5], mm1nI[8][5][5], mm23I[8][5][5], mmM[8][5][5], mmExt[8][5][5];
   int16_t d5[8][5], d3[8][5];
   int16_t bulge[32], hairpin[32];
-  int16_t F[31][34];  // row stride of 17 words: lanes with different a hit distinct banks
-  int16_t loop_e[3][40];
-  int32_t loop_key[3][40];
-  int32_t n_loop[3];
 };
 
 // Per-cell terms that depend on the sequence only, precomputed one wavefront
@@ -437,10 +433,6 @@ struct ZuPre {
   int16_t boff;   // B - cl = interior mismatch of (i,j) as an inner pair (kNoB if unused)
   int16_t outI;   // generic interior-loop outer mismatch + sr
   int16_t in0;    // interior mismatch of (i+1,j-1): recovers cl(i+1,j-1) from B (rule 0)
-  int16_t out1n;  // 1xn loops: outer mismatch1nI + sr
-  int16_t out23;  // 2x3 loops: il(5) + ninio + outer mismatch23I + sr
-  int16_t ocb;    // pair code of (i+1, j-2)
-  int16_t tauo;   // termau(i+1, j-2)
 };
 constexpr int kNoB = -32768;
 
@@ -469,8 +461,8 @@ struct ZuView {
     if (cls >= 0) {  // special loops: the window lies on the row side (size <= 6)
       int key = 0;
       for (int m = 0; m <= size + 1; ++m) key = key * 8 + R(a + m);
-      for (int e = 0; e < E.n_loop[cls]; ++e)
-        if (E.loop_key[cls][e] == key) return E.loop_e[cls][e];
+      for (int e = 0; e < P->n_loop[cls]; ++e)
+        if (P->loop_key[cls][e] == key) return P->loop_e[cls][e];
     }
     const int ent = size <= 30 ? E.hairpin[size] : hl[size];
     if (size == 3) return ent + tau(R(a), C(b));
@@ -491,7 +483,7 @@ struct ZuView {
     if (sl == 1) {
       if (srr == 1) return P->int11[ocb][bpc(C(j - 4), R(i + 3))][ol][orr];
       if (srr == 2) return P->int21[ocb][bpc(C(j - 5), R(i + 3))][ol][C(j - 4)][orr];
-      return E.F[sl][srr] + E.mm1nI[ocb][ol][orr] + E.mm1nI[icb][il_][ir];
+      return P->F[sl][srr] + E.mm1nI[ocb][ol][orr] + E.mm1nI[icb][il_][ir];
     }
     if (sl == 2) {
       if (srr == 1) return P->int21[bpc(C(j - 4), R(i + 4))][ocb][orr][ol][R(i + 3)];
@@ -500,9 +492,9 @@ struct ZuView {
     } else if (sl == 3 && srr == 2) {
       return P->il5 + P->ninio0 + E.mm23I[ocb][ol][orr] + E.mm23I[icb][il_][ir];
     } else if (srr == 1) {
-      return E.F[sl][srr] + E.mm1nI[ocb][ol][orr] + E.mm1nI[icb][il_][ir];
+      return P->F[sl][srr] + E.mm1nI[ocb][ol][orr] + E.mm1nI[icb][il_][ir];
     }
-    return E.F[sl][srr] + E.mmI[ocb][ol][orr] + E.mmI[icb][il_][ir];
+    return P->F[sl][srr] + E.mmI[ocb][ol][orr] + E.mmI[icb][il_][ir];
   }
   // bl_energy / br_energy terms of rules 2 and 3 (librna_impl.h:199-215)
   __device__ int bulge_l(int i, int j, int k) const {  // inner cl(k, j-2), size k-i-2
@@ -529,10 +521,6 @@ struct ZuView {
     const int ocb = bpc(R(i + 1), C(j - 2)), ol = R(i + 2), orr = C(j - 3);
     o->outI = E.mmI[ocb][ol][orr] + s_;
     o->in0 = d >= 9 ? inner(i + 1, j - 1) : 0;
-    o->out1n = E.mm1nI[ocb][ol][orr] + s_;
-    o->out23 = P->il5 + P->ninio0 + E.mm23I[ocb][ol][orr] + s_;
-    o->ocb = ocb;
-    o->tauo = tau(R(i + 1), C(j - 2));
   }
 };
 
@@ -559,7 +547,7 @@ struct GZu {
     } u;
     int X[T][T + 1];
     ZuTab E;
-    ZuPre pre[G][T];
+    ZuPre pre[2][G][T];  // double buffer: light steps of one band, sequence terms of the next
     uint8_t rs[72], cs[72];
     int acc[G][T];
     int16_t list[G * T];
@@ -583,18 +571,12 @@ struct GZu {
       (&E.d3[0][0])[e] = (int16_t)(&P->d3[0][0])[e];
     }
     for (int e = threadIdx.x; e < 31; e += NTHR) { E.bulge[e] = (int16_t)P->bulge[e]; E.hairpin[e] = (int16_t)P->hairpin[e]; }
-    for (int e = threadIdx.x; e < 31 * 31; e += NTHR) E.F[e / 31][e % 31] = P->F[e / 31][e % 31];
-    for (int e = threadIdx.x; e < 120; e += NTHR) {
-      (&E.loop_key[0][0])[e] = (&P->loop_key[0][0])[e];
-      (&E.loop_e[0][0])[e] = (int16_t)(&P->loop_e[0][0])[e];
-    }
-    if (threadIdx.x < 3) E.n_loop[threadIdx.x] = P->n_loop[threadIdx.x];
   }
 
   // sequence-only terms of the cells of band [ld0, ld0+G) and the compacted
   // list of cells whose cl can close bulges / interior loops; all threads of
   // `nthr` starting at thread index `t0` take part, wave `lw` builds the list
-  __device__ static void prep_band(const ZuView& z, int IT, int JT, int ld0, Smem& sm, int t) {
+  __device__ static void prep_band(const ZuView& z, int IT, int JT, int ld0, Smem& sm, int t, int buf) {
     // t in [0, 192): t < 64 -> list (one wave), else per-cell terms
     if (t < 64) {
       int base = 0;
@@ -620,7 +602,7 @@ struct GZu {
         const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
         if (p >= ncell) continue;
         const int r = rmin + p;
-        z.pre(IT + r, JT + r + ld, &sm.pre[q][p]);
+        z.pre(IT + r, JT + r + ld, &sm.pre[buf][q][p]);
       }
     }
   }
@@ -637,6 +619,9 @@ struct GZu {
   __device__ static int window_lane(const ZuView& z, Smem& sm, const ZuPre& pc, int ap, int i, int j, int r, int c,
                                     const int* fg, int f_a1, int f_1a) {
     const ZuTab& E = sm.E;
+    // outer pair (i+1, j-2) of the loops and its mismatch bases
+    const int Ri1 = z.R(i + 1), Ri2 = z.R(i + 2), Cj2 = z.C(j - 2), Cj3 = z.C(j - 3);
+    const int ocb = bpc(Ri1, Cj2);
     int best = INV;
     if (ap < 30) {
       const int aa = ap + 1;
@@ -645,33 +630,34 @@ struct GZu {
 #pragma unroll
       for (int b = 2; b <= 30; ++b) g = imin(g, Brow[-b] + fg[b - 2]);
       best = g + pc.outI;
-      const int Rl = z.R(i + 2 + aa), Rl1 = z.R(i + 1 + aa), Cj3 = z.C(j - 3);
+      const int Rl = z.R(i + 2 + aa), Rl1 = z.R(i + 1 + aa);
+      const int out1n = E.mm1nI[ocb][Ri2][Cj3] + pc.sr, tauo = z.tau(Ri1, Cj2);
       if (aa >= 2) {
         // (aa, 1): inner pair (i+2+aa, j-3); 2x1 from int21, else 1xn
         const int Bv = Brow[-1], icb = bpc(z.C(j - 4), Rl);
         const int cl = Bv - E.mmI[icb][Cj3][Rl1];
-        const int e = aa == 2 ? z.il(i, j, i + 4, j - 3) + pc.sr : f_a1 + E.mm1nI[icb][Cj3][Rl1] + pc.out1n;
+        const int e = aa == 2 ? z.il(i, j, i + 4, j - 3) + pc.sr : f_a1 + E.mm1nI[icb][Cj3][Rl1] + out1n;
         if (Bv < INV_THR) best = imin(best, cl + e);
         // (1, aa): inner pair (i+3, j-2-aa); 1x2 from int21, else 1xn
         const int k = j - 2 - aa, Bw = sm.u.w.B[r + 3][c + 30 - aa];
-        const int Ri2 = z.R(i + 2), Ck = z.C(k), icb2 = bpc(z.C(k - 1), z.R(i + 3));
+        const int Ck = z.C(k), icb2 = bpc(z.C(k - 1), z.R(i + 3));
         const int cl2 = Bw - E.mmI[icb2][Ck][Ri2];
-        const int e2 = aa == 2 ? z.il(i, j, i + 3, k) + pc.sr : f_1a + E.mm1nI[icb2][Ck][Ri2] + pc.out1n;
+        const int e2 = aa == 2 ? z.il(i, j, i + 3, k) + pc.sr : f_1a + E.mm1nI[icb2][Ck][Ri2] + out1n;
         if (Bw < INV_THR) best = imin(best, cl2 + e2);
       }
       {
         // left bulge of size aa: inner pair (i+2+aa, j-2)
         const int Bv = Brow[0], icb = bpc(Cj3, Rl);
-        const int cl = Bv - E.mmI[icb][z.C(j - 2)][Rl1];
-        const int e = aa == 1 ? E.bulge[1] + E.stack[pc.ocb][icb] : E.bulge[aa] + pc.tauo + z.tau(Cj3, Rl);
+        const int cl = Bv - E.mmI[icb][Cj2][Rl1];
+        const int e = aa == 1 ? E.bulge[1] + E.stack[ocb][icb] : E.bulge[aa] + tauo + z.tau(Cj3, Rl);
         if (Bv < INV_THR) best = imin(best, cl + e + pc.sr);
       }
       {
         // right bulge of size aa: inner pair (i+2, j-2-aa)
         const int k = j - 2 - aa, Bv = sm.u.w.B[r + 2][c + 30 - aa];
-        const int Ri2 = z.R(i + 2), Ck1 = z.C(k - 1), icb = bpc(Ck1, Ri2);
-        const int cl = Bv - E.mmI[icb][z.C(k)][z.R(i + 1)];
-        const int e = aa == 1 ? E.bulge[1] + E.stack[pc.ocb][icb] : E.bulge[aa] + pc.tauo + z.tau(Ck1, Ri2);
+        const int Ck1 = z.C(k - 1), icb = bpc(Ck1, Ri2);
+        const int cl = Bv - E.mmI[icb][z.C(k)][Ri1];
+        const int e = aa == 1 ? E.bulge[1] + E.stack[ocb][icb] : E.bulge[aa] + tauo + z.tau(Ck1, Ri2);
         if (Bv < INV_THR) best = imin(best, cl + e + pc.sr);
       }
     } else if (ap == 30) {
@@ -682,13 +668,14 @@ struct GZu {
     } else {
       // 2x3 and 3x2 loops
       const int B23 = sm.u.w.B[r + 4][c + 27], B32 = sm.u.w.B[r + 5][c + 28];
+      const int out23 = z.P->il5 + z.P->ninio0 + E.mm23I[ocb][Ri2][Cj3] + pc.sr;
       if (B23 < INV_THR) {
         const int l = i + 4, k = j - 5, icb = bpc(z.C(k - 1), z.R(l)), Ck = z.C(k), Rl1 = z.R(l - 1);
-        best = imin(best, B23 - E.mmI[icb][Ck][Rl1] + E.mm23I[icb][Ck][Rl1] + pc.out23);
+        best = imin(best, B23 - E.mmI[icb][Ck][Rl1] + E.mm23I[icb][Ck][Rl1] + out23);
       }
       if (B32 < INV_THR) {
         const int l = i + 5, k = j - 4, icb = bpc(z.C(k - 1), z.R(l)), Ck = z.C(k), Rl1 = z.R(l - 1);
-        best = imin(best, B32 - E.mmI[icb][Ck][Rl1] + E.mm23I[icb][Ck][Rl1] + pc.out23);
+        best = imin(best, B32 - E.mmI[icb][Ck][Rl1] + E.mm23I[icb][Ck][Rl1] + out23);
       }
     }
     for (int m = 16; m >= 1; m >>= 1) best = imin(best, __shfl_xor(best, m));
@@ -700,17 +687,6 @@ struct GZu {
     const uint8_t* seq = a.codes + in.seq_off;
     const DevRna* P = (const DevRna*)a.rna;
     const ZuView z{sm.E, P, a.hl_ent, sm.rs, sm.cs, IT - 1, JT - 34, n};
-    // interior-loop lane of this thread: left size a = ap + 1 (ap < 30); its
-    // row of generic loop energies F[a][2..30] in registers, the special
-    // positions (all of a == 1, (2,2), (2,3), (3,2)) masked with kMask
-    const int ap = tid & 31, aa = ap + 1;
-    int fg[29];
-#pragma unroll
-    for (int b = 2; b <= 30; ++b) {
-      const bool special = ap >= 30 || aa == 1 || (aa == 2 && b <= 3) || (aa == 3 && b == 2);
-      fg[b - 2] = special ? kMask : sm.E.F[ap < 30 ? aa : 1][b];
-    }
-    const int f_a1 = sm.E.F[ap < 30 ? aa : 1][1], f_1a = sm.E.F[1][ap < 30 ? aa : 1];
     int acc[4][4];
     for (int t = 0; t < 4; ++t)
       for (int u = 0; u < 4; ++u) acc[t][u] = INV;
@@ -773,10 +749,21 @@ struct GZu {
     }
     const int dlo = I == J ? 0 : -(T - 1);
     __syncthreads();  // sequence segments visible to prep_band
-    if (tid >= 64) prep_band(z, IT, JT, dlo, sm, tid - 64);
+    if (tid >= 64) prep_band(z, IT, JT, dlo, sm, tid - 64, 0);
     __syncthreads();
     int32_t* gcl = rec_ptr(a, in, NT_ZU, I, J) + ZT_CL * T * T;
-    for (int ld0 = dlo; ld0 <= T - 1; ld0 += G) {
+    // interior-loop lane of this thread: left size a = ap + 1 (ap < 30); its
+    // row of generic loop energies F[a][2..30] in registers, the special
+    // positions (all of a == 1, (2,2), (2,3), (3,2)) masked with kMask
+    const int ap = tid & 31, aa = ap + 1;
+    int fg[29];
+#pragma unroll
+    for (int b = 2; b <= 30; ++b) {
+      const bool special = ap >= 30 || aa == 1 || (aa == 2 && b <= 3) || (aa == 3 && b == 2);
+      fg[b - 2] = special ? kMask : P->F[ap < 30 ? aa : 1][b];
+    }
+    const int f_a1 = P->F[ap < 30 ? aa : 1][1], f_1a = P->F[1][ap < 30 ? aa : 1];
+    for (int ld0 = dlo, buf = 0; ld0 <= T - 1; ld0 += G, buf ^= 1) {
       // ---------------- heavy phase: band [ld0, ld0+G) ----------------
       // Lanes [0, wl): 32 per listed cell (its interior loops and bulges),
       // padded to whole waves; lanes [wl, wl + 2*ncell): two per cell of the
@@ -794,7 +781,7 @@ struct GZu {
           if (v >= nv) continue;
           const int qp = sm.list[v], q = qp / T, p = qp % T, ld = ld0 + q;
           const int r = (ld < 0 ? -ld : 0) + p, c = r + ld, i = IT + r, j = JT + c;
-          const int best = window_lane(z, sm, sm.pre[q][p], ap, i, j, r, c, fg, f_a1, f_1a);
+          const int best = window_lane(z, sm, sm.pre[buf][q][p], ap, i, j, r, c, fg, f_a1, f_1a);
           if (ap == 0) sm.acc[q][p] = best;
         } else {
           const int e = (w - wl) >> 1, part = w & 1;
@@ -826,13 +813,19 @@ struct GZu {
       }
       __syncthreads();
       // ---------------- light steps: finalize the band diagonal by diagonal ----------------
-      for (int q = 0; q < qn; ++q) {
-        const int ld = ld0 + q, rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
-        if (tid < ncell) {
+      // Wave 0 alone walks the band's diagonals (ordered by its own LDS
+      // writes, no workgroup barrier); waves 1-3 meanwhile compute the
+      // sequence terms and cell list of the next band into the other buffer.
+      if (tid >= 64) {
+        if (ld0 + G <= T - 1) prep_band(z, IT, JT, ld0 + G, sm, tid - 64, buf ^ 1);
+      } else {
+        for (int q = 0; q < qn; ++q) {
+          const int ld = ld0 + q, rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
+          if (tid < ncell) {
           const int p = tid, r = rmin + p, c = r + ld, i = IT + r, j = JT + c, d = j - i;
           int cl = INV, U = INV, ml = INV, ml1 = INV, R11 = INV, Bv = INV;
           if (i <= j && j <= n) {
-            const ZuPre& pc = sm.pre[q][p];
+            const ZuPre& pc = sm.pre[buf][q][p];
             if (pc.valid) {
               cl = imin(sm.acc[q][p], pc.r1);
               if (d >= 9) {
@@ -867,11 +860,10 @@ struct GZu {
           sm.u.w.R11[r][c + 1] = R11;
           gcl[r * T + c] = cl;
           sm.acc[q][p] = INV;
+          }
+          wave_lds_sync();
         }
-        __syncthreads();
       }
-      // terms of the next band (reads only the sequence: safe to overwrite now)
-      if (ld0 + G <= T - 1 && tid >= 64) prep_band(z, IT, JT, ld0 + G, sm, tid - 64);
       __syncthreads();
     }
     int32_t* g = rec_ptr(a, in, NT_ZU, I, J);
