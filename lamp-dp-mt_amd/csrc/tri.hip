@@ -413,13 +413,14 @@ struct GMc {
 //   ml1(i,j) = min{ ml1(i+1,j) [j-i>=8]; X; U(i,j); R11(i,j) }
 //   R11(i,j) = min_{i+7<=k<j} U(i,k) = min{R11(i,j-1), U(i,j-1)}     (rule 11)
 // INV marks "no valid candidate", i.e. the reference's 999999.
-// Energy tables read by the Zuker fill, staged once per workgroup in LDS
-// (int16, the width of the reference's paramT, vienna.h:24-67).
+// Energy tables read by the loop phase of the Zuker fill, staged once per
+// workgroup in LDS (int16, the width of the reference's paramT,
+// vienna.h:24-67); the tables used only by the sequence terms are read from
+// the device copy of the parameters.
 struct ZuTab {
   int16_t stack[8][8];
-  int16_t mmH[8][5][5], mmI[8][5][5], mm1nI[8][5][5], mm23I[8][5][5], mmM[8][5][5], mmExt[8][5][5];
-  int16_t d5[8][5], d3[8][5];
-  int16_t bulge[32], hairpin[32];
+  int16_t mmI[8][5][5], mm1nI[8][5][5], mm23I[8][5][5];
+  int16_t bulge[32];
 };
 
 // Per-cell terms that depend on the sequence only, precomputed one wavefront
@@ -433,6 +434,9 @@ struct ZuPre {
   int16_t boff;   // B - cl = interior mismatch of (i,j) as an inner pair (kNoB if unused)
   int16_t outI;   // generic interior-loop outer mismatch + sr
   int16_t in0;    // interior mismatch of (i+1,j-1): recovers cl(i+1,j-1) from B (rule 0)
+  // 1x1, 1x2, 2x1, 2x2 loops: il_energy (int11/int21/int22) minus the inner
+  // pair's interior mismatch, so that the candidate is B(l,k) + f + sr
+  int16_t f11, f12, f21, f22;
 };
 constexpr int kNoB = -32768;
 
@@ -464,15 +468,15 @@ struct ZuView {
       for (int e = 0; e < P->n_loop[cls]; ++e)
         if (P->loop_key[cls][e] == key) return P->loop_e[cls][e];
     }
-    const int ent = size <= 30 ? E.hairpin[size] : hl[size];
+    const int ent = size <= 30 ? P->hairpin[size] : hl[size];
     if (size == 3) return ent + tau(R(a), C(b));
-    return ent + E.mmH[bpc(R(a), C(b))][R(a + 1)][C(b - 1)];
+    return ent + P->mmH[bpc(R(a), C(b))][R(a + 1)][C(b - 1)];
   }
   __device__ int ext(int i, int j) const {  // ext_mismatch_energy(i, j) with j = cell column - 1
     const int bb = bpc(R(i), C(j));
-    if (i > 0 && j + 1 < n) return E.mmExt[bb][R(i - 1)][C(j + 1)];
-    if (i > 0) return E.d5[bb][R(i - 1)];
-    if (j + 1 < n) return E.d3[bb][C(j + 1)];
+    if (i > 0 && j + 1 < n) return P->mmExt[bb][R(i - 1)][C(j + 1)];
+    if (i > 0) return P->d5[bb][R(i - 1)];
+    if (j + 1 < n) return P->d3[bb][C(j + 1)];
     return 0;
   }
   // il_energy(i+1, l, k-1, j-2) (librna_impl.h:230-265), sizes sl = l-i-2, sr = j-k-2 >= 1
@@ -496,17 +500,6 @@ struct ZuView {
     }
     return P->F[sl][srr] + E.mmI[ocb][ol][orr] + E.mmI[icb][il_][ir];
   }
-  // bl_energy / br_energy terms of rules 2 and 3 (librna_impl.h:199-215)
-  __device__ int bulge_l(int i, int j, int k) const {  // inner cl(k, j-2), size k-i-2
-    const int size = k - i - 2;
-    if (size == 1) return E.bulge[1] + E.stack[bpc(R(i + 1), C(j - 2))][bpc(C(j - 3), R(k))];
-    return E.bulge[size] + tau(R(i + 1), C(j - 2)) + tau(C(j - 3), R(k));
-  }
-  __device__ int bulge_r(int i, int j, int k) const {  // inner cl(i+2, k), size j-k-2
-    const int size = j - k - 2;
-    if (size == 1) return E.bulge[1] + E.stack[bpc(R(i + 1), C(j - 2))][bpc(C(k - 1), R(i + 2))];
-    return E.bulge[size] + tau(R(i + 1), C(j - 2)) + tau(C(k - 1), R(i + 2));
-  }
   __device__ void pre(int i, int j, ZuPre* o) const {
     const int d = j - i;
     o->valid = 0;
@@ -515,12 +508,18 @@ struct ZuView {
     const int s_ = sr(i, j);
     o->sr = s_;
     o->r1 = hairpin(i, j) + s_;
-    o->r5c = P->ml_closing + P->ml_intern + tau(R(i + 1), C(j - 2)) + E.mmM[bpc(C(j - 2), R(i + 1))][C(j - 3)][R(i + 2)] + s_;
+    o->r5c = P->ml_closing + P->ml_intern + tau(R(i + 1), C(j - 2)) + P->mmM[bpc(C(j - 2), R(i + 1))][C(j - 3)][R(i + 2)] + s_;
     o->uoff = ext(i, j - 1) + tau(R(i), C(j - 1)) + P->ml_intern;
     o->boff = (i >= 1 && j <= n - 1) ? inner(i, j) : kNoB;
     const int ocb = bpc(R(i + 1), C(j - 2)), ol = R(i + 2), orr = C(j - 3);
     o->outI = E.mmI[ocb][ol][orr] + s_;
     o->in0 = d >= 9 ? inner(i + 1, j - 1) : 0;
+    if (d >= 12) {  // the smallest cell with interior loops (l = i+3, k = j-3, k-l >= 7 needs d >= 13)
+      o->f11 = (int16_t)(il(i, j, i + 3, j - 3) - inner(i + 3, j - 3));
+      o->f12 = (int16_t)(il(i, j, i + 3, j - 4) - inner(i + 3, j - 4));
+      o->f21 = (int16_t)(il(i, j, i + 4, j - 3) - inner(i + 4, j - 3));
+      o->f22 = (int16_t)(il(i, j, i + 4, j - 4) - inner(i + 4, j - 4));
+    }
   }
 };
 
@@ -559,25 +558,19 @@ struct GZu {
     ZuTab& E = sm.E;
     for (int e = threadIdx.x; e < 64; e += NTHR) (&E.stack[0][0])[e] = (int16_t)(&P->stack[0][0])[e];
     for (int e = threadIdx.x; e < 200; e += NTHR) {
-      (&E.mmH[0][0][0])[e] = (int16_t)(&P->mmH[0][0][0])[e];
       (&E.mmI[0][0][0])[e] = (int16_t)(&P->mmI[0][0][0])[e];
       (&E.mm1nI[0][0][0])[e] = (int16_t)(&P->mm1nI[0][0][0])[e];
       (&E.mm23I[0][0][0])[e] = (int16_t)(&P->mm23I[0][0][0])[e];
-      (&E.mmM[0][0][0])[e] = (int16_t)(&P->mmM[0][0][0])[e];
-      (&E.mmExt[0][0][0])[e] = (int16_t)(&P->mmExt[0][0][0])[e];
     }
-    for (int e = threadIdx.x; e < 40; e += NTHR) {
-      (&E.d5[0][0])[e] = (int16_t)(&P->d5[0][0])[e];
-      (&E.d3[0][0])[e] = (int16_t)(&P->d3[0][0])[e];
-    }
-    for (int e = threadIdx.x; e < 31; e += NTHR) { E.bulge[e] = (int16_t)P->bulge[e]; E.hairpin[e] = (int16_t)P->hairpin[e]; }
+    for (int e = threadIdx.x; e < 31; e += NTHR) E.bulge[e] = (int16_t)P->bulge[e];
   }
 
   // sequence-only terms of the cells of band [ld0, ld0+G) and the compacted
   // list of cells whose cl can close bulges / interior loops; all threads of
   // `nthr` starting at thread index `t0` take part, wave `lw` builds the list
   __device__ static void prep_band(const ZuView& z, int IT, int JT, int ld0, Smem& sm, int t, int buf) {
-    // t in [0, 192): t < 64 -> list (one wave), else per-cell terms
+    // t in [0, 192): the first wave builds the cell list, then all 192
+    // threads compute the per-cell terms of the band's G*T <= 160 cells
     if (t < 64) {
       int base = 0;
       for (int q = 0; q < G; ++q) {
@@ -595,12 +588,12 @@ struct GZu {
         base += __popcll(m);
       }
       if (t == 0) sm.nlist = base;
-    } else {
-      for (int e = t - 64; e < G * T; e += 128) {
-        const int q = e / T, p = e % T, ld = ld0 + q;
-        if (ld > T - 1) continue;
-        const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
-        if (p >= ncell) continue;
+    }
+    const int e = t < 64 ? 128 + t : t - 64;
+    if (e < G * T) {
+      const int q = e / T, p = e % T, ld = ld0 + q;
+      const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
+      if (ld <= T - 1 && p < ncell) {
         const int r = rmin + p;
         z.pre(IT + r, JT + r + ld, &sm.pre[buf][q][p]);
       }
@@ -636,14 +629,14 @@ struct GZu {
         // (aa, 1): inner pair (i+2+aa, j-3); 2x1 from int21, else 1xn
         const int Bv = Brow[-1], icb = bpc(z.C(j - 4), Rl);
         const int cl = Bv - E.mmI[icb][Cj3][Rl1];
-        const int e = aa == 2 ? z.il(i, j, i + 4, j - 3) + pc.sr : f_a1 + E.mm1nI[icb][Cj3][Rl1] + out1n;
-        if (Bv < INV_THR) best = imin(best, cl + e);
+        const int cand = aa == 2 ? Bv + pc.f21 + pc.sr : cl + f_a1 + E.mm1nI[icb][Cj3][Rl1] + out1n;
+        if (Bv < INV_THR) best = imin(best, cand);
         // (1, aa): inner pair (i+3, j-2-aa); 1x2 from int21, else 1xn
         const int k = j - 2 - aa, Bw = sm.u.w.B[r + 3][c + 30 - aa];
         const int Ck = z.C(k), icb2 = bpc(z.C(k - 1), z.R(i + 3));
         const int cl2 = Bw - E.mmI[icb2][Ck][Ri2];
-        const int e2 = aa == 2 ? z.il(i, j, i + 3, k) + pc.sr : f_1a + E.mm1nI[icb2][Ck][Ri2] + out1n;
-        if (Bw < INV_THR) best = imin(best, cl2 + e2);
+        const int cand2 = aa == 2 ? Bw + pc.f12 + pc.sr : cl2 + f_1a + E.mm1nI[icb2][Ck][Ri2] + out1n;
+        if (Bw < INV_THR) best = imin(best, cand2);
       }
       {
         // left bulge of size aa: inner pair (i+2+aa, j-2)
@@ -663,8 +656,8 @@ struct GZu {
     } else if (ap == 30) {
       // 1x1 and 2x2 loops (int11, int22)
       const int B11 = sm.u.w.B[r + 3][c + 29], B22 = sm.u.w.B[r + 4][c + 28];
-      if (B11 < INV_THR) best = imin(best, B11 - z.inner(i + 3, j - 3) + z.il(i, j, i + 3, j - 3) + pc.sr);
-      if (B22 < INV_THR) best = imin(best, B22 - z.inner(i + 4, j - 4) + z.il(i, j, i + 4, j - 4) + pc.sr);
+      if (B11 < INV_THR) best = imin(best, B11 + pc.f11 + pc.sr);
+      if (B22 < INV_THR) best = imin(best, B22 + pc.f22 + pc.sr);
     } else {
       // 2x3 and 3x2 loops
       const int B23 = sm.u.w.B[r + 4][c + 27], B32 = sm.u.w.B[r + 5][c + 28];
