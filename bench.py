@@ -17,9 +17,12 @@ requests solved alone.
 value = cell updates per second of the batched step, with inputs resident in
 HBM: the device time of the fill and finalize kernels (HIP events on the
 library's stream).  Also reported: the same sequences as plain per-instance
-launches (a fixed 1-in-16 sample), the end-to-end rate including PCIe copies,
-the roofline of the fill kernel and a single-thread CPU baseline (the oracle,
-a bounded sample of the same sequences).
+launches (a fixed 1-in-16 sample), the end-to-end rate of the C call
+including PCIe copies, the roofline of the fill kernel and a single-thread
+CPU baseline (the oracle, a bounded sample of the same sequences).
+
+--workload selects one of the single-instance configurations instead
+(zuker2048, nussinov1024, swat16384); each prints its own JSON line.
 """
 import argparse
 import json
@@ -41,11 +44,34 @@ def cells(n):
     return (n + 1) * (n + 2) // 2
 
 
-def x_candidates(n):
-    """Split points of the ml/ml1 min-plus term, k in [i+7, j-7]: sum over
-    diagonals d=j-i of (n-d+1)*(d-13)."""
-    m = n - 13
+def _tet(m):
     return m * (m + 1) * (m + 2) // 6 if m > 0 else 0
+
+
+def x_candidates(n):
+    """Split points k in [i+7, j-7] of one ml/ml1 min-plus term over all cells:
+    sum over diagonals d=j-i of (n-d+1)*(d-13)."""
+    return _tet(n - 13)
+
+
+def ml_candidates(n):
+    """Split candidates of the reference's ml/ml1 k-loops over all cells
+    (zu_kern.h:92-184): rules 7 and 9, k in [i+7, j-7], and rule 11,
+    k in [i+7, j-1]."""
+    return 2 * _tet(n - 13) + _tet(n - 7)
+
+
+def load_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (scaled by the copy-kernel calibration), from the committed summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1]), "profiles/" + name
+    except OSError:
+        pass
+    return None, None
 
 
 def pct(xs, p):
@@ -167,9 +193,63 @@ def serve(ctx, rate, seconds, budget, slots, max_batch, check_every):
     }
 
 
+def single(args):
+    """One instance of a single-GPU configuration, timed over --steps runs."""
+    import dpx
+    from dpx import gen
+    import oracle_lib as o
+    ctx = dpx.Context()
+    line = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "int32", "data": "synthetic", "unit": "GCUPS"}
+    if args.workload == "swat16384":
+        a, b = gen.dna_pair(16384, 16384, 1)
+        run = lambda: ctx.swat(a, b, 2, -3, -5, -2, trace_from=1)
+        n_cells = 16385 * 16385
+        line["config"] = {"workload": "Smith-Waterman/Gotoh 16384 x 16384, i.i.d. ACGT seed 1, +2/-3/-5/-2, "
+                                      "best local score + traceback from the best cell"}
+        check = lambda r: (r[0], r[1], [t[:3] for t in r[2]]) == (lambda x: (x[0], x[1], [t[:3] for t in x[2]]))(
+            o.swat(a, b, 2, -3, -5, -2, from_best=True))
+    elif args.workload == "nussinov1024":
+        s = gen.rna_uniform(1024, 0)
+        run = lambda: ctx.nussinov(s, min_loop=3)
+        n_cells = cells(1024)
+        line["config"] = {"workload": "Nussinov n=1024, i.i.d. uniform ACGU seed 0, min hairpin loop 3, score + backtrace"}
+        check = lambda r: (r[0], [t[:4] for t in r[1]]) == (lambda x: (x[0], [t[:4] for t in x[1]]))(o.nussinov(s, min_loop=3))
+    else:
+        s = gen.rna_gc(2048, 2)
+        run = lambda: ctx.zuker(s)
+        n_cells = cells(2048)
+        line["config"] = {"workload": "Zuker MFE n=2048, i.i.d. 50% GC seed 2, Turner 2004, MFE + backtrace"}
+        check = None  # O(n^3) oracle run is minutes; the GPU tests pin n=2048 against the reference fixture
+    for _ in range(args.warmup):
+        r = run()
+    fills, fins = [], []
+    for _ in range(args.steps):
+        r = run()
+        f, z = ctx.last_timing()
+        fills.append(f)
+        fins.append(z)
+        print("bench: step fill=%.3fms finalize=%.3fms" % (f, z), file=sys.stderr, flush=True)
+    ms = (sum(fills) + sum(fins)) / args.steps
+    line.update({"metric": "GCUPS (%s)" % args.workload, "value": round(n_cells / (ms * 1e-3) / 1e9, 4),
+                 "ms_per_step": round(ms, 4), "fill_ms": round(sum(fills) / args.steps, 4),
+                 "finalize_ms": round(sum(fins) / args.steps, 4), "cells": n_cells,
+                 "bitexact": check(r) if (check and not args.no_cpu) else None})
+    if args.workload == "swat16384":
+        fill_s = sum(fills) / args.steps * 1e-3
+        # traceback store: 4 bits per cell written once; score rows: 8 bytes per cell of every 64th row
+        line["roofline"] = {"bound": "valu", "achieved": round(n_cells * 10 / fill_s / 1e12, 4),
+                            "peak": round(PEAK_INT32_TOPS, 2), "unit": "TOP/s",
+                            "frac": round(n_cells * 10 / fill_s / 1e12 / PEAK_INT32_TOPS, 5),
+                            "ops": "about 10 int32 ops per cell", "traffic": None,
+                            "hbm_frac": round(n_cells * 0.5 / fill_s / 8e12, 5)}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="batch", choices=["batch", "zuker2048", "nussinov1024", "swat16384"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)
@@ -191,6 +271,10 @@ def main():
     import dpx
     from dpx import gen
 
+    if args.workload != "batch":
+        single(args)
+        return
+
     seqs = gen.rna_batch(args.count, 50, 2000, 3, gc=0.5)
     total_cells = sum(cells(len(s)) for s in seqs)
     total_x = sum(x_candidates(len(s)) for s in seqs)
@@ -202,9 +286,8 @@ def main():
     fills, fins, walls = [], [], []
     results = traces = None
     for _ in range(args.steps):
-        t0 = time.perf_counter()
         results, traces = ctx.zuker_batch(seqs)
-        walls.append(time.perf_counter() - t0)
+        walls.append(ctx.last_call_s)
         f, z = ctx.last_timing()
         fills.append(f)
         fins.append(z)
@@ -231,10 +314,18 @@ def main():
     exact = oracle_check.zuker_sample(seqs, results, traces, k=8)
 
     fill_ms = sum(fills) / args.steps
-    achieved = 2.0 * total_x / (fill_ms * 1e-3) / 1e12
+    # algorithmic work of the reference's formulation: 2 int32 ops (add, min)
+    # per ml/ml1 split candidate (rules 7, 9, 11); the loop-energy candidates
+    # of cl are not counted, so this is a lower bound of the work per cell
+    ops = 2.0 * sum(ml_candidates(len(s)) for s in seqs)
+    achieved = ops / (fill_ms * 1e-3) / 1e12
+    traffic, tsrc = load_traffic("tri_fill_zuker_batch4096_hbm.txt")
     roofline = {"bound": "valu", "achieved": round(achieved, 4), "peak": round(PEAK_INT32_TOPS, 2), "unit": "TOP/s",
-                "frac": round(achieved / PEAK_INT32_TOPS, 5), "traffic": None,
-                "kernel": "tri_fill<GZu>", "ops": "2 int32 ops (add, min) per ml/ml1 split candidate"}
+                "frac": round(achieved / PEAK_INT32_TOPS, 5), "traffic": traffic, "traffic_source": tsrc,
+                "kernel": "tri_fill<GZu>",
+                "ops": "2 int32 ops (add, min) per split candidate of the reference's ml/ml1 loops (rules 7, 9, 11): "
+                       "%.4e ops per launch" % ops,
+                "executed_ops_frac": round(2.0 * total_x / (fill_ms * 1e-3) / 1e12 / PEAK_INT32_TOPS, 5)}
 
     served = None
     if not args.no_serve:
@@ -255,7 +346,7 @@ def main():
                    "instances": len(seqs), "cells": total_cells, "parallelism": "single GPU, persistent tile queue"},
         "plain_per_instance_gcups": round(plain, 4) if plain else None,
         "speedup_vs_plain": round(value / plain, 2) if plain else None,
-        "wall_ms_per_step_pcie": round(wall_ms, 3), "fill_ms": round(fill_ms, 3),
+        "c_call_ms_per_step_pcie": round(wall_ms, 3), "fill_ms": round(fill_ms, 3),
         "finalize_ms": round(sum(fins) / args.steps, 3), "bitexact_sample": exact,
         "roofline": roofline, "cpu_baseline": cpu, "served": served,
     }

@@ -33,7 +33,7 @@ using namespace dpx;
 
 namespace {
 
-constexpr int kMaxZuker = 15999;      // finalize keeps the st column (n+1 int32) in 64 KiB of LDS
+constexpr int kMaxZuker = 14000;      // finalize keeps the st column (n+1 int32) and a 32x33 block in 64 KiB of LDS
 constexpr int kMaxTri = 30000;        // Nussinov / matrix chain
 constexpr int kHairpinTable = 32768;  // hairpin entropy entries kept on the device
 

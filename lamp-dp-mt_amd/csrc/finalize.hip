@@ -252,24 +252,51 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
   const int n = in.n;
   const ZuF z{a.codes + in.seq_off, (const DevRna*)a.rna, a.hl_ent, n};
   const int mlint = z.P->ml_intern;
-  __shared__ int red;
   __shared__ Bfs b;
-  // ---- st column (zu_kern.h:166-182): rule 12 st(i+1), rule 13 D(i,k)+st(k), rule 14 i==n
+  // ---- st column (zu_kern.h:166-182): rule 12 st(i+1), rule 13 D(i,k)+st(k), rule 14 i==n.
+  // Rows are taken in blocks of 32 from the bottom.  For a block [lo, hi] the
+  // split points k > hi (st(k) already known) are reduced by all threads at
+  // once; the split points inside the block are then walked row by row by
+  // one wave from D values staged in LDS.
+  __shared__ int far[T], nd[T][T + 1];
   if (threadIdx.x == 0) st[n] = 0;
-  for (int i = n - 1; i >= 0; --i) {
-    if (threadIdx.x == 0) red = st[i + 1];
-    __syncthreads();
-    int m = INV;
-    for (int k = i + 7 + (int)threadIdx.x; k <= n; k += NTHR) {
-      const int U = C.get(ZT_U, i, k);
-      if (U < INV_THR) m = fmin_(m, U - mlint + st[k]);
+  const int tid = threadIdx.x, lane = tid & 31;
+  for (int hi = n - 1; hi >= 0; hi -= T) {
+    const int lo = fmax_(0, hi - T + 1);
+    for (int pass = 0; pass < T / 8; ++pass) {
+      const int r = pass * 8 + (tid >> 5), i = lo + r;
+      int m = INV;
+      if (i <= hi)
+        for (int k = fmax_(hi + 1, i + 7) + lane; k <= n; k += 32) {
+          const int U = C.get(ZT_U, i, k);
+          if (U < INV_THR) m = fmin_(m, U - mlint + st[k]);
+        }
+      for (int off = 16; off >= 1; off >>= 1) m = fmin_(m, __shfl_xor(m, off));
+      if (lane == 0) far[r] = m;
     }
-    for (int off = 32; off >= 1; off >>= 1) m = fmin_(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0 && m < red) atomicMin(&red, m);
+    for (int e = tid; e < T * T; e += NTHR) {
+      const int r = e / T, c = e % T, i = lo + r, k = lo + c;
+      int v = INV;
+      if (i <= hi && k <= hi && k >= i + 7) {
+        const int U = C.get(ZT_U, i, k);
+        if (U < INV_THR) v = U - mlint;
+      }
+      nd[r][c] = v;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) st[i] = red;
+    if (tid < 64) {
+      for (int i = hi; i >= lo; --i) {
+        const int r = i - lo, k = lo + lane;
+        int m = INV;
+        if (tid < 32 && k >= i + 7 && k <= hi && nd[r][lane] < INV_THR) m = nd[r][lane] + st[k];
+        for (int off = 16; off >= 1; off >>= 1) m = fmin_(m, __shfl_xor(m, off));
+        if (tid == 0) st[i] = fmin_(fmin_(st[i + 1], far[r]), m);
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): st[i] visible to the wave's next row
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
   if (threadIdx.x == 0) a.results[in.out] = st[0];
   if (!a.want_trace) return;
   TraceRec* tr = (TraceRec*)a.traces + in.trace_off;

@@ -34,7 +34,7 @@ __device__ __forceinline__ int sw_shr1(int lane0_val, int v) {
   return __builtin_amdgcn_update_dpp(lane0_val, v, 0x138, 0xf, 0xf, false);
 }
 
-constexpr int SW_CH = 64;  // publish granularity (columns)
+constexpr int SW_CH = 32;  // publish granularity (columns); a band lags the one above by 64 + SW_CH steps
 
 __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
   const int lane = threadIdx.x;
@@ -53,78 +53,103 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     uint32_t* btb = A.bt + (size_t)band * A.nwords * 64;
     int Mc = 0, G1c = 0, G2c = 0, upPrev = 0;
     int best = -1, bestj = 0;
-    uint32_t word = 0;
     int known = 0;  // columns of `above` known to be published
-    int chM = 0, chG = 0;
-    const int tend = A.n2 + 64;
-    for (int t = 0; t < tend; ++t) {
-      const int tc = t & (SW_CH - 1);
-      if (tc == 0) {
-        // make the next chunk of the row above available, then load it
-        const int need = min(t + SW_CH, W);
-        if (known < need) {
-          int k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&A.prog[band], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-          for (int spin = 0; k < need; ++spin) {
-            if (spin > (1 << 24)) {  // bounded wait: flag the error, stop the band
-              if (lane == 0) A.ctrl[1] = 1;
-              k = need;
-              break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&A.prog[band], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    int chM = 0, chG = 0, chB = 0;  // chunk of the row above and of b, one column per lane
+    int bj = 0;  // b[j-1] of this lane's column j = t - lane, shifted in from lane 0
+    int outv = 0;  // M(n1, n2) when this lane's row is n1
+    const int tend = (A.n2 + 64 + 7) & ~7;  // steps of the band, rounded up to whole code words
+    for (int t0 = 0; t0 < tend; t0 += SW_CH) {
+      // make the next chunk of the row above available, then load it; the
+      // steps of the chunk below issue no loads, so no step waits on memory
+      const int need = min(t0 + SW_CH, W);
+      if (known < need) {
+        int k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&A.prog[band], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        for (int spin = 0; k < need; ++spin) {
+          if (spin > (1 << 24)) {  // bounded wait: flag the error, stop the band
+            if (lane == 0) A.ctrl[1] = 1;
+            k = need;
+            break;
           }
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-          known = k;
+          __builtin_amdgcn_s_sleep(1);
+          k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&A.prog[band], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         }
-        const int col = t + lane;
-        if (col < W) { const int2 v = above[col]; chM = v.x; chG = v.y; }
+        known = k;
       }
-      const int hM = __shfl(chM, tc), hG = __shfl(chG, tc);
-      const int upM = sw_shr1(hM, Mc);
-      const int upG = sw_shr1(hG, G1c);
-      const int j = t - lane;
-      const bool act = row_ok && j >= 0 && j <= A.n2;
-      // g2
-      int g2, r2;
-      if (j == 0) { g2 = 0; r2 = 0; }
-      else {
-        const int o = max(0, Mc + A.open), e = max(0, G2c + A.ext);
-        r2 = e > o; g2 = r2 ? e : o;
+      // The handoff row is written and read with device-coherent (agent
+      // scope) accesses, so neither side needs a cache writeback/invalidate:
+      // the producer waits for its row stores before it raises the counter.
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      const int col = t0 + lane;
+      if (col < W) {
+        const unsigned long long v = __hip_atomic_load((const unsigned long long*)(above + col), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+        chM = (int)(uint32_t)v;
+        chG = (int)(uint32_t)(v >> 32);
       }
-      // g1
-      const int o1 = max(0, upM + A.open), e1 = max(0, upG + A.ext);
-      const int r1 = e1 > o1;
-      const int g1 = r1 ? e1 : o1;
-      // M
-      int M, rm;
-      if (j == 0) { M = g1; rm = 2; }
-      else {
-        const int bj = A.b[j - 1 < A.n2 ? (j - 1 >= 0 ? j - 1 : 0) : 0];
-        M = upPrev + (ai == bj ? A.match : A.mismatch); rm = 1;
-        if (g1 > M) { M = g1; rm = 2; }
+      // lane l holds b[t0 - 1 + l]: the symbol lane 0 needs at step t0 + l
+      const int bc = t0 - 1 + lane;
+      chB = (bc >= 0 && bc < A.n2) ? A.b[bc] : 0;
+      const int tn = min(SW_CH, tend - t0);  // a multiple of 8
+      int sM = 0, sG = 0;  // lane l keeps (M, g1) of the band's last row at step t0 + l
+      for (int tc = 0; tc < tn; tc += 8) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int t = t0 + tc + u;
+          // chunk values of this step, broadcast from lane tc+u (wave-uniform index)
+          const int hM = __builtin_amdgcn_readlane(chM, tc + u), hG = __builtin_amdgcn_readlane(chG, tc + u);
+          bj = sw_shr1(__builtin_amdgcn_readlane(chB, tc + u), bj);
+          const int upM = sw_shr1(hM, Mc);
+          const int upG = sw_shr1(hG, G1c);
+          const int j = t - lane;
+          // g2 (j == 0: rule 7)
+          const int o = max(0, Mc + A.open), e = max(0, G2c + A.ext);
+          const int r2 = j != 0 && e > o;
+          const int g2 = j == 0 ? 0 : (e > o ? e : o);
+          // g1
+          const int o1 = max(0, upM + A.open), e1 = max(0, upG + A.ext);
+          const int r1 = e1 > o1;
+          const int g1 = r1 ? e1 : o1;
+          // M (j == 0: g1, rule 2)
+          const int dg = upPrev + (ai == bj ? A.match : A.mismatch);
+          int M = j == 0 ? g1 : dg, rm = j == 0 ? 2 : 1;
+          if (j != 0 && g1 > M) { M = g1; rm = 2; }
+          if (g2 > M) { M = g2; rm = 3; }
+          upPrev = upM;
+          // The state of lanes outside the matrix (j < 0, j > n2, rows past n1)
+          // is never read by a valid cell: column 0 uses neither M(i,j-1) nor
+          // g2(i,j-1), and invalid rows only feed rows below them.
+          Mc = M;
+          G1c = g1;
+          G2c = g2;
+          const bool act = row_ok && j >= 0 && j <= A.n2;
+          if (act && M > best) { best = M; bestj = j; }
+          const int lastM = __builtin_amdgcn_readlane(M, 63), lastG = __builtin_amdgcn_readlane(g1, 63);
+          if (lane == tc + u) { sM = lastM; sG = lastG; }
+          if (act && i == A.n1 && j == A.n2) outv = M;
+          word |= (uint32_t)(rm | (r1 << 2) | (r2 << 3)) << (4 * u);
+        }
+        btb[(size_t)((t0 + tc) >> 3) * 64 + lane] = word;
       }
-      if (g2 > M) { M = g2; rm = 3; }
-      upPrev = upM;
-      if (act) {
-        Mc = M; G1c = g1; G2c = g2;
-        if (M > best) { best = M; bestj = j; }
-        if (i == A.n1 && j == A.n2) A.out[0] = M;
-        if (lane == 63) below[j] = make_int2(M, g1);
-      }
-      word |= (uint32_t)(rm | (r1 << 2) | (r2 << 3)) << (4 * (t & 7));
-      if ((t & 7) == 7 || t == tend - 1) {
-        btb[(size_t)(t >> 3) * 64 + lane] = word;
-        word = 0;
+      // the chunk's cells of the band's last row, one column per lane
+      {
+        const int jl = t0 + lane - 63;
+        if (lane < tn && jl >= 0 && jl <= A.n2)
+          __hip_atomic_store((unsigned long long*)(below + jl),
+                             (unsigned long long)(uint32_t)sM | ((unsigned long long)(uint32_t)sG << 32),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       // publish the finished columns of this band's last row
-      if (tc == SW_CH - 1 || t == tend - 1) {
-        const int done = min(t - 63 + 1, W);  // columns 0..t-63 of row 64*band+64
-        if (done > 0) {
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-          if (lane == 63) __hip_atomic_store(&A.prog[band + 1], done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+      const int t = t0 + tn - 1;
+      const int done = min(t - 63 + 1, W);  // columns 0..t-63 of row 64*band+64
+      if (done > 0) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this band's row stores are complete
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (lane == 63) __hip_atomic_store(&A.prog[band + 1], done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
+    if (row_ok && i == A.n1) A.out[0] = outv;
     // best cell of this band: largest M, then smallest (i, j)
     if (row_ok && best >= 0) {
       const unsigned long long key = ((unsigned long long)(unsigned)best << 32) |

@@ -5,6 +5,7 @@ loudly when libdpx.so is missing: there is no CPU fallback.
 """
 import ctypes
 import os
+import time
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libdpx.so")
@@ -186,7 +187,10 @@ class Context:
         res = (ctypes.c_int32 * max(cnt, 1))()
         tr = (Trace * max(t, 1))() if trace else None
         tl = (ctypes.c_int32 * max(cnt, 1))()
-        self._check(fn(self.h, blob, offs, cnt, *extra, res, tr, toffs if trace else None, tl if trace else None))
+        t0 = time.perf_counter()
+        rc = fn(self.h, blob, offs, cnt, *extra, res, tr, toffs if trace else None, tl if trace else None)
+        self.last_call_s = time.perf_counter() - t0  # the C call alone: copies, kernels, staging
+        self._check(rc)
         out = list(res[:cnt])
         if not trace:
             return out, None
