@@ -90,3 +90,106 @@ def matrix_chain(n, seed, lo=1, hi=1000):
     r = SplitMix64(seed)
     dims = [lo + r.below(hi - lo + 1) for _ in range(n + 1)]
     return dims[:-1], dims[1:]
+
+
+# ---------------------------------------------------------------------------
+# The reference's own input generators, restated so that its benchmark
+# inputs can be reproduced exactly.
+
+class JavaRandom:
+    """java.util.Random (the generator behind scala.util.Random): 48-bit LCG,
+    seed scrambled with 0x5DEECE66D; nextInt() = next(32) as signed int32."""
+
+    MULT, ADD, MASK48 = 0x5DEECE66D, 0xB, (1 << 48) - 1
+
+    def __init__(self, seed):
+        self.seed = (seed ^ self.MULT) & self.MASK48
+
+    def next(self, bits):
+        self.seed = (self.seed * self.MULT + self.ADD) & self.MASK48
+        r = self.seed >> (48 - bits)
+        return r - (1 << bits) if r >= 1 << (bits - 1) else r
+
+    def next_int(self):
+        return self.next(32)
+
+
+SCALA_UTILS_SEED = 123456748299  # Utils.reset (Utils.scala:7)
+
+
+def _jabs_mod(x, m):
+    """Math.abs(x) % m with JVM int semantics (abs(MIN_VALUE) stays negative;
+    % truncates toward zero)."""
+    a = abs(x) if x != -(1 << 31) else x
+    r = abs(a) % m
+    return -r if a < 0 else r
+
+
+def scala_gen_rna(n, rnd):
+    """Utils.genRNA (Utils.scala:29): n symbols of acgu from abs(nextInt) % 4."""
+    return "".join("acgu"[_jabs_mod(rnd.next_int(), 4)] for _ in range(n))
+
+
+def scala_gen_dna(n, rnd):
+    """Utils.genDNA (Utils.scala:28): n symbols of ACGT."""
+    return "".join("ACGT"[_jabs_mod(rnd.next_int(), 4)] for _ in range(n))
+
+
+def scala_gen_mats(n, rnd):
+    """Utils.genMats (Utils.scala:21-25): s = n draws, then one more draw d0;
+    matrices (d0,s0), (s0,s1), ... ; each draw abs(nextInt) % 253 + 1.
+    Returns (rows, cols)."""
+    s = [_jabs_mod(rnd.next_int(), 253) + 1 for _ in range(n)]
+    d0 = _jabs_mod(rnd.next_int(), 253) + 1
+    rows = [d0] + s[:-1]
+    return rows, s
+
+
+class GlibcRandom:
+    """glibc random()/srandom() with the default state (TYPE_3: additive
+    feedback r[i] = r[i-3] + r[i-31] mod 2**32, output r >> 1; the state is
+    seeded by r[i] = 16807 * r[i-1] mod (2**31 - 1) and 310 outputs are
+    discarded), as used by the reference's bench2 harness."""
+
+    def __init__(self, seed):
+        seed &= 0xFFFFFFFF
+        if seed == 0:
+            seed = 1
+        r = [0] * 34
+        r[0] = seed - (1 << 32) if seed >= 1 << 31 else seed  # int32_t word
+        for i in range(1, 31):
+            # 16807 * r mod (2^31 - 1) by Schrage's method, as signed 32-bit words
+            hi, lo = divmod(r[i - 1], 127773) if r[i - 1] >= 0 else (-((-r[i - 1]) // 127773), -((-r[i - 1]) % 127773))
+            word = 16807 * lo - 2836 * hi
+            if word < 0:
+                word += 2147483647
+            r[i] = word
+        for i in range(31, 34):
+            r[i] = r[i - 31]
+        self.r = [x & 0xFFFFFFFF for x in r]
+        for _ in range(310):
+            self._step()
+
+    def _step(self):
+        v = (self.r[-31] + self.r[-3]) & 0xFFFFFFFF
+        self.r.append(v)
+        if len(self.r) > 64:
+            del self.r[:-34]
+        return v >> 1
+
+    def random(self):
+        return self._step()
+
+
+BENCH2_SEED = 0x87654321  # srandom in bench2/wrapper.c:24 and gen.c:10
+
+
+def bench2_seq(size, index, rnd=None):
+    """Sequence number `index` (0-based) of the bench2 generator (gen.c:5-8,
+    wrapper.c:14-21): `size` symbols "nacgu_"[random() % 4 + 1] each, after
+    srandom(0x87654321); sequences are drawn back to back."""
+    if rnd is None:
+        rnd = GlibcRandom(BENCH2_SEED)
+        for _ in range(index * size):
+            rnd.random()
+    return "".join("nacgu_"[rnd.random() % 4 + 1] for _ in range(size))

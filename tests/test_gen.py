@@ -1,6 +1,14 @@
 """Synthetic inputs: the vectorised generators reproduce the scalar
-SplitMix64 stream exactly (so every seeded workload is fixed)."""
+SplitMix64 stream exactly (so every seeded workload is fixed), and the
+restated reference generators (java.util.Random behind Utils.scala, glibc
+random() behind bench2) reproduce their originals."""
+import os
+
+import pytest
+
 from dpx import gen
+
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def test_stream_matches_scalar():
@@ -27,3 +35,45 @@ def test_known_prefix():
     assert gen.rna_gc(24, 2) == "CUCCGUUAUCAGUCUGAUGCAGCA"
     assert [len(s) for s in gen.rna_batch(5, 50, 2000, 3)] == [71, 100, 76, 199, 104]
     assert all(50 <= len(s) <= 2000 for s in gen.rna_batch(100, 50, 2000, 3))
+
+
+# ------------------------------------------------- the reference's generators
+def test_glibc_random_matches_libc():
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    for seed in [gen.BENCH2_SEED, 0, 1, 12345, 2 ** 31 - 1, 2 ** 31, 2 ** 32 - 1]:
+        libc.srandom(ctypes.c_uint(seed))
+        want = [libc.random() for _ in range(700)]
+        g = gen.GlibcRandom(seed)
+        assert [g.random() for _ in range(700)] == want, seed
+
+
+def test_bench2_sequences_match_reference_generator():
+    """bench2/gen.c built from the reference tree by oracle/Makefile (_ref/gen)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "gen")
+    if not os.path.exists(exe):
+        pytest.skip("reference generator not built (needs the reference tree: make -C oracle _ref/gen)")
+    for size, k in [(20, 0), (20, 3), (100, 1), (1, 7)]:
+        out = subprocess.run([exe, str(size), str(k)], capture_output=True, text=True, check=True).stdout.strip()
+        assert gen.bench2_seq(size, k) == out
+
+
+def test_java_random_known_values():
+    # java.util.Random published outputs: new Random(42).nextInt(), new Random(0).nextInt()
+    r = gen.JavaRandom(42)
+    assert [r.next_int(), r.next_int()] == [-1170105035, 234785527]
+    r = gen.JavaRandom(0)
+    assert [r.next_int(), r.next_int()] == [-1155484576, -723955400]
+
+
+def test_scala_generators_shape():
+    r = gen.JavaRandom(gen.SCALA_UTILS_SEED)
+    s = gen.scala_gen_rna(200, r)
+    assert len(s) == 200 and set(s) <= set("acgu")
+    rows, cols = gen.scala_gen_mats(64, gen.JavaRandom(gen.SCALA_UTILS_SEED))
+    assert len(rows) == len(cols) == 64
+    assert all(rows[q + 1] == cols[q] for q in range(63))
+    assert all(1 <= x <= 253 for x in rows + cols)
+    assert gen._jabs_mod(-(1 << 31), 4) == 0 and gen._jabs_mod(-7, 4) == 3 and gen._jabs_mod(7, 4) == 3
+    assert gen._jabs_mod(-(1 << 31), 253) == -((1 << 31) % 253)  # JVM: abs(MIN_VALUE) stays negative
