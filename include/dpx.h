@@ -111,6 +111,22 @@ int dpx_nussinov_batch(dpx_ctx* ctx, const char* seqs, const int64_t* offs, int 
  * finalize kernels (answer, st column, backtrace). */
 int dpx_last_timing(const dpx_ctx* ctx, float* fill_ms, float* finalize_ms);
 
+/* ---- plans: a batch uploaded once and replayed as a HIP graph -------------
+ * No reference counterpart.  dpx_plan_create copies the batch (Zuker or
+ * Nussinov, as DPX_REQ_ZUKER / DPX_REQ_NUSSINOV) to the device, reserves its
+ * DP tables and captures the queue reset, the persistent fill launch and the
+ * finalize launch as one HIP graph; dpx_plan_run replays it (synchronous,
+ * *device_ms = device time of the replay) and dpx_plan_results copies the
+ * answers of the last replay out (same conventions as dpx_zuker_batch).
+ * dpx_set_rna_params returns DPX_EBUSY while a plan exists. */
+typedef struct dpx_plan dpx_plan;
+int dpx_plan_create(dpx_ctx* ctx, int kind, const char* seqs, const int64_t* offs, int count, int min_loop,
+                    int want_trace, dpx_plan** out);
+int dpx_plan_run(dpx_plan* plan, float* device_ms);
+int dpx_plan_results(dpx_plan* plan, int32_t* results, dpx_trace_t* traces, const int64_t* trace_offs,
+                     int32_t* trace_lens);
+void dpx_plan_destroy(dpx_plan* plan);
+
 /* Seconds on the host's monotonic clock, the time base of dpx_result. */
 double dpx_now(void);
 

@@ -136,7 +136,8 @@ struct dpx_ctx {
   Workspace ws;
   DevBuf block;  // device block of the synchronous entry points (grow-only)
   float t_fill = 0, t_fin = 0;
-  std::atomic<int> servers{0};
+  std::atomic<int> servers{0};  // servers and plans hold the energy tables' address
+  std::atomic<int> plans{0};
 };
 
 namespace {
@@ -850,7 +851,8 @@ int dpx_set_rna_params(dpx_ctx* c, const char* path) {
   if (!c || !path) return DPX_EINVAL;
   std::lock_guard<std::mutex> lk(c->mu);
   std::string& err = c->err;
-  if (c->servers.load() > 0) return fail(err, DPX_EBUSY, "energy parameters cannot change while a server runs");
+  if (c->servers.load() > 0 || c->plans.load() > 0)
+    return fail(err, DPX_EBUSY, "energy parameters cannot change while a server or a plan exists");
   (void)hipSetDevice(c->device);
   HostRna* h = new HostRna();
   char msg[256] = {0};
@@ -960,6 +962,155 @@ int dpx_swat(dpx_ctx* c, const char* a, int n1, const char* b, int n2, const dpx
     std::memcpy(trace, j.tr, sizeof(dpx_trace_t) * tl);
   }
   return DPX_OK;
+}
+
+// ---------------------------------------------------------------- plans
+
+}  // extern "C"
+
+struct dpx_plan {
+  dpx_ctx* ctx = nullptr;
+  Job job;
+  DevBuf block;
+  Workspace ws;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ran = false;
+};
+
+namespace {
+
+void plan_free(dpx_plan* P) {
+  if (P->exec) (void)hipGraphExecDestroy(P->exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  P->ws.release();
+  P->block.release();
+}
+
+// Upload the job's inputs once and capture reset + fill + finalize as a graph.
+int plan_build(dpx_plan* P) {
+  dpx_ctx* c = P->ctx;
+  std::string& err = c->err;
+  Job& j = P->job;
+  if (!P->ws.init()) return fail(err, DPX_EHIP, "stream creation failed");
+  if (!P->block.ensure(j.dev_bytes))
+    return fail(err, DPX_ENOMEM, "device allocation for the DP tables failed (" + std::to_string(j.dev_bytes >> 20) + " MiB)");
+  char* dev = (char*)P->block.p;
+  HIPCHK(hipMemcpy(dev + j.off[TL_INST], j.inst.data(), sizeof(TriInst) * j.count, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dev + j.off[TL_SEEDS], j.seeds.data(), j.seeds.size() * 8, hipMemcpyHostToDevice));
+  if (!j.data.empty()) HIPCHK(hipMemcpy(dev + j.off[TL_DATA], j.data.data(), j.data.size(), hipMemcpyHostToDevice));
+  TriArgs a{};
+  a.inst = (const TriInst*)(dev + j.off[TL_INST]);
+  a.n_inst = j.count;
+  a.total_tiles = (int32_t)j.tiles;
+  a.queue = (unsigned long long*)(dev + j.off[TL_QUEUE]);
+  a.ctrl = (int32_t*)(dev + j.off[TL_CTRL]);
+  a.dep = (int32_t*)(dev + j.off[TL_DEP]);
+  a.tab = (int32_t*)(dev + j.off[TL_TAB]);
+  a.codes = (const uint8_t*)(dev + j.off[TL_DATA]);
+  a.dims = (const int32_t*)(dev + j.off[TL_DATA]);
+  a.rna = c->d_rna.p;
+  a.hl_ent = (const int32_t*)c->d_hl.p;
+  a.min_loop = j.min_loop;
+  a.results = (int32_t*)(dev + j.off[TL_RES]);
+  a.traces = dev + j.off[TL_TRACES];
+  a.trace_lens = (int32_t*)(dev + j.off[TL_LENS]);
+  a.want_trace = j.want_trace ? 1 : 0;
+  const int grid = (int)std::min<int64_t>(j.tiles, (int64_t)c->occ[j.g] * c->cu_count);
+  hipStream_t s = P->ws.stream;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int r1 = launch_fill(j.g, a, (const unsigned long long*)(dev + j.off[TL_SEEDS]), (int)j.seeds.size(), grid, s);
+  const int r2 = launch_finalize(j.g, a, j.count, j.n_max, s);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  P->graph = g;
+  if (r1 || r2 || e != hipSuccess) return fail(err, DPX_EHIP, "graph capture of the batch launch failed");
+  HIPCHK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+  return DPX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpx_plan_create(dpx_ctx* c, int kind, const char* seqs, const int64_t* offs, int count, int min_loop, int want_trace,
+                    dpx_plan** out) {
+  if (!c) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::string& err = c->err;
+  if (!out || count < 1 || !seqs || !offs || min_loop < 0 || (kind != DPX_REQ_ZUKER && kind != DPX_REQ_NUSSINOV))
+    return fail(err, DPX_EINVAL, "bad argument");
+  *out = nullptr;
+  const int g = kind == DPX_REQ_ZUKER ? G_ZUKER : G_NUSSINOV;
+  if (g == G_ZUKER && !c->has_rna) return fail(err, DPX_ENOPARAMS, "dpx_set_rna_params was not called");
+  (void)hipSetDevice(c->device);
+  dpx_plan* P = new dpx_plan();
+  P->ctx = c;
+  int r = rna_job(err, P->job, g, seqs, offs, count, g == G_NUSSINOV ? min_loop : 0, want_trace != 0);
+  if (!r) r = plan_build(P);
+  if (r) {
+    plan_free(P);
+    delete P;
+    return r;
+  }
+  c->plans++;
+  *out = P;
+  return DPX_OK;
+}
+
+int dpx_plan_run(dpx_plan* P, float* device_ms) {
+  if (!P) return DPX_EINVAL;
+  dpx_ctx* c = P->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::string& err = c->err;
+  (void)hipSetDevice(c->device);
+  hipStream_t s = P->ws.stream;
+  HIPCHK(hipEventRecord(P->ws.ev[0], s));
+  HIPCHK(hipGraphLaunch(P->exec, s));
+  HIPCHK(hipEventRecord(P->ws.ev[1], s));
+  int32_t flags[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(flags, (char*)P->block.p + P->job.off[TL_CTRL], sizeof flags, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, P->ws.ev[0], P->ws.ev[1]);
+  if (device_ms) *device_ms = ms;
+  if (flags[2]) return fail(err, DPX_EHIP, "tile queue wait timed out (scheduler error)");
+  P->ran = true;
+  return DPX_OK;
+}
+
+int dpx_plan_results(dpx_plan* P, int32_t* results, dpx_trace_t* traces, const int64_t* trace_offs, int32_t* trace_lens) {
+  if (!P) return DPX_EINVAL;
+  dpx_ctx* c = P->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::string& err = c->err;
+  if (!results || (traces && (!trace_offs || !trace_lens))) return fail(err, DPX_EINVAL, "bad argument");
+  if (!P->ran) return fail(err, DPX_EINVAL, "dpx_plan_run was not called");
+  (void)hipSetDevice(c->device);
+  Job& j = P->job;
+  const char* dev = (const char*)P->block.p;
+  const size_t out_bytes = 8 * (size_t)j.count + 64 + (j.want_trace ? sizeof(dpx_trace_t) * (size_t)j.trace_total : 0);
+  if (!P->ws.h_out.ensure(out_bytes)) return fail(err, DPX_ENOMEM, "pinned host staging");
+  int32_t* o_res = (int32_t*)P->ws.h_out.p;
+  int32_t* o_lens = o_res + j.count;
+  dpx_trace_t* o_tr = (dpx_trace_t*)(((uintptr_t)(o_lens + j.count + 4) + 15) & ~(uintptr_t)15);
+  HIPCHK(hipMemcpy(o_res, dev + j.off[TL_RES], sizeof(int32_t) * j.count, hipMemcpyDeviceToHost));
+  if (j.want_trace) {
+    HIPCHK(hipMemcpy(o_lens, dev + j.off[TL_LENS], sizeof(int32_t) * j.count, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(o_tr, dev + j.off[TL_TRACES], sizeof(dpx_trace_t) * j.trace_total, hipMemcpyDeviceToHost));
+  }
+  j.res = o_res;
+  j.lens = o_lens;
+  j.tr = o_tr;
+  return tri_copy_out(err, j, results, traces, trace_offs, trace_lens);
+}
+
+void dpx_plan_destroy(dpx_plan* P) {
+  if (!P) return;
+  (void)hipSetDevice(P->ctx->device);
+  plan_free(P);
+  P->ctx->plans--;
+  delete P;
 }
 
 int dpx_last_timing(const dpx_ctx* c, float* fill_ms, float* finalize_ms) {

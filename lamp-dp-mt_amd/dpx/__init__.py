@@ -20,6 +20,7 @@ EXPORTS = [
     "dpx_zuker_batch", "dpx_nussinov_batch", "dpx_last_timing", "dpx_now",
     "dpx_server_create", "dpx_server_destroy", "dpx_server_submit", "dpx_server_poll",
     "dpx_server_trace", "dpx_server_get_stats",
+    "dpx_plan_create", "dpx_plan_run", "dpx_plan_results", "dpx_plan_destroy",
 ]
 
 REQ_NUSSINOV, REQ_ZUKER, REQ_SWAT, REQ_MATCHAIN = 1, 2, 3, 4
@@ -92,6 +93,12 @@ def lib():
         L.dpx_server_trace.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(Trace), ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int)]
         L.dpx_server_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
+        L.dpx_plan_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.dpx_plan_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.dpx_plan_results.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Trace),
+                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]
+        L.dpx_plan_destroy.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -203,6 +210,10 @@ class Context:
     def nussinov_batch(self, seqs, min_loop=0, trace=True):
         return self._batch(self._L.dpx_nussinov_batch, seqs, trace, ctypes.c_int(min_loop))
 
+    def plan(self, seqs, kind="zuker", min_loop=0, trace=True):
+        """A batch uploaded once and replayed as a HIP graph (dpx_plan_*)."""
+        return Plan(self, seqs, kind, min_loop, trace)
+
     def last_timing(self):
         f, z = ctypes.c_float(), ctypes.c_float()
         self._check(self._L.dpx_last_timing(self.h, ctypes.byref(f), ctypes.byref(z)))
@@ -310,3 +321,58 @@ class Server:
                 got[r.id] = (r, tr)
                 want.discard(r.id)
         return got
+
+
+class Plan:
+    """dpx_plan_*: a Zuker or Nussinov batch captured once as a HIP graph."""
+
+    def __init__(self, ctx, seqs, kind="zuker", min_loop=0, trace=True):
+        self._L = ctx._L
+        self.ctx = ctx
+        self.count = len(seqs)
+        self.trace = trace
+        blob = "".join(seqs).encode()
+        offs = (ctypes.c_int64 * (self.count + 1))()
+        self.toffs = (ctypes.c_int64 * (self.count + 1))()
+        p = t = 0
+        for q, s in enumerate(seqs):
+            offs[q] = p
+            self.toffs[q] = t
+            p += len(s)
+            t += 4 * len(s) + 8
+        offs[self.count] = p
+        self.toffs[self.count] = t
+        self.total = t
+        h = ctypes.c_void_p()
+        k = REQ_ZUKER if kind == "zuker" else REQ_NUSSINOV
+        ctx._check(self._L.dpx_plan_create(ctx.h, k, blob, offs, self.count, min_loop, 1 if trace else 0,
+                                           ctypes.byref(h)))
+        self.h = h
+
+    def run(self):
+        """Replay the graph; returns the device time in ms."""
+        ms = ctypes.c_float()
+        self.ctx._check(self._L.dpx_plan_run(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def results(self):
+        res = (ctypes.c_int32 * self.count)()
+        tr = (Trace * max(self.total, 1))() if self.trace else None
+        tl = (ctypes.c_int32 * self.count)()
+        self.ctx._check(self._L.dpx_plan_results(self.h, res, tr, self.toffs if self.trace else None,
+                                                 tl if self.trace else None))
+        out = list(res)
+        if not self.trace:
+            return out, None
+        return out, [_tuples(tr[self.toffs[q]:self.toffs[q + 1]], tl[q]) for q in range(self.count)]
+
+    def close(self):
+        if self.h:
+            self._L.dpx_plan_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

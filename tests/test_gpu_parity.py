@@ -165,3 +165,32 @@ def test_trace_buffer_too_small(ctx):
     rc = ctx._L.dpx_zuker(ctx.h, seq.encode(), len(seq), ctypes.byref(score), tr, 2, ctypes.byref(tl))
     assert rc == dpx.ETRACE and tl.value == len(ZU[3]["trace"])
     assert score.value == ZU[3]["score"]
+
+
+def test_plan_graph_replays(ctx):
+    """A batch captured as a HIP graph: every replay, with eager work of the
+    same context in between, gives the eager batch's results."""
+    import dpx
+    from dpx import gen
+    seqs = gen.rna_batch(60, 5, 600, 91)
+    want = ctx.zuker_batch(seqs)
+    with ctx.plan(seqs) as p:
+        for rep in range(4):
+            ms = p.run()
+            assert ms > 0
+            assert p.results() == want, rep
+            ctx.nussinov("GGGGAAAACCCC")  # eager work between replays
+    nus = gen.rna_batch(30, 1, 300, 92)
+    want_n = ctx.nussinov_batch(nus, min_loop=3)
+    with ctx.plan(nus, kind="nussinov", min_loop=3) as p:
+        for rep in range(3):
+            p.run()
+            assert p.results() == want_n, rep
+    # energy parameters are pinned while a plan exists
+    p = ctx.plan(seqs[:3], trace=False)
+    with pytest.raises(dpx.DpxError) as e:
+        ctx.set_rna_params(dpx.PAR_TURNER2004)
+    assert e.value.code == dpx.EBUSY
+    p.run()
+    assert p.results()[0] == want[0][:3]
+    p.close()
