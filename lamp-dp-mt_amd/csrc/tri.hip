@@ -137,9 +137,8 @@ __device__ __forceinline__ unsigned long long claim_item(const TriArgs& a) {
   if (threadIdx.x == 0) slot = __hip_atomic_fetch_add(&a.ctrl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   slot = __builtin_amdgcn_readfirstlane(slot);
   if (slot >= a.total_tiles) return ~0ull;
-  // Poll with relaxed loads (an acquire per poll would invalidate this XCD's
-  // L2 every time), then acquire once.  Bounded wait (a few seconds): a
-  // scheduling bug ends the kernel with an error flag instead of a hang.
+  // Poll with relaxed loads.  Bounded wait (a few seconds): a scheduling bug
+  // ends the kernel with an error flag instead of a hang.
   unsigned long long it = uniform64(__hip_atomic_load(&a.queue[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   for (int spin = 0; it == ~0ull; ++spin) {
     if (spin > (1 << 23)) {
@@ -149,7 +148,13 @@ __device__ __forceinline__ unsigned long long claim_item(const TriArgs& a) {
     __builtin_amdgcn_s_sleep(4);
     it = uniform64(__hip_atomic_load(&a.queue[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  // No cache invalidation on the consumer side: a tile record is written
+  // once per launch, by its producer, and no cache line of it is read before
+  // the producer has written the record back to memory and released the
+  // item (caches are clean at kernel start), so no L1/L2 can hold a stale
+  // copy.  Invalidating this XCD's L2 at every claim would instead evict the
+  // operand tiles that every workgroup of the XCD is reusing.
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   return it;
 }
 
@@ -170,7 +175,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) void 
     if (it == ~0ull) {
       live = false;
     } else {
-      if (threadIdx.x >= 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
       const int q = (int)(it >> 32), I = (int)((it >> 16) & 0xffff), J = (int)(it & 0xffff);
       const TriInst in = a.inst[q];
       G::tile(a, in, I, J, sm);
@@ -180,15 +185,17 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) void 
       __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
       __syncthreads();
       if (threadIdx.x == 0) {
+        // one device-scope release (L2 writeback) covers the whole record
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         // successors (I-1,J) and (I,J+1): queued by whichever predecessor finishes second
         for (int s2 = 0; s2 < 2; ++s2) {
           const int I2 = s2 == 0 ? I - 1 : I, J2 = s2 == 0 ? J : J + 1;
           if (I2 < 0 || J2 >= in.nb) continue;
           const int g = (int)(in.tile0 + tile_index(in.nb, I2, J2));
-          const int old = __hip_atomic_fetch_add(&a.dep[g], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+          const int old = __hip_atomic_fetch_add(&a.dep[g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if (old == 1) {
             const int pos = __hip_atomic_fetch_add(&a.ctrl[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&a.queue[pos], pack_item(q, I2, J2), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.queue[pos], pack_item(q, I2, J2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           }
         }
       }
@@ -421,6 +428,7 @@ struct ZuTab {
   int16_t stack[8][8];
   int16_t mmI[8][5][5], mm1nI[8][5][5], mm23I[8][5][5];
   int16_t bulge[32];
+  int32_t term_au, il23;  // terminal AU penalty; 2x3 loop energy il(5) + ninio
 };
 
 // Per-cell terms that depend on the sequence only, precomputed one wavefront
@@ -451,7 +459,7 @@ struct ZuView {
   __device__ int R(int x) const { return rs[x - IT1]; }
   __device__ int C(int x) const { return cs[x - JT34]; }
   __device__ int tau(int x, int y) const {  // termau_energy on bases (librna_impl.h:152)
-    return ((x == 3 && y == 2) || (x == 2 && y == 3)) ? 0 : P->term_au;
+    return ((x == 3 && y == 2) || (x == 2 && y == 3)) ? 0 : E.term_au;
   }
   __device__ bool stackpair(int i, int j) const {  // fun6 (zu.c:35)
     return i + 4 <= j && bpc(R(i), C(j - 1)) != 7 && bpc(R(i + 1), C(j - 2)) != 7;
@@ -459,66 +467,56 @@ struct ZuView {
   // interior mismatch of the pair closing cell (l,k) (pair l..k-1), seen from outside
   __device__ int inner(int l, int k) const { return E.mmI[bpc(C(k - 1), R(l))][C(k)][R(l - 1)]; }
   __device__ int sr(int i, int j) const { return E.stack[bpc(R(i), C(j - 1))][bpc(C(j - 2), R(i + 1))]; }
-  __device__ int hairpin(int i, int j) const {  // hl_energy(i+1, j-2) (librna_impl.h:181-193)
-    const int a = i + 1, b = j - 2, size = b - a - 1;
-    const int cls = size == 3 ? 0 : size == 4 ? 1 : size == 6 ? 2 : -1;
-    if (cls >= 0) {  // special loops: the window lies on the row side (size <= 6)
-      int key = 0;
-      for (int m = 0; m <= size + 1; ++m) key = key * 8 + R(a + m);
-      for (int e = 0; e < P->n_loop[cls]; ++e)
-        if (P->loop_key[cls][e] == key) return P->loop_e[cls][e];
-    }
-    const int ent = size <= 30 ? P->hairpin[size] : hl[size];
-    if (size == 3) return ent + tau(R(a), C(b));
-    return ent + P->mmH[bpc(R(a), C(b))][R(a + 1)][C(b - 1)];
-  }
-  __device__ int ext(int i, int j) const {  // ext_mismatch_energy(i, j) with j = cell column - 1
-    const int bb = bpc(R(i), C(j));
-    if (i > 0 && j + 1 < n) return P->mmExt[bb][R(i - 1)][C(j + 1)];
-    if (i > 0) return P->d5[bb][R(i - 1)];
-    if (j + 1 < n) return P->d3[bb][C(j + 1)];
-    return 0;
-  }
-  // il_energy(i+1, l, k-1, j-2) (librna_impl.h:230-265), sizes sl = l-i-2, sr = j-k-2 >= 1
-  __device__ int il(int i, int j, int l, int k) const {
-    const int sl = l - i - 2, srr = j - k - 2;
-    const int ocb = bpc(R(i + 1), C(j - 2)), icb = bpc(C(k - 1), R(l));
-    const int ol = R(i + 2), orr = C(j - 3), il_ = C(k), ir = R(l - 1);
-    if (sl == 1) {
-      if (srr == 1) return P->int11[ocb][bpc(C(j - 4), R(i + 3))][ol][orr];
-      if (srr == 2) return P->int21[ocb][bpc(C(j - 5), R(i + 3))][ol][C(j - 4)][orr];
-      return P->F[sl][srr] + E.mm1nI[ocb][ol][orr] + E.mm1nI[icb][il_][ir];
-    }
-    if (sl == 2) {
-      if (srr == 1) return P->int21[bpc(C(j - 4), R(i + 4))][ocb][orr][ol][R(i + 3)];
-      if (srr == 2) return P->int22[ocb][bpc(C(j - 5), R(i + 4))][ol][R(i + 3)][C(j - 4)][orr];
-      if (srr == 3) return P->il5 + P->ninio0 + E.mm23I[ocb][ol][orr] + E.mm23I[icb][il_][ir];
-    } else if (sl == 3 && srr == 2) {
-      return P->il5 + P->ninio0 + E.mm23I[ocb][ol][orr] + E.mm23I[icb][il_][ir];
-    } else if (srr == 1) {
-      return P->F[sl][srr] + E.mm1nI[ocb][ol][orr] + E.mm1nI[icb][il_][ir];
-    }
-    return P->F[sl][srr] + E.mmI[ocb][ol][orr] + E.mmI[icb][il_][ir];
-  }
+  // Sequence-only terms of cell (i,j).  The device-memory table reads are
+  // independent of each other, so they are all issued before any is used.
   __device__ void pre(int i, int j, ZuPre* o) const {
     const int d = j - i;
     o->valid = 0;
     if (j > n || d < 7 || !stackpair(i, j)) return;
     o->valid = 1;
-    const int s_ = sr(i, j);
+    const int Ri = R(i), Ri1 = R(i + 1), Ri2 = R(i + 2), Ri3 = R(i + 3), Ri4 = R(i + 4);
+    const int Cj1 = C(j - 1), Cj2 = C(j - 2), Cj3 = C(j - 3), Cj4 = C(j - 4), Cj5 = C(j - 5);
+    const int ocb = bpc(Ri1, Cj2), bo = bpc(Ri, Cj1);
+    // hl_energy(i+1, j-2) generic part (librna_impl.h:181-193), loop size d-4
+    const int size = d - 4;
+    const int hent = size <= 30 ? P->hairpin[size] : hl[size];
+    const int hmm = P->mmH[ocb][Ri2][Cj3];
+    // ml_mismatch of the closing pair (rule 5), ext_mismatch(i, j-1) (rule 10/U)
+    const int mlm = P->mmM[bpc(Cj2, Ri1)][Cj3][Ri2];
+    const int* ep = (i > 0 && j < n) ? &P->mmExt[bo][R(i - 1)][C(j)] : (i > 0 ? &P->d5[bo][R(i - 1)] : &P->d3[bo][C(j)]);
+    const int ex = (i > 0 || j < n) ? *ep : 0;
+    // 1x1, 1x2, 2x1, 2x2 loop energies (il11/12/21/22_energy, librna_impl.h:222-225)
+    int e11 = 0, e12 = 0, e21 = 0, e22 = 0;
+    if (d >= 12) {
+      e11 = P->int11[ocb][bpc(Cj4, Ri3)][Ri2][Cj3];
+      e12 = P->int21[ocb][bpc(Cj5, Ri3)][Ri2][Cj4][Cj3];
+      e21 = P->int21[bpc(Cj4, Ri4)][ocb][Cj3][Ri2][Ri3];
+      e22 = P->int22[ocb][bpc(Cj5, Ri4)][Ri2][Ri3][Cj4][Cj3];
+    }
+    int hp = size == 3 ? hent + tau(Ri1, Cj2) : hent + hmm;
+    const int cls = size == 3 ? 0 : size == 4 ? 1 : size == 6 ? 2 : -1;
+    if (cls >= 0) {  // special tri-/tetra-/hexaloops: the loop lies on the row side (size <= 6)
+      int key = 0;
+      for (int m = 0; m <= size + 1; ++m) key = key * 8 + R(i + 1 + m);
+      for (int e = 0; e < P->n_loop[cls]; ++e)
+        if (P->loop_key[cls][e] == key) {
+          hp = P->loop_e[cls][e];
+          break;
+        }
+    }
+    const int s_ = E.stack[bo][bpc(Cj2, Ri1)];
     o->sr = s_;
-    o->r1 = hairpin(i, j) + s_;
-    o->r5c = P->ml_closing + P->ml_intern + tau(R(i + 1), C(j - 2)) + P->mmM[bpc(C(j - 2), R(i + 1))][C(j - 3)][R(i + 2)] + s_;
-    o->uoff = ext(i, j - 1) + tau(R(i), C(j - 1)) + P->ml_intern;
+    o->r1 = hp + s_;
+    o->r5c = P->ml_closing + P->ml_intern + tau(Ri1, Cj2) + mlm + s_;
+    o->uoff = ex + tau(Ri, Cj1) + P->ml_intern;
     o->boff = (i >= 1 && j <= n - 1) ? inner(i, j) : kNoB;
-    const int ocb = bpc(R(i + 1), C(j - 2)), ol = R(i + 2), orr = C(j - 3);
-    o->outI = E.mmI[ocb][ol][orr] + s_;
+    o->outI = E.mmI[ocb][Ri2][Cj3] + s_;
     o->in0 = d >= 9 ? inner(i + 1, j - 1) : 0;
-    if (d >= 12) {  // the smallest cell with interior loops (l = i+3, k = j-3, k-l >= 7 needs d >= 13)
-      o->f11 = (int16_t)(il(i, j, i + 3, j - 3) - inner(i + 3, j - 3));
-      o->f12 = (int16_t)(il(i, j, i + 3, j - 4) - inner(i + 3, j - 4));
-      o->f21 = (int16_t)(il(i, j, i + 4, j - 3) - inner(i + 4, j - 3));
-      o->f22 = (int16_t)(il(i, j, i + 4, j - 4) - inner(i + 4, j - 4));
+    if (d >= 12) {
+      o->f11 = (int16_t)(e11 - inner(i + 3, j - 3));
+      o->f12 = (int16_t)(e12 - inner(i + 3, j - 4));
+      o->f21 = (int16_t)(e21 - inner(i + 4, j - 3));
+      o->f22 = (int16_t)(e22 - inner(i + 4, j - 4));
     }
   }
 };
@@ -563,6 +561,10 @@ struct GZu {
       (&E.mm23I[0][0][0])[e] = (int16_t)(&P->mm23I[0][0][0])[e];
     }
     for (int e = threadIdx.x; e < 31; e += NTHR) E.bulge[e] = (int16_t)P->bulge[e];
+    if (threadIdx.x == 0) {
+      E.term_au = P->term_au;
+      E.il23 = P->il5 + P->ninio0;
+    }
   }
 
   // sequence-only terms of the cells of band [ld0, ld0+G) and the compacted
@@ -661,7 +663,7 @@ struct GZu {
     } else {
       // 2x3 and 3x2 loops
       const int B23 = sm.u.w.B[r + 4][c + 27], B32 = sm.u.w.B[r + 5][c + 28];
-      const int out23 = z.P->il5 + z.P->ninio0 + E.mm23I[ocb][Ri2][Cj3] + pc.sr;
+      const int out23 = E.il23 + E.mm23I[ocb][Ri2][Cj3] + pc.sr;
       if (B23 < INV_THR) {
         const int l = i + 4, k = j - 5, icb = bpc(z.C(k - 1), z.R(l)), Ck = z.C(k), Rl1 = z.R(l - 1);
         best = imin(best, B23 - E.mmI[icb][Ck][Rl1] + E.mm23I[icb][Ck][Rl1] + out23);
