@@ -75,6 +75,14 @@ int dpx_nussinov(dpx_ctx* ctx, const char* seq, int n, int min_loop, int32_t* sc
 int dpx_zuker(dpx_ctx* ctx, const char* seq, int n, int32_t* mfe,
               dpx_trace_t* trace, int trace_cap, int* trace_len);
 
+/* RNAfold MFE grammar (v4/examples/RNAFold.scala:20-64): tables Q (rules
+ * 0 QM, 1 Q(i,j-1), 2 Q(i+1,j), 3 Qp), QM (rule 4 split(pos k)), Qp (rules
+ * 5 hairpin, 6 stack, 7 interior loop (pos k, l) with both unpaired
+ * stretches < 30, 8 multiloop); energies in dcal/mol; 999999 when the
+ * grammar admits no structure (Q(0,n) empty). */
+int dpx_rnafold(dpx_ctx* ctx, const char* seq, int n, int32_t* mfe,
+                dpx_trace_t* trace, int trace_cap, int* trace_len);
+
 /* Smith-Waterman with affine gaps, two-track grammar (v4/examples/
  * SWatAffine.scala:22-69; bench/dp_swat/comp1.cu:50-132) with parametric
  * scores: pair = M + (a==b ? match : mismatch); open = max(0, x + gap_open);
@@ -105,6 +113,8 @@ int dpx_zuker_batch(dpx_ctx* ctx, const char* seqs, const int64_t* offs, int cou
                     dpx_trace_t* traces, const int64_t* trace_offs, int32_t* trace_lens);
 int dpx_nussinov_batch(dpx_ctx* ctx, const char* seqs, const int64_t* offs, int count, int min_loop,
                        int32_t* results, dpx_trace_t* traces, const int64_t* trace_offs, int32_t* trace_lens);
+int dpx_rnafold_batch(dpx_ctx* ctx, const char* seqs, const int64_t* offs, int count, int32_t* results,
+                      dpx_trace_t* traces, const int64_t* trace_offs, int32_t* trace_lens);
 
 /* Timing of the last synchronous call on this context: device time (ms) of
  * the fill kernel (persistent tile queue / alignment bands) and of the
@@ -112,8 +122,8 @@ int dpx_nussinov_batch(dpx_ctx* ctx, const char* seqs, const int64_t* offs, int 
 int dpx_last_timing(const dpx_ctx* ctx, float* fill_ms, float* finalize_ms);
 
 /* ---- plans: a batch uploaded once and replayed as a HIP graph -------------
- * No reference counterpart.  dpx_plan_create copies the batch (Zuker or
- * Nussinov, as DPX_REQ_ZUKER / DPX_REQ_NUSSINOV) to the device, reserves its
+ * No reference counterpart.  dpx_plan_create copies the batch (Zuker,
+ * Nussinov or RNAfold, as DPX_REQ_ZUKER / NUSSINOV / RNAFOLD) to the device, reserves its
  * DP tables and captures the queue reset, the persistent fill launch and the
  * finalize launch as one HIP graph; dpx_plan_run replays it (synchronous,
  * *device_ms = device time of the replay) and dpx_plan_results copies the
@@ -146,7 +156,7 @@ double dpx_now(void);
  * DPX_EBUSY while a server exists. */
 typedef struct dpx_server dpx_server;
 
-enum { DPX_REQ_NUSSINOV = 1, DPX_REQ_ZUKER = 2, DPX_REQ_SWAT = 3, DPX_REQ_MATCHAIN = 4 };
+enum { DPX_REQ_NUSSINOV = 1, DPX_REQ_ZUKER = 2, DPX_REQ_SWAT = 3, DPX_REQ_MATCHAIN = 4, DPX_REQ_RNAFOLD = 5 };
 enum { DPX_SUBMIT_TRACE = 1, DPX_SUBMIT_TRACE_FROM_BEST = 2 };
 
 typedef struct {

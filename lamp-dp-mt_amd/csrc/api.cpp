@@ -128,7 +128,7 @@ struct dpx_ctx {
   std::string err;
   std::mutex mu;
   int cu_count = 0;
-  int occ[3] = {1, 1, 1};
+  int occ[N_GRAMMARS] = {1, 1, 1, 1};
   int sw_occ = 1;
   bool has_rna = false;
   HostRna* rna = nullptr;
@@ -174,10 +174,14 @@ int encode_rna(std::string& err, const char* seq, int n, uint8_t* out) {
   return DPX_OK;
 }
 
-int nt_of(int g) { return g == G_ZUKER ? NT_ZU : g == G_NUSSINOV ? NT_NU : NT_MC; }
+int nt_of(int g) { return g == G_ZUKER ? NT_ZU : g == G_NUSSINOV ? NT_NU : g == G_RNAFOLD ? NT_RF : NT_MC; }
 int64_t tri_cells(int n) { return (int64_t)(n + 1) * (n + 2) / 2; }
 int tri_trace_cap(int n) { return 4 * n + 8; }
 int max_len(int g) { return g == G_ZUKER ? kMaxZuker : kMaxTri; }
+// grammar of a folding request kind (DPX_REQ_NUSSINOV / ZUKER / RNAFOLD)
+int grammar_of(int kind) {
+  return kind == DPX_REQ_NUSSINOV ? G_NUSSINOV : kind == DPX_REQ_RNAFOLD ? G_RNAFOLD : G_ZUKER;
+}
 
 // ---------------------------------------------------------------- layouts
 
@@ -509,7 +513,7 @@ int rna_batch(dpx_ctx* c, int g, const char* seqs, const int64_t* offs, int coun
   std::string& err = c->err;
   if (count < 0 || (count > 0 && (!seqs || !offs || !results))) return fail(err, DPX_EINVAL, "null argument");
   if (traces && (!trace_offs || !trace_lens)) return fail(err, DPX_EINVAL, "trace offsets missing");
-  if (g == G_ZUKER && !c->has_rna) return fail(err, DPX_ENOPARAMS, "dpx_set_rna_params was not called");
+  if (g != G_NUSSINOV && !c->has_rna) return fail(err, DPX_ENOPARAMS, "dpx_set_rna_params was not called");
   if (count == 0) return DPX_OK;
   Job j;
   int r = rna_job(err, j, g, seqs, offs, count, min_loop, traces != nullptr);
@@ -683,7 +687,7 @@ int prepare_slot(dpx_server* S, Slot& sl) {
     int rc = encode_rna(sl.err, r.s1.data(), lens[q], data.data() + at);
     if (rc) return rc;
   }
-  const int g = r0.kind == DPX_REQ_NUSSINOV ? G_NUSSINOV : G_ZUKER;
+  const int g = grammar_of(r0.kind);
   return tri_prepare(sl.err, j, g, lens, off, std::move(data), r0.min_loop, r0.trace != 0);
 }
 
@@ -828,7 +832,7 @@ int dpx_create(dpx_ctx** out, int device) {
     return DPX_EHIP;
   }
   (void)hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, device);
-  for (int g = 0; g < 3; ++g) c->occ[g] = std::max(1, fill_occupancy(g));
+  for (int g = 0; g < N_GRAMMARS; ++g) c->occ[g] = std::max(1, fill_occupancy(g));
   c->sw_occ = std::max(1, sw_occupancy());
   *out = c;
   return DPX_OK;
@@ -914,6 +918,18 @@ int dpx_nussinov(dpx_ctx* c, const char* seq, int n, int min_loop, int32_t* scor
 
 int dpx_zuker(dpx_ctx* c, const char* seq, int n, int32_t* mfe, dpx_trace_t* trace, int trace_cap, int* trace_len) {
   return single_rna(c, G_ZUKER, seq, n, 0, mfe, trace, trace_cap, trace_len);
+}
+
+int dpx_rnafold(dpx_ctx* c, const char* seq, int n, int32_t* mfe, dpx_trace_t* trace, int trace_cap, int* trace_len) {
+  return single_rna(c, G_RNAFOLD, seq, n, 0, mfe, trace, trace_cap, trace_len);
+}
+
+int dpx_rnafold_batch(dpx_ctx* c, const char* seqs, const int64_t* offs, int count, int32_t* results,
+                      dpx_trace_t* traces, const int64_t* trace_offs, int32_t* trace_lens) {
+  if (!c) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  return rna_batch(c, G_RNAFOLD, seqs, offs, count, 0, results, traces, trace_offs, trace_lens);
 }
 
 int dpx_matchain(dpx_ctx* c, const int32_t* rows, const int32_t* cols, int n, int32_t* cost, dpx_trace_t* trace,
@@ -1038,11 +1054,12 @@ int dpx_plan_create(dpx_ctx* c, int kind, const char* seqs, const int64_t* offs,
   if (!c) return DPX_EINVAL;
   std::lock_guard<std::mutex> lk(c->mu);
   std::string& err = c->err;
-  if (!out || count < 1 || !seqs || !offs || min_loop < 0 || (kind != DPX_REQ_ZUKER && kind != DPX_REQ_NUSSINOV))
+  if (!out || count < 1 || !seqs || !offs || min_loop < 0 ||
+      (kind != DPX_REQ_ZUKER && kind != DPX_REQ_NUSSINOV && kind != DPX_REQ_RNAFOLD))
     return fail(err, DPX_EINVAL, "bad argument");
   *out = nullptr;
-  const int g = kind == DPX_REQ_ZUKER ? G_ZUKER : G_NUSSINOV;
-  if (g == G_ZUKER && !c->has_rna) return fail(err, DPX_ENOPARAMS, "dpx_set_rna_params was not called");
+  const int g = grammar_of(kind);
+  if (g != G_NUSSINOV && !c->has_rna) return fail(err, DPX_ENOPARAMS, "dpx_set_rna_params was not called");
   (void)hipSetDevice(c->device);
   dpx_plan* P = new dpx_plan();
   P->ctx = c;
@@ -1175,10 +1192,10 @@ int dpx_server_submit(dpx_server* S, const dpx_request* req, uint64_t* id) {
   q.trace = (req->flags & DPX_SUBMIT_TRACE) ? 1 : 0;
   q.trace_from = (req->flags & DPX_SUBMIT_TRACE_FROM_BEST) ? 1 : 0;
   const int n1 = req->n1, n2 = req->n2;
-  if (q.kind == DPX_REQ_ZUKER || q.kind == DPX_REQ_NUSSINOV) {
-    const int g = q.kind == DPX_REQ_ZUKER ? G_ZUKER : G_NUSSINOV;
+  if (q.kind == DPX_REQ_ZUKER || q.kind == DPX_REQ_NUSSINOV || q.kind == DPX_REQ_RNAFOLD) {
+    const int g = grammar_of(q.kind);
     if (!req->a || n1 < 0 || n1 > max_len(g) || req->min_loop < 0) return DPX_EINVAL;
-    if (g == G_ZUKER && !S->ctx->has_rna) return DPX_ENOPARAMS;
+    if (g != G_NUSSINOV && !S->ctx->has_rna) return DPX_ENOPARAMS;
     for (int x = 0; x < n1; ++x)
       if (!req->a[x] || !std::strchr("ACGUTacgut", req->a[x])) return DPX_EINVAL;
     q.s1.assign(req->a, n1);

@@ -27,7 +27,8 @@ constexpr int INV_THR = 1 << 27;
 constexpr int NEG = -(1 << 28); // "no candidate" for max-plus tables
 constexpr int ZU_EMPTY = 999999;  // the reference's empty value (Zuker.scala:154)
 
-enum Grammar { G_NUSSINOV = 0, G_ZUKER = 1, G_MATCHAIN = 2 };
+enum Grammar { G_NUSSINOV = 0, G_ZUKER = 1, G_MATCHAIN = 2, G_RNAFOLD = 3 };
+constexpr int N_GRAMMARS = 4;
 // tables per tile record
 constexpr int NT_NU = 1;  // s
 constexpr int NT_MC = 1;  // M
@@ -35,6 +36,9 @@ constexpr int NT_MC = 1;  // M
 // B (= cl + interior mismatch of the pair as the inner pair of a loop)
 constexpr int NT_ZU = 6;
 enum { ZT_CL = 0, ZT_U = 1, ZT_ML = 2, ZT_ML1 = 3, ZT_R11 = 4, ZT_B = 5 };
+// RNAfold grammar: Q, QM, Qp, B (= Qp + interior mismatch of the pair as an inner pair)
+constexpr int NT_RF = 4;
+enum { RT_Q = 0, RT_QM = 1, RT_QP = 2, RT_B = 3 };
 
 struct TriInst {
   int32_t n;         // sequence length / chain length; cells 0..n

@@ -457,12 +457,128 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
   if (threadIdx.x == 0) a.trace_lens[in.out] = b.overflow ? -1 : b.wr;
 }
 
+// ------------------------------------------------------------------ RNAfold
+// Interior loop of outer pair (i,j) and inner pair (k,l) for every size
+// split, bulges and the stacked pair included (librna_impl.h:199-265).
+__device__ int rf_il(const ZuF& z, int i, int k, int l, int j) {
+  const int sl = k - i - 1, sr_ = j - l - 1;
+  if (sl != 0 && sr_ != 0) return z.il(i, k, l, j);
+  const int size = sl + sr_, ocb = z.bp(i, j), icb = z.bp(l, k);
+  if (size == 0) return z.P->stack[ocb][icb];
+  if (size == 1) return z.P->bulge[1] + z.P->stack[ocb][icb];
+  return z.P->bulge[size] + z.termau(i, j) + z.termau(l, k);
+}
+
+// Tables: record rule 0 = Q, 4 = QM, 5 = Qp (the first rule id of each
+// table).  Candidates in the grammar's order (RNAFold.scala:20-64):
+//   Q : QM(i,j) [0], Q(i,j-1) [1], Q(i+1,j) [2], Qp(i,j) [3]
+//   QM: Q(i,k) + Q(k,j), k = i+3 .. j-3 ascending [4]
+//   Qp: hairpin [5], stack [6], interior loops with l outer ascending from
+//       max(i+4, j-30) to j-1 and k inner from i+1 to min(i+30, l-3) [7],
+//       multiloop QM(i+1, j-1) [8]
+__global__ __launch_bounds__(NTHR) void rf_finalize(TriArgs a) {
+  const TriInst in = a.inst[blockIdx.x];
+  const Cells C{a.tab, in.nb, NT_RF, in.tile0};
+  const int n = in.n;
+  const ZuF z{a.codes + in.seq_off, (const DevRna*)a.rna, a.hl_ent, n};
+  __shared__ Bfs b;
+  const int top = C.get(RT_Q, 0, n);
+  if (threadIdx.x == 0) a.results[in.out] = top < INV_THR ? top : ZU_EMPTY;
+  if (!a.want_trace) return;
+  TraceRec* tr = (TraceRec*)a.traces + in.trace_off;
+  const int cap = in.trace_cap;
+  if (threadIdx.x == 0) {
+    b.rd = 0; b.wr = 0; b.overflow = 0;
+    if (top < INV_THR) {
+      if (cap > 0) { tr[0] = TraceRec{0, n, 0, {0, 0}}; b.wr = 1; } else b.overflow = 1;
+    }
+  }
+  __syncthreads();
+  for (;;) {
+    if (threadIdx.x == 0 && b.rd < b.wr) b.cur = tr[b.rd];
+    __syncthreads();
+    if (b.rd >= b.wr || b.overflow) break;
+    const int i = b.cur.i, j = b.cur.j, tabrule = b.cur.rule, d = j - i;
+    const int l0 = fmax_(i + 4, j - 30), nl = fmax_(0, j - 1 - l0 + 1);
+    int ncand, target;
+    if (tabrule == 0) { target = C.get(RT_Q, i, j); ncand = 4; }
+    else if (tabrule == 4) { target = C.get(RT_QM, i, j); ncand = fmax_(0, j - 3 - (i + 3) + 1); }
+    else { target = C.get(RT_QP, i, j); ncand = 3 + 30 * nl; }
+    auto cand = [&](int c, int* v) -> bool {
+      if (tabrule == 0) {
+        int x;
+        if (c == 0) x = C.get(RT_QM, i, j);
+        else if (c == 1) x = C.get(RT_Q, i, j - 1);
+        else if (c == 2) x = C.get(RT_Q, i + 1, j);
+        else x = C.get(RT_QP, i, j);
+        if (x >= INV_THR) return false;
+        *v = x; return true;
+      }
+      if (tabrule == 4) {
+        const int k = i + 3 + c;
+        const int x = C.get(RT_Q, i, k), y = C.get(RT_Q, k, j);
+        if (x >= INV_THR || y >= INV_THR) return false;
+        *v = x + y; return true;
+      }
+      if (c == 0) { if (d < 3) return false; *v = z.hairpin(i, j - 1); return true; }
+      if (c == 1) {
+        const int x = C.get(RT_QP, i + 1, j - 1);
+        if (x >= INV_THR) return false;
+        *v = z.sr(i, j - 1) + x; return true;
+      }
+      if (c < 2 + 30 * nl) {
+        const int l = l0 + (c - 2) / 30, k = i + 1 + (c - 2) % 30;
+        if (k > l - 3) return false;
+        const int x = C.get(RT_QP, k, l);
+        if (x >= INV_THR) return false;
+        *v = rf_il(z, i, k, l - 1, j - 1) + x; return true;
+      }
+      const int x = C.get(RT_QM, i + 1, j - 1);
+      if (x >= INV_THR) return false;
+      *v = x; return true;
+    };
+    first_match(ncand, target, cand, &b.first);
+    if (threadIdx.x == 0) {
+      const int c = b.first;
+      TraceRec& e = tr[b.rd];
+      int rule = -1, p0 = 0, p1 = 0;
+      if (c == 0x7fffffff) b.overflow = 1;
+      else if (tabrule == 0) rule = c;
+      else if (tabrule == 4) { rule = 4; p0 = i + 3 + c; }
+      else if (c == 0) rule = 5;
+      else if (c == 1) rule = 6;
+      else if (c < 2 + 30 * nl) { rule = 7; p0 = i + 1 + (c - 2) % 30; p1 = l0 + (c - 2) / 30; }
+      else rule = 8;
+      e.rule = rule; e.pos[0] = p0; e.pos[1] = p1;
+      auto push = [&](int pi, int pj, int pr) {
+        if (b.wr >= cap) { b.overflow = 1; return; }
+        tr[b.wr++] = TraceRec{pi, pj, pr, {0, 0}};
+      };
+      switch (rule) {
+        case 0: push(i, j, 4); break;
+        case 1: push(i, j - 1, 0); break;
+        case 2: push(i + 1, j, 0); break;
+        case 3: push(i, j, 5); break;
+        case 4: push(i, p0, 0); push(p0, j, 0); break;
+        case 6: push(i + 1, j - 1, 5); break;
+        case 7: push(p0, p1, 5); break;
+        case 8: push(i + 1, j - 1, 4); break;
+        default: break;
+      }
+      b.rd++;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.trace_lens[in.out] = b.overflow ? -1 : b.wr;
+}
+
 // ------------------------------------------------------------------ launch
 int launch_finalize(int grammar, const TriArgs& a, int count, int n_max, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   if (count <= 0) return 0;
   if (grammar == G_NUSSINOV) hipLaunchKernelGGL(nu_finalize, dim3(count), dim3(NTHR), 0, st, a);
   else if (grammar == G_MATCHAIN) hipLaunchKernelGGL(mc_finalize, dim3(count), dim3(NTHR), 0, st, a);
+  else if (grammar == G_RNAFOLD) hipLaunchKernelGGL(rf_finalize, dim3(count), dim3(NTHR), 0, st, a);
   else hipLaunchKernelGGL(zu_finalize, dim3(count), dim3(NTHR), (size_t)(n_max + 1) * sizeof(int), st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

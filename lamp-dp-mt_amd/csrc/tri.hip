@@ -521,6 +521,22 @@ struct ZuView {
   }
 };
 
+// energy tables of the loop phases, loaded once per workgroup
+__device__ void load_tab(const TriArgs& a, ZuTab& E) {
+  const DevRna* P = (const DevRna*)a.rna;
+  for (int e = threadIdx.x; e < 64; e += NTHR) (&E.stack[0][0])[e] = (int16_t)(&P->stack[0][0])[e];
+  for (int e = threadIdx.x; e < 200; e += NTHR) {
+    (&E.mmI[0][0][0])[e] = (int16_t)(&P->mmI[0][0][0])[e];
+    (&E.mm1nI[0][0][0])[e] = (int16_t)(&P->mm1nI[0][0][0])[e];
+    (&E.mm23I[0][0][0])[e] = (int16_t)(&P->mm23I[0][0][0])[e];
+  }
+  for (int e = threadIdx.x; e < 31; e += NTHR) E.bulge[e] = (int16_t)P->bulge[e];
+  if (threadIdx.x == 0) {
+    E.term_au = P->term_au;
+    E.il23 = P->il5 + P->ninio0;
+  }
+}
+
 struct GZu {
   static constexpr int BR = 64, BC = 64;   // B region: rows IT..IT+63, cols JT-32..JT+31
   static constexpr int BS = BC + 1;        // odd row stride: lanes walking a column hit distinct banks
@@ -551,21 +567,7 @@ struct GZu {
     int nlist;
   };
 
-  __device__ static void init(const TriArgs& a, Smem& sm) {
-    const DevRna* P = (const DevRna*)a.rna;
-    ZuTab& E = sm.E;
-    for (int e = threadIdx.x; e < 64; e += NTHR) (&E.stack[0][0])[e] = (int16_t)(&P->stack[0][0])[e];
-    for (int e = threadIdx.x; e < 200; e += NTHR) {
-      (&E.mmI[0][0][0])[e] = (int16_t)(&P->mmI[0][0][0])[e];
-      (&E.mm1nI[0][0][0])[e] = (int16_t)(&P->mm1nI[0][0][0])[e];
-      (&E.mm23I[0][0][0])[e] = (int16_t)(&P->mm23I[0][0][0])[e];
-    }
-    for (int e = threadIdx.x; e < 31; e += NTHR) E.bulge[e] = (int16_t)P->bulge[e];
-    if (threadIdx.x == 0) {
-      E.term_au = P->term_au;
-      E.il23 = P->il5 + P->ninio0;
-    }
-  }
+  __device__ static void init(const TriArgs& a, Smem& sm) { load_tab(a, sm.E); }
 
   // sequence-only terms of the cells of band [ld0, ld0+G) and the compacted
   // list of cells whose cl can close bulges / interior loops; all threads of
@@ -611,14 +613,15 @@ struct GZu {
   // and right bulges of size a.  Lane 30: (1,1) and (2,2); lane 31: (2,3) and
   // (3,2).  B(l,k) = cl(l,k) + interior mismatch of (l,k), so
   // cl(l,k) = B(l,k) - inner(l,k) (librna_impl.h:199-265).
-  __device__ static int window_lane(const ZuView& z, Smem& sm, const ZuPre& pc, int ap, int i, int j, int r, int c,
-                                    const int* fg, int f_a1, int f_1a) {
+  template <class S>
+  __device__ static int window_lane(const ZuView& z, S& sm, const ZuPre& pc, int ap, int i, int j, int r, int c,
+                                    const int* fg, int f_a1, int f_1a, int amax = 30) {
     const ZuTab& E = sm.E;
     // outer pair (i+1, j-2) of the loops and its mismatch bases
     const int Ri1 = z.R(i + 1), Ri2 = z.R(i + 2), Cj2 = z.C(j - 2), Cj3 = z.C(j - 3);
     const int ocb = bpc(Ri1, Cj2);
     int best = INV;
-    if (ap < 30) {
+    if (ap < amax) {
       const int aa = ap + 1;
       const int* Brow = &sm.u.w.B[r + 2 + aa][c + 30];  // Brow[-b] = B(i+2+aa, j-2-b)
       int g = INV;
@@ -655,6 +658,8 @@ struct GZu {
         const int e = aa == 1 ? E.bulge[1] + E.stack[ocb][icb] : E.bulge[aa] + tauo + z.tau(Ck1, Ri2);
         if (Bv < INV_THR) best = imin(best, cl + e + pc.sr);
       }
+    } else if (ap < 30) {
+      // rows beyond amax: nothing
     } else if (ap == 30) {
       // 1x1 and 2x2 loops (int11, int22)
       const int B11 = sm.u.w.B[r + 3][c + 29], B22 = sm.u.w.B[r + 4][c + 28];
@@ -873,12 +878,300 @@ struct GZu {
   }
 };
 
+// ============================================================== RNAfold
+// RNAfold grammar with the MFE algebra (RNAFold.scala:20-64), tables per tile
+// Q, QM, Qp, B:
+//   Qp(i,j) [pair (i,j-1)] = min{ hl(i,j-1); sr(i,j-1) + Qp(i+1,j-1);
+//                                 il(i,k,l-1,j-1) + Qp(k,l), 1<=k-i<=30, 1<=j-l<=30; QM(i+1,j-1) }
+//   QM(i,j) = min_k Q(i,k) + Q(k,j)                                   [j-i >= 4]
+//   Q(i,j)  = min{ QM(i,j); Q(i,j-1); Q(i+1,j); Qp(i,j) }               [j-i >= 2]
+// The interior-loop window of Qp(i,j) (outer pair (i, j-1), left / right
+// unpaired sizes sl, sr in [0,29]) is the loop window of the Zuker closed
+// table at (i-1, j+1) with sizes a = sl, b = sr, the sl = 0 / sr = 0 rows
+// being its bulges; so the same 32-lane evaluation is used with the stacked
+// pair's energy set to zero and sizes capped at 29.  (0,0) is the stack rule.
+struct GRf {
+  static constexpr int BR = 64, BS = 65;
+  // window candidates of a cell on diagonal d read diagonals <= d-3 (except
+  // (0,0), the stack), the split points of QM <= d-3: bands of 3 diagonals
+  static constexpr int G = 3;
+  struct Smem {
+    union {
+      int stage[NWAVE * (T * (T + 1) + T * T + T)];
+      struct {
+        int B[BR][BS];          // Qp + inner mismatch, origin (IT, JT-32)
+        int Q[T + 1][T + 1];    // origin (IT, JT-1); row T = frontier row IT+32
+        int QM[T + 1][T + 1];   // origin (IT, JT-1)
+        int RQ[T][T + 1];       // Q of tile (J,J)
+      } w;
+    } u;
+    int X[T][T + 1];
+    ZuTab E;
+    ZuPre pre[2][G][T];
+    uint8_t rs[72], cs[72];
+    int acc[G][T];
+    int16_t list[G * T];
+    int nlist;
+  };
+
+  __device__ static void init(const TriArgs& a, Smem& sm) { load_tab(a, sm.E); }
+
+  // Sequence-only terms of cell (i,j) for the RNAfold tables (ZuPre reused):
+  // valid = pair (i, j-1) canonical; r1 = hl_energy(i, j-1); r5c = the stack
+  // energy sr(i, j-1); sr = 0 (loops of this grammar add no stacked pair);
+  // outI = generic outer mismatch of (i, j-1); boff / in0 = interior
+  // mismatches of (i,j) and (i+1,j-1) as inner pairs; f11..f22 the int-table
+  // loops minus their inner mismatch (sizes (1,1), (1,2), (2,1), (2,2)).
+  __device__ static void pre(const ZuView& z, int i, int j, ZuPre* o) {
+    const int d = j - i, n = z.n;
+    o->valid = 0;
+    if (j > n || d < 2 || bpc(z.R(i), z.C(j - 1)) == 7) return;
+    o->valid = 1;
+    const DevRna* P = z.P;
+    const ZuTab& E = z.E;
+    const int Ri = z.R(i), Ri1 = z.R(i + 1), Ri2 = z.R(i + 2), Ri3 = z.R(i + 3);
+    const int Cj1 = z.C(j - 1), Cj2 = z.C(j - 2), Cj3 = z.C(j - 3), Cj4 = z.C(j - 4);
+    const int ocb = bpc(Ri, Cj1);
+    // hl_energy(i, j-1) (librna_impl.h:181-193): size d-2
+    const int size = d - 2;
+    int hp = 600;
+    if (size >= 3) {
+      const int ent = size <= 30 ? P->hairpin[size] : z.hl[size];
+      const int hmm = P->mmH[ocb][Ri1][Cj2];
+      hp = size == 3 ? ent + z.tau(Ri, Cj1) : ent + hmm;
+      const int cls = size == 3 ? 0 : size == 4 ? 1 : size == 6 ? 2 : -1;
+      if (cls >= 0) {
+        int key = 0;
+        for (int m = 0; m <= size + 1; ++m) key = key * 8 + z.R(i + m);
+        for (int e = 0; e < P->n_loop[cls]; ++e)
+          if (P->loop_key[cls][e] == key) {
+            hp = P->loop_e[cls][e];
+            break;
+          }
+      }
+    }
+    int e11 = 0, e12 = 0, e21 = 0, e22 = 0;
+    if (d >= 7) {  // (sl, sr) = (1,1): inner pair (i+2, j-3)
+      e11 = P->int11[ocb][bpc(Cj3, Ri2)][Ri1][Cj2];
+      e12 = P->int21[ocb][bpc(Cj4, Ri2)][Ri1][Cj3][Cj2];
+      e21 = P->int21[bpc(Cj3, Ri3)][ocb][Cj2][Ri1][Ri2];
+      e22 = P->int22[ocb][bpc(Cj4, Ri3)][Ri1][Ri2][Cj3][Cj2];
+    }
+    o->r1 = d >= 3 ? hp : 0;
+    o->r5c = E.stack[ocb][bpc(Cj2, Ri1)];
+    o->sr = 0;
+    o->uoff = 0;
+    o->boff = (i >= 1 && j <= n - 1) ? z.inner(i, j) : kNoB;
+    o->outI = E.mmI[ocb][Ri1][Cj2];
+    o->in0 = d >= 4 ? z.inner(i + 1, j - 1) : 0;
+    if (d >= 7) {
+      o->f11 = (int16_t)(e11 - z.inner(i + 2, j - 2));
+      o->f12 = (int16_t)(e12 - z.inner(i + 2, j - 3));
+      o->f21 = (int16_t)(e21 - z.inner(i + 3, j - 2));
+      o->f22 = (int16_t)(e22 - z.inner(i + 3, j - 3));
+    }
+  }
+
+  __device__ static void prep_band(const ZuView& z, int IT, int JT, int ld0, Smem& sm, int t, int buf) {
+    if (t < 64) {
+      int base = 0;
+      for (int q = 0; q < G; ++q) {
+        const int ld = ld0 + q;
+        bool ok = false;
+        if (ld <= T - 1) {
+          const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
+          if (t < ncell) {
+            const int r = rmin + t, i = IT + r, j = JT + r + ld;
+            ok = j <= z.n && j - i >= 6 && bpc(z.R(i), z.C(j - 1)) != 7;
+          }
+        }
+        const unsigned long long m = __ballot(ok);
+        if (ok) sm.list[base + __popcll(m & ((1ull << t) - 1))] = (int16_t)(q * T + t);
+        base += __popcll(m);
+      }
+      if (t == 0) sm.nlist = base;
+    }
+    const int e = t < 64 ? t + 128 : t - 64;  // the band's G*T = 96 cells
+    if (e < G * T) {
+      const int q = e / T, p = e % T, ld = ld0 + q;
+      const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
+      if (ld <= T - 1 && p < ncell) pre(z, IT + rmin + p, JT + rmin + p + ld, &sm.pre[buf][q][p]);
+    }
+  }
+
+  __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm) {
+    const int tid = threadIdx.x, IT = I * T, JT = J * T, n = in.n, nb = in.nb;
+    const uint8_t* seq = a.codes + in.seq_off;
+    const DevRna* P = (const DevRna*)a.rna;
+    const ZuView z{sm.E, P, a.hl_ent, sm.rs, sm.cs, IT - 1, JT - 34, n};
+    int acc[4][4];
+    for (int t = 0; t < 4; ++t)
+      for (int u = 0; u < 4; ++u) acc[t][u] = INV;
+    for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = INV;
+    for (int e = tid; e < G * T; e += NTHR) (&sm.acc[0][0])[e] = INV;
+    if (tid < 72) {
+      const int x = IT - 1 + tid, y = JT - 34 + tid;
+      sm.rs[tid] = (x >= 0 && x < n) ? seq[x] : 0;
+      sm.cs[tid] = (y >= 0 && y < n) ? seq[y] : 0;
+    }
+    // QM split points in blocks strictly between I and J: Q (x) Q
+    if (J - I >= 2) middle_product<false, false>(a, in, NT_RF, RT_Q, RT_Q, I, J, sm.u.stage, acc);
+    __syncthreads();
+    {
+      const int lane = tid & 63, r0 = (lane >> 3) * 4, c0 = (lane & 7) * 4;
+      if (J - I >= 2)
+        for (int t = 0; t < 4; ++t)
+          for (int u = 0; u < 4; ++u)
+            if (acc[t][u] < INV_THR) atomicMin(&sm.X[r0 + t][c0 + u], acc[t][u]);
+    }
+    auto fetch = [&](int tab, int i, int j) -> int {
+      if (i < 0 || j < 0 || i > j || j > n) return INV;
+      const int I2 = i / T, J2 = j / T;
+      if (I2 >= nb || J2 >= nb) return INV;
+      return rec_ptr(a, in, NT_RF, I2, J2)[tab * T * T + (i % T) * T + (j % T)];
+    };
+    {
+      const int* qb[3] = {J - 1 >= I ? rec_ptr(a, in, NT_RF, I, J - 1) + RT_B * T * T : nullptr,
+                          I + 1 <= J - 1 ? rec_ptr(a, in, NT_RF, I + 1, J - 1) + RT_B * T * T : nullptr,
+                          I + 1 <= J ? rec_ptr(a, in, NT_RF, I + 1, J) + RT_B * T * T : nullptr};
+      for (int quad = 0; quad < 3; ++quad) {
+        const int x = tid * 4, rr = x / T + (quad ? T : 0), cc = x % T + (quad == 2 ? T : 0);
+        const int4 v = qb[quad] ? *(const int4*)(qb[quad] + x) : make_int4(INV, INV, INV, INV);
+        int* dst = &sm.u.w.B[rr][cc];
+        dst[0] = v.x;
+        dst[1] = v.y;
+        dst[2] = v.z;
+        dst[3] = v.w;
+      }
+      for (int e = tid; e < T * T; e += NTHR) sm.u.w.B[e / T][T + e % T] = INV;
+    }
+    for (int e = tid; e < (T + 1) * (T + 1); e += NTHR) {
+      const int rr = e / (T + 1), cc = e % (T + 1);
+      const bool halo = rr == T || cc == 0;
+      sm.u.w.Q[rr][cc] = halo ? fetch(RT_Q, IT + rr, JT - 1 + cc) : INV;
+      sm.u.w.QM[rr][cc] = halo ? fetch(RT_QM, IT + rr, JT - 1 + cc) : INV;
+    }
+    const int* gLQ = rec_ptr(a, in, NT_RF, I, I) + RT_Q * T * T;  // Q tile (I,I), read in place
+    if (I < J) {
+      const int* gR = rec_ptr(a, in, NT_RF, J, J) + RT_Q * T * T;
+      for (int e = tid; e < T * T; e += NTHR) sm.u.w.RQ[e / T][e % T] = gR[e];
+    }
+    const int dlo = I == J ? 0 : -(T - 1);
+    __syncthreads();
+    if (tid >= 64) prep_band(z, IT, JT, dlo, sm, tid - 64, 0);
+    __syncthreads();
+    int32_t* gqp = rec_ptr(a, in, NT_RF, I, J) + RT_QP * T * T;
+    // interior-loop lane of this thread (sizes capped at 29: row 30 and
+    // column 30 of the loop energies masked)
+    const int ap = tid & 31, aa = ap + 1;
+    int fg[29];
+#pragma unroll
+    for (int b = 2; b <= 30; ++b) {
+      const bool special = ap >= 29 || aa == 1 || (aa == 2 && b <= 3) || (aa == 3 && b == 2) || b == 30;
+      fg[b - 2] = special ? GZu::kMask : P->F[ap < 29 ? aa : 1][b];
+    }
+    const int f_a1 = P->F[ap < 29 ? aa : 1][1], f_1a = P->F[1][ap < 29 ? aa : 1];
+    for (int ld0 = dlo, buf = 0; ld0 <= T - 1; ld0 += G, buf ^= 1) {
+      const int qn = imin(G, T - ld0);
+      int cpre[G + 1];
+      cpre[0] = 0;
+      for (int q = 0; q < G; ++q) cpre[q + 1] = cpre[q] + (q < qn ? T - abs(ld0 + q) : 0);
+      const int nv = sm.nlist, wl = ((nv + 1) >> 1) * 64, total = wl + 2 * cpre[G];
+      for (int w = tid; w < total; w += NTHR) {
+        if (w < wl) {
+          const int v = w >> 5;
+          if (v >= nv) continue;
+          const int qp = sm.list[v], q = qp / T, p = qp % T, ld = ld0 + q;
+          const int r = (ld < 0 ? -ld : 0) + p, c = r + ld, i = IT + r, j = JT + c;
+          // the Zuker loop window of the cell (i-1, j+1) with sizes <= 29
+          const int best = GZu::window_lane(z, sm, sm.pre[buf][q][p], ap, i - 1, j + 1, r - 1, c + 1, fg, f_a1, f_1a, 29);
+          if (ap == 0) sm.acc[q][p] = best;
+        } else {
+          // QM split points inside blocks I and J (Q(i,k) and Q(k,j) are
+          // empty for j-k < 3 and k-i < 3, so all 32 k of a block are taken)
+          const int e = (w - wl) >> 1, part = w & 1;
+          int q = 0;
+          while (e >= cpre[q + 1]) ++q;
+          const int ld = ld0 + q, r = (ld < 0 ? -ld : 0) + (e - cpre[q]), c = r + ld;
+          int best = INV;
+          if (part == 0) {
+            if (I == J) {
+#pragma unroll 8
+              for (int x = 0; x < T; ++x) best = imin(best, sm.u.w.Q[r][x + 1] + sm.u.w.Q[x][c + 1]);
+            } else {
+              const int4* g4 = (const int4*)(gLQ + r * T);
+#pragma unroll 2
+              for (int x4 = 0; x4 < T / 4; ++x4) {
+                const int4 u = g4[x4];
+                const int x = x4 * 4;
+                best = imin(imin(best, u.x + sm.u.w.Q[x][c + 1]), u.y + sm.u.w.Q[x + 1][c + 1]);
+                best = imin(imin(best, u.z + sm.u.w.Q[x + 2][c + 1]), u.w + sm.u.w.Q[x + 3][c + 1]);
+              }
+            }
+          } else if (I < J) {
+#pragma unroll 8
+            for (int x = 0; x < T; ++x) best = imin(best, sm.u.w.Q[r][x + 1] + sm.u.w.RQ[x][c]);
+          }
+          best = imin(best, __shfl_xor(best, 1));
+          if (part == 0 && best < INV_THR) sm.X[r][c] = imin(sm.X[r][c], best);
+        }
+      }
+      __syncthreads();
+      // light phase: wave 0 walks the band's diagonals; waves 1-3 prepare the next band
+      if (tid >= 64) {
+        if (ld0 + G <= T - 1) prep_band(z, IT, JT, ld0 + G, sm, tid - 64, buf ^ 1);
+      } else {
+        for (int q = 0; q < qn; ++q) {
+          const int ld = ld0 + q, rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
+          if (tid < ncell) {
+            const int p = tid, r = rmin + p, c = r + ld, i = IT + r, j = JT + c, d = j - i;
+            int qp = INV, qm = INV, qv = INV, Bv = INV;
+            if (i <= j && j <= n) {
+              const ZuPre& pc = sm.pre[buf][q][p];
+              if (pc.valid) {
+                qp = imin(sm.acc[q][p], d >= 3 ? (int)pc.r1 : INV);
+                const int Bi = sm.u.w.B[r + 1][c + 31];  // B(i+1, j-1): the stack
+                if (d >= 4 && Bi < INV_THR) qp = imin(qp, Bi - pc.in0 + pc.r5c);
+                const int mlv = sm.u.w.QM[r + 1][c];      // QM(i+1, j-1): the multiloop
+                if (mlv < INV_THR) qp = imin(qp, mlv);
+                if (qp >= INV_THR) qp = INV;
+                if (qp < INV_THR && pc.boff != kNoB) Bv = qp + pc.boff;
+              }
+              qm = sm.X[r][c];
+              if (qm >= INV_THR || d < 4) qm = INV;
+              if (d >= 2) {
+                qv = imin(imin(qm, qp), imin(sm.u.w.Q[r][c], sm.u.w.Q[r + 1][c + 1]));
+                if (qv >= INV_THR) qv = INV;
+              }
+            }
+            sm.u.w.B[r][c + 32] = Bv;
+            sm.u.w.QM[r][c + 1] = qm;
+            sm.u.w.Q[r][c + 1] = qv;
+            gqp[r * T + c] = qp;
+            sm.acc[q][p] = INV;
+          }
+          wave_lds_sync();
+        }
+      }
+      __syncthreads();
+    }
+    int32_t* g = rec_ptr(a, in, NT_RF, I, J);
+    for (int e = tid; e < T * T; e += NTHR) {
+      const int r = e / T, c = e % T;
+      g[RT_Q * T * T + e] = sm.u.w.Q[r][c + 1];
+      g[RT_QM * T * T + e] = sm.u.w.QM[r][c + 1];
+      g[RT_B * T * T + e] = sm.u.w.B[r][T + c];
+    }
+  }
+};
+
 // ------------------------------------------------------------------ launch
 int fill_occupancy(int grammar) {
   int blocks = 0;
   hipError_t e;
   if (grammar == G_NUSSINOV) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, tri_fill<GNu>, NTHR, 0);
   else if (grammar == G_MATCHAIN) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, tri_fill<GMc>, NTHR, 0);
+  else if (grammar == G_RNAFOLD) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, tri_fill<GRf>, NTHR, 0);
   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, tri_fill<GZu>, NTHR, 0);
   return e == hipSuccess ? blocks : 0;
 }
@@ -888,6 +1181,7 @@ int launch_fill(int grammar, const TriArgs& a, const unsigned long long* seeds, 
   hipLaunchKernelGGL(tri_reset, dim3((a.total_tiles + 255) / 256), dim3(256), 0, st, a, seeds, nseed);
   if (grammar == G_NUSSINOV) hipLaunchKernelGGL(tri_fill<GNu>, dim3(grid), dim3(NTHR), 0, st, a);
   else if (grammar == G_MATCHAIN) hipLaunchKernelGGL(tri_fill<GMc>, dim3(grid), dim3(NTHR), 0, st, a);
+  else if (grammar == G_RNAFOLD) hipLaunchKernelGGL(tri_fill<GRf>, dim3(grid), dim3(NTHR), 0, st, a);
   else hipLaunchKernelGGL(tri_fill<GZu>, dim3(grid), dim3(NTHR), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

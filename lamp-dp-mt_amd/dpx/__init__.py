@@ -16,14 +16,15 @@ OK, EINVAL, ENOMEM, EHIP, ENOPARAMS, EPARAMS, ETRACE, EBUSY, EAGAIN = range(9)
 # exported symbols, as declared in include/dpx.h
 EXPORTS = [
     "dpx_create", "dpx_destroy", "dpx_last_error", "dpx_set_rna_params",
-    "dpx_nussinov", "dpx_zuker", "dpx_swat", "dpx_matchain",
-    "dpx_zuker_batch", "dpx_nussinov_batch", "dpx_last_timing", "dpx_now",
+    "dpx_nussinov", "dpx_zuker", "dpx_rnafold", "dpx_swat", "dpx_matchain",
+    "dpx_zuker_batch", "dpx_nussinov_batch", "dpx_rnafold_batch", "dpx_last_timing", "dpx_now",
     "dpx_server_create", "dpx_server_destroy", "dpx_server_submit", "dpx_server_poll",
     "dpx_server_trace", "dpx_server_get_stats",
     "dpx_plan_create", "dpx_plan_run", "dpx_plan_results", "dpx_plan_destroy",
 ]
 
-REQ_NUSSINOV, REQ_ZUKER, REQ_SWAT, REQ_MATCHAIN = 1, 2, 3, 4
+REQ_NUSSINOV, REQ_ZUKER, REQ_SWAT, REQ_MATCHAIN, REQ_RNAFOLD = 1, 2, 3, 4, 5
+REQ_OF_KIND = {"nussinov": REQ_NUSSINOV, "zuker": REQ_ZUKER, "rnafold": REQ_RNAFOLD}
 SUBMIT_TRACE, SUBMIT_TRACE_FROM_BEST = 1, 2
 
 
@@ -156,6 +157,14 @@ class Context:
         self._check(self._L.dpx_zuker(self.h, seq.encode(), n, ctypes.byref(score), tr, cap, ctypes.byref(tl)))
         return score.value, (_tuples(tr, tl.value) if trace else None)
 
+    def rnafold(self, seq, trace=True):
+        n = len(seq)
+        cap = 4 * n + 8
+        tr = (Trace * cap)() if trace else None
+        score, tl = ctypes.c_int32(), ctypes.c_int()
+        self._check(self._L.dpx_rnafold(self.h, seq.encode(), n, ctypes.byref(score), tr, cap, ctypes.byref(tl)))
+        return score.value, (_tuples(tr, tl.value) if trace else None)
+
     def matchain(self, rows, cols, trace=True):
         n = len(rows)
         cap = 2 * n + 8
@@ -210,6 +219,9 @@ class Context:
     def nussinov_batch(self, seqs, min_loop=0, trace=True):
         return self._batch(self._L.dpx_nussinov_batch, seqs, trace, ctypes.c_int(min_loop))
 
+    def rnafold_batch(self, seqs, trace=True):
+        return self._batch(self._L.dpx_rnafold_batch, seqs, trace)
+
     def plan(self, seqs, kind="zuker", min_loop=0, trace=True):
         """A batch uploaded once and replayed as a HIP graph (dpx_plan_*)."""
         return Plan(self, seqs, kind, min_loop, trace)
@@ -257,6 +269,10 @@ class Server:
 
     def submit_zuker(self, seq, trace=True):
         return self._submit(Request(kind=REQ_ZUKER, flags=SUBMIT_TRACE if trace else 0, a=seq.encode(), n1=len(seq)))
+
+    def submit_rnafold(self, seq, trace=True):
+        return self._submit(Request(kind=REQ_RNAFOLD, flags=SUBMIT_TRACE if trace else 0, a=seq.encode(),
+                                    n1=len(seq)))
 
     def submit_nussinov(self, seq, min_loop=0, trace=True):
         return self._submit(Request(kind=REQ_NUSSINOV, flags=SUBMIT_TRACE if trace else 0, a=seq.encode(),
@@ -324,7 +340,7 @@ class Server:
 
 
 class Plan:
-    """dpx_plan_*: a Zuker or Nussinov batch captured once as a HIP graph."""
+    """dpx_plan_*: a Zuker, Nussinov or RNAfold batch captured once as a HIP graph."""
 
     def __init__(self, ctx, seqs, kind="zuker", min_loop=0, trace=True):
         self._L = ctx._L
@@ -344,7 +360,7 @@ class Plan:
         self.toffs[self.count] = t
         self.total = t
         h = ctypes.c_void_p()
-        k = REQ_ZUKER if kind == "zuker" else REQ_NUSSINOV
+        k = REQ_OF_KIND[kind]
         ctx._check(self._L.dpx_plan_create(ctx.h, k, blob, offs, self.count, min_loop, 1 if trace else 0,
                                            ctypes.byref(h)))
         self.h = h

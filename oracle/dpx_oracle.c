@@ -490,6 +490,107 @@ int32_t orc_zuker(const orc_params_t* P, const uint8_t* seq, int n,
 /* ======================================================================== */
 /*  Nussinov (bench2/nu_kern.h:23-45; nu.c:17-73; Nussinov.scala:49-76)     */
 /* ======================================================================== */
+/* RNAfold grammar (RNAFold.scala:20-64); restated from the grammar and the
+ * C code generator's conventions (CodeGen.scala:100-208: alternatives in
+ * order, split indices ascending with the outer split outside, a candidate
+ * replaces the best only when strictly smaller, the first always). */
+
+typedef struct { int32_t c; int32_t rule, p0, p1; } rf_slot_t;
+
+int32_t orc_rnafold(const orc_params_t* P, const uint8_t* seq, int n,
+                    orc_trace_t* trace, int trace_cap, int* trace_len, int32_t* tables) {
+  if (n < 0 || !P) return INT32_MIN;
+  for (int q = 0; q < n; q++) if (seq[q] < 1 || seq[q] > 4) return INT32_MIN;
+  const int W = n + 1;
+  orc_rna_t R = {P, seq, n};
+  const orc_rna_t* r = &R;
+  rf_slot_t* Q = (rf_slot_t*)malloc(sizeof(rf_slot_t) * (size_t)W * W);
+  rf_slot_t* QM = (rf_slot_t*)malloc(sizeof(rf_slot_t) * (size_t)W * W);
+  rf_slot_t* QP = (rf_slot_t*)malloc(sizeof(rf_slot_t) * (size_t)W * W);
+  if (!Q || !QM || !QP) { free(Q); free(QM); free(QP); return INT32_MIN; }
+  for (size_t q = 0; q < (size_t)W * W; q++) {
+    Q[q].c = QM[q].c = QP[q].c = RF_EMPTY;
+    Q[q].rule = QM[q].rule = QP[q].rule = -1;
+  }
+#define RX(a, b) ((size_t)(a) * W + (b))
+#define RTAKE(slot, val, rl, q0, q1) do { int _c = (val); if ((slot).rule == -1 || _c < (slot).c) { \
+    (slot).c = _c; (slot).rule = (rl); (slot).p0 = (q0); (slot).p1 = (q1); } } while (0)
+  for (int d = 0; d <= n; ++d) {
+    for (int i = 0; i + d <= n; ++i) {
+      const int j = i + d;
+      /* Qp: basepairing (ADPParsers.scala:96) */
+      if (i + 2 <= j && BP(i, j - 1) != 7) {
+        rf_slot_t* s = &QP[RX(i, j)];
+        if (j - i >= 3) RTAKE(*s, orc_hl_energy(r, i, j - 1), 5, 0, 0);
+        if (QP[RX(i + 1, j - 1)].rule != -1 && i + 1 <= j - 1)
+          RTAKE(*s, orc_sr_energy(r, i, j - 1) + QP[RX(i + 1, j - 1)].c, 6, 0, 0);
+        {
+          const int l0 = i + 4 > j - 30 ? i + 4 : j - 30;
+          for (int l = l0; l <= j - 1; ++l) {
+            const int k1 = i + 30 < l - 3 ? i + 30 : l - 3;
+            for (int k = i + 1; k <= k1; ++k)
+              if (QP[RX(k, l)].rule != -1) RTAKE(*s, orc_il_energy(r, i, k, l - 1, j - 1) + QP[RX(k, l)].c, 7, k, l);
+          }
+        }
+        if (i + 1 <= j - 1 && QM[RX(i + 1, j - 1)].rule != -1) RTAKE(*s, QM[RX(i + 1, j - 1)].c, 8, 0, 0);
+      }
+      /* QM: Q ~ Q, filter length(4) */
+      if (j - i >= 4) {
+        rf_slot_t* s = &QM[RX(i, j)];
+        for (int k = i + 3; k <= j - 3; ++k)
+          if (Q[RX(i, k)].rule != -1 && Q[RX(k, j)].rule != -1) RTAKE(*s, Q[RX(i, k)].c + Q[RX(k, j)].c, 4, k, 0);
+      }
+      /* Q: filter length(2) */
+      if (j - i >= 2) {
+        rf_slot_t* s = &Q[RX(i, j)];
+        if (QM[RX(i, j)].rule != -1) RTAKE(*s, QM[RX(i, j)].c, 0, 0, 0);
+        if (Q[RX(i, j - 1)].rule != -1) RTAKE(*s, Q[RX(i, j - 1)].c, 1, 0, 0);
+        if (Q[RX(i + 1, j)].rule != -1) RTAKE(*s, Q[RX(i + 1, j)].c, 2, 0, 0);
+        if (QP[RX(i, j)].rule != -1) RTAKE(*s, QP[RX(i, j)].c, 3, 0, 0);
+      }
+    }
+  }
+  const int32_t res = Q[RX(0, n)].rule == -1 ? RF_EMPTY : Q[RX(0, n)].c;
+  if (tables) {
+    for (size_t q = 0; q < (size_t)W * W; q++) {
+      tables[3 * q + 0] = Q[q].rule == -1 ? RF_EMPTY : Q[q].c;
+      tables[3 * q + 1] = QM[q].rule == -1 ? RF_EMPTY : QM[q].c;
+      tables[3 * q + 2] = QP[q].rule == -1 ? RF_EMPTY : QP[q].c;
+    }
+  }
+  if (trace && trace_len) {
+    int rd = 0, wr = 0;
+#define RPUSH(a, b, rl) do { if (wr < trace_cap) { trace[wr].i = (a); trace[wr].j = (b); trace[wr].rule = (rl); trace[wr].pos[0] = trace[wr].pos[1] = 0; } wr++; } while (0)
+    if (res != RF_EMPTY) RPUSH(0, n, 0);
+    for (; rd < wr && rd < trace_cap; ++rd) {
+      orc_trace_t* t = &trace[rd];
+      const rf_slot_t* s = t->rule <= 3 ? &Q[RX(t->i, t->j)] : t->rule == 4 ? &QM[RX(t->i, t->j)] : &QP[RX(t->i, t->j)];
+      const int i = t->i, j = t->j;
+      t->rule = s->rule;
+      if (t->rule == 4) t->pos[0] = s->p0;
+      if (t->rule == 7) { t->pos[0] = s->p0; t->pos[1] = s->p1; }
+      switch (t->rule) {
+        case 0: RPUSH(i, j, 4); break;
+        case 1: RPUSH(i, j - 1, 0); break;
+        case 2: RPUSH(i + 1, j, 0); break;
+        case 3: RPUSH(i, j, 5); break;
+        case 4: RPUSH(i, s->p0, 0); RPUSH(s->p0, j, 0); break;
+        case 6: RPUSH(i + 1, j - 1, 5); break;
+        case 7: RPUSH(s->p0, s->p1, 5); break;
+        case 8: RPUSH(i + 1, j - 1, 4); break;
+        default: break;
+      }
+    }
+    *trace_len = wr;
+#undef RPUSH
+  }
+#undef RX
+#undef RTAKE
+  free(Q); free(QM); free(QP);
+  return res;
+}
+
+/* ======================================================================== */
 
 int32_t orc_nussinov(const uint8_t* seq, int n, int min_loop,
                      orc_trace_t* trace, int trace_cap, int* trace_len, int32_t* table) {

@@ -72,6 +72,23 @@ int32_t orc_zuker(const orc_params_t* P, const uint8_t* seq, int n,
                   orc_trace_t* trace, int trace_cap, int* trace_len,
                   int32_t* tables /* optional (n+1)^2 x 4: cl,ml,ml1,st */);
 
+/* RNAfold grammar with the MFE algebra (v4/examples/RNAFold.scala:20-64):
+ *   Qp(i,j) [pair (i,j-1)] = min{ hairpin hl(i,j-1) [5]; stack sr(i,j-1)+Qp(i+1,j-1) [6];
+ *                                 iloop il(i,k,l-1,j-1)+Qp(k,l), 1<=k-i<=30, 1<=j-l<=30 [7];
+ *                                 mloop QM(i+1,j-1) [8] }
+ *   QM(i,j) [j-i>=4] = min_k Q(i,k)+Q(k,j) [4]
+ *   Q(i,j) [j-i>=2]  = min{ QM(i,j) [0]; Q(i,j-1) [1]; Q(i+1,j) [2]; Qp(i,j) [3] }
+ * Rule ids as the reference's analyze numbers them (tables sorted by name:
+ * Q 0-3, QM 4, Qp 5-8; BaseParsers.scala:140).  Answer Q(0,n); RF_EMPTY when
+ * no structure exists.  Candidates in the order of the generated C code
+ * (alternatives in order, split loops ascending, the outer split of
+ * seq ~ Qp ~ seq (l) outside the inner one (k)); first optimum wins.
+ * Trace: BFS from (0,n) like the other grammars; pos = (k, l) for rule 7, k
+ * for rule 4.  tables (optional): (n+1)^2 x 3 = Q, QM, Qp. */
+#define RF_EMPTY 999999
+int32_t orc_rnafold(const orc_params_t* P, const uint8_t* seq, int n,
+                    orc_trace_t* trace, int trace_cap, int* trace_len, int32_t* tables);
+
 /* Nussinov (bench2/nu_kern.h:23-45, nu.c:17-73).  seq codes 1..4.
  * min_loop = 0 is the reference grammar (pair needs i+2<=j); min_loop = m
  * additionally requires m unpaired bases inside a pair. */

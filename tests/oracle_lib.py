@@ -118,3 +118,40 @@ def matchain(rows, cols, trace=True):
     f.restype = ctypes.c_int32
     score = f(r, c, n, tr if trace else None, cap, ctypes.byref(tl))
     return score, (_trace_list(tr, tl.value, tl) if trace else None)
+
+
+RF_EMPTY = 999999
+
+
+def rnafold(seq, trace=True):
+    n = len(seq)
+    cap = 8 * (n + 2)
+    tr = (Trace * cap)()
+    tl = ctypes.c_int(0)
+    f = lib().orc_rnafold
+    f.restype = ctypes.c_int32
+    score = f(params(), encode(seq), n, tr if trace else None, cap, ctypes.byref(tl), None)
+    return score, (_trace_list(tr, tl.value, tl) if trace else None)
+
+
+class Rna(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_void_p), ("s", ctypes.c_void_p), ("n", ctypes.c_int)]
+
+
+def energy_fns(seq):
+    """(hl_energy, il_energy, sr_energy) of the oracle bound to seq."""
+    codes = encode(seq)
+    r = Rna(ctypes.cast(params(), ctypes.c_void_p), ctypes.cast(codes, ctypes.c_void_p), len(seq))
+    L = lib()
+    keep = (codes, r)
+
+    def hl(i, j):
+        return L.orc_hl_energy(ctypes.byref(keep[1]), i, j)
+
+    def il(i, k, l, j):
+        return L.orc_il_energy(ctypes.byref(keep[1]), i, k, l, j)
+
+    def sr(i, j):
+        return L.orc_sr_energy(ctypes.byref(keep[1]), i, j)
+
+    return hl, il, sr
