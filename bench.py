@@ -22,7 +22,8 @@ including PCIe copies, the roofline of the fill kernel and a single-thread
 CPU baseline (the oracle, a bounded sample of the same sequences).
 
 --workload selects one of the single-instance configurations instead
-(zuker2048, nussinov1024, swat16384); each prints its own JSON line.
+(zuker2048, rnafold2048, nussinov1024, swat16384); each prints its own JSON
+line.
 """
 import argparse
 import json
@@ -215,6 +216,12 @@ def single(args):
         n_cells = cells(1024)
         line["config"] = {"workload": "Nussinov n=1024, i.i.d. uniform ACGU seed 0, min hairpin loop 3, score + backtrace"}
         check = lambda r: (r[0], [t[:4] for t in r[1]]) == (lambda x: (x[0], [t[:4] for t in x[1]]))(o.nussinov(s, min_loop=3))
+    elif args.workload == "rnafold2048":
+        s = gen.rna_gc(2048, 2)
+        run = lambda: ctx.rnafold(s)
+        n_cells = cells(2048)
+        line["config"] = {"workload": "RNAfold grammar MFE n=2048, i.i.d. 50% GC seed 2, Turner 2004, MFE + backtrace"}
+        check = None  # the oracle takes about 20 s here; the GPU tests pin this input against a committed fixture
     else:
         s = gen.rna_gc(2048, 2)
         run = lambda: ctx.zuker(s)
@@ -249,7 +256,7 @@ def single(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="batch", choices=["batch", "zuker2048", "nussinov1024", "swat16384"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "zuker2048", "rnafold2048", "nussinov1024", "swat16384"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)

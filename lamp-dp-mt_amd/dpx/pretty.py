@@ -12,6 +12,7 @@ pretty algebra of its grammar:
   nussinov(seq, trace)     -> dot-bracket string   (NussinovString, Nussinov.scala:27-34)
   matchain(rows, cols, tr) -> "((|10x100|*|100x5|)*|5x50|)"  (MatrixPrettyPrint, MatrixMult.scala:39-44)
   swat(a, b, trace)        -> (track 1, track 2)   (SWatPretty, SWatAffine.scala:36-47)
+  rnafold(seq, trace)      -> dot-bracket string   (RNAFoldPrettyPrint, RNAFold.scala:31-41)
 
 Rule ids are the reference's (bench2/zu.h:2-5, bench2/nu.h, dp_swat/comp1.h).
 """
@@ -71,6 +72,43 @@ def zuker(seq, trace):
         ans[(tab, i, j)] = v
     i, j = trace[0][0], trace[0][1]
     return ans[("st", i, j)]
+
+
+def rnafold(seq, trace):
+    """Dot-bracket structure of an RNAfold backtrace (None for the empty answer).
+
+    Rule ids: Q 0 QM, 1 left, 2 right, 3 Qp; QM 4 join(pos0); Qp 5 hairpin,
+    6 stack, 7 iloop(pos0 = k, pos1 = l), 8 mloop.  Productions of the pretty
+    algebra (RNAFold.scala:31-41, grammar :43-64): hairpin "("+dots+")",
+    stack and mloop "("+x+")", iloop "("+dots(k-1-i)+Qp(k,l)+dots(j-1-l)+")",
+    left x+".", right "."+x, join x+y.
+    """
+    if not trace:
+        return None
+    ans = {}
+    for (i, j, rule, p0, p1) in reversed(trace):
+        if rule == 0:
+            v, tab = ans[("QM", i, j)], "Q"
+        elif rule == 1:
+            v, tab = ans[("Q", i, j - 1)] + ".", "Q"
+        elif rule == 2:
+            v, tab = "." + ans[("Q", i + 1, j)], "Q"
+        elif rule == 3:
+            v, tab = ans[("Qp", i, j)], "Q"
+        elif rule == 4:
+            v, tab = ans[("Q", i, p0)] + ans[("Q", p0, j)], "QM"
+        elif rule == 5:
+            v, tab = "(" + _dots(j - i - 2) + ")", "Qp"
+        elif rule == 6:
+            v, tab = "(" + ans[("Qp", i + 1, j - 1)] + ")", "Qp"
+        elif rule == 7:
+            v, tab = "(" + _dots(p0 - 1 - i) + ans[("Qp", p0, p1)] + _dots(j - 1 - p1) + ")", "Qp"
+        elif rule == 8:
+            v, tab = "(" + ans[("QM", i + 1, j - 1)] + ")", "Qp"
+        else:
+            raise ValueError("not an RNAfold rule id: %r" % (rule,))
+        ans[(tab, i, j)] = v
+    return ans[("Q", trace[0][0], trace[0][1])]
 
 
 def nussinov(seq, trace):

@@ -92,3 +92,22 @@ def test_rnafold_served_equals_alone(ctx):
         assert r.status == dpx.OK and r.score == osc and tr == otr, len(s)
     r, _ = got[ids[-1]]
     assert r.score == o.zuker(seqs[0])[0]
+
+
+def _large():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rnafold_large.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("fx", _large(), ids=lambda f: "%s-n%d" % (f["gen"], f["n"]))
+def test_rnafold_full_size_fixture(ctx, fx):
+    """n = 2048 (the Zuker configuration's input) and n = 1500 against the
+    oracle's answers committed by tests/golden/make_rnafold_large.py."""
+    from dpx import gen
+    seq = getattr(gen, fx["gen"])(fx["n"], fx["seed"])
+    print("rnafold fixture n=%d" % len(seq), flush=True)
+    score, tr = ctx.rnafold(seq)
+    assert score == fx["score"]
+    assert [list(t) for t in tr] == fx["trace"]

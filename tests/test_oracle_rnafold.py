@@ -82,3 +82,17 @@ def test_rnafold_gc_rich_long_loops():
     fns = o.energy_fns(seq)
     pairs = sorted({(i, j - 1) for (i, j, r, _, _) in tr if 5 <= r <= 8})
     assert _energy(seq, pairs, fns) == score < 0
+
+
+def test_rnafold_large_fixture_matches_generator_and_oracle_prefix():
+    """The committed large fixtures name generator calls that still produce
+    sequences the oracle folds to a trace starting at the axiom (0, n)."""
+    import json
+    import os
+    from dpx import gen
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rnafold_large.json")) as f:
+        fx = json.load(f)
+    for c in fx:
+        seq = getattr(gen, c["gen"])(c["n"], c["seed"])
+        assert len(seq) == c["n"]
+        assert c["trace"][0][:2] == [0, c["n"]] and c["score"] < 0

@@ -136,3 +136,22 @@ def test_swat_tracks(sc):
                     g2 = max(0, (score if g2 is None else g2) + (go if g2 is None else ge))
                     score, g1 = g2, None
             assert score == (best if fb else axiom), (a, b, t1, t2)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_rnafold_build(seed):
+    """Dot-bracket of RNAfold oracle traces: right length, canonical pairs,
+    exactly the pairs the trace's Qp records close (rules 5-8 close (i, j-1)),
+    and every unpaired stretch of an interior loop under 30 bases."""
+    seq = gen.rna_gc(60 + 70 * seed, 400 + seed)
+    score, tr = o.rnafold(seq)
+    db = pretty.rnafold(seq, tr)
+    assert len(db) == len(seq)
+    ps = _pairs(db)
+    assert ps == sorted({(i, j - 1) for i, j, r, _, _ in tr if 5 <= r <= 8})
+    for i, j in ps:
+        assert (seq[i], seq[j]) in PAIRS
+    for i, j, r, k, l in tr:
+        if r == 7:
+            assert 0 <= k - 1 - i < 30 and 0 <= j - 1 - l < 30
+    assert pretty.rnafold("AAAA", o.rnafold("AAAA")[1]) is None
