@@ -10,6 +10,11 @@ import time
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libdpx.so")
 PAR_TURNER2004 = os.path.join(PKG, "params", "rna_turner2004.par")
+# every parameter file shipped (ViennaRNA 2.0 format), by name
+PAR_FILES = {"turner2004": PAR_TURNER2004,
+             "turner1999": os.path.join(PKG, "params", "rna_turner1999.par"),
+             "andronescu2007": os.path.join(PKG, "params", "rna_andronescu2007.par"),
+             "dna_mathews2004": os.path.join(PKG, "params", "dna_mathews2004.par")}
 
 OK, EINVAL, ENOMEM, EHIP, ENOPARAMS, EPARAMS, ETRACE, EBUSY, EAGAIN = range(9)
 

@@ -35,15 +35,22 @@ ZUKER_TESTS = [
 ]
 
 
-def run_ref(kind, seqs):
+# other parameter files of the reference (src/librna/vienna/*.par), read by
+# the same reference code (bench2/zu.c built per set by oracle/Makefile, PSET=...)
+PARAM_SETS = ["turner1999", "andronescu2007", "dna_mathews2004"]
+
+
+def run_ref(kind, seqs, pset=None):
     by_len = {}
     for s in seqs:
         by_len.setdefault(len(s), []).append(s)
     out = {}
     for n, group in sorted(by_len.items()):
-        exe = os.path.join(ORACLE, "_ref", "%s_%d" % (kind, n))
+        target = "_ref/%s_%d" % (kind, n) if pset is None else "_ref/p_%s/%s_%d" % (pset, kind, n)
+        exe = os.path.join(ORACLE, target)
         if not os.path.exists(exe):
-            subprocess.check_call(["make", "-s", "-C", ORACLE, "_ref/%s_%d" % (kind, n)])
+            cmd = ["make", "-s", "-C", ORACLE, target] + ([] if pset is None else ["PSET=" + pset])
+            subprocess.check_call(cmd)
         res = subprocess.run([exe], input=("\n".join(group) + "\n").encode(), capture_output=True, check=True)
         for s, line in zip(group, res.stdout.decode().strip().split("\n")):
             out[s] = json.loads(line)
@@ -68,7 +75,22 @@ def main():
     with open(os.path.join(HERE, "nussinov_ref.json"), "w") as f:
         json.dump(nu, f, separators=(",", ":"))
     print("zuker fixtures: %d, nussinov fixtures: %d" % (len(z), len(nu)))
+    param_sets()
+
+
+def param_sets():
+    ps = [s.upper() for s in ZUKER_TESTS[:8]]
+    for k, n in enumerate([7, 12, 20, 33, 64, 100, 150, 257, 400]):
+        ps.append(gen.rna_gc(n, 500 + k))
+    ps.append(gen.rna_gc(120, 520, gc=0.7))
+    ps.append(gen.rna_gc(800, 521))
+    for pset in PARAM_SETS:
+        z = run_ref("zu", ps, pset)
+        with open(os.path.join(HERE, "zuker_ref_%s.json" % pset), "w") as f:
+            json.dump(z, f, separators=(",", ":"))
+        print("zuker fixtures with %s: %d" % (pset, len(z)))
 
 
 if __name__ == "__main__":
-    main()
+    # "params": only the fixtures of the other parameter files
+    param_sets() if sys.argv[1:] == ["params"] else main()

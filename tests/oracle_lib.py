@@ -12,7 +12,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "build", "liboracle.so")
-PAR_PATH = os.path.join(ROOT, "lamp-dp-mt_amd", "params", "rna_turner2004.par")
+PAR_DIR = os.path.join(ROOT, "lamp-dp-mt_amd", "params")
+PAR_PATH = os.path.join(PAR_DIR, "rna_turner2004.par")
+# parameter files shipped beside Turner 2004 (the reference's src/librna/vienna/*.par)
+PAR_SETS = {"turner2004": "rna_turner2004.par", "turner1999": "rna_turner1999.par",
+            "andronescu2007": "rna_andronescu2007.par", "dna_mathews2004": "dna_mathews2004.par"}
 
 CODE = {"A": 1, "C": 2, "G": 3, "U": 4, "T": 4}
 
@@ -23,7 +27,7 @@ class Trace(ctypes.Structure):
 
 
 _lib = None
-_params = None
+_params = {}
 
 
 def lib():
@@ -36,16 +40,17 @@ def lib():
     return _lib
 
 
-def params():
-    """Scaled Turner 2004 parameters as an opaque buffer."""
-    global _params
-    if _params is None:
+def params(par=None):
+    """Scaled parameters of one .par file (Turner 2004 by default, else a
+    PAR_SETS name) as an opaque buffer."""
+    name = par or "turner2004"
+    if name not in _params:
         buf = ctypes.create_string_buffer(1 << 20)
-        rc = lib().orc_load_params(PAR_PATH.encode(), buf)
+        rc = lib().orc_load_params(os.path.join(PAR_DIR, PAR_SETS[name]).encode(), buf)
         if rc != 0:
             raise RuntimeError("orc_load_params failed: %d" % rc)
-        _params = buf
-    return _params
+        _params[name] = buf
+    return _params[name]
 
 
 def encode(seq):
@@ -60,14 +65,14 @@ def _trace_list(tr, n, tl):
     return out
 
 
-def zuker(seq, trace=True):
+def zuker(seq, trace=True, par=None):
     n = len(seq)
     cap = 4 * (n + 2)
     tr = (Trace * cap)()
     tl = ctypes.c_int(0)
     f = lib().orc_zuker
     f.restype = ctypes.c_int32
-    score = f(params(), encode(seq), n, tr if trace else None, cap, ctypes.byref(tl), None)
+    score = f(params(par), encode(seq), n, tr if trace else None, cap, ctypes.byref(tl), None)
     return score, (_trace_list(tr, tl.value, tl) if trace else None)
 
 
@@ -123,14 +128,14 @@ def matchain(rows, cols, trace=True):
 RF_EMPTY = 999999
 
 
-def rnafold(seq, trace=True):
+def rnafold(seq, trace=True, par=None):
     n = len(seq)
     cap = 8 * (n + 2)
     tr = (Trace * cap)()
     tl = ctypes.c_int(0)
     f = lib().orc_rnafold
     f.restype = ctypes.c_int32
-    score = f(params(), encode(seq), n, tr if trace else None, cap, ctypes.byref(tl), None)
+    score = f(params(par), encode(seq), n, tr if trace else None, cap, ctypes.byref(tl), None)
     return score, (_trace_list(tr, tl.value, tl) if trace else None)
 
 

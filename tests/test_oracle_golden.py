@@ -32,3 +32,21 @@ def test_nussinov_matches_reference(fx):
     score, tr = o.nussinov(fx["seq"])
     assert score == fx["score"]
     assert [[a, b, r, p0, 0] for a, b, r, p0, _ in tr] == fx["trace"]
+
+
+PSET_FIXTURES = [(p, f) for p in ("turner1999", "andronescu2007", "dna_mathews2004")
+                 for f in _load("zuker_ref_%s.json" % p)]
+
+
+@pytest.mark.parametrize("pset,fx", PSET_FIXTURES, ids=lambda x: x if isinstance(x, str) else "n%d" % len(x["seq"]))
+def test_zuker_other_parameter_files_match_reference(pset, fx):
+    """The same reference code reading the other .par files of its tree."""
+    score, tr = o.zuker(fx["seq"], par=pset)
+    assert score == fx["score"]
+    assert _norm_zu(tr) == fx["trace"]
+
+
+def test_parameter_files_differ():
+    # the fixtures are not all the same answers under another name
+    s = "GGGGAAAACCCCAUGCAUGCGGCAAAAGCC"
+    assert len({o.zuker(s, trace=False, par=p)[0] for p in o.PAR_SETS}) >= 3
