@@ -124,8 +124,8 @@ int dpx_last_timing(const dpx_ctx* ctx, float* fill_ms, float* finalize_ms);
 /* ---- plans: a batch uploaded once and replayed as a HIP graph -------------
  * No reference counterpart.  dpx_plan_create copies the batch (Zuker,
  * Nussinov or RNAfold, as DPX_REQ_ZUKER / NUSSINOV / RNAFOLD) to the device, reserves its
- * DP tables and captures the queue reset, the persistent fill launch and the
- * finalize launch as one HIP graph; dpx_plan_run replays it (synchronous,
+ * DP tables and captures the queue reset with the persistent fill launch, and
+ * the finalize launch, as HIP graphs; dpx_plan_run replays them (synchronous,
  * *device_ms = device time of the replay) and dpx_plan_results copies the
  * answers of the last replay out (same conventions as dpx_zuker_batch).
  * dpx_set_rna_params returns DPX_EBUSY while a plan exists. */
@@ -133,12 +133,50 @@ typedef struct dpx_plan dpx_plan;
 int dpx_plan_create(dpx_ctx* ctx, int kind, const char* seqs, const int64_t* offs, int count, int min_loop,
                     int want_trace, dpx_plan** out);
 int dpx_plan_run(dpx_plan* plan, float* device_ms);
+/* Device time of the last replay's fill (queue reset + persistent fill
+ * kernel) and finalize launches: the plan replays them as two graphs on its
+ * stream, with events recorded before, between and after. */
+int dpx_plan_timing(dpx_plan* plan, float* fill_ms, float* finalize_ms);
 int dpx_plan_results(dpx_plan* plan, int32_t* results, dpx_trace_t* traces, const int64_t* trace_offs,
                      int32_t* trace_lens);
 void dpx_plan_destroy(dpx_plan* plan);
 
 /* Seconds on the host's monotonic clock, the time base of dpx_result. */
 double dpx_now(void);
+
+/* ---- launch shape: tuning table and autotuner ------------------------------
+ * No reference counterpart (the reference fixes its launch shape when it
+ * generates the code, CodeGen.scala:277-278,547).  The tile edge (32) and the
+ * waves per tile (4) are fixed by the kernels' data layout and LDS budget;
+ * what varies per grammar (request kind DPX_REQ_*) and length bucket is:
+ *   wg_per_cu  persistent workgroups per compute unit of the fill launch
+ *              (0 = as many as fit, 1..that number)
+ *   chunk      Smith-Waterman only: steps a band computes between two
+ *              handoffs with its neighbours (16, 32 or 64; 0 = 32).
+ * Length bucket b holds n in (32 << b, 64 << b] (b = 0 from n = 0, the last
+ * bucket everything above 4096); n is the sequence / chain length, for SW
+ * max(n1, n2).  A triangular batch takes the tuning of the bucket holding
+ * most of its cells.  Results never depend on the launch shape. */
+#define DPX_TUNE_BUCKETS 8
+typedef struct { int32_t wg_per_cu, chunk, reserved[2]; } dpx_tuning;
+int dpx_tune_bucket(int n);
+int dpx_get_tuning(dpx_ctx* ctx, int kind, int bucket, dpx_tuning* out);
+int dpx_set_tuning(dpx_ctx* ctx, int kind, int bucket, const dpx_tuning* t);
+
+/* Times every candidate shape of `kind` on seeded synthetic launches of each
+ * bucket in [bucket_lo, bucket_hi] (about 4e7 cells each, or one instance;
+ * device time of fill + finalize, best of `reps`), checks that all shapes
+ * return identical results (else DPX_EHIP), and keeps the fastest in the
+ * context's table.  One report entry per candidate (chosen = 1 for the
+ * winner); *report_len receives the number of candidates timed. */
+typedef struct {
+  int32_t kind, bucket, n_lo, n_hi, count;
+  dpx_tuning t;
+  float ms;
+  int32_t chosen;
+} dpx_tune_report;
+int dpx_autotune(dpx_ctx* ctx, int kind, int bucket_lo, int bucket_hi, int reps, dpx_tune_report* report,
+                 int report_cap, int* report_len);
 
 /* ---- serving: continuous batching -----------------------------------------
  * No reference counterpart (the reference solves one problem per loaded

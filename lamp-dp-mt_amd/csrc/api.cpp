@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -138,6 +139,10 @@ struct dpx_ctx {
   float t_fill = 0, t_fin = 0;
   std::atomic<int> servers{0};  // servers and plans hold the energy tables' address
   std::atomic<int> plans{0};
+  // launch shape per request kind (DPX_REQ_*) and length bucket; read by
+  // server threads at launch time, hence its own lock
+  std::mutex tune_mu;
+  dpx_tuning tune[6][DPX_TUNE_BUCKETS] = {};
 };
 
 namespace {
@@ -181,6 +186,39 @@ int max_len(int g) { return g == G_ZUKER ? kMaxZuker : kMaxTri; }
 // grammar of a folding request kind (DPX_REQ_NUSSINOV / ZUKER / RNAFOLD)
 int grammar_of(int kind) {
   return kind == DPX_REQ_NUSSINOV ? G_NUSSINOV : kind == DPX_REQ_RNAFOLD ? G_RNAFOLD : G_ZUKER;
+}
+int kind_of(int g) {
+  return g == G_NUSSINOV ? DPX_REQ_NUSSINOV : g == G_RNAFOLD ? DPX_REQ_RNAFOLD : g == G_MATCHAIN ? DPX_REQ_MATCHAIN
+                                                                                                   : DPX_REQ_ZUKER;
+}
+// length bucket: n <= 64 -> 0, (64,128] -> 1, ... (4096, inf) -> 7
+int tune_bucket(int64_t n) {
+  int b = 0;
+  for (int64_t hi = 64; n > hi && b < DPX_TUNE_BUCKETS - 1; hi *= 2) ++b;
+  return b;
+}
+
+// ---------------------------------------------------------------- launch shape
+
+dpx_tuning get_tune(dpx_ctx* c, int kind, int bucket) {
+  std::lock_guard<std::mutex> lk(c->tune_mu);
+  return c->tune[kind][bucket];
+}
+
+// the length bucket holding most of a triangular job's cells
+int job_bucket(const Job& j) {
+  int64_t w[DPX_TUNE_BUCKETS] = {};
+  for (const TriInst& t : j.inst) w[tune_bucket(t.n)] += tri_cells(t.n);
+  return (int)(std::max_element(w, w + DPX_TUNE_BUCKETS) - w);
+}
+
+// persistent workgroups of a triangular launch: tuned workgroups per CU
+// (0 = the kernel's occupancy) times the CU count, at most one per tile
+int tri_grid(dpx_ctx* c, const Job& j) {
+  const dpx_tuning t = get_tune(c, kind_of(j.g), job_bucket(j));
+  const int occ = c->occ[j.g];
+  const int w = t.wg_per_cu >= 1 && t.wg_per_cu <= occ ? t.wg_per_cu : occ;
+  return (int)std::min<int64_t>(j.tiles, (int64_t)w * c->cu_count);
 }
 
 // ---------------------------------------------------------------- layouts
@@ -311,7 +349,7 @@ int tri_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
   a.traces = dev + j.off[TL_TRACES];
   a.trace_lens = (int32_t*)(dev + j.off[TL_LENS]);
   a.want_trace = j.want_trace ? 1 : 0;
-  const int grid = (int)std::min<int64_t>(j.tiles, (int64_t)c->occ[g] * c->cu_count);
+  const int grid = tri_grid(c, j);
   HIPCHK(hipEventRecord(w.ev[0], s));
   if (launch_fill(g, a, (const unsigned long long*)(dev + j.off[TL_SEEDS]), (int)j.seeds.size(), grid, s))
     return fail(err, DPX_EHIP, "fill launch failed");
@@ -422,8 +460,12 @@ int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j)
   HIPCHK(hipMemcpyAsync(d_row0, h_row0, sizeof(int2) * W + 2 * (size_t)W, hipMemcpyHostToDevice, s));
   HIPCHK(hipEventRecord(w.ev[0], s));
   if (A.nbands > 0) {
-    const int grid = std::min(A.nbands, c->sw_occ * c->cu_count);
-    if (sw_launch(A, d_row0, grid, s)) return fail(err, DPX_EHIP, "alignment launch failed");
+    // tuned: steps per chunk and bands in flight per CU (0 = occupancy)
+    const dpx_tuning t = get_tune(c, DPX_REQ_SWAT, tune_bucket(std::max(n1, n2)));
+    const int chunk = t.chunk == 16 || t.chunk == 64 ? t.chunk : 32;
+    const int w = t.wg_per_cu >= 1 && t.wg_per_cu <= c->sw_occ ? t.wg_per_cu : c->sw_occ;
+    const int grid = std::min(A.nbands, w * c->cu_count);
+    if (sw_launch(A, d_row0, grid, chunk, s)) return fail(err, DPX_EHIP, "alignment launch failed");
   }
   HIPCHK(hipEventRecord(w.ev[1], s));
   void* d_tr = want_trace ? dev + j.off[SL_TR] : nullptr;
@@ -989,9 +1031,12 @@ struct dpx_plan {
   Job job;
   DevBuf block;
   Workspace ws;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // two graphs replayed back to back on the plan's stream: queue reset +
+  // persistent fill, then finalize; events between them time the kernels
+  hipGraph_t graph = nullptr, graph_fin = nullptr;
+  hipGraphExec_t exec = nullptr, exec_fin = nullptr;
   bool ran = false;
+  float fill_ms = -1, fin_ms = -1;  // kernel times of the last replay
 };
 
 namespace {
@@ -999,6 +1044,8 @@ namespace {
 void plan_free(dpx_plan* P) {
   if (P->exec) (void)hipGraphExecDestroy(P->exec);
   if (P->graph) (void)hipGraphDestroy(P->graph);
+  if (P->exec_fin) (void)hipGraphExecDestroy(P->exec_fin);
+  if (P->graph_fin) (void)hipGraphDestroy(P->graph_fin);
   P->ws.release();
   P->block.release();
 }
@@ -1032,16 +1079,22 @@ int plan_build(dpx_plan* P) {
   a.traces = dev + j.off[TL_TRACES];
   a.trace_lens = (int32_t*)(dev + j.off[TL_LENS]);
   a.want_trace = j.want_trace ? 1 : 0;
-  const int grid = (int)std::min<int64_t>(j.tiles, (int64_t)c->occ[j.g] * c->cu_count);
+  const int grid = tri_grid(c, j);
   hipStream_t s = P->ws.stream;
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   const int r1 = launch_fill(j.g, a, (const unsigned long long*)(dev + j.off[TL_SEEDS]), (int)j.seeds.size(), grid, s);
-  const int r2 = launch_finalize(j.g, a, j.count, j.n_max, s);
   hipGraph_t g = nullptr;
-  const hipError_t e = hipStreamEndCapture(s, &g);
+  hipError_t e = hipStreamEndCapture(s, &g);
   P->graph = g;
-  if (r1 || r2 || e != hipSuccess) return fail(err, DPX_EHIP, "graph capture of the batch launch failed");
+  if (r1 || e != hipSuccess) return fail(err, DPX_EHIP, "graph capture of the fill launch failed");
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int r2 = launch_finalize(j.g, a, j.count, j.n_max, s);
+  g = nullptr;
+  e = hipStreamEndCapture(s, &g);
+  P->graph_fin = g;
+  if (r2 || e != hipSuccess) return fail(err, DPX_EHIP, "graph capture of the finalize launch failed");
   HIPCHK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+  HIPCHK(hipGraphInstantiate(&P->exec_fin, P->graph_fin, nullptr, nullptr, 0));
   return DPX_OK;
 }
 
@@ -1084,6 +1137,8 @@ int dpx_plan_run(dpx_plan* P, float* device_ms) {
   hipStream_t s = P->ws.stream;
   HIPCHK(hipEventRecord(P->ws.ev[0], s));
   HIPCHK(hipGraphLaunch(P->exec, s));
+  HIPCHK(hipEventRecord(P->ws.ev[2], s));
+  HIPCHK(hipGraphLaunch(P->exec_fin, s));
   HIPCHK(hipEventRecord(P->ws.ev[1], s));
   int32_t flags[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(flags, (char*)P->block.p + P->job.off[TL_CTRL], sizeof flags, hipMemcpyDeviceToHost, s));
@@ -1091,8 +1146,26 @@ int dpx_plan_run(dpx_plan* P, float* device_ms) {
   float ms = 0;
   (void)hipEventElapsedTime(&ms, P->ws.ev[0], P->ws.ev[1]);
   if (device_ms) *device_ms = ms;
+  // fill = queue reset + persistent fill kernel; finalize = the second graph
+  P->fill_ms = -1;
+  P->fin_ms = -1;
+  float f = 0, z = 0;
+  if (hipEventElapsedTime(&f, P->ws.ev[0], P->ws.ev[2]) == hipSuccess &&
+      hipEventElapsedTime(&z, P->ws.ev[2], P->ws.ev[1]) == hipSuccess) {
+    P->fill_ms = f;
+    P->fin_ms = z;
+  }
   if (flags[2]) return fail(err, DPX_EHIP, "tile queue wait timed out (scheduler error)");
   P->ran = true;
+  return DPX_OK;
+}
+
+int dpx_plan_timing(dpx_plan* P, float* fill_ms, float* finalize_ms) {
+  if (!P || !fill_ms || !finalize_ms) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(P->ctx->mu);
+  if (!P->ran || P->fill_ms < 0) return fail(P->ctx->err, DPX_EHIP, "no timed replay");
+  *fill_ms = P->fill_ms;
+  *finalize_ms = P->fin_ms;
   return DPX_OK;
 }
 
@@ -1138,6 +1211,193 @@ int dpx_last_timing(const dpx_ctx* c, float* fill_ms, float* finalize_ms) {
 }
 
 double dpx_now(void) { return now_s(); }
+
+// ================================================================== tuning
+
+int dpx_tune_bucket(int n) { return n < 0 ? -1 : tune_bucket(n); }
+
+int dpx_get_tuning(dpx_ctx* c, int kind, int bucket, dpx_tuning* out) {
+  if (!c || !out || kind < DPX_REQ_NUSSINOV || kind > DPX_REQ_RNAFOLD || bucket < 0 || bucket >= DPX_TUNE_BUCKETS)
+    return DPX_EINVAL;
+  *out = get_tune(c, kind, bucket);
+  return DPX_OK;
+}
+
+int dpx_set_tuning(dpx_ctx* c, int kind, int bucket, const dpx_tuning* t) {
+  if (!c || !t || kind < DPX_REQ_NUSSINOV || kind > DPX_REQ_RNAFOLD || bucket < 0 || bucket >= DPX_TUNE_BUCKETS)
+    return DPX_EINVAL;
+  if (t->wg_per_cu < 0 || (t->chunk != 0 && t->chunk != 16 && t->chunk != 32 && t->chunk != 64)) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(c->tune_mu);
+  c->tune[kind][bucket] = *t;
+  return DPX_OK;
+}
+
+namespace {
+
+struct Mix64 {  // splitmix64: seeded synthetic inputs of the tuning runs
+  uint64_t x;
+  uint64_t next() {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double unit() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+// lengths of bucket b: (32 << b, 64 << b], bucket 0 from 16
+void bucket_range(int b, int* lo, int* hi) {
+  *lo = b == 0 ? 16 : (32 << b) + 1;
+  *hi = 64 << b;
+}
+
+// A seeded synthetic launch of `kind` with lengths in bucket b, about
+// 4e7 cells (at least one instance): the shape the tuning is chosen for.
+int tune_job(std::string& err, int kind, int b, int max_n, Job& j, int* count_out) {
+  int lo, hi;
+  bucket_range(b, &lo, &hi);
+  hi = std::min(hi, max_n);
+  lo = std::min(lo, hi);
+  Mix64 r{0x7475ull * 1000003ull + (uint64_t)kind * 131ull + (uint64_t)b};
+  auto len = [&]() { return (int)std::lround(std::exp(std::log((double)lo) + r.unit() * (std::log((double)hi) - std::log((double)lo)))); };
+  if (kind == DPX_REQ_SWAT) {
+    const int n1 = hi, n2 = hi;
+    std::string a(n1, 'A'), bb(n2, 'A');
+    for (auto* sq : {&a, &bb})
+      for (char& ch : *sq) ch = "ACGT"[r.next() & 3];
+    *count_out = 1;
+    return sw_prepare(err, j, a.data(), n1, bb.data(), n2, dpx_swat_scores{2, -3, -5, -2}, 1, sw_trace_cap(n1, n2));
+  }
+  std::vector<int> lens;
+  std::vector<int64_t> off;
+  std::vector<uint8_t> data;
+  int64_t cells = 0;
+  while ((cells < 40000000 && (int)lens.size() < 4096) || lens.empty()) {
+    const int n = len();
+    lens.push_back(n);
+    // matrix chain offsets count int32 dims, RNA offsets bytes
+    off.push_back(kind == DPX_REQ_MATCHAIN ? (int64_t)(data.size() / 4) : (int64_t)data.size());
+    cells += tri_cells(n);
+    if (kind == DPX_REQ_MATCHAIN) {
+      std::vector<int32_t> rc(2 * (size_t)n);
+      int32_t prev = 1 + (int32_t)(r.next() % 1000);
+      for (int q = 0; q < n; ++q) {  // matrix q is rows[q] x cols[q], chained
+        const int32_t nx = 1 + (int32_t)(r.next() % 1000);
+        rc[q] = prev;
+        rc[n + q] = nx;
+        prev = nx;
+      }
+      const size_t at = data.size();
+      data.resize(at + rc.size() * 4);
+      std::memcpy(data.data() + at, rc.data(), rc.size() * 4);
+    } else {
+      for (int q = 0; q < n; ++q) {  // 50% GC
+        const double u = r.unit();
+        data.push_back(u < 0.25 ? 3 : u < 0.5 ? 2 : u < 0.75 ? 1 : 4);
+      }
+      data.push_back(0);
+    }
+  }
+  *count_out = (int)lens.size();
+  const int g = kind == DPX_REQ_MATCHAIN ? G_MATCHAIN : grammar_of(kind);
+  return tri_prepare(err, j, g, lens, off, std::move(data), kind == DPX_REQ_NUSSINOV ? 3 : 0, true);
+}
+
+// everything a launch returns, to check that the shape does not change it
+std::vector<int32_t> job_outputs(const Job& j) {
+  std::vector<int32_t> v;
+  if (j.kind == JOB_SW) {
+    v.assign(j.res, j.res + 7);
+    const int32_t* t = (const int32_t*)j.tr;
+    if (j.res[6] > 0) v.insert(v.end(), t, t + 5 * (size_t)j.res[6]);
+    return v;
+  }
+  v.assign(j.res, j.res + j.count);
+  v.insert(v.end(), j.lens, j.lens + j.count);
+  const int32_t* t = (const int32_t*)j.tr;
+  v.insert(v.end(), t, t + 5 * (size_t)j.trace_total);
+  return v;
+}
+
+}  // namespace
+
+int dpx_autotune(dpx_ctx* c, int kind, int bucket_lo, int bucket_hi, int reps, dpx_tune_report* report,
+                 int report_cap, int* report_len) {
+  if (!c) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::string& err = c->err;
+  if (kind < DPX_REQ_NUSSINOV || kind > DPX_REQ_RNAFOLD || bucket_lo < 0 || bucket_hi >= DPX_TUNE_BUCKETS ||
+      bucket_lo > bucket_hi || reps < 1 || (report && report_cap < 0))
+    return fail(err, DPX_EINVAL, "bad argument");
+  if ((kind == DPX_REQ_ZUKER || kind == DPX_REQ_RNAFOLD) && !c->has_rna)
+    return fail(err, DPX_ENOPARAMS, "dpx_set_rna_params was not called");
+  (void)hipSetDevice(c->device);
+  const int g = kind == DPX_REQ_MATCHAIN ? G_MATCHAIN : kind == DPX_REQ_SWAT ? -1 : grammar_of(kind);
+  // candidate shapes: workgroups per CU (triangular), steps per chunk (alignment)
+  std::vector<dpx_tuning> cands;
+  if (kind == DPX_REQ_SWAT) {
+    for (int ch : {16, 32, 64}) cands.push_back(dpx_tuning{0, ch, {0, 0}});
+  } else {
+    for (int w : {1, 2, 3, 4, 6, 8})
+      if (w <= c->occ[g]) cands.push_back(dpx_tuning{w, 0, {0, 0}});
+  }
+  const int max_n = kind == DPX_REQ_SWAT ? 8192 : std::min(8192, max_len(g));
+  int nrep = 0;
+  for (int b = bucket_lo; b <= bucket_hi; ++b) {
+    Job j;
+    int count = 0;
+    int r = tune_job(err, kind, b, max_n, j, &count);
+    if (r) return r;
+    const dpx_tuning saved = get_tune(c, kind, b);
+    std::vector<int32_t> want;
+    float best_ms = 0;
+    int best = -1, first_rep = nrep;
+    for (size_t q = 0; q < cands.size(); ++q) {
+      {
+        std::lock_guard<std::mutex> tl(c->tune_mu);
+        c->tune[kind][b] = cands[q];
+      }
+      float ms = 0;
+      for (int k = 0; k < reps; ++k) {
+        r = run_sync(c, j);
+        if (r) {
+          std::lock_guard<std::mutex> tl(c->tune_mu);
+          c->tune[kind][b] = saved;
+          return r;
+        }
+        const float t = j.t_fill + j.t_fin;
+        ms = k == 0 ? t : std::min(ms, t);
+      }
+      std::vector<int32_t> got = job_outputs(j);
+      if (q == 0) want = std::move(got);
+      else if (got != want) {
+        std::lock_guard<std::mutex> tl(c->tune_mu);
+        c->tune[kind][b] = saved;
+        return fail(err, DPX_EHIP, "results differ between launch shapes");
+      }
+      if (best < 0 || ms < best_ms) { best_ms = ms; best = (int)q; }
+      if (report && nrep < report_cap) {
+        dpx_tune_report& e = report[nrep];
+        e.kind = kind;
+        e.bucket = b;
+        bucket_range(b, &e.n_lo, &e.n_hi);
+        e.n_hi = std::min(e.n_hi, max_n);
+        e.count = count;
+        e.t = cands[q];
+        e.ms = ms;
+        e.chosen = 0;
+      }
+      ++nrep;
+    }
+    {
+      std::lock_guard<std::mutex> tl(c->tune_mu);
+      c->tune[kind][b] = cands[best];
+    }
+    if (report && first_rep + best < report_cap) report[first_rep + best].chosen = 1;
+  }
+  if (report_len) *report_len = nrep;
+  return DPX_OK;
+}
 
 // ---------------------------------------------------------------- server
 

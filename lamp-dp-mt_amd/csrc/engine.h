@@ -91,7 +91,7 @@ struct SwArgs {
   unsigned long long* best;  // packed (M << 32) | ~(i*(n2+1)+j)
   int* out;             // [0] = M(n1,n2)
 };
-int sw_launch(const SwArgs& A, const int2* row0, int grid, void* stream);
+int sw_launch(const SwArgs& A, const int2* row0, int grid, int chunk, void* stream);  // chunk 16, 32 or 64
 int sw_finish_launch(const SwArgs& A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out,
                      const uint8_t* r0m, const uint8_t* r0g2, void* tr, int cap, int* len, void* stream);
 int sw_occupancy();

@@ -34,9 +34,13 @@ __device__ __forceinline__ int sw_shr1(int lane0_val, int v) {
   return __builtin_amdgcn_update_dpp(lane0_val, v, 0x138, 0xf, 0xf, false);
 }
 
-constexpr int SW_CH = 32;  // publish granularity (columns); a band lags the one above by 64 + SW_CH steps
-
+// SW_CH = steps per chunk = publish granularity (columns), a multiple of 8 up
+// to 64: a band lags the one above by 64 + SW_CH steps, and waits for it (or
+// publishes to the band below) once per chunk.  Variants 16, 32, 64; the
+// tuning table picks one per length bucket.
+template <int SW_CH>
 __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
+  static_assert(SW_CH % 8 == 0 && SW_CH <= 64, "chunk: whole code words, one column per lane");
   const int lane = threadIdx.x;
   for (;;) {
     // one wave per workgroup: the band claim and every wait are wave-uniform
@@ -234,12 +238,14 @@ __global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mru
   if (lane == 0) *len = ok ? n : -1;
 }
 
-int sw_launch(const SwArgs& A, const int2* row0, int grid, void* stream) {
+int sw_launch(const SwArgs& A, const int2* row0, int grid, int chunk, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const int W = A.n2 + 1;
   const int nthr = max(W, A.nbands + 1);
   hipLaunchKernelGGL(sw_reset, dim3((nthr + 255) / 256), dim3(256), 0, s, A, row0);
-  hipLaunchKernelGGL(sw_fill, dim3(grid), dim3(64), 0, s, A);
+  if (chunk == 16) hipLaunchKernelGGL(sw_fill<16>, dim3(grid), dim3(64), 0, s, A);
+  else if (chunk == 64) hipLaunchKernelGGL(sw_fill<64>, dim3(grid), dim3(64), 0, s, A);
+  else hipLaunchKernelGGL(sw_fill<32>, dim3(grid), dim3(64), 0, s, A);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -253,7 +259,7 @@ int sw_finish_launch(const SwArgs& A, int row0_best, int row0_bestj, int row0_ax
 
 int sw_occupancy() {
   int blocks = 0;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sw_fill, 64, 0) == hipSuccess ? blocks : 0;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sw_fill<32>, 64, 0) == hipSuccess ? blocks : 0;
 }
 
 }  // namespace dpx

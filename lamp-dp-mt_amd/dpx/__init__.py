@@ -25,7 +25,8 @@ EXPORTS = [
     "dpx_zuker_batch", "dpx_nussinov_batch", "dpx_rnafold_batch", "dpx_last_timing", "dpx_now",
     "dpx_server_create", "dpx_server_destroy", "dpx_server_submit", "dpx_server_poll",
     "dpx_server_trace", "dpx_server_get_stats",
-    "dpx_plan_create", "dpx_plan_run", "dpx_plan_results", "dpx_plan_destroy",
+    "dpx_plan_create", "dpx_plan_run", "dpx_plan_timing", "dpx_plan_results", "dpx_plan_destroy",
+    "dpx_tune_bucket", "dpx_get_tuning", "dpx_set_tuning", "dpx_autotune",
 ]
 
 REQ_NUSSINOV, REQ_ZUKER, REQ_SWAT, REQ_MATCHAIN, REQ_RNAFOLD = 1, 2, 3, 4, 5
@@ -69,6 +70,21 @@ class Result(ctypes.Structure):
                 ("cells", ctypes.c_int64), ("footprint", ctypes.c_int64),
                 ("t_submit", ctypes.c_double), ("t_launch", ctypes.c_double), ("t_done", ctypes.c_double),
                 ("device_ms", ctypes.c_float), ("reserved", ctypes.c_int32)]
+
+
+TUNE_BUCKETS = 8
+KIND_NAMES = {REQ_NUSSINOV: "nussinov", REQ_ZUKER: "zuker", REQ_SWAT: "swat", REQ_MATCHAIN: "matchain",
+              REQ_RNAFOLD: "rnafold"}
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("wg_per_cu", ctypes.c_int32), ("chunk", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("bucket", ctypes.c_int32), ("n_lo", ctypes.c_int32),
+                ("n_hi", ctypes.c_int32), ("count", ctypes.c_int32), ("t", Tuning), ("ms", ctypes.c_float),
+                ("chosen", ctypes.c_int32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -236,6 +252,41 @@ class Context:
         self._check(self._L.dpx_last_timing(self.h, ctypes.byref(f), ctypes.byref(z)))
         return f.value, z.value
 
+    # ---- launch shape (dpx_*_tuning, dpx_autotune) -------------------------
+    @staticmethod
+    def _kind(kind):
+        return kind if isinstance(kind, int) else {v: k for k, v in KIND_NAMES.items()}[kind]
+
+    def get_tuning(self, kind, bucket):
+        t = Tuning()
+        self._check(self._L.dpx_get_tuning(self.h, self._kind(kind), bucket, ctypes.byref(t)))
+        return {"wg_per_cu": t.wg_per_cu, "chunk": t.chunk}
+
+    def set_tuning(self, kind, bucket, wg_per_cu=0, chunk=0):
+        t = Tuning(wg_per_cu, chunk)
+        self._check(self._L.dpx_set_tuning(self.h, self._kind(kind), bucket, ctypes.byref(t)))
+
+    def tuning_table(self):
+        """{kind name: [per-bucket tuning]} -- what save/load exchange as JSON."""
+        return {name: [self.get_tuning(k, b) for b in range(TUNE_BUCKETS)] for k, name in KIND_NAMES.items()}
+
+    def load_tuning_table(self, table):
+        for name, rows in table.items():
+            for b, t in enumerate(rows):
+                self.set_tuning(name, b, t["wg_per_cu"], t["chunk"])
+
+    def autotune(self, kind, buckets=range(TUNE_BUCKETS), reps=2):
+        """Time every launch shape on synthetic launches of each bucket and
+        keep the fastest; returns one dict per candidate timed."""
+        lo, hi = min(buckets), max(buckets)
+        cap = 16 * (hi - lo + 1)
+        rep = (TuneReport * cap)()
+        n = ctypes.c_int()
+        self._check(self._L.dpx_autotune(self.h, self._kind(kind), lo, hi, reps, rep, cap, ctypes.byref(n)))
+        return [{"kind": KIND_NAMES[r.kind], "bucket": r.bucket, "n_lo": r.n_lo, "n_hi": r.n_hi, "count": r.count,
+                 "wg_per_cu": r.t.wg_per_cu, "chunk": r.t.chunk, "ms": round(r.ms, 4), "chosen": bool(r.chosen)}
+                for r in rep[:min(n.value, cap)]]
+
 
 def now():
     return lib().dpx_now()
@@ -375,6 +426,12 @@ class Plan:
         ms = ctypes.c_float()
         self.ctx._check(self._L.dpx_plan_run(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def timing(self):
+        """(fill ms, finalize ms) of the last replay, from event nodes in the graph."""
+        f, z = ctypes.c_float(), ctypes.c_float()
+        self.ctx._check(self._L.dpx_plan_timing(self.h, ctypes.byref(f), ctypes.byref(z)))
+        return f.value, z.value
 
     def results(self):
         res = (ctypes.c_int32 * self.count)()

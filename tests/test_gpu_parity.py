@@ -178,6 +178,8 @@ def test_plan_graph_replays(ctx):
         for rep in range(4):
             ms = p.run()
             assert ms > 0
+            f, z = p.timing()  # events between the fill and finalize graphs
+            assert f > 0 and z > 0 and f + z <= ms * 1.05 + 0.05, (f, z, ms)
             assert p.results() == want, rep
             ctx.nussinov("GGGGAAAACCCC")  # eager work between replays
     nus = gen.rna_batch(30, 1, 300, 92)
