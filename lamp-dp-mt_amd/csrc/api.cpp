@@ -91,7 +91,7 @@ struct Workspace {
 
 enum { JOB_TRI = 0, JOB_SW = 1 };
 // device block of a triangular job
-enum { TL_TAB, TL_QUEUE, TL_DEP, TL_CTRL, TL_INST, TL_SEEDS, TL_DATA, TL_RES, TL_LENS, TL_TRACES, TL_N };
+enum { TL_TAB, TL_QUEUE, TL_DEP, TL_CTRL, TL_INST, TL_SEEDS, TL_DATA, TL_SPEC, TL_RES, TL_LENS, TL_TRACES, TL_N };
 // device block of an alignment job
 enum { SL_A, SL_B, SL_ROWS, SL_FLAGS, SL_BEST, SL_BT, SL_TR, SL_N };
 
@@ -233,6 +233,8 @@ size_t tri_layout(int g, int64_t tiles, int count, size_t data_bytes, int64_t tr
   sz[TL_INST] = sizeof(TriInst) * (size_t)count;
   sz[TL_SEEDS] = (size_t)tiles * 8;
   sz[TL_DATA] = data_bytes + 64;
+  // special-hairpin energies by sequence position (RNA energy grammars)
+  sz[TL_SPEC] = (g == G_ZUKER || g == G_RNAFOLD) ? 3 * sizeof(int16_t) * data_bytes : 0;
   sz[TL_RES] = 4 * (size_t)count;
   sz[TL_LENS] = 4 * (size_t)count;
   sz[TL_TRACES] = sizeof(dpx_trace_t) * (size_t)trace_total;
@@ -316,6 +318,30 @@ int tri_prepare(std::string& err, Job& j, int g, const std::vector<int>& lens, c
   return DPX_OK;
 }
 
+// kernel arguments of a triangular job whose device block starts at dev
+TriArgs tri_args(const dpx_ctx* c, const Job& j, char* dev) {
+  TriArgs a{};
+  a.inst = (const TriInst*)(dev + j.off[TL_INST]);
+  a.n_inst = j.count;
+  a.total_tiles = (int32_t)j.tiles;
+  a.queue = (unsigned long long*)(dev + j.off[TL_QUEUE]);
+  a.ctrl = (int32_t*)(dev + j.off[TL_CTRL]);
+  a.dep = (int32_t*)(dev + j.off[TL_DEP]);
+  a.tab = (int32_t*)(dev + j.off[TL_TAB]);
+  a.codes = (const uint8_t*)(dev + j.off[TL_DATA]);
+  a.dims = (const int32_t*)(dev + j.off[TL_DATA]);
+  a.n_codes = (int64_t)j.data.size();
+  a.spec = (j.g == G_ZUKER || j.g == G_RNAFOLD) ? (int16_t*)(dev + j.off[TL_SPEC]) : nullptr;
+  a.rna = c->d_rna.p;
+  a.hl_ent = (const int32_t*)c->d_hl.p;
+  a.min_loop = j.min_loop;
+  a.results = (int32_t*)(dev + j.off[TL_RES]);
+  a.traces = dev + j.off[TL_TRACES];
+  a.trace_lens = (int32_t*)(dev + j.off[TL_LENS]);
+  a.want_trace = j.want_trace ? 1 : 0;
+  return a;
+}
+
 int tri_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
   const int g = j.g, count = j.count;
   const size_t in_bytes = sizeof(TriInst) * count + j.seeds.size() * 8 + j.data.size() + 64;
@@ -332,23 +358,7 @@ int tri_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
   HIPCHK(hipMemcpyAsync(dev + j.off[TL_INST], p_inst, sizeof(TriInst) * count, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dev + j.off[TL_SEEDS], p_seeds, j.seeds.size() * 8, hipMemcpyHostToDevice, s));
   if (!j.data.empty()) HIPCHK(hipMemcpyAsync(dev + j.off[TL_DATA], p_data, j.data.size(), hipMemcpyHostToDevice, s));
-  TriArgs a{};
-  a.inst = (const TriInst*)(dev + j.off[TL_INST]);
-  a.n_inst = count;
-  a.total_tiles = (int32_t)j.tiles;
-  a.queue = (unsigned long long*)(dev + j.off[TL_QUEUE]);
-  a.ctrl = (int32_t*)(dev + j.off[TL_CTRL]);
-  a.dep = (int32_t*)(dev + j.off[TL_DEP]);
-  a.tab = (int32_t*)(dev + j.off[TL_TAB]);
-  a.codes = (const uint8_t*)(dev + j.off[TL_DATA]);
-  a.dims = (const int32_t*)(dev + j.off[TL_DATA]);
-  a.rna = c->d_rna.p;
-  a.hl_ent = (const int32_t*)c->d_hl.p;
-  a.min_loop = j.min_loop;
-  a.results = (int32_t*)(dev + j.off[TL_RES]);
-  a.traces = dev + j.off[TL_TRACES];
-  a.trace_lens = (int32_t*)(dev + j.off[TL_LENS]);
-  a.want_trace = j.want_trace ? 1 : 0;
+  const TriArgs a = tri_args(c, j, dev);
   const int grid = tri_grid(c, j);
   HIPCHK(hipEventRecord(w.ev[0], s));
   if (launch_fill(g, a, (const unsigned long long*)(dev + j.off[TL_SEEDS]), (int)j.seeds.size(), grid, s))
@@ -1062,23 +1072,7 @@ int plan_build(dpx_plan* P) {
   HIPCHK(hipMemcpy(dev + j.off[TL_INST], j.inst.data(), sizeof(TriInst) * j.count, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dev + j.off[TL_SEEDS], j.seeds.data(), j.seeds.size() * 8, hipMemcpyHostToDevice));
   if (!j.data.empty()) HIPCHK(hipMemcpy(dev + j.off[TL_DATA], j.data.data(), j.data.size(), hipMemcpyHostToDevice));
-  TriArgs a{};
-  a.inst = (const TriInst*)(dev + j.off[TL_INST]);
-  a.n_inst = j.count;
-  a.total_tiles = (int32_t)j.tiles;
-  a.queue = (unsigned long long*)(dev + j.off[TL_QUEUE]);
-  a.ctrl = (int32_t*)(dev + j.off[TL_CTRL]);
-  a.dep = (int32_t*)(dev + j.off[TL_DEP]);
-  a.tab = (int32_t*)(dev + j.off[TL_TAB]);
-  a.codes = (const uint8_t*)(dev + j.off[TL_DATA]);
-  a.dims = (const int32_t*)(dev + j.off[TL_DATA]);
-  a.rna = c->d_rna.p;
-  a.hl_ent = (const int32_t*)c->d_hl.p;
-  a.min_loop = j.min_loop;
-  a.results = (int32_t*)(dev + j.off[TL_RES]);
-  a.traces = dev + j.off[TL_TRACES];
-  a.trace_lens = (int32_t*)(dev + j.off[TL_LENS]);
-  a.want_trace = j.want_trace ? 1 : 0;
+  const TriArgs a = tri_args(c, j, dev);
   const int grid = tri_grid(c, j);
   hipStream_t s = P->ws.stream;
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1373,7 +1367,13 @@ int dpx_autotune(dpx_ctx* c, int kind, int bucket_lo, int bucket_hi, int reps, d
       else if (got != want) {
         std::lock_guard<std::mutex> tl(c->tune_mu);
         c->tune[kind][b] = saved;
-        return fail(err, DPX_EHIP, "results differ between launch shapes");
+        size_t at = 0;
+        while (at < got.size() && at < want.size() && got[at] == want[at]) ++at;
+        return fail(err, DPX_EHIP, "results differ between launch shapes (bucket " + std::to_string(b) + ", shape " +
+                                       std::to_string(q) + " vs 0, first difference at output word " +
+                                       std::to_string(at) + " of " + std::to_string(want.size()) + ": " +
+                                       std::to_string(at < got.size() ? got[at] : 0) + " vs " +
+                                       std::to_string(at < want.size() ? want[at] : 0) + ")");
       }
       if (best < 0 || ms < best_ms) { best_ms = ms; best = (int)q; }
       if (report && nrep < report_cap) {

@@ -26,6 +26,7 @@ constexpr int INV = 1 << 28;    // "no valid candidate" for min-plus tables
 constexpr int INV_THR = 1 << 27;
 constexpr int NEG = -(1 << 28); // "no candidate" for max-plus tables
 constexpr int ZU_EMPTY = 999999;  // the reference's empty value (Zuker.scala:154)
+constexpr int SPEC_NONE = -32768; // no special hairpin starts here
 
 enum Grammar { G_NUSSINOV = 0, G_ZUKER = 1, G_MATCHAIN = 2, G_RNAFOLD = 3 };
 constexpr int N_GRAMMARS = 4;
@@ -58,7 +59,10 @@ struct TriArgs {
   int32_t* ctrl;               // [0] next slot to claim, [1] next slot to fill
   int32_t* dep;                // completed predecessors per tile
   int32_t* tab;                // tile records
-  const uint8_t* codes;        // bases, codes 1..4
+  const uint8_t* codes;        // bases, codes 1..4 (each instance followed by a 0)
+  int64_t n_codes;             // bytes of codes[]
+  int16_t* spec;               // special-hairpin energies: [3][n_codes] (tri-, tetra-,
+                               // hexaloop whose first base is at that position), or SPEC_NONE
   const int32_t* dims;         // matrix chain: rows then cols, n each
   const void* rna;             // DevRna (Zuker)
   const int32_t* hl_ent;       // hairpin entropy by loop size (Zuker)

@@ -29,6 +29,27 @@ typedef struct { int32_t i, j, rule, pos[2]; } TraceRec;  // == dpx_trace_t
 // ---------------------------------------------------------------- helpers
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+
+// min over x in [0, T) of a[x] + b[x * bstride], both in LDS.  The loads go
+// out 8 pairs at a time ahead of their adds (left alone, the scheduler
+// reuses two registers and waits for each load in turn).
+__device__ __forceinline__ int lds_minplus8(const int* a, const int* b, int bstride) {
+  int best = INV;
+#pragma unroll
+  for (int x0 = 0; x0 < T; x0 += 8) {
+    int av[8], bv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      av[u] = a[x0 + u];
+      bv[u] = b[(x0 + u) * bstride];
+    }
+    __builtin_amdgcn_sched_group_barrier(0x100, 12, 0);  // LDS reads
+    __builtin_amdgcn_sched_group_barrier(0x002, 16, 0);  // VALU
+#pragma unroll
+    for (int u = 0; u < 8; ++u) best = imin(best, av[u] + bv[u]);
+  }
+  return best;
+}
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 
 // canonical pair code of bases (x,y) (librna_impl.h:106-114): CG1 GC2 GU3 UG4 AU5 UA6, else 7
@@ -83,19 +104,32 @@ __device__ void middle_product(const TriArgs& a, const TriInst& in, int NTv, int
         alpha[t][u] = (i < in.n && j >= 1 && j <= in.n) ? (uint32_t)rows[i] * (uint32_t)cols[j - 1] : 0u;
       }
   }
-  for (int K = I + 1 + w; K < J; K += NWAVE) {
+  // operand tiles of block K travel in registers: those of the next block
+  // are loaded while this block's products are computed
+  int4 vl[4], vr[4];
+  auto load = [&](int K) {
     const int* gL = rec_ptr(a, in, NTv, I, K) + LT * T * T;
     const int* gR = rec_ptr(a, in, NTv, K, J) + RT * T * T;
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = (q * 64 + lane) * 4;
-      const int4 v = *(const int4*)(gL + e);
-      const int r = e / T, k = e % T;
-      sL[r * (T + 1) + k + 0] = v.x;
-      sL[r * (T + 1) + k + 1] = v.y;
-      sL[r * (T + 1) + k + 2] = v.z;
-      sL[r * (T + 1) + k + 3] = v.w;
-      *(int4*)(sR + e) = *(const int4*)(gR + e);
+      vl[q] = *(const int4*)(gL + e);
+      vr[q] = *(const int4*)(gR + e);
     }
+  };
+  if (I + 1 + w < J) load(I + 1 + w);
+  for (int K = I + 1 + w; K < J; K += NWAVE) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = (q * 64 + lane) * 4;
+      const int r = e / T, k = e % T;
+      sL[r * (T + 1) + k + 0] = vl[q].x;
+      sL[r * (T + 1) + k + 1] = vl[q].y;
+      sL[r * (T + 1) + k + 2] = vl[q].z;
+      sL[r * (T + 1) + k + 3] = vl[q].w;
+      *(int4*)(sR + e) = vr[q];
+    }
+    if (K + NWAVE < J) load(K + NWAVE);
     if (CHAIN && lane < T) {
       const int kk = K * T + lane;
       sK[lane] = kk < in.n ? a.dims[in.seq_off + kk] : 0;
@@ -148,13 +182,12 @@ __device__ __forceinline__ unsigned long long claim_item(const TriArgs& a) {
     __builtin_amdgcn_s_sleep(4);
     it = uniform64(__hip_atomic_load(&a.queue[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   }
-  // No cache invalidation on the consumer side: a tile record is written
-  // once per launch, by its producer, and no cache line of it is read before
-  // the producer has written the record back to memory and released the
-  // item (caches are clean at kernel start), so no L1/L2 can hold a stale
-  // copy.  Invalidating this XCD's L2 at every claim would instead evict the
-  // operand tiles that every workgroup of the XCD is reusing.
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  // Acquire at device scope (invalidates this CU's L1 and this XCD's L2
+  // copies of other XCDs' data): the records this tile reads were written
+  // back by their producers before the item was published, and no copy of
+  // them cached here, from this launch or an earlier one using the same
+  // addresses, may be read in their place.
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return it;
 }
 
@@ -201,6 +234,31 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) void 
       }
       __syncthreads();
     }
+  }
+}
+
+// Special hairpins (hl_energy's strMatch over Triloops / Tetraloops /
+// Hexaloops, librna_impl.h:154-193): spec[cls][p] = energy of the special
+// loop of size 3, 4 or 6 (cls 0, 1, 2) whose closing pair starts at code
+// position p, i.e. the size+2 bases codes[p..], or SPEC_NONE.  One thread
+// per position; instances are separated by a 0 code, which no entry holds.
+__global__ void tri_spec(TriArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n_codes) return;
+  const DevRna* P = (const DevRna*)a.rna;
+  for (int cls = 0; cls < 3; ++cls) {
+    const int len = (cls == 0 ? 3 : cls == 1 ? 4 : 6) + 2;
+    int v = SPEC_NONE;
+    if (p + len <= a.n_codes) {
+      int key = 0;
+      for (int m = 0; m < len; ++m) key = key * 8 + a.codes[p + m];
+      for (int e = 0; e < P->n_loop[cls]; ++e)
+        if (P->loop_key[cls][e] == key) {
+          v = P->loop_e[cls][e];
+          break;
+        }
+    }
+    a.spec[cls * a.n_codes + p] = (int16_t)v;
   }
 }
 
@@ -456,7 +514,12 @@ struct ZuView {
   const uint8_t* rs;
   const uint8_t* cs;
   int IT1, JT34, n;
+  const int16_t* sp;  // special-hairpin energies of this instance, [cls * spn + position]
+  int64_t spn;
   __device__ int R(int x) const { return rs[x - IT1]; }
+  // energy of the special tri- (cls 0), tetra- (1) or hexaloop (2) closed by
+  // the pair starting at position x, SPEC_NONE if there is none
+  __device__ int spec(int x, int cls) const { return sp[cls * spn + x]; }
   __device__ int C(int x) const { return cs[x - JT34]; }
   __device__ int tau(int x, int y) const {  // termau_energy on bases (librna_impl.h:152)
     return ((x == 3 && y == 2) || (x == 2 && y == 3)) ? 0 : E.term_au;
@@ -469,40 +532,35 @@ struct ZuView {
   __device__ int sr(int i, int j) const { return E.stack[bpc(R(i), C(j - 1))][bpc(C(j - 2), R(i + 1))]; }
   // Sequence-only terms of cell (i,j).  The device-memory table reads are
   // independent of each other, so they are all issued before any is used.
-  __device__ void pre(int i, int j, ZuPre* o) const {
+  // Returns whether cl(i,j) exists (o->valid).
+  __device__ bool pre(int i, int j, ZuPre* o) const {
     const int d = j - i;
     o->valid = 0;
-    if (j > n || d < 7 || !stackpair(i, j)) return;
+    if (j > n || d < 7 || !stackpair(i, j)) return false;
     o->valid = 1;
     const int Ri = R(i), Ri1 = R(i + 1), Ri2 = R(i + 2), Ri3 = R(i + 3), Ri4 = R(i + 4);
     const int Cj1 = C(j - 1), Cj2 = C(j - 2), Cj3 = C(j - 3), Cj4 = C(j - 4), Cj5 = C(j - 5);
     const int ocb = bpc(Ri1, Cj2), bo = bpc(Ri, Cj1);
     // hl_energy(i+1, j-2) generic part (librna_impl.h:181-193), loop size d-4
     const int size = d - 4;
-    const int hent = size <= 30 ? P->hairpin[size] : hl[size];
+    const int hent = hl[size];  // hairpin[size] up to 30, extrapolated beyond
     const int hmm = P->mmH[ocb][Ri2][Cj3];
     // ml_mismatch of the closing pair (rule 5), ext_mismatch(i, j-1) (rule 10/U)
     const int mlm = P->mmM[bpc(Cj2, Ri1)][Cj3][Ri2];
     const int* ep = (i > 0 && j < n) ? &P->mmExt[bo][R(i - 1)][C(j)] : (i > 0 ? &P->d5[bo][R(i - 1)] : &P->d3[bo][C(j)]);
     const int ex = (i > 0 || j < n) ? *ep : 0;
-    // 1x1, 1x2, 2x1, 2x2 loop energies (il11/12/21/22_energy, librna_impl.h:222-225)
-    int e11 = 0, e12 = 0, e21 = 0, e22 = 0;
-    if (d >= 12) {
-      e11 = P->int11[ocb][bpc(Cj4, Ri3)][Ri2][Cj3];
-      e12 = P->int21[ocb][bpc(Cj5, Ri3)][Ri2][Cj4][Cj3];
-      e21 = P->int21[bpc(Cj4, Ri4)][ocb][Cj3][Ri2][Ri3];
-      e22 = P->int22[ocb][bpc(Cj5, Ri4)][Ri2][Ri3][Cj4][Cj3];
-    }
+    // 1x1, 1x2, 2x1, 2x2 loop energies (il11/12/21/22_energy, librna_impl.h:222-225):
+    // read for every cell with the other loads (the indices are base codes
+    // and pair types, always inside the tables), used when d >= 12
+    const int e11 = P->int11[ocb][bpc(Cj4, Ri3)][Ri2][Cj3];
+    const int e12 = P->int21[ocb][bpc(Cj5, Ri3)][Ri2][Cj4][Cj3];
+    const int e21 = P->int21[bpc(Cj4, Ri4)][ocb][Cj3][Ri2][Ri3];
+    const int e22 = P->int22[ocb][bpc(Cj5, Ri4)][Ri2][Ri3][Cj4][Cj3];
     int hp = size == 3 ? hent + tau(Ri1, Cj2) : hent + hmm;
     const int cls = size == 3 ? 0 : size == 4 ? 1 : size == 6 ? 2 : -1;
-    if (cls >= 0) {  // special tri-/tetra-/hexaloops: the loop lies on the row side (size <= 6)
-      int key = 0;
-      for (int m = 0; m <= size + 1; ++m) key = key * 8 + R(i + 1 + m);
-      for (int e = 0; e < P->n_loop[cls]; ++e)
-        if (P->loop_key[cls][e] == key) {
-          hp = P->loop_e[cls][e];
-          break;
-        }
+    if (cls >= 0) {  // special tri-/tetra-/hexaloops, matched once per position by tri_spec
+      const int sv = spec(i + 1, cls);
+      if (sv != SPEC_NONE) hp = sv;
     }
     const int s_ = E.stack[bo][bpc(Cj2, Ri1)];
     o->sr = s_;
@@ -518,6 +576,7 @@ struct ZuView {
       o->f21 = (int16_t)(e21 - inner(i + 4, j - 3));
       o->f22 = (int16_t)(e22 - inner(i + 4, j - 4));
     }
+    return true;
   }
 };
 
@@ -564,43 +623,32 @@ struct GZu {
     uint8_t rs[72], cs[72];
     int acc[G][T];
     int16_t list[G * T];
-    int nlist;
+    int nlist[2];  // by band parity: the next band's list is counted while this one's is read
   };
 
   __device__ static void init(const TriArgs& a, Smem& sm) { load_tab(a, sm.E); }
 
-  // sequence-only terms of the cells of band [ld0, ld0+G) and the compacted
-  // list of cells whose cl can close bulges / interior loops; all threads of
-  // `nthr` starting at thread index `t0` take part, wave `lw` builds the list
+  // Sequence-only terms of the cells of band [ld0, ld0+G) into pre[buf], and
+  // the list of cells whose cl can close bulges / interior loops (the cells
+  // with cl and j - i >= 12), in any order: each wave appends its cells with
+  // one LDS atomic on nlist[buf], which must be 0 beforehand.  Called by
+  // whole waves, t in [0, 192) (cell t of the band's G*T <= 160).
   __device__ static void prep_band(const ZuView& z, int IT, int JT, int ld0, Smem& sm, int t, int buf) {
-    // t in [0, 192): the first wave builds the cell list, then all 192
-    // threads compute the per-cell terms of the band's G*T <= 160 cells
-    if (t < 64) {
-      int base = 0;
-      for (int q = 0; q < G; ++q) {
-        const int ld = ld0 + q;
-        bool ok = false;
-        if (ld <= T - 1) {
-          const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
-          if (t < ncell) {
-            const int r = rmin + t, i = IT + r, j = JT + r + ld;
-            ok = j <= z.n && j - i >= 12 && z.stackpair(i, j);
-          }
-        }
-        const unsigned long long m = __ballot(ok);
-        if (ok) sm.list[base + __popcll(m & ((1ull << t) - 1))] = (int16_t)(q * T + t);
-        base += __popcll(m);
-      }
-      if (t == 0) sm.nlist = base;
-    }
-    const int e = t < 64 ? 128 + t : t - 64;
-    if (e < G * T) {
-      const int q = e / T, p = e % T, ld = ld0 + q;
+    bool listed = false;
+    const int q = t / T, p = t % T, ld = ld0 + q;
+    if (t < G * T) {
       const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
       if (ld <= T - 1 && p < ncell) {
         const int r = rmin + p;
-        z.pre(IT + r, JT + r + ld, &sm.pre[buf][q][p]);
+        listed = z.pre(IT + r, JT + r + ld, &sm.pre[buf][q][p]) && ld + JT - IT >= 12;
       }
+    }
+    const unsigned long long m = __ballot(listed);
+    if (m) {
+      int base = 0;
+      if ((threadIdx.x & 63) == 0) base = atomicAdd(&sm.nlist[buf], __popcll(m));
+      base = __builtin_amdgcn_readfirstlane(base);
+      if (listed) sm.list[base + __popcll(m & ((1ull << (threadIdx.x & 63)) - 1))] = (int16_t)(q * T + p);
     }
   }
 
@@ -625,8 +673,15 @@ struct GZu {
       const int aa = ap + 1;
       const int* Brow = &sm.u.w.B[r + 2 + aa][c + 30];  // Brow[-b] = B(i+2+aa, j-2-b)
       int g = INV;
+      // all 29 loads issued before the adds: the scheduler would otherwise
+      // reuse two registers and wait for every load in turn
+      int v[29];
 #pragma unroll
-      for (int b = 2; b <= 30; ++b) g = imin(g, Brow[-b] + fg[b - 2]);
+      for (int b = 2; b <= 30; ++b) v[b - 2] = Brow[-b];
+      __builtin_amdgcn_sched_group_barrier(0x100, 15, 0);  // LDS reads
+      __builtin_amdgcn_sched_group_barrier(0x002, 58, 0);  // VALU
+#pragma unroll
+      for (int b = 2; b <= 30; ++b) g = imin(g, v[b - 2] + fg[b - 2]);
       best = g + pc.outI;
       const int Rl = z.R(i + 2 + aa), Rl1 = z.R(i + 1 + aa);
       const int out1n = E.mm1nI[ocb][Ri2][Cj3] + pc.sr, tauo = z.tau(Ri1, Cj2);
@@ -686,12 +741,13 @@ struct GZu {
     const int tid = threadIdx.x, IT = I * T, JT = J * T, n = in.n, nb = in.nb;
     const uint8_t* seq = a.codes + in.seq_off;
     const DevRna* P = (const DevRna*)a.rna;
-    const ZuView z{sm.E, P, a.hl_ent, sm.rs, sm.cs, IT - 1, JT - 34, n};
+    const ZuView z{sm.E, P, a.hl_ent, sm.rs, sm.cs, IT - 1, JT - 34, n, a.spec + in.seq_off, a.n_codes};
     int acc[4][4];
     for (int t = 0; t < 4; ++t)
       for (int u = 0; u < 4; ++u) acc[t][u] = INV;
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = INV;
     for (int e = tid; e < G * T; e += NTHR) (&sm.acc[0][0])[e] = INV;
+    if (tid == 0) sm.nlist[0] = 0;
     if (tid < 72) {
       const int x = IT - 1 + tid, y = JT - 34 + tid;
       sm.rs[tid] = (x >= 0 && x < n) ? seq[x] : 0;
@@ -774,7 +830,8 @@ struct GZu {
       int cpre[G + 1];
       cpre[0] = 0;
       for (int q = 0; q < G; ++q) cpre[q + 1] = cpre[q] + (q < qn ? T - abs(ld0 + q) : 0);
-      const int nv = sm.nlist, wl = ((nv + 1) >> 1) * 64, total = wl + 2 * cpre[G];
+      const int nv = sm.nlist[buf], wl = ((nv + 1) >> 1) * 64, total = wl + 2 * cpre[G];
+      if (tid == 0) sm.nlist[buf ^ 1] = 0;  // counted by the next band's prep, after the barrier
       for (int w = tid; w < total; w += NTHR) {
         if (w < wl) {
           const int v = w >> 5;
@@ -791,21 +848,29 @@ struct GZu {
           int best = INV;
           if (part == 0) {
             if (I == J) {
-#pragma unroll 8
-              for (int x = 0; x < T; ++x) best = imin(best, sm.u.w.U[r][x + 1] + sm.u.w.ML1[x][c]);
+              best = lds_minplus8(&sm.u.w.U[r][1], &sm.u.w.ML1[0][c], T + 1);
             } else {
+              // U(I,I) row r from HBM/L2, eight values ahead of their use
               const int4* g4 = (const int4*)(gLU + r * T);
-#pragma unroll 2
-              for (int x4 = 0; x4 < T / 4; ++x4) {
-                const int4 u = g4[x4];
-                const int x = x4 * 4;
-                best = imin(imin(best, u.x + sm.u.w.ML1[x][c]), u.y + sm.u.w.ML1[x + 1][c]);
-                best = imin(imin(best, u.z + sm.u.w.ML1[x + 2][c]), u.w + sm.u.w.ML1[x + 3][c]);
+              int4 ua = g4[0], ub = g4[1];
+#pragma unroll
+              for (int x0 = 0; x0 < T; x0 += 8) {
+                int4 na = ua, nb = ub;
+                if (x0 + 8 < T) {
+                  na = g4[(x0 + 8) / 4];
+                  nb = g4[(x0 + 8) / 4 + 1];
+                }
+                int bv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) bv[u] = sm.u.w.ML1[x0 + u][c];
+                best = imin(best, imin(imin(ua.x + bv[0], ua.y + bv[1]), imin(ua.z + bv[2], ua.w + bv[3])));
+                best = imin(best, imin(imin(ub.x + bv[4], ub.y + bv[5]), imin(ub.z + bv[6], ub.w + bv[7])));
+                ua = na;
+                ub = nb;
               }
             }
           } else if (I < J) {
-#pragma unroll 8
-            for (int x = 0; x < T; ++x) best = imin(best, sm.u.w.U[r][x + 1] + sm.u.w.RM1[x][c]);
+            best = lds_minplus8(&sm.u.w.U[r][1], &sm.u.w.RM1[0][c], T + 1);
           }
           best = imin(best, __shfl_xor(best, 1));
           if (part == 0 && best < INV_THR) sm.X[r][c] = imin(sm.X[r][c], best);
@@ -823,36 +888,28 @@ struct GZu {
           const int ld = ld0 + q, rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
           if (tid < ncell) {
           const int p = tid, r = rmin + p, c = r + ld, i = IT + r, j = JT + c, d = j - i;
-          int cl = INV, U = INV, ml = INV, ml1 = INV, R11 = INV, Bv = INV;
-          if (i <= j && j <= n) {
-            const ZuPre& pc = sm.pre[buf][q][p];
-            if (pc.valid) {
-              cl = imin(sm.acc[q][p], pc.r1);
-              if (d >= 9) {
-                const int Bi = sm.u.w.B[r + 1][c + 31];
-                if (Bi < INV_THR) cl = imin(cl, Bi - pc.in0 + pc.sr);
-              }
-              if (d >= 18) {
-                const int mlv = sm.u.w.ML[r + 2][c];
-                if (mlv < INV_THR) cl = imin(cl, mlv + pc.r5c);
-              }
-              U = cl + pc.uoff;
-              if (pc.boff != kNoB) Bv = cl + pc.boff;
-            }
-            int X = sm.X[r][c];
-            if (X >= INV_THR) X = INV;
-            ml = X;
-            if (d >= 15) ml = imin(ml, sm.u.w.ML[r + 1][c + 2]);
-            if (d >= 8) {
-              const int Rp = sm.u.w.R11[r][c], Up = sm.u.w.U[r][c];
-              R11 = (d >= 9) ? imin(Rp, Up) : Up;
-            }
-            ml1 = imin(imin(X, U), R11);
-            if (d >= 8) ml1 = imin(ml1, sm.u.w.ML1[r + 1][c]);
-            if (ml >= INV_THR) ml = INV;
-            if (ml1 >= INV_THR) ml1 = INV;
-            if (R11 >= INV_THR) R11 = INV;
-          }
+          // every operand first (finished diagonals and this cell's own
+          // terms; all addresses inside the arrays), then branch-free
+          // arithmetic: one LDS round trip per diagonal step
+          const ZuPre pc = sm.pre[buf][q][p];
+          const int accv = sm.acc[q][p], Bi = sm.u.w.B[r + 1][c + 31], ml2 = sm.u.w.ML[r + 2][c];
+          const int Xv = sm.X[r][c], mlp = sm.u.w.ML[r + 1][c + 2], m1p = sm.u.w.ML1[r + 1][c];
+          const int Rp = sm.u.w.R11[r][c], Up = sm.u.w.U[r][c];
+          const bool live = i <= j && j <= n, v = live && pc.valid;
+          int cl = imin(accv, pc.r1);
+          cl = d >= 9 && Bi < INV_THR ? imin(cl, Bi - pc.in0 + pc.sr) : cl;
+          cl = d >= 18 && ml2 < INV_THR ? imin(cl, ml2 + pc.r5c) : cl;
+          cl = v ? cl : INV;
+          const int U = v ? cl + pc.uoff : INV;
+          const int Bv = v && pc.boff != kNoB ? cl + pc.boff : INV;
+          const int X = Xv >= INV_THR ? INV : Xv;
+          int ml = d >= 15 ? imin(X, mlp) : X;
+          int R11 = d >= 8 ? (d >= 9 ? imin(Rp, Up) : Up) : INV;
+          int ml1 = imin(imin(X, U), R11);
+          ml1 = d >= 8 ? imin(ml1, m1p) : ml1;
+          ml = live && ml < INV_THR ? ml : INV;
+          ml1 = live && ml1 < INV_THR ? ml1 : INV;
+          R11 = live && R11 < INV_THR ? R11 : INV;
           sm.u.w.B[r][c + 32] = Bv;
           sm.u.w.U[r][c + 1] = U;
           sm.u.w.ML[r][c + 2] = ml;
@@ -911,7 +968,7 @@ struct GRf {
     uint8_t rs[72], cs[72];
     int acc[G][T];
     int16_t list[G * T];
-    int nlist;
+    int nlist[2];  // by band parity
   };
 
   __device__ static void init(const TriArgs& a, Smem& sm) { load_tab(a, sm.E); }
@@ -922,10 +979,10 @@ struct GRf {
   // outI = generic outer mismatch of (i, j-1); boff / in0 = interior
   // mismatches of (i,j) and (i+1,j-1) as inner pairs; f11..f22 the int-table
   // loops minus their inner mismatch (sizes (1,1), (1,2), (2,1), (2,2)).
-  __device__ static void pre(const ZuView& z, int i, int j, ZuPre* o) {
+  __device__ static bool pre(const ZuView& z, int i, int j, ZuPre* o) {
     const int d = j - i, n = z.n;
     o->valid = 0;
-    if (j > n || d < 2 || bpc(z.R(i), z.C(j - 1)) == 7) return;
+    if (j > n || d < 2 || bpc(z.R(i), z.C(j - 1)) == 7) return false;
     o->valid = 1;
     const DevRna* P = z.P;
     const ZuTab& E = z.E;
@@ -936,18 +993,13 @@ struct GRf {
     const int size = d - 2;
     int hp = 600;
     if (size >= 3) {
-      const int ent = size <= 30 ? P->hairpin[size] : z.hl[size];
+      const int ent = z.hl[size];  // hairpin[size] up to 30, extrapolated beyond
       const int hmm = P->mmH[ocb][Ri1][Cj2];
       hp = size == 3 ? ent + z.tau(Ri, Cj1) : ent + hmm;
       const int cls = size == 3 ? 0 : size == 4 ? 1 : size == 6 ? 2 : -1;
       if (cls >= 0) {
-        int key = 0;
-        for (int m = 0; m <= size + 1; ++m) key = key * 8 + z.R(i + m);
-        for (int e = 0; e < P->n_loop[cls]; ++e)
-          if (P->loop_key[cls][e] == key) {
-            hp = P->loop_e[cls][e];
-            break;
-          }
+        const int sv = z.spec(i, cls);
+        if (sv != SPEC_NONE) hp = sv;
       }
     }
     int e11 = 0, e12 = 0, e21 = 0, e22 = 0;
@@ -970,32 +1022,28 @@ struct GRf {
       o->f21 = (int16_t)(e21 - z.inner(i + 3, j - 2));
       o->f22 = (int16_t)(e22 - z.inner(i + 3, j - 3));
     }
+    return true;
   }
 
+  // Sequence-only terms of band [ld0, ld0+G) into pre[buf] and the list of
+  // cells that can close an interior loop (Qp exists, j - i >= 6), in any
+  // order: each wave appends its cells with one LDS atomic on nlist[buf],
+  // which must be 0 beforehand.  Called by whole waves, t in [0, 192)
+  // (cell t of the band's G*T = 96).
   __device__ static void prep_band(const ZuView& z, int IT, int JT, int ld0, Smem& sm, int t, int buf) {
-    if (t < 64) {
-      int base = 0;
-      for (int q = 0; q < G; ++q) {
-        const int ld = ld0 + q;
-        bool ok = false;
-        if (ld <= T - 1) {
-          const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
-          if (t < ncell) {
-            const int r = rmin + t, i = IT + r, j = JT + r + ld;
-            ok = j <= z.n && j - i >= 6 && bpc(z.R(i), z.C(j - 1)) != 7;
-          }
-        }
-        const unsigned long long m = __ballot(ok);
-        if (ok) sm.list[base + __popcll(m & ((1ull << t) - 1))] = (int16_t)(q * T + t);
-        base += __popcll(m);
-      }
-      if (t == 0) sm.nlist = base;
-    }
-    const int e = t < 64 ? t + 128 : t - 64;  // the band's G*T = 96 cells
-    if (e < G * T) {
-      const int q = e / T, p = e % T, ld = ld0 + q;
+    bool listed = false;
+    const int q = t / T, p = t % T, ld = ld0 + q;
+    if (t < G * T) {
       const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
-      if (ld <= T - 1 && p < ncell) pre(z, IT + rmin + p, JT + rmin + p + ld, &sm.pre[buf][q][p]);
+      if (ld <= T - 1 && p < ncell)
+        listed = pre(z, IT + rmin + p, JT + rmin + p + ld, &sm.pre[buf][q][p]) && ld + JT - IT >= 6;
+    }
+    const unsigned long long m = __ballot(listed);
+    if (m) {
+      int base = 0;
+      if ((threadIdx.x & 63) == 0) base = atomicAdd(&sm.nlist[buf], __popcll(m));
+      base = __builtin_amdgcn_readfirstlane(base);
+      if (listed) sm.list[base + __popcll(m & ((1ull << (threadIdx.x & 63)) - 1))] = (int16_t)(q * T + p);
     }
   }
 
@@ -1003,12 +1051,13 @@ struct GRf {
     const int tid = threadIdx.x, IT = I * T, JT = J * T, n = in.n, nb = in.nb;
     const uint8_t* seq = a.codes + in.seq_off;
     const DevRna* P = (const DevRna*)a.rna;
-    const ZuView z{sm.E, P, a.hl_ent, sm.rs, sm.cs, IT - 1, JT - 34, n};
+    const ZuView z{sm.E, P, a.hl_ent, sm.rs, sm.cs, IT - 1, JT - 34, n, a.spec + in.seq_off, a.n_codes};
     int acc[4][4];
     for (int t = 0; t < 4; ++t)
       for (int u = 0; u < 4; ++u) acc[t][u] = INV;
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = INV;
     for (int e = tid; e < G * T; e += NTHR) (&sm.acc[0][0])[e] = INV;
+    if (tid == 0) sm.nlist[0] = 0;
     if (tid < 72) {
       const int x = IT - 1 + tid, y = JT - 34 + tid;
       sm.rs[tid] = (x >= 0 && x < n) ? seq[x] : 0;
@@ -1076,7 +1125,8 @@ struct GRf {
       int cpre[G + 1];
       cpre[0] = 0;
       for (int q = 0; q < G; ++q) cpre[q + 1] = cpre[q] + (q < qn ? T - abs(ld0 + q) : 0);
-      const int nv = sm.nlist, wl = ((nv + 1) >> 1) * 64, total = wl + 2 * cpre[G];
+      const int nv = sm.nlist[buf], wl = ((nv + 1) >> 1) * 64, total = wl + 2 * cpre[G];
+      if (tid == 0) sm.nlist[buf ^ 1] = 0;  // counted by the next band's prep, after the barrier
       for (int w = tid; w < total; w += NTHR) {
         if (w < wl) {
           const int v = w >> 5;
@@ -1125,25 +1175,23 @@ struct GRf {
           const int ld = ld0 + q, rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
           if (tid < ncell) {
             const int p = tid, r = rmin + p, c = r + ld, i = IT + r, j = JT + c, d = j - i;
-            int qp = INV, qm = INV, qv = INV, Bv = INV;
-            if (i <= j && j <= n) {
-              const ZuPre& pc = sm.pre[buf][q][p];
-              if (pc.valid) {
-                qp = imin(sm.acc[q][p], d >= 3 ? (int)pc.r1 : INV);
-                const int Bi = sm.u.w.B[r + 1][c + 31];  // B(i+1, j-1): the stack
-                if (d >= 4 && Bi < INV_THR) qp = imin(qp, Bi - pc.in0 + pc.r5c);
-                const int mlv = sm.u.w.QM[r + 1][c];      // QM(i+1, j-1): the multiloop
-                if (mlv < INV_THR) qp = imin(qp, mlv);
-                if (qp >= INV_THR) qp = INV;
-                if (qp < INV_THR && pc.boff != kNoB) Bv = qp + pc.boff;
-              }
-              qm = sm.X[r][c];
-              if (qm >= INV_THR || d < 4) qm = INV;
-              if (d >= 2) {
-                qv = imin(imin(qm, qp), imin(sm.u.w.Q[r][c], sm.u.w.Q[r + 1][c + 1]));
-                if (qv >= INV_THR) qv = INV;
-              }
-            }
+            // every operand first (finished diagonals and this cell's own
+            // terms; all addresses inside the arrays), then branch-free
+            // arithmetic: one LDS round trip per diagonal step
+            const ZuPre pc = sm.pre[buf][q][p];
+            const int accv = sm.acc[q][p];
+            const int Bi = sm.u.w.B[r + 1][c + 31];  // B(i+1, j-1): the stack
+            const int mlv = sm.u.w.QM[r + 1][c];     // QM(i+1, j-1): the multiloop
+            const int Xv = sm.X[r][c], Ql = sm.u.w.Q[r][c], Qd = sm.u.w.Q[r + 1][c + 1];
+            const bool live = i <= j && j <= n, v = live && pc.valid;
+            int qp = imin(accv, d >= 3 ? (int)pc.r1 : INV);
+            qp = d >= 4 && Bi < INV_THR ? imin(qp, Bi - pc.in0 + pc.r5c) : qp;
+            qp = mlv < INV_THR ? imin(qp, mlv) : qp;
+            qp = v && qp < INV_THR ? qp : INV;
+            const int Bv = qp < INV_THR && pc.boff != kNoB ? qp + pc.boff : INV;
+            const int qm = live && Xv < INV_THR && d >= 4 ? Xv : INV;
+            int qv = imin(imin(qm, qp), imin(Ql, Qd));
+            qv = live && d >= 2 && qv < INV_THR ? qv : INV;
             sm.u.w.B[r][c + 32] = Bv;
             sm.u.w.QM[r][c + 1] = qm;
             sm.u.w.Q[r][c + 1] = qv;
@@ -1179,6 +1227,8 @@ int fill_occupancy(int grammar) {
 int launch_fill(int grammar, const TriArgs& a, const unsigned long long* seeds, int nseed, int grid, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(tri_reset, dim3((a.total_tiles + 255) / 256), dim3(256), 0, st, a, seeds, nseed);
+  if (a.spec && a.n_codes > 0)
+    hipLaunchKernelGGL(tri_spec, dim3((unsigned)((a.n_codes + 255) / 256)), dim3(256), 0, st, a);
   if (grammar == G_NUSSINOV) hipLaunchKernelGGL(tri_fill<GNu>, dim3(grid), dim3(NTHR), 0, st, a);
   else if (grammar == G_MATCHAIN) hipLaunchKernelGGL(tri_fill<GMc>, dim3(grid), dim3(NTHR), 0, st, a);
   else if (grammar == G_RNAFOLD) hipLaunchKernelGGL(tri_fill<GRf>, dim3(grid), dim3(NTHR), 0, st, a);
