@@ -260,6 +260,7 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
   // one wave from D values staged in LDS.
   __shared__ int far[T], nd[T][T + 1];
   if (threadIdx.x == 0) st[n] = 0;
+  __syncthreads();  // st(n) is read by every thread of the first block's far pass
   const int tid = threadIdx.x, lane = tid & 31;
   for (int hi = n - 1; hi >= 0; hi -= T) {
     const int lo = fmax_(0, hi - T + 1);

@@ -89,3 +89,16 @@ def test_autotune_reports_and_keeps_fastest(ctx, kind):
         assert all(r["count"] >= 1 and r["n_lo"] <= r["n_hi"] for r in rows)
     for b in range(0, 3):
         ctx.set_tuning(kind, b)
+
+
+def test_zuker_launch_shapes_agree_repeatedly():
+    """Regression: the Zuker finalize once read st(n) before thread 0 had
+    written it (an LDS race, visible as launches of the same batch disagreeing
+    now and then).  Every autotune compares all launch shapes bit for bit."""
+    import dpx
+    for it in range(12):
+        c = dpx.Context()
+        try:
+            c.autotune("zuker", buckets=range(0, 2), reps=1)
+        finally:
+            c.close()
