@@ -264,6 +264,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--autotune", action="store_true", help="re-run the autotuner instead of loading its table")
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
@@ -287,15 +288,19 @@ def main():
     total_x = sum(x_candidates(len(s)) for s in seqs)
     print("bench: %d sequences, %.3e cells" % (len(seqs), total_cells), file=sys.stderr, flush=True)
 
+    # launch shape per length bucket: the context loads the table the
+    # autotuner produced on MI355X; --autotune re-runs the autotuner here
     ctx = dpx.Context()
-    # launch shape per length bucket: the autotuner times the candidate
-    # shapes on synthetic launches of each bucket and keeps the fastest
-    t0 = time.perf_counter()
-    tune_rep = ctx.autotune("zuker", reps=1)
-    tuning = {"seconds": round(time.perf_counter() - t0, 2),
-              "chosen": {"b%d(%d-%d)" % (r["bucket"], r["n_lo"], r["n_hi"]): {"wg_per_cu": r["wg_per_cu"], "ms": r["ms"]}
-                         for r in tune_rep if r["chosen"]}}
-    print("bench: autotune %s" % json.dumps(tuning), file=sys.stderr, flush=True)
+    if args.autotune:
+        t0 = time.perf_counter()
+        tune_rep = ctx.autotune("zuker", reps=1)
+        tuning = {"source": "dpx_autotune in this run (%.2f s)" % (time.perf_counter() - t0),
+                  "zuker": {"b%d(%d-%d)" % (r["bucket"], r["n_lo"], r["n_hi"]): r["wg_per_cu"]
+                            for r in tune_rep if r["chosen"]}}
+    else:
+        tuning = {"source": os.path.relpath(dpx.TUNING_MI355X, ROOT) + " (dpx_autotune on MI355X)",
+                  "zuker": {"b%d" % b: ctx.get_tuning("zuker", b)["wg_per_cu"] for b in range(dpx.TUNE_BUCKETS)}}
+    print("bench: tuning %s" % json.dumps(tuning), file=sys.stderr, flush=True)
     # eager launches: warm-up, the fill / finalize split and the C call time
     fills, fins, walls = [], [], []
     results = traces = None

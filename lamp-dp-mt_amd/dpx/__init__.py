@@ -10,6 +10,8 @@ import time
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libdpx.so")
 PAR_TURNER2004 = os.path.join(PKG, "params", "rna_turner2004.par")
+# launch shapes chosen by the autotuner on MI355X (dpx_autotune), per kind and length bucket
+TUNING_MI355X = os.path.join(PKG, "tuning", "mi355x.json")
 # every parameter file shipped (ViennaRNA 2.0 format), by name
 PAR_FILES = {"turner2004": PAR_TURNER2004,
              "turner1999": os.path.join(PKG, "params", "rna_turner1999.par"),
@@ -132,7 +134,7 @@ def _tuples(buf, n):
 class Context:
     """One device context: resident energy tables and DP scratch."""
 
-    def __init__(self, device=0, rna_params=PAR_TURNER2004):
+    def __init__(self, device=0, rna_params=PAR_TURNER2004, tuning=TUNING_MI355X):
         self._L = lib()
         h = ctypes.c_void_p()
         rc = self._L.dpx_create(ctypes.byref(h), device)
@@ -141,6 +143,8 @@ class Context:
         self.h = h
         if rna_params:
             self.set_rna_params(rna_params)
+        if tuning:
+            self.load_tuning_file(tuning)
 
     def close(self):
         if self.h:
@@ -269,6 +273,12 @@ class Context:
     def tuning_table(self):
         """{kind name: [per-bucket tuning]} -- what save/load exchange as JSON."""
         return {name: [self.get_tuning(k, b) for b in range(TUNE_BUCKETS)] for k, name in KIND_NAMES.items()}
+
+    def load_tuning_file(self, path):
+        """A tuning table saved as JSON ({"table": {kind: [per-bucket shape]}})."""
+        import json
+        with open(path) as f:
+            self.load_tuning_table(json.load(f)["table"])
 
     def load_tuning_table(self, table):
         for name, rows in table.items():

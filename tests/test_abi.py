@@ -52,3 +52,23 @@ def test_struct_layouts_match_header(tmp_path):
         want.append(ctypes.sizeof(py))
         want += [getattr(py, f[0]).offset for f in py._fields_]
     assert got == want
+
+
+def test_tuning_table_file():
+    """The committed MI355X tuning table: every kind, every bucket, shapes the
+    library accepts (dpx_set_tuning's rules)."""
+    import json
+    with open(dpx.TUNING_MI355X) as f:
+        d = json.load(f)
+    t = d["table"]
+    assert set(t) == set(dpx.KIND_NAMES.values())
+    for kind, rows in t.items():
+        assert len(rows) == dpx.TUNE_BUCKETS
+        for r in rows:
+            assert r["wg_per_cu"] >= 0 and r["chunk"] in (0, 16, 32, 64)
+    for kind, rep in d["report"].items():
+        for b in range(dpx.TUNE_BUCKETS):
+            rows = [x for x in rep if x["bucket"] == b]
+            chosen = [x for x in rows if x["chosen"]]
+            assert len(chosen) == 1 and chosen[0]["ms"] == min(x["ms"] for x in rows)
+            assert t[kind][b] == {"wg_per_cu": chosen[0]["wg_per_cu"], "chunk": chosen[0]["chunk"]}
