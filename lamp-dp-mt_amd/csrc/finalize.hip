@@ -51,27 +51,80 @@ struct Cells {
   }
 };
 
-// Shared BFS driver.  G provides: ncand(e), value(e,c,&v) -> bool, target(e),
-// decode(e,c,&rule,&p0,&p1), push children via the lambda.
-struct Bfs {
-  int rd, wr, first, overflow;
-  TraceRec cur;
-};
-
+// Smallest c in [0, ncand) whose candidate value equals target, -1 if none:
+// the 64 lanes of a wave test 64 candidates at a time (wave-uniform result).
 template <class Fn>
-__device__ void first_match(int ncand, int target, Fn&& cand, int* s_first) {
-  // smallest c in [0,ncand) with cand(c) == target, 0x7fffffff if none
-  if (threadIdx.x == 0) *s_first = 0x7fffffff;
-  __syncthreads();
-  for (int base = 0; base < ncand; base += NTHR) {
-    const int c = base + threadIdx.x;
-    int v;
-    if (c < ncand && cand(c, &v) && v == target) atomicMin(s_first, c);
-    __syncthreads();
-    const bool found = *s_first != 0x7fffffff;
-    __syncthreads();
-    if (found) return;
+__device__ int wave_first_match(int ncand, int target, Fn&& cand) {
+  const int lane = threadIdx.x & 63;
+  for (int base = 0; base < ncand; base += 64) {
+    const int c = base + lane;
+    int v = 0;
+    const bool hit = c < ncand && cand(c, &v) && v == target;
+    const unsigned long long m = __ballot(hit);
+    if (m) return base + (int)__ffsll((unsigned long long)m) - 1;
   }
+  return -1;
+}
+
+// BFS backtrace driver.  The reference takes its queued records one at a
+// time (zu.c:49-93, CodeGen.scala:233-252); a record's rule depends only on
+// its own cell, so each round here resolves up to NWAVE queued records, one
+// per wave, and then appends their children in record order: the same
+// records, in the same order.  rule_of(rec, &rule, &p0, &p1) runs on one
+// wave and returns false when no candidate matches; kids(rec, push) appends
+// the children of a resolved record.  *len receives the record count, or -1
+// (no match, or more records than cap).
+template <class RuleOf, class Kids>
+__device__ void bfs_rounds(TraceRec* tr, int cap, TraceRec root, RuleOf&& rule_of, Kids&& kids, int32_t* len) {
+  __shared__ int s_rd, s_wr, s_over;
+  __shared__ TraceRec s_rec[NWAVE];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) {
+    s_rd = 0;
+    s_wr = 0;
+    s_over = 0;
+    if (cap > 0) {
+      tr[0] = root;
+      s_wr = 1;
+    } else {
+      s_over = 1;
+    }
+  }
+  __syncthreads();
+  for (;;) {
+    const int rd = s_rd, wr = s_wr;
+    if (rd >= wr || s_over) break;
+    const int m = wr - rd < NWAVE ? wr - rd : NWAVE;
+    if (w < m) {
+      TraceRec cur = tr[rd + w];
+      int rule = -1, p0 = 0, p1 = 0;
+      const bool ok = rule_of(cur, &rule, &p0, &p1);
+      if (lane == 0) {
+        if (!ok) s_over = 1;
+        cur.rule = rule;
+        cur.pos[0] = p0;
+        cur.pos[1] = p1;
+        tr[rd + w] = cur;
+        s_rec[w] = cur;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int wr2 = wr;
+      auto push = [&](int pi, int pj, int pr) {
+        if (wr2 >= cap) {
+          s_over = 1;
+          return;
+        }
+        tr[wr2++] = TraceRec{pi, pj, pr, {0, 0}};
+      };
+      for (int q = 0; q < m && !s_over; ++q) kids(s_rec[q], push);
+      s_rd = rd + m;
+      s_wr = wr2;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *len = s_over ? -1 : s_wr;
 }
 
 // ----------------------------------------------------------------- Nussinov
@@ -80,21 +133,10 @@ __global__ __launch_bounds__(NTHR) void nu_finalize(TriArgs a) {
   const Cells C{a.tab, in.nb, NT_NU, in.tile0};
   const uint8_t* s = a.codes + in.seq_off;
   const int n = in.n;
-  __shared__ Bfs b;
   if (threadIdx.x == 0) a.results[in.out] = C.get(0, 0, n);
   if (!a.want_trace) return;
-  TraceRec* tr = (TraceRec*)a.traces + in.trace_off;
-  const int cap = in.trace_cap;
-  if (threadIdx.x == 0) {
-    b.rd = 0; b.wr = 0; b.overflow = 0;
-    if (cap > 0) { tr[0] = TraceRec{0, n, 0, {0, 0}}; b.wr = 1; } else b.overflow = 1;
-  }
-  __syncthreads();
-  for (;;) {
-    if (threadIdx.x == 0 && b.rd < b.wr) b.cur = tr[b.rd];
-    __syncthreads();
-    if (b.rd >= b.wr || b.overflow) break;
-    const int i = b.cur.i, j = b.cur.j;
+  auto rule_of = [&](const TraceRec& cur, int* rule, int* p0, int* p1) -> bool {
+    const int i = cur.i, j = cur.j;
     const int target = C.get(0, i, j);
     // candidates in nu_kern.h order: nil, left, right, pair, split(k asc)
     auto cand = [&](int c, int* v) -> bool {
@@ -111,30 +153,24 @@ __global__ __launch_bounds__(NTHR) void nu_finalize(TriArgs a) {
       *v = C.get(0, i, k) + C.get(0, k, j);
       return true;
     };
-    first_match(4 + fmax_(0, j - i - 1), target, cand, &b.first);
-    if (threadIdx.x == 0) {
-      const int c = b.first;
-      if (c == 0x7fffffff) b.overflow = 1;
-      TraceRec& e = tr[b.rd];
-      e.rule = c < 4 ? c : 4;
-      e.pos[0] = c >= 4 ? i + 1 + (c - 4) : 0;
-      e.pos[1] = 0;
-      auto push = [&](int pi, int pj) {
-        if (b.wr >= cap) { b.overflow = 1; return; }
-        tr[b.wr++] = TraceRec{pi, pj, 0, {0, 0}};
-      };
-      switch (e.rule) {
-        case 1: push(i + 1, j); break;
-        case 2: push(i, fmax_(i, j - 1)); break;
-        case 3: push(i + 1, fmax_(i + 1, j - 1)); break;
-        case 4: push(i, e.pos[0]); push(e.pos[0], j); break;
-        default: break;
-      }
-      b.rd++;
+    const int c = wave_first_match(4 + fmax_(0, j - i - 1), target, cand);
+    *rule = c < 4 ? c : 4;
+    *p0 = c >= 4 ? i + 1 + (c - 4) : 0;
+    *p1 = 0;
+    return c >= 0;
+  };
+  auto kids = [&](const TraceRec& e, auto&& push) {
+    const int i = e.i, j = e.j;
+    switch (e.rule) {
+      case 1: push(i + 1, j, 0); break;
+      case 2: push(i, fmax_(i, j - 1), 0); break;
+      case 3: push(i + 1, fmax_(i + 1, j - 1), 0); break;
+      case 4: push(i, e.pos[0], 0); push(e.pos[0], j, 0); break;
+      default: break;
     }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) a.trace_lens[in.out] = b.overflow ? -1 : b.wr;
+  };
+  bfs_rounds((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
+             a.trace_lens + in.out);
 }
 
 // ------------------------------------------------------------- matrix chain
@@ -144,21 +180,10 @@ __global__ __launch_bounds__(NTHR) void mc_finalize(TriArgs a) {
   const int32_t* rows = a.dims + in.seq_off;
   const int32_t* cols = rows + in.n;
   const int n = in.n;
-  __shared__ Bfs b;
   if (threadIdx.x == 0) a.results[in.out] = C.get(0, 0, n);
   if (!a.want_trace) return;
-  TraceRec* tr = (TraceRec*)a.traces + in.trace_off;
-  const int cap = in.trace_cap;
-  if (threadIdx.x == 0) {
-    b.rd = 0; b.wr = 0; b.overflow = 0;
-    if (cap > 0) { tr[0] = TraceRec{0, n, 0, {0, 0}}; b.wr = 1; } else b.overflow = 1;
-  }
-  __syncthreads();
-  for (;;) {
-    if (threadIdx.x == 0 && b.rd < b.wr) b.cur = tr[b.rd];
-    __syncthreads();
-    if (b.rd >= b.wr || b.overflow) break;
-    const int i = b.cur.i, j = b.cur.j;
+  auto rule_of = [&](const TraceRec& cur, int* rule, int* p0, int* p1) -> bool {
+    const int i = cur.i, j = cur.j;
     const int target = C.get(0, i, j);
     auto cand = [&](int c, int* v) -> bool {
       if (c == 0) { *v = 0; return i + 1 == j; }
@@ -168,23 +193,20 @@ __global__ __launch_bounds__(NTHR) void mc_finalize(TriArgs a) {
                  (uint32_t)rows[i] * (uint32_t)rows[k] * (uint32_t)cols[j - 1]);
       return true;
     };
-    first_match(1 + fmax_(0, j - i - 1), target, cand, &b.first);
-    if (threadIdx.x == 0) {
-      const int c = b.first;
-      if (c == 0x7fffffff) b.overflow = 1;
-      TraceRec& e = tr[b.rd];
-      e.rule = c == 0 ? 0 : 1;
-      e.pos[0] = c == 0 ? 0 : i + c;
-      e.pos[1] = 0;
-      if (e.rule == 1) {
-        if (b.wr + 2 > cap) b.overflow = 1;
-        else { tr[b.wr++] = TraceRec{i, e.pos[0], 0, {0, 0}}; tr[b.wr++] = TraceRec{e.pos[0], j, 0, {0, 0}}; }
-      }
-      b.rd++;
+    const int c = wave_first_match(1 + fmax_(0, j - i - 1), target, cand);
+    *rule = c == 0 ? 0 : 1;
+    *p0 = c <= 0 ? 0 : i + c;
+    *p1 = 0;
+    return c >= 0;
+  };
+  auto kids = [&](const TraceRec& e, auto&& push) {
+    if (e.rule == 1) {
+      push(e.i, e.pos[0], 0);
+      push(e.pos[0], e.j, 0);
     }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) a.trace_lens[in.out] = b.overflow ? -1 : b.wr;
+  };
+  bfs_rounds((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
+             a.trace_lens + in.out);
 }
 
 // -------------------------------------------------------------------- Zuker
@@ -252,7 +274,6 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
   const int n = in.n;
   const ZuF z{a.codes + in.seq_off, (const DevRna*)a.rna, a.hl_ent, n};
   const int mlint = z.P->ml_intern;
-  __shared__ Bfs b;
   // ---- st column (zu_kern.h:166-182): rule 12 st(i+1), rule 13 D(i,k)+st(k), rule 14 i==n.
   // Rows are taken in blocks of 32 from the bottom.  For a block [lo, hi] the
   // split points k > hi (st(k) already known) are reduced by all threads at
@@ -267,11 +288,23 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
     for (int pass = 0; pass < T / 8; ++pass) {
       const int r = pass * 8 + (tid >> 5), i = lo + r;
       int m = INV;
-      if (i <= hi)
-        for (int k = fmax_(hi + 1, i + 7) + lane; k <= n; k += 32) {
-          const int U = C.get(ZT_U, i, k);
-          if (U < INV_THR) m = fmin_(m, U - mlint + st[k]);
+      if (i <= hi) {
+        // eight split points per lane in flight at a time
+        const int k0 = fmax_(hi + 1, i + 7) + lane;
+        for (int kb = k0; kb <= n; kb += 8 * 32) {
+          int u[8];
+#pragma unroll
+          for (int x = 0; x < 8; ++x) {
+            const int k = kb + 32 * x;
+            u[x] = k <= n ? C.get(ZT_U, i, k) : INV;
+          }
+#pragma unroll
+          for (int x = 0; x < 8; ++x) {
+            const int k = kb + 32 * x;
+            if (u[x] < INV_THR) m = fmin_(m, u[x] - mlint + st[k]);
+          }
         }
+      }
       for (int off = 16; off >= 1; off >>= 1) m = fmin_(m, __shfl_xor(m, off));
       if (lane == 0) far[r] = m;
     }
@@ -300,18 +333,8 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
   }
   if (threadIdx.x == 0) a.results[in.out] = st[0];
   if (!a.want_trace) return;
-  TraceRec* tr = (TraceRec*)a.traces + in.trace_off;
-  const int cap = in.trace_cap;
-  if (threadIdx.x == 0) {
-    b.rd = 0; b.wr = 0; b.overflow = 0;
-    if (cap > 0) { tr[0] = TraceRec{0, n, 12, {0, 0}}; b.wr = 1; } else b.overflow = 1;
-  }
-  __syncthreads();
-  for (;;) {
-    if (threadIdx.x == 0 && b.rd < b.wr) b.cur = tr[b.rd];
-    __syncthreads();
-    if (b.rd >= b.wr || b.overflow) break;
-    const int i = b.cur.i, j = b.cur.j, tabrule = b.cur.rule, d = j - i;
+  auto rule_of = [&](const TraceRec& cur, int* rule, int* p0, int* p1) -> bool {
+    const int i = cur.i, j = cur.j, tabrule = cur.rule, d = j - i;
     int ncand, target;
     if (tabrule == 12) { target = st[i]; ncand = 2 + fmax_(0, n - (i + 7) + 1); }
     else if (tabrule == 0) { target = C.get(ZT_CL, i, j); ncand = 963; }
@@ -406,56 +429,50 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
       if (U >= INV_THR) return false;
       *v = U; return true;
     };
-    first_match(ncand, target, cand, &b.first);
-    if (threadIdx.x == 0) {
-      const int c = b.first;
-      TraceRec& e = tr[b.rd];
-      int rule = -1, p0 = 0, p1 = 0;
-      if (c == 0x7fffffff) { b.overflow = 1; }
-      else if (tabrule == 12) {
-        if (c == 0) rule = 12;
-        else if (c - 1 < fmax_(0, n - (i + 7) + 1)) { rule = 13; p0 = i + 7 + (c - 1); }
-        else rule = 14;
-      } else if (tabrule == 0) {
-        if (c == 0) rule = 0;
-        else if (c == 1) rule = 1;
-        else if (c < 32) { rule = 2; p0 = i + 3 + (c - 2); }
-        else if (c < 62) { rule = 3; p0 = j - 32 + (c - 32); }
-        else if (c < 962) { rule = 4; p0 = i + 3 + (c - 62) % 30; p1 = j - 32 + (c - 62) / 30; }
-        else rule = 5;
-      } else if (tabrule == 6) {
-        if (c == 0) rule = 6; else { rule = 7; p0 = i + 7 + (c - 1); }
-      } else {
-        if (c == 0) rule = 8;
-        else if (c <= nk9) { rule = 9; p0 = i + 7 + (c - 1); }
-        else if (c == nk9 + 1) rule = 10;
-        else { rule = 11; p0 = i + 7 + (c - nk9 - 2); }
-      }
-      e.rule = rule; e.pos[0] = p0; e.pos[1] = p1;
-      auto push = [&](int pi, int pj, int pr) {
-        if (b.wr >= cap) { b.overflow = 1; return; }
-        tr[b.wr++] = TraceRec{pi, pj, pr, {0, 0}};
-      };
-      switch (rule) {  // children as zu.c:77-91
-        case 0: push(i + 1, fmax_(i + 8, j - 1), 0); break;
-        case 2: push(p0, fmax_(i + 10, fmax_(i + 11, j - 1) - 1), 0); break;
-        case 3: push(i + 2, p0, 0); break;
-        case 4: push(p0, p1, 0); break;
-        case 5: push(i + 2, fmax_(i + 16, fmax_(i + 17, j - 1) - 1), 6); break;
-        case 6: push(i + 1, j, 6); break;
-        case 7: case 9: case 13:
-          push(i, fmax_(i + 7, p0), 0); push(p0, j, rule == 13 ? 12 : 8); break;
-        case 8: push(i + 1, j, 8); break;
-        case 10: push(i, fmax_(i + 7, j), 0); break;
-        case 11: push(i, fmax_(i + 7, p0), 0); break;
-        case 12: push(i + 1, j, 12); break;
-        default: break;
-      }
-      b.rd++;
+    const int c = wave_first_match(ncand, target, cand);
+    if (c < 0) {
+      // no candidate: the record is dropped with the whole trace
+    } else if (tabrule == 12) {
+      if (c == 0) *rule = 12;
+      else if (c - 1 < fmax_(0, n - (i + 7) + 1)) { *rule = 13; *p0 = i + 7 + (c - 1); }
+      else *rule = 14;
+    } else if (tabrule == 0) {
+      if (c == 0) *rule = 0;
+      else if (c == 1) *rule = 1;
+      else if (c < 32) { *rule = 2; *p0 = i + 3 + (c - 2); }
+      else if (c < 62) { *rule = 3; *p0 = j - 32 + (c - 32); }
+      else if (c < 962) { *rule = 4; *p0 = i + 3 + (c - 62) % 30; *p1 = j - 32 + (c - 62) / 30; }
+      else *rule = 5;
+    } else if (tabrule == 6) {
+      if (c == 0) *rule = 6; else { *rule = 7; *p0 = i + 7 + (c - 1); }
+    } else {
+      if (c == 0) *rule = 8;
+      else if (c <= nk9) { *rule = 9; *p0 = i + 7 + (c - 1); }
+      else if (c == nk9 + 1) *rule = 10;
+      else { *rule = 11; *p0 = i + 7 + (c - nk9 - 2); }
     }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) a.trace_lens[in.out] = b.overflow ? -1 : b.wr;
+    return c >= 0;
+  };
+  auto kids = [&](const TraceRec& e, auto&& push) {
+    const int i = e.i, j = e.j;
+    switch (e.rule) {  // children as zu.c:77-91
+      case 0: push(i + 1, fmax_(i + 8, j - 1), 0); break;
+      case 2: push(e.pos[0], fmax_(i + 10, fmax_(i + 11, j - 1) - 1), 0); break;
+      case 3: push(i + 2, e.pos[0], 0); break;
+      case 4: push(e.pos[0], e.pos[1], 0); break;
+      case 5: push(i + 2, fmax_(i + 16, fmax_(i + 17, j - 1) - 1), 6); break;
+      case 6: push(i + 1, j, 6); break;
+      case 7: case 9: case 13:
+        push(i, fmax_(i + 7, e.pos[0]), 0); push(e.pos[0], j, e.rule == 13 ? 12 : 8); break;
+      case 8: push(i + 1, j, 8); break;
+      case 10: push(i, fmax_(i + 7, j), 0); break;
+      case 11: push(i, fmax_(i + 7, e.pos[0]), 0); break;
+      case 12: push(i + 1, j, 12); break;
+      default: break;
+    }
+  };
+  bfs_rounds((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 12, {0, 0}}, rule_of, kids,
+             a.trace_lens + in.out);
 }
 
 // ------------------------------------------------------------------ RNAfold
@@ -482,24 +499,15 @@ __global__ __launch_bounds__(NTHR) void rf_finalize(TriArgs a) {
   const Cells C{a.tab, in.nb, NT_RF, in.tile0};
   const int n = in.n;
   const ZuF z{a.codes + in.seq_off, (const DevRna*)a.rna, a.hl_ent, n};
-  __shared__ Bfs b;
   const int top = C.get(RT_Q, 0, n);
   if (threadIdx.x == 0) a.results[in.out] = top < INV_THR ? top : ZU_EMPTY;
   if (!a.want_trace) return;
-  TraceRec* tr = (TraceRec*)a.traces + in.trace_off;
-  const int cap = in.trace_cap;
-  if (threadIdx.x == 0) {
-    b.rd = 0; b.wr = 0; b.overflow = 0;
-    if (top < INV_THR) {
-      if (cap > 0) { tr[0] = TraceRec{0, n, 0, {0, 0}}; b.wr = 1; } else b.overflow = 1;
-    }
+  if (top >= INV_THR) {  // no structure: an empty trace
+    if (threadIdx.x == 0) a.trace_lens[in.out] = 0;
+    return;
   }
-  __syncthreads();
-  for (;;) {
-    if (threadIdx.x == 0 && b.rd < b.wr) b.cur = tr[b.rd];
-    __syncthreads();
-    if (b.rd >= b.wr || b.overflow) break;
-    const int i = b.cur.i, j = b.cur.j, tabrule = b.cur.rule, d = j - i;
+  auto rule_of = [&](const TraceRec& cur, int* rule, int* p0, int* p1) -> bool {
+    const int i = cur.i, j = cur.j, tabrule = cur.rule, d = j - i;
     const int l0 = fmax_(i + 4, j - 30), nl = fmax_(0, j - 1 - l0 + 1);
     int ncand, target;
     if (tabrule == 0) { target = C.get(RT_Q, i, j); ncand = 4; }
@@ -538,39 +546,43 @@ __global__ __launch_bounds__(NTHR) void rf_finalize(TriArgs a) {
       if (x >= INV_THR) return false;
       *v = x; return true;
     };
-    first_match(ncand, target, cand, &b.first);
-    if (threadIdx.x == 0) {
-      const int c = b.first;
-      TraceRec& e = tr[b.rd];
-      int rule = -1, p0 = 0, p1 = 0;
-      if (c == 0x7fffffff) b.overflow = 1;
-      else if (tabrule == 0) rule = c;
-      else if (tabrule == 4) { rule = 4; p0 = i + 3 + c; }
-      else if (c == 0) rule = 5;
-      else if (c == 1) rule = 6;
-      else if (c < 2 + 30 * nl) { rule = 7; p0 = i + 1 + (c - 2) % 30; p1 = l0 + (c - 2) / 30; }
-      else rule = 8;
-      e.rule = rule; e.pos[0] = p0; e.pos[1] = p1;
-      auto push = [&](int pi, int pj, int pr) {
-        if (b.wr >= cap) { b.overflow = 1; return; }
-        tr[b.wr++] = TraceRec{pi, pj, pr, {0, 0}};
-      };
-      switch (rule) {
-        case 0: push(i, j, 4); break;
-        case 1: push(i, j - 1, 0); break;
-        case 2: push(i + 1, j, 0); break;
-        case 3: push(i, j, 5); break;
-        case 4: push(i, p0, 0); push(p0, j, 0); break;
-        case 6: push(i + 1, j - 1, 5); break;
-        case 7: push(p0, p1, 5); break;
-        case 8: push(i + 1, j - 1, 4); break;
-        default: break;
-      }
-      b.rd++;
+    const int c = wave_first_match(ncand, target, cand);
+    if (c < 0) {
+      // no candidate: the record is dropped with the whole trace
+    } else if (tabrule == 0) {
+      *rule = c;
+    } else if (tabrule == 4) {
+      *rule = 4;
+      *p0 = i + 3 + c;
+    } else if (c == 0) {
+      *rule = 5;
+    } else if (c == 1) {
+      *rule = 6;
+    } else if (c < 2 + 30 * nl) {
+      *rule = 7;
+      *p0 = i + 1 + (c - 2) % 30;
+      *p1 = l0 + (c - 2) / 30;
+    } else {
+      *rule = 8;
     }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) a.trace_lens[in.out] = b.overflow ? -1 : b.wr;
+    return c >= 0;
+  };
+  auto kids = [&](const TraceRec& e, auto&& push) {
+    const int i = e.i, j = e.j;
+    switch (e.rule) {
+      case 0: push(i, j, 4); break;
+      case 1: push(i, j - 1, 0); break;
+      case 2: push(i + 1, j, 0); break;
+      case 3: push(i, j, 5); break;
+      case 4: push(i, e.pos[0], 0); push(e.pos[0], j, 0); break;
+      case 6: push(i + 1, j - 1, 5); break;
+      case 7: push(e.pos[0], e.pos[1], 5); break;
+      case 8: push(i + 1, j - 1, 4); break;
+      default: break;
+    }
+  };
+  bfs_rounds((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
+             a.trace_lens + in.out);
 }
 
 // ------------------------------------------------------------------ launch
