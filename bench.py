@@ -208,6 +208,10 @@ def single(args):
         n_cells = 16385 * 16385
         line["config"] = {"workload": "Smith-Waterman/Gotoh 16384 x 16384, i.i.d. ACGT seed 1, +2/-3/-5/-2, "
                                       "best local score + traceback from the best cell"}
+        if args.swat_trace_budget is not None:
+            # below the code size (128 MiB here) the traceback recomputes segments from checkpoint rows
+            ctx.set_swat_trace_budget(args.swat_trace_budget)
+            line["config"]["swat_trace_budget_bytes"] = args.swat_trace_budget
         check = lambda r: (r[0], r[1], [t[:3] for t in r[2]]) == (lambda x: (x[0], x[1], [t[:3] for t in x[2]]))(
             o.swat(a, b, 2, -3, -5, -2, from_best=True))
     elif args.workload == "nussinov1024":
@@ -265,6 +269,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="re-run the autotuner instead of loading its table")
+    ap.add_argument("--swat-trace-budget", type=int, default=None,
+                    help="swat16384: bytes of traceback codes kept before checkpointed recompute")
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)

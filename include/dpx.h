@@ -92,6 +92,13 @@ int dpx_rnafold(dpx_ctx* ctx, const char* seq, int n, int32_t* mfe,
  * trace_from: 0 = trace from (n1,n2) like the reference, 1 = from the best
  * cell.  Sequences are letters ACGT/ACGU. */
 typedef struct { int32_t match, mismatch, gap_open, gap_ext; } dpx_swat_scores;
+/* Traceback memory of dpx_swat: when the 4-bit codes of every cell
+ * (about n1*n2/2 bytes) would exceed `bytes`, only checkpoint rows every
+ * ~sqrt(n1/64) bands are kept (8 bytes per column each) and the traceback
+ * recomputes one segment at a time from its checkpoint: the same trace, for
+ * about one more fill (the checkpoint strategy of docs/report/design.tex:
+ * 226-262).  Default 32 GiB.  No reference counterpart. */
+int dpx_set_swat_trace_budget(dpx_ctx* ctx, int64_t bytes);
 int dpx_swat(dpx_ctx* ctx, const char* a, int n1, const char* b, int n2, const dpx_swat_scores* sc,
              int32_t* score, int32_t* best, int32_t* best_i, int32_t* best_j, int trace_from,
              dpx_trace_t* trace, int trace_cap, int* trace_len);

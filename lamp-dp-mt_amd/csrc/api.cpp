@@ -93,7 +93,7 @@ enum { JOB_TRI = 0, JOB_SW = 1 };
 // device block of a triangular job
 enum { TL_TAB, TL_QUEUE, TL_DEP, TL_CTRL, TL_INST, TL_SEEDS, TL_DATA, TL_SPEC, TL_RES, TL_LENS, TL_TRACES, TL_N };
 // device block of an alignment job
-enum { SL_A, SL_B, SL_ROWS, SL_FLAGS, SL_BEST, SL_BT, SL_TR, SL_N };
+enum { SL_A, SL_B, SL_ROWS, SL_FLAGS, SL_BEST, SL_BT, SL_CP, SL_TR, SL_N };
 
 struct Job {
   int kind = JOB_TRI;
@@ -110,6 +110,7 @@ struct Job {
   std::string a, b;
   dpx_swat_scores sc{};
   int trace_from = 0, sw_cap = 0;
+  int sw_seg = 0;  // bands per checkpoint segment (0: traceback codes of every band kept)
   std::vector<int2> row0;
   std::vector<uint8_t> r0m, r0g2;
   int row0_best = 0, row0_bestj = 0, row0_axiom = 0;
@@ -143,6 +144,7 @@ struct dpx_ctx {
   // server threads at launch time, hence its own lock
   std::mutex tune_mu;
   dpx_tuning tune[6][DPX_TUNE_BUCKETS] = {};
+  int64_t sw_code_budget = (int64_t)32 << 30;  // traceback codes of one alignment before checkpointing
 };
 
 namespace {
@@ -253,15 +255,28 @@ size_t tri_footprint(int g, int n, bool trace) {
   return tri_layout(g, tiles_of(nb), 1, data, trace ? tri_trace_cap(n) : 0, nullptr);
 }
 
-size_t sw_layout(int n1, int n2, int cap, size_t* off) {
+// boundary-row ring of an alignment: more slots than bands can be in flight
+constexpr int kSwMaxGrid = 4096;
+int sw_slots(int nbands) { return std::min(nbands + 1, kSwMaxGrid + 2); }
+// bands per checkpoint segment when the codes of every band would not fit the
+// budget: about sqrt(nbands), balancing checkpoint rows against one segment's codes
+int sw_segment(int n1, int n2, int64_t code_budget) {
+  const int64_t nbands = (n1 + 63) / 64, nwords = (n2 + 64 + 7) / 8;
+  if (nbands * nwords * 64 * 4 <= code_budget) return 0;
+  return std::max(1, (int)std::ceil(std::sqrt((double)nbands)));
+}
+
+size_t sw_layout(int n1, int n2, int cap, int seg, size_t* off) {
   const size_t nbands = (n1 + 63) / 64, nwords = (n2 + 64 + 7) / 8, W = (size_t)n2 + 1;
+  const size_t slots = sw_slots((int)nbands);
   size_t sz[SL_N];
   sz[SL_A] = (size_t)n1 + 64;
   sz[SL_B] = (size_t)n2 + 64;
-  sz[SL_ROWS] = (nbands + 2) * W * 8 + 2 * W + 64;  // boundary rows, then row 0 and its rule bytes
-  sz[SL_FLAGS] = 4 * (nbands + 2) + 64;
-  sz[SL_BEST] = 64;
-  sz[SL_BT] = nbands * nwords * 64 * 4;
+  sz[SL_ROWS] = (slots + 1) * W * 8 + 2 * W + 64;  // boundary-row ring, then row 0 and its rule bytes
+  sz[SL_FLAGS] = 8 * slots + 64;            // ring progress counters, then two ctrl blocks
+  sz[SL_BEST] = 128;                         // best, axiom, pick, trace state, recompute scratch
+  sz[SL_BT] = cap > 0 ? (seg ? std::min((size_t)seg, nbands) : nbands) * nwords * 64 * 4 : 0;
+  sz[SL_CP] = seg ? (nbands / seg + 1) * W * 8 : 0;
   sz[SL_TR] = sizeof(dpx_trace_t) * (size_t)(cap + 1);
   size_t at = 0;
   for (int k = 0; k < SL_N; ++k) {
@@ -385,8 +400,10 @@ int tri_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
   return DPX_OK;
 }
 
+// seg > 0: keep checkpoint rows every seg bands instead of every band's
+// traceback codes (only with a trace; the synchronous entry point's choice)
 int sw_prepare(std::string& err, Job& j, const char* a, int n1, const char* b, int n2, const dpx_swat_scores& sc,
-               int trace_from, int trace_cap) {
+               int trace_from, int trace_cap, int seg = 0) {
   if ((int64_t)(n1 + 1) * (n2 + 1) >= ((int64_t)1 << 32)) return fail(err, DPX_EINVAL, "matrix too large");
   j.kind = JOB_SW;
   j.a.assign(a, n1);
@@ -394,6 +411,7 @@ int sw_prepare(std::string& err, Job& j, const char* a, int n1, const char* b, i
   j.sc = sc;
   j.trace_from = trace_from;
   j.sw_cap = trace_cap;
+  j.sw_seg = trace_cap > 0 ? seg : 0;
   const int W = n2 + 1;
   // row 0 (i == 0): g1 = 0 [4]; g2 [7/8/9]; M = g1 [2], g2 when strictly greater [3]
   j.row0.resize(W);
@@ -423,8 +441,20 @@ int sw_prepare(std::string& err, Job& j, const char* a, int n1, const char* b, i
   }
   j.row0_axiom = j.row0[n2].x;
   j.cells = (int64_t)(n1 + 1) * (n2 + 1);
-  j.dev_bytes = sw_layout(n1, n2, trace_cap, j.off);
+  j.dev_bytes = sw_layout(n1, n2, trace_cap, j.sw_seg, j.off);
   return DPX_OK;
+}
+
+int sw_grid(dpx_ctx* c, int nbands, int n) {
+  // tuned: bands in flight per CU (0 = occupancy); never more than the ring allows
+  const dpx_tuning t = get_tune(c, DPX_REQ_SWAT, tune_bucket(n));
+  const int w = t.wg_per_cu >= 1 && t.wg_per_cu <= c->sw_occ ? t.wg_per_cu : c->sw_occ;
+  return std::max(1, std::min(std::min(nbands, w * c->cu_count), sw_slots(nbands) - 1));
+}
+
+int sw_chunk(dpx_ctx* c, int n) {
+  const dpx_tuning t = get_tune(c, DPX_REQ_SWAT, tune_bucket(n));
+  return t.chunk == 16 || t.chunk == 64 ? t.chunk : 32;
 }
 
 int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
@@ -438,6 +468,7 @@ int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j)
   A.ext = j.sc.gap_ext;
   A.nbands = (n1 + 63) / 64;
   A.nwords = (n2 + 64 + 7) / 8;
+  A.nslots = sw_slots(A.nbands);
   const bool want_trace = j.sw_cap > 0;
   const size_t in_bytes = (size_t)n1 + n2 + sizeof(int2) * W + 2 * (size_t)W + 64;
   const size_t out_bytes = 64 + (want_trace ? sizeof(dpx_trace_t) * (size_t)j.sw_cap : 0);
@@ -445,19 +476,22 @@ int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j)
   hipStream_t s = w.stream;
   char* rows = dev + j.off[SL_ROWS];
   A.hrow = (int2*)rows;
-  int2* d_row0 = (int2*)(rows + (size_t)(A.nbands + 1) * W * 8);
+  int2* d_row0 = (int2*)(rows + (size_t)A.nslots * W * 8);
   uint8_t* d_r0m = (uint8_t*)(d_row0 + W);
   uint8_t* d_r0g2 = d_r0m + W;
   A.a = (const uint8_t*)(dev + j.off[SL_A]);
   A.b = (const uint8_t*)(dev + j.off[SL_B]);
-  A.prog = (int*)(dev + j.off[SL_FLAGS]);
-  A.ctrl = A.prog + A.nbands + 1;
-  A.bt = (uint32_t*)(dev + j.off[SL_BT]);
+  A.prog = (unsigned long long*)(dev + j.off[SL_FLAGS]);
+  A.ctrl = (int*)(A.prog + A.nslots);  // [0] claims, [1] time-out; [4], [5] for recomputed segments
+  uint32_t* d_codes = want_trace ? (uint32_t*)(dev + j.off[SL_BT]) : nullptr;
+  A.bt = j.sw_seg ? nullptr : d_codes;
+  A.cp = j.sw_seg ? (int2*)(dev + j.off[SL_CP]) : nullptr;
+  A.cp_every = j.sw_seg ? j.sw_seg : 1;
   char* bestblk = dev + j.off[SL_BEST];
   A.best = (unsigned long long*)bestblk;
   A.out = (int*)(bestblk + 8);
-  int* d_pick = (int*)(bestblk + 16);  // 6 ints, see sw_pick
-  int* d_len = (int*)(bestblk + 48);
+  int* d_pick = (int*)(bestblk + 16);   // 6 ints, see sw_pick
+  int* d_state = (int*)(bestblk + 48);  // SW_STATE ints
   char* h = (char*)w.h_in.p;
   std::memcpy(h, j.a.data(), n1);
   std::memcpy(h + n1, j.b.data(), n2);
@@ -469,25 +503,57 @@ int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j)
   if (n2) HIPCHK(hipMemcpyAsync((void*)A.b, h + n1, n2, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_row0, h_row0, sizeof(int2) * W + 2 * (size_t)W, hipMemcpyHostToDevice, s));
   HIPCHK(hipEventRecord(w.ev[0], s));
-  if (A.nbands > 0) {
-    // tuned: steps per chunk and bands in flight per CU (0 = occupancy)
-    const dpx_tuning t = get_tune(c, DPX_REQ_SWAT, tune_bucket(std::max(n1, n2)));
-    const int chunk = t.chunk == 16 || t.chunk == 64 ? t.chunk : 32;
-    const int w = t.wg_per_cu >= 1 && t.wg_per_cu <= c->sw_occ ? t.wg_per_cu : c->sw_occ;
-    const int grid = std::min(A.nbands, w * c->cu_count);
-    if (sw_launch(A, d_row0, grid, chunk, s)) return fail(err, DPX_EHIP, "alignment launch failed");
-  }
+  const int n = std::max(n1, n2), chunk = sw_chunk(c, n);
+  if (A.nbands > 0 && sw_launch(A, d_row0, sw_grid(c, A.nbands, n), chunk, s))
+    return fail(err, DPX_EHIP, "alignment launch failed");
   HIPCHK(hipEventRecord(w.ev[1], s));
   void* d_tr = want_trace ? dev + j.off[SL_TR] : nullptr;
   if (sw_finish_launch(A, j.row0_best, j.row0_bestj, j.row0_axiom, j.trace_from, d_pick, d_r0m, d_r0g2, d_tr, j.sw_cap,
-                       d_len, s))
+                       d_state, s))
     return fail(err, DPX_EHIP, "traceback launch failed");
+  if (want_trace && j.sw_seg) {
+    // Checkpointed traceback: segment by segment from the end of the walk,
+    // recompute the segment's codes (its rows, the columns up to the walk's)
+    // from the checkpoint row above it, then walk through them.
+    const int seg_rows = 64 * j.sw_seg;
+    int st[SW_STATE], late = 0;
+    for (;;) {
+      HIPCHK(hipMemcpyAsync(st, d_state, sizeof st, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (late) return fail(err, DPX_EHIP, "band wait timed out in a recomputed segment (scheduler error)");
+      if (st[4] || !st[5]) break;
+      const int i = st[0], jc = st[1];
+      if (i == 0) {  // row 0: its rules are known, no codes needed
+        if (sw_trace_launch(A, d_r0m, d_r0g2, 0, d_tr, j.sw_cap, d_state, s))
+          return fail(err, DPX_EHIP, "traceback launch failed");
+        continue;
+      }
+      const int sg = (i - 1) / seg_rows, base = sg * seg_rows;
+      SwArgs B = A;
+      B.a = A.a + base;
+      B.n1 = std::min(n1, base + seg_rows) - base;
+      B.n2 = jc;
+      B.nbands = (B.n1 + 63) / 64;
+      B.nwords = (jc + 64 + 7) / 8;
+      B.nslots = sw_slots(B.nbands);
+      B.ctrl = A.ctrl + 4;
+      B.bt = d_codes;
+      B.cp = nullptr;
+      B.best = (unsigned long long*)(bestblk + 80);
+      B.out = (int*)(bestblk + 88);
+      const int2* top = sg == 0 ? d_row0 : A.cp + (size_t)sg * W;
+      if (sw_launch(B, top, sw_grid(c, B.nbands, n), chunk, s) ||
+          sw_trace_launch(B, d_r0m, d_r0g2, base, d_tr, j.sw_cap, d_state, s))
+        return fail(err, DPX_EHIP, "traceback recompute failed");
+      HIPCHK(hipMemcpyAsync(&late, B.ctrl + 1, sizeof late, hipMemcpyDeviceToHost, s));
+    }
+  }
   HIPCHK(hipEventRecord(w.ev[2], s));
   char* ho = (char*)w.h_out.p;
-  // [0..5] sw_pick output, [6] trace length, [7] band-wait time-out flag, trace records at +64
+  // [0..5] sw_pick output, [6] trace length (set at harvest), [7] band-wait
+  // time-out flag, [8..13] trace state, trace records at +64
   HIPCHK(hipMemcpyAsync(ho, d_pick, 6 * sizeof(int), hipMemcpyDeviceToHost, s));
-  if (want_trace) HIPCHK(hipMemcpyAsync(ho + 24, d_len, sizeof(int), hipMemcpyDeviceToHost, s));
-  else std::memset(ho + 24, 0, 4);
+  HIPCHK(hipMemcpyAsync(ho + 32, d_state, SW_STATE * sizeof(int), hipMemcpyDeviceToHost, s));
   if (A.nbands > 0) HIPCHK(hipMemcpyAsync(ho + 28, A.ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
   else std::memset(ho + 28, 0, 4);
   if (want_trace) HIPCHK(hipMemcpyAsync(ho + 64, d_tr, sizeof(dpx_trace_t) * j.sw_cap, hipMemcpyDeviceToHost, s));
@@ -506,7 +572,11 @@ int job_harvest(std::string& err, Workspace& w, Job& j) {
   (void)hipEventElapsedTime(&j.t_fill, w.ev[0], w.ev[1]);
   (void)hipEventElapsedTime(&j.t_fin, w.ev[1], w.ev[2]);
   if (j.kind == JOB_TRI && j.flags[2]) return fail(err, DPX_EHIP, "tile queue wait timed out (scheduler error)");
-  if (j.kind == JOB_SW && j.res[7]) return fail(err, DPX_EHIP, "band wait timed out (scheduler error)");
+  if (j.kind == JOB_SW) {
+    int32_t* r = (int32_t*)j.res;
+    if (r[7]) return fail(err, DPX_EHIP, "band wait timed out (scheduler error)");
+    r[6] = j.sw_cap > 0 ? (r[8 + 5] ? r[8 + 3] : -1) : 0;  // trace length from the trace state
+  }
   return DPX_OK;
 }
 
@@ -1016,7 +1086,8 @@ int dpx_swat(dpx_ctx* c, const char* a, int n1, const char* b, int n2, const dpx
     return fail(err, DPX_EINVAL, "bad argument");
   (void)hipSetDevice(c->device);
   Job j;
-  int r = sw_prepare(err, j, a, n1, b, n2, *sc, trace_from, trace ? trace_cap : 0);
+  int r = sw_prepare(err, j, a, n1, b, n2, *sc, trace_from, trace ? trace_cap : 0,
+                     sw_segment(n1, n2, c->sw_code_budget));
   if (!r) r = run_sync(c, j);
   if (r) return r;
   *score = j.res[0];
@@ -1210,6 +1281,13 @@ double dpx_now(void) { return now_s(); }
 
 int dpx_tune_bucket(int n) { return n < 0 ? -1 : tune_bucket(n); }
 
+int dpx_set_swat_trace_budget(dpx_ctx* c, int64_t bytes) {
+  if (!c || bytes < 0) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->sw_code_budget = bytes;
+  return DPX_OK;
+}
+
 int dpx_get_tuning(dpx_ctx* c, int kind, int bucket, dpx_tuning* out) {
   if (!c || !out || kind < DPX_REQ_NUSSINOV || kind > DPX_REQ_RNAFOLD || bucket < 0 || bucket >= DPX_TUNE_BUCKETS)
     return DPX_EINVAL;
@@ -1225,6 +1303,8 @@ int dpx_set_tuning(dpx_ctx* c, int kind, int bucket, const dpx_tuning* t) {
   c->tune[kind][bucket] = *t;
   return DPX_OK;
 }
+
+}  // extern "C"
 
 namespace {
 
@@ -1314,6 +1394,8 @@ std::vector<int32_t> job_outputs(const Job& j) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int dpx_autotune(dpx_ctx* c, int kind, int bucket_lo, int bucket_hi, int reps, dpx_tune_report* report,
                  int report_cap, int* report_len) {
@@ -1475,7 +1557,7 @@ int dpx_server_submit(dpx_server* S, const dpx_request* req, uint64_t* id) {
     q.s1.assign(req->a, n1);
     q.s2.assign(req->b, n2);
     q.sc = req->sc;
-    q.footprint = sw_layout(n1, n2, q.trace ? sw_trace_cap(n1, n2) : 0, nullptr);
+    q.footprint = sw_layout(n1, n2, q.trace ? sw_trace_cap(n1, n2) : 0, 0, nullptr);
     q.cells = (int64_t)(n1 + 1) * (n2 + 1);
   } else {
     return DPX_EINVAL;

@@ -88,16 +88,27 @@ struct SwArgs {
   int n1, n2;
   int match, mismatch, open, ext;
   int nbands, nwords;   // nwords = words per lane per band
-  int2* hrow;           // (nbands+1) x (n2+1): (M, g1) of rows 64*band
-  int* prog;            // nbands+1: columns of hrow[band] published
-  int* ctrl;            // [0] band claim counter
-  uint32_t* bt;         // nbands x nwords x 64
+  int nslots;           // boundary rows in flight: band b reads slot b % nslots, writes (b+1) % nslots
+  int2* hrow;           // nslots x (n2+1): (M, g1) of rows 64*band
+  unsigned long long* prog;  // nslots: ((writer band + 1) << 32) | columns of the slot's row published
+  int* ctrl;            // [0] band claim counter, [1] band-wait time-out flag
+  uint32_t* bt;         // nbands x nwords x 64 traceback codes, or null (none kept)
+  int2* cp;             // checkpoint rows: row 64*cp_every*s at cp + s*(n2+1), s >= 1; or null
+  int cp_every;         // bands per checkpoint segment
   unsigned long long* best;  // packed (M << 32) | ~(i*(n2+1)+j)
   int* out;             // [0] = M(n1,n2)
 };
+// trace state: [0] i, [1] j, [2] table (0 M, 4 g1, 7 g2), [3] records written,
+// [4] 1 when the trace has ended, [5] 0 after an overflow
+constexpr int SW_STATE = 6;
 int sw_launch(const SwArgs& A, const int2* row0, int grid, int chunk, void* stream);  // chunk 16, 32 or 64
 int sw_finish_launch(const SwArgs& A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out,
-                     const uint8_t* r0m, const uint8_t* r0g2, void* tr, int cap, int* len, void* stream);
+                     const uint8_t* r0m, const uint8_t* r0g2, void* tr, int cap, int* state, void* stream);
+// Traceback through the codes of rows i_base+1 .. i_base+A.n1 (A covers those
+// rows), from the state to the first cell in row <= i_base (i_base > 0) or to
+// its end (i_base = 0, where row 0's rules apply).
+int sw_trace_launch(const SwArgs& A, const uint8_t* r0m, const uint8_t* r0g2, int i_base, void* tr, int cap,
+                    int* state, void* stream);
 int sw_occupancy();
 
 // launchers (tri.hip, finalize.hip); stream is a hipStream_t

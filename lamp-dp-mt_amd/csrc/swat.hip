@@ -17,6 +17,14 @@
 // The traceback keeps one 4-bit code per cell (M rule, g1 ext?, g2 ext?),
 // 8 steps of a lane per 32-bit word, words of a band stored lane-major so each
 // store is a coalesced 256 B row.
+//
+// Boundary rows travel through a ring of nslots rows (more than the bands in
+// flight), each with a progress counter tagged by its writer band, so the
+// forward pass needs O(nslots * n2) memory.  Without codes (bt == null) the
+// pass keeps every cp_every-th boundary row as a checkpoint instead, and the
+// traceback recomputes one segment of cp_every bands at a time from its
+// checkpoint (the checkpoint strategy of the reference's design notes,
+// docs/report/design.tex:226-262).
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -37,8 +45,11 @@ __device__ __forceinline__ int sw_shr1(int lane0_val, int v) {
 // SW_CH = steps per chunk = publish granularity (columns), a multiple of 8 up
 // to 64: a band lags the one above by 64 + SW_CH steps, and waits for it (or
 // publishes to the band below) once per chunk.  Variants 16, 32, 64; the
-// tuning table picks one per length bucket.
-template <int SW_CH>
+// tuning table picks one per length bucket.  CODES: store the traceback codes
+// (A.bt); a separate instance without them keeps the step loop free of a
+// store branch, which the compiler would otherwise hoist by duplicating the
+// loop at twice the registers.
+template <int SW_CH, bool CODES>
 __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
   static_assert(SW_CH % 8 == 0 && SW_CH <= 64, "chunk: whole code words, one column per lane");
   const int lane = threadIdx.x;
@@ -52,9 +63,13 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     const bool row_ok = i <= A.n1;
     const int ai = row_ok ? A.a[i - 1] : -1;
     const int W = A.n2 + 1;
-    const int2* above = A.hrow + (size_t)band * W;
-    int2* below = A.hrow + (size_t)(band + 1) * W;
+    const int2* above = A.hrow + (size_t)(band % A.nslots) * W;
+    int2* below = A.hrow + (size_t)((band + 1) % A.nslots) * W;
+    const unsigned long long* pa = A.prog + band % A.nslots;
+    unsigned long long* pb = A.prog + (band + 1) % A.nslots;
+    const unsigned long long tag_above = (unsigned long long)band << 32;  // written by band - 1
     uint32_t* btb = A.bt + (size_t)band * A.nwords * 64;
+    int2* cpb = (A.cp && (band + 1) % A.cp_every == 0) ? A.cp + (size_t)((band + 1) / A.cp_every) * W : nullptr;
     int Mc = 0, G1c = 0, G2c = 0, upPrev = 0;
     int best = -1, bestj = 0;
     int known = 0;  // columns of `above` known to be published
@@ -67,7 +82,13 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
       // steps of the chunk below issue no loads, so no step waits on memory
       const int need = min(t0 + SW_CH, W);
       if (known < need) {
-        int k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&A.prog[band], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        // columns published by band - 1 (an older tag is an earlier use of the slot)
+        auto cols = [&]() {
+          const unsigned long long v = __hip_atomic_load(pa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const int c = (v >> 32) == (tag_above >> 32) ? (int)(uint32_t)v : 0;
+          return __builtin_amdgcn_readfirstlane(c);
+        };
+        int k = cols();
         for (int spin = 0; k < need; ++spin) {
           if (spin > (1 << 24)) {  // bounded wait: flag the error, stop the band
             if (lane == 0) A.ctrl[1] = 1;
@@ -75,7 +96,7 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
             break;
           }
           __builtin_amdgcn_s_sleep(1);
-          k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&A.prog[band], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          k = cols();
         }
         known = k;
       }
@@ -133,15 +154,17 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
           if (act && i == A.n1 && j == A.n2) outv = M;
           word |= (uint32_t)(rm | (r1 << 2) | (r2 << 3)) << (4 * u);
         }
-        btb[(size_t)((t0 + tc) >> 3) * 64 + lane] = word;
+        if (CODES) btb[(size_t)((t0 + tc) >> 3) * 64 + lane] = word;
       }
       // the chunk's cells of the band's last row, one column per lane
       {
         const int jl = t0 + lane - 63;
-        if (lane < tn && jl >= 0 && jl <= A.n2)
+        if (lane < tn && jl >= 0 && jl <= A.n2) {
           __hip_atomic_store((unsigned long long*)(below + jl),
                              (unsigned long long)(uint32_t)sM | ((unsigned long long)(uint32_t)sG << 32),
                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (cpb) cpb[jl] = make_int2(sM, sG);  // read by later launches only
+        }
       }
       // publish the finished columns of this band's last row
       const int t = t0 + tn - 1;
@@ -150,7 +173,9 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this band's row stores are complete
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (lane == 63) __hip_atomic_store(&A.prog[band + 1], done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 63)
+          __hip_atomic_store(pb, ((unsigned long long)(band + 1) << 32) | (unsigned)done, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     if (row_ok && i == A.n1) A.out[0] = outv;
@@ -166,8 +191,8 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
 __global__ void sw_reset(SwArgs A, const int2* row0) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int W = A.n2 + 1;
-  if (t < W) A.hrow[t] = row0[t];
-  if (t <= A.nbands) A.prog[t] = t == 0 ? W : 0;
+  if (t < W) A.hrow[t] = row0[t];  // slot 0: row 0 (tag 0: "written by band -1")
+  if (t < A.nslots) A.prog[t] = t == 0 ? (unsigned long long)W : 0ull;
   if (t == 0) { A.ctrl[0] = 0; A.ctrl[1] = 0; *A.best = 0; A.out[0] = 0; }
 }
 
@@ -177,7 +202,8 @@ __global__ void sw_reset(SwArgs A, const int2* row0) {
 // Best cell over row 0 (host-computed: value, column) and the device rows
 // (largest M, then smallest (i, j)); start of the trace.  out: [0]=axiom
 // M(n1,n2), [1]=best, [2]=best_i, [3]=best_j, [4]=trace i0, [5]=trace j0.
-__global__ void sw_pick(SwArgs A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out) {
+__global__ void sw_pick(SwArgs A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out,
+                        int* state) {
   const unsigned long long key = *A.best;
   const int W = A.n2 + 1;
   int bv = row0_best, bi = 0, bj = row0_bestj;
@@ -190,17 +216,24 @@ __global__ void sw_pick(SwArgs A, int row0_best, int row0_bestj, int row0_axiom,
   out[1] = bv; out[2] = bi; out[3] = bj;
   out[4] = trace_from ? bi : A.n1;
   out[5] = trace_from ? bj : A.n2;
+  state[0] = out[4];
+  state[1] = out[5];
+  state[2] = 0;
+  state[3] = 0;
+  state[4] = 0;
+  state[5] = 1;
 }
 
 __global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mrule, const uint8_t* row0_g2rule,
-                                               const int* start, SwTrace* tr, int cap, int* len) {
+                                               int i_base, SwTrace* tr, int cap, int* state) {
   constexpr int NW = 64;
   __shared__ uint32_t win[NW][64];
   const int lane = threadIdx.x;
   int wb = -1, w0 = 0;
-  int i = start[4], j = start[5], tab = 0, n = 0;
-  bool ok = true;
-  for (;;) {
+  int i = state[0], j = state[1], tab = state[2], n = state[3];
+  bool ok = state[5] != 0, ended = state[4] != 0;
+  while (ok && !ended) {
+    if (i_base > 0 && i <= i_base) break;  // the rest of the walk is in an earlier segment
     int rule;
     if (tab <= 3) {
       if (i == 0 && j == 0) rule = 0;
@@ -212,7 +245,8 @@ __global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mru
       rule = j == 0 ? 7 : (i == 0 ? row0_g2rule[j] : -1);
     }
     if (rule < 0) {
-      const int b = (i - 1) / 64, r = (i - 1) % 64, t = j + r, w = t >> 3;
+      const int ir = i - i_base;  // row within the codes of A
+      const int b = (ir - 1) / 64, r = (ir - 1) % 64, t = j + r, w = t >> 3;
       if (b != wb || w < w0 || w >= w0 + NW) {
         wb = b;
         w0 = max(0, w - NW + 1);
@@ -233,33 +267,49 @@ __global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mru
     else if (rule == 6) { --i; tab = 4; }
     else if (rule == 8) { --j; tab = 0; }
     else if (rule == 9) { --j; tab = 7; }
-    else break;
+    else ended = true;
   }
-  if (lane == 0) *len = ok ? n : -1;
+  if (lane == 0) {
+    state[0] = i;
+    state[1] = j;
+    state[2] = tab;
+    state[3] = n;
+    state[4] = ended ? 1 : 0;
+    state[5] = ok ? 1 : 0;
+  }
 }
 
 int sw_launch(const SwArgs& A, const int2* row0, int grid, int chunk, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const int W = A.n2 + 1;
-  const int nthr = max(W, A.nbands + 1);
+  const int nthr = max(W, A.nslots);
   hipLaunchKernelGGL(sw_reset, dim3((nthr + 255) / 256), dim3(256), 0, s, A, row0);
-  if (chunk == 16) hipLaunchKernelGGL(sw_fill<16>, dim3(grid), dim3(64), 0, s, A);
-  else if (chunk == 64) hipLaunchKernelGGL(sw_fill<64>, dim3(grid), dim3(64), 0, s, A);
-  else hipLaunchKernelGGL(sw_fill<32>, dim3(grid), dim3(64), 0, s, A);
+  const bool codes = A.bt != nullptr;
+  auto k = chunk == 16 ? (codes ? sw_fill<16, true> : sw_fill<16, false>)
+         : chunk == 64 ? (codes ? sw_fill<64, true> : sw_fill<64, false>)
+                       : (codes ? sw_fill<32, true> : sw_fill<32, false>);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, A);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int sw_finish_launch(const SwArgs& A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out,
-                     const uint8_t* r0m, const uint8_t* r0g2, void* tr, int cap, int* len, void* stream) {
+                     const uint8_t* r0m, const uint8_t* r0g2, void* tr, int cap, int* state, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(sw_pick, dim3(1), dim3(1), 0, s, A, row0_best, row0_bestj, row0_axiom, trace_from, out);
-  if (tr) hipLaunchKernelGGL(sw_trace, dim3(1), dim3(64), 0, s, A, r0m, r0g2, (const int*)out, (SwTrace*)tr, cap, len);
+  hipLaunchKernelGGL(sw_pick, dim3(1), dim3(1), 0, s, A, row0_best, row0_bestj, row0_axiom, trace_from, out, state);
+  if (tr && A.bt) hipLaunchKernelGGL(sw_trace, dim3(1), dim3(64), 0, s, A, r0m, r0g2, 0, (SwTrace*)tr, cap, state);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sw_trace_launch(const SwArgs& A, const uint8_t* r0m, const uint8_t* r0g2, int i_base, void* tr, int cap,
+                    int* state, void* stream) {
+  hipLaunchKernelGGL(sw_trace, dim3(1), dim3(64), 0, (hipStream_t)stream, A, r0m, r0g2, i_base, (SwTrace*)tr, cap,
+                     state);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int sw_occupancy() {
   int blocks = 0;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sw_fill<32>, 64, 0) == hipSuccess ? blocks : 0;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sw_fill<32, true>, 64, 0) == hipSuccess ? blocks : 0;
 }
 
 }  // namespace dpx

@@ -28,7 +28,7 @@ EXPORTS = [
     "dpx_server_create", "dpx_server_destroy", "dpx_server_submit", "dpx_server_poll",
     "dpx_server_trace", "dpx_server_get_stats",
     "dpx_plan_create", "dpx_plan_run", "dpx_plan_timing", "dpx_plan_results", "dpx_plan_destroy",
-    "dpx_tune_bucket", "dpx_get_tuning", "dpx_set_tuning", "dpx_autotune",
+    "dpx_tune_bucket", "dpx_get_tuning", "dpx_set_tuning", "dpx_autotune", "dpx_set_swat_trace_budget",
 ]
 
 REQ_NUSSINOV, REQ_ZUKER, REQ_SWAT, REQ_MATCHAIN, REQ_RNAFOLD = 1, 2, 3, 4, 5
@@ -163,6 +163,11 @@ class Context:
 
     def set_rna_params(self, path):
         self._check(self._L.dpx_set_rna_params(self.h, path.encode()))
+
+    def set_swat_trace_budget(self, nbytes):
+        """Above this many bytes of traceback codes, swat() keeps checkpoint
+        rows and recomputes segments during the traceback (same trace)."""
+        self._check(self._L.dpx_set_swat_trace_budget(self.h, ctypes.c_int64(nbytes)))
 
     # ---- single instances -------------------------------------------------
     def nussinov(self, seq, min_loop=0, trace=True):
