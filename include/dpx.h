@@ -253,6 +253,52 @@ int dpx_server_poll(dpx_server* srv, dpx_result* out, int max, int wait_us, int*
 int dpx_server_trace(dpx_server* srv, uint64_t id, dpx_trace_t* buf, int cap, int* len);
 int dpx_server_get_stats(dpx_server* srv, dpx_server_stats* out);
 
+/* ---- Generic ADP grammars, compiled to HIP at run time ------------------
+ * Replaces the reference's per-grammar code generation and JNI library
+ * (CodeGen.genTab/genKern/genBT/parseC/backtrackC, CodeGen.scala:100-252,
+ * 291-358, 565-568) for grammars other than the built-in ones.  The grammar
+ * is a JSON tree of the reference's combinators (BaseParsers.scala,
+ * ADPParsers.scala, TTParsers.scala):
+ *   {"tracks": 1|2, "alphabet": C type, "alphabet_bytes": n, "answer": C type,
+ *    "answer_bytes": n, "header": C declarations, "h": "max"|"min"|
+ *    {"maxBy": fn}|{"minBy": fn}, "empty": C literal (optional, the
+ *    reference's cudaEmpty), "window": w (single track), "axiom": name,
+ *    "functions": {name: {"args": "a,b", "body": "return a+b;"}},
+ *    "tabulations": {name: {"always_valid": bool, "p": parser}}}
+ *   parser: {"tab": name} | {"term": "empty"|"emptyi"|"el"|"eli"|"seq"|
+ *    "empty1"|"empty2"|"el1"|"el2", "min": m, "max": M (seq)} |
+ *    {"or": [p, p, ...]} | {"cat": [p, p], "track": 0|1|2 (~, -~, ~-),
+ *    "hint": [minl, maxl, minr, maxr] (optional)} | {"map": p, "f": fn} |
+ *    {"filter": p, "f": fn} | {"agg": p, "h": h}
+ * Function bodies are C/C++ (`_in1[k]`, `_in2[k]` read the inputs; argument
+ * types are deduced).  Rule ids, split positions, the backtrace order and
+ * ties (first strictly better candidate in loop order) are the reference's
+ * generated code's; trace records hold up to 2 split positions. */
+typedef struct dpx_grammar dpx_grammar;
+typedef struct {
+  int32_t tracks, n_tabs, n_rules, max_pos, axiom_id, window;
+  int32_t answer_bytes, alphabet_bytes;
+  int32_t loops;       /* 1 if some rule has a split loop (O(n) work per cell) */
+  int32_t reserved;
+  int64_t code_bytes;  /* size of the compiled code object */
+} dpx_grammar_info;
+/* Analyses, generates and compiles (hipRTC, gfx950) the grammar and loads it
+ * on the context's device.  Errors (JSON, unsupported construct, compiler
+ * log) are in dpx_last_error(ctx). */
+int dpx_grammar_create(dpx_ctx* ctx, const char* spec_json, dpx_grammar** out);
+void dpx_grammar_destroy(dpx_grammar* g);
+/* Generates and compiles without a device: *src points at the generated HIP
+ * source until the next call on this thread; *msg at the error, if any. */
+int dpx_grammar_check(const char* spec_json, const char** src, const char** msg);
+int dpx_grammar_get_info(const dpx_grammar* g, dpx_grammar_info* out);
+const char* dpx_grammar_source(const dpx_grammar* g);
+/* One parse: in1[n1] (and in2[n2] for two tracks) of alphabet_bytes each.
+ * answer receives answer_bytes; *found = 0 when the axiom has no parse
+ * (then no trace).  Records are the reference's (i, j, rule, pos) in BFS
+ * order; DPX_ETRACE when trace_cap is too small. */
+int dpx_grammar_parse(dpx_ctx* ctx, dpx_grammar* g, const void* in1, int n1, const void* in2, int n2,
+                      void* answer, int* found, dpx_trace_t* trace, int trace_cap, int* trace_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,1150 @@
+// adp.cpp -- generic ADP grammars: analysis, HIP code generation, hipRTC
+// compilation and the device wavefront (declared in adp.h).
+//
+// Semantics follow the reference's native code generator:
+//  * yield analysis, unused-tabulation stripping, evaluation order and rule
+//    ids: BaseParsers.scala:95-142 (min/max yields, `rulesOrder`, ids by
+//    tabulation name, each tabulation owning `inner.alt` consecutive ids);
+//  * alternatives: the normal form of CodeGen.norm (CodeGen.scala:31-61),
+//    Or lifted above Concat/Map/Filter, left alternatives first, a
+//    concatenation of alternatives numbered left-major (alt = l.alt * r.alt);
+//  * cell body: CodeGen.genTab (CodeGen.scala:100-186) -- alternatives in
+//    order, split loops ascending with the outer concatenation outermost, a
+//    candidate kept only when strictly better (`_c > _c0 || rule == -1`, or
+//    against the empty value when one is given), back-pointer = (rule, split
+//    positions of the concatenations that need one, left to right);
+//  * concatenation index ranges: Concat.apply / CodeGen.gen
+//    (BaseParsers.scala:282-345, CodeGen.scala:142-160);
+//  * backtrace: CodeGen.genBT / genUnapp (CodeGen.scala:213-252), a BFS
+//    queue of (i, j, tabulation id) records rewritten to (i, j, rule, pos);
+//  * window: CodeGen.scala:325-348 (best axiom cell (i, i+window) by the
+//    signature's h, first one on ties).
+// Not supported (rejected with a message): nested aggregations (an
+// aggregate below a concatenation or a map), sum/count aggregations, more
+// than two split positions per tabulation.
+#include "adp.h"
+
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <set>
+#include <sstream>
+#include <vector>
+
+namespace dpx {
+namespace adp {
+
+namespace {
+
+// ------------------------------------------------------------------ JSON
+
+struct J {
+  enum T { NUL, BOOL, NUM, STR, ARR, OBJ } t = NUL;
+  bool b = false;
+  double num = 0;
+  std::string s;
+  std::vector<J> a;
+  std::vector<std::pair<std::string, J>> o;
+  const J* get(const char* k) const {
+    for (auto& kv : o)
+      if (kv.first == k) return &kv.second;
+    return nullptr;
+  }
+};
+
+struct JParser {
+  const char* p;
+  const char* base;
+  std::string err;
+  void ws() {
+    while (*p == ' ' || *p == '\n' || *p == '\t' || *p == '\r') ++p;
+  }
+  bool fail(const std::string& m) {
+    if (err.empty()) err = m + " at offset " + std::to_string(p - base);
+    return false;
+  }
+  bool str(std::string& out) {
+    if (*p != '"') return fail("expected string");
+    ++p;
+    while (*p && *p != '"') {
+      if (*p == '\\') {
+        ++p;
+        switch (*p) {
+          case 'n': out += '\n'; break;
+          case 't': out += '\t'; break;
+          case 'r': out += '\r'; break;
+          case 'b': out += '\b'; break;
+          case 'f': out += '\f'; break;
+          case '/': out += '/'; break;
+          case '\\': out += '\\'; break;
+          case '"': out += '"'; break;
+          case 'u': {
+            unsigned v = 0;
+            for (int k = 1; k <= 4; ++k) {
+              const char c = p[k];
+              v = v * 16 + (c >= '0' && c <= '9' ? c - '0' : c >= 'a' && c <= 'f' ? c - 'a' + 10 : c >= 'A' && c <= 'F' ? c - 'A' + 10 : 0);
+            }
+            if (v >= 128) return fail("non-ASCII \\u escape");
+            out += (char)v;
+            p += 4;
+            break;
+          }
+          default: return fail("bad escape");
+        }
+        ++p;
+      } else {
+        out += *p++;
+      }
+    }
+    if (*p != '"') return fail("unterminated string");
+    ++p;
+    return true;
+  }
+  bool val(J& v) {
+    ws();
+    if (*p == '{') {
+      v.t = J::OBJ;
+      ++p;
+      ws();
+      if (*p == '}') { ++p; return true; }
+      for (;;) {
+        ws();
+        std::string k;
+        if (!str(k)) return false;
+        ws();
+        if (*p != ':') return fail("expected ':'");
+        ++p;
+        J x;
+        if (!val(x)) return false;
+        v.o.emplace_back(k, std::move(x));
+        ws();
+        if (*p == ',') { ++p; continue; }
+        if (*p == '}') { ++p; return true; }
+        return fail("expected ',' or '}'");
+      }
+    }
+    if (*p == '[') {
+      v.t = J::ARR;
+      ++p;
+      ws();
+      if (*p == ']') { ++p; return true; }
+      for (;;) {
+        J x;
+        if (!val(x)) return false;
+        v.a.push_back(std::move(x));
+        ws();
+        if (*p == ',') { ++p; continue; }
+        if (*p == ']') { ++p; return true; }
+        return fail("expected ',' or ']'");
+      }
+    }
+    if (*p == '"') { v.t = J::STR; return str(v.s); }
+    if (!strncmp(p, "true", 4)) { v.t = J::BOOL; v.b = true; p += 4; return true; }
+    if (!strncmp(p, "false", 5)) { v.t = J::BOOL; p += 5; return true; }
+    if (!strncmp(p, "null", 4)) { v.t = J::NUL; p += 4; return true; }
+    char* e = nullptr;
+    v.num = strtod(p, &e);
+    if (e == p) return fail("unexpected character");
+    v.t = J::NUM;
+    p = e;
+    return true;
+  }
+};
+
+// ------------------------------------------------------------------ model
+
+constexpr int INF = -1;  // unbounded yield (the reference's maxN)
+constexpr int MAX_POS = 2;
+
+enum Kind { K_TAB, K_TERM, K_OR, K_MAP, K_FILTER, K_CAT, K_AGG };
+enum Term { T_EMPTY, T_EMPTYI, T_EL, T_ELI, T_SEQ, T_EMPTY1, T_EMPTY2, T_EL1, T_EL2 };
+enum AggKind { A_MAX, A_MIN, A_MAXBY, A_MINBY };
+
+struct Agg {
+  AggKind k = A_MAX;
+  int fn = -1;  // maxBy/minBy key function
+  bool operator==(const Agg& o) const { return k == o.k && fn == o.fn; }
+};
+
+struct Node {
+  Kind k = K_TERM;
+  int tab = -1;
+  Term term = T_EMPTY;
+  int tmin = 0, tmax = 0;
+  std::shared_ptr<Node> l, r;
+  int fn = -1;
+  int track = 0;
+  int hint[4] = {0, INF, 0, INF};
+  Agg agg;
+};
+using NP = std::shared_ptr<Node>;
+
+struct Fn {
+  std::string name;
+  std::vector<std::string> args;
+  std::string body;
+};
+
+struct Tab {
+  std::string name;
+  bool always_valid = false;
+  NP p;
+  int minv = 0, maxv = 0, id = 0, alt = 1, cat = 0;
+  bool used = false;
+};
+
+int hm(int a, int b) { return a == INF ? b : b == INF ? a : std::min(a, b); }
+
+struct Model {
+  int tracks = 1, window = 0, axiom = -1;
+  std::string alphabet = "char", answer = "int", header, empty;
+  int alphabet_bytes = 1, answer_bytes = 4;
+  Agg h;
+  std::vector<Fn> fns;
+  std::map<std::string, int> fn_ix;
+  std::vector<Tab> tabs;
+  std::map<std::string, int> tab_ix;
+  std::vector<int> order;  // evaluation order within a cell (rulesOrder)
+  int n_rules = 0, max_pos = 0;
+  bool loops = false;
+
+  // yields (BaseParsers.scala: Parser.min/max, Concat hints)
+  int nmin(const Node* n) const {
+    switch (n->k) {
+      case K_TAB: return tabs[n->tab].minv;
+      case K_TERM: return n->tmin;
+      case K_OR: return std::min(nmin(n->l.get()), nmin(n->r.get()));
+      case K_CAT: return std::max(nmin(n->l.get()), n->hint[0]) + std::max(nmin(n->r.get()), n->hint[2]);
+      default: return nmin(n->l.get());
+    }
+  }
+  int nmax(const Node* n) const {
+    switch (n->k) {
+      case K_TAB: return tabs[n->tab].maxv;
+      case K_TERM: return n->tmax;
+      case K_OR: {
+        const int a = nmax(n->l.get()), b = nmax(n->r.get());
+        return a == INF || b == INF ? INF : std::max(a, b);
+      }
+      case K_CAT: {
+        const int a = hm(nmax(n->l.get()), n->hint[1]), b = hm(nmax(n->r.get()), n->hint[3]);
+        return a == INF || b == INF ? INF : a + b;
+      }
+      default: return nmax(n->l.get());
+    }
+  }
+  // recursive maximal yield with a depth bound on tabulations (analyze's rmax)
+  int rmax(const Node* n, int d) const {
+    switch (n->k) {
+      case K_TAB: return d == 1 ? INF : rmax(tabs[n->tab].p.get(), d - 1);
+      case K_TERM: return n->tmax;
+      case K_OR: {
+        const int a = rmax(n->l.get(), d);
+        if (a == INF) return INF;
+        const int b = rmax(n->r.get(), d);
+        return b == INF ? INF : std::max(a, b);
+      }
+      case K_CAT: {
+        const int a = rmax(n->l.get(), d);
+        if (a == INF) return INF;
+        const int b = rmax(n->r.get(), d);
+        return b == INF ? INF : a + b;
+      }
+      default: return rmax(n->l.get(), d);
+    }
+  }
+  int alt(const Node* n) const {
+    switch (n->k) {
+      case K_TAB: case K_TERM: return 1;
+      case K_OR: return alt(n->l.get()) + alt(n->r.get());
+      case K_CAT: return alt(n->l.get()) * alt(n->r.get());
+      default: return alt(n->l.get());
+    }
+  }
+  bool has_bt(const Node* n) const {  // Concat.hasBt: no split position when a side has a fixed yield
+    const int a = nmin(n->l.get()), b = nmax(n->l.get()), c = nmin(n->r.get()), d = nmax(n->r.get());
+    if (n->track == 0) return !((a == b && a >= 0) || (c == d && c >= 0));
+    if (n->track == 1) return !(a == b && a >= 0);
+    return !(c == d && c >= 0);
+  }
+  int cat(const Node* n) const {
+    switch (n->k) {
+      case K_TAB: case K_TERM: return 0;
+      case K_OR: return std::max(cat(n->l.get()), cat(n->r.get()));
+      case K_CAT: return cat(n->l.get()) + (has_bt(n) ? 1 : 0) + cat(n->r.get());
+      default: return cat(n->l.get());
+    }
+  }
+  void indices(const Node* n, int ix[4]) const {
+    ix[0] = std::max(nmin(n->l.get()), n->hint[0]);
+    ix[1] = hm(nmax(n->l.get()), n->hint[1]);
+    ix[2] = std::max(nmin(n->r.get()), n->hint[2]);
+    ix[3] = hm(nmax(n->r.get()), n->hint[3]);
+  }
+};
+
+bool same(const Node* a, const Node* b) {  // structural equality (the reference compares case classes)
+  if (a->k != b->k) return false;
+  switch (a->k) {
+    case K_TAB: return a->tab == b->tab;
+    case K_TERM: return a->term != T_SEQ && a->term == b->term;  // seq() builds a new parser each call
+    case K_OR: return same(a->l.get(), b->l.get()) && same(a->r.get(), b->r.get());
+    case K_MAP: case K_FILTER: return a->fn == b->fn && same(a->l.get(), b->l.get());
+    case K_CAT:
+      return a->track == b->track && same(a->l.get(), b->l.get()) && same(a->r.get(), b->r.get()) &&
+             !memcmp(a->hint, b->hint, sizeof a->hint);
+    case K_AGG: return a->agg == b->agg && same(a->l.get(), b->l.get());
+  }
+  return false;
+}
+
+// ------------------------------------------------------------------ spec -> model
+
+struct Builder {
+  Model& m;
+  std::string err;
+  explicit Builder(Model& mm) : m(mm) {}
+  bool fail(const std::string& s) {
+    if (err.empty()) err = s;
+    return false;
+  }
+  bool agg_of(const J* j, Agg& a) {
+    if (!j) return fail("missing aggregation");
+    if (j->t == J::STR) {
+      if (j->s == "max") { a.k = A_MAX; return true; }
+      if (j->s == "min") { a.k = A_MIN; return true; }
+      return fail("unsupported aggregation '" + j->s + "' (max, min, maxBy, minBy)");
+    }
+    if (j->t == J::OBJ) {
+      const J* f = j->get("maxBy");
+      a.k = A_MAXBY;
+      if (!f) { f = j->get("minBy"); a.k = A_MINBY; }
+      if (!f || f->t != J::STR || !m.fn_ix.count(f->s)) return fail("maxBy/minBy needs a known function");
+      a.fn = m.fn_ix[f->s];
+      return true;
+    }
+    return fail("bad aggregation");
+  }
+  int fn_of(const J* j) {
+    if (!j || j->t != J::STR || !m.fn_ix.count(j->s)) { fail("unknown function" + std::string(j && j->t == J::STR ? " '" + j->s + "'" : "")); return -1; }
+    return m.fn_ix[j->s];
+  }
+  NP node(const J& j, int depth = 0) {
+    if (depth > 200) { fail("parser nesting too deep"); return nullptr; }
+    if (j.t != J::OBJ) { fail("parser node must be an object"); return nullptr; }
+    auto n = std::make_shared<Node>();
+    if (const J* t = j.get("tab")) {
+      if (t->t != J::STR || !m.tab_ix.count(t->s)) { fail("unknown tabulation"); return nullptr; }
+      n->k = K_TAB;
+      n->tab = m.tab_ix[t->s];
+      return n;
+    }
+    if (const J* t = j.get("term")) {
+      static const std::map<std::string, Term> terms1 = {{"empty", T_EMPTY}, {"emptyi", T_EMPTYI}, {"el", T_EL}, {"eli", T_ELI}, {"seq", T_SEQ}};
+      static const std::map<std::string, Term> terms2 = {{"empty", T_EMPTY}, {"empty1", T_EMPTY1}, {"empty2", T_EMPTY2}, {"el1", T_EL1}, {"el2", T_EL2}, {"eli", T_ELI}, {"seq", T_SEQ}};
+      const auto& terms = m.tracks == 1 ? terms1 : terms2;
+      if (t->t != J::STR || !terms.count(t->s)) { fail("unknown terminal for a " + std::to_string(m.tracks) + "-track grammar"); return nullptr; }
+      n->k = K_TERM;
+      n->term = terms.at(t->s);
+      switch (n->term) {
+        case T_EMPTY: case T_EMPTYI: n->tmin = 0; n->tmax = 0; break;
+        case T_EMPTY1: case T_EMPTY2: n->tmin = 0; n->tmax = INF; break;
+        case T_EL: case T_ELI: case T_EL1: case T_EL2: n->tmin = 1; n->tmax = 1; break;
+        case T_SEQ: {
+          const J* a = j.get("min");
+          const J* b = j.get("max");
+          n->tmin = a ? (int)a->num : 1;
+          n->tmax = b && b->t == J::NUM ? (int)b->num : INF;
+          if (n->tmin < 0 || (n->tmax != INF && n->tmax < n->tmin)) { fail("bad seq bounds"); return nullptr; }
+          break;
+        }
+      }
+      return n;
+    }
+    if (const J* t = j.get("or")) {
+      if (t->t != J::ARR || t->a.size() < 2) { fail("'or' needs two or more parsers"); return nullptr; }
+      NP acc = node(t->a[0], depth + 1);  // left-nested like Scala's left-associative |
+      for (size_t q = 1; q < t->a.size() && acc; ++q) {
+        auto o = std::make_shared<Node>();
+        o->k = K_OR;
+        o->l = acc;
+        o->r = node(t->a[q], depth + 1);
+        if (!o->r) return nullptr;
+        acc = o;
+      }
+      return acc;
+    }
+    if (const J* t = j.get("cat")) {
+      if (t->t != J::ARR || t->a.size() != 2) { fail("'cat' needs two parsers"); return nullptr; }
+      n->k = K_CAT;
+      n->l = node(t->a[0], depth + 1);
+      n->r = node(t->a[1], depth + 1);
+      if (!n->l || !n->r) return nullptr;
+      const J* tr = j.get("track");
+      n->track = tr ? (int)tr->num : 0;
+      if (n->track < 0 || n->track > 2 || (m.tracks == 1) != (n->track == 0)) { fail("concatenation track does not fit the grammar"); return nullptr; }
+      if (const J* h = j.get("hint")) {
+        if (h->t != J::ARR || h->a.size() != 4) { fail("hint needs [minl,maxl,minr,maxr]"); return nullptr; }
+        for (int q = 0; q < 4; ++q) n->hint[q] = h->a[q].t == J::NUM ? (int)h->a[q].num : INF;
+      } else if (n->track == 0 && same(n->l.get(), n->r.get())) {
+        n->hint[0] = 1;  // Concat.hint: a parser concatenated with itself yields at least 1 on each side
+        n->hint[2] = 1;
+      }
+      return n;
+    }
+    if (const J* t = j.get("map")) {
+      n->k = K_MAP;
+      n->l = node(*t, depth + 1);
+      n->fn = fn_of(j.get("f"));
+      return n->l && n->fn >= 0 ? n : nullptr;
+    }
+    if (const J* t = j.get("filter")) {
+      n->k = K_FILTER;
+      n->l = node(*t, depth + 1);
+      n->fn = fn_of(j.get("f"));
+      return n->l && n->fn >= 0 ? n : nullptr;
+    }
+    if (const J* t = j.get("agg")) {
+      n->k = K_AGG;
+      n->l = node(*t, depth + 1);
+      if (!n->l || !agg_of(j.get("h"), n->agg)) return nullptr;
+      return n;
+    }
+    fail("unknown parser node");
+    return nullptr;
+  }
+  bool build(const J& root) {
+    if (root.t != J::OBJ) return fail("spec must be a JSON object");
+    auto num = [&](const char* k, int dflt) { const J* v = root.get(k); return v && v->t == J::NUM ? (int)v->num : dflt; };
+    auto strv = [&](const char* k, const std::string& dflt) { const J* v = root.get(k); return v && v->t == J::STR ? v->s : dflt; };
+    m.tracks = num("tracks", 1);
+    if (m.tracks != 1 && m.tracks != 2) return fail("tracks must be 1 or 2");
+    m.window = num("window", 0);
+    if (m.window < 0 || (m.window > 0 && m.tracks != 1)) return fail("window applies to single-track grammars");
+    m.alphabet = strv("alphabet", "char");
+    m.answer = strv("answer", "int");
+    m.alphabet_bytes = num("alphabet_bytes", 1);
+    m.answer_bytes = num("answer_bytes", 4);
+    if (m.alphabet_bytes <= 0 || m.answer_bytes <= 0 || m.answer_bytes > 256) return fail("bad type sizes");
+    m.header = strv("header", "");
+    if (const J* e = root.get("empty"); e && e->t == J::STR) m.empty = e->s;
+    const J* fns = root.get("functions");
+    if (fns && fns->t == J::OBJ) {
+      for (auto& kv : fns->o) {
+        Fn f;
+        f.name = kv.first;
+        const J* a = kv.second.get("args");
+        const J* b = kv.second.get("body");
+        if (!b || b->t != J::STR) return fail("function '" + f.name + "' needs a body");
+        if (a && a->t == J::STR) {
+          std::stringstream ss(a->s);
+          std::string x;
+          while (std::getline(ss, x, ',')) {
+            x.erase(0, x.find_first_not_of(" \t"));
+            x.erase(x.find_last_not_of(" \t") + 1);
+            if (!x.empty()) f.args.push_back(x);
+          }
+        }
+        f.body = b->s;
+        m.fn_ix[f.name] = (int)m.fns.size();
+        m.fns.push_back(f);
+      }
+    }
+    if (!agg_of(root.get("h"), m.h)) return false;
+    const J* tabs = root.get("tabulations");
+    if (!tabs || tabs->t != J::OBJ || tabs->o.empty()) return fail("no tabulations");
+    for (auto& kv : tabs->o) {
+      Tab t;
+      t.name = kv.first;
+      if (const J* v = kv.second.get("always_valid")) t.always_valid = v->t == J::BOOL && v->b;
+      if (m.tab_ix.count(t.name)) return fail("duplicate tabulation name");
+      m.tab_ix[t.name] = (int)m.tabs.size();
+      m.tabs.push_back(t);
+    }
+    for (auto& kv : tabs->o) {
+      const J* p = kv.second.get("p");
+      if (!p) return fail("tabulation '" + kv.first + "' has no parser");
+      NP n = node(*p);
+      if (!n) return false;
+      m.tabs[m.tab_ix[kv.first]].p = n;
+    }
+    const std::string ax = strv("axiom", "");
+    if (!m.tab_ix.count(ax)) return fail("axiom must name a tabulation");
+    m.axiom = m.tab_ix[ax];
+    return true;
+  }
+};
+
+// ------------------------------------------------------------------ analysis
+
+bool analyze(Model& m, std::string& err) {
+  // tabulations reachable from the axiom (analyze strips the others)
+  std::function<void(const Node*)> use = [&](const Node* n) {
+    if (n->k == K_TAB) {
+      Tab& t = m.tabs[n->tab];
+      if (t.used) return;
+      t.used = true;
+      use(t.p.get());
+      return;
+    }
+    if (n->l) use(n->l.get());
+    if (n->r) use(n->r.get());
+  };
+  use(m.tabs[m.axiom].p.get());
+  m.tabs[m.axiom].used = true;
+  std::vector<int> used;
+  for (int t = 0; t < (int)m.tabs.size(); ++t)
+    if (m.tabs[t].used) used.push_back(t);
+  const int R = (int)used.size();
+  // yields: minimum by iteration from above, maximum by bounded unfolding
+  for (int t : used) m.tabs[t].minv = 100000;
+  for (int it = 0; it < R; ++it)
+    for (int t : used) m.tabs[t].minv = m.nmin(m.tabs[t].p.get());
+  for (int t : used) m.tabs[t].maxv = m.tabs[t].minv;
+  for (int t : used) m.tabs[t].maxv = m.rmax(m.tabs[t].p.get(), R);
+  // dependencies within a cell: a concatenation side of yield 0 puts the
+  // other side at the same subword
+  std::function<void(const Node*, std::set<int>&)> deps = [&](const Node* n, std::set<int>& out) {
+    switch (n->k) {
+      case K_TAB: out.insert(n->tab); return;
+      case K_TERM: return;
+      case K_OR: deps(n->l.get(), out); deps(n->r.get(), out); return;
+      case K_CAT: {
+        int ix[4];
+        m.indices(n, ix);
+        if (ix[0] == 0) deps(n->r.get(), out);
+        if (ix[2] == 0) deps(n->l.get(), out);
+        return;
+      }
+      default: deps(n->l.get(), out); return;
+    }
+  };
+  std::map<int, std::set<int>> dep;
+  for (int t : used) deps(m.tabs[t].p.get(), dep[t]);
+  std::set<int> done;
+  while ((int)done.size() < R) {
+    bool any = false;
+    for (int t : used) {  // by tabulation index: independent tabulations in spec order
+      if (done.count(t)) continue;
+      bool ready = true;
+      for (int d : dep[t])
+        if (!done.count(d)) ready = false;
+      if (ready) {
+        m.order.push_back(t);
+        done.insert(t);
+        any = true;
+      }
+    }
+    if (!any) { err = "loop between tabulations within one cell, error in grammar"; return false; }
+  }
+  // rule ids: tabulations sorted by name, each owning inner.alt ids
+  std::vector<int> byname = used;
+  std::sort(byname.begin(), byname.end(), [&](int a, int b) { return m.tabs[a].name < m.tabs[b].name; });
+  int id = 0;
+  for (int t : byname) {
+    Tab& T = m.tabs[t];
+    T.id = id;
+    T.alt = m.alt(T.p.get());
+    T.cat = m.cat(T.p.get());
+    id += T.alt;
+    m.max_pos = std::max(m.max_pos, T.cat);
+  }
+  m.n_rules = id;
+  if (m.max_pos > MAX_POS) { err = "a rule needs more than " + std::to_string(MAX_POS) + " split positions (trace records hold 2)"; return false; }
+  if (m.n_rules > 32767) { err = "too many rules"; return false; }
+  return true;
+}
+
+// alternatives of a tabulation in rule-id order: Or lifted above
+// Concat/Map/Filter; aggregations by the signature's h are transparent at
+// the top (Aggregate(Aggregate(p,h),h) = Aggregate(p,h)); any other
+// aggregation would be nested
+bool flatten(const Model& m, const NP& n, bool top, std::vector<NP>& out, std::string& err) {
+  switch (n->k) {
+    case K_OR: return flatten(m, n->l, top, out, err) && flatten(m, n->r, top, out, err);
+    case K_AGG:
+      if (!top || !(n->agg == m.h)) {
+        err = "nested or differing aggregation is not supported (aggregate below a concatenation or map, or not the signature's h)";
+        return false;
+      }
+      return flatten(m, n->l, top, out, err);
+    case K_FILTER: case K_MAP: {
+      std::vector<NP> in;
+      if (!flatten(m, n->l, top && n->k == K_FILTER, in, err)) return false;
+      for (auto& a : in) {
+        auto c = std::make_shared<Node>(*n);
+        c->l = a;
+        out.push_back(c);
+      }
+      return true;
+    }
+    case K_CAT: {
+      std::vector<NP> L, Rr;
+      if (!flatten(m, n->l, false, L, err) || !flatten(m, n->r, false, Rr, err)) return false;
+      for (auto& a : L)
+        for (auto& b : Rr) {
+          auto c = std::make_shared<Node>(*n);
+          c->l = a;
+          c->r = b;
+          out.push_back(c);
+        }
+      return true;
+    }
+    default: out.push_back(n); return true;
+  }
+}
+
+// ------------------------------------------------------------------ code generation
+
+std::string add(const std::string& a, int c) {
+  if (c == 0) return a;
+  return "(" + a + (c > 0 ? "+" : "-") + std::to_string(c > 0 ? c : -c) + ")";
+}
+
+struct Loop {
+  std::string var;
+  std::vector<std::string> lo, hi;  // var from max(lo) to min(hi), ascending
+};
+
+struct AltCode {
+  int rule = 0;
+  std::vector<Loop> loops;          // outer concatenation first
+  std::vector<std::string> arith;   // index conditions
+  std::vector<std::string> other;   // filters and VALID, tree order
+  std::vector<std::string> pos;     // split positions, left to right
+  std::vector<std::string> var_of_pos;  // loop variable behind each position ("" if an expression)
+  struct Kid { int tab; std::string a, b; };
+  std::vector<Kid> kids;
+  std::string value;
+};
+
+struct Gen {
+  const Model& m;
+  std::string err;
+  int nvar = 0;
+  explicit Gen(const Model& mm) : m(mm) {}
+  std::string cost(int t, const std::string& a, const std::string& b) const {
+    return "((const answer_t*)T.c[" + std::to_string(t) + "])[idx_(z," + a + "," + b + ")]";
+  }
+  std::string valid(int t, const std::string& a, const std::string& b) const {
+    if (!m.empty.empty()) return "!(" + cost(t, a, b) + "==(answer_t)(" + m.empty + "))";
+    return "(T.r[" + std::to_string(t) + "][idx_(z," + a + "," + b + ")]!=-1)";
+  }
+  void scs(int mn, int mx, const std::string& a, const std::string& b, AltCode& o) {
+    if (mn == mx) { o.arith.push_back(add(a, mn) + "==" + b); return; }
+    if (mx != INF) o.arith.push_back(add(b, -mx) + "<=" + a);
+    if (mn > 0) o.arith.push_back(add(a, mn) + "<=" + b);
+  }
+  std::string call(int fn, const std::vector<std::string>& args) const {
+    std::string s = "f_" + m.fns[fn].name + "(_in1,_in2";
+    for (auto& x : args) s += "," + x;
+    return s + ")";
+  }
+  bool gen(const Node* n, const std::string& a, const std::string& b, AltCode& o, std::vector<std::string>& vals) {
+    switch (n->k) {
+      case K_TAB: {
+        const Tab& T = m.tabs[n->tab];
+        if (m.tracks == 1) scs(T.minv, T.maxv, a, b, o);
+        if (!T.always_valid) o.other.push_back(valid(n->tab, a, b));
+        o.kids.push_back({n->tab, a, b});
+        vals.push_back(cost(n->tab, a, b));
+        return true;
+      }
+      case K_TERM: {
+        switch (n->term) {
+          case T_EMPTY:
+            if (m.tracks == 2) { o.arith.push_back("0==" + a); o.arith.push_back("0==" + b); }
+            scs(0, 0, a, b, o);
+            return true;  // Unit: no value
+          case T_EMPTYI: scs(0, 0, a, b, o); vals.push_back("(" + a + ")"); return true;
+          case T_EMPTY1: o.arith.push_back("0==" + a); vals.push_back("(" + b + ")"); return true;
+          case T_EMPTY2: o.arith.push_back("0==" + b); vals.push_back("(" + a + ")"); return true;
+          case T_EL: case T_EL1: scs(1, 1, a, b, o); vals.push_back("_in1[" + a + "]"); return true;
+          case T_EL2: scs(1, 1, a, b, o); vals.push_back("_in2[" + a + "]"); return true;
+          case T_ELI: scs(1, 1, a, b, o); vals.push_back("(" + a + ")"); return true;
+          case T_SEQ: scs(n->tmin, n->tmax, a, b, o); vals.push_back("T2ii{" + a + "," + b + "}"); return true;
+        }
+        return true;
+      }
+      case K_FILTER:
+        o.other.push_back(call(n->fn, {a, b}));
+        return gen(n->l.get(), a, b, o, vals);
+      case K_MAP: {
+        std::vector<std::string> in;
+        if (!gen(n->l.get(), a, b, o, in)) return false;
+        vals.push_back(call(n->fn, in));
+        return true;
+      }
+      case K_CAT: {
+        int ix[4];
+        m.indices(n, ix);
+        const bool bt = m.has_bt(n);
+        std::string k, var;
+        std::string la = a, lb = b, ra = a, rb = b;  // subwords of the two sides
+        if (n->track == 0) {
+          const int iL = ix[0], iU = ix[1], jL = ix[2], jU = ix[3];
+          if (iL == iU && jL == jU) {
+            o.arith.push_back(add(a, iL + jL) + "==" + b);
+            k = add(a, iL);
+          } else if (iL == iU) {
+            o.arith.push_back(add(a, iL + jL) + "<=" + b);
+            if (jU != INF) o.arith.push_back(add(b, -iL - jU) + "<=" + a);
+            k = add(a, iL);
+          } else if (jL == jU) {
+            o.arith.push_back(add(a, jL + iL) + "<=" + b);
+            if (iU != INF) o.arith.push_back(add(b, -jL - iU) + "<=" + a);
+            k = add(b, -jL);
+          } else {
+            var = "k" + std::to_string(nvar++);
+            Loop L;
+            L.var = var;
+            L.lo.push_back(add(a, iL));
+            if (jU != INF) L.lo.push_back(add(b, -jU));
+            L.hi.push_back(add(b, -jL));
+            if (iU != INF) L.hi.push_back(add(a, iU));
+            o.loops.push_back(L);
+            k = var;
+          }
+          lb = k;
+          ra = k;
+        } else {
+          // two tracks: -~ splits track 1 (left parses in1[k..i), right the
+          // cell (k, j)); ~- splits track 2 (left the cell (i, k), right in2[k..j))
+          const int l = n->track == 1 ? ix[0] : ix[2], u = n->track == 1 ? ix[1] : ix[3];
+          const std::string& v = n->track == 1 ? a : b;
+          if (l == u) {
+            o.arith.push_back(std::to_string(l) + "<=" + v);
+            k = add(v, -l);
+          } else {
+            var = "k" + std::to_string(nvar++);
+            Loop L;
+            L.var = var;
+            L.lo.push_back("0");
+            if (u != INF) L.lo.push_back(add(v, -u));  // left yield <= u (BaseParsers.scala:296-299)
+            L.hi.push_back(add(v, -l));
+            o.loops.push_back(L);
+            k = var;
+          }
+          if (n->track == 1) { la = k; lb = a; ra = k; rb = b; }
+          else { la = a; lb = k; ra = k; rb = b; }
+        }
+        if (!var.empty() && !bt) { err = "internal: split loop without a back-pointer"; return false; }
+        if (!gen(n->l.get(), la, lb, o, vals)) return false;
+        if (bt) { o.pos.push_back(k); o.var_of_pos.push_back(var); }
+        return gen(n->r.get(), ra, rb, o, vals);
+      }
+      default:
+        err = "internal: unexpected node";
+        return false;
+    }
+  }
+};
+
+std::string join(const std::vector<std::string>& v, const char* sep) {
+  std::string s;
+  for (size_t q = 0; q < v.size(); ++q) s += (q ? sep : "") + v[q];
+  return s;
+}
+
+std::string bound(const std::vector<std::string>& v, const char* f) {
+  std::string s = v[0];
+  for (size_t q = 1; q < v.size(); ++q) s = std::string(f) + "(" + s + "," + v[q] + ")";
+  return s;
+}
+
+bool generate(Model& m, std::string& src, std::string& err) {
+  std::vector<std::vector<AltCode>> alts(m.tabs.size());
+  for (int t : m.order) {
+    const Tab& T = m.tabs[t];
+    std::vector<NP> flat;
+    if (!flatten(m, T.p, true, flat, err)) return false;
+    if ((int)flat.size() != T.alt) { err = "internal: alternative count"; return false; }
+    for (int q = 0; q < (int)flat.size(); ++q) {
+      Gen g(m);
+      AltCode o;
+      o.rule = T.id + q;
+      std::vector<std::string> vals;
+      if (!g.gen(flat[q].get(), "i", "j", o, vals)) { err = g.err; return false; }
+      if (vals.size() != 1) {
+        err = "alternative " + std::to_string(o.rule) + " of '" + T.name + "' does not yield one answer (map the concatenation with a function)";
+        return false;
+      }
+      if (!o.loops.empty()) m.loops = true;
+      o.value = vals[0];
+      alts[t].push_back(o);
+    }
+  }
+  const int NT = (int)m.tabs.size();
+  std::ostringstream s;
+  s << "// generated by libdpx from an ADP grammar (hipRTC provides the HIP runtime declarations)\n"
+       "struct T2ii { int _1, _2; };\n"
+    << m.header << "\n"
+    << "typedef " << m.alphabet << " input_t;\n"
+    << "typedef " << m.answer << " answer_t;\n"
+    << "static_assert(sizeof(input_t) == " << m.alphabet_bytes << ", \"alphabet size\");\n"
+    << "static_assert(sizeof(answer_t) == " << m.answer_bytes << ", \"answer size\");\n"
+    << "struct Dims { int n1, n2; };\n"
+    << "struct Tabs { void* c[" << NT << "]; short* r[" << NT << "]; int* p[" << NT << "][2]; };\n"
+    << "struct Rec { int i, j, rule, pos[2]; };\n";
+  if (m.tracks == 1)
+    s << "__device__ __forceinline__ long long idx_(const Dims& z, int i, int j) {\n"
+         "  const long long d = j - i;  // diagonal-major: the cells of one diagonal are contiguous\n"
+         "  return d * (z.n1 + 1) - d * (d - 1) / 2 + i;\n}\n";
+  else
+    s << "__device__ __forceinline__ long long idx_(const Dims& z, int i, int j) { return (long long)i * (z.n2 + 1) + j; }\n";
+  for (auto& f : m.fns) {
+    if (!f.args.empty()) {
+      s << "template <";
+      for (size_t q = 0; q < f.args.size(); ++q) s << (q ? ", " : "") << "class A" << q;
+      s << ">\n";
+    }
+    s << "__device__ __forceinline__ auto f_" << f.name << "(const input_t* _in1, const input_t* _in2";
+    for (size_t q = 0; q < f.args.size(); ++q) s << ", A" << q << " " << f.args[q];
+    s << ") {\n  (void)_in1; (void)_in2;\n  " << f.body << "\n}\n";
+  }
+  auto better = [&](const std::string& c, const std::string& c0) {
+    std::string cmp;
+    switch (m.h.k) {
+      case A_MAX: cmp = c + ">" + c0; break;
+      case A_MIN: cmp = c + "<" + c0; break;
+      case A_MAXBY: cmp = "f_" + m.fns[m.h.fn].name + "(_in1,_in2," + c + ")>f_" + m.fns[m.h.fn].name + "(_in1,_in2," + c0 + ")"; break;
+      case A_MINBY: cmp = "f_" + m.fns[m.h.fn].name + "(_in1,_in2," + c + ")<f_" + m.fns[m.h.fn].name + "(_in1,_in2," + c0 + ")"; break;
+    }
+    return cmp;
+  };
+  for (int t : m.order) {
+    const Tab& T = m.tabs[t];
+    s << "// ---- " << T.name << "[i,j]: rules " << T.id << ".." << T.id + T.alt - 1 << "\n"
+      << "__device__ __forceinline__ void tab_" << t
+      << "(const input_t* _in1, const input_t* _in2, const Dims& z, const Tabs& T, int i, int j) {\n"
+      << "  answer_t _c0" << (m.empty.empty() ? "{}" : " = (answer_t)(" + m.empty + ")") << ";\n"
+      << "  int _r0 = -1, _p0 = 0, _p1 = 0;\n";
+    for (auto& o : alts[t]) {
+      std::string ind = "  ";
+      s << ind << "{  // rule " << o.rule << "\n";
+      int close = 1;
+      for (auto& L : o.loops) {
+        ind += "  ";
+        s << ind << "for (int " << L.var << " = " << bound(L.lo, "max") << ", " << L.var << "_u = " << bound(L.hi, "min")
+          << "; " << L.var << " <= " << L.var << "_u; ++" << L.var << ") {\n";
+        ++close;
+      }
+      std::vector<std::string> conds = o.arith;
+      conds.insert(conds.end(), o.other.begin(), o.other.end());
+      ind += "  ";
+      if (!conds.empty()) { s << ind << "if (" << join(conds, " && ") << ") {\n"; ++close; ind += "  "; }
+      s << ind << "const answer_t _c = " << o.value << ";\n"
+        << ind << "if (" << better("_c", "_c0") << (m.empty.empty() ? " || _r0 == -1" : "") << ") { _c0 = _c; _r0 = " << o.rule
+        << "; _p0 = " << (o.pos.size() > 0 ? o.pos[0] : "0") << "; _p1 = " << (o.pos.size() > 1 ? o.pos[1] : "0") << "; }\n";
+      for (; close > 0; --close) {
+        ind.resize(ind.size() - 2);
+        s << ind << "}\n";
+      }
+    }
+    s << "  const long long x = idx_(z, i, j);\n"
+      << "  ((answer_t*)T.c[" << t << "])[x] = _c0;\n"
+      << "  T.r[" << t << "][x] = (short)_r0;\n";
+    for (int q = 0; q < T.cat; ++q) s << "  T.p[" << t << "][" << q << "][x] = _p" << q << ";\n";
+    s << "}\n";
+  }
+  // fill: one diagonal per launch (d0 == d1 - 1), or a run of small
+  // diagonals by one workgroup with a barrier between them
+  s << "extern \"C\" __global__ void adp_fill(const input_t* _in1, const input_t* _in2, Dims z, Tabs T, int d0, int d1) {\n"
+       "  for (int d = d0; d < d1; ++d) {\n";
+  if (m.tracks == 1)
+    s << "    const int cnt = z.n1 + 1 - d;\n";
+  else
+    s << "    const int ilo = d > z.n2 ? d - z.n2 : 0, ihi = d < z.n1 ? d : z.n1, cnt = ihi - ilo + 1;\n";
+  s << "    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {\n";
+  if (m.tracks == 1)
+    s << "      const int i = t, j = t + d;\n";
+  else
+    s << "      const int i = ilo + t, j = d - i;\n";
+  for (int t : m.order) s << "      tab_" << t << "(_in1, _in2, z, T, i, j);\n";
+  s << "    }\n"
+       "    if (d + 1 < d1) __syncthreads();\n"
+       "  }\n}\n";
+  // finish: axiom cell (or best window), answer, backtrace (one thread)
+  const Tab& AX = m.tabs[m.axiom];
+  s << "extern \"C\" __global__ void adp_finish(const input_t* _in1, const input_t* _in2, Dims z, Tabs T, "
+       "answer_t* ans, int* out, Rec* tr, int cap) {\n"
+       "  int i0, j0;\n  bool found;\n";
+  if (m.window > 0) {
+    s << "  found = false; i0 = 0; j0 = " << m.window << ";\n"
+      << "  answer_t best{};\n"
+      << "  for (int i = 0, j = " << m.window << "; j <= z.n1; ++i, ++j) {\n";
+    if (!AX.always_valid) s << "    if (!" << Gen(m).valid(m.axiom, "i", "j") << ") continue;\n";
+    s << "    const answer_t c = " << Gen(m).cost(m.axiom, "i", "j") << ";\n"
+      << "    if (!found || " << better("c", "best") << ") { best = c; i0 = i; j0 = j; found = true; }\n"
+      << "  }\n"
+      << "  if (found) *ans = best;\n";
+  } else {
+    if (m.tracks == 1) s << "  i0 = 0; j0 = z.n1;\n";
+    else s << "  i0 = z.n1; j0 = z.n2;\n";
+    s << "  found = " << (AX.always_valid ? "true" : Gen(m).valid(m.axiom, "i0", "j0")) << ";\n"
+      << "  *ans = " << Gen(m).cost(m.axiom, "i0", "j0") << ";\n";
+  }
+  s << "  out[0] = found ? 1 : 0; out[2] = i0; out[3] = j0;\n"
+       "  if (!tr || !found) { out[1] = 0; return; }\n"
+       "  int wr = 0;\n"
+       "  bool over = false;\n"
+       "  auto push = [&](int i, int j, int rule) {\n"
+       "    if (wr < cap) { tr[wr].i = i; tr[wr].j = j; tr[wr].rule = rule; tr[wr].pos[0] = 0; tr[wr].pos[1] = 0; ++wr; }\n"
+       "    else over = true;\n"
+       "  };\n"
+    << "  push(i0, j0, " << AX.id << ");\n"
+    << "  for (int rd = 0; rd < wr; ++rd) {\n"
+       "    const int i = tr[rd].i, j = tr[rd].j;\n"
+       "    const long long x = idx_(z, i, j);\n"
+       "    int r = -1;\n"
+       "    switch (tr[rd].rule) {\n";
+  for (int t : m.order) {
+    const Tab& T = m.tabs[t];
+    s << "      case " << T.id << ": r = T.r[" << t << "][x];";
+    for (int q = 0; q < T.cat; ++q) s << " tr[rd].pos[" << q << "] = T.p[" << t << "][" << q << "][x];";
+    s << " break;\n";
+  }
+  s << "    }\n"
+       "    tr[rd].rule = r;\n"
+       "    const int p0 = tr[rd].pos[0], p1 = tr[rd].pos[1];\n"
+       "    (void)p0; (void)p1;\n"
+       "    switch (r) {\n";
+  for (int t : m.order)
+    for (auto& o : alts[t]) {
+      s << "      case " << o.rule << ": {";
+      for (size_t q = 0; q < o.var_of_pos.size(); ++q)
+        if (!o.var_of_pos[q].empty()) s << " const int " << o.var_of_pos[q] << " = p" << q << ";";
+      for (auto& kd : o.kids) s << " push(" << kd.a << ", " << kd.b << ", " << m.tabs[kd.tab].id << ");";
+      s << " break; }\n";
+    }
+  s << "    }\n"
+       "  }\n"
+       "  out[1] = over ? -1 : wr;\n"
+       "}\n";
+  src = s.str();
+  return true;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ compiled grammar
+
+struct Grammar {
+  Model m;
+  std::string src;
+  std::vector<char> code;
+  hipModule_t mod = nullptr;
+  hipFunction_t fill = nullptr, finish = nullptr;
+  void* dev = nullptr;
+  size_t dev_cap = 0;
+  void* host = nullptr;  // pinned: answer, out[4], trace
+  size_t host_cap = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+Grammar* compile(const char* spec, bool load, std::string& err) {
+  if (!spec) { err = "null spec"; return nullptr; }
+  std::unique_ptr<Grammar> g(new Grammar());
+  J root;
+  JParser jp{spec, spec, ""};
+  if (!jp.val(root)) { err = "grammar JSON: " + jp.err; return nullptr; }
+  Builder b(g->m);
+  if (!b.build(root)) { err = "grammar: " + b.err; return nullptr; }
+  if (!analyze(g->m, err)) { err = "grammar: " + err; return nullptr; }
+  if (!generate(g->m, g->src, err)) { err = "grammar: " + err; return nullptr; }
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, g->src.c_str(), "adp_grammar.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+    err = "hiprtcCreateProgram failed";
+    return nullptr;
+  }
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-fwrapv", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    err = "grammar does not compile (hipRTC):\n" + log;
+    return nullptr;
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  g->code.resize(cs);
+  hiprtcGetCode(prog, g->code.data());
+  hiprtcDestroyProgram(&prog);
+  if (load) {
+    if (hipModuleLoadData(&g->mod, g->code.data()) != hipSuccess ||
+        hipModuleGetFunction(&g->fill, g->mod, "adp_fill") != hipSuccess ||
+        hipModuleGetFunction(&g->finish, g->mod, "adp_finish") != hipSuccess) {
+      err = "loading the compiled grammar failed";
+      if (g->mod) (void)hipModuleUnload(g->mod);
+      return nullptr;
+    }
+    if (hipEventCreate(&g->ev[0]) != hipSuccess || hipEventCreate(&g->ev[1]) != hipSuccess) {
+      err = "event creation failed";
+      (void)hipModuleUnload(g->mod);
+      return nullptr;
+    }
+  }
+  return g.release();
+}
+
+void destroy(Grammar* g) {
+  if (!g) return;
+  if (g->dev) (void)hipFree(g->dev);
+  if (g->host) (void)hipHostFree(g->host);
+  for (auto& e : g->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (g->mod) (void)hipModuleUnload(g->mod);
+  delete g;
+}
+
+const std::string& source(const Grammar* g) { return g->src; }
+
+void info(const Grammar* g, dpx_grammar_info* o) {
+  const Model& m = g->m;
+  o->tracks = m.tracks;
+  int used = 0;
+  for (auto& t : m.tabs) used += t.used ? 1 : 0;
+  o->n_tabs = used;
+  o->n_rules = m.n_rules;
+  o->max_pos = m.max_pos;
+  o->axiom_id = m.tabs[m.axiom].id;
+  o->window = m.window;
+  o->answer_bytes = m.answer_bytes;
+  o->alphabet_bytes = m.alphabet_bytes;
+  o->loops = m.loops ? 1 : 0;
+  o->code_bytes = (int64_t)g->code.size();
+}
+
+namespace {
+size_t r256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+int parse(Grammar* g, hipStream_t s, const void* in1, int n1, const void* in2, int n2, void* answer, int* found,
+          dpx_trace_t* tr, int cap, int* len, float* fill_ms, std::string& err) {
+  const Model& m = g->m;
+  if (!g->mod) { err = "grammar was compiled without loading"; return DPX_EINVAL; }
+  if (n1 < 0 || (m.tracks == 2 && n2 < 0) || (n1 > 0 && !in1) || (m.tracks == 2 && n2 > 0 && !in2)) {
+    err = "bad input";
+    return DPX_EINVAL;
+  }
+  if (m.tracks == 1) n2 = 0;
+  // cells: single track, diagonals 0..D of the (n1+1)-wide triangle
+  const int64_t W1 = (int64_t)n1 + 1;
+  const int64_t D = m.tracks == 1 ? (m.window > 0 ? std::min<int64_t>(m.window, n1) : n1) : (int64_t)n1 + n2;
+  const int64_t cells = m.tracks == 1 ? (D + 1) * W1 - (D + 1) * D / 2 : W1 * ((int64_t)n2 + 1);
+  if (cells > ((int64_t)1 << 34)) { err = "grammar tables too large"; return DPX_EINVAL; }
+  const int NT = (int)m.tabs.size();
+  // device block: inputs, then per tabulation cost | rule | positions, then out and trace
+  size_t off = 0;
+  const size_t o_in1 = off; off += r256((size_t)n1 * m.alphabet_bytes + 16);
+  const size_t o_in2 = off; off += r256((size_t)n2 * m.alphabet_bytes + 16);
+  std::vector<size_t> oc(NT), orr(NT), op(NT);
+  for (int t = 0; t < NT; ++t) {
+    if (!m.tabs[t].used) continue;
+    oc[t] = off; off += r256((size_t)cells * m.answer_bytes);
+    orr[t] = off; off += r256((size_t)cells * 2);
+    op[t] = off; off += r256((size_t)cells * 4 * m.tabs[t].cat);
+  }
+  const size_t ans4 = ((size_t)m.answer_bytes + 3) & ~(size_t)3;  // answer, then out[4]
+  const size_t o_ans = off; off += r256(ans4 + 16);
+  const size_t o_tr = off; off += r256(sizeof(dpx_trace_t) * (size_t)(cap > 0 ? cap : 1));
+  if (off > g->dev_cap) {
+    if (g->dev) (void)hipFree(g->dev);
+    g->dev = nullptr;
+    g->dev_cap = 0;
+    if (hipMalloc(&g->dev, off) != hipSuccess) { err = "device allocation for the grammar tables failed"; return DPX_ENOMEM; }
+    g->dev_cap = off;
+  }
+  const size_t hbytes = 256 + (size_t)m.answer_bytes + sizeof(dpx_trace_t) * (size_t)(cap > 0 ? cap : 1);
+  if (hbytes > g->host_cap) {
+    if (g->host) (void)hipHostFree(g->host);
+    g->host = nullptr;
+    g->host_cap = 0;
+    if (hipHostMalloc(&g->host, hbytes, hipHostMallocDefault) != hipSuccess) { err = "pinned staging failed"; return DPX_ENOMEM; }
+    g->host_cap = hbytes;
+  }
+  char* dev = (char*)g->dev;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess) err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipSuccess;
+  };
+  if (n1 > 0 && !chk(hipMemcpyAsync(dev + o_in1, in1, (size_t)n1 * m.alphabet_bytes, hipMemcpyHostToDevice, s), "input copy")) return DPX_EHIP;
+  if (n2 > 0 && !chk(hipMemcpyAsync(dev + o_in2, in2, (size_t)n2 * m.alphabet_bytes, hipMemcpyHostToDevice, s), "input copy")) return DPX_EHIP;
+  struct Dims { int n1, n2; } z{n1, n2};
+  // the kernel's Tabs {c[NT], r[NT], p[NT][2]}: 4 NT pointers
+  std::vector<char> tabs_arg(sizeof(void*) * NT * 4, 0);
+  {
+    void** c = (void**)tabs_arg.data();
+    void** r = c + NT;
+    void** p = r + NT;
+    for (int t = 0; t < NT; ++t) {
+      if (!m.tabs[t].used) continue;
+      c[t] = dev + oc[t];
+      r[t] = dev + orr[t];
+      p[2 * t] = m.tabs[t].cat > 0 ? dev + op[t] : nullptr;
+      p[2 * t + 1] = m.tabs[t].cat > 1 ? dev + op[t] + (size_t)cells * 4 : nullptr;
+    }
+  }
+  const void* d_in1 = dev + o_in1;
+  const void* d_in2 = dev + o_in2;
+  if (!chk(hipEventRecord(g->ev[0], s), "event")) return DPX_EHIP;
+  // Diagonals: a diagonal of many cells gets its own launch (64 threads per
+  // workgroup, one cell each); when the cells have no split loop, runs of
+  // diagonals of at most 256 cells go to one workgroup that steps through
+  // them with a barrier in between.
+  auto count = [&](int64_t d) -> int64_t {
+    if (m.tracks == 1) return W1 - d;
+    const int64_t lo = d > n2 ? d - n2 : 0, hi = d < n1 ? d : n1;
+    return hi - lo + 1;
+  };
+  for (int64_t d = 0; d <= D;) {
+    const int64_t c = count(d);
+    int d1 = (int)d + 1;
+    unsigned grid, block = 64;
+    if (!m.loops && c <= 256) {
+      while (d1 <= D && count(d1) <= 256) ++d1;
+      grid = 1;
+      block = 256;
+    } else {
+      grid = (unsigned)((c + 63) / 64);
+    }
+    int d0 = (int)d;
+    void* args[] = {(void*)&d_in1, (void*)&d_in2, (void*)&z, (void*)tabs_arg.data(), (void*)&d0, (void*)&d1};
+    if (!chk(hipModuleLaunchKernel(g->fill, grid, 1, 1, block, 1, 1, 0, s, args, nullptr), "fill launch")) return DPX_EHIP;
+    d = d1;
+  }
+  if (!chk(hipEventRecord(g->ev[1], s), "event")) return DPX_EHIP;
+  void* d_ans = dev + o_ans;
+  int* d_out = (int*)(dev + o_ans + ans4);  // found, trace length (-1: capacity), i0, j0
+  void* d_tr = (tr && cap > 0) ? dev + o_tr : nullptr;
+  int capv = cap;
+  void* fargs[] = {(void*)&d_in1, (void*)&d_in2, (void*)&z, (void*)tabs_arg.data(), (void*)&d_ans, (void*)&d_out, (void*)&d_tr, (void*)&capv};
+  if (!chk(hipModuleLaunchKernel(g->finish, 1, 1, 1, 1, 1, 1, 0, s, fargs, nullptr), "finish launch")) return DPX_EHIP;
+  char* h = (char*)g->host;
+  if (!chk(hipMemcpyAsync(h, d_ans, ans4 + 16, hipMemcpyDeviceToHost, s), "result copy")) return DPX_EHIP;
+  if (!chk(hipStreamSynchronize(s), "grammar run")) return DPX_EHIP;
+  const int* hout = (const int*)(h + ans4);
+  const int n_tr = hout[1];
+  if (d_tr && n_tr > 0) {
+    if (!chk(hipMemcpyAsync(h + 256 + m.answer_bytes, d_tr, sizeof(dpx_trace_t) * (size_t)n_tr, hipMemcpyDeviceToHost, s), "trace copy") ||
+        !chk(hipStreamSynchronize(s), "trace copy"))
+      return DPX_EHIP;
+    std::memcpy(tr, h + 256 + m.answer_bytes, sizeof(dpx_trace_t) * (size_t)n_tr);
+  }
+  if (answer) std::memcpy(answer, h, m.answer_bytes);
+  if (found) *found = hout[0];
+  if (len) *len = d_tr ? n_tr : 0;
+  if (fill_ms) (void)hipEventElapsedTime(fill_ms, g->ev[0], g->ev[1]);
+  if (d_tr && n_tr < 0) { err = "trace capacity too small"; return DPX_ETRACE; }
+  return DPX_OK;
+}
+
+}  // namespace adp
+}  // namespace dpx

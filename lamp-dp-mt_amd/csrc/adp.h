@@ -1,0 +1,31 @@
+// adp.h -- generic ADP grammars compiled to HIP at run time (internal).
+//
+// A grammar arrives as a JSON tree of the reference's parser combinators
+// (tabulate, terminals, |, ~, -~, ~-, ^^, filter, aggregate with C bodies
+// for the algebra functions).  compile() runs the reference's analysis
+// (yield sizes, evaluation order, rule ids), flattens every tabulation into
+// its alternatives, generates the cell bodies and the backtrace as HIP
+// source and compiles it for gfx950 with hipRTC.  parse() fills the tables
+// diagonal by diagonal and walks the backtrace on the device.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/dpx.h"
+
+namespace dpx {
+namespace adp {
+
+struct Grammar;
+
+// load == false: generate and compile only (no device needed)
+Grammar* compile(const char* spec, bool load, std::string& err);
+void destroy(Grammar* g);
+const std::string& source(const Grammar* g);
+void info(const Grammar* g, dpx_grammar_info* out);
+int parse(Grammar* g, hipStream_t s, const void* in1, int n1, const void* in2, int n2, void* answer, int* found,
+          dpx_trace_t* tr, int cap, int* len, float* fill_ms, std::string& err);
+
+}  // namespace adp
+}  // namespace dpx

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/dpx.h"
+#include "adp.h"
 #include "engine.h"
 #include "rna_model.h"
 
@@ -1622,6 +1623,72 @@ int dpx_server_get_stats(dpx_server* S, dpx_server_stats* out) {
   std::lock_guard<std::mutex> lk(S->mu);
   *out = S->st;
   return DPX_OK;
+}
+
+// ---------------------------------------------------------------- generic grammars
+
+struct dpx_grammar {
+  dpx::adp::Grammar* g = nullptr;
+  int device = 0;
+  std::mutex mu;
+};
+
+int dpx_grammar_create(dpx_ctx* c, const char* spec, dpx_grammar** out) {
+  if (!c || !spec || !out) return DPX_EINVAL;
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c->err, DPX_EHIP, "hipSetDevice");
+  dpx::adp::Grammar* g = dpx::adp::compile(spec, true, c->err);
+  if (!g) return c->err.rfind("loading", 0) == 0 ? DPX_EHIP : DPX_EINVAL;
+  dpx_grammar* h = new dpx_grammar();
+  h->g = g;
+  h->device = c->device;
+  *out = h;
+  return DPX_OK;
+}
+
+void dpx_grammar_destroy(dpx_grammar* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  dpx::adp::destroy(h->g);
+  delete h;
+}
+
+int dpx_grammar_check(const char* spec, const char** src, const char** msg) {
+  static thread_local std::string s_src, s_err;
+  s_src.clear();
+  s_err.clear();
+  if (src) *src = s_src.c_str();
+  if (msg) *msg = s_err.c_str();
+  dpx::adp::Grammar* g = dpx::adp::compile(spec, false, s_err);
+  if (msg) *msg = s_err.c_str();
+  if (!g) return DPX_EINVAL;
+  s_src = dpx::adp::source(g);
+  dpx::adp::destroy(g);
+  if (src) *src = s_src.c_str();
+  return DPX_OK;
+}
+
+int dpx_grammar_get_info(const dpx_grammar* h, dpx_grammar_info* out) {
+  if (!h || !out) return DPX_EINVAL;
+  dpx::adp::info(h->g, out);
+  return DPX_OK;
+}
+
+const char* dpx_grammar_source(const dpx_grammar* h) { return h ? dpx::adp::source(h->g).c_str() : nullptr; }
+
+int dpx_grammar_parse(dpx_ctx* c, dpx_grammar* h, const void* in1, int n1, const void* in2, int n2, void* answer,
+                      int* found, dpx_trace_t* trace, int cap, int* len) {
+  if (!c || !h) return DPX_EINVAL;
+  if (h->device != c->device) return fail(c->err, DPX_EINVAL, "grammar was compiled for another device");
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::lock_guard<std::mutex> lk2(h->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c->err, DPX_EHIP, "hipSetDevice");
+  float ms = 0;
+  const int rc = dpx::adp::parse(h->g, c->ws.stream, in1, n1, in2, n2, answer, found, trace, cap, len, &ms, c->err);
+  c->t_fill = ms;
+  c->t_fin = 0;
+  return rc;
 }
 
 }  // extern "C"

@@ -29,6 +29,8 @@ EXPORTS = [
     "dpx_server_trace", "dpx_server_get_stats",
     "dpx_plan_create", "dpx_plan_run", "dpx_plan_timing", "dpx_plan_results", "dpx_plan_destroy",
     "dpx_tune_bucket", "dpx_get_tuning", "dpx_set_tuning", "dpx_autotune", "dpx_set_swat_trace_budget",
+    "dpx_grammar_create", "dpx_grammar_destroy", "dpx_grammar_check", "dpx_grammar_get_info",
+    "dpx_grammar_source", "dpx_grammar_parse",
 ]
 
 REQ_NUSSINOV, REQ_ZUKER, REQ_SWAT, REQ_MATCHAIN, REQ_RNAFOLD = 1, 2, 3, 4, 5
@@ -89,6 +91,12 @@ class TuneReport(ctypes.Structure):
                 ("chosen", ctypes.c_int32)]
 
 
+class GrammarInfo(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in ("tracks", "n_tabs", "n_rules", "max_pos", "axiom_id", "window",
+                                              "answer_bytes", "alphabet_bytes", "loops", "reserved")] + \
+               [("code_bytes", ctypes.c_int64)]
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int64) for k in
                 ("submitted", "completed", "failed", "rejected", "launches", "cells_done",
@@ -123,6 +131,15 @@ def lib():
         L.dpx_plan_results.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Trace),
                                        ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]
         L.dpx_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.dpx_grammar_create.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.dpx_grammar_destroy.argtypes = [ctypes.c_void_p]
+        L.dpx_grammar_check.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p)]
+        L.dpx_grammar_get_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(GrammarInfo)]
+        L.dpx_grammar_source.restype = ctypes.c_char_p
+        L.dpx_grammar_source.argtypes = [ctypes.c_void_p]
+        L.dpx_grammar_parse.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                                        ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Trace),
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
 
@@ -168,6 +185,11 @@ class Context:
         """Above this many bytes of traceback codes, swat() keeps checkpoint
         rows and recomputes segments during the traceback (same trace)."""
         self._check(self._L.dpx_set_swat_trace_budget(self.h, ctypes.c_int64(nbytes)))
+
+    # ---- generic grammars (dpx.adp) ---------------------------------------
+    def grammar(self, g):
+        """Compiles a dpx.adp.Grammar for this device (hipRTC)."""
+        return CompiledGrammar(self, g)
 
     # ---- single instances -------------------------------------------------
     def nussinov(self, seq, min_loop=0, trace=True):
@@ -301,6 +323,62 @@ class Context:
         return [{"kind": KIND_NAMES[r.kind], "bucket": r.bucket, "n_lo": r.n_lo, "n_hi": r.n_hi, "count": r.count,
                  "wg_per_cu": r.t.wg_per_cu, "chunk": r.t.chunk, "ms": round(r.ms, 4), "chosen": bool(r.chosen)}
                 for r in rep[:min(n.value, cap)]]
+
+
+def grammar_check(spec_json):
+    """(ok, generated HIP source, message) without a device."""
+    src, msg = ctypes.c_char_p(), ctypes.c_char_p()
+    rc = lib().dpx_grammar_check(spec_json.encode(), ctypes.byref(src), ctypes.byref(msg))
+    return rc == OK, (src.value or b"").decode(), (msg.value or b"").decode()
+
+
+class CompiledGrammar:
+    """A grammar compiled and loaded on a context's device."""
+
+    def __init__(self, ctx, g):
+        self.ctx, self.g = ctx, g
+        self._L = ctx._L
+        h = ctypes.c_void_p()
+        ctx._check(self._L.dpx_grammar_create(ctx.h, g.spec_json().encode(), ctypes.byref(h)))
+        self.h = h
+        self.info = GrammarInfo()
+        self._L.dpx_grammar_get_info(self.h, ctypes.byref(self.info))
+
+    @property
+    def source(self):
+        return self._L.dpx_grammar_source(self.h).decode()
+
+    def parse(self, in1, in2=None, trace=True, cap=None):
+        """(answer, found, trace records) of one input (two for two-track grammars)."""
+        g = self.g
+        b1 = g.pack(in1)
+        b2 = g.pack(in2) if in2 is not None else b""
+        n1 = len(b1) // self.info.alphabet_bytes
+        n2 = len(b2) // self.info.alphabet_bytes
+        cap = cap or 8 * (n1 + n2) + 64
+        while True:
+            tr = (Trace * cap)() if trace else None
+            ans = ctypes.create_string_buffer(max(8, self.info.answer_bytes))
+            found, tl = ctypes.c_int(), ctypes.c_int()
+            rc = self._L.dpx_grammar_parse(self.ctx.h, self.h, b1, n1, b2 if in2 is not None else None, n2, ans,
+                                           ctypes.byref(found), tr, cap if trace else 0, ctypes.byref(tl))
+            if rc == ETRACE and trace and cap < (1 << 26):
+                cap *= 4
+                continue
+            self.ctx._check(rc)
+            a = g.unpack_answer(ans.raw)
+            return a, bool(found.value), (_tuples(tr, tl.value) if trace else None)
+
+    def close(self):
+        if self.h:
+            self._L.dpx_grammar_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def now():

@@ -8,7 +8,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "build")
 LIB = os.path.join(PKG, "libdpx.so")
 COPY_PROBE = os.path.join(OBJ, "hbm_copy")  # reference copy kernel for the HBM byte counters
-SOURCES = ["api.cpp", "rna_params.cpp", "tri.hip", "finalize.hip", "swat.hip"]
+SOURCES = ["api.cpp", "adp.cpp", "rna_params.cpp", "tri.hip", "finalize.hip", "swat.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -33,7 +33,7 @@ def build(verbose=False, jobs=4):
     cmd = [HIPCC, "-x", "hip"] + FLAGS + [os.path.join(CSRC, "hbm_copy.hip"), "-o", COPY_PROBE]
     procs.append(("hbm_copy.hip", subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
     _drain(procs)
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB] + objs + ["-lhiprtc"]
     out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if out.returncode != 0:
         raise RuntimeError("link failed:\n" + out.stdout.decode())

@@ -35,7 +35,7 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
     structs = {"dpx_trace_t": dpx.Trace, "dpx_swat_scores": dpx.SwatScores, "dpx_server_config": dpx.ServerConfig,
                "dpx_request": dpx.Request, "dpx_result": dpx.Result, "dpx_server_stats": dpx.ServerStats,
-               "dpx_tuning": dpx.Tuning, "dpx_tune_report": dpx.TuneReport}
+               "dpx_tuning": dpx.Tuning, "dpx_tune_report": dpx.TuneReport, "dpx_grammar_info": dpx.GrammarInfo}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dpx.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append('printf("%%zu\\n", sizeof(%s));' % cname)

@@ -1,0 +1,140 @@
+"""Generic ADP grammars on the GPU (dpx_grammar_*): every example grammar of
+the reference compiled to HIP at run time and checked against the oracle
+(score, found flag and full trace), and against the built-in kernels where
+the grammar is theirs: Nussinov.scala vs dpx_nussinov (and bench2/nu.c's
+golden vectors), SWatAffine.scala vs dpx_swat, MatrixMult2.scala vs
+dpx_matchain."""
+import json
+import random
+
+import pytest
+
+import adp_grammars as G
+import oracle_lib as o
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import dpx
+    c = dpx.Context()
+    yield c
+    c.close()
+
+
+def _vs_oracle(cg, g, in1, in2=None):
+    ans, found, tr = cg.parse(in1, in2)
+    _, (oans, ofound, otr) = G.run_oracle(g, in1, in2)
+    assert found == bool(ofound)
+    if found:
+        assert ans == oans
+        assert [tuple(t) for t in tr] == [tuple(t) for t in otr]
+    return ans, found, tr
+
+
+def test_nussinov_grammar_vs_builtin_and_golden(ctx):
+    from dpx import gen
+    g = G.nussinov()
+    cg = ctx.grammar(g)
+    assert cg.info.n_rules == 5 and cg.info.max_pos == 1 and cg.info.loops == 1
+    for case in json.load(open(o.ROOT + "/tests/golden/nussinov_ref.json")):
+        print("nussinov golden n=%d" % len(case["seq"]), flush=True)
+        ans, found, tr = cg.parse(case["seq"])
+        assert found and ans == case["score"]
+        assert [list(t) for t in tr] == case["trace"]
+    for n in [0, 1, 2, 31, 64, 65, 200, 700]:
+        s = gen.rna_uniform(n, n + 11)
+        ans, found, tr = cg.parse(s)
+        score, btr = ctx.nussinov(s, min_loop=0)
+        assert ans == score and tr == btr, n
+    for n in [0, 3, 17, 40]:
+        _vs_oracle(cg, g, gen.rna_uniform(n, 7 * n + 1))
+
+
+@pytest.mark.parametrize("sc", [(2, -3, -5, -2), (10, -3, -3, -1)])
+def test_swat_affine_grammar_vs_builtin(ctx, sc):
+    from dpx import gen
+    g = G.swat_affine(*sc)
+    cg = ctx.grammar(g)
+    assert cg.info.tracks == 2 and cg.info.n_rules == 10 and cg.info.loops == 0
+    for n1, n2 in [(0, 0), (0, 5), (6, 0), (1, 1), (63, 64), (200, 77), (333, 1000), (1500, 1400)]:
+        print("swat grammar %dx%d" % (n1, n2), flush=True)
+        a, b = gen.dna_pair(n1, n2, n1 * 1000 + n2)
+        ans, found, tr = cg.parse(a, b)
+        s, best, btr = ctx.swat(a, b, *sc, trace_from=0)
+        assert found and ans == s
+        assert [t[:3] for t in tr] == [t[:3] for t in btr]
+    for n1, n2 in [(5, 9), (20, 13)]:
+        a, b = gen.dna_pair(n1, n2, n1 + n2)
+        _vs_oracle(cg, g, a, b)
+
+
+def test_matrix_chain_grammar_vs_builtin(ctx):
+    g = G.matrix_chain()
+    cg = ctx.grammar(g)
+    rng = random.Random(4)
+    for n in [1, 2, 3, 10, 33, 64, 129, 300]:
+        dims = [rng.randint(1, 100) for _ in range(n + 1)]
+        mats = [(dims[q], dims[q + 1]) for q in range(n)]
+        print("chain grammar n=%d" % n, flush=True)
+        ans, found, tr = cg.parse(mats)
+        s, btr = ctx.matchain([m[0] for m in mats], [m[1] for m in mats])
+        assert found and ans == s
+        assert [t[:4] for t in tr] == [t[:4] for t in btr]
+    mats = [(10, 100), (100, 5), (5, 50)]
+    assert cg.parse(mats)[0] == 7500
+
+
+@pytest.mark.parametrize("h", ["min", "max"])
+def test_elmamun_vs_oracle(ctx, h):
+    g = G.elmamun(h)
+    cg = ctx.grammar(g)
+    rng = random.Random(1 if h == "min" else 2)
+    exprs = ["1+2*3*4+5", "9", "+", "1+", ""]
+    for _ in range(12):
+        n = rng.randint(1, 12)
+        exprs.append("".join(str(rng.randint(0, 9)) + rng.choice("+*") for _ in range(n)) + str(rng.randint(0, 9)))
+    for e in exprs:
+        _vs_oracle(cg, g, e)
+    assert cg.parse("1+2*3*4+5")[0] == (30 if h == "min" else 81)
+
+
+def test_brackets_window_vs_oracle(ctx):
+    g = G.brackets(8)
+    cg = ctx.grammar(g)
+    assert cg.info.window == 8 and cg.info.n_tabs == 1  # the unused tabulation is stripped
+    rng = random.Random(6)
+    cases = ["(((3)))((2))(1)((6))((((8))))(3)((4))", "((1)(2))", "(5)", "1234", "", "((((((((9))))))))"]
+    for _ in range(20):
+        cases.append("".join(rng.choice("()123456789") for _ in range(rng.randint(3, 60))))
+    for s in cases:
+        _vs_oracle(cg, g, s)
+
+
+@pytest.mark.parametrize("alg", ["sw", "nw"])
+def test_seqalign_vs_oracle(ctx, alg):
+    from dpx import gen
+    g = G.seqalign(alg)
+    cg = ctx.grammar(g)
+    for n1, n2 in [(0, 0), (0, 4), (3, 0), (1, 1), (7, 9), (25, 31), (40, 12)]:
+        a, b = gen.dna_pair(n1, n2, 31 * n1 + n2)
+        _vs_oracle(cg, g, a, b)
+
+
+def test_trace_capacity_and_errors(ctx):
+    import dpx
+    g = G.nussinov()
+    cg = ctx.grammar(g)
+    s = "GGGAAACCCAGGGAAACCCA" * 3
+    full = cg.parse(s)
+    assert cg.parse(s, cap=2) == full  # the binding grows the buffer after DPX_ETRACE
+    bad = G.nussinov()
+    spec = bad.spec()
+    with pytest.raises(dpx.DpxError):
+        ctx._check(ctx._L.dpx_grammar_create(ctx.h, b"{}", ctypes_ptr()))
+
+
+def ctypes_ptr():
+    import ctypes
+    return ctypes.byref(ctypes.c_void_p())

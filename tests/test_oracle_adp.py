@@ -1,0 +1,162 @@
+"""The generic-grammar oracle (oracle/adp_oracle.py) pinned on the CPU:
+
+* against the reference's own generated code: the Nussinov grammar of
+  Nussinov.scala must reproduce bench2/nu.c's golden vectors (score and full
+  trace: rule ids, split positions, BFS order, ties);
+* against the C oracle's Smith-Waterman/Gotoh and matrix chain (themselves
+  pinned by brute force and the reference's values) for SWatAffine.scala and
+  MatrixMult2.scala written as grammars;
+* against brute force: the optimum over every derivation (the reference's
+  Scala `apply` without aggregation) for ElMamun, Brackets (window), SeqAlign
+  (both algebras), Nussinov, and every trace replayed (`build`) gives the
+  answer;
+* the library generates and compiles (hipRTC, no device) each grammar, and
+  rejects what it does not support with a message.
+"""
+import ctypes
+import json
+import random
+
+import pytest
+
+import adp_grammars as G
+import oracle_lib as o
+
+
+def _ok_trace(orc, g, in1, in2, ans, tr):
+    assert orc.replay(g.py_input(in1), g.py_input(in2) if in2 is not None else None, tr) == ans
+
+
+def test_nussinov_grammar_matches_reference_generated_code():
+    fx = json.load(open(o.ROOT + "/tests/golden/nussinov_ref.json"))
+    g = G.nussinov()
+    for case in fx:
+        if len(case["seq"]) > 60:
+            continue
+        orc, (ans, found, tr) = G.run_oracle(g, case["seq"])
+        assert found and ans == case["score"], case["seq"]
+        assert [list(t) for t in tr] == case["trace"], case["seq"]
+
+
+@pytest.mark.parametrize("n1,n2", [(0, 0), (0, 3), (4, 0), (1, 1), (5, 9), (12, 7), (17, 17)])
+def test_swat_affine_grammar_matches_c_oracle(n1, n2):
+    from dpx import gen
+    a, b = gen.dna_pair(n1, n2, 100 * n1 + n2)
+    for sc in [(2, -3, -5, -2), (10, -3, -3, -1)]:
+        g = G.swat_affine(*sc)
+        orc, (ans, found, tr) = G.run_oracle(g, a, b)
+        s, best, otr = o.swat(a, b, *sc, from_best=False)
+        assert found and ans == s
+        assert [t[:3] for t in tr] == [t[:3] for t in otr]
+        _ok_trace(orc, g, a, b, ans, tr)
+
+
+def test_matrix_chain_grammar_matches_c_oracle():
+    rng = random.Random(5)
+    for n in [1, 2, 3, 5, 9, 14]:
+        dims = [rng.randint(1, 60) for _ in range(n + 1)]
+        mats = [(dims[q], dims[q + 1]) for q in range(n)]
+        g = G.matrix_chain()
+        orc, (ans, found, tr) = G.run_oracle(g, mats)
+        s, otr = o.matchain([m[0] for m in mats], [m[1] for m in mats])
+        assert found and ans == s
+        assert [t[:4] for t in tr] == [t[:4] for t in otr]
+        _ok_trace(orc, g, mats, None, ans, tr)
+    # the reference's test values (Tests.scala:16-25)
+    for mats, want in [([(10, 100), (100, 5), (5, 50)], 7500), ([(1, 2), (2, 20), (20, 2), (2, 4), (4, 2), (2, 1), (1, 7), (7, 3)], 122)]:
+        _, (ans, _, _) = G.run_oracle(G.matrix_chain(), mats)
+        assert ans == want
+
+
+def _brute(g, in1, in2=None):
+    orc, (ans, found, tr) = G.run_oracle(g, in1, in2)
+    vals = orc.enumerate_all(g.py_input(in1), g.py_input(in2) if in2 is not None else None)
+    return orc, ans, found, tr, vals
+
+
+@pytest.mark.parametrize("expr", ["1+2*3*4+5", "9", "2*3", "1+1*1+1*2", "3*0+4*2*1", "+", "1+"])
+def test_elmamun_vs_brute_force(expr):
+    for h, pick in [("min", min), ("max", max)]:
+        g = G.elmamun(h)
+        orc, ans, found, tr, vals = _brute(g, expr)
+        assert found == bool(vals)
+        if vals:
+            assert ans == pick(vals)
+            _ok_trace(orc, g, expr, None, ans, tr)
+    if expr == "1+2*3*4+5":  # ADP's classic: the seller gets (1+2)*3*(4+5), the buyer 1+(2*3*4)+5
+        assert G.run_oracle(G.elmamun("max"), expr)[1][0] == 81
+        assert G.run_oracle(G.elmamun("min"), expr)[1][0] == 30
+
+
+def test_brackets_window_vs_brute_force():
+    rng = random.Random(3)
+    cases = ["(((3)))((2))(1)((6))((((8))))(3)((4))", "((1)(2))", "(5)", "1234", "((((((((9))))))))"]
+    for _ in range(8):
+        cases.append("".join(rng.choice("()123") for _ in range(rng.randint(3, 14))))
+    for s in cases:
+        g = G.brackets(8)
+        orc, ans, found, tr, vals = _brute(g, s)
+        assert found == bool(vals), s
+        if vals:
+            assert ans == max(vals), s
+            _ok_trace(orc, g, s, None, ans, tr)
+
+
+@pytest.mark.parametrize("alg", ["sw", "nw"])
+def test_seqalign_vs_brute_force(alg):
+    from dpx import gen
+    for n1, n2, seed in [(0, 0, 1), (1, 2, 2), (3, 3, 3), (4, 2, 4), (5, 4, 5)]:
+        a, b = gen.dna_pair(n1, n2, seed)
+        g = G.seqalign(alg)
+        orc, ans, found, tr, vals = _brute(g, a, b)
+        assert found and ans == max(vals)
+        _ok_trace(orc, g, a, b, ans, tr)
+
+
+def test_nussinov_vs_brute_force():
+    rng = random.Random(9)
+    for n in list(range(0, 7)) * 2:  # the grammar is ambiguous: derivations grow exponentially
+        s = "".join(rng.choice("ACGU") for _ in range(n))
+        g = G.nussinov()
+        orc, ans, found, tr, vals = _brute(g, s)
+        assert ans == max(vals)
+        _ok_trace(orc, g, s, None, ans, tr)
+
+
+def _check(spec):
+    L = ctypes.CDLL(o.ROOT + "/lamp-dp-mt_amd/libdpx.so")
+    L.dpx_grammar_check.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p)]
+    src, msg = ctypes.c_char_p(), ctypes.c_char_p()
+    rc = L.dpx_grammar_check(spec.encode(), ctypes.byref(src), ctypes.byref(msg))
+    return rc, (src.value or b"").decode(), (msg.value or b"").decode()
+
+
+@pytest.mark.parametrize("make", [G.nussinov, lambda: G.elmamun("min"), G.brackets, G.seqalign,
+                                  lambda: G.seqalign("nw"), G.swat_affine, G.matrix_chain])
+def test_library_generates_and_compiles(make):
+    rc, src, msg = _check(make().spec_json())
+    assert rc == 0, msg
+    assert "adp_fill" in src and "adp_finish" in src
+
+
+def test_library_rejects_unsupported():
+    g = G.nussinov()
+    spec = g.spec()
+    bad = dict(spec, h="sum")
+    rc, _, msg = _check(json.dumps(bad))
+    assert rc != 0 and "aggregation" in msg
+    bad = dict(spec, axiom="nope")
+    assert _check(json.dumps(bad))[0] != 0
+    rc, _, msg = _check("{not json")
+    assert rc != 0 and "JSON" in msg
+    # a body that does not compile: the compiler log comes back
+    bad = json.loads(g.spec_json())
+    fn = sorted(bad["functions"])[0]
+    bad["functions"][fn]["body"] = "return undeclared_name;"
+    rc, _, msg = _check(json.dumps(bad))
+    assert rc != 0 and "undeclared_name" in msg
+    # a cycle within one cell
+    cyc = {"tracks": 1, "h": "max", "axiom": "A", "functions": {},
+           "tabulations": {"A": {"p": {"tab": "B"}}, "B": {"p": {"tab": "A"}}}}
+    rc, _, msg = _check(json.dumps(cyc))
+    assert rc != 0 and "loop" in msg
