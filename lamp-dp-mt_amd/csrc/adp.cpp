@@ -816,21 +816,45 @@ bool generate(Model& m, std::string& src, std::string& err) {
     }
     return cmp;
   };
+  // A cell is evaluated by a group of G lanes of one wave (G = 1, 8 or 64,
+  // chosen per diagonal): the outer split loop of each rule is strided over
+  // the group, rules without a loop run on lane 0, and the group's
+  // candidates are merged by value, then by (rule, outer split, inner split)
+  // -- the first strictly better candidate of the sequential loop order.
+  s << "template <class V> __device__ __forceinline__ V shfl_xor_(V v, int m, int w) {\n"
+       "  constexpr int N = (sizeof(V) + 3) / 4;\n"
+       "  int u[N] = {};\n"
+       "  __builtin_memcpy(u, &v, sizeof(V));\n"
+       "  for (int q = 0; q < N; ++q) u[q] = __shfl_xor(u[q], m, w);\n"
+       "  __builtin_memcpy(&v, u, sizeof(V));\n"
+       "  return v;\n}\n";
+  int used_tabs = 0;
+  for (auto& T : m.tabs) used_tabs += T.used ? 1 : 0;
   for (int t : m.order) {
     const Tab& T = m.tabs[t];
     s << "// ---- " << T.name << "[i,j]: rules " << T.id << ".." << T.id + T.alt - 1 << "\n"
-      << "__device__ __forceinline__ void tab_" << t
-      << "(const input_t* _in1, const input_t* _in2, const Dims& z, const Tabs& T, int i, int j) {\n"
+      << "template <int G> __device__ __forceinline__ void tab_" << t
+      << "(const input_t* _in1, const input_t* _in2, const Dims& z, const Tabs& T, int i, int j, int lane) {\n"
       << "  answer_t _c0" << (m.empty.empty() ? "{}" : " = (answer_t)(" + m.empty + ")") << ";\n"
-      << "  int _r0 = -1, _p0 = 0, _p1 = 0;\n";
+      << "  int _r0 = -1, _p0 = 0, _p1 = 0, _o0 = 0, _o1 = 0;\n";
     for (auto& o : alts[t]) {
       std::string ind = "  ";
       s << ind << "{  // rule " << o.rule << "\n";
       int close = 1;
-      for (auto& L : o.loops) {
+      if (o.loops.empty()) {
         ind += "  ";
-        s << ind << "for (int " << L.var << " = " << bound(L.lo, "max") << ", " << L.var << "_u = " << bound(L.hi, "min")
-          << "; " << L.var << " <= " << L.var << "_u; ++" << L.var << ") {\n";
+        s << ind << "if (lane == 0) {\n";
+        ++close;
+      }
+      for (size_t q = 0; q < o.loops.size(); ++q) {
+        const Loop& L = o.loops[q];
+        ind += "  ";
+        if (q == 0)
+          s << ind << "for (int " << L.var << " = " << bound(L.lo, "max") << " + lane, " << L.var << "_u = " << bound(L.hi, "min")
+            << "; " << L.var << " <= " << L.var << "_u; " << L.var << " += G) {\n";
+        else
+          s << ind << "for (int " << L.var << " = " << bound(L.lo, "max") << ", " << L.var << "_u = " << bound(L.hi, "min")
+            << "; " << L.var << " <= " << L.var << "_u; ++" << L.var << ") {\n";
         ++close;
       }
       std::vector<std::string> conds = o.arith;
@@ -839,35 +863,59 @@ bool generate(Model& m, std::string& src, std::string& err) {
       if (!conds.empty()) { s << ind << "if (" << join(conds, " && ") << ") {\n"; ++close; ind += "  "; }
       s << ind << "const answer_t _c = " << o.value << ";\n"
         << ind << "if (" << better("_c", "_c0") << (m.empty.empty() ? " || _r0 == -1" : "") << ") { _c0 = _c; _r0 = " << o.rule
-        << "; _p0 = " << (o.pos.size() > 0 ? o.pos[0] : "0") << "; _p1 = " << (o.pos.size() > 1 ? o.pos[1] : "0") << "; }\n";
+        << "; _p0 = " << (o.pos.size() > 0 ? o.pos[0] : "0") << "; _p1 = " << (o.pos.size() > 1 ? o.pos[1] : "0")
+        << "; _o0 = " << (o.loops.size() > 0 ? o.loops[0].var : "0") << "; _o1 = " << (o.loops.size() > 1 ? o.loops[1].var : "0")
+        << "; }\n";
       for (; close > 0; --close) {
         ind.resize(ind.size() - 2);
         s << ind << "}\n";
       }
     }
-    s << "  const long long x = idx_(z, i, j);\n"
-      << "  ((answer_t*)T.c[" << t << "])[x] = _c0;\n"
-      << "  T.r[" << t << "][x] = (short)_r0;\n";
-    for (int q = 0; q < T.cat; ++q) s << "  T.p[" << t << "][" << q << "][x] = _p" << q << ";\n";
+    s << "  for (int off = G / 2; off > 0; off >>= 1) {\n"
+         "    const answer_t c = shfl_xor_(_c0, off, G);\n"
+         "    const int r = __shfl_xor(_r0, off, G), p0 = __shfl_xor(_p0, off, G), p1 = __shfl_xor(_p1, off, G);\n"
+         "    const int o0 = __shfl_xor(_o0, off, G), o1 = __shfl_xor(_o1, off, G);\n"
+         "    bool take;\n"
+         "    if (r == -1) take = false;\n"
+         "    else if (_r0 == -1) take = true;\n"
+      << "    else if (" << better("c", "_c0") << ") take = true;\n"
+      << "    else if (" << better("_c0", "c") << ") take = false;\n"
+      << "    else take = r < _r0 || (r == _r0 && (o0 < _o0 || (o0 == _o0 && o1 < _o1)));\n"
+         "    if (take) { _c0 = c; _r0 = r; _p0 = p0; _p1 = p1; _o0 = o0; _o1 = o1; }\n"
+         "  }\n"
+         "  (void)_o0; (void)_o1;\n"
+         "  if (lane == 0) {\n"
+         "    const long long x = idx_(z, i, j);\n"
+      << "    ((answer_t*)T.c[" << t << "])[x] = _c0;\n"
+      << "    T.r[" << t << "][x] = (short)_r0;\n";
+    for (int q = 0; q < T.cat; ++q) s << "    T.p[" << t << "][" << q << "][x] = _p" << q << ";\n";
+    s << "  }\n";
+    if (used_tabs > 1)  // a later tabulation of this cell may read it from another lane of the wave
+      s << "  if (G > 1) { __builtin_amdgcn_s_waitcnt(0x0F70); __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, \"workgroup\"); }\n";
     s << "}\n";
   }
   // fill: one diagonal per launch (d0 == d1 - 1), or a run of small
   // diagonals by one workgroup with a barrier between them
-  s << "extern \"C\" __global__ void adp_fill(const input_t* _in1, const input_t* _in2, Dims z, Tabs T, int d0, int d1) {\n"
+  s << "template <int G> __device__ __forceinline__ void fill_(const input_t* _in1, const input_t* _in2, const Dims& z, "
+       "const Tabs& T, int d0, int d1) {\n"
+       "  const int tid = blockIdx.x * blockDim.x + threadIdx.x, lane = tid & (G - 1);\n"
        "  for (int d = d0; d < d1; ++d) {\n";
   if (m.tracks == 1)
     s << "    const int cnt = z.n1 + 1 - d;\n";
   else
     s << "    const int ilo = d > z.n2 ? d - z.n2 : 0, ihi = d < z.n1 ? d : z.n1, cnt = ihi - ilo + 1;\n";
-  s << "    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {\n";
+  s << "    for (int t = tid / G; t < cnt; t += gridDim.x * blockDim.x / G) {\n";
   if (m.tracks == 1)
     s << "      const int i = t, j = t + d;\n";
   else
     s << "      const int i = ilo + t, j = d - i;\n";
-  for (int t : m.order) s << "      tab_" << t << "(_in1, _in2, z, T, i, j);\n";
+  for (int t : m.order) s << "      tab_" << t << "<G>(_in1, _in2, z, T, i, j, lane);\n";
   s << "    }\n"
        "    if (d + 1 < d1) __syncthreads();\n"
        "  }\n}\n";
+  for (int G : {1, 8, 64})
+    s << "extern \"C\" __global__ void adp_fill_" << G << "(const input_t* _in1, const input_t* _in2, Dims z, Tabs T, int d0, int d1) "
+      << "{ fill_<" << G << ">(_in1, _in2, z, T, d0, d1); }\n";
   // finish: axiom cell (or best window), answer, backtrace (one thread)
   const Tab& AX = m.tabs[m.axiom];
   s << "extern \"C\" __global__ void adp_finish(const input_t* _in1, const input_t* _in2, Dims z, Tabs T, "
@@ -938,7 +986,7 @@ struct Grammar {
   std::string src;
   std::vector<char> code;
   hipModule_t mod = nullptr;
-  hipFunction_t fill = nullptr, finish = nullptr;
+  hipFunction_t fill[3] = {nullptr, nullptr, nullptr}, finish = nullptr;  // fill: 1, 8, 64 lanes per cell
   void* dev = nullptr;
   size_t dev_cap = 0;
   void* host = nullptr;  // pinned: answer, out[4], trace
@@ -979,7 +1027,9 @@ Grammar* compile(const char* spec, bool load, std::string& err) {
   hiprtcDestroyProgram(&prog);
   if (load) {
     if (hipModuleLoadData(&g->mod, g->code.data()) != hipSuccess ||
-        hipModuleGetFunction(&g->fill, g->mod, "adp_fill") != hipSuccess ||
+        hipModuleGetFunction(&g->fill[0], g->mod, "adp_fill_1") != hipSuccess ||
+        hipModuleGetFunction(&g->fill[1], g->mod, "adp_fill_8") != hipSuccess ||
+        hipModuleGetFunction(&g->fill[2], g->mod, "adp_fill_64") != hipSuccess ||
         hipModuleGetFunction(&g->finish, g->mod, "adp_finish") != hipSuccess) {
       err = "loading the compiled grammar failed";
       if (g->mod) (void)hipModuleUnload(g->mod);
@@ -1095,10 +1145,11 @@ int parse(Grammar* g, hipStream_t s, const void* in1, int n1, const void* in2, i
   const void* d_in1 = dev + o_in1;
   const void* d_in2 = dev + o_in2;
   if (!chk(hipEventRecord(g->ev[0], s), "event")) return DPX_EHIP;
-  // Diagonals: a diagonal of many cells gets its own launch (64 threads per
-  // workgroup, one cell each); when the cells have no split loop, runs of
-  // diagonals of at most 256 cells go to one workgroup that steps through
-  // them with a barrier in between.
+  // Diagonals: without split loops a diagonal of many cells gets its own
+  // launch (one lane per cell) and runs of diagonals of at most 256 cells go
+  // to one workgroup that steps through them with a barrier in between; with
+  // split loops every diagonal gets its own launch, with 1, 8 or 64 lanes per
+  // cell as the loops grow (about d split points on diagonal d).
   auto count = [&](int64_t d) -> int64_t {
     if (m.tracks == 1) return W1 - d;
     const int64_t lo = d > n2 ? d - n2 : 0, hi = d < n1 ? d : n1;
@@ -1107,17 +1158,20 @@ int parse(Grammar* g, hipStream_t s, const void* in1, int n1, const void* in2, i
   for (int64_t d = 0; d <= D;) {
     const int64_t c = count(d);
     int d1 = (int)d + 1;
-    unsigned grid, block = 64;
+    unsigned grid, block = 256;
+    int gi = 0;  // lanes per cell: 1, 8, 64
     if (!m.loops && c <= 256) {
       while (d1 <= D && count(d1) <= 256) ++d1;
       grid = 1;
-      block = 256;
     } else {
-      grid = (unsigned)((c + 63) / 64);
+      gi = !m.loops || d < 24 ? 0 : d < 192 ? 1 : 2;
+      const int64_t lanes = c * (gi == 0 ? 1 : gi == 1 ? 8 : 64);
+      block = gi == 0 || lanes < 256 ? 64 : 256;
+      grid = (unsigned)((lanes + block - 1) / block);
     }
     int d0 = (int)d;
     void* args[] = {(void*)&d_in1, (void*)&d_in2, (void*)&z, (void*)tabs_arg.data(), (void*)&d0, (void*)&d1};
-    if (!chk(hipModuleLaunchKernel(g->fill, grid, 1, 1, block, 1, 1, 0, s, args, nullptr), "fill launch")) return DPX_EHIP;
+    if (!chk(hipModuleLaunchKernel(g->fill[gi], grid, 1, 1, block, 1, 1, 0, s, args, nullptr), "fill launch")) return DPX_EHIP;
     d = d1;
   }
   if (!chk(hipEventRecord(g->ev[1], s), "event")) return DPX_EHIP;
