@@ -202,6 +202,8 @@ def single(args):
     ctx = dpx.Context()
     line = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "int32", "data": "synthetic", "unit": "GCUPS"}
+    if args.workload.startswith("grammar-"):
+        return single_grammar(args, ctx, line)
     if args.workload == "swat16384":
         a, b = gen.dna_pair(16384, 16384, 1)
         run = lambda: ctx.swat(a, b, 2, -3, -5, -2, trace_from=1)
@@ -257,10 +259,65 @@ def single(args):
     print(json.dumps(line))
 
 
+def single_grammar(args, ctx, line):
+    """A reference example grammar through the generic path (compiled to HIP
+    at run time), with the built-in kernel of the same grammar beside it."""
+    import random
+    import dpx.examples as G
+    from dpx import gen
+    w = args.workload
+    if w == "grammar-nussinov1024":
+        s = gen.rna_uniform(1024, 0)
+        g = G.nussinov()
+        run_in = (s,)
+        builtin = lambda: ctx.nussinov(s, min_loop=0)
+        n_cells = cells(1024)
+        line["config"] = {"workload": "Nussinov.scala grammar, n=1024, i.i.d. uniform ACGU seed 0, score + backtrace, "
+                                      "generic path (dpx_grammar_parse)"}
+    elif w == "grammar-chain1024":
+        rng = random.Random(1)
+        dims = [rng.randint(1, 100) for _ in range(1025)]
+        mats = [(dims[q], dims[q + 1]) for q in range(1024)]
+        g = G.matrix_chain()
+        run_in = (mats,)
+        builtin = lambda: ctx.matchain(dims[:-1], dims[1:])
+        n_cells = cells(1024)
+        line["config"] = {"workload": "MatrixMult2.scala grammar, 1024 matrices (dims uniform 1..100, seed 1), "
+                                      "cost + backtrace, generic path"}
+    else:  # grammar-swat4000
+        a, b = gen.dna_pair(4000, 4000, 1)
+        g = G.swat_affine(2, -3, -5, -2)
+        run_in = (a, b)
+        builtin = lambda: ctx.swat(a, b, 2, -3, -5, -2)
+        n_cells = 4001 * 4001
+        line["config"] = {"workload": "SWatAffine.scala grammar, 4000 x 4000 i.i.d. ACGT seed 1, +2/-3/-5/-2, "
+                                      "M(n1,n2) + backtrace, generic path"}
+    t0 = time.perf_counter()
+    cg = ctx.grammar(g)
+    compile_s = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        r = cg.parse(*run_in)
+    fills = []
+    for _ in range(args.steps):
+        r = cg.parse(*run_in)
+        fills.append(ctx.last_timing()[0])
+        print("bench: step fill=%.3fms" % fills[-1], file=sys.stderr, flush=True)
+    builtin()
+    bres = builtin()
+    bfill = sum(ctx.last_timing())
+    ms = sum(fills) / len(fills)
+    same = (r[0] == bres[0]) and [t[:3] for t in r[2]] == [t[:3] for t in bres[-1]]
+    line.update({"metric": "GCUPS (%s)" % w, "value": round(n_cells / (ms * 1e-3) / 1e9, 4), "ms_per_step": round(ms, 4),
+                 "fill_ms": round(ms, 4), "cells": n_cells, "compile_s": round(compile_s, 3),
+                 "builtin_ms": round(bfill, 4), "same_as_builtin": bool(same)})
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="batch", choices=["batch", "zuker2048", "rnafold2048", "nussinov1024", "swat16384"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "zuker2048", "rnafold2048", "nussinov1024", "swat16384",
+                                                            "grammar-nussinov1024", "grammar-chain1024", "grammar-swat4000"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)

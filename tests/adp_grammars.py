@@ -1,116 +1,6 @@
-"""The reference's example grammars (src/main/scala/v4/examples) written with
-dpx.adp, each with the C bodies of its CodeGen algebra and Python callables
-of the same meaning.  Shared by the oracle tests and the GPU tests."""
-import dpx.adp as A
-
-
-def nussinov():
-    """Nussinov.scala:8-73 (Nussinov78 grammar, NussinovAlgebra), the
-    grammar behind bench2/nu.c; uppercase ACGU input."""
-    g = A.Grammar(tracks=1, h="max")
-    pairs = {("A", "U"), ("U", "A"), ("G", "U"), ("U", "G"), ("C", "G"), ("G", "C")}
-    nil = A.cfun(lambda: 0, "", "return 0;")
-    left = A.cfun(lambda c, a: a, "c,a", "return a;")
-    right = A.cfun(lambda a, c: a, "a,c", "return a;")
-    pair = A.cfun(lambda l, a, r: a + 1, "l,a,r", "return a+1;")
-    split = A.cfun(lambda l, r: l + r, "l,r", "return l+r;")
-    bp = A.cfun(lambda i, j: i + 2 <= j and (chr(g.in1(i)), chr(g.in1(j - 1))) in pairs, "i,j",
-                "if (i+2>j) return false; char a=_in1[i],b=_in1[j-1]; "
-                "return (a=='A'&&b=='U') || (a=='U'&&b=='A') || (a=='G'&&b=='U') || (a=='U'&&b=='G') || "
-                "(a=='C'&&b=='G') || (a=='G'&&b=='C');")
-    s = g.tabulate("s", lambda: ((g.empty ^ nil) | (g.el * s ^ left) | (s * g.el ^ right) |
-                                 ((g.el * s * g.el).filter(bp) ^ pair) | (s * s ^ split)).aggregate("max"),
-                   always_valid=True)
-    g.axiom = s
-    return g
-
-
-def elmamun(h):
-    """ElMamun.scala: the bill grammar with the buyer (min) or seller (max) algebra."""
-    g = A.Grammar(tracks=1, h=h)
-    add = A.cfun(lambda l, c, r: l + r, "l,c,r", "return l + r;")
-    mul = A.cfun(lambda l, c, r: l * r, "l,c,r", "return l * r;")
-    b = g.tabulate("M", lambda: (g.digit | (b * g.charf("+") * b ^ add) | (b * g.charf("*") * b ^ mul)).aggregate(h))
-    g.axiom = b
-    return g
-
-
-def brackets(window=8):
-    """Brackets.scala (window 8): digits inside correctly nested brackets, summed."""
-    g = A.Grammar(tracks=1, h="max", window=window)
-    are = A.cfun(lambda i, j: j > i + 1 and g.in1(i) == 40 and g.in1(j - 1) == 41, "i,j",
-                 "return j > i+1 && _in1[i] == '(' && _in1[j-1] == ')';")
-    mid = A.cfun(lambda c1, i, c2: i, "c1,i,c2", "return i;")
-    add = A.cfun(lambda x, y: x + y, "x,y", "return x+y;")
-    zero = A.cfun(lambda: 0, "", "return 0;")
-    m = g.tabulate("M", lambda: (g.digit | ((g.char * m * g.char).filter(are) ^ mid) | (m * m ^ add)).aggregate("max"))
-    g.tabulate("N", lambda: ((g.empty ^ zero) | ((g.char * g.tabs["N"] * g.char).filter(are) ^ mid) |
-                             (g.tabs["N"] * g.tabs["N"] ^ add)).aggregate("max"))  # unused: stripped like the reference
-    g.axiom = m
-    return g
-
-
-def seqalign(algebra="sw"):
-    """SeqAlign.scala: arbitrary gap cost, O(n^3) two-track grammar; the
-    Smith-Waterman (+10/-3, open -3, extend -1, clamped at 0) or the
-    Needleman-Wunsch (+4/-3, open -15, extend -1) algebra."""
-    g = A.Grammar(tracks=2, h="max")
-    if algebra == "sw":
-        op, ex, ma, mi, clamp = -3, -1, 10, -3, True
-    else:
-        op, ex, ma, mi, clamp = -15, -1, 4, -3, False
-    cl = (lambda v: max(0, v)) if clamp else (lambda v: v)
-    cc = "return s>0?s:0;" if clamp else "return s;"
-    start = A.cfun(lambda: 0, "", "return 0;")
-    gap1 = A.cfun(lambda s, a: cl(a + op + ex * (s[1] - s[0] - 1)), "g,a",
-                  "int s=a+(%d)+(%d)*(g._2-g._1-1); %s" % (op, ex, cc))
-    gap2 = A.cfun(lambda a, s: cl(a + op + ex * (s[1] - s[0] - 1)), "a,g",
-                  "int s=a+(%d)+(%d)*(g._2-g._1-1); %s" % (op, ex, cc))
-    pair = A.cfun(lambda c1, a, c2: a + (ma if c1 == c2 else mi), "c1,a,c2", "return a+(c1==c2?%d:%d);" % (ma, mi))
-    m = g.tabulate("M", lambda: ((g.empty ^ start) | (g.seq() << m ^ gap1) | (m >> g.seq() ^ gap2) |
-                                 (g.el1 << m >> g.el2 ^ pair)).aggregate("max"), always_valid=True)
-    g.axiom = m
-    return g
-
-
-def swat_affine(match=2, mismatch=-3, gap_open=-5, gap_ext=-2):
-    """SWatAffine.scala with the scores as parameters (the C bodies of
-    SWatAlgebra with open/ext/match/mismatch substituted): the grammar of
-    bench/dp_swat/comp1.cu."""
-    g = A.Grammar(tracks=2, h="max")
-    start = A.cfun(lambda: 0, "", "return 0;")
-    z = A.cfun(lambda p: 0, "p", "return 0;")
-    o1 = A.cfun(lambda c, a: max(0, a + gap_open), "c1,a", "return a+(%d)>0 ? a+(%d) : 0;" % (gap_open, gap_open))
-    o2 = A.cfun(lambda a, c: max(0, a + gap_open), "a,c2", "return a+(%d)>0 ? a+(%d) : 0;" % (gap_open, gap_open))
-    e1 = A.cfun(lambda c, a: max(0, a + gap_ext), "c1,a", "return a+(%d)>0 ? a+(%d) : 0;" % (gap_ext, gap_ext))
-    e2 = A.cfun(lambda a, c: max(0, a + gap_ext), "a,c2", "return a+(%d)>0 ? a+(%d) : 0;" % (gap_ext, gap_ext))
-    pair = A.cfun(lambda c1, a, c2: a + (match if c1 == c2 else mismatch), "c1,a,c2",
-                  "return a + (c1==c2 ? %d : %d);" % (match, mismatch))
-    M = g.tabulate("M", lambda: ((g.empty ^ start) | (g.el1 << M >> g.el2 ^ pair) | g1 | g2).aggregate("max"),
-                   always_valid=True)
-    g1 = g.tabulate("g1", lambda: ((g.empty1 ^ z) | (g.el1 << M ^ o1) | (g.el1 << g1 ^ e1)).aggregate("max"),
-                    always_valid=True)
-    g2 = g.tabulate("g2", lambda: ((g.empty2 ^ z) | (M >> g.el2 ^ o2) | (g2 >> g.el2 ^ e2)).aggregate("max"),
-                    always_valid=True)
-    g.axiom = M
-    return g
-
-
-def matrix_chain():
-    """MatrixMult2.scala:10-27: matrices (rows, cols) as T2ii, cost min."""
-    g = A.Grammar(tracks=1, alphabet="T2ii", alphabet_fmt="ii", h="min")
-    single = A.cfun(lambda m: 0, "i", "return 0;")
-    mult = A.cfun(lambda i, l, k, r, j: _wrap(l + r + g.in1(i)[0] * g.in1(k)[0] * g.in1(j - 1)[1]), "i,l,k,r,j",
-                  "return l+r + _in1[i]._1*_in1[k]._1*_in1[j-1]._2;")
-    m = g.tabulate("M", lambda: ((g.el ^ single) | (g.emptyi * m * g.emptyi * m * g.emptyi ^ mult)).aggregate("min"),
-                   always_valid=True)
-    g.axiom = m
-    return g
-
-
-def _wrap(v):
-    """int32 arithmetic as on the JVM and on the device (-fwrapv)."""
-    return (v + (1 << 31)) % (1 << 32) - (1 << 31)
+"""Example grammars (dpx.examples) plus the oracle helpers of the tests."""
+from dpx.examples import *  # noqa: F401,F403
+from dpx.examples import _wrap  # noqa: F401
 
 
 def oracle(g):
