@@ -698,6 +698,11 @@ struct Gen {
             o.arith.push_back(add(a, jL + iL) + "<=" + b);
             if (iU != INF) o.arith.push_back(add(b, -jL - iU) + "<=" + a);
             k = add(b, -jL);
+          } else if (!bt) {
+            // a side of fixed yield whose hint excludes it (lower bound above
+            // upper): the reference's loop has an empty range
+            o.arith.push_back("0==1");
+            k = add(a, iL);
           } else {
             var = "k" + std::to_string(nvar++);
             Loop L;

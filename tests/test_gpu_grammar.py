@@ -138,3 +138,30 @@ def test_trace_capacity_and_errors(ctx):
 def ctypes_ptr():
     import ctypes
     return ctypes.byref(ctypes.c_void_p())
+
+
+def test_random_grammars_vs_oracle(ctx):
+    """Seeded random grammars (every construct the library accepts) on
+    random inputs: answer, found flag and full trace equal the oracle's."""
+    import adp_random as R
+    import dpx
+    done = 0
+    for seed in range(48):
+        g = R.single_track(seed) if seed % 3 else R.two_track(seed)
+        try:
+            G.oracle(g)
+        except ValueError:
+            with pytest.raises(dpx.DpxError):
+                ctx.grammar(g)
+            continue
+        print("random grammar seed %d" % seed, flush=True)
+        cg = ctx.grammar(g)
+        rng = random.Random(1000 + seed)
+        for _ in range(4):
+            if g.tracks == 1:
+                _vs_oracle(cg, g, R.text(rng, rng.randint(0, 24)))
+            else:
+                _vs_oracle(cg, g, R.text(rng, rng.randint(0, 12)), R.text(rng, rng.randint(0, 12)))
+        cg.close()
+        done += 1
+    assert done > 30

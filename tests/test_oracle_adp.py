@@ -160,3 +160,63 @@ def test_library_rejects_unsupported():
            "tabulations": {"A": {"p": {"tab": "B"}}, "B": {"p": {"tab": "A"}}}}
     rc, _, msg = _check(json.dumps(cyc))
     assert rc != 0 and "loop" in msg
+
+
+def test_random_grammars_oracle_vs_brute_force():
+    """Seeded random grammars (every construct) against brute force and
+    trace replay; grammars the analysis rejects (same-cell cycles) skipped."""
+    import adp_random as R
+    checked = 0
+    for seed in range(60):
+        g = R.single_track(seed) if seed % 3 else R.two_track(seed)
+        try:
+            orc = G.oracle(g)
+        except ValueError:
+            continue
+        if g.tracks == 2 and any(t.always_valid for t in g.tabs.values()):
+            continue  # an always_valid table reads 0 from cells without a derivation: no brute-force meaning
+        if isinstance(g.h, tuple):
+            continue  # maxBy a non-monotone key: the DP answer is not the best derivation (checked on the GPU only)
+        rng = random.Random(seed)
+        for _ in range(3):
+            if g.tracks == 1:
+                in1, in2 = R.text(rng, rng.randint(0, 6)), None
+            else:
+                in1, in2 = R.text(rng, rng.randint(0, 3)), R.text(rng, rng.randint(0, 3))
+            _, (ans, found, tr) = G.run_oracle(g, in1, in2)
+            try:
+                vals = orc.enumerate_all(g.py_input(in1), g.py_input(in2) if in2 is not None else None, limit=50000)
+            except OverflowError:
+                continue
+            if g.empty_value is not None:
+                ev = int(g.empty_value)
+                vals = [v for v in vals if v != ev]
+            assert found == bool(vals), (seed, in1, in2)
+            if vals:
+                key = orc._key
+                want = max(vals, key=key) if (g.h in ("max",) or isinstance(g.h, tuple)) else min(vals, key=key)
+                assert key(ans) == key(want), (seed, in1, in2, ans, want)
+                assert key(orc.replay(g.py_input(in1), g.py_input(in2) if in2 is not None else None, tr)) == key(ans)
+            checked += 1
+    assert checked > 60
+
+
+def test_random_grammars_compile():
+    import adp_random as R
+    for seed in range(0, 60, 7):
+        g = R.single_track(seed) if seed % 3 else R.two_track(seed)
+        rc, src, msg = _check(g.spec_json())
+        assert rc == 0 or "loop" in msg, (seed, msg[:2000])
+
+
+def test_random_grammars_compile_all():
+    """Every random grammar the oracle accepts generates and compiles."""
+    import adp_random as R
+    for seed in range(48):
+        g = R.single_track(seed) if seed % 3 else R.two_track(seed)
+        try:
+            G.oracle(g)
+        except ValueError:
+            continue
+        rc, src, msg = _check(g.spec_json())
+        assert rc == 0, (seed, msg[:2000])
