@@ -274,6 +274,14 @@ def single_grammar(args, ctx, line):
         n_cells = cells(1024)
         line["config"] = {"workload": "Nussinov.scala grammar, n=1024, i.i.d. uniform ACGU seed 0, score + backtrace, "
                                       "generic path (dpx_grammar_parse)"}
+    elif w == "grammar-nussinov-batch":
+        seqs = gen.rna_batch(1024, 50, 500, 5, gc=0.5)
+        g = G.nussinov()
+        run_in = None
+        builtin = lambda: ctx.nussinov_batch(seqs, min_loop=0)
+        n_cells = sum(cells(len(x)) for x in seqs)
+        line["config"] = {"workload": "Nussinov.scala grammar, batch of 1024 sequences, lengths log-uniform [50,500] "
+                                      "(seed 5), scores + backtraces, generic path (dpx_grammar_parse_batch)"}
     elif w == "grammar-chain1024":
         rng = random.Random(1)
         dims = [rng.randint(1, 100) for _ in range(1025)]
@@ -295,18 +303,22 @@ def single_grammar(args, ctx, line):
     t0 = time.perf_counter()
     cg = ctx.grammar(g)
     compile_s = time.perf_counter() - t0
+    run = (lambda: cg.parse_batch(seqs)) if run_in is None else (lambda: cg.parse(*run_in))
     for _ in range(args.warmup):
-        r = cg.parse(*run_in)
+        r = run()
     fills = []
     for _ in range(args.steps):
-        r = cg.parse(*run_in)
+        r = run()
         fills.append(ctx.last_timing()[0])
         print("bench: step fill=%.3fms" % fills[-1], file=sys.stderr, flush=True)
     builtin()
     bres = builtin()
     bfill = sum(ctx.last_timing())
     ms = sum(fills) / len(fills)
-    same = (r[0] == bres[0]) and [t[:3] for t in r[2]] == [t[:3] for t in bres[-1]]
+    if run_in is None:
+        same = [x[0] for x in r] == list(bres[0]) and [x[2] for x in r] == list(bres[1])
+    else:
+        same = (r[0] == bres[0]) and [t[:3] for t in r[2]] == [t[:3] for t in bres[-1]]
     line.update({"metric": "GCUPS (%s)" % w, "value": round(n_cells / (ms * 1e-3) / 1e9, 4), "ms_per_step": round(ms, 4),
                  "fill_ms": round(ms, 4), "cells": n_cells, "compile_s": round(compile_s, 3),
                  "builtin_ms": round(bfill, 4), "same_as_builtin": bool(same)})
@@ -317,7 +329,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="batch", choices=["batch", "zuker2048", "rnafold2048", "nussinov1024", "swat16384",
-                                                            "grammar-nussinov1024", "grammar-chain1024", "grammar-swat4000"])
+                                                            "grammar-nussinov1024", "grammar-chain1024", "grammar-swat4000",
+                                                            "grammar-nussinov-batch"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)

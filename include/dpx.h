@@ -298,6 +298,15 @@ const char* dpx_grammar_source(const dpx_grammar* g);
  * order; DPX_ETRACE when trace_cap is too small. */
 int dpx_grammar_parse(dpx_ctx* ctx, dpx_grammar* g, const void* in1, int n1, const void* in2, int n2,
                       void* answer, int* found, dpx_trace_t* trace, int trace_cap, int* trace_len);
+/* A batch: instance q reads in1 elements [offs1[q], offs1[q+1]) (and in2 by
+ * offs2 for two tracks), answers + q * answer_bytes receives its answer,
+ * found[q] its flag; with traces != NULL its trace goes to traces +
+ * trace_offs[q] (capacity trace_offs[q+1] - trace_offs[q]), length in
+ * trace_lens[q].  One launch per diagonal covers every instance still on it
+ * (largest first).  No reference counterpart (one instance per call there). */
+int dpx_grammar_parse_batch(dpx_ctx* ctx, dpx_grammar* g, const void* in1, const int64_t* offs1, const void* in2,
+                            const int64_t* offs2, int count, void* answers, int32_t* found, dpx_trace_t* traces,
+                            const int64_t* trace_offs, int32_t* trace_lens);
 
 #ifdef __cplusplus
 }

@@ -794,7 +794,11 @@ bool generate(Model& m, std::string& src, std::string& err) {
     << "static_assert(sizeof(answer_t) == " << m.answer_bytes << ", \"answer size\");\n"
     << "struct Dims { int n1, n2; };\n"
     << "struct Tabs { void* c[" << NT << "]; short* r[" << NT << "]; int* p[" << NT << "][2]; };\n"
-    << "struct Rec { int i, j, rule, pos[2]; };\n";
+    << "struct Rec { int i, j, rule, pos[2]; };\n"
+    << "// one instance of a batch: inputs, outputs and tables (host writes this layout)\n"
+    << "struct Inst { const input_t* in1; const input_t* in2; answer_t* ans; int* out; Rec* tr; Dims z; int cap, pad; Tabs T; };\n"
+    << "static_assert(__builtin_offsetof(Inst, z) == 40 && __builtin_offsetof(Inst, cap) == 48 && "
+       "__builtin_offsetof(Inst, T) == 56 && sizeof(Inst) == 56 + " << 32 * NT << ", \"instance layout\");\n";
   if (m.tracks == 1)
     s << "__device__ __forceinline__ long long idx_(const Dims& z, int i, int j) {\n"
          "  const long long d = j - i;  // diagonal-major: the cells of one diagonal are contiguous\n"
@@ -901,12 +905,14 @@ bool generate(Model& m, std::string& src, std::string& err) {
   }
   // fill: one diagonal per launch (d0 == d1 - 1), or a run of small
   // diagonals by one workgroup with a barrier between them
-  s << "template <int G> __device__ __forceinline__ void fill_(const input_t* _in1, const input_t* _in2, const Dims& z, "
-       "const Tabs& T, int d0, int d1) {\n"
+  s << "template <int G> __device__ __forceinline__ void fill_(const Inst* __restrict__ I, int d0, int d1) {\n"
+       "  const Inst& q = I[blockIdx.y];  // instances sorted by size, largest first: blockIdx.y < active ones\n"
+       "  const input_t* _in1 = q.in1;\n  const input_t* _in2 = q.in2;\n  const Dims z = q.z;\n  const Tabs& T = q.T;\n"
        "  const int tid = blockIdx.x * blockDim.x + threadIdx.x, lane = tid & (G - 1);\n"
        "  for (int d = d0; d < d1; ++d) {\n";
   if (m.tracks == 1)
-    s << "    const int cnt = z.n1 + 1 - d;\n";
+    s << "    const int D = " << (m.window > 0 ? "min(" + std::to_string(m.window) + ", z.n1)" : std::string("z.n1")) << ";\n"
+      << "    const int cnt = d <= D ? z.n1 + 1 - d : 0;\n";
   else
     s << "    const int ilo = d > z.n2 ? d - z.n2 : 0, ihi = d < z.n1 ? d : z.n1, cnt = ihi - ilo + 1;\n";
   s << "    for (int t = tid / G; t < cnt; t += gridDim.x * blockDim.x / G) {\n";
@@ -919,12 +925,14 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "    if (d + 1 < d1) __syncthreads();\n"
        "  }\n}\n";
   for (int G : {1, 8, 64})
-    s << "extern \"C\" __global__ void adp_fill_" << G << "(const input_t* _in1, const input_t* _in2, Dims z, Tabs T, int d0, int d1) "
-      << "{ fill_<" << G << ">(_in1, _in2, z, T, d0, d1); }\n";
-  // finish: axiom cell (or best window), answer, backtrace (one thread)
+    s << "extern \"C\" __global__ void adp_fill_" << G << "(const Inst* I, int d0, int d1) { fill_<" << G << ">(I, d0, d1); }\n";
+  // finish: axiom cell (or best window), answer, backtrace; one thread per instance
   const Tab& AX = m.tabs[m.axiom];
-  s << "extern \"C\" __global__ void adp_finish(const input_t* _in1, const input_t* _in2, Dims z, Tabs T, "
-       "answer_t* ans, int* out, Rec* tr, int cap) {\n"
+  s << "extern \"C\" __global__ void adp_finish(const Inst* I) {\n"
+       "  const Inst& q = I[blockIdx.x];\n"
+       "  const Dims z = q.z;\n  const Tabs& T = q.T;\n  const input_t* _in1 = q.in1;\n  const input_t* _in2 = q.in2;\n"
+       "  (void)_in1; (void)_in2;\n"
+       "  answer_t* ans = q.ans;\n  int* out = q.out;\n  Rec* tr = q.tr;\n  const int cap = q.cap;\n"
        "  int i0, j0;\n  bool found;\n";
   if (m.window > 0) {
     s << "  found = false; i0 = 0; j0 = " << m.window << ";\n"
@@ -934,7 +942,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
     s << "    const answer_t c = " << Gen(m).cost(m.axiom, "i", "j") << ";\n"
       << "    if (!found || " << better("c", "best") << ") { best = c; i0 = i; j0 = j; found = true; }\n"
       << "  }\n"
-      << "  if (found) *ans = best;\n";
+      << "  *ans = best;  // zero when no window cell parses\n";
   } else {
     if (m.tracks == 1) s << "  i0 = 0; j0 = z.n1;\n";
     else s << "  i0 = z.n1; j0 = z.n2;\n";
@@ -1081,35 +1089,45 @@ namespace {
 size_t r256(size_t x) { return (x + 255) & ~(size_t)255; }
 }  // namespace
 
-int parse(Grammar* g, hipStream_t s, const void* in1, int n1, const void* in2, int n2, void* answer, int* found,
-          dpx_trace_t* tr, int cap, int* len, float* fill_ms, std::string& err) {
+int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int64_t* off1, const char* in2,
+                const int64_t* off2, void* answers, int32_t* found, dpx_trace_t* tr, const int64_t* tr_off,
+                int32_t* tr_len, float* fill_ms, std::string& err) {
   const Model& m = g->m;
   if (!g->mod) { err = "grammar was compiled without loading"; return DPX_EINVAL; }
-  if (n1 < 0 || (m.tracks == 2 && n2 < 0) || (n1 > 0 && !in1) || (m.tracks == 2 && n2 > 0 && !in2)) {
-    err = "bad input";
-    return DPX_EINVAL;
-  }
-  if (m.tracks == 1) n2 = 0;
-  // cells: single track, diagonals 0..D of the (n1+1)-wide triangle
-  const int64_t W1 = (int64_t)n1 + 1;
-  const int64_t D = m.tracks == 1 ? (m.window > 0 ? std::min<int64_t>(m.window, n1) : n1) : (int64_t)n1 + n2;
-  const int64_t cells = m.tracks == 1 ? (D + 1) * W1 - (D + 1) * D / 2 : W1 * ((int64_t)n2 + 1);
-  if (cells > ((int64_t)1 << 34)) { err = "grammar tables too large"; return DPX_EINVAL; }
+  if (count <= 0 || !off1 || (m.tracks == 2 && !off2) || (tr && !tr_off)) { err = "bad batch arguments"; return DPX_EINVAL; }
   const int NT = (int)m.tabs.size();
-  // device block: inputs, then per tabulation cost | rule | positions, then out and trace
-  size_t off = 0;
-  const size_t o_in1 = off; off += r256((size_t)n1 * m.alphabet_bytes + 16);
-  const size_t o_in2 = off; off += r256((size_t)n2 * m.alphabet_bytes + 16);
-  std::vector<size_t> oc(NT), orr(NT), op(NT);
-  for (int t = 0; t < NT; ++t) {
-    if (!m.tabs[t].used) continue;
-    oc[t] = off; off += r256((size_t)cells * m.answer_bytes);
-    orr[t] = off; off += r256((size_t)cells * 2);
-    op[t] = off; off += r256((size_t)cells * 4 * m.tabs[t].cat);
+  const size_t ab = (size_t)m.alphabet_bytes, ans4 = ((size_t)m.answer_bytes + 3) & ~(size_t)3;
+  const size_t inst_bytes = 56 + 32 * (size_t)NT;
+  struct Q { int64_t n1, n2, D, cells; size_t o_in1, o_in2, o_tab, o_ans, o_tr; int64_t cap; };
+  std::vector<Q> qs(count);
+  size_t off = r256(inst_bytes * count);
+  int64_t n1max = 0, cmax = 0, Dmax = 0;
+  for (int q = 0; q < count; ++q) {
+    Q& x = qs[q];
+    x.n1 = off1[q + 1] - off1[q];
+    x.n2 = m.tracks == 2 ? off2[q + 1] - off2[q] : 0;
+    if (x.n1 < 0 || x.n2 < 0 || x.n1 > (1 << 30) || x.n2 > (1 << 30)) { err = "bad input offsets"; return DPX_EINVAL; }
+    if (x.n1 > 0 && !in1) { err = "null input"; return DPX_EINVAL; }
+    if (x.n2 > 0 && !in2) { err = "null input"; return DPX_EINVAL; }
+    const int64_t W1 = x.n1 + 1;
+    x.D = m.tracks == 1 ? (m.window > 0 ? std::min<int64_t>(m.window, x.n1) : x.n1) : x.n1 + x.n2;
+    x.cells = m.tracks == 1 ? (x.D + 1) * W1 - (x.D + 1) * x.D / 2 : W1 * (x.n2 + 1);
+    if (x.cells > ((int64_t)1 << 34)) { err = "grammar tables too large"; return DPX_EINVAL; }
+    x.o_in1 = off; off += r256((size_t)x.n1 * ab + 16);
+    x.o_in2 = off; off += r256((size_t)x.n2 * ab + 16);
+    x.o_tab = off;
+    for (int t = 0; t < NT; ++t) {
+      if (!m.tabs[t].used) continue;
+      off += r256((size_t)x.cells * m.answer_bytes) + r256((size_t)x.cells * 2) + r256((size_t)x.cells * 4 * m.tabs[t].cat);
+    }
+    x.o_ans = off; off += r256(ans4 + 16);
+    x.cap = tr ? tr_off[q + 1] - tr_off[q] : 0;
+    if (x.cap < 0) { err = "bad trace offsets"; return DPX_EINVAL; }
+    x.o_tr = off; off += r256(sizeof(dpx_trace_t) * (size_t)std::max<int64_t>(x.cap, 1));
+    n1max = std::max(n1max, x.n1);
+    cmax = std::max(cmax, m.tracks == 1 ? x.n1 + 1 : std::min(x.n1, x.n2) + 1);
+    Dmax = std::max(Dmax, x.D);
   }
-  const size_t ans4 = ((size_t)m.answer_bytes + 3) & ~(size_t)3;  // answer, then out[4]
-  const size_t o_ans = off; off += r256(ans4 + 16);
-  const size_t o_tr = off; off += r256(sizeof(dpx_trace_t) * (size_t)(cap > 0 ? cap : 1));
   if (off > g->dev_cap) {
     if (g->dev) (void)hipFree(g->dev);
     g->dev = nullptr;
@@ -1117,7 +1135,15 @@ int parse(Grammar* g, hipStream_t s, const void* in1, int n1, const void* in2, i
     if (hipMalloc(&g->dev, off) != hipSuccess) { err = "device allocation for the grammar tables failed"; return DPX_ENOMEM; }
     g->dev_cap = off;
   }
-  const size_t hbytes = 256 + (size_t)m.answer_bytes + sizeof(dpx_trace_t) * (size_t)(cap > 0 ? cap : 1);
+  // host staging: instance records (largest D first), inputs, then results
+  std::vector<int> order(count);
+  for (int q = 0; q < count; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return qs[a].D > qs[b].D; });
+  int64_t tr_total = 0;
+  for (auto& x : qs) tr_total += x.cap;
+  const size_t in_bytes = (size_t)(off1[count] - off1[0]) * ab + (m.tracks == 2 ? (size_t)(off2[count] - off2[0]) * ab : 0);
+  const size_t res_bytes = (size_t)count * (ans4 + 16);
+  const size_t hbytes = r256(inst_bytes * count) + r256(in_bytes + 16) + r256(res_bytes) + sizeof(dpx_trace_t) * (size_t)std::max<int64_t>(tr_total, 1);
   if (hbytes > g->host_cap) {
     if (g->host) (void)hipHostFree(g->host);
     g->host = nullptr;
@@ -1126,83 +1152,134 @@ int parse(Grammar* g, hipStream_t s, const void* in1, int n1, const void* in2, i
     g->host_cap = hbytes;
   }
   char* dev = (char*)g->dev;
+  char* h = (char*)g->host;
+  char* h_inst = h;
+  char* h_res = h + r256(inst_bytes * count) + r256(in_bytes + 16);
+  char* h_tr = h_res + r256(res_bytes);
   auto chk = [&](hipError_t e, const char* what) {
     if (e != hipSuccess) err = std::string(what) + ": " + hipGetErrorString(e);
     return e == hipSuccess;
   };
-  if (n1 > 0 && !chk(hipMemcpyAsync(dev + o_in1, in1, (size_t)n1 * m.alphabet_bytes, hipMemcpyHostToDevice, s), "input copy")) return DPX_EHIP;
-  if (n2 > 0 && !chk(hipMemcpyAsync(dev + o_in2, in2, (size_t)n2 * m.alphabet_bytes, hipMemcpyHostToDevice, s), "input copy")) return DPX_EHIP;
-  struct Dims { int n1, n2; } z{n1, n2};
-  // the kernel's Tabs {c[NT], r[NT], p[NT][2]}: 4 NT pointers
-  std::vector<char> tabs_arg(sizeof(void*) * NT * 4, 0);
-  {
-    void** c = (void**)tabs_arg.data();
-    void** r = c + NT;
-    void** p = r + NT;
+  for (int k = 0; k < count; ++k) {
+    const int q = order[k];
+    const Q& x = qs[q];
+    char* rec = h_inst + inst_bytes * k;
+    std::memset(rec, 0, inst_bytes);
+    auto put = [&](size_t at, const void* v, size_t n) { std::memcpy(rec + at, v, n); };
+    void* p;
+    p = dev + x.o_in1; put(0, &p, 8);
+    p = dev + x.o_in2; put(8, &p, 8);
+    p = dev + x.o_ans; put(16, &p, 8);
+    p = dev + x.o_ans + ans4; put(24, &p, 8);
+    p = x.cap > 0 ? dev + x.o_tr : nullptr; put(32, &p, 8);
+    const int dims[4] = {(int)x.n1, (int)x.n2, (int)x.cap, 0};
+    put(40, dims, 16);
+    size_t o = x.o_tab;
     for (int t = 0; t < NT; ++t) {
       if (!m.tabs[t].used) continue;
-      c[t] = dev + oc[t];
-      r[t] = dev + orr[t];
-      p[2 * t] = m.tabs[t].cat > 0 ? dev + op[t] : nullptr;
-      p[2 * t + 1] = m.tabs[t].cat > 1 ? dev + op[t] + (size_t)cells * 4 : nullptr;
+      void* c = dev + o; o += r256((size_t)x.cells * m.answer_bytes);
+      void* r = dev + o; o += r256((size_t)x.cells * 2);
+      void* p0 = m.tabs[t].cat > 0 ? dev + o : nullptr;
+      void* p1 = m.tabs[t].cat > 1 ? dev + o + (size_t)x.cells * 4 : nullptr;
+      o += r256((size_t)x.cells * 4 * m.tabs[t].cat);
+      put(56 + 8 * (size_t)t, &c, 8);
+      put(56 + 8 * ((size_t)NT + t), &r, 8);
+      put(56 + 8 * (2 * (size_t)NT + 2 * t), &p0, 8);
+      put(56 + 8 * (2 * (size_t)NT + 2 * t + 1), &p1, 8);
     }
+    if (x.n1 > 0 && !chk(hipMemcpyAsync(dev + x.o_in1, in1 + off1[q] * ab, (size_t)x.n1 * ab, hipMemcpyHostToDevice, s), "input copy")) return DPX_EHIP;
+    if (x.n2 > 0 && !chk(hipMemcpyAsync(dev + x.o_in2, in2 + off2[q] * ab, (size_t)x.n2 * ab, hipMemcpyHostToDevice, s), "input copy")) return DPX_EHIP;
   }
-  const void* d_in1 = dev + o_in1;
-  const void* d_in2 = dev + o_in2;
+  if (!chk(hipMemcpyAsync(dev, h_inst, inst_bytes * count, hipMemcpyHostToDevice, s), "instance copy")) return DPX_EHIP;
+  const void* d_inst = dev;
   if (!chk(hipEventRecord(g->ev[0], s), "event")) return DPX_EHIP;
   // Diagonals: without split loops a diagonal of many cells gets its own
   // launch (one lane per cell) and runs of diagonals of at most 256 cells go
-  // to one workgroup that steps through them with a barrier in between; with
-  // split loops every diagonal gets its own launch, with 1, 8 or 64 lanes per
-  // cell as the loops grow (about d split points on diagonal d).
-  auto count = [&](int64_t d) -> int64_t {
-    if (m.tracks == 1) return W1 - d;
-    const int64_t lo = d > n2 ? d - n2 : 0, hi = d < n1 ? d : n1;
-    return hi - lo + 1;
+  // to one workgroup per instance that steps through them with a barrier in
+  // between; with split loops every diagonal gets its own launch, with 1, 8
+  // or 64 lanes per cell as the loops grow (about d split points on diagonal
+  // d).  blockIdx.y picks the instance: those still active on diagonal d are
+  // the first ones of the sorted records.
+  // cells of diagonal d in the largest instance (a bound for every instance)
+  auto cnt_bound = [&](int64_t d) -> int64_t {
+    if (m.tracks == 1) return n1max + 1 - d;
+    return std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(cmax, d + 1), Dmax - d + 1));
   };
-  for (int64_t d = 0; d <= D;) {
-    const int64_t c = count(d);
+  int active = count;
+  for (int64_t d = 0; d <= Dmax;) {
+    while (active > 0 && qs[order[active - 1]].D < d) --active;
+    const int64_t c = cnt_bound(d);
     int d1 = (int)d + 1;
     unsigned grid, block = 256;
     int gi = 0;  // lanes per cell: 1, 8, 64
     if (!m.loops && c <= 256) {
-      while (d1 <= D && count(d1) <= 256) ++d1;
+      while (d1 <= Dmax && cnt_bound(d1) <= 256) ++d1;
       grid = 1;
     } else {
-      gi = !m.loops || d < 24 ? 0 : d < 192 ? 1 : 2;
+      // lanes per cell: enough to spread long split loops over the chip
+      // when few cells are on the diagonal (one large instance), one lane per
+      // cell when a batch already fills it
+      int64_t cells_d = 0;
+      for (int k = 0; k < active; ++k) {
+        const Q& x = qs[order[k]];
+        cells_d += m.tracks == 1 ? x.n1 + 1 - d
+                                 : std::max<int64_t>(0, std::min<int64_t>(d, x.n1) - std::max<int64_t>(0, d - x.n2) + 1);
+      }
+      gi = !m.loops || d < 24 ? 0 : (d >= 192 && cells_d * 64 <= ((int64_t)1 << 18)) ? 2 : (cells_d * 8 <= ((int64_t)1 << 19)) ? 1 : 0;
       const int64_t lanes = c * (gi == 0 ? 1 : gi == 1 ? 8 : 64);
       block = gi == 0 || lanes < 256 ? 64 : 256;
       grid = (unsigned)((lanes + block - 1) / block);
     }
     int d0 = (int)d;
-    void* args[] = {(void*)&d_in1, (void*)&d_in2, (void*)&z, (void*)tabs_arg.data(), (void*)&d0, (void*)&d1};
-    if (!chk(hipModuleLaunchKernel(g->fill[gi], grid, 1, 1, block, 1, 1, 0, s, args, nullptr), "fill launch")) return DPX_EHIP;
+    void* args[] = {(void*)&d_inst, (void*)&d0, (void*)&d1};
+    if (!chk(hipModuleLaunchKernel(g->fill[gi], grid, (unsigned)active, 1, block, 1, 1, 0, s, args, nullptr), "fill launch")) return DPX_EHIP;
     d = d1;
   }
   if (!chk(hipEventRecord(g->ev[1], s), "event")) return DPX_EHIP;
-  void* d_ans = dev + o_ans;
-  int* d_out = (int*)(dev + o_ans + ans4);  // found, trace length (-1: capacity), i0, j0
-  void* d_tr = (tr && cap > 0) ? dev + o_tr : nullptr;
-  int capv = cap;
-  void* fargs[] = {(void*)&d_in1, (void*)&d_in2, (void*)&z, (void*)tabs_arg.data(), (void*)&d_ans, (void*)&d_out, (void*)&d_tr, (void*)&capv};
-  if (!chk(hipModuleLaunchKernel(g->finish, 1, 1, 1, 1, 1, 1, 0, s, fargs, nullptr), "finish launch")) return DPX_EHIP;
-  char* h = (char*)g->host;
-  if (!chk(hipMemcpyAsync(h, d_ans, ans4 + 16, hipMemcpyDeviceToHost, s), "result copy")) return DPX_EHIP;
+  void* fargs[] = {(void*)&d_inst};
+  if (!chk(hipModuleLaunchKernel(g->finish, (unsigned)count, 1, 1, 1, 1, 1, 0, s, fargs, nullptr), "finish launch")) return DPX_EHIP;
+  for (int q = 0; q < count; ++q)
+    if (!chk(hipMemcpyAsync(h_res + (ans4 + 16) * q, dev + qs[q].o_ans, ans4 + 16, hipMemcpyDeviceToHost, s), "result copy")) return DPX_EHIP;
   if (!chk(hipStreamSynchronize(s), "grammar run")) return DPX_EHIP;
-  const int* hout = (const int*)(h + ans4);
-  const int n_tr = hout[1];
-  if (d_tr && n_tr > 0) {
-    if (!chk(hipMemcpyAsync(h + 256 + m.answer_bytes, d_tr, sizeof(dpx_trace_t) * (size_t)n_tr, hipMemcpyDeviceToHost, s), "trace copy") ||
-        !chk(hipStreamSynchronize(s), "trace copy"))
+  int rc = DPX_OK;
+  int64_t tro = 0;
+  for (int q = 0; q < count; ++q) {
+    const int* hout = (const int*)(h_res + (ans4 + 16) * q + ans4);
+    const int n_tr = hout[1];
+    if (tr && qs[q].cap > 0 && n_tr > 0 &&
+        !chk(hipMemcpyAsync(h_tr + sizeof(dpx_trace_t) * tro, dev + qs[q].o_tr, sizeof(dpx_trace_t) * (size_t)n_tr, hipMemcpyDeviceToHost, s), "trace copy"))
       return DPX_EHIP;
-    std::memcpy(tr, h + 256 + m.answer_bytes, sizeof(dpx_trace_t) * (size_t)n_tr);
+    tro += std::max(n_tr, 0);
   }
-  if (answer) std::memcpy(answer, h, m.answer_bytes);
-  if (found) *found = hout[0];
-  if (len) *len = d_tr ? n_tr : 0;
+  if (tr && !chk(hipStreamSynchronize(s), "trace copy")) return DPX_EHIP;
+  tro = 0;
+  for (int q = 0; q < count; ++q) {
+    const char* hr = h_res + (ans4 + 16) * q;
+    const int* hout = (const int*)(hr + ans4);
+    if (answers) std::memcpy((char*)answers + (size_t)m.answer_bytes * q, hr, m.answer_bytes);
+    if (found) found[q] = hout[0];
+    const int n_tr = hout[1];
+    if (tr) {
+      if (n_tr > 0) std::memcpy(tr + tr_off[q], h_tr + sizeof(dpx_trace_t) * tro, sizeof(dpx_trace_t) * (size_t)n_tr);
+      if (tr_len) tr_len[q] = qs[q].cap > 0 ? n_tr : 0;
+      if (qs[q].cap > 0 && n_tr < 0) { err = "trace capacity too small for instance " + std::to_string(q); rc = DPX_ETRACE; }
+      tro += std::max(n_tr, 0);
+    }
+  }
   if (fill_ms) (void)hipEventElapsedTime(fill_ms, g->ev[0], g->ev[1]);
-  if (d_tr && n_tr < 0) { err = "trace capacity too small"; return DPX_ETRACE; }
-  return DPX_OK;
+  return rc;
+}
+
+int parse(Grammar* g, hipStream_t s, const void* in1, int n1, const void* in2, int n2, void* answer, int* found,
+          dpx_trace_t* tr, int cap, int* len, float* fill_ms, std::string& err) {
+  if (n1 < 0 || n2 < 0) { err = "bad input"; return DPX_EINVAL; }
+  const int64_t o1[2] = {0, n1}, o2[2] = {0, g->m.tracks == 2 ? n2 : 0}, to[2] = {0, tr && cap > 0 ? cap : 0};
+  int32_t f = 0, l = 0;
+  const int rc = parse_batch(g, s, 1, (const char*)in1, o1, (const char*)in2, g->m.tracks == 2 ? o2 : nullptr, answer, &f,
+                             tr && cap > 0 ? tr : nullptr, tr && cap > 0 ? to : nullptr, &l, fill_ms, err);
+  if (found) *found = f;
+  if (len) *len = l;
+  return rc;
 }
 
 }  // namespace adp

@@ -1691,4 +1691,20 @@ int dpx_grammar_parse(dpx_ctx* c, dpx_grammar* h, const void* in1, int n1, const
   return rc;
 }
 
+int dpx_grammar_parse_batch(dpx_ctx* c, dpx_grammar* h, const void* in1, const int64_t* offs1, const void* in2,
+                            const int64_t* offs2, int count, void* answers, int32_t* found, dpx_trace_t* traces,
+                            const int64_t* trace_offs, int32_t* trace_lens) {
+  if (!c || !h) return DPX_EINVAL;
+  if (h->device != c->device) return fail(c->err, DPX_EINVAL, "grammar was compiled for another device");
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::lock_guard<std::mutex> lk2(h->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c->err, DPX_EHIP, "hipSetDevice");
+  float ms = 0;
+  const int rc = dpx::adp::parse_batch(h->g, c->ws.stream, count, (const char*)in1, offs1, (const char*)in2, offs2, answers,
+                                       found, traces, trace_offs, trace_lens, &ms, c->err);
+  c->t_fill = ms;
+  c->t_fin = 0;
+  return rc;
+}
+
 }  // extern "C"

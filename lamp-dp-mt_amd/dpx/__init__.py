@@ -30,7 +30,7 @@ EXPORTS = [
     "dpx_plan_create", "dpx_plan_run", "dpx_plan_timing", "dpx_plan_results", "dpx_plan_destroy",
     "dpx_tune_bucket", "dpx_get_tuning", "dpx_set_tuning", "dpx_autotune", "dpx_set_swat_trace_budget",
     "dpx_grammar_create", "dpx_grammar_destroy", "dpx_grammar_check", "dpx_grammar_get_info",
-    "dpx_grammar_source", "dpx_grammar_parse",
+    "dpx_grammar_source", "dpx_grammar_parse", "dpx_grammar_parse_batch",
 ]
 
 REQ_NUSSINOV, REQ_ZUKER, REQ_SWAT, REQ_MATCHAIN, REQ_RNAFOLD = 1, 2, 3, 4, 5
@@ -140,6 +140,10 @@ def lib():
         L.dpx_grammar_parse.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
                                         ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Trace),
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        P64 = ctypes.POINTER(ctypes.c_int64)
+        L.dpx_grammar_parse_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, P64, ctypes.c_char_p, P64,
+                                              ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.POINTER(Trace), P64, ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
@@ -368,6 +372,39 @@ class CompiledGrammar:
             self.ctx._check(rc)
             a = g.unpack_answer(ans.raw)
             return a, bool(found.value), (_tuples(tr, tl.value) if trace else None)
+
+    def parse_batch(self, inputs, trace=True, cap_per=None):
+        """[(answer, found, trace)] for a list of inputs (pairs for two-track
+        grammars), all instances in one launch per diagonal."""
+        g = self.g
+        two = g.tracks == 2
+        b1 = [g.pack(x[0] if two else x) for x in inputs]
+        b2 = [g.pack(x[1]) for x in inputs] if two else None
+        ab = self.info.alphabet_bytes
+        cnt = len(inputs)
+        o1 = (ctypes.c_int64 * (cnt + 1))()
+        o2 = (ctypes.c_int64 * (cnt + 1))()
+        to = (ctypes.c_int64 * (cnt + 1))()
+        p = p2 = t = 0
+        for q in range(cnt):
+            o1[q], o2[q], to[q] = p, p2, t
+            p += len(b1[q]) // ab
+            p2 += len(b2[q]) // ab if two else 0
+            t += cap_per or 8 * (len(b1[q]) // ab + (len(b2[q]) // ab if two else 0)) + 64
+        o1[cnt], o2[cnt], to[cnt] = p, p2, t
+        ans = ctypes.create_string_buffer(max(8, self.info.answer_bytes * cnt))
+        found = (ctypes.c_int32 * max(cnt, 1))()
+        lens = (ctypes.c_int32 * max(cnt, 1))()
+        tr = (Trace * max(t, 1))() if trace else None
+        self.ctx._check(self._L.dpx_grammar_parse_batch(self.ctx.h, self.h, b"".join(b1), o1,
+                                                        b"".join(b2) if two else None, o2 if two else None, cnt, ans,
+                                                        found, tr, to if trace else None, lens if trace else None))
+        out = []
+        n = self.info.answer_bytes
+        for q in range(cnt):
+            a = g.unpack_answer(ans.raw[q * n:(q + 1) * n])
+            out.append((a, bool(found[q]), _tuples(tr[to[q]:to[q + 1]], lens[q]) if trace else None))
+        return out
 
     def close(self):
         if self.h:

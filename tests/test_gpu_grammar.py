@@ -165,3 +165,63 @@ def test_random_grammars_vs_oracle(ctx):
         cg.close()
         done += 1
     assert done > 30
+
+
+def test_batches_equal_single_parses(ctx):
+    """dpx_grammar_parse_batch: ragged batches (empty inputs included) give
+    every instance's single-parse result; one track, two tracks, window."""
+    from dpx import gen
+    rng = random.Random(12)
+    g = G.nussinov()
+    cg = ctx.grammar(g)
+    seqs = [gen.rna_uniform(rng.choice([0, 1, 2, 5, 31, 64, 65, 130, 257]), q) for q in range(40)]
+    print("nussinov grammar batch", flush=True)
+    got = cg.parse_batch(seqs)
+    builtin, btr = ctx.nussinov_batch(seqs, min_loop=0)
+    for q, s in enumerate(seqs):
+        assert got[q] == cg.parse(s), q
+        assert got[q][0] == builtin[q] and got[q][2] == btr[q], q
+    sw = ctx.grammar(G.swat_affine())
+    pairs = [gen.dna_pair(rng.choice([0, 1, 7, 64, 100]), rng.choice([0, 3, 65, 90]), q) for q in range(24)]
+    print("swat grammar batch", flush=True)
+    got = sw.parse_batch(pairs)
+    for q, (a, b) in enumerate(pairs):
+        assert got[q] == sw.parse(a, b), q
+        s, _, btr = ctx.swat(a, b, 2, -3, -5, -2)
+        assert got[q][0] == s and [t[:3] for t in got[q][2]] == [t[:3] for t in btr]
+    br = ctx.grammar(G.brackets(8))
+    texts = ["", "(1)", "((2))", "(((3)))((2))(1)", "12", "((((((((9))))))))"] + \
+        ["".join(rng.choice("()123") for _ in range(rng.randint(0, 40))) for _ in range(10)]
+    print("brackets grammar batch", flush=True)
+    got = br.parse_batch(texts)
+    for q, t in enumerate(texts):
+        assert got[q] == br.parse(t), q
+        _vs_oracle(br, G.brackets(8), t)
+
+
+def test_random_grammar_batches_vs_oracle(ctx):
+    import adp_random as R
+    done = 0
+    for seed in range(1, 40, 3):
+        g = R.single_track(seed) if seed % 3 else R.two_track(seed)
+        try:
+            G.oracle(g)
+        except ValueError:
+            continue
+        cg = ctx.grammar(g)
+        rng = random.Random(seed)
+        if g.tracks == 1:
+            ins = [R.text(rng, rng.randint(0, 20)) for _ in range(7)]
+            got = cg.parse_batch(ins)
+            for q, x in enumerate(ins):
+                _, (a, f, tr) = G.run_oracle(g, x)
+                assert got[q][1] == bool(f) and (not f or (got[q][0] == a and got[q][2] == [tuple(t) for t in tr])), (seed, q)
+        else:
+            ins = [(R.text(rng, rng.randint(0, 10)), R.text(rng, rng.randint(0, 10))) for _ in range(7)]
+            got = cg.parse_batch(ins)
+            for q, (x, y) in enumerate(ins):
+                _, (a, f, tr) = G.run_oracle(g, x, y)
+                assert got[q][1] == bool(f) and (not f or (got[q][0] == a and got[q][2] == [tuple(t) for t in tr])), (seed, q)
+        cg.close()
+        done += 1
+    assert done >= 8
