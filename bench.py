@@ -274,6 +274,14 @@ def single_grammar(args, ctx, line):
         n_cells = cells(1024)
         line["config"] = {"workload": "Nussinov.scala grammar, n=1024, i.i.d. uniform ACGU seed 0, score + backtrace, "
                                       "generic path (dpx_grammar_parse)"}
+    elif w == "grammar-zuker2048":
+        s = gen.rna_gc(2048, 2)
+        g = G.zuker()
+        run_in = (s,)
+        builtin = lambda: ctx.zuker(s)
+        n_cells = cells(2048)
+        line["config"] = {"workload": "Zuker.scala grammar (ZukerMFE, librna on the device), n=2048, i.i.d. 50% GC seed 2, "
+                                      "Turner 2004, MFE + backtrace, generic path"}
     elif w == "grammar-nussinov-batch":
         seqs = gen.rna_batch(1024, 50, 500, 5, gc=0.5)
         g = G.nussinov()
@@ -330,7 +338,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="batch", choices=["batch", "zuker2048", "rnafold2048", "nussinov1024", "swat16384",
                                                             "grammar-nussinov1024", "grammar-chain1024", "grammar-swat4000",
-                                                            "grammar-nussinov-batch"])
+                                                            "grammar-nussinov-batch", "grammar-zuker2048"])
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)

@@ -270,8 +270,13 @@ int dpx_server_get_stats(dpx_server* srv, dpx_server_stats* out);
  *    {"or": [p, p, ...]} | {"cat": [p, p], "track": 0|1|2 (~, -~, ~-),
  *    "hint": [minl, maxl, minr, maxr] (optional)} | {"map": p, "f": fn} |
  *    {"filter": p, "f": fn} | {"agg": p, "h": h}
- * Function bodies are C/C++ (`_in1[k]`, `_in2[k]` read the inputs; argument
- * types are deduced).  Rule ids, split positions, the backtrace order and
+ * Function bodies are C/C++ (`_in1[k]`, `_in2[k]` read the inputs, M_W and
+ * M_H are the table sizes as in the reference's generated code; argument
+ * types are deduced).  With "rna": true (single track, input = base codes
+ * A=1 C=2 G=3 U=4 as RNASignature.convert) the bodies may call the librna
+ * functions of librna_impl.h (hl_energy, bl_energy, br_energy, il_energy,
+ * sr_energy, termau_energy, ext_mismatch_energy, ml_mismatch_energy,
+ * ml_energy, ul_energy, bp_index/NO_BP) over the context's parameter file.  Rule ids, split positions, the backtrace order and
  * ties (first strictly better candidate in loop order) are the reference's
  * generated code's; trace records hold up to 2 split positions. */
 typedef struct dpx_grammar dpx_grammar;
@@ -279,7 +284,7 @@ typedef struct {
   int32_t tracks, n_tabs, n_rules, max_pos, axiom_id, window;
   int32_t answer_bytes, alphabet_bytes;
   int32_t loops;       /* 1 if some rule has a split loop (O(n) work per cell) */
-  int32_t reserved;
+  int32_t rna;         /* 1 if the algebra calls the librna energy functions */
   int64_t code_bytes;  /* size of the compiled code object */
 } dpx_grammar_info;
 /* Analyses, generates and compiles (hipRTC, gfx950) the grammar and loads it

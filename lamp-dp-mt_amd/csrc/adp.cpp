@@ -26,6 +26,10 @@
 
 #include <hip/hiprtc.h>
 
+#include <cstddef>
+
+#include "rna_model.h"
+
 #include <algorithm>
 #include <cstring>
 #include <functional>
@@ -201,6 +205,7 @@ int hm(int a, int b) { return a == INF ? b : b == INF ? a : std::min(a, b); }
 
 struct Model {
   int tracks = 1, window = 0, axiom = -1;
+  bool rna = false;  // algebra bodies call the librna energy functions
   std::string alphabet = "char", answer = "int", header, empty;
   int alphabet_bytes = 1, answer_bytes = 4;
   Agg h;
@@ -425,6 +430,8 @@ struct Builder {
     if (m.tracks != 1 && m.tracks != 2) return fail("tracks must be 1 or 2");
     m.window = num("window", 0);
     if (m.window < 0 || (m.window > 0 && m.tracks != 1)) return fail("window applies to single-track grammars");
+    if (const J* r = root.get("rna")) m.rna = r->t == J::BOOL && r->b;
+    if (m.rna && m.tracks != 1) return fail("librna functions apply to single-track grammars");
     m.alphabet = strv("alphabet", "char");
     m.answer = strv("answer", "int");
     m.alphabet_bytes = num("alphabet_bytes", 1);
@@ -640,7 +647,7 @@ struct Gen {
     if (mn > 0) o.arith.push_back(add(a, mn) + "<=" + b);
   }
   std::string call(int fn, const std::vector<std::string>& args) const {
-    std::string s = "f_" + m.fns[fn].name + "(_in1,_in2";
+    std::string s = "f_" + m.fns[fn].name + "(_x";
     for (auto& x : args) s += "," + x;
     return s + ")";
   }
@@ -761,6 +768,129 @@ std::string bound(const std::vector<std::string>& v, const char* f) {
   return s;
 }
 
+// The energy parameters as the device holds them (DevRna, rna_model.h), under
+// another name; the offsets are checked against the host's layout.
+std::string rna_prelude(const Model& m) {
+  std::ostringstream s;
+  s << "struct RnaPar {\n"
+       "  int stack[8][8];\n"
+       "  int mmH[8][5][5], mmI[8][5][5], mm1nI[8][5][5], mm23I[8][5][5], mmM[8][5][5], mmExt[8][5][5];\n"
+       "  int d5[8][5], d3[8][5];\n"
+       "  int hairpin[31], bulge[31];\n"
+       "  int il_ent[61];\n"
+       "  short F[31][31];\n"
+       "  int ninio0, ninio_max, il5, ml_closing, ml_intern, term_au;\n"
+       "  int n_loop[3];\n"
+       "  int loop_key[3][40];\n"
+       "  int loop_e[3][40];\n"
+       "  short int11[8][8][5][5];\n"
+       "  short int21[8][8][5][5][5];\n"
+       "  short int22[8][8][5][5][5][5];\n"
+       "};\n";
+  if (m.rna) {
+#define OFF(f) << " && __builtin_offsetof(RnaPar, " #f ") == " << offsetof(DevRna, f)
+    s << "static_assert(sizeof(RnaPar) == " << sizeof(DevRna) OFF(mmH) OFF(mmExt) OFF(d5) OFF(hairpin) OFF(bulge) OFF(il_ent)
+      OFF(F) OFF(ninio0) OFF(term_au) OFF(n_loop) OFF(loop_key) OFF(loop_e) OFF(int11) OFF(int21) OFF(int22)
+      << ", \"energy parameter layout\");\n";
+#undef OFF
+  }
+  return s.str();
+}
+
+// The librna functions the reference's RNA algebras call
+// (librna_impl.h:106-291), over the context's resident tables: the sequence
+// is in1 as codes (A=1, C=2, G=3, U=4, as RNASignature.convert), hl is the
+// hairpin entropy by loop size.  Interior loops and bulges up to 30 per side
+// (the reference's grammars bound them with seq(1,30)).
+std::string rna_functions() {
+  return R"(#define NO_BP 7
+__device__ __forceinline__ int bp_index(int x, int y) {
+  switch (x) {
+    case 1: if (y == 4) return 5; break;
+    case 2: if (y == 3) return 1; break;
+    case 3: if (y == 2) return 2; if (y == 4) return 3; break;
+    case 4: if (y == 3) return 4; if (y == 1) return 6; break;
+  }
+  return 7;
+}
+__device__ __forceinline__ int rS_(const Ctx& x, int k) { return x.in1[k]; }
+__device__ __forceinline__ int rBP_(const Ctx& x, int a, int b) { return bp_index(rS_(x, a), rS_(x, b)); }
+__device__ __forceinline__ int rTermau_(const Ctx& x, int i, int j) {
+  const int p = rS_(x, i), q = rS_(x, j);
+  return ((p == 3 && q == 2) || (p == 2 && q == 3)) ? 0 : x.rna->term_au;
+}
+__device__ __forceinline__ int rSr_(const Ctx& x, int i, int j) { return x.rna->stack[rBP_(x, i, j)][rBP_(x, j - 1, i + 1)]; }
+__device__ __forceinline__ int rHl_(const Ctx& x, int i, int j) {
+  const int size = j - i - 1;
+  if (size < 3) return 600;
+  const int cls = size == 3 ? 0 : size == 4 ? 1 : size == 6 ? 2 : -1;
+  if (cls >= 0) {
+    int key = 0;
+    for (int m = 0; m <= size + 1; ++m) key = key * 8 + rS_(x, i + m);
+    for (int e = 0; e < x.rna->n_loop[cls]; ++e)
+      if (x.rna->loop_key[cls][e] == key) return x.rna->loop_e[cls][e];
+  }
+  const int ent = x.hl[size];
+  if (size == 3) return ent + rTermau_(x, i, j);
+  return ent + x.rna->mmH[rBP_(x, i, j)][rS_(x, i + 1)][rS_(x, j - 1)];
+}
+__device__ __forceinline__ int rBl_(const Ctx& x, int i, int k, int l, int j) {
+  const int size = l - k + 1;
+  if (size < 0 || size > 30) return 10000000;
+  if (size <= 1) return (size ? x.rna->bulge[1] : 0) + x.rna->stack[rBP_(x, i, j)][rBP_(x, j - 1, l + 1)];
+  return x.rna->bulge[size] + rTermau_(x, i, j) + rTermau_(x, j - 1, l + 1);
+}
+__device__ __forceinline__ int rBr_(const Ctx& x, int i, int k, int l, int j) {
+  const int size = l - k + 1;
+  if (size < 0 || size > 30) return 10000000;
+  if (size <= 1) return (size ? x.rna->bulge[1] : 0) + x.rna->stack[rBP_(x, i, j)][rBP_(x, k - 1, i + 1)];
+  return x.rna->bulge[size] + rTermau_(x, i, j) + rTermau_(x, k - 1, i + 1);
+}
+__device__ __forceinline__ int rIl_(const Ctx& x, int i, int k, int l, int j) {
+  const RnaPar* P = x.rna;
+  const int sl = k - i - 1, sr = j - l - 1;
+  if (sl == 0) return rBr_(x, i, l + 1, j - 1, j);
+  if (sr == 0) return rBl_(x, i, i + 1, k - 1, j);
+  if (sl < 0 || sr < 0 || sl > 30 || sr > 30) return 10000000;
+  const int ocb = rBP_(x, i, j), icb = rBP_(x, l, k);
+  const int olb = rS_(x, i + 1), orb = rS_(x, j - 1), ilb = rS_(x, l + 1), irb = rS_(x, k - 1);
+  if (sl == 1) {
+    if (sr == 1) return P->int11[ocb][rBP_(x, j - 2, i + 2)][olb][orb];
+    if (sr == 2) return P->int21[ocb][rBP_(x, j - 3, i + 2)][olb][rS_(x, j - 2)][orb];
+    return P->F[sl][sr] + P->mm1nI[ocb][olb][orb] + P->mm1nI[icb][ilb][irb];
+  }
+  if (sl == 2) {
+    if (sr == 1) return P->int21[rBP_(x, j - 2, i + 3)][ocb][orb][olb][rS_(x, i + 2)];
+    if (sr == 2) return P->int22[ocb][rBP_(x, j - 3, i + 3)][olb][rS_(x, i + 2)][rS_(x, j - 2)][orb];
+    if (sr == 3) return P->il5 + P->ninio0 + P->mm23I[ocb][olb][orb] + P->mm23I[icb][ilb][irb];
+  } else if (sl == 3 && sr == 2) {
+    return P->il5 + P->ninio0 + P->mm23I[ocb][olb][orb] + P->mm23I[icb][ilb][irb];
+  } else if (sr == 1) {
+    return P->F[sl][sr] + P->mm1nI[ocb][olb][orb] + P->mm1nI[icb][ilb][irb];
+  }
+  return P->F[sl][sr] + P->mmI[ocb][olb][orb] + P->mmI[icb][ilb][irb];
+}
+__device__ __forceinline__ int rExt_(const Ctx& x, int i, int j) {
+  const int b = rBP_(x, i, j);
+  if (i > 0 && j + 1 < x.n1) return x.rna->mmExt[b][rS_(x, i - 1)][rS_(x, j + 1)];
+  if (i > 0) return x.rna->d5[b][rS_(x, i - 1)];
+  if (j + 1 < x.n1) return x.rna->d3[b][rS_(x, j + 1)];
+  return 0;
+}
+__device__ __forceinline__ int rMlmm_(const Ctx& x, int i, int j) { return x.rna->mmM[rBP_(x, j, i)][rS_(x, j - 1)][rS_(x, i + 1)]; }
+#define hl_energy(i, j) rHl_(_x, (i), (j))
+#define bl_energy(i, k, l, j, xr) rBl_(_x, (i), (k), (l), (j))
+#define br_energy(i, k, l, j, xl) rBr_(_x, (i), (k), (l), (j))
+#define il_energy(i, k, l, j) rIl_(_x, (i), (k), (l), (j))
+#define sr_energy(i, j) rSr_(_x, (i), (j))
+#define termau_energy(i, j) rTermau_(_x, (i), (j))
+#define ext_mismatch_energy(i, j) rExt_(_x, (i), (j))
+#define ml_mismatch_energy(i, j) rMlmm_(_x, (i), (j))
+#define ml_energy() (_x.rna->ml_closing)
+#define ul_energy() (_x.rna->ml_intern)
+)";
+}
+
 bool generate(Model& m, std::string& src, std::string& err) {
   std::vector<std::vector<AltCode>> alts(m.tabs.size());
   for (int t : m.order) {
@@ -796,9 +926,15 @@ bool generate(Model& m, std::string& src, std::string& err) {
     << "struct Tabs { void* c[" << NT << "]; short* r[" << NT << "]; int* p[" << NT << "][2]; };\n"
     << "struct Rec { int i, j, rule, pos[2]; };\n"
     << "// one instance of a batch: inputs, outputs and tables (host writes this layout)\n"
-    << "struct Inst { const input_t* in1; const input_t* in2; answer_t* ans; int* out; Rec* tr; Dims z; int cap, pad; Tabs T; };\n"
-    << "static_assert(__builtin_offsetof(Inst, z) == 40 && __builtin_offsetof(Inst, cap) == 48 && "
-       "__builtin_offsetof(Inst, T) == 56 && sizeof(Inst) == 56 + " << 32 * NT << ", \"instance layout\");\n";
+    << "struct Inst { const input_t* in1; const input_t* in2; answer_t* ans; int* out; Rec* tr; const void* rna; "
+       "const int* hl; Dims z; int cap, pad; Tabs T; };\n"
+    << "static_assert(__builtin_offsetof(Inst, z) == 56 && __builtin_offsetof(Inst, cap) == 64 && "
+       "__builtin_offsetof(Inst, T) == 72 && sizeof(Inst) == 72 + " << 32 * NT << ", \"instance layout\");\n"
+    << rna_prelude(m)
+    << "// what every algebra function sees: the inputs, their sizes, the energy tables\n"
+       "struct Ctx { const input_t* in1; const input_t* in2; int n1, n2; const RnaPar* rna; const int* hl; };\n"
+    << (m.tracks == 1 ? "#define M_W (_x.n1 + 1)\n#define M_H (_x.n1 + 1)\n" : "#define M_H (_x.n1 + 1)\n#define M_W (_x.n2 + 1)\n")
+    << (m.rna ? rna_functions() : std::string());
   if (m.tracks == 1)
     s << "__device__ __forceinline__ long long idx_(const Dims& z, int i, int j) {\n"
          "  const long long d = j - i;  // diagonal-major: the cells of one diagonal are contiguous\n"
@@ -811,17 +947,17 @@ bool generate(Model& m, std::string& src, std::string& err) {
       for (size_t q = 0; q < f.args.size(); ++q) s << (q ? ", " : "") << "class A" << q;
       s << ">\n";
     }
-    s << "__device__ __forceinline__ auto f_" << f.name << "(const input_t* _in1, const input_t* _in2";
+    s << "__device__ __forceinline__ auto f_" << f.name << "(const Ctx& _x";
     for (size_t q = 0; q < f.args.size(); ++q) s << ", A" << q << " " << f.args[q];
-    s << ") {\n  (void)_in1; (void)_in2;\n  " << f.body << "\n}\n";
+    s << ") {\n  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n  " << f.body << "\n}\n";
   }
   auto better = [&](const std::string& c, const std::string& c0) {
     std::string cmp;
     switch (m.h.k) {
       case A_MAX: cmp = c + ">" + c0; break;
       case A_MIN: cmp = c + "<" + c0; break;
-      case A_MAXBY: cmp = "f_" + m.fns[m.h.fn].name + "(_in1,_in2," + c + ")>f_" + m.fns[m.h.fn].name + "(_in1,_in2," + c0 + ")"; break;
-      case A_MINBY: cmp = "f_" + m.fns[m.h.fn].name + "(_in1,_in2," + c + ")<f_" + m.fns[m.h.fn].name + "(_in1,_in2," + c0 + ")"; break;
+      case A_MAXBY: cmp = "f_" + m.fns[m.h.fn].name + "(_x," + c + ")>f_" + m.fns[m.h.fn].name + "(_x," + c0 + ")"; break;
+      case A_MINBY: cmp = "f_" + m.fns[m.h.fn].name + "(_x," + c + ")<f_" + m.fns[m.h.fn].name + "(_x," + c0 + ")"; break;
     }
     return cmp;
   };
@@ -843,7 +979,8 @@ bool generate(Model& m, std::string& src, std::string& err) {
     const Tab& T = m.tabs[t];
     s << "// ---- " << T.name << "[i,j]: rules " << T.id << ".." << T.id + T.alt - 1 << "\n"
       << "template <int G> __device__ __forceinline__ void tab_" << t
-      << "(const input_t* _in1, const input_t* _in2, const Dims& z, const Tabs& T, int i, int j, int lane) {\n"
+      << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, int lane) {\n"
+      << "  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n"
       << "  answer_t _c0" << (m.empty.empty() ? "{}" : " = (answer_t)(" + m.empty + ")") << ";\n"
       << "  int _r0 = -1, _p0 = 0, _p1 = 0, _o0 = 0, _o1 = 0;\n";
     for (auto& o : alts[t]) {
@@ -907,7 +1044,8 @@ bool generate(Model& m, std::string& src, std::string& err) {
   // diagonals by one workgroup with a barrier between them
   s << "template <int G> __device__ __forceinline__ void fill_(const Inst* __restrict__ I, int d0, int d1) {\n"
        "  const Inst& q = I[blockIdx.y];  // instances sorted by size, largest first: blockIdx.y < active ones\n"
-       "  const input_t* _in1 = q.in1;\n  const input_t* _in2 = q.in2;\n  const Dims z = q.z;\n  const Tabs& T = q.T;\n"
+       "  const Dims z = q.z;\n  const Tabs& T = q.T;\n"
+       "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl};\n"
        "  const int tid = blockIdx.x * blockDim.x + threadIdx.x, lane = tid & (G - 1);\n"
        "  for (int d = d0; d < d1; ++d) {\n";
   if (m.tracks == 1)
@@ -920,7 +1058,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
     s << "      const int i = t, j = t + d;\n";
   else
     s << "      const int i = ilo + t, j = d - i;\n";
-  for (int t : m.order) s << "      tab_" << t << "<G>(_in1, _in2, z, T, i, j, lane);\n";
+  for (int t : m.order) s << "      tab_" << t << "<G>(_x, z, T, i, j, lane);\n";
   s << "    }\n"
        "    if (d + 1 < d1) __syncthreads();\n"
        "  }\n}\n";
@@ -930,8 +1068,9 @@ bool generate(Model& m, std::string& src, std::string& err) {
   const Tab& AX = m.tabs[m.axiom];
   s << "extern \"C\" __global__ void adp_finish(const Inst* I) {\n"
        "  const Inst& q = I[blockIdx.x];\n"
-       "  const Dims z = q.z;\n  const Tabs& T = q.T;\n  const input_t* _in1 = q.in1;\n  const input_t* _in2 = q.in2;\n"
-       "  (void)_in1; (void)_in2;\n"
+       "  const Dims z = q.z;\n  const Tabs& T = q.T;\n"
+       "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl};\n"
+       "  const input_t* _in1 = q.in1;\n  const input_t* _in2 = q.in2;\n  (void)_in1; (void)_in2;\n"
        "  answer_t* ans = q.ans;\n  int* out = q.out;\n  Rec* tr = q.tr;\n  const int cap = q.cap;\n"
        "  int i0, j0;\n  bool found;\n";
   if (m.window > 0) {
@@ -1082,6 +1221,7 @@ void info(const Grammar* g, dpx_grammar_info* o) {
   o->answer_bytes = m.answer_bytes;
   o->alphabet_bytes = m.alphabet_bytes;
   o->loops = m.loops ? 1 : 0;
+  o->rna = m.rna ? 1 : 0;
   o->code_bytes = (int64_t)g->code.size();
 }
 
@@ -1091,13 +1231,14 @@ size_t r256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int64_t* off1, const char* in2,
                 const int64_t* off2, void* answers, int32_t* found, dpx_trace_t* tr, const int64_t* tr_off,
-                int32_t* tr_len, float* fill_ms, std::string& err) {
+                int32_t* tr_len, const void* rna, const int* hl, float* fill_ms, std::string& err) {
   const Model& m = g->m;
   if (!g->mod) { err = "grammar was compiled without loading"; return DPX_EINVAL; }
+  if (m.rna && (!rna || !hl)) { err = "grammar calls librna functions: load an energy parameter file first"; return DPX_ENOPARAMS; }
   if (count <= 0 || !off1 || (m.tracks == 2 && !off2) || (tr && !tr_off)) { err = "bad batch arguments"; return DPX_EINVAL; }
   const int NT = (int)m.tabs.size();
   const size_t ab = (size_t)m.alphabet_bytes, ans4 = ((size_t)m.answer_bytes + 3) & ~(size_t)3;
-  const size_t inst_bytes = 56 + 32 * (size_t)NT;
+  const size_t inst_bytes = 72 + 32 * (size_t)NT;
   struct Q { int64_t n1, n2, D, cells; size_t o_in1, o_in2, o_tab, o_ans, o_tr; int64_t cap; };
   std::vector<Q> qs(count);
   size_t off = r256(inst_bytes * count);
@@ -1172,8 +1313,10 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
     p = dev + x.o_ans; put(16, &p, 8);
     p = dev + x.o_ans + ans4; put(24, &p, 8);
     p = x.cap > 0 ? dev + x.o_tr : nullptr; put(32, &p, 8);
+    put(40, &rna, 8);
+    put(48, &hl, 8);
     const int dims[4] = {(int)x.n1, (int)x.n2, (int)x.cap, 0};
-    put(40, dims, 16);
+    put(56, dims, 16);
     size_t o = x.o_tab;
     for (int t = 0; t < NT; ++t) {
       if (!m.tabs[t].used) continue;
@@ -1182,10 +1325,10 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
       void* p0 = m.tabs[t].cat > 0 ? dev + o : nullptr;
       void* p1 = m.tabs[t].cat > 1 ? dev + o + (size_t)x.cells * 4 : nullptr;
       o += r256((size_t)x.cells * 4 * m.tabs[t].cat);
-      put(56 + 8 * (size_t)t, &c, 8);
-      put(56 + 8 * ((size_t)NT + t), &r, 8);
-      put(56 + 8 * (2 * (size_t)NT + 2 * t), &p0, 8);
-      put(56 + 8 * (2 * (size_t)NT + 2 * t + 1), &p1, 8);
+      put(72 + 8 * (size_t)t, &c, 8);
+      put(72 + 8 * ((size_t)NT + t), &r, 8);
+      put(72 + 8 * (2 * (size_t)NT + 2 * t), &p0, 8);
+      put(72 + 8 * (2 * (size_t)NT + 2 * t + 1), &p1, 8);
     }
     if (x.n1 > 0 && !chk(hipMemcpyAsync(dev + x.o_in1, in1 + off1[q] * ab, (size_t)x.n1 * ab, hipMemcpyHostToDevice, s), "input copy")) return DPX_EHIP;
     if (x.n2 > 0 && !chk(hipMemcpyAsync(dev + x.o_in2, in2 + off2[q] * ab, (size_t)x.n2 * ab, hipMemcpyHostToDevice, s), "input copy")) return DPX_EHIP;
@@ -1271,12 +1414,12 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
 }
 
 int parse(Grammar* g, hipStream_t s, const void* in1, int n1, const void* in2, int n2, void* answer, int* found,
-          dpx_trace_t* tr, int cap, int* len, float* fill_ms, std::string& err) {
+          dpx_trace_t* tr, int cap, int* len, const void* rna, const int* hl, float* fill_ms, std::string& err) {
   if (n1 < 0 || n2 < 0) { err = "bad input"; return DPX_EINVAL; }
   const int64_t o1[2] = {0, n1}, o2[2] = {0, g->m.tracks == 2 ? n2 : 0}, to[2] = {0, tr && cap > 0 ? cap : 0};
   int32_t f = 0, l = 0;
   const int rc = parse_batch(g, s, 1, (const char*)in1, o1, (const char*)in2, g->m.tracks == 2 ? o2 : nullptr, answer, &f,
-                             tr && cap > 0 ? tr : nullptr, tr && cap > 0 ? to : nullptr, &l, fill_ms, err);
+                             tr && cap > 0 ? tr : nullptr, tr && cap > 0 ? to : nullptr, &l, rna, hl, fill_ms, err);
   if (found) *found = f;
   if (len) *len = l;
   return rc;

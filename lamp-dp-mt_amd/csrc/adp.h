@@ -25,13 +25,14 @@ void destroy(Grammar* g);
 const std::string& source(const Grammar* g);
 void info(const Grammar* g, dpx_grammar_info* out);
 int parse(Grammar* g, hipStream_t s, const void* in1, int n1, const void* in2, int n2, void* answer, int* found,
-          dpx_trace_t* tr, int cap, int* len, float* fill_ms, std::string& err);
+          dpx_trace_t* tr, int cap, int* len, const void* rna, const int* hl, float* fill_ms, std::string& err);
 // count instances: instance q reads in1[off1[q]..off1[q+1]) (and in2 by off2
 // for two tracks), elements of alphabet_bytes; answers q * answer_bytes;
-// trace of q at tr + tr_off[q] with capacity tr_off[q+1] - tr_off[q]
+// trace of q at tr + tr_off[q] with capacity tr_off[q+1] - tr_off[q];
+// rna / hl: the energy tables and hairpin entropies (grammars with "rna")
 int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int64_t* off1, const char* in2,
                 const int64_t* off2, void* answers, int32_t* found, dpx_trace_t* tr, const int64_t* tr_off,
-                int32_t* tr_len, float* fill_ms, std::string& err);
+                int32_t* tr_len, const void* rna, const int* hl, float* fill_ms, std::string& err);
 
 }  // namespace adp
 }  // namespace dpx

@@ -1685,7 +1685,8 @@ int dpx_grammar_parse(dpx_ctx* c, dpx_grammar* h, const void* in1, int n1, const
   std::lock_guard<std::mutex> lk2(h->mu);
   if (hipSetDevice(c->device) != hipSuccess) return fail(c->err, DPX_EHIP, "hipSetDevice");
   float ms = 0;
-  const int rc = dpx::adp::parse(h->g, c->ws.stream, in1, n1, in2, n2, answer, found, trace, cap, len, &ms, c->err);
+  const int rc = dpx::adp::parse(h->g, c->ws.stream, in1, n1, in2, n2, answer, found, trace, cap, len,
+                                 c->has_rna ? c->d_rna.p : nullptr, c->has_rna ? (const int*)c->d_hl.p : nullptr, &ms, c->err);
   c->t_fill = ms;
   c->t_fin = 0;
   return rc;
@@ -1701,7 +1702,8 @@ int dpx_grammar_parse_batch(dpx_ctx* c, dpx_grammar* h, const void* in1, const i
   if (hipSetDevice(c->device) != hipSuccess) return fail(c->err, DPX_EHIP, "hipSetDevice");
   float ms = 0;
   const int rc = dpx::adp::parse_batch(h->g, c->ws.stream, count, (const char*)in1, offs1, (const char*)in2, offs2, answers,
-                                       found, traces, trace_offs, trace_lens, &ms, c->err);
+                                       found, traces, trace_offs, trace_lens, c->has_rna ? c->d_rna.p : nullptr,
+                                       c->has_rna ? (const int*)c->d_hl.p : nullptr, &ms, c->err);
   c->t_fill = ms;
   c->t_fin = 0;
   return rc;

@@ -93,7 +93,7 @@ class TuneReport(ctypes.Structure):
 
 class GrammarInfo(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int32) for k in ("tracks", "n_tabs", "n_rules", "max_pos", "axiom_id", "window",
-                                              "answer_bytes", "alphabet_bytes", "loops", "reserved")] + \
+                                              "answer_bytes", "alphabet_bytes", "loops", "rna")] + \
                [("code_bytes", ctypes.c_int64)]
 
 

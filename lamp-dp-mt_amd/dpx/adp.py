@@ -119,7 +119,7 @@ class Grammar:
     input element / one answer as the C types lay them out."""
 
     def __init__(self, tracks=1, alphabet="char", alphabet_fmt="b", answer="int", answer_fmt="i", h="max",
-                 window=0, empty_value=None, header=""):
+                 window=0, empty_value=None, header="", rna=False):
         self.tracks = tracks
         self.alphabet, self.alphabet_fmt = alphabet, alphabet_fmt
         self.answer, self.answer_fmt = answer, answer_fmt
@@ -127,6 +127,7 @@ class Grammar:
         self.window = window
         self.empty_value = empty_value  # the reference's cudaEmpty (CodeGen.scala:519)
         self.header = header
+        self.rna = rna  # bodies call librna; inputs are RNA letters, passed as codes A=1 C=2 G=3 U=4
         self.tabs = {}
         self.axiom = None
         self._in1 = self._in2 = None
@@ -207,6 +208,8 @@ class Grammar:
              "tabulations": {n: {"always_valid": t.always_valid, "p": t.inner().node} for n, t in self.tabs.items()}}
         if self.empty_value is not None:
             d["empty"] = self.empty_value
+        if self.rna:
+            d["rna"] = True
         return d
 
     def spec_json(self):
@@ -217,6 +220,12 @@ class Grammar:
         if isinstance(seq, (bytes, bytearray)):
             return bytes(seq)
         if isinstance(seq, str):
+            if self.rna:  # RNASignature.convert (the reference maps other letters to 5, past its tables)
+                codes = {"A": 1, "C": 2, "G": 3, "U": 4, "T": 4}
+                try:
+                    return bytes(codes[c] for c in seq.upper())
+                except KeyError:
+                    raise ValueError("RNA input must be ACGU/T")
             return seq.encode()
         fmt = "=" + self.alphabet_fmt
         return b"".join(struct.pack(fmt, *(x if isinstance(x, tuple) else (x,))) for x in seq)

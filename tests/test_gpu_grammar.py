@@ -225,3 +225,26 @@ def test_random_grammar_batches_vs_oracle(ctx):
         cg.close()
         done += 1
     assert done >= 8
+
+
+def test_zuker_grammar_vs_reference_vectors_and_builtin(ctx):
+    """Zuker.scala (ZukerGrammar + ZukerMFE, librna calls evaluated on the
+    device) through the generic path: bench2/zu.c's golden vectors (the
+    reference's generated code for this grammar) and dpx_zuker."""
+    import dpx.examples as ex
+    from dpx import gen
+    g = ex.zuker()
+    cg = ctx.grammar(g)
+    i = cg.info
+    assert (i.n_rules, i.max_pos, i.n_tabs, i.rna, i.axiom_id) == (15, 2, 4, 1, 12)  # zu.h:1-5
+    fx = json.load(open(o.ROOT + "/tests/golden/zuker_ref.json"))
+    for case in fx:
+        print("zuker grammar golden n=%d" % len(case["seq"]), flush=True)
+        ans, found, tr = cg.parse(case["seq"])
+        assert found and ans == case["score"], len(case["seq"])
+        assert [list(t) for t in tr] == case["trace"], len(case["seq"])
+    seqs = [gen.rna_gc(n, 40 + n) for n in (0, 1, 5, 9, 31, 64, 150, 400)]
+    got = cg.parse_batch(seqs)
+    for s, r in zip(seqs, got):
+        score, btr = ctx.zuker(s)
+        assert r[0] == score and r[2] == btr, len(s)
