@@ -165,3 +165,26 @@ def zuker():
     g.axiom = st
     return g
 
+
+def rnafold():
+    """RNAFold.scala:10-66 (RNAFoldGrammar + RNAFoldAlgebra), librna calls
+    evaluated on the device."""
+    g = A.Grammar(tracks=1, h="min", rna=True)
+    f = lambda args, body: A.cfun(None, args, body)
+    hairpin = f("ij", "return hl_energy(ij._1,ij._2-1);")
+    stack = f("i,s,j", "return sr_energy(i,j) + s;")
+    iloop = f("ik,s,lj", "return il_energy(ik._1,ik._2,lj._1-1,lj._2-1) + s;")
+    mloop = f("i,s,j", "return s;")
+    left = f("l,r", "return l;")
+    right = f("l,r", "return r;")
+    join = f("l,r", "return l+r;")
+    basepairing = f("i,j", "return i+2<=j && bp_index(_in1[i],_in1[j-1])!=NO_BP;")
+    length = lambda mn: f("i,j", "return j-i>=%d;" % mn)
+    Qp = g.tabulate("Qp", lambda: ((g.seq(3) ^ hairpin) | (g.eli * Qp * g.eli ^ stack) |
+                                   (g.seq(1, 30) * Qp * g.seq(1, 30) ^ iloop) |
+                                   (g.eli * QM * g.eli ^ mloop)).aggregate("min").filter(basepairing))
+    QM = g.tabulate("QM", lambda: (Q * Q ^ join).filter(length(4)).aggregate("min"))
+    Q = g.tabulate("Q", lambda: (QM | (Q * g.eli ^ left) | (g.eli * Q ^ right) | Qp).filter(length(2)).aggregate("min"))
+    g.axiom = Q
+    return g
+

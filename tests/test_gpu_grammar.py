@@ -248,3 +248,29 @@ def test_zuker_grammar_vs_reference_vectors_and_builtin(ctx):
     for s, r in zip(seqs, got):
         score, btr = ctx.zuker(s)
         assert r[0] == score and r[2] == btr, len(s)
+
+
+def test_rnafold_grammar_vs_builtin(ctx):
+    """RNAFold.scala through the generic path equals dpx_rnafold (rule ids
+    Q 0-3, QM 4, Qp 5-8) and the committed oracle fixture."""
+    import dpx.examples as ex
+    from dpx import gen
+    g = ex.rnafold()
+    cg = ctx.grammar(g)
+    assert (cg.info.n_rules, cg.info.n_tabs, cg.info.rna) == (9, 3, 1)
+    seqs = [gen.rna_gc(n, 7 * n + 3) for n in (0, 1, 2, 5, 12, 33, 64, 150, 300)] + ["AAAAAAAAAA", "GGGAAACCC"]
+    got = cg.parse_batch(seqs)
+    for s, r in zip(seqs, got):
+        print("rnafold grammar n=%d" % len(s), flush=True)
+        score, btr = ctx.rnafold(s)
+        assert r == cg.parse(s)
+        if r[1]:
+            assert r[0] == score and r[2] == btr, len(s)
+        else:  # no structure: the built-in reports 999999 and no trace
+            assert (score, btr) == (999999, []), len(s)
+    for fx in json.load(open(o.ROOT + "/tests/golden/rnafold_large.json")):  # oracle answers at n = 1500, 2048
+        seq = getattr(gen, fx["gen"])(fx["n"], fx["seed"])
+        print("rnafold grammar fixture n=%d" % len(seq), flush=True)
+        ans, found, tr = cg.parse(seq)
+        assert found and ans == fx["score"] and [list(t) for t in tr] == fx["trace"]
+
