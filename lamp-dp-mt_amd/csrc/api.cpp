@@ -405,7 +405,12 @@ int tri_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
 // traceback codes (only with a trace; the synchronous entry point's choice)
 int sw_prepare(std::string& err, Job& j, const char* a, int n1, const char* b, int n2, const dpx_swat_scores& sc,
                int trace_from, int trace_cap, int seg = 0) {
-  if ((int64_t)(n1 + 1) * (n2 + 1) >= ((int64_t)1 << 32)) return fail(err, DPX_EINVAL, "matrix too large");
+  // the best cell travels as one 64-bit key: a 36-bit cell index and a
+  // 28-bit score (bounded by the largest gain per step times the steps)
+  if ((int64_t)(n1 + 1) * (n2 + 1) > (int64_t)SW_KEY_MASK) return fail(err, DPX_EINVAL, "matrix too large (over 2^36 cells)");
+  const int64_t gain = std::max({0, sc.match, sc.gap_open, sc.gap_ext});
+  if (gain * ((int64_t)n1 + n2 + 1) >= ((int64_t)1 << (64 - SW_KEY_BITS - 1)))
+    return fail(err, DPX_EINVAL, "scores too large for the best-cell key");
   j.kind = JOB_SW;
   j.a.assign(a, n1);
   j.b.assign(b, n2);

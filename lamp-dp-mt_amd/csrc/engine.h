@@ -101,6 +101,9 @@ struct SwArgs {
 // trace state: [0] i, [1] j, [2] table (0 M, 4 g1, 7 g2), [3] records written,
 // [4] 1 when the trace has ended, [5] 0 after an overflow
 constexpr int SW_STATE = 6;
+// best-cell key of an alignment: score << SW_KEY_BITS | (SW_KEY_MASK - i*(n2+1)+j)
+constexpr int SW_KEY_BITS = 36;
+constexpr unsigned long long SW_KEY_MASK = (1ull << SW_KEY_BITS) - 1;
 int sw_launch(const SwArgs& A, const int2* row0, int grid, int chunk, void* stream);  // chunk 16, 32 or 64
 int sw_finish_launch(const SwArgs& A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out,
                      const uint8_t* r0m, const uint8_t* r0g2, void* tr, int cap, int* state, void* stream);

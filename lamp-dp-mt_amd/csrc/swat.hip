@@ -181,8 +181,10 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     if (row_ok && i == A.n1) A.out[0] = outv;
     // best cell of this band: largest M, then smallest (i, j)
     if (row_ok && best >= 0) {
-      const unsigned long long key = ((unsigned long long)(unsigned)best << 32) |
-                                     (unsigned long long)(0xffffffffu - (uint32_t)((uint64_t)i * (uint64_t)W + (uint64_t)bestj));
+      // score in the top 28 bits, then the complement of the 36-bit linear
+      // index (the host bounds both): largest score, then smallest (i, j)
+      const unsigned long long key = ((unsigned long long)(unsigned)best << SW_KEY_BITS) |
+                                     (SW_KEY_MASK - ((unsigned long long)i * (unsigned long long)W + (unsigned long long)bestj));
       atomicMax(A.best, key);
     }
   }
@@ -208,8 +210,8 @@ __global__ void sw_pick(SwArgs A, int row0_best, int row0_bestj, int row0_axiom,
   const int W = A.n2 + 1;
   int bv = row0_best, bi = 0, bj = row0_bestj;
   if (A.nbands > 0) {
-    const int dv = (int)(key >> 32);
-    const uint32_t lin = 0xffffffffu - (uint32_t)(key & 0xffffffffu);
+    const int dv = (int)(key >> SW_KEY_BITS);
+    const unsigned long long lin = SW_KEY_MASK - (key & SW_KEY_MASK);
     if (dv > bv) { bv = dv; bi = (int)(lin / (uint32_t)W); bj = (int)(lin % (uint32_t)W); }
   }
   out[0] = A.n1 == 0 ? row0_axiom : A.out[0];

@@ -96,3 +96,34 @@ def test_checkpointed_matches_full_at_40kbp(ctx):
     assert full[1][0] >= 2 * 30000 * 0.9  # the shared segment aligns
     assert [x[:3] for x in full[2]] == [x[:3] for x in cp[2]]
     assert _moves_consistent([x[:3] for x in cp[2]])
+
+
+def test_checkpointed_matches_full_at_100kbp(ctx):
+    """100000 x 100000 (1e10 cells; every code would take 5 GB): with a
+    1 GiB budget the traceback recomputes segments and must equal the walk
+    through every code."""
+    from dpx import gen
+    base = gen.dna_uniform(100000, 31)
+    a = base
+    b = gen.dna_uniform(5000, 32) + base[10000:90000] + gen.dna_uniform(15000, 33)
+    print("swat 100000x100000 full", flush=True)
+    full = ctx.swat(a, b, 2, -3, -5, -2, trace_from=1)
+    ctx.set_swat_trace_budget(1 << 30)
+    try:
+        print("swat 100000x100000 checkpointed", flush=True)
+        cp = ctx.swat(a, b, 2, -3, -5, -2, trace_from=1)
+    finally:
+        ctx.set_swat_trace_budget(32 << 30)
+    assert full[:2] == cp[:2] and full[1][0] >= 2 * 80000 * 0.9
+    assert full[2] == cp[2]
+    assert _moves_consistent([x[:3] for x in cp[2]])
+
+
+def test_size_and_score_limits(ctx):
+    """The best cell travels as one 64-bit key (36-bit cell index, 28-bit
+    score): larger problems are refused before any allocation."""
+    import dpx
+    with pytest.raises(dpx.DpxError, match="2\\^36"):
+        ctx.swat("A" * 270000, "C" * 270000, 2, -3, -5, -2)
+    with pytest.raises(dpx.DpxError, match="scores too large"):
+        ctx.swat("A" * 1000, "A" * 1000, 1 << 20, -3, -5, -2)
