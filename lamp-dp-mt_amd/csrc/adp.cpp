@@ -64,6 +64,7 @@ struct JParser {
   const char* p;
   const char* base;
   std::string err;
+  int depth = 0;  // nesting bound: a hostile spec cannot exhaust the stack
   void ws() {
     while (*p == ' ' || *p == '\n' || *p == '\t' || *p == '\r') ++p;
   }
@@ -110,6 +111,12 @@ struct JParser {
   }
   bool val(J& v) {
     ws();
+    if (depth > 512) return fail("JSON nested too deeply");
+    struct Nest {
+      int& d;
+      explicit Nest(int& x) : d(x) { ++d; }
+      ~Nest() { --d; }
+    } nest(depth);
     if (*p == '{') {
       v.t = J::OBJ;
       ++p;
@@ -1146,7 +1153,8 @@ struct Grammar {
   hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
-Grammar* compile(const char* spec, bool load, std::string& err) {
+Grammar* compile(const char* spec, bool load, std::string& err, int* code) {
+  if (code) *code = DPX_EINVAL;
   if (!spec) { err = "null spec"; return nullptr; }
   std::unique_ptr<Grammar> g(new Grammar());
   J root;
@@ -1178,6 +1186,7 @@ Grammar* compile(const char* spec, bool load, std::string& err) {
   hiprtcGetCode(prog, g->code.data());
   hiprtcDestroyProgram(&prog);
   if (load) {
+    if (code) *code = DPX_EHIP;
     if (hipModuleLoadData(&g->mod, g->code.data()) != hipSuccess ||
         hipModuleGetFunction(&g->fill[0], g->mod, "adp_fill_1") != hipSuccess ||
         hipModuleGetFunction(&g->fill[1], g->mod, "adp_fill_8") != hipSuccess ||

@@ -19,8 +19,9 @@ namespace adp {
 
 struct Grammar;
 
-// load == false: generate and compile only (no device needed)
-Grammar* compile(const char* spec, bool load, std::string& err);
+// load == false: generate and compile only (no device needed); on failure
+// *code is DPX_EINVAL (spec, analysis, compiler) or DPX_EHIP (loading)
+Grammar* compile(const char* spec, bool load, std::string& err, int* code = nullptr);
 void destroy(Grammar* g);
 const std::string& source(const Grammar* g);
 void info(const Grammar* g, dpx_grammar_info* out);

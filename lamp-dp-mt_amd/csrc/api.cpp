@@ -1643,8 +1643,9 @@ int dpx_grammar_create(dpx_ctx* c, const char* spec, dpx_grammar** out) {
   *out = nullptr;
   std::lock_guard<std::mutex> lk(c->mu);
   if (hipSetDevice(c->device) != hipSuccess) return fail(c->err, DPX_EHIP, "hipSetDevice");
-  dpx::adp::Grammar* g = dpx::adp::compile(spec, true, c->err);
-  if (!g) return c->err.rfind("loading", 0) == 0 ? DPX_EHIP : DPX_EINVAL;
+  int code = DPX_EINVAL;
+  dpx::adp::Grammar* g = dpx::adp::compile(spec, true, c->err, &code);
+  if (!g) return code;
   dpx_grammar* h = new dpx_grammar();
   h->g = g;
   h->device = c->device;

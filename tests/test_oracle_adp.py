@@ -149,6 +149,8 @@ def test_library_rejects_unsupported():
     assert _check(json.dumps(bad))[0] != 0
     rc, _, msg = _check("{not json")
     assert rc != 0 and "JSON" in msg
+    rc, _, msg = _check("[" * 100000)  # nesting is bounded, not a stack overflow
+    assert rc != 0 and "deeply" in msg
     # a body that does not compile: the compiler log comes back
     bad = json.loads(g.spec_json())
     fn = sorted(bad["functions"])[0]
