@@ -629,6 +629,7 @@ struct AltCode {
   std::vector<Loop> loops;          // outer concatenation first
   std::vector<std::string> arith;   // index conditions
   std::vector<std::string> other;   // filters and VALID, tree order
+  std::vector<bool> other_free;     // may be evaluated before the split loops that do not bind its names
   std::vector<std::string> pos;     // split positions, left to right
   std::vector<std::string> var_of_pos;  // loop variable behind each position ("" if an expression)
   struct Kid { int tab; std::string a, b; };
@@ -663,7 +664,13 @@ struct Gen {
       case K_TAB: {
         const Tab& T = m.tabs[n->tab];
         if (m.tracks == 1) scs(T.minv, T.maxv, a, b, o);
-        if (!T.always_valid) o.other.push_back(valid(n->tab, a, b));
+        if (!T.always_valid) {
+          // guarded, so that it is safe wherever it is evaluated
+          const std::string g = m.tracks == 1 ? "(" + a + ")<=(" + b + ") && (" + b + ")<=j && "
+                                              : "(" + a + ")>=0 && (" + b + ")>=0 && ";
+          o.other.push_back("(" + g + valid(n->tab, a, b) + ")");
+          o.other_free.push_back(true);
+        }
         o.kids.push_back({n->tab, a, b});
         vals.push_back(cost(n->tab, a, b));
         return true;
@@ -686,6 +693,7 @@ struct Gen {
       }
       case K_FILTER:
         o.other.push_back(call(n->fn, {a, b}));
+        o.other_free.push_back(a == "i" && b == "j");  // a filter of the cell itself
         return gen(n->l.get(), a, b, o, vals);
       case K_MAP: {
         std::vector<std::string> in;
@@ -999,7 +1007,34 @@ bool generate(Model& m, std::string& src, std::string& err) {
         s << ind << "if (lane == 0) {\n";
         ++close;
       }
-      for (size_t q = 0; q < o.loops.size(); ++q) {
+      // each condition at the outermost loop level that binds its names
+      // (index conditions and the guarded table tests anywhere, filters of
+      // the cell itself before the loops, other filters innermost); pure, so
+      // only the work changes, as in the reference's emit (CodeGen.scala:162-184)
+      const int NL = (int)o.loops.size();
+      std::vector<std::vector<std::string>> at(NL + 1);
+      auto level = [&](const std::string& c) {
+        int lv = 0;
+        for (int q = 0; q < NL; ++q) {
+          const std::string& v = o.loops[q].var;
+          for (size_t pos = c.find(v); pos != std::string::npos; pos = c.find(v, pos + 1)) {
+            const char nx = pos + v.size() < c.size() ? c[pos + v.size()] : ' ';
+            const char pv = pos ? c[pos - 1] : ' ';
+            if (!isalnum((unsigned char)nx) && nx != '_' && !isalnum((unsigned char)pv) && pv != '_') { lv = q + 1; break; }
+          }
+        }
+        return lv;
+      };
+      for (auto& c : o.arith) at[level(c)].push_back(c);
+      for (size_t q = 0; q < o.other.size(); ++q) at[o.other_free[q] ? level(o.other[q]) : NL].push_back(o.other[q]);
+      auto open_if = [&](int lv) {
+        if (at[lv].empty()) return;
+        ind += "  ";
+        s << ind << "if (" << join(at[lv], " && ") << ") {\n";
+        ++close;
+      };
+      open_if(0);
+      for (int q = 0; q < NL; ++q) {
         const Loop& L = o.loops[q];
         ind += "  ";
         if (q == 0)
@@ -1009,11 +1044,9 @@ bool generate(Model& m, std::string& src, std::string& err) {
           s << ind << "for (int " << L.var << " = " << bound(L.lo, "max") << ", " << L.var << "_u = " << bound(L.hi, "min")
             << "; " << L.var << " <= " << L.var << "_u; ++" << L.var << ") {\n";
         ++close;
+        open_if(q + 1);
       }
-      std::vector<std::string> conds = o.arith;
-      conds.insert(conds.end(), o.other.begin(), o.other.end());
       ind += "  ";
-      if (!conds.empty()) { s << ind << "if (" << join(conds, " && ") << ") {\n"; ++close; ind += "  "; }
       s << ind << "const answer_t _c = " << o.value << ";\n"
         << ind << "if (" << better("_c", "_c0") << (m.empty.empty() ? " || _r0 == -1" : "") << ") { _c0 = _c; _r0 = " << o.rule
         << "; _p0 = " << (o.pos.size() > 0 ? o.pos[0] : "0") << "; _p1 = " << (o.pos.size() > 1 ? o.pos[1] : "0")
