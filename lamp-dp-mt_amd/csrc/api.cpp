@@ -104,7 +104,8 @@ struct Job {
   int g = G_ZUKER, count = 0, n_max = 0, min_loop = 0;
   bool want_trace = false;
   int64_t tiles = 0, trace_total = 0;
-  std::vector<TriInst> inst;
+  std::vector<TriInst> inst;  // longest first (TriInst::out is the caller's index)
+  std::vector<int64_t> trace_off_by_out;
   std::vector<unsigned long long> seeds;
   std::vector<uint8_t> data;  // codes (RNA) or rows|cols (int32, matrix chain)
   // Smith-Waterman
@@ -322,13 +323,15 @@ int tri_prepare(std::string& err, Job& j, int g, const std::vector<int>& lens, c
     j.cells += tri_cells(t.n);
   }
   if (j.tiles >= ((int64_t)1 << 31)) return fail(err, DPX_ENOMEM, "too many tiles in one launch");
-  // seed the longest instances first: their chain of tile diagonals is the longest
-  std::vector<int> order(count);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return lens[x] > lens[y]; });
+  // longest instances first: their chain of tile diagonals is the longest,
+  // so they are seeded first, and the finalize launch (one workgroup per
+  // instance, in array order) starts them first too
+  j.trace_off_by_out.resize(count);
+  for (int q = 0; q < count; ++q) j.trace_off_by_out[q] = j.inst[q].trace_off;
+  std::stable_sort(j.inst.begin(), j.inst.end(), [](const TriInst& x, const TriInst& y) { return x.n > y.n; });
   j.seeds.clear();
   j.seeds.reserve(j.tiles);
-  for (int q : order)
+  for (int q = 0; q < count; ++q)
     for (int I = 0; I < j.inst[q].nb; ++I) j.seeds.push_back(pack_item(q, I, I));
   j.dev_bytes = tri_layout(g, j.tiles, count, j.data.size(), j.trace_total, j.off);
   return DPX_OK;
@@ -612,7 +615,7 @@ int tri_copy_out(std::string& err, const Job& j, int32_t* results, dpx_trace_t* 
     trace_lens[q] = tl;
     if (tl < 0) { rc = fail(err, DPX_ETRACE, "backtrace failed"); continue; }
     if (tl > cap) { rc = fail(err, DPX_ETRACE, "trace buffer too small"); continue; }
-    std::memcpy(traces + trace_offs[q], j.tr + j.inst[q].trace_off, sizeof(dpx_trace_t) * tl);
+    std::memcpy(traces + trace_offs[q], j.tr + j.trace_off_by_out[q], sizeof(dpx_trace_t) * tl);
   }
   return rc;
 }
@@ -840,7 +843,7 @@ void harvest_slot(dpx_server* S, Slot& sl, int status) {
       } else {
         f.r.score = j.res[q];
         tl = j.want_trace ? j.lens[q] : 0;
-        src = j.tr + j.inst[q].trace_off;
+        src = j.tr + j.trace_off_by_out[q];
       }
       f.r.trace_len = tl;
       if (tl < 0) f.r.status = DPX_ETRACE;
