@@ -1277,7 +1277,8 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
   const Model& m = g->m;
   if (!g->mod) { err = "grammar was compiled without loading"; return DPX_EINVAL; }
   if (m.rna && (!rna || !hl)) { err = "grammar calls librna functions: load an energy parameter file first"; return DPX_ENOPARAMS; }
-  if (count <= 0 || !off1 || (m.tracks == 2 && !off2) || (tr && !tr_off)) { err = "bad batch arguments"; return DPX_EINVAL; }
+  if (count == 0) return DPX_OK;
+  if (count < 0 || !off1 || (m.tracks == 2 && !off2) || (tr && !tr_off)) { err = "bad batch arguments"; return DPX_EINVAL; }
   const int NT = (int)m.tabs.size();
   const size_t ab = (size_t)m.alphabet_bytes, ans4 = ((size_t)m.answer_bytes + 3) & ~(size_t)3;
   const size_t inst_bytes = 72 + 32 * (size_t)NT;
