@@ -71,7 +71,8 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     uint32_t* btb = A.bt + (size_t)band * A.nwords * 64;
     int2* cpb = (A.cp && (band + 1) % A.cp_every == 0) ? A.cp + (size_t)((band + 1) / A.cp_every) * W : nullptr;
     int Mc = 0, G1c = 0, G2c = 0, upPrev = 0;
-    int best = -1, bestj = 0;
+    int best = row_ok ? -1 : 0x7fffffff, bestj = 0;  // rows past n1 never improve
+    const int jstar = row_ok && i == A.n1 ? A.n2 : -0x40000000;  // column of M(n1, n2) on this lane
     int known = 0;  // columns of `above` known to be published
     int chM = 0, chG = 0, chB = 0;  // chunk of the row above and of b, one column per lane
     int bj = 0;  // b[j-1] of this lane's column j = t - lane, shifted in from lane 0
@@ -147,11 +148,10 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
           Mc = M;
           G1c = g1;
           G2c = g2;
-          const bool act = row_ok && j >= 0 && j <= A.n2;
-          if (act && M > best) { best = M; bestj = j; }
+          if ((unsigned)j <= (unsigned)A.n2 && M > best) { best = M; bestj = j; }
+          if (j == jstar) outv = M;
           const int lastM = __builtin_amdgcn_readlane(M, 63), lastG = __builtin_amdgcn_readlane(g1, 63);
           if (lane == tc + u) { sM = lastM; sG = lastG; }
-          if (act && i == A.n1 && j == A.n2) outv = M;
           word |= (uint32_t)(rm | (r1 << 2) | (r2 << 3)) << (4 * u);
         }
         if (CODES) btb[(size_t)((t0 + tc) >> 3) * 64 + lane] = word;
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
                              __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    if (row_ok && i == A.n1) A.out[0] = outv;
+    if (jstar >= 0) A.out[0] = outv;
     // best cell of this band: largest M, then smallest (i, j)
     if (row_ok && best >= 0) {
       // score in the top 28 bits, then the complement of the 36-bit linear
