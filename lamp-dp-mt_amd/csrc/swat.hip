@@ -70,7 +70,11 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     const unsigned long long tag_above = (unsigned long long)band << 32;  // written by band - 1
     uint32_t* btb = A.bt + (size_t)band * A.nwords * 64;
     int2* cpb = (A.cp && (band + 1) % A.cp_every == 0) ? A.cp + (size_t)((band + 1) / A.cp_every) * W : nullptr;
-    int Mc = 0, G1c = 0, G2c = 0, upPrev = 0;
+    // column -1 acts as -infinity for M and g2 (NEG + score stays far below
+    // 0 and never wraps): then column 0 needs no case of its own -- g2 = 0,
+    // M = g1 (rule 2) -- and the trace reads rule 7 for g2 there by itself
+    constexpr int NEG = -0x20000000;
+    int Mc = NEG, G1c = 0, G2c = NEG, upPrev = NEG;
     int best = row_ok ? -1 : 0x7fffffff, bestj = 0;  // rows past n1 never improve
     const int jstar = row_ok && i == A.n1 ? A.n2 : -0x40000000;  // column of M(n1, n2) on this lane
     int known = 0;  // columns of `above` known to be published
@@ -128,26 +132,25 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
           const int upM = sw_shr1(hM, Mc);
           const int upG = sw_shr1(hG, G1c);
           const int j = t - lane;
-          // g2 (j == 0: rule 7)
+          // g2 (j == 0: both candidates are 0, rule 7 is the trace's)
           const int o = max(0, Mc + A.open), e = max(0, G2c + A.ext);
-          const int r2 = j != 0 && e > o;
-          const int g2 = j == 0 ? 0 : (e > o ? e : o);
+          const int r2 = e > o;
+          const int g2 = r2 ? e : o;
           // g1
           const int o1 = max(0, upM + A.open), e1 = max(0, upG + A.ext);
           const int r1 = e1 > o1;
           const int g1 = r1 ? e1 : o1;
-          // M (j == 0: g1, rule 2)
+          // M (j == 0: the diagonal comes from column -1, so g1 wins, rule 2)
           const int dg = upPrev + (ai == bj ? A.match : A.mismatch);
-          int M = j == 0 ? g1 : dg, rm = j == 0 ? 2 : 1;
-          if (j != 0 && g1 > M) { M = g1; rm = 2; }
+          int M = dg, rm = 1;
+          if (g1 > M) { M = g1; rm = 2; }
           if (g2 > M) { M = g2; rm = 3; }
           upPrev = upM;
-          // The state of lanes outside the matrix (j < 0, j > n2, rows past n1)
-          // is never read by a valid cell: column 0 uses neither M(i,j-1) nor
-          // g2(i,j-1), and invalid rows only feed rows below them.
-          Mc = M;
+          // Lanes left of the matrix (j < 0) hold column -1: -infinity for M
+          // and g2; lanes past n2 and rows past n1 are never read by a valid cell.
+          Mc = j < 0 ? NEG : M;
           G1c = g1;
-          G2c = g2;
+          G2c = j < 0 ? NEG : g2;
           if ((unsigned)j <= (unsigned)A.n2 && M > best) { best = M; bestj = j; }
           if (j == jstar) outv = M;
           const int lastM = __builtin_amdgcn_readlane(M, 63), lastG = __builtin_amdgcn_readlane(g1, 63);
