@@ -334,30 +334,27 @@ struct GNu {
     const int dlo = I == J ? 0 : -(T - 1);
     for (int ld = dlo; ld <= T - 1; ++ld) {
       const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
-      // phase A: split points inside blocks I and J, 4 threads per cell
-      if (tid < ncell * 4) {
-        const int p = tid >> 2, part = tid & 3, r = rmin + p, c = r + ld, i = IT + r, j = JT + c;
+      // phase A: split points inside blocks I and J, 8 lanes per cell (a
+      // group of 8 in one wave), each lane a strided share of both ranges
+      // with independent loads, then a max over the group
+      if (tid < ncell * 8) {
+        const int p = tid >> 3, part = tid & 7, r = rmin + p, c = r + ld, i = IT + r, j = JT + c;
+        int best = NEG;
         if (i < j && j <= n) {
           int k1lo, k1hi, k2lo, k2hi;
           if (I == J) { k1lo = i + 1; k1hi = j - 1; k2lo = 1; k2hi = 0; }
           else { k1lo = i + 1; k1hi = IT + T - 1; k2lo = JT; k2hi = j - 1; }
-          const int n1 = imax(0, k1hi - k1lo + 1), n2 = imax(0, k2hi - k2lo + 1), len = n1 + n2;
-          const int b0 = part * len / 4, b1 = (part + 1) * len / 4;
-          int best = NEG;
-          for (int x = b0; x < b1; ++x) {
-            int v;
-            if (x < n1) {
-              const int k = k1lo + x;
-              const int left = I == J ? sm.u.w.S[r][k - JT + 1] : sm.u.w.L[r][k - IT];
-              v = left + sm.u.w.S[k - IT][c + 1];
-            } else {
-              const int k = k2lo + (x - n1);
-              v = sm.u.w.S[r][k - JT + 1] + sm.u.w.R[k - JT][c];
-            }
-            best = imax(best, v);
+          if (I == J) {
+            for (int k = k1lo + part; k <= k1hi; k += 8) best = imax(best, sm.u.w.S[r][k - JT + 1] + sm.u.w.S[k - IT][c + 1]);
+          } else {
+            for (int k = k1lo + part; k <= k1hi; k += 8) best = imax(best, sm.u.w.L[r][k - IT] + sm.u.w.S[k - IT][c + 1]);
+            for (int k = k2lo + part; k <= k2hi; k += 8) best = imax(best, sm.u.w.S[r][k - JT + 1] + sm.u.w.R[k - JT][c]);
           }
-          if (best > NEG) atomicMax(&sm.X[r][c], best);
         }
+        best = imax(best, __shfl_xor(best, 1));
+        best = imax(best, __shfl_xor(best, 2));
+        best = imax(best, __shfl_xor(best, 4));
+        if (part == 0 && best > NEG) sm.X[r][c] = imax(sm.X[r][c], best);  // one writer per cell
       }
       __syncthreads();
       // phase B: one thread per cell
@@ -430,30 +427,32 @@ struct GMc {
     const int dlo = I == J ? 0 : -(T - 1);
     for (int ld = dlo; ld <= T - 1; ++ld) {
       const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
-      if (tid < ncell * 4) {
-        const int p = tid >> 2, part = tid & 3, r = rmin + p, c = r + ld, i = IT + r, j = JT + c;
-        if (i + 1 < j && j <= n) {
-          int k1lo, k1hi, k2lo, k2hi;
-          if (I == J) { k1lo = i + 1; k1hi = j - 1; k2lo = 1; k2hi = 0; }
-          else { k1lo = i + 1; k1hi = IT + T - 1; k2lo = JT; k2hi = j - 1; }
-          const int n1 = imax(0, k1hi - k1lo + 1), n2 = imax(0, k2hi - k2lo + 1), len = n1 + n2;
-          const int b0 = part * len / 4, b1 = (part + 1) * len / 4;
+      // split points inside blocks I and J: 8 lanes per cell, strided
+      // shares with independent loads, min over the group (int32 wrap: the
+      // minimum of the wrapped candidates, as the sequential loop keeps)
+      if (tid < ncell * 8) {
+        const int p = tid >> 3, part = tid & 7, r = rmin + p, c = r + ld, i = IT + r, j = JT + c;
+        int best = 0x7fffffff;
+        const bool live = i + 1 < j && j <= n;
+        if (live) {
           const uint32_t alpha = (uint32_t)rows[i] * (uint32_t)cols[j - 1];
-          int best = 0x7fffffff;
-          for (int x = b0; x < b1; ++x) {
-            uint32_t v;
-            if (x < n1) {
-              const int k = k1lo + x;
-              const int left = I == J ? sm.u.w.S[r][k - JT] : sm.u.w.L[r][k - IT];
-              v = (uint32_t)left + (uint32_t)sm.u.w.S[k - IT][c] + alpha * (uint32_t)sm.u.w.rows[k - IT];
-            } else {
-              const int k = k2lo + (x - n1);
-              v = (uint32_t)sm.u.w.S[r][k - JT] + (uint32_t)sm.u.w.R[k - JT][c] + alpha * (uint32_t)sm.u.w.rows[T + k - JT];
-            }
-            best = imin(best, (int)v);
+          if (I == J) {
+            for (int k = i + 1 + part; k <= j - 1; k += 8)
+              best = imin(best, (int)((uint32_t)sm.u.w.S[r][k - JT] + (uint32_t)sm.u.w.S[k - IT][c] +
+                                      alpha * (uint32_t)sm.u.w.rows[k - IT]));
+          } else {
+            for (int k = i + 1 + part; k <= IT + T - 1; k += 8)
+              best = imin(best, (int)((uint32_t)sm.u.w.L[r][k - IT] + (uint32_t)sm.u.w.S[k - IT][c] +
+                                      alpha * (uint32_t)sm.u.w.rows[k - IT]));
+            for (int k = JT + part; k <= j - 1; k += 8)
+              best = imin(best, (int)((uint32_t)sm.u.w.S[r][k - JT] + (uint32_t)sm.u.w.R[k - JT][c] +
+                                      alpha * (uint32_t)sm.u.w.rows[T + k - JT]));
           }
-          if (b1 > b0) atomicMin(&sm.X[r][c], best);
         }
+        best = imin(best, __shfl_xor(best, 1));
+        best = imin(best, __shfl_xor(best, 2));
+        best = imin(best, __shfl_xor(best, 4));
+        if (part == 0 && live) sm.X[r][c] = imin(sm.X[r][c], best);  // one writer per cell
       }
       __syncthreads();
       if (tid < ncell) {
