@@ -354,20 +354,18 @@ struct GNu {
         best = imax(best, __shfl_xor(best, 1));
         best = imax(best, __shfl_xor(best, 2));
         best = imax(best, __shfl_xor(best, 4));
-        if (part == 0 && best > NEG) sm.X[r][c] = imax(sm.X[r][c], best);  // one writer per cell
-      }
-      __syncthreads();
-      // phase B: one thread per cell
-      if (tid < ncell) {
-        const int r = rmin + tid, c = r + ld, i = IT + r, j = JT + c;
-        int v = NEG;
-        if (i <= j && j <= n) {
-          v = sm.X[r][c];
-          if (i == j) v = imax(v, 0);
-          if (i + 1 <= j) { v = imax(v, sm.u.w.S[r + 1][c + 1]); v = imax(v, sm.u.w.S[r][c]); }
-          if (i + 2 + a.min_loop <= j && nu_pair(s[i], s[j - 1])) v = imax(v, sm.u.w.S[r + 1][c] + 1);
+        // phase B on the group's first lane: the cell's other rules, then its
+        // value (the cells of this diagonal read only earlier diagonals)
+        if (part == 0) {
+          int v = NEG;
+          if (i <= j && j <= n) {
+            v = imax(sm.X[r][c], best);
+            if (i == j) v = imax(v, 0);
+            if (i + 1 <= j) { v = imax(v, sm.u.w.S[r + 1][c + 1]); v = imax(v, sm.u.w.S[r][c]); }
+            if (i + 2 + a.min_loop <= j && nu_pair(s[i], s[j - 1])) v = imax(v, sm.u.w.S[r + 1][c] + 1);
+          }
+          sm.u.w.S[r][c + 1] = v;
         }
-        sm.u.w.S[r][c + 1] = v;
       }
       __syncthreads();
     }
@@ -452,15 +450,9 @@ struct GMc {
         best = imin(best, __shfl_xor(best, 1));
         best = imin(best, __shfl_xor(best, 2));
         best = imin(best, __shfl_xor(best, 4));
-        if (part == 0 && live) sm.X[r][c] = imin(sm.X[r][c], best);  // one writer per cell
-      }
-      __syncthreads();
-      if (tid < ncell) {
-        const int r = rmin + tid, c = r + ld, i = IT + r, j = JT + c;
-        int v = 0;
-        if (i + 1 == j && j <= n) v = 0;
-        else if (i + 1 < j && j <= n) v = sm.X[r][c];
-        sm.u.w.S[r][c] = v;
+        // the cell's value on the group's first lane (this diagonal's cells
+        // read only earlier diagonals): M(i,i+1) = 0, else the minimum
+        if (part == 0) sm.u.w.S[r][c] = live ? imin(sm.X[r][c], best) : 0;
       }
       __syncthreads();
     }
