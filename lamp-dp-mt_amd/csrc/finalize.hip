@@ -289,17 +289,17 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
       const int r = pass * 8 + (tid >> 5), i = lo + r;
       int m = INV;
       if (i <= hi) {
-        // eight split points per lane in flight at a time
+        // sixteen split points per lane in flight at a time
         const int k0 = fmax_(hi + 1, i + 7) + lane;
-        for (int kb = k0; kb <= n; kb += 8 * 32) {
-          int u[8];
+        for (int kb = k0; kb <= n; kb += 16 * 32) {
+          int u[16];
 #pragma unroll
-          for (int x = 0; x < 8; ++x) {
+          for (int x = 0; x < 16; ++x) {
             const int k = kb + 32 * x;
             u[x] = k <= n ? C.get(ZT_U, i, k) : INV;
           }
 #pragma unroll
-          for (int x = 0; x < 8; ++x) {
+          for (int x = 0; x < 16; ++x) {
             const int k = kb + 32 * x;
             if (u[x] < INV_THR) m = fmin_(m, u[x] - mlint + st[k]);
           }
