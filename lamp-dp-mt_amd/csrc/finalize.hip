@@ -68,16 +68,16 @@ __device__ int wave_first_match(int ncand, int target, Fn&& cand) {
 
 // BFS backtrace driver.  The reference takes its queued records one at a
 // time (zu.c:49-93, CodeGen.scala:233-252); a record's rule depends only on
-// its own cell, so each round here resolves up to NWAVE queued records, one
+// its own cell, so each round here resolves up to NW queued records, one
 // per wave, and then appends their children in record order: the same
 // records, in the same order.  rule_of(rec, &rule, &p0, &p1) runs on one
 // wave and returns false when no candidate matches; kids(rec, push) appends
 // the children of a resolved record.  *len receives the record count, or -1
 // (no match, or more records than cap).
-template <class RuleOf, class Kids>
+template <int NW, class RuleOf, class Kids>
 __device__ void bfs_rounds(TraceRec* tr, int cap, TraceRec root, RuleOf&& rule_of, Kids&& kids, int32_t* len) {
   __shared__ int s_rd, s_wr, s_over;
-  __shared__ TraceRec s_rec[NWAVE];
+  __shared__ TraceRec s_rec[NW];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (threadIdx.x == 0) {
     s_rd = 0;
@@ -94,7 +94,7 @@ __device__ void bfs_rounds(TraceRec* tr, int cap, TraceRec root, RuleOf&& rule_o
   for (;;) {
     const int rd = s_rd, wr = s_wr;
     if (rd >= wr || s_over) break;
-    const int m = wr - rd < NWAVE ? wr - rd : NWAVE;
+    const int m = wr - rd < NW ? wr - rd : NW;
     if (w < m) {
       TraceRec cur = tr[rd + w];
       int rule = -1, p0 = 0, p1 = 0;
@@ -128,7 +128,8 @@ __device__ void bfs_rounds(TraceRec* tr, int cap, TraceRec root, RuleOf&& rule_o
 }
 
 // ----------------------------------------------------------------- Nussinov
-__global__ __launch_bounds__(NTHR) void nu_finalize(TriArgs a) {
+template <int NT>
+__global__ __launch_bounds__(NT) void nu_finalize(TriArgs a) {
   const TriInst in = a.inst[blockIdx.x];
   const Cells C{a.tab, in.nb, NT_NU, in.tile0};
   const uint8_t* s = a.codes + in.seq_off;
@@ -169,12 +170,13 @@ __global__ __launch_bounds__(NTHR) void nu_finalize(TriArgs a) {
       default: break;
     }
   };
-  bfs_rounds((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
+  bfs_rounds<NT / 64>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
              a.trace_lens + in.out);
 }
 
 // ------------------------------------------------------------- matrix chain
-__global__ __launch_bounds__(NTHR) void mc_finalize(TriArgs a) {
+template <int NT>
+__global__ __launch_bounds__(NT) void mc_finalize(TriArgs a) {
   const TriInst in = a.inst[blockIdx.x];
   const Cells C{a.tab, in.nb, NT_MC, in.tile0};
   const int32_t* rows = a.dims + in.seq_off;
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(NTHR) void mc_finalize(TriArgs a) {
       push(e.pos[0], e.j, 0);
     }
   };
-  bfs_rounds((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
+  bfs_rounds<NT / 64>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
              a.trace_lens + in.out);
 }
 
@@ -267,7 +269,8 @@ struct ZuF {
   }
 };
 
-__global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
+template <int NT>
+__global__ __launch_bounds__(NT) void zu_finalize(TriArgs a) {
   extern __shared__ int st[];  // st(i,n), i = 0..n
   const TriInst in = a.inst[blockIdx.x];
   const Cells C{a.tab, in.nb, NT_ZU, in.tile0};
@@ -285,8 +288,8 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
   const int tid = threadIdx.x, lane = tid & 31;
   for (int hi = n - 1; hi >= 0; hi -= T) {
     const int lo = fmax_(0, hi - T + 1);
-    for (int pass = 0; pass < T / 8; ++pass) {
-      const int r = pass * 8 + (tid >> 5), i = lo + r;
+    for (int pass = 0; pass < T * 32 / NT; ++pass) {  // NT / 32 rows at a time, 32 lanes each
+      const int r = pass * (NT / 32) + (tid >> 5), i = lo + r;
       int m = INV;
       if (i <= hi) {
         // sixteen split points per lane in flight at a time
@@ -308,7 +311,7 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
       for (int off = 16; off >= 1; off >>= 1) m = fmin_(m, __shfl_xor(m, off));
       if (lane == 0) far[r] = m;
     }
-    for (int e = tid; e < T * T; e += NTHR) {
+    for (int e = tid; e < T * T; e += NT) {
       const int r = e / T, c = e % T, i = lo + r, k = lo + c;
       int v = INV;
       if (i <= hi && k <= hi && k >= i + 7) {
@@ -471,7 +474,7 @@ __global__ __launch_bounds__(NTHR) void zu_finalize(TriArgs a) {
       default: break;
     }
   };
-  bfs_rounds((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 12, {0, 0}}, rule_of, kids,
+  bfs_rounds<NT / 64>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 12, {0, 0}}, rule_of, kids,
              a.trace_lens + in.out);
 }
 
@@ -494,7 +497,8 @@ __device__ int rf_il(const ZuF& z, int i, int k, int l, int j) {
 //   Qp: hairpin [5], stack [6], interior loops with l outer ascending from
 //       max(i+4, j-30) to j-1 and k inner from i+1 to min(i+30, l-3) [7],
 //       multiloop QM(i+1, j-1) [8]
-__global__ __launch_bounds__(NTHR) void rf_finalize(TriArgs a) {
+template <int NT>
+__global__ __launch_bounds__(NT) void rf_finalize(TriArgs a) {
   const TriInst in = a.inst[blockIdx.x];
   const Cells C{a.tab, in.nb, NT_RF, in.tile0};
   const int n = in.n;
@@ -581,7 +585,7 @@ __global__ __launch_bounds__(NTHR) void rf_finalize(TriArgs a) {
       default: break;
     }
   };
-  bfs_rounds((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
+  bfs_rounds<NT / 64>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
              a.trace_lens + in.out);
 }
 
@@ -589,10 +593,20 @@ __global__ __launch_bounds__(NTHR) void rf_finalize(TriArgs a) {
 int launch_finalize(int grammar, const TriArgs& a, int count, int n_max, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   if (count <= 0) return 0;
-  if (grammar == G_NUSSINOV) hipLaunchKernelGGL(nu_finalize, dim3(count), dim3(NTHR), 0, st, a);
-  else if (grammar == G_MATCHAIN) hipLaunchKernelGGL(mc_finalize, dim3(count), dim3(NTHR), 0, st, a);
-  else if (grammar == G_RNAFOLD) hipLaunchKernelGGL(rf_finalize, dim3(count), dim3(NTHR), 0, st, a);
-  else hipLaunchKernelGGL(zu_finalize, dim3(count), dim3(NTHR), (size_t)(n_max + 1) * sizeof(int), st, a);
+  // a few instances: 16 waves each (more backtrace records per round, more
+  // st split points in flight); a batch: 4 waves each, several per CU
+  const bool big = count <= 32;
+  const size_t lds = grammar == G_ZUKER ? (size_t)(n_max + 1) * sizeof(int) : 0;
+#define DPX_FIN(K)                                                          \
+  do {                                                                      \
+    if (big) hipLaunchKernelGGL(K<1024>, dim3(count), dim3(1024), lds, st, a); \
+    else hipLaunchKernelGGL(K<NTHR>, dim3(count), dim3(NTHR), lds, st, a);     \
+  } while (0)
+  if (grammar == G_NUSSINOV) DPX_FIN(nu_finalize);
+  else if (grammar == G_MATCHAIN) DPX_FIN(mc_finalize);
+  else if (grammar == G_RNAFOLD) DPX_FIN(rf_finalize);
+  else DPX_FIN(zu_finalize);
+#undef DPX_FIN
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
