@@ -274,3 +274,20 @@ def test_rnafold_grammar_vs_builtin(ctx):
         ans, found, tr = cg.parse(seq)
         assert found and ans == fx["score"] and [list(t) for t in tr] == fx["trace"]
 
+
+
+def test_repeated_shapes(ctx):
+    """Different inputs of one shape through one compiled grammar, one after
+    the other (the tables and instance records are reused), each equal to
+    the built-in kernel."""
+    from dpx import gen
+    sw = ctx.grammar(G.swat_affine())
+    for seed in range(4):
+        a, b = gen.dna_pair(300, 280, 500 + seed)
+        ans, found, tr = sw.parse(a, b)
+        s, _, btr = ctx.swat(a, b, 2, -3, -5, -2)
+        assert found and ans == s and [t[:3] for t in tr] == [t[:3] for t in btr], seed
+    nu = ctx.grammar(G.nussinov())
+    for seed in range(3):
+        s = gen.rna_uniform(400, 900 + seed)
+        assert nu.parse(s)[::2] == ctx.nussinov(s, min_loop=0), seed
