@@ -260,8 +260,8 @@ int dpx_server_get_stats(dpx_server* srv, dpx_server_stats* out);
  * is a JSON tree of the reference's combinators (BaseParsers.scala,
  * ADPParsers.scala, TTParsers.scala):
  *   {"tracks": 1|2, "alphabet": C type, "alphabet_bytes": n, "answer": C type,
- *    "answer_bytes": n, "header": C declarations, "h": "max"|"min"|
- *    {"maxBy": fn}|{"minBy": fn}, "empty": C literal (optional, the
+ *    "answer_bytes": n, "header": C declarations, "h": "max"|"min"|"sum"|
+ *    "count"|{"maxBy": fn}|{"minBy": fn}, "empty": C literal (optional, the
  *    reference's cudaEmpty), "window": w (single track), "axiom": name,
  *    "functions": {name: {"args": "a,b", "body": "return a+b;"}},
  *    "tabulations": {name: {"always_valid": bool, "p": parser}}}
@@ -278,7 +278,10 @@ int dpx_server_get_stats(dpx_server* srv, dpx_server_stats* out);
  * sr_energy, termau_energy, ext_mismatch_energy, ml_mismatch_energy,
  * ml_energy, ul_energy, bp_index/NO_BP) over the context's parameter file.  Rule ids, split positions, the backtrace order and
  * ties (first strictly better candidate in loop order) are the reference's
- * generated code's; trace records hold up to 2 split positions. */
+ * generated code's; trace records hold up to 2 split positions.  "sum" and
+ * "count" (BaseParsers.scala:385-392) add up a cell's candidates (count:
+ * their number) and have no backtrace: answer and found flag only, trace
+ * length 0 (BaseParsers.scala:150). */
 typedef struct dpx_grammar dpx_grammar;
 typedef struct {
   int32_t tracks, n_tabs, n_rules, max_pos, axiom_id, window;

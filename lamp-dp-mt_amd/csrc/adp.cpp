@@ -19,9 +19,13 @@
 //    queue of (i, j, tabulation id) records rewritten to (i, j, rule, pos);
 //  * window: CodeGen.scala:325-348 (best axiom cell (i, i+window) by the
 //    signature's h, first one on ties).
+//  * sum / count (BaseParsers.scala:145-153,385-392): the cell's value is
+//    the sum of its candidates' values (count: their number), a cell with no
+//    candidate has none; there is no backtrace (no candidate matches a sum,
+//    BaseParsers.scala:150), so those grammars return answers only.
 // Not supported (rejected with a message): nested aggregations (an
-// aggregate below a concatenation or a map), sum/count aggregations, more
-// than two split positions per tabulation.
+// aggregate below a concatenation or a map), more than two split positions
+// per tabulation.
 #include "adp.h"
 
 #include <hip/hiprtc.h>
@@ -173,7 +177,7 @@ constexpr int MAX_POS = 2;
 
 enum Kind { K_TAB, K_TERM, K_OR, K_MAP, K_FILTER, K_CAT, K_AGG };
 enum Term { T_EMPTY, T_EMPTYI, T_EL, T_ELI, T_SEQ, T_EMPTY1, T_EMPTY2, T_EL1, T_EL2 };
-enum AggKind { A_MAX, A_MIN, A_MAXBY, A_MINBY };
+enum AggKind { A_MAX, A_MIN, A_MAXBY, A_MINBY, A_SUM, A_COUNT };
 
 struct Agg {
   AggKind k = A_MAX;
@@ -216,6 +220,21 @@ struct Model {
   std::string alphabet = "char", answer = "int", header, empty;
   int alphabet_bytes = 1, answer_bytes = 4;
   Agg h;
+  // sum / count: candidates are added up instead of compared
+  bool additive() const { return h.k == A_SUM || h.k == A_COUNT; }
+  // integer answers add up to the same value in any order (wrapping), so a
+  // cell's candidates may be split over lanes; other types keep the
+  // sequential order on one lane
+  bool any_order() const {
+    static const char* const ints[] = {"int", "unsigned", "unsigned int", "long", "long int", "long long", "unsigned long",
+                                       "unsigned long long", "short", "unsigned short", "char", "signed char",
+                                       "unsigned char", "int8_t", "int16_t", "int32_t", "int64_t", "uint8_t", "uint16_t",
+                                       "uint32_t", "uint64_t", "size_t"};
+    if (!additive()) return true;
+    for (const char* t : ints)
+      if (answer == t) return true;
+    return false;
+  }
   std::vector<Fn> fns;
   std::map<std::string, int> fn_ix;
   std::vector<Tab> tabs;
@@ -329,7 +348,9 @@ struct Builder {
     if (j->t == J::STR) {
       if (j->s == "max") { a.k = A_MAX; return true; }
       if (j->s == "min") { a.k = A_MIN; return true; }
-      return fail("unsupported aggregation '" + j->s + "' (max, min, maxBy, minBy)");
+      if (j->s == "sum") { a.k = A_SUM; return true; }
+      if (j->s == "count") { a.k = A_COUNT; return true; }
+      return fail("unsupported aggregation '" + j->s + "' (max, min, sum, count, maxBy, minBy)");
     }
     if (j->t == J::OBJ) {
       const J* f = j->get("maxBy");
@@ -973,6 +994,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
       case A_MIN: cmp = c + "<" + c0; break;
       case A_MAXBY: cmp = "f_" + m.fns[m.h.fn].name + "(_x," + c + ")>f_" + m.fns[m.h.fn].name + "(_x," + c0 + ")"; break;
       case A_MINBY: cmp = "f_" + m.fns[m.h.fn].name + "(_x," + c + ")<f_" + m.fns[m.h.fn].name + "(_x," + c0 + ")"; break;
+      case A_SUM: case A_COUNT: cmp = "false"; break;
     }
     return cmp;
   };
@@ -1047,11 +1069,19 @@ bool generate(Model& m, std::string& src, std::string& err) {
         open_if(q + 1);
       }
       ind += "  ";
-      s << ind << "const answer_t _c = " << o.value << ";\n"
-        << ind << "if (" << better("_c", "_c0") << (m.empty.empty() ? " || _r0 == -1" : "") << ") { _c0 = _c; _r0 = " << o.rule
-        << "; _p0 = " << (o.pos.size() > 0 ? o.pos[0] : "0") << "; _p1 = " << (o.pos.size() > 1 ? o.pos[1] : "0")
-        << "; _o0 = " << (o.loops.size() > 0 ? o.loops[0].var : "0") << "; _o1 = " << (o.loops.size() > 1 ? o.loops[1].var : "0")
-        << "; }\n";
+      const std::string take = "_r0 = " + std::to_string(o.rule) + "; _p0 = " + (o.pos.size() > 0 ? o.pos[0] : "0") +
+                               "; _p1 = " + (o.pos.size() > 1 ? o.pos[1] : "0") + "; _o0 = " +
+                               (o.loops.size() > 0 ? o.loops[0].var : "0") + "; _o1 = " +
+                               (o.loops.size() > 1 ? o.loops[1].var : "0") + ";";
+      if (m.h.k == A_SUM)  // the first candidate keeps the back-pointer
+        s << ind << "const answer_t _c = " << o.value << ";\n"
+          << ind << "if (_r0 == -1) { _c0 = answer_t(answer_t(0) + _c); " << take << " } else _c0 = answer_t(_c0 + _c);\n";
+      else if (m.h.k == A_COUNT)
+        s << ind << "if (_r0 == -1) { _c0 = answer_t(1); " << take << " } else _c0 = answer_t(_c0 + 1);\n";
+      else
+        s << ind << "const answer_t _c = " << o.value << ";\n"
+          << ind << "if (" << better("_c", "_c0") << (m.empty.empty() ? " || _r0 == -1" : "") << ") { _c0 = _c; " << take
+          << " }\n";
       for (; close > 0; --close) {
         ind.resize(ind.size() - 2);
         s << ind << "}\n";
@@ -1063,10 +1093,18 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "    const int o0 = __shfl_xor(_o0, off, G), o1 = __shfl_xor(_o1, off, G);\n"
          "    bool take;\n"
          "    if (r == -1) take = false;\n"
-         "    else if (_r0 == -1) take = true;\n"
-      << "    else if (" << better("c", "_c0") << ") take = true;\n"
-      << "    else if (" << better("_c0", "c") << ") take = false;\n"
-      << "    else take = r < _r0 || (r == _r0 && (o0 < _o0 || (o0 == _o0 && o1 < _o1)));\n"
+         "    else if (_r0 == -1) take = true;\n";
+    if (m.additive())  // both lanes hold candidates: add them up, keep the first back-pointer
+      s << "    else {\n"
+           "      _c0 = answer_t(_c0 + c);\n"
+           "      if (r < _r0 || (r == _r0 && (o0 < _o0 || (o0 == _o0 && o1 < _o1)))) { _r0 = r; _p0 = p0; _p1 = p1; _o0 = o0; _o1 = o1; }\n"
+           "      take = false;\n"
+           "    }\n";
+    else
+      s << "    else if (" << better("c", "_c0") << ") take = true;\n"
+        << "    else if (" << better("_c0", "c") << ") take = false;\n"
+        << "    else take = r < _r0 || (r == _r0 && (o0 < _o0 || (o0 == _o0 && o1 < _o1)));\n";
+    s <<
          "    if (take) { _c0 = c; _r0 = r; _p0 = p0; _p1 = p1; _o0 = o0; _o1 = o1; }\n"
          "  }\n"
          "  (void)_o0; (void)_o1;\n"
@@ -1119,7 +1157,9 @@ bool generate(Model& m, std::string& src, std::string& err) {
       << "  for (int i = 0, j = " << m.window << "; j <= z.n1; ++i, ++j) {\n";
     if (!AX.always_valid) s << "    if (!" << Gen(m).valid(m.axiom, "i", "j") << ") continue;\n";
     s << "    const answer_t c = " << Gen(m).cost(m.axiom, "i", "j") << ";\n"
-      << "    if (!found || " << better("c", "best") << ") { best = c; i0 = i; j0 = j; found = true; }\n"
+      << (m.h.k == A_SUM     ? "    if (found) best = answer_t(best + c); else { best = answer_t(answer_t(0) + c); i0 = i; j0 = j; found = true; }\n"
+          : m.h.k == A_COUNT ? "    if (found) best = answer_t(best + 1); else { best = answer_t(1); i0 = i; j0 = j; found = true; }\n"
+                             : "    if (!found || " + better("c", "best") + ") { best = c; i0 = i; j0 = j; found = true; }\n")
       << "  }\n"
       << "  *ans = best;  // zero when no window cell parses\n";
   } else {
@@ -1129,7 +1169,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
       << "  *ans = " << Gen(m).cost(m.axiom, "i0", "j0") << ";\n";
   }
   s << "  out[0] = found ? 1 : 0; out[2] = i0; out[3] = j0;\n"
-       "  if (!tr || !found) { out[1] = 0; return; }\n"
+    << "  if (!tr || !found" << (m.additive() ? " || true" : "") << ") { out[1] = 0; return; }  // sums have no backtrace\n"
        "  int wr = 0;\n"
        "  bool over = false;\n"
        "  auto push = [&](int i, int j, int rule) {\n"
@@ -1411,7 +1451,7 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
         cells_d += m.tracks == 1 ? x.n1 + 1 - d
                                  : std::max<int64_t>(0, std::min<int64_t>(d, x.n1) - std::max<int64_t>(0, d - x.n2) + 1);
       }
-      gi = !m.loops || d < 24 ? 0 : (d >= 192 && cells_d * 64 <= ((int64_t)1 << 18)) ? 2 : (cells_d * 8 <= ((int64_t)1 << 19)) ? 1 : 0;
+      gi = !m.loops || d < 24 || !m.any_order() ? 0 : (d >= 192 && cells_d * 64 <= ((int64_t)1 << 18)) ? 2 : (cells_d * 8 <= ((int64_t)1 << 19)) ? 1 : 0;
       const int64_t lanes = c * (gi == 0 ? 1 : gi == 1 ? 8 : 64);
       block = gi == 0 || lanes < 256 ? 64 : 256;
       grid = (unsigned)((lanes + block - 1) / block);

@@ -231,7 +231,10 @@ class Grammar:
         return b"".join(struct.pack(fmt, *(x if isinstance(x, tuple) else (x,))) for x in seq)
 
     def py_input(self, seq):
-        """Input elements as the Python callables see them."""
+        """Input elements as the Python callables see them (RNA letters as
+        the codes the device reads)."""
+        if isinstance(seq, str) and self.rna:
+            return list(self.pack(seq))
         if isinstance(seq, (bytes, bytearray, str)):
             return list(seq.encode() if isinstance(seq, str) else seq)
         return list(seq)

@@ -120,48 +120,77 @@ def _wrap(v):
     return (v + (1 << 31)) % (1 << 32) - (1 << 31)
 
 
-def zuker():
-    """Zuker.scala:110-146 (ZukerGrammar) with the ZukerMFE algebra
-    (Zuker.scala:60-88): the grammar behind bench2/zu.c.  The bodies call the
-    librna functions; the device evaluates them over the context's parameter
-    file (no Python callables: the built-in kernel and the reference's
-    vectors are its checks)."""
-    g = A.Grammar(tracks=1, h="min", empty_value="999999", rna=True)
-    none = None
-    f = lambda args, body: A.cfun(none, args, body)
-    sadd = f("lb,e", "return e;")
-    cadd = f("x,e", "return x+e;")
-    dlr = f("lb,e,rb", "return e + ext_mismatch_energy(lb,rb-1) + termau_energy(lb,rb-1);")
-    sr = f("lb,e,rb", "return e + sr_energy(lb,rb);")
-    hl = f("lb,f1,x,f2,rb", "return hl_energy(f1,f2) + sr_energy(lb,rb);")
-    bl = f("lb,f1,b,x,f2,rb", "return x + bl_energy(f1,b._1,b._2-1,f2,f2-1) + sr_energy(lb,rb);")
-    br = f("lb,f1,x,b,f2,rb", "return x + br_energy(f1,b._1,b._2-1,f2,f1+1) + sr_energy(lb,rb);")
-    il = f("f1,f2,r1,x,r2,f3,f4", "return x + il_energy(f2,r1._2,r2._1-1,f3) + sr_energy(f1,f4);")
-    ml = f("lb,f1,x,f2,rb", "return ml_energy() + ul_energy() + x + termau_energy(f1,f2) + sr_energy(lb,rb) + "
-                            "ml_mismatch_energy(f1,f2);")
-    app = f("c1,c", "return c1+c;")
-    ul = f("c1", "return ul_energy()+c1;")
-    addss = f("c1,e", "return c1;")
-    nil = f("", "return 0;")
-    tail = f("i,j", "return j==M_W-1;")
-    stackpairing = f("i,j", "return i+4<=j && bp_index(_in1[i],_in1[j-1])!=NO_BP && bp_index(_in1[i+1],_in1[j-2])!=NO_BP;")
+# librna's bp_index pairs (CG, GC, GU, UG, AU, UA) over the codes A=1 C=2 G=3 U=4
+_PAIRS = {(2, 3), (3, 2), (3, 4), (4, 3), (1, 4), (4, 1)}
+
+
+def zuker(algebra="mfe"):
+    """Zuker.scala:110-146 (ZukerGrammar), the grammar behind bench2/zu.c,
+    with one of its algebras:
+
+    * "mfe": ZukerMFE (Zuker.scala:60-88).  The bodies call the librna
+      functions; the device evaluates them over the context's parameter file
+      (no Python callables: the built-in kernel and the reference's vectors
+      are its checks);
+    * "count": ZukerCount (Zuker.scala:28-44), h = sum: the number of
+      derivations of the sequence (int32, wrapping as on the JVM), with
+      Python callables for the oracle."""
+    count = algebra == "count"
+    if algebra not in ("mfe", "count"):
+        raise ValueError("algebra is mfe or count")
+    h = "sum" if count else "min"
+    g = A.Grammar(tracks=1, h=h, empty_value=None if count else "999999", rna=True)
+    if count:
+        sadd = A.cfun(lambda lb, e: e, "lb,e", "return e;")
+        cadd = A.cfun(lambda x, e: _wrap(x * e), "x,e", "return x*e;")
+        dlr = A.cfun(lambda lb, e, rb: e, "lb,e,rb", "return e;")
+        sr = A.cfun(lambda lb, e, rb: e, "lb,e,rb", "return e;")
+        hl = A.cfun(lambda lb, f1, x, f2, rb: 1, "lb,f1,x,f2,rb", "return 1;")
+        bl = A.cfun(lambda lb, f1, b, x, f2, rb: x, "lb,f1,b,x,f2,rb", "return x;")
+        br = A.cfun(lambda lb, f1, x, b, f2, rb: x, "lb,f1,x,b,f2,rb", "return x;")
+        il = A.cfun(lambda f1, f2, r1, x, r2, f3, f4: x, "f1,f2,r1,x,r2,f3,f4", "return x;")
+        ml = A.cfun(lambda lb, f1, x, f2, rb: x, "lb,f1,x,f2,rb", "return x;")
+        app = A.cfun(lambda c1, c: _wrap(c1 * c), "c1,c", "return c1*c;")
+        ul = A.cfun(lambda c1: c1, "c1", "return c1;")
+        addss = A.cfun(lambda c1, e: c1, "c1,e", "return c1;")
+        nil = A.cfun(lambda: 1, "", "return 1;")
+    else:
+        f = lambda args, body: A.cfun(None, args, body)
+        sadd = f("lb,e", "return e;")
+        cadd = f("x,e", "return x+e;")
+        dlr = f("lb,e,rb", "return e + ext_mismatch_energy(lb,rb-1) + termau_energy(lb,rb-1);")
+        sr = f("lb,e,rb", "return e + sr_energy(lb,rb);")
+        hl = f("lb,f1,x,f2,rb", "return hl_energy(f1,f2) + sr_energy(lb,rb);")
+        bl = f("lb,f1,b,x,f2,rb", "return x + bl_energy(f1,b._1,b._2-1,f2,f2-1) + sr_energy(lb,rb);")
+        br = f("lb,f1,x,b,f2,rb", "return x + br_energy(f1,b._1,b._2-1,f2,f1+1) + sr_energy(lb,rb);")
+        il = f("f1,f2,r1,x,r2,f3,f4", "return x + il_energy(f2,r1._2,r2._1-1,f3) + sr_energy(f1,f4);")
+        ml = f("lb,f1,x,f2,rb", "return ml_energy() + ul_energy() + x + termau_energy(f1,f2) + sr_energy(lb,rb) + "
+                                "ml_mismatch_energy(f1,f2);")
+        app = f("c1,c", "return c1+c;")
+        ul = f("c1", "return ul_energy()+c1;")
+        addss = f("c1,e", "return c1;")
+        nil = f("", "return 0;")
+    tail = A.cfun(lambda i, j: j == len(g._in1), "i,j", "return j==M_W-1;")
+    stackpairing = A.cfun(lambda i, j: i + 4 <= j and (g.in1(i), g.in1(j - 1)) in _PAIRS and
+                          (g.in1(i + 1), g.in1(j - 2)) in _PAIRS, "i,j",
+                          "return i+4<=j && bp_index(_in1[i],_in1[j-1])!=NO_BP && bp_index(_in1[i+1],_in1[j-2])!=NO_BP;")
     BASE, LOC, REG, REG3, REG30 = g.eli, g.emptyi, g.seq(), g.seq(3), g.seq(1, 30)
-    st = g.tabulate("st", lambda: ((BASE * st ^ sadd) | (dangle() * st ^ cadd) | (g.empty ^ nil)).aggregate("min").filter(tail),
+    st = g.tabulate("st", lambda: ((BASE * st ^ sadd) | (dangle() * st ^ cadd) | (g.empty ^ nil)).aggregate(h).filter(tail),
                     always_valid=True)
     cl = g.tabulate("cl", lambda: (
         (BASE * cl * BASE ^ sr) |
         (BASE * BASE * REG3 * BASE * BASE ^ hl) |
-        (BASE * BASE * REG30 * cl * BASE * BASE ^ bl).aggregate("min") |
-        (BASE * BASE * cl * REG30 * BASE * BASE ^ br).aggregate("min") |
-        (BASE * BASE * REG30 * cl * REG30 * BASE * BASE ^ il).aggregate("min") |
-        (BASE * BASE * mlc * BASE * BASE ^ ml)).aggregate("min").filter(stackpairing))
+        (BASE * BASE * REG30 * cl * BASE * BASE ^ bl).aggregate(h) |
+        (BASE * BASE * cl * REG30 * BASE * BASE ^ br).aggregate(h) |
+        (BASE * BASE * REG30 * cl * REG30 * BASE * BASE ^ il).aggregate(h) |
+        (BASE * BASE * mlc * BASE * BASE ^ ml)).aggregate(h).filter(stackpairing))
 
     def dangle():
         return LOC * cl * LOC ^ dlr
 
-    mlc = g.tabulate("ml", lambda: ((BASE * mlc ^ sadd) | ((dangle() ^ ul) * ml1 ^ app)).aggregate("min"))
+    mlc = g.tabulate("ml", lambda: ((BASE * mlc ^ sadd) | ((dangle() ^ ul) * ml1 ^ app)).aggregate(h))
     ml1 = g.tabulate("ml1", lambda: ((BASE * ml1 ^ sadd) | ((dangle() ^ ul) * ml1 ^ app) | (dangle() ^ ul) |
-                                     ((dangle() ^ ul) * REG ^ addss)).aggregate("min"))
+                                     ((dangle() ^ ul) * REG ^ addss)).aggregate(h))
     g.axiom = st
     return g
 

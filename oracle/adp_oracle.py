@@ -17,7 +17,12 @@ Python loops (small inputs only):
 * concatenation ranges (BaseParsers.scala:282-345): ~ with hints (1 on each
   side for p ~ p), -~ splitting track 1, ~- splitting track 2;
 * backtrace (CodeGen.scala:213-252): BFS over (i, j, tabulation id) records;
-* window (CodeGen.scala:325-348, ADPParsers.scala:9-13).
+* window (CodeGen.scala:325-348, ADPParsers.scala:9-13);
+* sum / count (BaseParsers.scala:145-153,385-392, the Scala `aggr`): the
+  sum of the candidates' values (count: their number), a cell without a
+  candidate has no answer; no backtrace (BaseParsers.scala:150: no candidate
+  matches a sum); a window adds up the axiom's answers of every window
+  (count: counts the windows that parse), ADPParsers.scala:11.
 
 Independent checks of these semantics live beside it: `enumerate_all`
 (every derivation's value, the reference's Scala `apply` without
@@ -25,6 +30,14 @@ aggregation, BaseParsers.scala:230-300) and `replay` (the reference's `build`,
 BaseParsers.scala:178-183: a trace evaluated bottom-up through `reapply`).
 """
 INF = -1
+
+# integer answer types: (bits, signed) on gfx950 and x86-64
+_INT_TYPES = {"int": (32, True), "int32_t": (32, True), "unsigned": (32, False), "unsigned int": (32, False),
+              "uint32_t": (32, False), "long": (64, True), "long int": (64, True), "long long": (64, True),
+              "int64_t": (64, True), "unsigned long": (64, False), "unsigned long long": (64, False),
+              "uint64_t": (64, False), "size_t": (64, False), "short": (16, True), "int16_t": (16, True),
+              "unsigned short": (16, False), "uint16_t": (16, False), "char": (8, True), "signed char": (8, True),
+              "int8_t": (8, True), "unsigned char": (8, False), "uint8_t": (8, False)}
 
 
 def _hm(a, b):
@@ -42,7 +55,17 @@ class Oracle:
         self.h = spec["h"]
         self.window = spec.get("window", 0)
         self.empty = spec.get("empty")
+        self.int_type = _INT_TYPES.get(spec.get("answer", "int"))
         self._analyze()
+
+    def wrap(self, v):
+        """A sum of integer answers wraps like the answer type (C with
+        -fwrapv, the JVM's Int); other values are left alone."""
+        if self.int_type is None or not isinstance(v, int):
+            return v
+        bits, signed = self.int_type
+        v &= (1 << bits) - 1
+        return v - (1 << bits) if signed and v >> (bits - 1) else v
 
     # ---- yields ----------------------------------------------------------------
     def _hint(self, n):
@@ -245,6 +268,9 @@ class Oracle:
             return self.f[list(h.values())[0]](v)
         return v
 
+    def _additive(self):
+        return self.h in ("sum", "count")
+
     def _better(self, c, c0):
         h = self.h
         kind = h if isinstance(h, str) else list(h.keys())[0]
@@ -370,7 +396,13 @@ class Oracle:
                         if len(v) != 1:
                             raise ValueError("alternative without one value")
                         c = v[0]
-                        if self._better(c, c0) or (self.empty is None and r0 == -1):
+                        if self._additive():
+                            inc = 1 if self.h == "count" else c
+                            if r0 == -1:
+                                c0, r0, p0 = self.wrap(inc), self.id[t] + q, (p + [0, 0])[:2]
+                            else:
+                                c0 = self.wrap(c0 + inc)
+                        elif self._better(c, c0) or (self.empty is None and r0 == -1):
                             c0, r0, p0 = c, self.id[t] + q, (p + [0, 0])[:2]
                 self.cost[t][(i, j)] = c0
                 self.rule[t][(i, j)] = r0
@@ -382,7 +414,12 @@ class Oracle:
             while j <= n1:
                 if self.tabs[ax].get("always_valid", False) or self._valid(ax, i, j):
                     c = self.cost[ax][(i, j)]
-                    if not found or self._better(c, best):
+                    if self._additive():
+                        inc = 1 if self.h == "count" else c
+                        best = self.wrap(best + inc) if found else self.wrap(inc)
+                        if not found:
+                            i0, j0, found = i, j, True
+                    elif not found or self._better(c, best):
                         best, i0, j0, found = c, i, j, True
                 i, j = i + 1, j + 1
             ans = best
@@ -390,7 +427,7 @@ class Oracle:
             i0, j0 = (0, n1) if self.tracks == 1 else (n1, n2)
             found = self.tabs[ax].get("always_valid", False) or self._valid(ax, i0, j0)
             ans = self.cost[ax][(i0, j0)]
-        if not trace or not found:
+        if not trace or not found or self._additive():
             return ans, found, [] if trace else None
         tr = [[i0, j0, self.id[ax], 0, 0]]
         rd = 0

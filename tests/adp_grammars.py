@@ -13,3 +13,61 @@ def run_oracle(g, in1, in2=None, trace=True):
     g._in1 = g.py_input(in1)
     g._in2 = g.py_input(in2) if in2 is not None else None
     return o, o.run(g._in1, g._in2, trace=trace)
+
+
+def _wrapper(fmt):
+    """Integer arithmetic of a struct format as on the device (-fwrapv)."""
+    import struct
+    bits = 8 * struct.calcsize("=" + fmt)
+    if fmt in "fd":
+        return lambda v: v
+    return lambda v: (v + (1 << (bits - 1))) % (1 << bits) - (1 << (bits - 1))
+
+
+def motzkin(answer="int", fmt="i", h="sum", window=0):
+    """s -> empty | el s | el s el s with the counting algebra (h = sum): the
+    number of derivations of s over n letters is the Motzkin number M_n
+    (secondary structures of n bases with any pairing, OEIS A001006)."""
+    import dpx.adp as A
+    w = _wrapper(fmt)
+    g = A.Grammar(tracks=1, answer=answer, answer_fmt=fmt, h=h, window=window)
+    one = A.cfun(lambda: 1, "", "return 1;")
+    skip = A.cfun(lambda c, x: x, "c,x", "return x;")
+    pair = A.cfun(lambda a, x, b, y: w(x * y), "a,x,b,y", "return x*y;")
+    s = g.tabulate("s", lambda: ((g.empty ^ one) | (g.el * s ^ skip) | (g.el * s * g.el * s ^ pair)).aggregate(h))
+    g.axiom = s
+    return g
+
+
+def splits(h="count", window=0):
+    """s -> el s s | el: with h = count a cell's answer is the number of
+    split points whose two sides parse (one for a single letter)."""
+    import dpx.adp as A
+    g = A.Grammar(tracks=1, h=h, window=window)
+    f3 = A.cfun(lambda c, x, y: x + y, "c,x,y", "return x+y;")
+    f1 = A.cfun(lambda c: 1, "c", "return 1;")
+    s = g.tabulate("s", lambda: ((g.el * s * s ^ f3) | (g.el ^ f1)).aggregate(h))
+    g.axiom = s
+    return g
+
+
+def motzkin_numbers(n, bits=None):
+    """M_0..M_n by M_L = M_{L-1} + sum_k M_k M_{L-2-k} (exact integers,
+    reduced to signed `bits` when given)."""
+    m = [1]
+    for L in range(1, n + 1):
+        m.append(m[L - 1] + sum(m[k] * m[L - 2 - k] for k in range(L - 1)))
+    if bits:
+        m = [(v + (1 << (bits - 1))) % (1 << bits) - (1 << (bits - 1)) for v in m]
+    return m
+
+
+def split_counts(n):
+    """(valid, count) of splits() over lengths 0..n, from the grammar by hand:
+    a length L >= 3 cell has the splits a + b = L - 1 with both sides valid."""
+    valid, count = [False, True], [0, 1]
+    for L in range(2, n + 1):
+        c = sum(1 for a in range(1, L - 1) if valid[a] and valid[L - 1 - a])
+        valid.append(c > 0)
+        count.append(c)
+    return valid[:n + 1], count[:n + 1]

@@ -26,20 +26,40 @@ def _mapfn(rng, nargs):
     return A.cfun(lambda *a, w=w, c=c: _wrap(c + sum(w[q] * _val(a[q]) for q in range(nargs))), args, body)
 
 
+def _mulfn(rng, nargs):
+    # products of the arguments' values and a weight: with h = sum the DP
+    # answer is the sum of every derivation's value (distributivity, exact in
+    # int32 arithmetic modulo 2^32)
+    w = rng.choice([1, 1, 2, 3, -1])
+    body = "return %d%s;" % (w, "".join(" * val_(a%d)" % q for q in range(nargs)))
+    args = ",".join("a%d" % q for q in range(nargs))
+
+    def f(*a, w=w):
+        v = w
+        for x in a:
+            v *= _val(x)
+        return _wrap(v)
+    return A.cfun(f, args, body)
+
+
 def _wrap(v):
     return (v + (1 << 31)) % (1 << 32) - (1 << 31)
 
 
-def single_track(seed):
+def single_track(seed, agg=None):
+    """agg "sum" or "count": that aggregation with product algebras."""
     rng = random.Random(seed)
     h = rng.choice(["max", "min", "maxBy"])
+    mapf = _mapfn
     key = None
-    if h == "maxBy":
+    if agg:
+        h, mapf = agg, _mulfn
+    elif h == "maxBy":
         key = A.cfun(lambda a: a % 5 if a >= 0 else -((-a) % 5), "a", "return a % 5;")  # C remainder: many ties
         h = A.max_by(key)
     window = rng.choice([0, 0, 0, 4, 6])
     empty = None
-    if isinstance(h, str) and rng.random() < 0.3:  # the reference's cudaEmpty: never accepted as a candidate
+    if isinstance(h, str) and not agg and rng.random() < 0.3:  # the reference's cudaEmpty: never accepted as a candidate
         empty = "-1000000" if h == "max" else "1000000"
     g = A.Grammar(tracks=1, h=h, window=window, header=HEADER, empty_value=empty)
     agg = h
@@ -85,14 +105,14 @@ def single_track(seed):
             p = p.filter(A.cfun(lambda i, j, lo=lo, hi=hi: lo <= j - i <= hi, "i,j", "return j-i>=%d && j-i<=%d;" % (lo, hi)))
         if nargs == 0:
             return p ^ _const(rng.randint(-2, 2), 0)
-        return p ^ _mapfn(rng, nargs)
+        return p ^ mapf(rng, nargs)
 
     for nm in names:
         nalt = rng.randint(1, 4)
         alts = [alt(0) for _ in range(nalt)]
         # a base case that always exists keeps most cells valid
         if rng.random() < 0.7:
-            alts.insert(rng.randint(0, len(alts)), g.el ^ _mapfn(rng, 1))
+            alts.insert(rng.randint(0, len(alts)), g.el ^ mapf(rng, 1))
 
         def thunk(alts=alts):
             p = alts[0]
@@ -115,9 +135,12 @@ def _const(v, nargs=0):
     return A.cfun(lambda *a, v=v: v, ",".join("a%d" % q for q in range(nargs)), "return %d;" % v)
 
 
-def two_track(seed):
+def two_track(seed, agg=None):
     rng = random.Random(seed)
     h = rng.choice(["max", "min"])
+    _mapfn_ = _mapfn
+    if agg:
+        h, _mapfn_ = agg, _mulfn
     g = A.Grammar(tracks=2, h=h, header=HEADER)
     names = ["M", "G"] if rng.random() < 0.5 else ["M"]
     tabs = {nm: g.tabulate(nm, None, always_valid=rng.random() < 0.3) for nm in names}
@@ -130,22 +153,22 @@ def two_track(seed):
         if r < 0.15:
             return g.empty ^ _const(rng.randint(-1, 2))
         if r < 0.25:
-            return g.empty1 ^ _mapfn(rng, 1)
+            return g.empty1 ^ _mapfn_(rng, 1)
         if r < 0.35:
-            return g.empty2 ^ _mapfn(rng, 1)
+            return g.empty2 ^ _mapfn_(rng, 1)
         if r < 0.5:
-            return g.el1 << tab() >> g.el2 ^ _mapfn(rng, 3)
+            return g.el1 << tab() >> g.el2 ^ _mapfn_(rng, 3)
         if r < 0.6:
             mn = rng.randint(1, 2)
-            return g.seq(mn, rng.choice([None, mn + 2])) << tab() ^ _mapfn(rng, 2)
+            return g.seq(mn, rng.choice([None, mn + 2])) << tab() ^ _mapfn_(rng, 2)
         if r < 0.7:
             mn = rng.randint(1, 2)
-            return tab() >> g.seq(mn, rng.choice([None, mn + 1])) ^ _mapfn(rng, 2)
+            return tab() >> g.seq(mn, rng.choice([None, mn + 1])) ^ _mapfn_(rng, 2)
         if r < 0.8:
-            return g.el1 << tab() ^ _mapfn(rng, 2)
+            return g.el1 << tab() ^ _mapfn_(rng, 2)
         if r < 0.9:
-            return tab() >> g.el2 ^ _mapfn(rng, 2)
-        return g.eli << tab() >> g.eli ^ _mapfn(rng, 3)
+            return tab() >> g.el2 ^ _mapfn_(rng, 2)
+        return g.eli << tab() >> g.eli ^ _mapfn_(rng, 3)
 
     for nm in names:
         alts = [alt() for _ in range(rng.randint(1, 4))]

@@ -291,3 +291,87 @@ def test_repeated_shapes(ctx):
     for seed in range(3):
         s = gen.rna_uniform(400, 900 + seed)
         assert nu.parse(s)[::2] == ctx.nussinov(s, min_loop=0), seed
+
+
+def test_sum_count_closed_forms(ctx):
+    """h = sum / count: Motzkin numbers (int64 and int32 answers, candidates
+    split over lanes; doubles in sequential order on one lane), split counts,
+    windows; no backtrace."""
+    cases = [("long long", "q", 64, [0, 1, 7, 33, 200, 1100]), ("int", "i", 32, [2, 40, 300]),
+             ("double", "d", None, [0, 20, 33])]
+    for answer, fmt, bits, ns in cases:
+        cg = ctx.grammar(G.motzkin(answer, fmt))
+        want = G.motzkin_numbers(max(ns), bits)
+        for n in ns:
+            print("motzkin %s n=%d" % (answer, n), flush=True)
+            assert bits or want[n] < 2 ** 53
+            ans, found, tr = cg.parse("a" * n)
+            assert found and ans == want[n] and tr == [], (answer, n)
+        got = cg.parse_batch(["a" * n for n in ns])
+        assert [x[0] for x in got] == [want[n] for n in ns]
+    valid, count = G.split_counts(400)
+    cg = ctx.grammar(G.splits())
+    for n in [0, 1, 2, 3, 4, 5, 64, 65, 199, 400]:
+        ans, found, tr = cg.parse("a" * n)
+        assert found == valid[n] and (not found or ans == count[n]) and tr == [], n
+    for w in [3, 4, 7]:
+        cg = ctx.grammar(G.splits(window=w))
+        for n in [2, 7, 50]:
+            ans, found, _ = cg.parse("a" * n)
+            assert found == (n >= w and valid[w]) and (not found or ans == n - w + 1), (w, n)
+    mz = G.motzkin_numbers(9)
+    cg = ctx.grammar(G.motzkin(window=9))
+    for n in [5, 9, 100]:
+        ans, found, _ = cg.parse("a" * n)
+        assert found == (n >= 9) and (not found or ans == (n - 8) * mz[9]), n
+
+
+def test_random_sum_count_grammars(ctx):
+    """Random grammars under h = sum (product algebras) and h = count against
+    the oracle: answers and found flags, batched and one by one."""
+    import adp_random as R
+    done = 0
+    for seed in range(24):
+        agg = "sum" if seed % 2 else "count"
+        g = R.single_track(seed, agg) if seed % 3 else R.two_track(seed, agg)
+        try:
+            G.oracle(g)
+        except ValueError:
+            continue
+        print("random %s seed %d" % (agg, seed), flush=True)
+        cg = ctx.grammar(g)
+        rng = random.Random(seed)
+        ins = []
+        for _ in range(4):
+            if g.tracks == 1:
+                ins.append((R.text(rng, rng.randint(0, 14)), None))
+            else:
+                ins.append((R.text(rng, rng.randint(0, 9)), R.text(rng, rng.randint(0, 9))))
+        want = [G.run_oracle(g, a, b)[1] for a, b in ins]
+        for (a, b), (oans, ofound, _) in zip(ins, want):
+            ans, found, tr = cg.parse(a, b)
+            assert found == bool(ofound) and tr == [], (seed, a, b)
+            if found:
+                assert ans == oans, (seed, a, b, ans, oans)
+        got = cg.parse_batch([a if b is None else (a, b) for a, b in ins])
+        for (ans, found, _), (oans, ofound, _) in zip(got, want):
+            assert found == bool(ofound) and (not found or ans == oans), seed
+        done += 1
+    assert done >= 12
+
+
+def test_zuker_count(ctx):
+    """ZukerCount (Zuker.scala:28-44, h = sum) on the device against the
+    oracle (itself checked against brute-force derivation counts), and a
+    longer sequence whose count wraps in int32 both ways."""
+    from dpx import gen
+    g = G.zuker("count")
+    cg = ctx.grammar(g)
+    seqs = [gen.rna_uniform(n, 77 + n) for n in [0, 4, 7, 12, 20, 33, 45, 60]]
+    for s in seqs:
+        print("zuker count n=%d" % len(s), flush=True)
+        _, (oans, ofound, _) = G.run_oracle(g, s)
+        ans, found, tr = cg.parse(s)
+        assert found and ofound and ans == oans and tr == [], len(s)
+    got = cg.parse_batch(seqs)
+    assert [x[0] for x in got] == [cg.parse(s)[0] for s in seqs]

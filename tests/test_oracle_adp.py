@@ -142,7 +142,7 @@ def test_library_generates_and_compiles(make):
 def test_library_rejects_unsupported():
     g = G.nussinov()
     spec = g.spec()
-    bad = dict(spec, h="sum")
+    bad = dict(spec, h="product")
     rc, _, msg = _check(json.dumps(bad))
     assert rc != 0 and "aggregation" in msg
     bad = dict(spec, axiom="nope")
@@ -222,3 +222,100 @@ def test_random_grammars_compile_all():
             continue
         rc, src, msg = _check(g.spec_json())
         assert rc == 0, (seed, msg[:2000])
+
+
+def test_sum_aggregation_closed_form():
+    """h = sum with the counting algebra: Motzkin numbers (exact, then int32
+    and int64 wrapping as the answer type does), windows add up."""
+    exact = G.motzkin_numbers(40)
+    assert exact[:11] == [1, 1, 2, 4, 9, 21, 51, 127, 323, 835, 2188]  # OEIS A001006
+    for answer, fmt, bits in [("int", "i", 32), ("long long", "q", 64)]:
+        g = G.motzkin(answer, fmt)
+        want = G.motzkin_numbers(40, bits)
+        for n in [0, 1, 2, 5, 11, 25, 40]:
+            _, (ans, found, tr) = G.run_oracle(g, "a" * n)
+            assert found and ans == want[n] and tr == [], (answer, n)
+    g = G.motzkin(h="sum", window=6)
+    for n in [3, 6, 13]:
+        _, (ans, found, _) = G.run_oracle(g, "a" * n)
+        assert found == (n >= 6) and (not found or ans == (n - 5) * exact[6])
+
+
+def test_count_aggregation_closed_form():
+    valid, count = G.split_counts(30)
+    g = G.splits()
+    for n in range(0, 31, 3):
+        _, (ans, found, tr) = G.run_oracle(g, "a" * (n or 1) if n == 1 else "a" * n)
+        assert found == valid[n] and (not found or ans == count[n]) and tr == [], n
+    for w in [3, 4, 7]:
+        g = G.splits(window=w)
+        for n in [2, 7, 12]:
+            _, (ans, found, _) = G.run_oracle(g, "a" * n)
+            assert found == (n >= w and valid[w]) and (not found or ans == n - w + 1), (w, n)
+    # sum over the same grammar adds the candidates' values
+    _, (ans, found, _) = G.run_oracle(G.splits(h="sum"), "a" * 3)
+    assert found and ans == 2  # el (s=1) (s=1) -> 1 + 1
+
+
+def test_random_sum_grammars_vs_brute_force():
+    """Product algebras under h = sum: the DP answer is the sum of every
+    derivation's value, modulo 2^32 (distributivity)."""
+    import adp_random as R
+    checked = 0
+    for seed in range(60):
+        g = R.single_track(seed, "sum") if seed % 3 else R.two_track(seed, "sum")
+        try:
+            orc = G.oracle(g)
+        except ValueError:
+            continue
+        av = any(t.always_valid for t in g.tabs.values())
+        rng = random.Random(seed)
+        for _ in range(3):
+            if g.tracks == 1:
+                in1, in2 = R.text(rng, rng.randint(0, 6)), None
+            else:
+                in1, in2 = R.text(rng, rng.randint(0, 3)), R.text(rng, rng.randint(0, 3))
+            _, (ans, found, tr) = G.run_oracle(g, in1, in2)
+            try:
+                vals = orc.enumerate_all(g.py_input(in1), g.py_input(in2) if in2 is not None else None, limit=50000)
+            except OverflowError:
+                continue
+            assert tr == []
+            if not av:
+                assert found == bool(vals), (seed, in1, in2)
+            if found:
+                assert ans == R._wrap(sum(vals)), (seed, in1, in2, ans, vals)
+            checked += 1
+    assert checked > 60
+
+
+def test_zuker_count_vs_brute_force():
+    """ZukerCount (h = sum): the number of derivations of the Zuker grammar."""
+    from dpx import gen
+    g = G.zuker("count")
+    for n in [0, 5, 9, 12, 14]:
+        for seed in range(3):
+            s = gen.rna_uniform(n, 31 * n + seed)
+            orc, (ans, found, tr) = G.run_oracle(g, s)
+            vals = orc.enumerate_all(g.py_input(s), limit=400000)
+            assert found and ans == len(vals) and all(v == 1 for v in vals), (n, seed)
+    # structures appear: a sequence that can fold has more than the open chain
+    s = "GGGGAAAACCCC"
+    orc, (ans, _, _) = G.run_oracle(g, s)
+    assert ans > 1
+
+
+def test_sum_count_grammars_compile():
+    import adp_random as R
+    gs = [G.motzkin(), G.motzkin("double", "d"), G.splits(), G.splits(window=4), G.zuker("count")]
+    for seed in range(12):
+        g = R.single_track(seed, "count") if seed % 3 else R.two_track(seed, "sum")
+        try:
+            G.oracle(g)
+        except ValueError:
+            continue
+        gs.append(g)
+    assert len(gs) > 10
+    for g in gs:
+        rc, src, msg = _check(g.spec_json())
+        assert rc == 0, msg[:2000]
