@@ -35,6 +35,7 @@
 #include "rna_model.h"
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -242,6 +243,7 @@ struct Model {
   std::vector<int> order;  // evaluation order within a cell (rulesOrder)
   int n_rules = 0, max_pos = 0;
   bool loops = false;
+  bool fixed_refs = true;  // every table reference of a cell body is (i + c, j + c')
 
   // yields (BaseParsers.scala: Parser.min/max, Concat hints)
   int nmin(const Node* n) const {
@@ -640,6 +642,46 @@ std::string add(const std::string& a, int c) {
   return "(" + a + (c > 0 ? "+" : "-") + std::to_string(c > 0 ? c : -c) + ")";
 }
 
+// e == v + c for an index expression built from i, j, integers, + - and
+// parentheses (the forms add() and the concatenation rules produce)
+bool affine(const std::string& e, char v, int& c) {
+  size_t p = 0;
+  long ci = 0, cj = 0, k = 0;
+  bool ok = true;
+  std::function<void(int)> expr = [&](int sign) {
+    int sg = sign;
+    for (;;) {
+      if (p >= e.size()) { ok = false; return; }
+      const char ch = e[p];
+      if (ch == '(') {
+        ++p;
+        expr(sg);
+        if (p >= e.size() || e[p] != ')') { ok = false; return; }
+        ++p;
+      } else if (ch == 'i' || ch == 'j') {
+        if (p + 1 < e.size() && (isalnum((unsigned char)e[p + 1]) || e[p + 1] == '_')) { ok = false; return; }
+        (ch == 'i' ? ci : cj) += sg;
+        ++p;
+      } else if (isdigit((unsigned char)ch)) {
+        long x = 0;
+        while (p < e.size() && isdigit((unsigned char)e[p]) && x < (1L << 30)) x = 10 * x + (e[p++] - '0');
+        k += sg * x;
+      } else {
+        ok = false;
+        return;
+      }
+      if (!ok || p >= e.size() || e[p] == ')') return;
+      if (e[p] != '+' && e[p] != '-') { ok = false; return; }
+      sg = e[p] == '+' ? sign : -sign;
+      ++p;
+    }
+  };
+  expr(1);
+  if (!ok || p != e.size()) return false;
+  c = (int)k;
+  return v == 'i' ? ci == 1 && cj == 0 : cj == 1 && ci == 0;
+}
+
 struct Loop {
   std::string var;
   std::vector<std::string> lo, hi;  // var from max(lo) to min(hi), ascending
@@ -662,18 +704,57 @@ struct Gen {
   const Model& m;
   std::string err;
   int nvar = 0;
-  explicit Gen(const Model& mm) : m(mm) {}
-  std::string cost(int t, const std::string& a, const std::string& b) const {
+  // cell bodies: references to the cell itself read the values the earlier
+  // tabulations of the cell left in registers (cv); fixed records whether
+  // every reference is (i + c, j + c') -- then no reference reads what the
+  // cell writes, and the cell's stores may wait until its last tabulation
+  bool cell = false;
+  bool fixed = true;
+  // with every reference fixed, the reads of other cells become loads at the
+  // top of the tabulation (pre: expression -> name, declarations in order),
+  // at an index that is in bounds whatever i and j are, so they need no
+  // guard and issue together; the rules' conditions still decide their use
+  std::map<std::string, std::string>* pre = nullptr;
+  std::vector<std::array<std::string, 3>>* pre_decl = nullptr;  // type, name, expression
+  explicit Gen(const Model& mm, bool in_cell = false) : m(mm), cell(in_cell) {}
+  std::string hoist(const std::string& type, const std::string& expr) {
+    auto it = pre->find(expr);
+    if (it != pre->end()) return it->second;
+    const std::string nm = "_l" + std::to_string(pre->size());
+    (*pre)[expr] = nm;
+    pre_decl->push_back({type, nm, expr});
+    return nm;
+  }
+  bool self(const std::string& a, const std::string& b) {
+    int ca = 0, cb = 0;
+    const bool fa = affine(a, 'i', ca), fb = affine(b, 'j', cb);
+    if (!fa || !fb) fixed = false;
+    return cell && fa && fb && ca == 0 && cb == 0;
+  }
+  std::string cost(int t, const std::string& a, const std::string& b) {
+    if (self(a, b)) return "cv.c" + std::to_string(t);
+    if (pre) return hoist("answer_t", "((const answer_t*)T.c[" + std::to_string(t) + "])[sidx_(z," + a + "," + b + ")]");
     return "((const answer_t*)T.c[" + std::to_string(t) + "])[idx_(z," + a + "," + b + ")]";
   }
-  std::string valid(int t, const std::string& a, const std::string& b) const {
+  std::string valid(int t, const std::string& a, const std::string& b) {
     if (!m.empty.empty()) return "!(" + cost(t, a, b) + "==(answer_t)(" + m.empty + "))";
+    if (self(a, b)) return "(cv.r" + std::to_string(t) + "!=-1)";
+    if (pre) return "(" + hoist("short", "T.r[" + std::to_string(t) + "][sidx_(z," + a + "," + b + ")]") + "!=-1)";
     return "(T.r[" + std::to_string(t) + "][idx_(z," + a + "," + b + ")]!=-1)";
   }
   void scs(int mn, int mx, const std::string& a, const std::string& b, AltCode& o) {
     if (mn == mx) { o.arith.push_back(add(a, mn) + "==" + b); return; }
     if (mx != INF) o.arith.push_back(add(b, -mx) + "<=" + a);
     if (mn > 0) o.arith.push_back(add(a, mn) + "<=" + b);
+  }
+  // an input element; hoisted (index clamped into the input and its padding)
+  // when the index is fixed
+  std::string input(int track, const std::string& a) {
+    int c = 0;
+    const std::string in = track == 1 ? "_in1" : "_in2";
+    if (pre && (affine(a, 'i', c) || affine(a, 'j', c)))
+      return hoist("input_t", in + "[min(max(" + a + ", 0), z.n" + std::to_string(m.tracks == 1 ? 1 : track) + ")]");
+    return in + "[" + a + "]";
   }
   std::string call(int fn, const std::vector<std::string>& args) const {
     std::string s = "f_" + m.fns[fn].name + "(_x";
@@ -705,8 +786,8 @@ struct Gen {
           case T_EMPTYI: scs(0, 0, a, b, o); vals.push_back("(" + a + ")"); return true;
           case T_EMPTY1: o.arith.push_back("0==" + a); vals.push_back("(" + b + ")"); return true;
           case T_EMPTY2: o.arith.push_back("0==" + b); vals.push_back("(" + a + ")"); return true;
-          case T_EL: case T_EL1: scs(1, 1, a, b, o); vals.push_back("_in1[" + a + "]"); return true;
-          case T_EL2: scs(1, 1, a, b, o); vals.push_back("_in2[" + a + "]"); return true;
+          case T_EL: case T_EL1: scs(1, 1, a, b, o); vals.push_back(input(1, a)); return true;
+          case T_EL2: scs(1, 1, a, b, o); vals.push_back(input(2, a)); return true;
           case T_ELI: scs(1, 1, a, b, o); vals.push_back("(" + a + ")"); return true;
           case T_SEQ: scs(n->tmin, n->tmax, a, b, o); vals.push_back("T2ii{" + a + "," + b + "}"); return true;
         }
@@ -929,17 +1010,28 @@ __device__ __forceinline__ int rMlmm_(const Ctx& x, int i, int j) { return x.rna
 
 bool generate(Model& m, std::string& src, std::string& err) {
   std::vector<std::vector<AltCode>> alts(m.tabs.size());
+  std::vector<std::map<std::string, std::string>> pre(m.tabs.size());
+  std::vector<std::vector<std::array<std::string, 3>>> pre_decl(m.tabs.size());
+  // second pass: every reference fixed, so the reads are hoisted
+  for (int pass = 0; pass < 2; ++pass)
   for (int t : m.order) {
+    if (pass == 1 && !m.fixed_refs) break;
+    alts[t].clear();
     const Tab& T = m.tabs[t];
     std::vector<NP> flat;
     if (!flatten(m, T.p, true, flat, err)) return false;
     if ((int)flat.size() != T.alt) { err = "internal: alternative count"; return false; }
     for (int q = 0; q < (int)flat.size(); ++q) {
-      Gen g(m);
+      Gen g(m, true);
+      if (pass == 1) {
+        g.pre = &pre[t];
+        g.pre_decl = &pre_decl[t];
+      }
       AltCode o;
       o.rule = T.id + q;
       std::vector<std::string> vals;
       if (!g.gen(flat[q].get(), "i", "j", o, vals)) { err = g.err; return false; }
+      if (!g.fixed) m.fixed_refs = false;
       if (vals.size() != 1) {
         err = "alternative " + std::to_string(o.rule) + " of '" + T.name + "' does not yield one answer (map the concatenation with a function)";
         return false;
@@ -977,6 +1069,14 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  return d * (z.n1 + 1) - d * (d - 1) / 2 + i;\n}\n";
   else
     s << "__device__ __forceinline__ long long idx_(const Dims& z, int i, int j) { return (long long)i * (z.n2 + 1) + j; }\n";
+  // (i, j) when it is a cell of the tables, else cell 0: in bounds for any i, j
+  if (m.tracks == 1)
+    s << "__device__ __forceinline__ long long sidx_(const Dims& z, int i, int j) {\n"
+      << "  const int D = " << (m.window > 0 ? "min(" + std::to_string(m.window) + ", z.n1)" : std::string("z.n1")) << ";\n"
+      << "  return i >= 0 && i <= j && j <= z.n1 && j - i <= D ? idx_(z, i, j) : 0;\n}\n";
+  else
+    s << "__device__ __forceinline__ long long sidx_(const Dims& z, int i, int j) {\n"
+         "  return i >= 0 && i <= z.n1 && j >= 0 && j <= z.n2 ? idx_(z, i, j) : 0;\n}\n";
   for (auto& f : m.fns) {
     if (!f.args.empty()) {
       s << "template <";
@@ -1003,6 +1103,10 @@ bool generate(Model& m, std::string& src, std::string& err) {
   // the group, rules without a loop run on lane 0, and the group's
   // candidates are merged by value, then by (rule, outer split, inner split)
   // -- the first strictly better candidate of the sequential loop order.
+  s << "template <class V> __device__ __forceinline__ void pin_(const V& v) {\n"
+       "  int u[(sizeof(V) + 3) / 4] = {};\n"
+       "  __builtin_memcpy(u, &v, sizeof(V));\n"
+       "  for (int q = 0; q < (int)((sizeof(V) + 3) / 4); ++q) __asm__ volatile(\"\" ::\"v\"(u[q]));\n}\n";
   s << "template <class V> __device__ __forceinline__ V shfl_xor_(V v, int m, int w) {\n"
        "  constexpr int N = (sizeof(V) + 3) / 4;\n"
        "  int u[N] = {};\n"
@@ -1012,14 +1116,41 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "  return v;\n}\n";
   int used_tabs = 0;
   for (auto& T : m.tabs) used_tabs += T.used ? 1 : 0;
+  // the cell's results in registers: read by the later tabulations of the
+  // cell, stored by the last one (DEFER) or each by its own
+  s << "struct CellV {";
+  for (int t : m.order) {
+    s << " answer_t c" << t << "; short r" << t << ";";
+    for (int q = 0; q < m.tabs[t].cat; ++q) s << " int p" << t << "_" << q << ";";
+  }
+  s << " };\n"
+       "__device__ __forceinline__ void store_cell(const Dims& z, const Tabs& T, int i, int j, const CellV& cv) {\n"
+       "  const long long x = idx_(z, i, j);\n";
+  for (int t : m.order) {
+    s << "  ((answer_t*)T.c[" << t << "])[x] = cv.c" << t << ";\n  T.r[" << t << "][x] = cv.r" << t << ";\n";
+    for (int q = 0; q < m.tabs[t].cat; ++q) s << "  T.p[" << t << "][" << q << "][x] = cv.p" << t << "_" << q << ";\n";
+  }
+  s << "}\n";
   for (int t : m.order) {
     const Tab& T = m.tabs[t];
     s << "// ---- " << T.name << "[i,j]: rules " << T.id << ".." << T.id + T.alt - 1 << "\n"
-      << "template <int G> __device__ __forceinline__ void tab_" << t
-      << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, int lane) {\n"
+      << "struct Pre_" << t << " {";
+    for (auto& d : pre_decl[t]) s << " " << d[0] << " " << d[1] << ";";
+    s << " int pad_; };\n"
+      << "__device__ __forceinline__ void load_" << t << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, Pre_" << t
+      << "& p) {\n  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n";
+    for (auto& d : pre_decl[t]) s << "  p." << d[1] << " = " << d[2] << ";\n";
+    s << "  p.pad_ = 0;\n}\n"
+      << "__device__ __forceinline__ void pin_" << t << "(const Pre_" << t << "& p) {";
+    for (auto& d : pre_decl[t]) s << " pin_(p." << d[1] << ");";
+    s << " (void)p; }\n"
+      << "template <int G, bool DEFER> __device__ __forceinline__ void tab_" << t
+      << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, int lane, CellV& cv, const Pre_" << t << "& _p) {\n"
       << "  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n"
       << "  answer_t _c0" << (m.empty.empty() ? "{}" : " = (answer_t)(" + m.empty + ")") << ";\n"
       << "  int _r0 = -1, _p0 = 0, _p1 = 0, _o0 = 0, _o1 = 0;\n";
+    for (auto& d : pre_decl[t]) s << "  const " << d[0] << " " << d[1] << " = _p." << d[1] << ";\n";
+    s << "  (void)_p;\n";
     for (auto& o : alts[t]) {
       std::string ind = "  ";
       s << ind << "{  // rule " << o.rule << "\n";
@@ -1108,13 +1239,15 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "    if (take) { _c0 = c; _r0 = r; _p0 = p0; _p1 = p1; _o0 = o0; _o1 = o1; }\n"
          "  }\n"
          "  (void)_o0; (void)_o1;\n"
-         "  if (lane == 0) {\n"
+      << "  cv.c" << t << " = _c0;\n  cv.r" << t << " = (short)_r0;\n";
+    for (int q = 0; q < T.cat; ++q) s << "  cv.p" << t << "_" << q << " = _p" << q << ";\n";
+    s << "  if (!DEFER && lane == 0) {\n"
          "    const long long x = idx_(z, i, j);\n"
       << "    ((answer_t*)T.c[" << t << "])[x] = _c0;\n"
       << "    T.r[" << t << "][x] = (short)_r0;\n";
     for (int q = 0; q < T.cat; ++q) s << "    T.p[" << t << "][" << q << "][x] = _p" << q << ";\n";
     s << "  }\n";
-    if (used_tabs > 1)  // a later tabulation of this cell may read it from another lane of the wave
+    if (used_tabs > 1 && !m.fixed_refs)  // a later tabulation of this cell may read it from another lane of the wave
       s << "  if (G > 1) { __builtin_amdgcn_s_waitcnt(0x0F70); __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, \"workgroup\"); }\n";
     s << "}\n";
   }
@@ -1122,7 +1255,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
   // diagonals by one workgroup with a barrier between them
   s << "template <int G> __device__ __forceinline__ void fill_(const Inst* __restrict__ I, int d0, int d1) {\n"
        "  const Inst& q = I[blockIdx.y];  // instances sorted by size, largest first: blockIdx.y < active ones\n"
-       "  const Dims z = q.z;\n  const Tabs& T = q.T;\n"
+       "  const Dims z = q.z;\n  const Tabs T = q.T;  // in registers: the stores below cannot change it\n"
        "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl};\n"
        "  const int tid = blockIdx.x * blockDim.x + threadIdx.x, lane = tid & (G - 1);\n"
        "  for (int d = d0; d < d1; ++d) {\n";
@@ -1136,12 +1269,86 @@ bool generate(Model& m, std::string& src, std::string& err) {
     s << "      const int i = t, j = t + d;\n";
   else
     s << "      const int i = ilo + t, j = d - i;\n";
-  for (int t : m.order) s << "      tab_" << t << "<G>(_x, z, T, i, j, lane);\n";
+  // every tabulation's reads of other cells first, all in flight together
+  // (pinned here rather than sunk into the rules), then the tabulations
+  auto cell_body = [&](const std::string& ind, const std::string& G, const std::string& lane, const std::string& defer) {
+    s << ind << "CellV cv;\n";
+    for (int t : m.order) s << ind << "cv.r" << t << " = -1;\n";
+    for (int t : m.order) s << ind << "Pre_" << t << " p" << t << ";\n" << ind << "load_" << t << "(_x, z, T, i, j, p" << t << ");\n";
+    for (int t : m.order) s << ind << "pin_" << t << "(p" << t << ");\n";
+    for (int t : m.order)
+      s << ind << "tab_" << t << "<" << G << ", " << defer << ">(_x, z, T, i, j, " << lane << ", cv, p" << t << ");\n";
+  };
+  cell_body("      ", "G", "lane", "false");
   s << "    }\n"
        "    if (d + 1 < d1) __syncthreads();\n"
        "  }\n}\n";
   for (int G : {1, 8, 64})
     s << "extern \"C\" __global__ void adp_fill_" << G << "(const Inst* I, int d0, int d1) { fill_<" << G << ">(I, d0, d1); }\n";
+  if (m.tracks == 2 && !m.loops) {
+    // Two tracks read only cells (i' <= i, j' <= j), and without split loops
+    // a cell is O(1) work: the whole table in one persistent launch instead of
+    // one launch per anti-diagonal.  A wave owns a band of 64 rows (lane =
+    // row) and sweeps the columns skewed by the lane (lane r is on column
+    // s - r at step s), so every cell it reads from its own band is finished
+    // by an earlier step; after each step the wave waits for its stores, which
+    // the other lanes of the CU then read.  Rows of earlier bands come from
+    // their publications: each band raises a progress word (columns whose 64
+    // rows are done) every BAND_CH steps with an agent-scope release, and
+    // acquires the band above's word before a chunk that reads past what it
+    // has seen.  Bands are claimed in order by a ticket over (instance, band),
+    // so a band only waits for a band that a running wave has claimed; the
+    // waits are bounded and raise ctrl[1] instead of hanging.
+    s << "#define BAND_CH 32\n"
+         "extern \"C\" __global__ __launch_bounds__(64) void adp_band(const Inst* __restrict__ I, const int* __restrict__ boff,\n"
+         "                                                           int count, int* ctrl, int* prog) {\n"
+         "  const int lane = threadIdx.x;\n"
+         "  const int total = boff[count];\n"
+         "  for (;;) {\n"
+         "    int tk = 0;\n"
+         "    if (lane == 0) tk = __hip_atomic_fetch_add(&ctrl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "    tk = __builtin_amdgcn_readfirstlane(tk);\n"
+         "    if (tk >= total) break;\n"
+         "    int lo = 0, hi = count;  // boff[lo] <= tk < boff[hi]\n"
+         "    while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (boff[mid] <= tk) lo = mid; else hi = mid; }\n"
+         "    const int q = __builtin_amdgcn_readfirstlane(lo);\n"
+         "    const int band = tk - boff[q];\n"
+         "    const Inst& In = I[q];\n"
+         "    const Dims z = In.z;\n    const Tabs T = In.T;\n"
+         "    const Ctx _x{In.in1, In.in2, z.n1, z.n2, (const RnaPar*)In.rna, In.hl};\n"
+         "    const int i = band * 64 + lane, W = z.n2 + 1, steps = W + 63;\n"
+         "    const bool row_ok = i <= z.n1;\n"
+         "    int known = band == 0 ? W : 0;  // columns of the band above seen published\n"
+         "    for (int s0 = 0; s0 < steps; s0 += BAND_CH) {\n"
+         "      const int s1 = min(s0 + BAND_CH, steps), need = min(s1, W);\n"
+         "      if (known < need) {\n"
+         "        int k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&prog[tk - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "        for (int spin = 0; k < need; ++spin) {\n"
+         "          if (spin > (1 << 24)) { if (lane == 0) __hip_atomic_store(&ctrl[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); k = need; break; }\n"
+         "          __builtin_amdgcn_s_sleep(1);\n"
+         "          k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&prog[tk - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "        }\n"
+         "        known = k;\n"
+         "        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"agent\");\n"
+         "      }\n"
+         "      for (int st = s0; st < s1; ++st) {\n"
+         "        const int j = st - lane;\n"
+         "        if (row_ok && j >= 0 && j < W) {\n";
+    cell_body("          ", "1", "0", m.fixed_refs ? "true" : "false");
+    if (m.fixed_refs) s << "          store_cell(z, T, i, j, cv);  // no reference of the cell reads what it writes\n";
+    s << "        }\n"
+         "        __builtin_amdgcn_s_waitcnt(0x0F70);\n"
+         "        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, \"workgroup\");\n"
+         "      }\n"
+         "      const int done = min(s1 - 63, W);  // lane 63 finished column s1 - 64\n"
+         "      if (done > 0) {\n"
+         "        __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"agent\");\n"
+         "        if (lane == 0) __hip_atomic_store(&prog[tk], done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "      }\n"
+         "    }\n"
+         "  }\n"
+         "}\n";
+  }
   // finish: axiom cell (or best window), answer, backtrace; one thread per instance
   const Tab& AX = m.tabs[m.axiom];
   s << "extern \"C\" __global__ void adp_finish(const Inst* I) {\n"
@@ -1169,8 +1376,9 @@ bool generate(Model& m, std::string& src, std::string& err) {
       << "  *ans = " << Gen(m).cost(m.axiom, "i0", "j0") << ";\n";
   }
   s << "  out[0] = found ? 1 : 0; out[2] = i0; out[3] = j0;\n"
-    << "  if (!tr || !found" << (m.additive() ? " || true" : "") << ") { out[1] = 0; return; }  // sums have no backtrace\n"
-       "  int wr = 0;\n"
+    << "  if (!tr || !found" << (m.additive() ? " || true) { out[1] = 0; return; }  // sums have no backtrace\n"
+                                             : ") { out[1] = 0; return; }\n")
+    << "  int wr = 0;\n"
        "  bool over = false;\n"
        "  auto push = [&](int i, int j, int rule) {\n"
        "    if (wr < cap) { tr[wr].i = i; tr[wr].j = j; tr[wr].rule = rule; tr[wr].pos[0] = 0; tr[wr].pos[1] = 0; ++wr; }\n"
@@ -1219,6 +1427,7 @@ struct Grammar {
   std::vector<char> code;
   hipModule_t mod = nullptr;
   hipFunction_t fill[3] = {nullptr, nullptr, nullptr}, finish = nullptr;  // fill: 1, 8, 64 lanes per cell
+  hipFunction_t band = nullptr;  // two tracks without split loops: the band pipeline
   void* dev = nullptr;
   size_t dev_cap = 0;
   void* host = nullptr;  // pinned: answer, out[4], trace
@@ -1264,7 +1473,8 @@ Grammar* compile(const char* spec, bool load, std::string& err, int* code) {
         hipModuleGetFunction(&g->fill[0], g->mod, "adp_fill_1") != hipSuccess ||
         hipModuleGetFunction(&g->fill[1], g->mod, "adp_fill_8") != hipSuccess ||
         hipModuleGetFunction(&g->fill[2], g->mod, "adp_fill_64") != hipSuccess ||
-        hipModuleGetFunction(&g->finish, g->mod, "adp_finish") != hipSuccess) {
+        hipModuleGetFunction(&g->finish, g->mod, "adp_finish") != hipSuccess ||
+        (g->m.tracks == 2 && !g->m.loops && hipModuleGetFunction(&g->band, g->mod, "adp_band") != hipSuccess)) {
       err = "loading the compiled grammar failed";
       if (g->mod) (void)hipModuleUnload(g->mod);
       return nullptr;
@@ -1337,8 +1547,9 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
     x.D = m.tracks == 1 ? (m.window > 0 ? std::min<int64_t>(m.window, x.n1) : x.n1) : x.n1 + x.n2;
     x.cells = m.tracks == 1 ? (x.D + 1) * W1 - (x.D + 1) * x.D / 2 : W1 * (x.n2 + 1);
     if (x.cells > ((int64_t)1 << 34)) { err = "grammar tables too large"; return DPX_EINVAL; }
-    x.o_in1 = off; off += r256((size_t)x.n1 * ab + 16);
-    x.o_in2 = off; off += r256((size_t)x.n2 * ab + 16);
+    // one element of padding past each input: hoisted reads clamp their index to n
+    x.o_in1 = off; off += r256((size_t)(x.n1 + 1) * ab + 16);
+    x.o_in2 = off; off += r256((size_t)(x.n2 + 1) * ab + 16);
     x.o_tab = off;
     for (int t = 0; t < NT; ++t) {
       if (!m.tabs[t].used) continue;
@@ -1352,6 +1563,31 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
     cmax = std::max(cmax, m.tracks == 1 ? x.n1 + 1 : std::min(x.n1, x.n2) + 1);
     Dmax = std::max(Dmax, x.D);
   }
+  // instance records go to the device largest D first
+  std::vector<int> order(count);
+  for (int q = 0; q < count; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return qs[a].D > qs[b].D; });
+  // band pipeline (two tracks without split loops): per record the first
+  // band's ticket (count + 1 words), then the ticket and stall words, then
+  // one progress word per band -- staged on the host with the zeros
+  std::vector<int> band_blk;
+  size_t o_band = 0, o_ctrl = 0;
+  int64_t bands = 0;
+  if (g->band) {
+    for (int k = 0; k < count; ++k) bands += (qs[order[k]].n1 + 64) / 64;
+    if (bands < ((int64_t)1 << 28)) {
+      o_ctrl = r256((size_t)(count + 1) * 4) / 4;
+      band_blk.assign(o_ctrl + 64 + (size_t)bands, 0);
+      int64_t b = 0;
+      for (int k = 0; k < count; ++k) {
+        band_blk[k] = (int)b;
+        b += (qs[order[k]].n1 + 64) / 64;
+      }
+      band_blk[count] = (int)b;
+      o_band = off;
+      off += r256(band_blk.size() * 4);
+    }
+  }
   if (off > g->dev_cap) {
     if (g->dev) (void)hipFree(g->dev);
     g->dev = nullptr;
@@ -1359,15 +1595,13 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
     if (hipMalloc(&g->dev, off) != hipSuccess) { err = "device allocation for the grammar tables failed"; return DPX_ENOMEM; }
     g->dev_cap = off;
   }
-  // host staging: instance records (largest D first), inputs, then results
-  std::vector<int> order(count);
-  for (int q = 0; q < count; ++q) order[q] = q;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return qs[a].D > qs[b].D; });
+  // host staging: instance records, inputs, then results
   int64_t tr_total = 0;
   for (auto& x : qs) tr_total += x.cap;
   const size_t in_bytes = (size_t)(off1[count] - off1[0]) * ab + (m.tracks == 2 ? (size_t)(off2[count] - off2[0]) * ab : 0);
   const size_t res_bytes = (size_t)count * (ans4 + 16);
-  const size_t hbytes = r256(inst_bytes * count) + r256(in_bytes + 16) + r256(res_bytes) + sizeof(dpx_trace_t) * (size_t)std::max<int64_t>(tr_total, 1);
+  const size_t hbytes = r256(inst_bytes * count) + r256(in_bytes + 16) + r256(res_bytes) +
+                        r256(sizeof(dpx_trace_t) * (size_t)std::max<int64_t>(tr_total, 1)) + 256;
   if (hbytes > g->host_cap) {
     if (g->host) (void)hipHostFree(g->host);
     g->host = nullptr;
@@ -1380,6 +1614,7 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
   char* h_inst = h;
   char* h_res = h + r256(inst_bytes * count) + r256(in_bytes + 16);
   char* h_tr = h_res + r256(res_bytes);
+  int* h_stall = (int*)(h_tr + r256(sizeof(dpx_trace_t) * (size_t)std::max<int64_t>(tr_total, 1)));
   auto chk = [&](hipError_t e, const char* what) {
     if (e != hipSuccess) err = std::string(what) + ": " + hipGetErrorString(e);
     return e == hipSuccess;
@@ -1431,8 +1666,21 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
     if (m.tracks == 1) return n1max + 1 - d;
     return std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(cmax, d + 1), Dmax - d + 1));
   };
-  int active = count;
-  for (int64_t d = 0; d <= Dmax;) {
+  if (!band_blk.empty()) {
+    // band offsets and zeroed counters by copy (no memset node, so the call
+    // can sit in a captured graph)
+    if (!chk(hipMemcpyAsync(dev + o_band, band_blk.data(), band_blk.size() * 4, hipMemcpyHostToDevice, s), "band setup")) return DPX_EHIP;
+    const int* d_boff = (const int*)(dev + o_band);
+    int* d_ctrl = (int*)(dev + o_band) + o_ctrl;
+    int* d_prog = d_ctrl + 64;
+    int cnt = count;
+    void* bargs[] = {(void*)&d_inst, (void*)&d_boff, (void*)&cnt, (void*)&d_ctrl, (void*)&d_prog};
+    const unsigned grid = (unsigned)std::min<int64_t>(bands, 2048);
+    if (!chk(hipModuleLaunchKernel(g->band, grid, 1, 1, 64, 1, 1, 0, s, bargs, nullptr), "band launch")) return DPX_EHIP;
+    if (!chk(hipMemcpyAsync(h_stall, d_ctrl + 1, 4, hipMemcpyDeviceToHost, s), "band status")) return DPX_EHIP;
+  }
+  int active = band_blk.empty() ? count : 0;
+  for (int64_t d = 0; d <= Dmax && active > 0;) {
     while (active > 0 && qs[order[active - 1]].D < d) --active;
     const int64_t c = cnt_bound(d);
     int d1 = (int)d + 1;
@@ -1467,6 +1715,7 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
   for (int q = 0; q < count; ++q)
     if (!chk(hipMemcpyAsync(h_res + (ans4 + 16) * q, dev + qs[q].o_ans, ans4 + 16, hipMemcpyDeviceToHost, s), "result copy")) return DPX_EHIP;
   if (!chk(hipStreamSynchronize(s), "grammar run")) return DPX_EHIP;
+  if (!band_blk.empty() && *h_stall) { err = "band pipeline stalled (a band waited too long for the band above)"; return DPX_EHIP; }
   int rc = DPX_OK;
   int64_t tro = 0;
   for (int q = 0; q < count; ++q) {
