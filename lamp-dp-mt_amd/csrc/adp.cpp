@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -244,6 +245,7 @@ struct Model {
   int n_rules = 0, max_pos = 0;
   bool loops = false;
   bool fixed_refs = true;  // every table reference of a cell body is (i + c, j + c')
+  bool reg_band = false;   // two tracks, references (i, j - c) / (i - 1, j - c): the register band kernel
 
   // yields (BaseParsers.scala: Parser.min/max, Concat hints)
   int nmin(const Node* n) const {
@@ -715,14 +717,28 @@ struct Gen {
   // at an index that is in bounds whatever i and j are, so they need no
   // guard and issue together; the rules' conditions still decide their use
   std::map<std::string, std::string>* pre = nullptr;
-  std::vector<std::array<std::string, 3>>* pre_decl = nullptr;  // type, name, expression
+  std::vector<std::array<std::string, 4>>* pre_decl = nullptr;  // type, name, expression, input "track var c"
+  // band kernel with registers: every reference is (i, j - c) or (i - 1,
+  // j - c) with c small; it reads R.v<t>_<di>_<dj> / R.r<t>_<di>_<dj>, filled
+  // each step from the lanes' histories and the row above the band
+  std::set<std::array<int, 3>>* regs = nullptr;  // (tab, -ci, -cj) referenced
+  bool reg_ok = true;
   explicit Gen(const Model& mm, bool in_cell = false) : m(mm), cell(in_cell) {}
-  std::string hoist(const std::string& type, const std::string& expr) {
+  std::string reg(const char* f, int t, const std::string& a, const std::string& b) {
+    int ci = 0, cj = 0;
+    if (!affine(a, 'i', ci) || !affine(b, 'j', cj) || ci < -1 || ci > 0 || cj > 0 || -cj + (ci ? 1 : 0) > 4) {
+      reg_ok = false;
+      return "0";
+    }
+    regs->insert({t, -ci, -cj});
+    return std::string("R.") + f + std::to_string(t) + "_" + std::to_string(-ci) + "_" + std::to_string(-cj);
+  }
+  std::string hoist(const std::string& type, const std::string& expr, const std::string& in = "") {
     auto it = pre->find(expr);
     if (it != pre->end()) return it->second;
     const std::string nm = "_l" + std::to_string(pre->size());
     (*pre)[expr] = nm;
-    pre_decl->push_back({type, nm, expr});
+    pre_decl->push_back({type, nm, expr, in});
     return nm;
   }
   bool self(const std::string& a, const std::string& b) {
@@ -733,13 +749,15 @@ struct Gen {
   }
   std::string cost(int t, const std::string& a, const std::string& b) {
     if (self(a, b)) return "cv.c" + std::to_string(t);
-    if (pre) return hoist("answer_t", "((const answer_t*)T.c[" + std::to_string(t) + "])[sidx_(z," + a + "," + b + ")]");
+    if (regs) return reg("v", t, a, b);
+    if (pre) return hoist("answer_t", "((const GLOBAL_ answer_t*)T.c[" + std::to_string(t) + "])[sidx_(z," + a + "," + b + ")]");
     return "((const answer_t*)T.c[" + std::to_string(t) + "])[idx_(z," + a + "," + b + ")]";
   }
   std::string valid(int t, const std::string& a, const std::string& b) {
     if (!m.empty.empty()) return "!(" + cost(t, a, b) + "==(answer_t)(" + m.empty + "))";
     if (self(a, b)) return "(cv.r" + std::to_string(t) + "!=-1)";
-    if (pre) return "(" + hoist("short", "T.r[" + std::to_string(t) + "][sidx_(z," + a + "," + b + ")]") + "!=-1)";
+    if (regs) return "(" + reg("r", t, a, b) + "!=-1)";
+    if (pre) return "(" + hoist("short", "((const GLOBAL_ short*)T.r[" + std::to_string(t) + "])[sidx_(z," + a + "," + b + ")]") + "!=-1)";
     return "(T.r[" + std::to_string(t) + "][idx_(z," + a + "," + b + ")]!=-1)";
   }
   void scs(int mn, int mx, const std::string& a, const std::string& b, AltCode& o) {
@@ -752,8 +770,10 @@ struct Gen {
   std::string input(int track, const std::string& a) {
     int c = 0;
     const std::string in = track == 1 ? "_in1" : "_in2";
-    if (pre && (affine(a, 'i', c) || affine(a, 'j', c)))
-      return hoist("input_t", in + "[min(max(" + a + ", 0), z.n" + std::to_string(m.tracks == 1 ? 1 : track) + ")]");
+    char v = 'i';
+    if (pre && (affine(a, 'i', c) || (v = 'j', affine(a, 'j', c))))
+      return hoist("input_t", in + "[min(max(" + a + ", 0), z.n" + std::to_string(m.tracks == 1 ? 1 : track) + ")]",
+                   std::to_string(track) + " " + v + " " + std::to_string(c));
     return in + "[" + a + "]";
   }
   std::string call(int fn, const std::vector<std::string>& args) const {
@@ -1011,7 +1031,7 @@ __device__ __forceinline__ int rMlmm_(const Ctx& x, int i, int j) { return x.rna
 bool generate(Model& m, std::string& src, std::string& err) {
   std::vector<std::vector<AltCode>> alts(m.tabs.size());
   std::vector<std::map<std::string, std::string>> pre(m.tabs.size());
-  std::vector<std::vector<std::array<std::string, 3>>> pre_decl(m.tabs.size());
+  std::vector<std::vector<std::array<std::string, 4>>> pre_decl(m.tabs.size());
   // second pass: every reference fixed, so the reads are hoisted
   for (int pass = 0; pass < 2; ++pass)
   for (int t : m.order) {
@@ -1041,6 +1061,33 @@ bool generate(Model& m, std::string& src, std::string& err) {
       alts[t].push_back(o);
     }
   }
+  // third pass: the register form of the band kernel, when every reference
+  // allows it
+  std::vector<std::vector<AltCode>> alts_r(m.tabs.size());
+  std::vector<std::map<std::string, std::string>> pre_r(m.tabs.size());
+  std::vector<std::vector<std::array<std::string, 4>>> pre_decl_r(m.tabs.size());
+  std::set<std::array<int, 3>> rset;
+  bool reg_band = m.tracks == 2 && !m.loops && m.fixed_refs;
+  for (int t : m.order) {
+    if (!reg_band) break;
+    const Tab& T = m.tabs[t];
+    std::vector<NP> flat;
+    if (!flatten(m, T.p, true, flat, err)) return false;
+    for (int q = 0; q < (int)flat.size(); ++q) {
+      Gen g(m, true);
+      g.pre = &pre_r[t];
+      g.pre_decl = &pre_decl_r[t];
+      g.regs = &rset;
+      AltCode o;
+      o.rule = T.id + q;
+      std::vector<std::string> vals;
+      if (!g.gen(flat[q].get(), "i", "j", o, vals) || vals.size() != 1) { err = g.err; return false; }
+      if (!g.reg_ok) reg_band = false;
+      o.value = vals[0];
+      alts_r[t].push_back(o);
+    }
+  }
+  m.reg_band = reg_band;
   const int NT = (int)m.tabs.size();
   std::ostringstream s;
   s << "// generated by libdpx from an ADP grammar (hipRTC provides the HIP runtime declarations)\n"
@@ -1103,6 +1150,27 @@ bool generate(Model& m, std::string& src, std::string& err) {
   // the group, rules without a loop run on lane 0, and the group's
   // candidates are merged by value, then by (rule, outer split, inner split)
   // -- the first strictly better candidate of the sequential loop order.
+  s << "template <class V> __device__ __forceinline__ V shfl_up1_(V v) {  // lane l gets lane l - 1's v\n"
+       "  constexpr int N = (sizeof(V) + 3) / 4;\n"
+       "  int u[N] = {};\n"
+       "  __builtin_memcpy(u, &v, sizeof(V));\n"
+       "  for (int q = 0; q < N; ++q) u[q] = __shfl_up(u[q], 1, 64);\n"
+       "  __builtin_memcpy(&v, u, sizeof(V));\n"
+       "  return v;\n}\n"
+       "template <class V> __device__ __forceinline__ V shfl_idx_(V v, int l) {  // lane l's v\n"
+       "  constexpr int N = (sizeof(V) + 3) / 4;\n"
+       "  int u[N] = {};\n"
+       "  __builtin_memcpy(u, &v, sizeof(V));\n"
+       "  for (int q = 0; q < N; ++q) u[q] = __shfl(u[q], l, 64);\n"
+       "  __builtin_memcpy(&v, u, sizeof(V));\n"
+       "  return v;\n}\n"
+       "template <class V> __device__ __forceinline__ V readlane_(V v, int l) {  // l is wave-uniform\n"
+       "  constexpr int N = (sizeof(V) + 3) / 4;\n"
+       "  int u[N] = {};\n"
+       "  __builtin_memcpy(u, &v, sizeof(V));\n"
+       "  for (int q = 0; q < N; ++q) u[q] = __builtin_amdgcn_readlane(u[q], l);\n"
+       "  __builtin_memcpy(&v, u, sizeof(V));\n"
+       "  return v;\n}\n";
   s << "template <class V> __device__ __forceinline__ void pin_(const V& v) {\n"
        "  int u[(sizeof(V) + 3) / 4] = {};\n"
        "  __builtin_memcpy(u, &v, sizeof(V));\n"
@@ -1118,6 +1186,9 @@ bool generate(Model& m, std::string& src, std::string& err) {
   for (auto& T : m.tabs) used_tabs += T.used ? 1 : 0;
   // the cell's results in registers: read by the later tabulations of the
   // cell, stored by the last one (DEFER) or each by its own
+  // table and input accesses through global pointers (not flat): they count
+  // only in vmcnt, so the lane shifts (LDS counter) never wait for them
+  s << "#define GLOBAL_ __attribute__((address_space(1)))\n";
   s << "struct CellV {";
   for (int t : m.order) {
     s << " answer_t c" << t << "; short r" << t << ";";
@@ -1127,31 +1198,36 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "__device__ __forceinline__ void store_cell(const Dims& z, const Tabs& T, int i, int j, const CellV& cv) {\n"
        "  const long long x = idx_(z, i, j);\n";
   for (int t : m.order) {
-    s << "  ((answer_t*)T.c[" << t << "])[x] = cv.c" << t << ";\n  T.r[" << t << "][x] = cv.r" << t << ";\n";
-    for (int q = 0; q < m.tabs[t].cat; ++q) s << "  T.p[" << t << "][" << q << "][x] = cv.p" << t << "_" << q << ";\n";
+    s << "  ((GLOBAL_ answer_t*)T.c[" << t << "])[x] = cv.c" << t << ";\n  ((GLOBAL_ short*)T.r[" << t << "])[x] = cv.r" << t << ";\n";
+    for (int q = 0; q < m.tabs[t].cat; ++q)
+      s << "  ((GLOBAL_ int*)T.p[" << t << "][" << q << "])[x] = cv.p" << t << "_" << q << ";\n";
   }
   s << "}\n";
+  auto emit_tabs = [&](const std::vector<std::vector<AltCode>>& A, const std::vector<std::vector<std::array<std::string, 4>>>& PD,
+                       const std::string& sx, const std::string& extra) {
   for (int t : m.order) {
     const Tab& T = m.tabs[t];
     s << "// ---- " << T.name << "[i,j]: rules " << T.id << ".." << T.id + T.alt - 1 << "\n"
-      << "struct Pre_" << t << " {";
-    for (auto& d : pre_decl[t]) s << " " << d[0] << " " << d[1] << ";";
+      << "struct Pre" << sx << "_" << t << " {";
+    for (auto& d : PD[t]) s << " " << d[0] << " " << d[1] << ";";
     s << " int pad_; };\n"
-      << "__device__ __forceinline__ void load_" << t << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, Pre_" << t
-      << "& p) {\n  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n";
-    for (auto& d : pre_decl[t]) s << "  p." << d[1] << " = " << d[2] << ";\n";
+      << "__device__ __forceinline__ void load" << sx << "_" << t << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, Pre" << sx << "_" << t
+      << "& p) {\n  const GLOBAL_ input_t* _in1 = (const GLOBAL_ input_t*)_x.in1;\n"
+         "  const GLOBAL_ input_t* _in2 = (const GLOBAL_ input_t*)_x.in2;\n  (void)_in1; (void)_in2;\n";
+    for (auto& d : PD[t]) s << "  p." << d[1] << " = " << d[2] << ";\n";
     s << "  p.pad_ = 0;\n}\n"
-      << "__device__ __forceinline__ void pin_" << t << "(const Pre_" << t << "& p) {";
-    for (auto& d : pre_decl[t]) s << " pin_(p." << d[1] << ");";
+      << "__device__ __forceinline__ void pin" << sx << "_" << t << "(const Pre" << sx << "_" << t << "& p) {";
+    for (auto& d : PD[t]) s << " pin_(p." << d[1] << ");";
     s << " (void)p; }\n"
-      << "template <int G, bool DEFER> __device__ __forceinline__ void tab_" << t
-      << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, int lane, CellV& cv, const Pre_" << t << "& _p) {\n"
+      << "template <int G, bool DEFER> __device__ __forceinline__ void tab" << sx << "_" << t
+      << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, int lane, CellV& cv, const Pre" << sx << "_" << t << "& _p" << extra
+      << ") {\n"
       << "  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n"
       << "  answer_t _c0" << (m.empty.empty() ? "{}" : " = (answer_t)(" + m.empty + ")") << ";\n"
       << "  int _r0 = -1, _p0 = 0, _p1 = 0, _o0 = 0, _o1 = 0;\n";
-    for (auto& d : pre_decl[t]) s << "  const " << d[0] << " " << d[1] << " = _p." << d[1] << ";\n";
+    for (auto& d : PD[t]) s << "  const " << d[0] << " " << d[1] << " = _p." << d[1] << ";\n";
     s << "  (void)_p;\n";
-    for (auto& o : alts[t]) {
+    for (auto& o : A[t]) {
       std::string ind = "  ";
       s << ind << "{  // rule " << o.rule << "\n";
       int close = 1;
@@ -1251,6 +1327,17 @@ bool generate(Model& m, std::string& src, std::string& err) {
       s << "  if (G > 1) { __builtin_amdgcn_s_waitcnt(0x0F70); __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, \"workgroup\"); }\n";
     s << "}\n";
   }
+  };
+  emit_tabs(alts, pre_decl, "", "");
+  if (m.reg_band) {
+    // what the register band kernel's cells read: R.v<t>_<di>_<dj> (value)
+    // and R.r<t>_<di>_<dj> (rule) of cell (i - di, j - dj)
+    s << "struct Regs {";
+    for (auto& k : rset)
+      s << " answer_t v" << k[0] << "_" << k[1] << "_" << k[2] << "; short r" << k[0] << "_" << k[1] << "_" << k[2] << ";";
+    s << " int pad_; };\n";
+    emit_tabs(alts_r, pre_decl_r, "R", ", const Regs& R");
+  }
   // fill: one diagonal per launch (d0 == d1 - 1), or a run of small
   // diagonals by one workgroup with a barrier between them
   s << "template <int G> __device__ __forceinline__ void fill_(const Inst* __restrict__ I, int d0, int d1) {\n"
@@ -1285,7 +1372,149 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "  }\n}\n";
   for (int G : {1, 8, 64})
     s << "extern \"C\" __global__ void adp_fill_" << G << "(const Inst* I, int d0, int d1) { fill_<" << G << ">(I, d0, d1); }\n";
-  if (m.tracks == 2 && !m.loops) {
+  if (m.reg_band) {
+    // The band pipeline below with the band's own cells in registers: every
+    // reference is (i, j - dj) or (i - 1, j - dj) with dj small.  Lane r
+    // keeps the cells it computed in the last steps (h<t>_k: k + 1 steps
+    // ago); (i, j - dj) is its own h_(dj-1), (i - 1, j - dj) is lane r - 1's
+    // h_dj (one lane shift), or for lane 0 the band above's last row, read
+    // per chunk of BAND_CH columns into lanes 0..BAND_CH-1 after the acquire
+    // and taken by readlane.  So no step waits for the stores of the one
+    // before; the stores (tables for the backtrace and the next band) are
+    // waited for only by the release before each publication.  The inputs
+    // of the next step are loaded before this step's stores.
+    std::map<int, int> H;   // history length per tabulation
+    std::set<int> above;    // tabulations read from the row above
+    for (auto& k : rset) {
+      H[k[0]] = std::max(H[k[0]], k[2] + k[1]);
+      if (k[1] == 1) above.insert(k[0]);
+    }
+    s << "#define BAND_CH 16\n"
+         "extern \"C\" __global__ __launch_bounds__(64) void adp_band(const Inst* __restrict__ I, const int* __restrict__ boff,\n"
+         "                                                           int count, int* ctrl, int* prog) {\n"
+         "  const int lane = threadIdx.x;\n"
+         "  const int total = boff[count];\n"
+         "  for (;;) {\n"
+         "    int tk = 0;\n"
+         "    if (lane == 0) tk = __hip_atomic_fetch_add(&ctrl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "    tk = __builtin_amdgcn_readfirstlane(tk);\n"
+         "    if (tk >= total) break;\n"
+         "    int lo = 0, hi = count;  // boff[lo] <= tk < boff[hi]\n"
+         "    while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (boff[mid] <= tk) lo = mid; else hi = mid; }\n"
+         "    const int q = __builtin_amdgcn_readfirstlane(lo);\n"
+         "    const int band = tk - boff[q];\n"
+         "    const Inst& In = I[q];\n"
+         "    const Dims z = In.z;\n    const Tabs T = In.T;\n"
+         "    const Ctx _x{In.in1, In.in2, z.n1, z.n2, (const RnaPar*)In.rna, In.hl};\n"
+         "    const int i = band * 64 + lane, W = z.n2 + 1, steps = W + 63;\n"
+         "    const bool row_ok = i <= z.n1;\n"
+         "    int known = band == 0 ? W : 0;  // columns of the band above seen published\n"
+         "    const long long up = (long long)(band * 64 - 1) * W;  // cell (last row of the band above, 0)\n";
+    for (auto& h : H)
+      for (int k = 0; k < h.second; ++k) s << "    answer_t h" << h.first << "_" << k << "{}; short g" << h.first << "_" << k << " = -1;\n";
+    for (int t : above) s << "    answer_t bv" << t << "{}, pv" << t << "{}; short br" << t << " = -1, pr" << t << " = -1;\n";
+    // inputs: index i + c once per band, index j + c per chunk as two
+    // windows of 64 (columns s0 - 64 + c + l and s0 + c + l on lane l)
+    struct InRef { int t; std::string name; int track; char v; int c; };
+    std::vector<InRef> ins;
+    for (int t : m.order)
+      for (auto& d : pre_decl_r[t]) {
+        InRef r{t, d[1], 0, 'i', 0};
+        char v = 0;
+        if (std::sscanf(d[3].c_str(), "%d %c %d", &r.track, &v, &r.c) != 3) { err = "internal: input reference"; return false; }
+        r.v = v;
+        ins.push_back(r);
+      }
+    auto in_at = [&](const InRef& r, const std::string& x) {
+      const std::string k = std::to_string(m.tracks == 1 ? 1 : r.track);
+      return "((const GLOBAL_ input_t*)In.in" + std::to_string(r.track) + ")[min(max(" + x + ", 0), z.n" + k + ")]";
+    };
+    for (auto& r : ins) {
+      const std::string nm = std::to_string(r.t) + r.name;
+      if (r.v == 'i') s << "    const input_t bi" << nm << " = " << in_at(r, "i + " + std::to_string(r.c)) << ";\n";
+      else s << "    input_t w0" << nm << "{}, w1" << nm << "{};\n";
+    }
+    s << "    for (int s0 = 0; s0 < steps; s0 += BAND_CH) {\n"
+         "      const int s1 = min(s0 + BAND_CH, steps), need = min(s1, W);\n"
+         "      if (known < need) {\n"
+         "        int k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&prog[tk - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "        for (int spin = 0; k < need; ++spin) {\n"
+         "          if (spin > (1 << 24)) { if (lane == 0) __hip_atomic_store(&ctrl[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); k = need; break; }\n"
+         "          __builtin_amdgcn_s_sleep(1);\n"
+         "          k = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&prog[tk - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "        }\n"
+         "        known = k;\n"
+         "        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"agent\");\n"
+         "      }\n";
+    if (!above.empty()) {
+      s << "      if (band > 0) {  // columns s0 .. s0 + BAND_CH - 1 of the row above, one per lane\n";
+      for (int t : above) s << "        pv" << t << " = bv" << t << "; pr" << t << " = br" << t << ";\n";
+      s << "        if (lane < BAND_CH) {\n"
+           "          const long long x = up + min(s0 + lane, W - 1);\n";
+      for (int t : above)
+        s << "          bv" << t << " = ((const GLOBAL_ answer_t*)T.c[" << t << "])[x]; br" << t << " = ((const GLOBAL_ short*)T.r[" << t << "])[x];\n";
+      s << "        }\n"
+           "      }\n";
+    }
+    for (auto& r : ins) {
+      if (r.v != 'j') continue;
+      const std::string nm = std::to_string(r.t) + r.name;
+      s << "      w0" << nm << " = " << in_at(r, "s0 - 64 + " + std::to_string(r.c) + " + lane") << ";\n"
+        << "      w1" << nm << " = " << in_at(r, "s0 + " + std::to_string(r.c) + " + lane") << ";\n";
+    }
+    s << "      for (int st = s0; st < s1; ++st) {\n"
+         "        const int j = st - lane, o = st - lane - s0;  // o: this lane's column in the chunk's windows\n"
+         "        (void)o;\n"
+         "        Regs R;\n";
+    for (auto& k : rset) {
+      const std::string f = std::to_string(k[0]) + "_" + std::to_string(k[1]) + "_" + std::to_string(k[2]);
+      const int t = k[0], di = k[1], dj = k[2];
+      if (di == 0) {
+        s << "        R.v" << f << " = h" << t << "_" << dj - 1 << "; R.r" << f << " = g" << t << "_" << dj - 1 << ";\n";
+      } else {
+        s << "        {\n"
+             "          const int c = st - " << dj << " - s0, l = c >= 0 ? c : max(c + BAND_CH, 0);\n"
+          << "          const answer_t bv = c >= 0 ? readlane_(bv" << t << ", l) : readlane_(pv" << t << ", l);\n"
+          << "          const short br = c >= 0 ? readlane_(br" << t << ", l) : readlane_(pr" << t << ", l);\n"
+          << "          const answer_t uv = shfl_up1_(h" << t << "_" << dj << ");\n"
+          << "          const short ur = shfl_up1_(g" << t << "_" << dj << ");\n"
+          << "          R.v" << f << " = lane == 0 ? bv : uv; R.r" << f << " = lane == 0 ? br : ur;\n"
+          << "        }\n";
+      }
+    }
+    s << "        CellV cv;\n";
+    for (int t : m.order) s << "        cv.c" << t << " = answer_t{}; cv.r" << t << " = -1;\n";
+    for (int t : m.order) s << "        PreR_" << t << " p" << t << ";\n        p" << t << ".pad_ = 0;\n";
+    for (auto& r : ins) {
+      const std::string nm = std::to_string(r.t) + r.name;
+      if (r.v == 'i') {
+        s << "        p" << r.t << "." << r.name << " = bi" << nm << ";\n";
+      } else {
+        s << "        {\n"
+          << "          const input_t a = shfl_idx_(w1" << nm << ", max(o, 0)), b = shfl_idx_(w0" << nm << ", min(o + 64, 63));\n"
+          << "          p" << r.t << "." << r.name << " = o >= 0 ? a : b;\n"
+          << "        }\n";
+      }
+    }
+    s << "        if (row_ok && j >= 0 && j < W) {\n";
+    for (int t : m.order) s << "          tabR_" << t << "<1, true>(_x, z, T, i, j, 0, cv, p" << t << ", R);\n";
+    s << "          store_cell(z, T, i, j, cv);\n"
+         "        }\n";
+    for (auto& h : H) {
+      for (int k = h.second - 1; k > 0; --k)
+        s << "        h" << h.first << "_" << k << " = h" << h.first << "_" << k - 1 << "; g" << h.first << "_" << k << " = g" << h.first << "_" << k - 1 << ";\n";
+      s << "        h" << h.first << "_0 = cv.c" << h.first << "; g" << h.first << "_0 = cv.r" << h.first << ";\n";
+    }
+    s << "      }\n"
+         "      const int done = min(s1 - 63, W);  // lane 63 finished column s1 - 64\n"
+         "      if (done > 0) {\n"
+         "        __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"agent\");\n"
+         "        if (lane == 0) __hip_atomic_store(&prog[tk], done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "      }\n"
+         "    }\n"
+         "  }\n"
+         "}\n";
+  } else if (m.tracks == 2 && !m.loops) {
     // Two tracks read only cells (i' <= i, j' <= j), and without split loops
     // a cell is O(1) work: the whole table in one persistent launch instead of
     // one launch per anti-diagonal.  A wave owns a band of 64 rows (lane =
