@@ -1115,7 +1115,12 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  const long long d = j - i;  // diagonal-major: the cells of one diagonal are contiguous\n"
          "  return d * (z.n1 + 1) - d * (d - 1) / 2 + i;\n}\n";
   else
-    s << "__device__ __forceinline__ long long idx_(const Dims& z, int i, int j) { return (long long)i * (z.n2 + 1) + j; }\n";
+    // bands of 64 rows, each stored by anti-diagonal (r + j), lanes (rows)
+    // contiguous: one step of the band pipeline writes 64 adjacent cells, and
+    // so does one anti-diagonal of a band in the diagonal kernels
+    s << "__device__ __forceinline__ long long idx_(const Dims& z, int i, int j) {\n"
+         "  const int b = i >> 6, r = i & 63;\n"
+         "  return ((long long)b * (z.n2 + 64) + (r + j)) * 64 + r;\n}\n";
   // (i, j) when it is a cell of the tables, else cell 0: in bounds for any i, j
   if (m.tracks == 1)
     s << "__device__ __forceinline__ long long sidx_(const Dims& z, int i, int j) {\n"
@@ -1154,7 +1159,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "  constexpr int N = (sizeof(V) + 3) / 4;\n"
        "  int u[N] = {};\n"
        "  __builtin_memcpy(u, &v, sizeof(V));\n"
-       "  for (int q = 0; q < N; ++q) u[q] = __shfl_up(u[q], 1, 64);\n"
+       "  for (int q = 0; q < N; ++q) u[q] = __builtin_amdgcn_update_dpp(0, u[q], 0x138, 0xf, 0xf, false);  // wave_shr:1\n"
        "  __builtin_memcpy(&v, u, sizeof(V));\n"
        "  return v;\n}\n"
        "template <class V> __device__ __forceinline__ V shfl_idx_(V v, int l) {  // lane l's v\n"
@@ -1195,14 +1200,15 @@ bool generate(Model& m, std::string& src, std::string& err) {
     for (int q = 0; q < m.tabs[t].cat; ++q) s << " int p" << t << "_" << q << ";";
   }
   s << " };\n"
-       "__device__ __forceinline__ void store_cell(const Dims& z, const Tabs& T, int i, int j, const CellV& cv) {\n"
-       "  const long long x = idx_(z, i, j);\n";
+       "__device__ __forceinline__ void store_at(const Tabs& T, long long x, const CellV& cv) {\n";
   for (int t : m.order) {
     s << "  ((GLOBAL_ answer_t*)T.c[" << t << "])[x] = cv.c" << t << ";\n  ((GLOBAL_ short*)T.r[" << t << "])[x] = cv.r" << t << ";\n";
     for (int q = 0; q < m.tabs[t].cat; ++q)
       s << "  ((GLOBAL_ int*)T.p[" << t << "][" << q << "])[x] = cv.p" << t << "_" << q << ";\n";
   }
-  s << "}\n";
+  s << "}\n"
+       "__device__ __forceinline__ void store_cell(const Dims& z, const Tabs& T, int i, int j, const CellV& cv) {\n"
+       "  store_at(T, idx_(z, i, j), cv);\n}\n";
   auto emit_tabs = [&](const std::vector<std::vector<AltCode>>& A, const std::vector<std::vector<std::array<std::string, 4>>>& PD,
                        const std::string& sx, const std::string& extra) {
   for (int t : m.order) {
@@ -1409,12 +1415,14 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "    const int i = band * 64 + lane, W = z.n2 + 1, steps = W + 63;\n"
          "    const bool row_ok = i <= z.n1;\n"
          "    int known = band == 0 ? W : 0;  // columns of the band above seen published\n"
-         "    const long long up = (long long)(band * 64 - 1) * W;  // cell (last row of the band above, 0)\n";
+         "    const int up = band * 64 - 1;  // last row of the band above\n"
+         "    const long long xb = (long long)band * (z.n2 + 64) * 64 + lane;\n";
     for (auto& h : H)
       for (int k = 0; k < h.second; ++k) s << "    answer_t h" << h.first << "_" << k << "{}; short g" << h.first << "_" << k << " = -1;\n";
     for (int t : above) s << "    answer_t bv" << t << "{}, pv" << t << "{}; short br" << t << " = -1, pr" << t << " = -1;\n";
-    // inputs: index i + c once per band, index j + c per chunk as two
-    // windows of 64 (columns s0 - 64 + c + l and s0 + c + l on lane l)
+    // inputs: index i + c once per band; index j + c is what the lane above
+    // had one step before, so it moves down one lane per step and lane 0
+    // takes column st + c from the chunk's window (s0 + c + l on lane l)
     struct InRef { int t; std::string name; int track; char v; int c; };
     std::vector<InRef> ins;
     for (int t : m.order)
@@ -1432,7 +1440,8 @@ bool generate(Model& m, std::string& src, std::string& err) {
     for (auto& r : ins) {
       const std::string nm = std::to_string(r.t) + r.name;
       if (r.v == 'i') s << "    const input_t bi" << nm << " = " << in_at(r, "i + " + std::to_string(r.c)) << ";\n";
-      else s << "    input_t w0" << nm << "{}, w1" << nm << "{};\n";
+      else
+        s << "    input_t w1" << nm << "{}, x" << nm << " = " << in_at(r, "-1 - lane + " + std::to_string(r.c)) << ";  // step -1\n";
     }
     s << "    for (int s0 = 0; s0 < steps; s0 += BAND_CH) {\n"
          "      const int s1 = min(s0 + BAND_CH, steps), need = min(s1, W);\n"
@@ -1450,7 +1459,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
       s << "      if (band > 0) {  // columns s0 .. s0 + BAND_CH - 1 of the row above, one per lane\n";
       for (int t : above) s << "        pv" << t << " = bv" << t << "; pr" << t << " = br" << t << ";\n";
       s << "        if (lane < BAND_CH) {\n"
-           "          const long long x = up + min(s0 + lane, W - 1);\n";
+           "          const long long x = idx_(z, up, min(s0 + lane, W - 1));\n";
       for (int t : above)
         s << "          bv" << t << " = ((const GLOBAL_ answer_t*)T.c[" << t << "])[x]; br" << t << " = ((const GLOBAL_ short*)T.r[" << t << "])[x];\n";
       s << "        }\n"
@@ -1459,12 +1468,10 @@ bool generate(Model& m, std::string& src, std::string& err) {
     for (auto& r : ins) {
       if (r.v != 'j') continue;
       const std::string nm = std::to_string(r.t) + r.name;
-      s << "      w0" << nm << " = " << in_at(r, "s0 - 64 + " + std::to_string(r.c) + " + lane") << ";\n"
-        << "      w1" << nm << " = " << in_at(r, "s0 + " + std::to_string(r.c) + " + lane") << ";\n";
+      s << "      w1" << nm << " = " << in_at(r, "s0 + " + std::to_string(r.c) + " + lane") << ";\n";
     }
     s << "      for (int st = s0; st < s1; ++st) {\n"
-         "        const int j = st - lane, o = st - lane - s0;  // o: this lane's column in the chunk's windows\n"
-         "        (void)o;\n"
+         "        const int j = st - lane;\n"
          "        Regs R;\n";
     for (auto& k : rset) {
       const std::string f = std::to_string(k[0]) + "_" + std::to_string(k[1]) + "_" + std::to_string(k[2]);
@@ -1491,14 +1498,15 @@ bool generate(Model& m, std::string& src, std::string& err) {
         s << "        p" << r.t << "." << r.name << " = bi" << nm << ";\n";
       } else {
         s << "        {\n"
-          << "          const input_t a = shfl_idx_(w1" << nm << ", max(o, 0)), b = shfl_idx_(w0" << nm << ", min(o + 64, 63));\n"
-          << "          p" << r.t << "." << r.name << " = o >= 0 ? a : b;\n"
+          << "          const input_t a = shfl_up1_(x" << nm << "), b = readlane_(w1" << nm << ", st - s0);\n"
+          << "          x" << nm << " = lane == 0 ? b : a;\n"
+          << "          p" << r.t << "." << r.name << " = x" << nm << ";\n"
           << "        }\n";
       }
     }
     s << "        if (row_ok && j >= 0 && j < W) {\n";
     for (int t : m.order) s << "          tabR_" << t << "<1, true>(_x, z, T, i, j, 0, cv, p" << t << ", R);\n";
-    s << "          store_cell(z, T, i, j, cv);\n"
+    s << "          store_at(T, xb + (long long)st * 64, cv);  // idx_(z, i, j): r + j == st\n"
          "        }\n";
     for (auto& h : H) {
       for (int k = h.second - 1; k > 0; --k)
@@ -1774,7 +1782,7 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
     if (x.n2 > 0 && !in2) { err = "null input"; return DPX_EINVAL; }
     const int64_t W1 = x.n1 + 1;
     x.D = m.tracks == 1 ? (m.window > 0 ? std::min<int64_t>(m.window, x.n1) : x.n1) : x.n1 + x.n2;
-    x.cells = m.tracks == 1 ? (x.D + 1) * W1 - (x.D + 1) * x.D / 2 : W1 * (x.n2 + 1);
+    x.cells = m.tracks == 1 ? (x.D + 1) * W1 - (x.D + 1) * x.D / 2 : (x.n1 + 64) / 64 * 64 * (x.n2 + 64);
     if (x.cells > ((int64_t)1 << 34)) { err = "grammar tables too large"; return DPX_EINVAL; }
     // one element of padding past each input: hoisted reads clamp their index to n
     x.o_in1 = off; off += r256((size_t)(x.n1 + 1) * ab + 16);
