@@ -1617,25 +1617,32 @@ bool generate(Model& m, std::string& src, std::string& err) {
                                              : ") { out[1] = 0; return; }\n")
     << "  int wr = 0;\n"
        "  bool over = false;\n"
+       // the queue's last 1024 records also in LDS: the walk reads them back
+       // without a round trip through the trace in memory
+       "  __shared__ int qi_[1024], qj_[1024], qt_[1024];\n"
        "  auto push = [&](int i, int j, int rule) {\n"
-       "    if (wr < cap) { tr[wr].i = i; tr[wr].j = j; tr[wr].rule = rule; tr[wr].pos[0] = 0; tr[wr].pos[1] = 0; ++wr; }\n"
-       "    else over = true;\n"
+       "    if (wr < cap) {\n"
+       "      tr[wr].i = i; tr[wr].j = j; tr[wr].rule = rule; tr[wr].pos[0] = 0; tr[wr].pos[1] = 0;\n"
+       "      qi_[wr & 1023] = i; qj_[wr & 1023] = j; qt_[wr & 1023] = rule;\n"
+       "      ++wr;\n"
+       "    } else over = true;\n"
        "  };\n"
     << "  push(i0, j0, " << AX.id << ");\n"
     << "  for (int rd = 0; rd < wr; ++rd) {\n"
-       "    const int i = tr[rd].i, j = tr[rd].j;\n"
+       "    const bool near = wr - rd <= 1024;  // not yet overwritten in the ring\n"
+       "    const int i = near ? qi_[rd & 1023] : tr[rd].i, j = near ? qj_[rd & 1023] : tr[rd].j;\n"
+       "    const int tid = near ? qt_[rd & 1023] : tr[rd].rule;\n"
        "    const long long x = idx_(z, i, j);\n"
-       "    int r = -1;\n"
-       "    switch (tr[rd].rule) {\n";
+       "    int r = -1, p0 = 0, p1 = 0;\n"
+       "    switch (tid) {\n";
   for (int t : m.order) {
     const Tab& T = m.tabs[t];
     s << "      case " << T.id << ": r = T.r[" << t << "][x];";
-    for (int q = 0; q < T.cat; ++q) s << " tr[rd].pos[" << q << "] = T.p[" << t << "][" << q << "][x];";
+    for (int q = 0; q < T.cat; ++q) s << " p" << q << " = T.p[" << t << "][" << q << "][x];";
     s << " break;\n";
   }
   s << "    }\n"
-       "    tr[rd].rule = r;\n"
-       "    const int p0 = tr[rd].pos[0], p1 = tr[rd].pos[1];\n"
+       "    tr[rd].rule = r; tr[rd].pos[0] = p0; tr[rd].pos[1] = p1;\n"
        "    (void)p0; (void)p1;\n"
        "    switch (r) {\n";
   for (int t : m.order)
