@@ -375,3 +375,24 @@ def test_zuker_count(ctx):
         assert found and ofound and ans == oans and tr == [], len(s)
     got = cg.parse_batch(seqs)
     assert [x[0] for x in got] == [cg.parse(s)[0] for s in seqs]
+
+
+def test_band_pipeline_edges(ctx):
+    """Two-track grammars without split loops run as one band-pipeline launch
+    (64 rows per band, columns published every 16 steps): shapes on every
+    side of a band and a chunk boundary, singly and as one batch of mixed
+    sizes, equal the built-in SW kernel (score and trace)."""
+    from dpx import gen
+    sc = (2, -3, -5, -2)
+    cg = ctx.grammar(G.swat_affine(*sc))
+    pairs = []
+    for n1 in [0, 1, 63, 64, 65, 127, 128, 200]:
+        for n2 in [0, 1, 15, 16, 17, 100]:
+            pairs.append(gen.dna_pair(n1, n2, 1000 + 7 * n1 + n2))
+    want = [ctx.swat(a, b, *sc) for a, b in pairs]
+    for (a, b), (s, _, btr) in zip(pairs, want):
+        ans, found, tr = cg.parse(a, b)
+        assert found and ans == s and [t[:3] for t in tr] == [t[:3] for t in btr], (len(a), len(b))
+    got = cg.parse_batch(pairs)
+    for (a, b), (s, _, btr), (ans, found, tr) in zip(pairs, want, got):
+        assert found and ans == s and [t[:3] for t in tr] == [t[:3] for t in btr], (len(a), len(b))

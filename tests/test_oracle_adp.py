@@ -319,3 +319,16 @@ def test_sum_count_grammars_compile():
     for g in gs:
         rc, src, msg = _check(g.spec_json())
         assert rc == 0, msg[:2000]
+
+
+def test_fill_strategy_of_generated_code():
+    """Two-track grammars without split loops get the band-pipeline kernel,
+    in its register form when every reference is (i, j - c) or (i - 1, j - c);
+    grammars with split loops keep one launch per diagonal."""
+    import dpx
+    for g, band, regs in [(G.swat_affine(), True, True), (G.seqalign("sw"), False, False),
+                          (G.nussinov(), False, False), (G.matrix_chain(), False, False)]:
+        rc, src, msg = _check(g.spec_json())
+        assert rc == 0, msg
+        assert ("void adp_band(" in src) == band and ("struct Regs" in src) == regs, g
+        assert "adp_fill_1" in src and "adp_finish" in src
