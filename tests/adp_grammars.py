@@ -71,3 +71,29 @@ def split_counts(n):
         valid.append(c > 0)
         count.append(c)
     return valid[:n + 1], count[:n + 1]
+
+
+def edit_distance():
+    """E -> empty | el1 E | E el2 | el1 E el2 (two tracks, h = min, with the
+    reference's empty value): the Levenshtein distance of the two inputs."""
+    import dpx.adp as A
+    g = A.Grammar(tracks=2, h="min", empty_value="1000000")
+    zero = A.cfun(lambda: 0, "", "return 0;")
+    dl = A.cfun(lambda c, a: a + 1, "c,a", "return a+1;")
+    ins = A.cfun(lambda a, c: a + 1, "a,c", "return a+1;")
+    sub = A.cfun(lambda c1, a, c2: a + (0 if c1 == c2 else 1), "c1,a,c2", "return a+(c1==c2?0:1);")
+    E = g.tabulate("E", lambda: ((g.empty ^ zero) | (g.el1 << E ^ dl) | (E >> g.el2 ^ ins) |
+                                 (g.el1 << E >> g.el2 ^ sub)).aggregate("min"))
+    g.axiom = E
+    return g
+
+
+def levenshtein(a, b):
+    """The textbook edit-distance recurrence, row by row."""
+    prev = list(range(len(b) + 1))
+    for x in range(1, len(a) + 1):
+        cur = [x] + [0] * len(b)
+        for y in range(1, len(b) + 1):
+            cur[y] = min(prev[y] + 1, cur[y - 1] + 1, prev[y - 1] + (a[x - 1] != b[y - 1]))
+        prev = cur
+    return prev[len(b)]

@@ -396,3 +396,25 @@ def test_band_pipeline_edges(ctx):
     got = cg.parse_batch(pairs)
     for (a, b), (s, _, btr), (ans, found, tr) in zip(pairs, want, got):
         assert found and ans == s and [t[:3] for t in tr] == [t[:3] for t in btr], (len(a), len(b))
+
+
+def test_band_pipeline_empty_value(ctx):
+    """A two-track grammar with the reference's empty value and no
+    always-valid table (validity read as `!= empty` from the band's
+    registers): edit distance against the oracle (score and trace) and
+    against the textbook recurrence at band and chunk edges, single and
+    batched."""
+    import adp_random as R
+    g = G.edit_distance()
+    cg = ctx.grammar(g)
+    assert "struct Regs" in cg.source
+    rng = random.Random(77)
+    for n1, n2 in [(0, 0), (0, 5), (7, 0), (12, 9), (30, 25)]:
+        _vs_oracle(cg, g, R.text(rng, n1, "ab"), R.text(rng, n2, "ab"))
+    pairs = [(R.text(rng, n1, "acgt"), R.text(rng, n2, "acgt"))
+             for n1, n2 in [(63, 200), (64, 17), (65, 100), (200, 150), (300, 64), (129, 16)]]
+    for a, b in pairs:
+        ans, found, _ = cg.parse(a, b)
+        assert found and ans == G.levenshtein(a, b), (len(a), len(b))
+    got = cg.parse_batch(pairs)
+    assert [x[0] for x in got] == [G.levenshtein(a, b) for a, b in pairs]

@@ -332,3 +332,16 @@ def test_fill_strategy_of_generated_code():
         assert rc == 0, msg
         assert ("void adp_band(" in src) == band and ("struct Regs" in src) == regs, g
         assert "adp_fill_1" in src and "adp_finish" in src
+
+
+def test_edit_distance_grammar_oracle_vs_textbook():
+    """The two-track edit-distance grammar with an empty value: the oracle's
+    answer is the Levenshtein distance, and its trace replays to it."""
+    rng = random.Random(5)
+    g = G.edit_distance()
+    for _ in range(40):
+        a = "".join(rng.choice("ab") for _ in range(rng.randint(0, 9)))
+        b = "".join(rng.choice("ab") for _ in range(rng.randint(0, 9)))
+        orc, (ans, found, tr) = G.run_oracle(g, a, b)
+        assert found and ans == G.levenshtein(a, b), (a, b)
+        assert orc.replay(g.py_input(a), g.py_input(b), tr) == ans
