@@ -356,8 +356,8 @@ def main():
     ap.add_argument("--serve-check-every", type=int, default=25)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(instances shard across GPUs without communication, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU; --gpus %d is not supported "
+              "(a batch's instances would shard across GPUs without communication)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import dpx

@@ -1,6 +1,6 @@
 """CPU oracle for generic ADP grammars (test infrastructure only).
 
-Used by tests/ as the checker of dpx_grammar_parse; nothing in the product
+Used by tests/ as the parity reference of dpx_grammar_parse; nothing in the product
 imports it.  It evaluates the same JSON grammar tree that the library
 compiles, with the Python callables of the algebra functions, in plain
 Python loops (small inputs only):

@@ -5,7 +5,7 @@
  * own CPU algorithms (the bench2 generated C of Nussinov/Zuker, the librna
  * energy model, and the generated SW/Gotoh and matrix-chain cell bodies).
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load it, and only as the checker or the timed CPU baseline.  The product
+ * load it, and only as the parity reference or the timed CPU baseline.  The product
  * (lamp-dp-mt_amd/) never links or calls it.
  *
  * Every routine cites the reference file:line it restates.  Parity status of

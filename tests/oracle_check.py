@@ -1,4 +1,4 @@
-"""Oracle-side helpers for bench.py and smoke(): the checker and the timed
+"""Oracle-side helpers for bench.py and smoke(): the parity reference and the timed
 CPU baseline.  Test infrastructure (uses oracle/build/liboracle.so)."""
 import os
 import time

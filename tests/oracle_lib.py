@@ -1,7 +1,7 @@
 """ctypes view of the CPU oracle (oracle/build/liboracle.so).
 
 Test infrastructure only: used by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker or the timed CPU baseline.
+cpu_baseline leg of bench.py as the parity reference or the timed CPU baseline.
 """
 import ctypes
 import os
