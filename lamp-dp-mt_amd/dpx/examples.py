@@ -1,6 +1,6 @@
 """The reference's example grammars (src/main/scala/v4/examples) written with
 dpx.adp, each with the C bodies of its CodeGen algebra and Python callables
-of the same meaning (the callables serve CPU checkers only; the device runs
+of the same meaning (the callables serve the CPU parity references only; the device runs
 the C bodies).  Usage:
 
     import dpx, dpx.examples as ex

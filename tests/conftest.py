@@ -4,7 +4,7 @@ import sys
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "lamp-dp-mt_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
-sys.path.insert(0, os.path.join(ROOT, "oracle"))  # adp_oracle (the generic-grammar checker)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))  # adp_oracle (the generic-grammar parity reference)
 
 
 def pytest_configure(config):
