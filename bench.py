@@ -84,28 +84,25 @@ def pct(xs, p):
 
 
 def served_requests(rate, seconds, seed=5):
-    """Seeded Poisson stream: (arrival s, kind, payload)."""
+    """Seeded Poisson stream of the served mix: (arrival s, kind, payload)."""
     from dpx import gen
-    r = gen.SplitMix64(seed)
+    return gen.served_stream(rate, seconds, seed)
 
-    def log_uniform(lo, hi):
-        return int(round(math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * r.uniform())))
 
-    out, t, q = [], 0.0, 0
-    while True:
-        t += -math.log(1.0 - r.uniform()) / rate
-        if t >= seconds:
-            return out
-        u = r.uniform()
-        if u < 0.1:
-            rows, cols = gen.matrix_chain(64 + r.below(449), 500000 + q)
-            out.append((t, "matchain", (rows, cols)))
-        elif u < 0.1 + 0.9 * 0.8:
-            out.append((t, "zuker", gen.rna_gc(log_uniform(30, 3000), 600000 + q, 0.5)))
-        else:
-            a, b = gen.dna_pair(log_uniform(500, 20000), log_uniform(500, 20000), 700000 + q)
-            out.append((t, "swat", (a, b)))
-        q += 1
+def golden_sample(seqs, results, traces):
+    """Compares the sampled instances of the 4096-sequence batch with the
+    committed answers of the reference's bench2/zu.c (score and trace; the
+    second position of a rule with one split position is not compared).
+    Returns (all equal, instances compared); (None, 0) for another batch."""
+    path = os.path.join(ROOT, "tests", "golden", "zuker_batch4096_sample.json")
+    with open(path) as f:
+        fx = json.load(f)
+    if len(seqs) != 4096 or any(seqs[e["index"]] != e["seq"] for e in fx):
+        return None, 0
+    tl = [2] * 6 + [1] * 9
+    norm = lambda tr: [[a, b, r, p0, p1 if tl[r] > 1 else 0] for a, b, r, p0, p1 in tr]
+    ok = all(results[e["index"]] == e["score"] and norm(traces[e["index"]]) == e["trace"] for e in fx)
+    return ok, len(fx)
 
 
 SW_SCORES = (2, -3, -5, -2)
@@ -429,9 +426,10 @@ def main():
         plain = p_cells / (p_ms * 1e-3) / 1e9
         print("bench: plain per-instance %.3f GCUPS over %d sequences" % (plain, len(sample)), file=sys.stderr, flush=True)
 
-    # bit-exact spot check against the oracle on the shortest sequences of the batch
-    import oracle_check
-    exact = oracle_check.zuker_sample(seqs, results, traces, k=8)
+    # bit-exact check of a stratified sample (every length bucket and the 8
+    # longest sequences) against the reference's own generated CPU code's
+    # answers for the same sequences, committed as fixtures
+    exact, n_exact = golden_sample(seqs, results, traces)
 
     # algorithmic work of the reference's formulation: 2 int32 ops (add, min)
     # per ml/ml1 split candidate (rules 7, 9, 11); the loop-energy candidates
@@ -454,6 +452,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        import oracle_check  # the timed CPU baseline (the oracle)
         cpu = oracle_check.cpu_baseline(seqs, args.cpu_seconds)
 
     line = {
@@ -470,6 +469,8 @@ def main():
         "finalize_ms": round(sum(g_fin) / len(g_fin), 3),
         "eager_fill_ms": round(sum(fills) / len(fills), 3), "eager_finalize_ms": round(sum(fins) / len(fins), 3),
         "tuning": tuning, "bitexact_sample": exact,
+        "bitexact_sample_desc": "%d sequences (every length bucket, the 8 longest) vs the reference's bench2/zu.c "
+                                "(tests/golden/zuker_batch4096_sample.json), MFE and full trace" % n_exact,
         "graph_replay_equals_eager": graph_same,
         "roofline": roofline, "cpu_baseline": cpu, "served": served,
     }

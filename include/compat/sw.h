@@ -1,0 +1,12 @@
+/* sw.h -- declarations of the reference's generated per-grammar ABI
+ * (bench/dp_swat/comp1.h) over libdpx.so: input_t, trace_t, trace_len[] and
+ * my_init/my_solve/my_backtrack/my_free.  A harness includes it in place of
+ * the generated header; the definitions are lamp-dp-mt_amd/compat/sw.c,
+ * built with the same -DSIZE (see dpx_compat.h). */
+#ifndef DPX_COMPAT_SWAT_H
+#define DPX_COMPAT_SWAT_H
+#define DPX_COMPAT_SWAT
+#define DPX_COMPAT_DECLS_ONLY
+#include "../dpx_compat.h"
+#undef DPX_COMPAT_DECLS_ONLY
+#endif

@@ -77,6 +77,7 @@ __device__ int wave_first_match(int ncand, int target, Fn&& cand) {
 template <int NW, class RuleOf, class Kids>
 __device__ void bfs_rounds(TraceRec* tr, int cap, TraceRec root, RuleOf&& rule_of, Kids&& kids, int32_t* len) {
   __shared__ int s_rd, s_wr, s_over;
+  __shared__ int s_fail[NW];  // per wave: no candidate matched its record this round
   __shared__ TraceRec s_rec[NW];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (threadIdx.x == 0) {
@@ -100,7 +101,9 @@ __device__ void bfs_rounds(TraceRec* tr, int cap, TraceRec root, RuleOf&& rule_o
       int rule = -1, p0 = 0, p1 = 0;
       const bool ok = rule_of(cur, &rule, &p0, &p1);
       if (lane == 0) {
-        if (!ok) s_over = 1;
+        // s_over is read by every wave at the top of the round, so a
+        // failure is recorded per wave and folded in below, between barriers
+        s_fail[w] = ok ? 0 : 1;
         cur.rule = rule;
         cur.pos[0] = p0;
         cur.pos[1] = p1;
@@ -110,6 +113,8 @@ __device__ void bfs_rounds(TraceRec* tr, int cap, TraceRec root, RuleOf&& rule_o
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+      for (int q = 0; q < m; ++q)
+        if (s_fail[q]) s_over = 1;
       int wr2 = wr;
       auto push = [&](int pi, int pj, int pr) {
         if (wr2 >= cap) {

@@ -193,3 +193,31 @@ def bench2_seq(size, index, rnd=None):
         for _ in range(index * size):
             rnd.random()
     return "".join("nacgu_"[rnd.random() % 4 + 1] for _ in range(size))
+
+
+def served_stream(rate, seconds, seed=5, chain_share=0.1, zuker_share=0.8):
+    """Seeded Poisson stream of the served mix: a list of (arrival s, kind,
+    payload).  A matrix-chain request (64-512 matrices, dimensions uniform in
+    [1,1000]) with probability chain_share; otherwise a Zuker fold (length
+    log-uniform 30-3000, 50% GC) with probability zuker_share, else a Gotoh
+    SW pair (each side log-uniform 500-20000 bp, ACGT)."""
+    r = SplitMix64(seed)
+
+    def log_uniform(lo, hi):
+        return int(round(math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * r.uniform())))
+
+    out, t, q = [], 0.0, 0
+    while True:
+        t += -math.log(1.0 - r.uniform()) / rate
+        if t >= seconds:
+            return out
+        u = r.uniform()
+        if u < chain_share:
+            rows, cols = matrix_chain(64 + r.below(449), 500000 + q)
+            out.append((t, "matchain", (rows, cols)))
+        elif u < chain_share + (1 - chain_share) * zuker_share:
+            out.append((t, "zuker", rna_gc(log_uniform(30, 3000), 600000 + q, 0.5)))
+        else:
+            a, b = dna_pair(log_uniform(500, 20000), log_uniform(500, 20000), 700000 + q)
+            out.append((t, "swat", (a, b)))
+        q += 1

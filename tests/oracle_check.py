@@ -1,26 +1,9 @@
-"""Oracle-side helpers for bench.py and smoke(): the parity reference and the timed
-CPU baseline.  Test infrastructure (uses oracle/build/liboracle.so)."""
+"""The timed CPU baseline of bench.py: the oracle on the batch's sequences.
+Test infrastructure (uses oracle/build/liboracle.so)."""
 import os
 import time
 
 import oracle_lib as o
-
-ZU_TL = [2] * 6 + [1] * 9
-
-
-def _zn(tr):
-    return [(a, b, r, p0, (p1 if 0 <= r < 15 and ZU_TL[r] > 1 else 0)) for a, b, r, p0, p1 in tr]
-
-
-def zuker_sample(seqs, results, traces, k=8):
-    """Bit-exact check of k instances of a batch (the k shortest) against the oracle."""
-    idx = sorted(range(len(seqs)), key=lambda q: len(seqs[q]))[:k]
-    for q in idx:
-        sc, tr = o.zuker(seqs[q])
-        if sc != results[q] or (traces is not None and _zn(tr) != _zn(traces[q])):
-            return False
-    return True
-
 
 def cpu_baseline(seqs, budget_s, threads=None):
     """The oracle (the reference loop order, bench2/zu.c:39-47) on the batch's

@@ -55,6 +55,89 @@ def test_swat_local_score_brute_force(scores):
         assert axiom == _best_alignment(a, b, *scores, must_end=True), (a, b)
 
 
+@pytest.fixture(scope="module")
+def sw_check(tmp_path_factory):
+    """tests/sw_check.c built with gcc: explicit enumeration of alignments
+    ("brute") and the textbook three-matrix Gotoh ("gotoh")."""
+    import os
+    import subprocess
+    exe = str(tmp_path_factory.mktemp("swc") / "sw_check")
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "sw_check.c")
+    subprocess.check_call(["gcc", "-O2", "-std=c99", "-o", exe, src])
+
+    def run(mode, cases):
+        lines = "".join("%s %s %s %d %d %d %d\n" % (mode, a or "-", b or "-", *sc) for a, b, sc in cases)
+        out = subprocess.run([exe], input=lines.encode(), capture_output=True, check=True).stdout.decode().split("\n")
+        return [tuple(int(x) for x in line.split()) for line in out if line.strip()]
+
+    return run
+
+
+def _rand_scores(rnd):
+    """Random scores under the affine-gap condition open <= ext <= 0: with a
+    cheaper open than extension both the reference's recurrence and the
+    textbook one would close and reopen a gap run, which the cost model of
+    whole gap runs does not describe."""
+    go = rnd.randint(-12, 0)
+    return (rnd.randint(1, 10), rnd.randint(-10, 0), go, rnd.randint(go, 0))
+
+
+def test_swat_brute_force_up_to_8(sw_check):
+    """Every alignment of every substring pair enumerated (n <= 8): the
+    oracle's max M is the best local score and M(n1,n2) the best local
+    alignment ending at (n1,n2), for the two reference score sets and random ones."""
+    rnd = random.Random(11)
+    cases = []
+    for k in range(60):
+        n1, n2 = (8, 8) if k < 8 else (rnd.randint(0, 8), rnd.randint(0, 8))
+        a = "".join(rnd.choice("ACGT") for _ in range(n1))
+        b = "".join(rnd.choice("ACGT") for _ in range(n2))
+        sc = [(10, -3, -3, -1), (2, -3, -5, -2)][k % 2] if k < 20 else _rand_scores(rnd)
+        cases.append((a, b, sc))
+    for (a, b, sc), (best, end) in zip(cases, sw_check("brute", cases)):
+        axiom, (obest, _, _), _ = o.swat(a, b, *sc, trace=False)
+        assert (obest, axiom) == (best, end), (a, b, sc)
+
+
+def test_swat_equals_textbook_gotoh(sw_check):
+    """max M equals the textbook local Gotoh score, and M(n1,n2) its H(n1,n2),
+    at sizes up to 2000, random scores (SWatAffine.scala:22-35: the clamps of
+    the gap tables at 0 do not change M)."""
+    from dpx import gen
+    rnd = random.Random(12)
+    cases = []
+    for k in range(24):
+        n1, n2 = rnd.randint(0, 2000), rnd.randint(0, 2000)
+        if k % 4 == 0:  # related pair: a long shared segment, high local scores
+            base = gen.dna_uniform(n1 + 400, 90 + k)
+            a, b = base[:n1], gen.dna_uniform(200, 190 + k) + base[n1 // 3: n1] + gen.dna_uniform(100, 290 + k)
+        else:
+            a, b = gen.dna_pair(n1, n2, 80 + k)
+        sc = [(10, -3, -3, -1), (2, -3, -5, -2)][k % 2] if k < 4 else _rand_scores(rnd)
+        cases.append((a, b, sc))
+    for (a, b, sc), (best, end) in zip(cases, sw_check("gotoh", cases)):
+        axiom, (obest, _, _), _ = o.swat(a, b, *sc, trace=False)
+        assert (obest, axiom) == (best, end), (len(a), len(b), sc)
+
+
+def test_swat_checker_catches_planted_mistakes(sw_check):
+    """The two checkers disagree with a wrong score (a sign, a dropped gap
+    extension), so they can tell such a mistake apart."""
+    rnd = random.Random(13)
+    cases = []
+    for _ in range(20):  # b = a with 1-3 bases deleted in the middle: the best alignment has a gap run
+        a = "".join(rnd.choice("ACGT") for _ in range(8))
+        d = rnd.randint(1, 3)
+        b = a[:3] + a[3 + d:]
+        cases.append((a, b, (10, -3, -5, -2)))
+    brute = sw_check("brute", cases)
+    assert brute == sw_check("gotoh", cases)
+    for a, b, _ in cases:
+        assert o.swat(a, b, 10, -3, -5, -2, trace=False)[1][0] == brute[cases.index((a, b, _))][0]
+    assert sw_check("gotoh", [(a, b, (10, -3, -5, -5)) for a, b, _ in cases]) != brute
+    assert sw_check("gotoh", [(a, b, (10, -3, -2, -2)) for a, b, _ in cases]) != brute
+
+
 def _trace_score(a, b, tr, match, mismatch, go, ge):
     """Score of the alignment a trace spells, by the SWatAffine pretty algebra:
     rule 1 pairs, rules 5/6 (8/9) open/extend a gap in track 1 (2), with the
