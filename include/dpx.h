@@ -75,6 +75,17 @@ int dpx_nussinov(dpx_ctx* ctx, const char* seq, int n, int min_loop, int32_t* sc
 int dpx_zuker(dpx_ctx* ctx, const char* seq, int n, int32_t* mfe,
               dpx_trace_t* trace, int trace_cap, int* trace_len);
 
+/* dpx_zuker without any allocation: no reference counterpart.  For callers
+ * that cannot allocate device or pinned host memory (code running beside a
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and traces as dpx_zuker, for n <= 1024: the DP
+ * tables live in a static 16 MiB scratch array per device (held until the
+ * process ends), inputs are staged from the stack.  n > 1024 gives
+ * DPX_EINVAL.  Calls on one device are serialised by the library; the
+ * scratch keeps the last call's contents (every call clears what it needs). */
+int dpx_zuker_noalloc(dpx_ctx* ctx, const char* seq, int n, int32_t* mfe,
+                      dpx_trace_t* trace, int trace_cap, int* trace_len);
+
 /* RNAfold MFE grammar (v4/examples/RNAFold.scala:20-64): tables Q (rules
  * 0 QM, 1 Q(i,j-1), 2 Q(i+1,j), 3 Qp), QM (rule 4 split(pos k)), Qp (rules
  * 5 hairpin, 6 stack, 7 interior loop (pos k, l) with both unpaired

@@ -149,7 +149,34 @@ struct dpx_ctx {
   int64_t sw_code_budget = (int64_t)32 << 30;  // traceback codes of one alignment before checkpointing
 };
 
+// Scratch of dpx_zuker_noalloc: the device block (tile records, queue,
+// inputs, results, trace) of one Zuker instance, n <= kNoAllocMaxZuker.  Each
+// device holds its own copy until the process ends; host code uses the
+// address hipGetSymbolAddress returns on that device (g_scratch_ptr), never
+// the array's name.
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+constexpr int kNoAllocMaxZuker = 1024;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+std::mutex g_scratch_mu;                     // guards g_scratch_ptr / g_scratch_rc
+void* g_scratch_ptr[kMaxDevices] = {};       // address of `scratch` on each device
+hipError_t g_scratch_rc[kMaxDevices] = {};   // result of hipGetSymbolAddress there
+std::mutex g_scratch_call_mu[kMaxDevices];   // one dpx_zuker_noalloc at a time per device
+
+// Called by dpx_create with `device` current: takes the scratch address of
+// the device once.
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_rc[device] = e;
+  if (e == hipSuccess) g_scratch_ptr[device] = p;
+}
 
 int fail(std::string& err, int code, const std::string& msg) {
   err = msg;
@@ -218,12 +245,14 @@ int job_bucket(const Job& j) {
 
 // persistent workgroups of a triangular launch: tuned workgroups per CU
 // (0 = the kernel's occupancy) times the CU count, at most one per tile
-int tri_grid(dpx_ctx* c, const Job& j) {
-  const dpx_tuning t = get_tune(c, kind_of(j.g), job_bucket(j));
-  const int occ = c->occ[j.g];
+int tri_grid_for(dpx_ctx* c, int g, int bucket, int64_t tiles) {
+  const dpx_tuning t = get_tune(c, kind_of(g), bucket);
+  const int occ = c->occ[g];
   const int w = t.wg_per_cu >= 1 && t.wg_per_cu <= occ ? t.wg_per_cu : occ;
-  return (int)std::min<int64_t>(j.tiles, (int64_t)w * c->cu_count);
+  return (int)std::min<int64_t>(tiles, (int64_t)w * c->cu_count);
 }
+
+int tri_grid(dpx_ctx* c, const Job& j) { return tri_grid_for(c, j.g, job_bucket(j), j.tiles); }
 
 // ---------------------------------------------------------------- layouts
 
@@ -963,6 +992,7 @@ int dpx_create(dpx_ctx** out, int device) {
     return DPX_EHIP;
   }
   (void)hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, device);
+  scratch_setup(device);
   for (int g = 0; g < N_GRAMMARS; ++g) c->occ[g] = std::max(1, fill_occupancy(g));
   c->sw_occ = std::max(1, sw_occupancy());
   *out = c;
@@ -1053,6 +1083,92 @@ int dpx_zuker(dpx_ctx* c, const char* seq, int n, int32_t* mfe, dpx_trace_t* tra
 
 int dpx_rnafold(dpx_ctx* c, const char* seq, int n, int32_t* mfe, dpx_trace_t* trace, int trace_cap, int* trace_len) {
   return single_rna(c, G_RNAFOLD, seq, n, 0, mfe, trace, trace_cap, trace_len);
+}
+
+// dpx_zuker on the device's static scratch: no device or pinned-host
+// allocation (inputs staged from the stack, results copied straight into the
+// caller's buffers).  Same kernels and launch shape as dpx_zuker.
+int dpx_zuker_noalloc(dpx_ctx* c, const char* seq, int n, int32_t* mfe, dpx_trace_t* trace, int trace_cap,
+                      int* trace_len) {
+  if (!c) return DPX_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::string& err = c->err;
+  if (!seq || n < 0 || !mfe || (trace && (!trace_len || trace_cap < 0))) return fail(err, DPX_EINVAL, "bad argument");
+  if (n > kNoAllocMaxZuker)
+    return fail(err, DPX_EINVAL, "dpx_zuker_noalloc: n = " + std::to_string(n) + " exceeds its maximum of " +
+                                     std::to_string(kNoAllocMaxZuker) + " (16 MiB of static scratch per device)");
+  if (!c->has_rna) return fail(err, DPX_ENOPARAMS, "dpx_set_rna_params was not called");
+  if (c->device >= kMaxDevices) return fail(err, DPX_EINVAL, "dpx_zuker_noalloc: device index too large");
+  char* dev = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    dev = (char*)g_scratch_ptr[c->device];
+    if (!dev)
+      return fail(err, DPX_EHIP, std::string("dpx_zuker_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                                     hipGetErrorString(g_scratch_rc[c->device]));
+  }
+  std::lock_guard<std::mutex> dev_lk(g_scratch_call_mu[c->device]);
+  (void)hipSetDevice(c->device);
+  // inputs on the stack: codes (followed by a 0, as in a batch), the
+  // instance record and the diagonal tiles that seed the queue
+  uint8_t codes[kNoAllocMaxZuker + 64];
+  int r = encode_rna(err, seq, n, codes);
+  if (r) return r;
+  codes[n] = 0;
+  const int nb = (n + 1 + T - 1) / T;
+  unsigned long long seeds[(kNoAllocMaxZuker + 1 + T - 1) / T];
+  for (int I = 0; I < nb; ++I) seeds[I] = pack_item(0, I, I);
+  Job j;
+  j.kind = JOB_TRI;
+  j.g = G_ZUKER;
+  j.count = 1;
+  j.n_max = n;
+  j.want_trace = trace != nullptr;
+  j.tiles = tiles_of(nb);
+  j.trace_total = trace ? tri_trace_cap(n) : 0;
+  j.dev_bytes = tri_layout(G_ZUKER, j.tiles, 1, (size_t)n + 1, j.trace_total, j.off);
+  if (j.dev_bytes > kScratchBytes) return fail(err, DPX_EINVAL, "dpx_zuker_noalloc: instance exceeds the scratch");
+  TriInst inst{};
+  inst.n = n;
+  inst.nb = nb;
+  inst.seq_off = 0;
+  inst.tile0 = 0;
+  inst.trace_off = 0;
+  inst.trace_cap = (int32_t)j.trace_total;
+  inst.out = 0;
+  TriArgs a = tri_args(c, j, dev);
+  a.n_codes = (int64_t)n + 1;
+  hipStream_t s = c->ws.stream;
+  HIPCHK(hipMemcpyAsync(dev + j.off[TL_INST], &inst, sizeof inst, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dev + j.off[TL_SEEDS], seeds, sizeof(unsigned long long) * nb, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dev + j.off[TL_DATA], codes, (size_t)n + 1, hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(c->ws.ev[0], s));
+  // the fill's reset kernel clears the queue, counters and flags the last
+  // call left in the scratch; every table cell is written before it is read
+  if (launch_fill(G_ZUKER, a, (const unsigned long long*)(dev + j.off[TL_SEEDS]), nb,
+                  tri_grid_for(c, G_ZUKER, tune_bucket(n), j.tiles), s))
+    return fail(err, DPX_EHIP, "fill launch failed");
+  HIPCHK(hipEventRecord(c->ws.ev[1], s));
+  if (launch_finalize(G_ZUKER, a, 1, n, s)) return fail(err, DPX_EHIP, "finalize launch failed");
+  HIPCHK(hipEventRecord(c->ws.ev[2], s));
+  int32_t out[8] = {0};  // [0] MFE, [1] trace length, [4..7] ctrl (queue cursors, time-out flag)
+  HIPCHK(hipMemcpyAsync(&out[0], a.results, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (trace) HIPCHK(hipMemcpyAsync(&out[1], a.trace_lens, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&out[4], a.ctrl, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(c->ws.ev[3], s));
+  HIPCHK(hipEventSynchronize(c->ws.ev[3]));
+  (void)hipEventElapsedTime(&c->t_fill, c->ws.ev[0], c->ws.ev[1]);
+  (void)hipEventElapsedTime(&c->t_fin, c->ws.ev[1], c->ws.ev[2]);
+  if (out[4 + 2]) return fail(err, DPX_EHIP, "tile queue wait timed out (scheduler error)");
+  *mfe = out[0];
+  if (trace) {
+    const int tl = out[1];
+    *trace_len = tl;
+    if (tl < 0) return fail(err, DPX_ETRACE, "backtrace failed");
+    if (tl > trace_cap) return fail(err, DPX_ETRACE, "trace buffer too small");
+    HIPCHK(hipMemcpy(trace, a.traces, sizeof(dpx_trace_t) * (size_t)tl, hipMemcpyDeviceToHost));
+  }
+  return DPX_OK;
 }
 
 int dpx_rnafold_batch(dpx_ctx* c, const char* seqs, const int64_t* offs, int count, int32_t* results,

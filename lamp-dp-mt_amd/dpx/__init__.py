@@ -23,7 +23,7 @@ OK, EINVAL, ENOMEM, EHIP, ENOPARAMS, EPARAMS, ETRACE, EBUSY, EAGAIN = range(9)
 # exported symbols, as declared in include/dpx.h
 EXPORTS = [
     "dpx_create", "dpx_destroy", "dpx_last_error", "dpx_set_rna_params",
-    "dpx_nussinov", "dpx_zuker", "dpx_rnafold", "dpx_swat", "dpx_matchain",
+    "dpx_nussinov", "dpx_zuker", "dpx_zuker_noalloc", "dpx_rnafold", "dpx_swat", "dpx_matchain",
     "dpx_zuker_batch", "dpx_nussinov_batch", "dpx_rnafold_batch", "dpx_last_timing", "dpx_now",
     "dpx_server_create", "dpx_server_destroy", "dpx_server_submit", "dpx_server_poll",
     "dpx_server_trace", "dpx_server_get_stats",
@@ -211,6 +211,16 @@ class Context:
         tr = (Trace * cap)() if trace else None
         score, tl = ctypes.c_int32(), ctypes.c_int()
         self._check(self._L.dpx_zuker(self.h, seq.encode(), n, ctypes.byref(score), tr, cap, ctypes.byref(tl)))
+        return score.value, (_tuples(tr, tl.value) if trace else None)
+
+    def zuker_noalloc(self, seq, trace=True):
+        """dpx_zuker_noalloc: dpx_zuker on the device's static scratch (n <= 1024)."""
+        n = len(seq)
+        cap = 4 * n + 8
+        tr = (Trace * cap)() if trace else None
+        score, tl = ctypes.c_int32(), ctypes.c_int()
+        self._check(self._L.dpx_zuker_noalloc(self.h, seq.encode(), n, ctypes.byref(score), tr, cap,
+                                              ctypes.byref(tl)))
         return score.value, (_tuples(tr, tl.value) if trace else None)
 
     def rnafold(self, seq, trace=True):
