@@ -62,6 +62,18 @@ def ml_candidates(n):
     return 2 * _tet(n - 13) + _tet(n - 7)
 
 
+def load_profile_value(name, key):
+    """A `key: value` line of a committed profile summary (profiles/<name>)."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith(key + ":"):
+                return float(line.split(":")[1]), "profiles/" + name
+    except OSError:
+        pass
+    return None, None
+
+
 def load_traffic(name):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
     (scaled by the copy-kernel calibration), from the committed summary."""
@@ -431,18 +443,24 @@ def main():
     # answers for the same sequences, committed as fixtures
     exact, n_exact = golden_sample(seqs, results, traces)
 
-    # algorithmic work of the reference's formulation: 2 int32 ops (add, min)
-    # per ml/ml1 split candidate (rules 7, 9, 11); the loop-energy candidates
-    # of cl are not counted, so this is a lower bound of the work per cell
-    ops = 2.0 * sum(ml_candidates(len(s)) for s in seqs)
+    # algorithmic work of the reference's formulation: 3 int32 ops per
+    # ml/ml1 split candidate (rules 7, 9, 11: add, compare, select of the
+    # value and its back-pointer, as the reference's generated loop does);
+    # the loop-energy candidates of cl are not counted, so this is a lower
+    # bound of the work per cell
+    cand = sum(ml_candidates(len(s)) for s in seqs)
+    ops = 3.0 * cand
     achieved = ops / (fill_ms * 1e-3) / 1e12
     traffic, tsrc = load_traffic("tri_fill_zuker_batch4096_hbm.txt")
+    valu_util, vsrc = load_profile_value("tri_fill_zuker_batch4096_counters.txt", "valu_issue_util")
     roofline = {"bound": "valu", "achieved": round(achieved, 4), "peak": round(PEAK_INT32_TOPS, 2), "unit": "TOP/s",
                 "frac": round(achieved / PEAK_INT32_TOPS, 5), "traffic": traffic, "traffic_source": tsrc,
                 "kernel": "tri_fill<GZu>",
-                "ops": "2 int32 ops (add, min) per split candidate of the reference's ml/ml1 loops (rules 7, 9, 11): "
-                       "%.4e ops per launch" % ops,
-                "executed_ops_frac": round(2.0 * total_x / (fill_ms * 1e-3) / 1e12 / PEAK_INT32_TOPS, 5)}
+                "ops": "3 int32 ops (add, compare, select) per split candidate of the reference's ml/ml1 loops "
+                       "(rules 7, 9, 11): %.4e candidates, %.4e ops per launch" % (cand, ops),
+                "frac_2op": round(2.0 * cand / (fill_ms * 1e-3) / 1e12 / PEAK_INT32_TOPS, 5),
+                "executed_ops_frac": round(2.0 * total_x / (fill_ms * 1e-3) / 1e12 / PEAK_INT32_TOPS, 5),
+                "valu_issue_util_counter": valu_util, "counter_source": vsrc}
 
     served = None
     if not args.no_serve:
