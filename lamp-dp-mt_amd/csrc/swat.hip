@@ -14,9 +14,9 @@
 // M(i,j-1), g2(i,j-1) and the diagonal M(i-1,j-1) in registers.  The band's
 // last row goes to HBM and is published to the next band in chunks through a
 // device-scope progress counter (persistent kernel, bands claimed in order).
-// The traceback keeps one 4-bit code per cell (M rule, g1 ext?, g2 ext?),
-// 8 steps of a lane per 32-bit word, words of a band stored lane-major so each
-// store is a coalesced 256 B row.
+// The traceback keeps one 4-bit code per cell (which of g1 / g2 wins M, and
+// whether g1 / g2 extend), 8 steps of a lane per 32-bit word, words of a band
+// stored lane-major so each store is a coalesced 256 B row.
 //
 // Boundary rows travel through a ring of nslots rows (more than the bands in
 // flight), each with a progress counter tagged by its writer band, so the
@@ -29,6 +29,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "engine.h"
 
 namespace dpx {
@@ -37,21 +39,41 @@ typedef struct { int32_t i, j, rule, pos[2]; } SwTrace;
 
 
 
-__device__ __forceinline__ int sw_shr1(int lane0_val, int v) {
-  // lanes 1..63 receive lane-1's v; lane 0 keeps lane0_val (DPP wave_shr:1)
-  return __builtin_amdgcn_update_dpp(lane0_val, v, 0x138, 0xf, 0xf, false);
+// DPP controls (wave-wide and 16-lane-row shifts)
+constexpr int DPP_WAVE_SHL1 = 0x130;  // lane l <- lane l+1 (lane 63 keeps its old value)
+constexpr int DPP_WAVE_SHR1 = 0x138;  // lane l <- lane l-1 (lane 0 keeps its old value)
+constexpr int DPP_ROW_SHL0 = 0x100;   // + n: lane l <- lane l+n within its row of 16
+
+// v shifted up one lane; lane 0 takes lane 0 of `lane0`
+__device__ __forceinline__ int sw_shr1(int lane0, int v) {
+  return __builtin_amdgcn_update_dpp(lane0, v, DPP_WAVE_SHR1, 0xf, 0xf, false);
+}
+// lane 0 of the result = lane N of v (N < 16)
+template <int N>
+__device__ __forceinline__ int sw_lane0_from(int v) {
+  if constexpr (N == 0) return v;
+  else return __builtin_amdgcn_mov_dpp(v, DPP_ROW_SHL0 + N, 0xf, 0xf, true);  // other lanes: don't care
 }
 
-// SW_CH = steps per chunk = publish granularity (columns), a multiple of 8 up
-// to 64: a band lags the one above by 64 + SW_CH steps, and waits for it (or
-// publishes to the band below) once per chunk.  Variants 16, 32, 64; the
-// tuning table picks one per length bucket.  CODES: store the traceback codes
+// One band of 64 rows per wave (lane r: row i = 64 band + r + 1), swept in
+// steps: at step t lane r computes column j = t - r, taking M(i-1,j) and
+// g1(i-1,j) from lane r-1 through a DPP wave shift (lane 0: the row above,
+// fed from the chunk registers) and keeping M(i,j-1), g2(i,j-1) and
+// M(i-1,j-1) in registers.
+//
+// SW_CH = steps per chunk (16, 32 or 64) = publish granularity: the row
+// above is read, and the band's last row written, once per chunk; a band
+// lags the one above by 64 + SW_CH steps.  CODES: store the traceback codes
 // (A.bt); a separate instance without them keeps the step loop free of a
-// store branch, which the compiler would otherwise hoist by duplicating the
-// loop at twice the registers.
+// store branch.  Per cell the traceback code is 4 bits, shifted into a
+// 32-bit word per lane every step (8 steps per word, the first step in the
+// top nibble): bit 3 = g1 > pair (M rule 2 unless bit 2), bit 2 = g2 > both
+// (M rule 3), bit 1 = g1 extends (rule 6, else 5), bit 0 = g2 extends
+// (rule 9, else 8).
 template <int SW_CH, bool CODES>
 __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
-  static_assert(SW_CH % 8 == 0 && SW_CH <= 64, "chunk: whole code words, one column per lane");
+  static_assert(SW_CH % 16 == 0 && SW_CH <= 64, "chunk: whole 16-lane rows of feed, one column per lane");
+  constexpr int NQ = SW_CH / 16;
   const int lane = threadIdx.x;
   for (;;) {
     // one wave per workgroup: the band claim and every wait are wave-uniform
@@ -63,6 +85,7 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     const bool row_ok = i <= A.n1;
     const int ai = row_ok ? A.a[i - 1] : -1;
     const int W = A.n2 + 1;
+    const int match = A.match, mismatch = A.mismatch, open = A.open, ext = A.ext;
     const int2* above = A.hrow + (size_t)(band % A.nslots) * W;
     int2* below = A.hrow + (size_t)((band + 1) % A.nslots) * W;
     const unsigned long long* pa = A.prog + band % A.nslots;
@@ -74,17 +97,16 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     // 0 and never wraps): then column 0 needs no case of its own -- g2 = 0,
     // M = g1 (rule 2) -- and the trace reads rule 7 for g2 there by itself
     constexpr int NEG = -0x20000000;
-    int Mc = NEG, G1c = 0, G2c = NEG, upPrev = NEG;
-    int best = row_ok ? -1 : 0x7fffffff, bestj = 0;  // rows past n1 never improve
+    int Mc = NEG, G1c = 0, G2c = NEG, upPrev = NEG, bj = 0;
+    int best = row_ok ? -1 : 0x7fffffff, bestt = 0;  // rows past n1 never improve
     const int jstar = row_ok && i == A.n1 ? A.n2 : -0x40000000;  // column of M(n1, n2) on this lane
+    int outv = 0;   // M(n1, n2) when this lane's row is n1
+    int capM = 0, capG = 0;  // the band's last row, collected by lane shifts during a chunk
     int known = 0;  // columns of `above` known to be published
-    int chM = 0, chG = 0, chB = 0;  // chunk of the row above and of b, one column per lane
-    int bj = 0;  // b[j-1] of this lane's column j = t - lane, shifted in from lane 0
-    int outv = 0;  // M(n1, n2) when this lane's row is n1
     const int tend = (A.n2 + 64 + 7) & ~7;  // steps of the band, rounded up to whole code words
     for (int t0 = 0; t0 < tend; t0 += SW_CH) {
       // make the next chunk of the row above available, then load it; the
-      // steps of the chunk below issue no loads, so no step waits on memory
+      // steps of the chunk issue no loads, so no step waits on memory
       const int need = min(t0 + SW_CH, W);
       if (known < need) {
         // columns published by band - 1 (an older tag is an earlier use of the slot)
@@ -109,64 +131,104 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
       // scope) accesses, so neither side needs a cache writeback/invalidate:
       // the producer waits for its row stores before it raises the counter.
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      // lane l: column t0 + l of the row above, and b[t0 - 1 + l] (the
+      // symbol lane 0 needs at step t0 + l)
       const int col = t0 + lane;
+      int chM = 0, chG = 0;
       if (col < W) {
         const unsigned long long v = __hip_atomic_load((const unsigned long long*)(above + col), __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_AGENT);
         chM = (int)(uint32_t)v;
         chG = (int)(uint32_t)(v >> 32);
       }
-      // lane l holds b[t0 - 1 + l]: the symbol lane 0 needs at step t0 + l
       const int bc = t0 - 1 + lane;
-      chB = (bc >= 0 && bc < A.n2) ? A.b[bc] : 0;
-      const int tn = min(SW_CH, tend - t0);  // a multiple of 8
-      int sM = 0, sG = 0;  // lane l keeps (M, g1) of the band's last row at step t0 + l
-      for (int tc = 0; tc < tn; tc += 8) {
-        uint32_t word = 0;
+      const int chB = (bc >= 0 && bc < A.n2) ? A.b[bc] : 0;
+      // feed registers: lanes 0..15 of fM[q] hold chunk columns 16q..16q+15
+      int fM[NQ], fG[NQ], fB[NQ];
+      fM[0] = chM;
+      fG[0] = chG;
+      fB[0] = chB;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int t = t0 + tc + u;
-          // chunk values of this step, broadcast from lane tc+u (wave-uniform index)
-          const int hM = __builtin_amdgcn_readlane(chM, tc + u), hG = __builtin_amdgcn_readlane(chG, tc + u);
-          bj = sw_shr1(__builtin_amdgcn_readlane(chB, tc + u), bj);
-          const int upM = sw_shr1(hM, Mc);
-          const int upG = sw_shr1(hG, G1c);
-          const int j = t - lane;
-          // g2 (j == 0: both candidates are 0, rule 7 is the trace's)
-          const int o = max(0, Mc + A.open), e = max(0, G2c + A.ext);
-          const int r2 = e > o;
-          const int g2 = r2 ? e : o;
-          // g1
-          const int o1 = max(0, upM + A.open), e1 = max(0, upG + A.ext);
-          const int r1 = e1 > o1;
-          const int g1 = r1 ? e1 : o1;
-          // M (j == 0: the diagonal comes from column -1, so g1 wins, rule 2)
-          const int dg = upPrev + (ai == bj ? A.match : A.mismatch);
-          int M = dg, rm = 1;
-          if (g1 > M) { M = g1; rm = 2; }
-          if (g2 > M) { M = g2; rm = 3; }
-          upPrev = upM;
-          // Lanes left of the matrix (j < 0) hold column -1: -infinity for M
-          // and g2; lanes past n2 and rows past n1 are never read by a valid cell.
-          Mc = j < 0 ? NEG : M;
-          G1c = g1;
-          G2c = j < 0 ? NEG : g2;
-          if ((unsigned)j <= (unsigned)A.n2 && M > best) { best = M; bestj = j; }
-          if (j == jstar) outv = M;
-          const int lastM = __builtin_amdgcn_readlane(M, 63), lastG = __builtin_amdgcn_readlane(g1, 63);
-          if (lane == tc + u) { sM = lastM; sG = lastG; }
-          word |= (uint32_t)(rm | (r1 << 2) | (r2 << 3)) << (4 * u);
-        }
-        if (CODES) btb[(size_t)((t0 + tc) >> 3) * 64 + lane] = word;
+      for (int q = 1; q < NQ; ++q) {
+        const int src = ((lane & 15) + 16 * q) << 2;
+        fM[q] = __builtin_amdgcn_ds_bpermute(src, chM);
+        fG[q] = __builtin_amdgcn_ds_bpermute(src, chG);
+        fB[q] = __builtin_amdgcn_ds_bpermute(src, chB);
       }
-      // the chunk's cells of the band's last row, one column per lane
+      const int tn = min(SW_CH, tend - t0);  // a multiple of 8
+      // every lane's column of every step of the chunk is inside [0, n2):
+      // no masks for the columns left of the matrix, past its end, or M(n1,n2)
+      const bool interior = t0 >= 64 && t0 + SW_CH <= A.n2;
+      auto run = [&](auto edge) {
+        constexpr bool EDGE = decltype(edge)::value;
+#pragma unroll
+        for (int g = 0; g < SW_CH / 8; ++g) {
+          if (g * 8 >= tn) break;
+          uint32_t word = 0;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int s = g * 8 + u;  // step within the chunk
+            const int t = t0 + s;
+            const int q = s >> 4;
+            int lM, lG, lB;
+            switch (s & 15) {
+#define SW_FEED(N) case N: lM = sw_lane0_from<N>(fM[q]); lG = sw_lane0_from<N>(fG[q]); lB = sw_lane0_from<N>(fB[q]); break;
+              SW_FEED(0) SW_FEED(1) SW_FEED(2) SW_FEED(3) SW_FEED(4) SW_FEED(5) SW_FEED(6) SW_FEED(7)
+              SW_FEED(8) SW_FEED(9) SW_FEED(10) SW_FEED(11) SW_FEED(12) SW_FEED(13) SW_FEED(14)
+              default: lM = sw_lane0_from<15>(fM[q]); lG = sw_lane0_from<15>(fG[q]); lB = sw_lane0_from<15>(fB[q]); break;
+#undef SW_FEED
+            }
+            const int upM = sw_shr1(lM, Mc);
+            const int upG = sw_shr1(lG, G1c);
+            bj = sw_shr1(lB, bj);
+            // g2 = max(0, M(i,j-1)+open, g2(i,j-1)+ext); extension wins only when strictly better
+            const int o2 = max(Mc + open, 0), e2 = G2c + ext;
+            const bool r2 = e2 > o2;
+            int g2 = max(o2, e2);
+            // g1, from the row above
+            const int o1 = max(upM + open, 0), e1 = upG + ext;
+            const bool r1 = e1 > o1;
+            const int g1 = max(o1, e1);
+            // M: pair first, then g1, then g2 when strictly greater
+            const int dg = upPrev + (ai == bj ? match : mismatch);
+            const int m1 = max(dg, g1);
+            const bool c1 = g1 > dg, c2 = g2 > m1;
+            int M = max(m1, g2);
+            upPrev = upM;
+            const int j = t - lane;
+            if (EDGE) {
+              // lanes left of the matrix (j < 0) hold column -1: -infinity
+              // for M and g2; lanes past n2 and rows past n1 are never read
+              if (j < 0) { M = NEG; g2 = NEG; }
+              if ((unsigned)j <= (unsigned)A.n2 && M > best) { best = M; bestt = t; }
+              if (j == jstar) outv = M;
+            } else {
+              if (M > best) { best = M; bestt = t; }
+            }
+            Mc = M;
+            G1c = g1;
+            G2c = g2;
+            word = word + word + (c1 ? 1u : 0u);
+            word = word + word + (c2 ? 1u : 0u);
+            word = word + word + (r1 ? 1u : 0u);
+            word = word + word + (r2 ? 1u : 0u);
+            // collect lane 63's cells: after the chunk's k-th step its value sits in lane 64 - tn + k
+            capM = __builtin_amdgcn_update_dpp(M, capM, DPP_WAVE_SHL1, 0xf, 0xf, false);
+            capG = __builtin_amdgcn_update_dpp(g1, capG, DPP_WAVE_SHL1, 0xf, 0xf, false);
+          }
+          if (CODES) btb[(size_t)((t0 + g * 8) >> 3) * 64 + lane] = word;
+        }
+      };
+      if (interior) run(std::false_type{});
+      else run(std::true_type{});
+      // the chunk's cells of the band's last row: lane l holds step l - 64 + tn, column t0 + l + tn - 127
       {
-        const int jl = t0 + lane - 63;
-        if (lane < tn && jl >= 0 && jl <= A.n2) {
+        const int jl = t0 + lane + tn - 127;
+        if (lane >= 64 - tn && jl >= 0 && jl <= A.n2) {
           __hip_atomic_store((unsigned long long*)(below + jl),
-                             (unsigned long long)(uint32_t)sM | ((unsigned long long)(uint32_t)sG << 32),
+                             (unsigned long long)(uint32_t)capM | ((unsigned long long)(uint32_t)capG << 32),
                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (cpb) cpb[jl] = make_int2(sM, sG);  // read by later launches only
+          if (cpb) cpb[jl] = make_int2(capM, capG);  // read by later launches only
         }
       }
       // publish the finished columns of this band's last row
@@ -184,6 +246,7 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     if (jstar >= 0) A.out[0] = outv;
     // best cell of this band: largest M, then smallest (i, j)
     if (row_ok && best >= 0) {
+      const int bestj = bestt - lane;
       // score in the top 28 bits, then the complement of the 36-bit linear
       // index (the host bounds both): largest score, then smallest (i, j)
       const unsigned long long key = ((unsigned long long)(unsigned)best << SW_KEY_BITS) |
@@ -259,8 +322,9 @@ __global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mru
           if (w0 + k < A.nwords) win[k][lane] = A.bt[((size_t)b * A.nwords + w0 + k) * 64 + lane];
         __syncthreads();
       }
-      const uint32_t nib = (win[w - w0][r] >> (4 * (t & 7))) & 15u;
-      rule = tab <= 3 ? (int)(nib & 3u) : tab <= 6 ? 5 + (int)((nib >> 2) & 1u) : 8 + (int)((nib >> 3) & 1u);
+      // the first step of a word is in its top nibble (see sw_fill)
+      const uint32_t nib = (win[w - w0][r] >> (4 * (7 - (t & 7)))) & 15u;
+      rule = tab <= 3 ? ((nib & 4u) ? 3 : (nib & 8u) ? 2 : 1) : tab <= 6 ? 5 + (int)((nib >> 1) & 1u) : 8 + (int)(nib & 1u);
     }
     if (n >= cap) { ok = false; break; }
     if (lane == 0) tr[n] = SwTrace{i, j, rule, {0, 0}};
