@@ -288,7 +288,10 @@ size_t tri_footprint(int g, int n, bool trace) {
 
 // boundary-row ring of an alignment: more slots than bands can be in flight
 constexpr int kSwMaxGrid = 4096;
-int sw_slots(int nbands) { return std::min(nbands + 1, kSwMaxGrid + 2); }
+int sw_slots(int nbands) {
+  const int s = std::min(nbands + 1, kSwMaxGrid + 2);
+  return s % SW_TAGS == 0 ? s + 1 : s;  // consecutive users of a slot carry different tags
+}
 // bands per checkpoint segment when the codes of every band would not fit the
 // budget: about sqrt(nbands), balancing checkpoint rows against one segment's codes
 int sw_segment(int n1, int n2, int64_t code_budget) {
@@ -304,7 +307,7 @@ size_t sw_layout(int n1, int n2, int cap, int seg, size_t* off) {
   sz[SL_A] = (size_t)n1 + 64;
   sz[SL_B] = (size_t)n2 + 64;
   sz[SL_ROWS] = (slots + 1) * W * 8 + 2 * W + 64;  // boundary-row ring, then row 0 and its rule bytes
-  sz[SL_FLAGS] = 8 * slots + 64;            // ring progress counters, then two ctrl blocks
+  sz[SL_FLAGS] = 64;                         // two ctrl blocks
   sz[SL_BEST] = 128;                         // best, axiom, pick, trace state, recompute scratch
   sz[SL_BT] = cap > 0 ? (seg ? std::min((size_t)seg, nbands) : nbands) * nwords * 64 * 4 : 0;
   sz[SL_CP] = seg ? (nbands / seg + 1) * W * 8 : 0;
@@ -513,14 +516,13 @@ int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j)
   if (!w.h_in.ensure(in_bytes) || !w.h_out.ensure(out_bytes)) return fail(err, DPX_ENOMEM, "pinned host staging");
   hipStream_t s = w.stream;
   char* rows = dev + j.off[SL_ROWS];
-  A.hrow = (int2*)rows;
+  A.hrow = (unsigned long long*)rows;
   int2* d_row0 = (int2*)(rows + (size_t)A.nslots * W * 8);
   uint8_t* d_r0m = (uint8_t*)(d_row0 + W);
   uint8_t* d_r0g2 = d_r0m + W;
   A.a = (const uint8_t*)(dev + j.off[SL_A]);
   A.b = (const uint8_t*)(dev + j.off[SL_B]);
-  A.prog = (unsigned long long*)(dev + j.off[SL_FLAGS]);
-  A.ctrl = (int*)(A.prog + A.nslots);  // [0] claims, [1] time-out; [4], [5] for recomputed segments
+  A.ctrl = (int*)(dev + j.off[SL_FLAGS]);  // [0] claims, [1] time-out; [4], [5] for recomputed segments
   uint32_t* d_codes = want_trace ? (uint32_t*)(dev + j.off[SL_BT]) : nullptr;
   A.bt = j.sw_seg ? nullptr : d_codes;
   A.cp = j.sw_seg ? (int2*)(dev + j.off[SL_CP]) : nullptr;

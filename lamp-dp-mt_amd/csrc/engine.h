@@ -89,8 +89,8 @@ struct SwArgs {
   int match, mismatch, open, ext;
   int nbands, nwords;   // nwords = words per lane per band
   int nslots;           // boundary rows in flight: band b reads slot b % nslots, writes (b+1) % nslots
-  int2* hrow;           // nslots x (n2+1): (M, g1) of rows 64*band
-  unsigned long long* prog;  // nslots: ((writer band + 1) << 32) | columns of the slot's row published
+                        // (never a multiple of SW_TAGS, so a slot's last two users have different tags)
+  unsigned long long* hrow;  // nslots x (n2+1) words: M | g1 << 27 | tag << 54 of rows 64*band
   int* ctrl;            // [0] band claim counter, [1] band-wait time-out flag
   uint32_t* bt;         // nbands x nwords x 64 traceback codes, or null (none kept)
   int2* cp;             // checkpoint rows: row 64*cp_every*s at cp + s*(n2+1), s >= 1; or null
@@ -98,6 +98,10 @@ struct SwArgs {
   unsigned long long* best;  // packed (M << 32) | ~(i*(n2+1)+j)
   int* out;             // [0] = M(n1,n2)
 };
+// A boundary-row word: M and g1 (both in [0, 2^27), which the host ensures)
+// and the tag (b mod SW_TAGS) + 1 of the band b that reads it; 0 = cleared.
+constexpr int SW_VAL_BITS = 27;
+constexpr int SW_TAGS = 1023;
 // trace state: [0] i, [1] j, [2] table (0 M, 4 g1, 7 g2), [3] records written,
 // [4] 1 when the trace has ended, [5] 0 after an overflow
 constexpr int SW_STATE = 6;

@@ -13,14 +13,14 @@
 // M(i-1,j), g1(i-1,j) from lane r-1 through a DPP wave shift and keeps its own
 // M(i,j-1), g2(i,j-1) and the diagonal M(i-1,j-1) in registers.  The band's
 // last row goes to HBM and is published to the next band in chunks through a
-// device-scope progress counter (persistent kernel, bands claimed in order).
+// tag in every word (persistent kernel, bands claimed in order).
 // The traceback keeps one 4-bit code per cell (which of g1 / g2 wins M, and
 // whether g1 / g2 extend), 8 steps of a lane per 32-bit word, words of a band
 // stored lane-major so each store is a coalesced 256 B row.
 //
 // Boundary rows travel through a ring of nslots rows (more than the bands in
-// flight), each with a progress counter tagged by its writer band, so the
-// forward pass needs O(nslots * n2) memory.  Without codes (bt == null) the
+// flight) of 64-bit words (M, g1, and the tag of the band that reads them),
+// so the forward pass needs O(nslots * n2) memory.  Without codes (bt == null) the
 // pass keeps every cp_every-th boundary row as a checkpoint instead, and the
 // traceback recomputes one segment of cp_every bands at a time from its
 // checkpoint (the checkpoint strategy of the reference's design notes,
@@ -29,6 +29,7 @@
 
 #include <stdint.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "engine.h"
@@ -86,63 +87,67 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     const int ai = row_ok ? A.a[i - 1] : -1;
     const int W = A.n2 + 1;
     const int match = A.match, mismatch = A.mismatch, open = A.open, ext = A.ext;
-    const int2* above = A.hrow + (size_t)(band % A.nslots) * W;
-    int2* below = A.hrow + (size_t)((band + 1) % A.nslots) * W;
-    const unsigned long long* pa = A.prog + band % A.nslots;
-    unsigned long long* pb = A.prog + (band + 1) % A.nslots;
-    const unsigned long long tag_above = (unsigned long long)band << 32;  // written by band - 1
+    const unsigned long long* above = A.hrow + (size_t)(band % A.nslots) * W;
+    unsigned long long* below = A.hrow + (size_t)((band + 1) % A.nslots) * W;
+    const unsigned long long want_tag = (unsigned long long)(band % SW_TAGS + 1);       // written by band - 1
+    const unsigned long long put_tag = (unsigned long long)((band + 1) % SW_TAGS + 1);  // read by band + 1
     uint32_t* btb = A.bt + (size_t)band * A.nwords * 64;
     int2* cpb = (A.cp && (band + 1) % A.cp_every == 0) ? A.cp + (size_t)((band + 1) / A.cp_every) * W : nullptr;
     // column -1 acts as -infinity for M and g2 (NEG + score stays far below
     // 0 and never wraps): then column 0 needs no case of its own -- g2 = 0,
     // M = g1 (rule 2) -- and the trace reads rule 7 for g2 there by itself
     constexpr int NEG = -0x20000000;
+    constexpr unsigned VMASK = (1u << SW_VAL_BITS) - 1;
     int Mc = NEG, G1c = 0, G2c = NEG, upPrev = NEG, bj = 0;
     int best = row_ok ? -1 : 0x7fffffff, bestt = 0;  // rows past n1 never improve
     const int jstar = row_ok && i == A.n1 ? A.n2 : -0x40000000;  // column of M(n1, n2) on this lane
     int outv = 0;   // M(n1, n2) when this lane's row is n1
     int capM = 0, capG = 0;  // the band's last row, collected by lane shifts during a chunk
-    int known = 0;  // columns of `above` known to be published
     const int tend = (A.n2 + 64 + 7) & ~7;  // steps of the band, rounded up to whole code words
-    for (int t0 = 0; t0 < tend; t0 += SW_CH) {
-      // make the next chunk of the row above available, then load it; the
-      // steps of the chunk issue no loads, so no step waits on memory
-      const int need = min(t0 + SW_CH, W);
-      if (known < need) {
-        // columns published by band - 1 (an older tag is an earlier use of the slot)
-        auto cols = [&]() {
-          const unsigned long long v = __hip_atomic_load(pa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          const int c = (v >> 32) == (tag_above >> 32) ? (int)(uint32_t)v : 0;
-          return __builtin_amdgcn_readfirstlane(c);
-        };
-        int k = cols();
-        for (int spin = 0; k < need; ++spin) {
-          if (spin > (1 << 24)) {  // bounded wait: flag the error, stop the band
-            if (lane == 0) A.ctrl[1] = 1;
-            k = need;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          k = cols();
-        }
-        known = k;
-      }
-      // The handoff row is written and read with device-coherent (agent
-      // scope) accesses, so neither side needs a cache writeback/invalidate:
-      // the producer waits for its row stores before it raises the counter.
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      // lane l: column t0 + l of the row above, and b[t0 - 1 + l] (the
-      // symbol lane 0 needs at step t0 + l)
+    // The handoff row is written and read with device-coherent (agent
+    // scope) accesses; every word carries the tag of the band that reads
+    // it, so the reader checks the words themselves (no separate progress
+    // counter, and the writer never waits for its stores).  A chunk's words
+    // are loaded one chunk ahead and reloaded only if some tag is stale.
+    auto load_row = [&](int t0) -> unsigned long long {
       const int col = t0 + lane;
-      int chM = 0, chG = 0;
-      if (col < W) {
-        const unsigned long long v = __hip_atomic_load((const unsigned long long*)(above + col), __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-        chM = (int)(uint32_t)v;
-        chG = (int)(uint32_t)(v >> 32);
+      return col < W && lane < SW_CH
+                 ? __hip_atomic_load(above + col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                 : want_tag << 54;
+    };
+    auto load_b = [&](int t0) -> int {
+      const int bc = t0 - 1 + lane;  // lane l: b[t0 - 1 + l], the symbol lane 0 needs at step t0 + l
+      return (bc >= 0 && bc < A.n2) ? A.b[bc] : 0;
+    };
+    unsigned long long nxt = load_row(0);
+    int nxtB = load_b(0);
+    // last-row cells collected in the previous chunk, stored during this one
+    int pendM = 0, pendG = 0, pend_t0 = -1, pend_tn = 0;
+    auto store_pending = [&]() {
+      if (pend_t0 < 0) return;
+      const int jl = pend_t0 + lane + pend_tn - 127;  // lane l holds step l - 64 + tn of that chunk
+      if (lane >= 64 - pend_tn && jl >= 0 && jl <= A.n2) {
+        const unsigned long long w = (unsigned long long)((unsigned)pendM & VMASK) |
+                                     ((unsigned long long)((unsigned)pendG & VMASK) << SW_VAL_BITS) | (put_tag << 54);
+        __hip_atomic_store(below + jl, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cpb) cpb[jl] = make_int2(pendM, pendG);  // read by later launches only
       }
-      const int bc = t0 - 1 + lane;
-      const int chB = (bc >= 0 && bc < A.n2) ? A.b[bc] : 0;
+      pend_t0 = -1;
+    };
+    for (int t0 = 0; t0 < tend; t0 += SW_CH) {
+      unsigned long long cur = nxt;
+      const int chB = nxtB;
+      // the chunk's words must carry this band's tag (lanes past the chunk
+      // or the row were filled with it); reload until they do, bounded
+      for (int spin = 0; __ballot((cur >> 54) != want_tag) != 0; ++spin) {
+        if (spin > (1 << 22)) {  // bounded wait: flag the error, stop waiting
+          if (lane == 0) A.ctrl[1] = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        cur = load_row(t0);
+      }
+      const int chM = (int)((unsigned)cur & VMASK), chG = (int)((unsigned)(cur >> SW_VAL_BITS) & VMASK);
       // feed registers: lanes 0..15 of fM[q] hold chunk columns 16q..16q+15
       int fM[NQ], fG[NQ], fB[NQ];
       fM[0] = chM;
@@ -154,6 +159,11 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
         fM[q] = __builtin_amdgcn_ds_bpermute(src, chM);
         fG[q] = __builtin_amdgcn_ds_bpermute(src, chG);
         fB[q] = __builtin_amdgcn_ds_bpermute(src, chB);
+      }
+      // the next chunk's words, in flight while this chunk computes
+      if (t0 + SW_CH < tend) {
+        nxt = load_row(t0 + SW_CH);
+        nxtB = load_b(t0 + SW_CH);
       }
       const int tn = min(SW_CH, tend - t0);  // a multiple of 8
       // every lane's column of every step of the chunk is inside [0, n2):
@@ -217,32 +227,18 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
             capG = __builtin_amdgcn_update_dpp(g1, capG, DPP_WAVE_SHL1, 0xf, 0xf, false);
           }
           if (CODES) btb[(size_t)((t0 + g * 8) >> 3) * 64 + lane] = word;
+          // the previous chunk's last row, once this chunk's first code word is out
+          if (g == 0) store_pending();
         }
       };
       if (interior) run(std::false_type{});
       else run(std::true_type{});
-      // the chunk's cells of the band's last row: lane l holds step l - 64 + tn, column t0 + l + tn - 127
-      {
-        const int jl = t0 + lane + tn - 127;
-        if (lane >= 64 - tn && jl >= 0 && jl <= A.n2) {
-          __hip_atomic_store((unsigned long long*)(below + jl),
-                             (unsigned long long)(uint32_t)capM | ((unsigned long long)(uint32_t)capG << 32),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (cpb) cpb[jl] = make_int2(capM, capG);  // read by later launches only
-        }
-      }
-      // publish the finished columns of this band's last row
-      const int t = t0 + tn - 1;
-      const int done = min(t - 63 + 1, W);  // columns 0..t-63 of row 64*band+64
-      if (done > 0) {
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this band's row stores are complete
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (lane == 63)
-          __hip_atomic_store(pb, ((unsigned long long)(band + 1) << 32) | (unsigned)done, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-      }
+      pendM = capM;
+      pendG = capG;
+      pend_t0 = t0;
+      pend_tn = tn;
     }
+    store_pending();
     if (jstar >= 0) A.out[0] = outv;
     // best cell of this band: largest M, then smallest (i, j)
     if (row_ok && best >= 0) {
@@ -256,12 +252,21 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
   }
 }
 
+// Slot 0 = row 0 tagged for band 0; every other slot cleared (tag 0), so no
+// word of an earlier launch or an earlier use of the slot passes as fresh.
 __global__ void sw_reset(SwArgs A, const int2* row0) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int W = A.n2 + 1;
-  if (t < W) A.hrow[t] = row0[t];  // slot 0: row 0 (tag 0: "written by band -1")
-  if (t < A.nslots) A.prog[t] = t == 0 ? (unsigned long long)W : 0ull;
-  if (t == 0) { A.ctrl[0] = 0; A.ctrl[1] = 0; *A.best = 0; A.out[0] = 0; }
+  const size_t total = (size_t)A.nslots * W;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    unsigned long long w = 0;
+    if (t < (size_t)W) {
+      const int2 v = row0[t];
+      w = (unsigned long long)((unsigned)v.x & ((1u << SW_VAL_BITS) - 1)) |
+          ((unsigned long long)((unsigned)v.y & ((1u << SW_VAL_BITS) - 1)) << SW_VAL_BITS) | (1ull << 54);
+    }
+    A.hrow[t] = w;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) { A.ctrl[0] = 0; A.ctrl[1] = 0; *A.best = 0; A.out[0] = 0; }
 }
 
 // Traceback: the reference BFS from (i0,j0) (comp1.cu:100-132) is a single
@@ -350,9 +355,8 @@ __global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mru
 
 int sw_launch(const SwArgs& A, const int2* row0, int grid, int chunk, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  const int W = A.n2 + 1;
-  const int nthr = max(W, A.nslots);
-  hipLaunchKernelGGL(sw_reset, dim3((nthr + 255) / 256), dim3(256), 0, s, A, row0);
+  const size_t words = (size_t)A.nslots * (A.n2 + 1);
+  hipLaunchKernelGGL(sw_reset, dim3((unsigned)std::min<size_t>((words + 255) / 256, 4096)), dim3(256), 0, s, A, row0);
   const bool codes = A.bt != nullptr;
   auto k = chunk == 16 ? (codes ? sw_fill<16, true> : sw_fill<16, false>)
          : chunk == 64 ? (codes ? sw_fill<64, true> : sw_fill<64, false>)
