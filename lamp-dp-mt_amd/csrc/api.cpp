@@ -295,18 +295,18 @@ int sw_slots(int nbands) {
 // bands per checkpoint segment when the codes of every band would not fit the
 // budget: about sqrt(nbands), balancing checkpoint rows against one segment's codes
 int sw_segment(int n1, int n2, int64_t code_budget) {
-  const int64_t nbands = (n1 + 63) / 64, nwords = (n2 + 64 + 7) / 8;
+  const int64_t nbands = (n1 + 63) / 64, nwords = sw_code_words(n2);
   if (nbands * nwords * 64 * 4 <= code_budget) return 0;
   return std::max(1, (int)std::ceil(std::sqrt((double)nbands)));
 }
 
 size_t sw_layout(int n1, int n2, int cap, int seg, size_t* off) {
-  const size_t nbands = (n1 + 63) / 64, nwords = (n2 + 64 + 7) / 8, W = (size_t)n2 + 1;
+  const size_t nbands = (n1 + 63) / 64, nwords = sw_code_words(n2), W = (size_t)n2 + 1;
   const size_t slots = sw_slots((int)nbands);
   size_t sz[SL_N];
   sz[SL_A] = (size_t)n1 + 64;
   sz[SL_B] = (size_t)n2 + 64;
-  sz[SL_ROWS] = (slots + 1) * W * 8 + 2 * W + 64;  // boundary-row ring, then row 0 and its rule bytes
+  sz[SL_ROWS] = slots * (W + 1) * 8 + W * 8 + 2 * W + 64;  // boundary-row ring, then row 0 and its rule bytes
   sz[SL_FLAGS] = 64;                         // two ctrl blocks
   sz[SL_BEST] = 128;                         // best, axiom, pick, trace state, recompute scratch
   sz[SL_BT] = cap > 0 ? (seg ? std::min((size_t)seg, nbands) : nbands) * nwords * 64 * 4 : 0;
@@ -508,7 +508,7 @@ int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j)
   A.open = j.sc.gap_open;
   A.ext = j.sc.gap_ext;
   A.nbands = (n1 + 63) / 64;
-  A.nwords = (n2 + 64 + 7) / 8;
+  A.nwords = sw_code_words(n2);
   A.nslots = sw_slots(A.nbands);
   const bool want_trace = j.sw_cap > 0;
   const size_t in_bytes = (size_t)n1 + n2 + sizeof(int2) * W + 2 * (size_t)W + 64;
@@ -517,7 +517,7 @@ int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j)
   hipStream_t s = w.stream;
   char* rows = dev + j.off[SL_ROWS];
   A.hrow = (unsigned long long*)rows;
-  int2* d_row0 = (int2*)(rows + (size_t)A.nslots * W * 8);
+  int2* d_row0 = (int2*)(rows + (size_t)A.nslots * (W + 1) * 8);
   uint8_t* d_r0m = (uint8_t*)(d_row0 + W);
   uint8_t* d_r0g2 = d_r0m + W;
   A.a = (const uint8_t*)(dev + j.off[SL_A]);
@@ -574,7 +574,7 @@ int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j)
       B.n1 = std::min(n1, base + seg_rows) - base;
       B.n2 = jc;
       B.nbands = (B.n1 + 63) / 64;
-      B.nwords = (jc + 64 + 7) / 8;
+      B.nwords = sw_code_words(jc);
       B.nslots = sw_slots(B.nbands);
       B.ctrl = A.ctrl + 4;
       B.bt = d_codes;

@@ -87,10 +87,11 @@ struct SwArgs {
   const uint8_t* b;     // n2 bytes
   int n1, n2;
   int match, mismatch, open, ext;
-  int nbands, nwords;   // nwords = words per lane per band
+  int nbands, nwords;   // nwords = code words per lane per band: sw_code_words(n2)
   int nslots;           // boundary rows in flight: band b reads slot b % nslots, writes (b+1) % nslots
                         // (never a multiple of SW_TAGS, so a slot's last two users have different tags)
-  unsigned long long* hrow;  // nslots x (n2+1) words: M | g1 << 27 | tag << 54 of rows 64*band
+  unsigned long long* hrow;  // nslots x (n2+2) words: M | g1 << 27 | tag << 54 of rows 64*band
+                             // (columns 0..n2, then one word that absorbs stores of idle lanes)
   int* ctrl;            // [0] band claim counter, [1] band-wait time-out flag
   uint32_t* bt;         // nbands x nwords x 64 traceback codes, or null (none kept)
   int2* cp;             // checkpoint rows: row 64*cp_every*s at cp + s*(n2+1), s >= 1; or null
@@ -98,6 +99,10 @@ struct SwArgs {
   unsigned long long* best;  // packed (M << 32) | ~(i*(n2+1)+j)
   int* out;             // [0] = M(n1,n2)
 };
+// Traceback code words per lane of a band: the steps of the band rounded up
+// to whole chunks of the largest chunk (64 steps = 8 words), plus 8 words
+// that absorb the stores of a chunk with nothing to store.
+__host__ __device__ inline int sw_code_words(int n2) { return (((n2 + 64 + 7) / 8 + 7) & ~7) + 8; }
 // A boundary-row word: M and g1 (both in [0, 2^27), which the host ensures)
 // and the tag (b mod SW_TAGS) + 1 of the band b that reads it; 0 = cleared.
 constexpr int SW_VAL_BITS = 27;

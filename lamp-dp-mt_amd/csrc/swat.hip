@@ -87,8 +87,8 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     const int ai = row_ok ? A.a[i - 1] : -1;
     const int W = A.n2 + 1;
     const int match = A.match, mismatch = A.mismatch, open = A.open, ext = A.ext;
-    const unsigned long long* above = A.hrow + (size_t)(band % A.nslots) * W;
-    unsigned long long* below = A.hrow + (size_t)((band + 1) % A.nslots) * W;
+    const unsigned long long* above = A.hrow + (size_t)(band % A.nslots) * (W + 1);
+    unsigned long long* below = A.hrow + (size_t)((band + 1) % A.nslots) * (W + 1);
     const unsigned long long want_tag = (unsigned long long)(band % SW_TAGS + 1);       // written by band - 1
     const unsigned long long put_tag = (unsigned long long)((band + 1) % SW_TAGS + 1);  // read by band + 1
     uint32_t* btb = A.bt + (size_t)band * A.nwords * 64;
@@ -121,31 +121,56 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
     };
     unsigned long long nxt = load_row(0);
     int nxtB = load_b(0);
-    // last-row cells collected in the previous chunk, stored during this one
-    int pendM = 0, pendG = 0, pend_t0 = -1, pend_tn = 0;
-    auto store_pending = [&]() {
-      if (pend_t0 < 0) return;
-      const int jl = pend_t0 + lane + pend_tn - 127;  // lane l holds step l - 64 + tn of that chunk
-      if (lane >= 64 - pend_tn && jl >= 0 && jl <= A.n2) {
-        const unsigned long long w = (unsigned long long)((unsigned)pendM & VMASK) |
-                                     ((unsigned long long)((unsigned)pendG & VMASK) << SW_VAL_BITS) | (put_tag << 54);
-        __hip_atomic_store(below + jl, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cpb) cpb[jl] = make_int2(pendM, pendG);  // read by later launches only
+    // nothing else is in flight before the loop: the loop's wait for its
+    // prefetched words then counts only the loop's own memory operations
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    // The previous chunk's stores -- its traceback code words (kept in
+    // registers) and the band's last-row cells it collected -- go out at the
+    // start of the next chunk, after that chunk's prefetch, and every chunk
+    // issues the same memory instructions (a chunk with nothing to store
+    // writes to the band's spare code words and the row's spare word): the
+    // compiler's wait for the prefetched words at the top of the next chunk
+    // can then leave these stores in flight.
+    uint32_t wbuf[SW_CH / 8];
+    int pendM = 0, pendG = 0, pend_t0 = -SW_CH, pend_tn = SW_CH;
+    const int spare_word = A.nwords - 8;
+    auto flush_prev = [&]() {
+      if (CODES) {
+#pragma unroll
+        for (int g = 0; g < SW_CH / 8; ++g) {
+          const int w = pend_t0 >= 0 && g * 8 < pend_tn ? ((pend_t0 + g * 8) >> 3) : spare_word + g;
+          btb[(size_t)w * 64 + lane] = wbuf[g];
+        }
       }
-      pend_t0 = -1;
+      const int jl = pend_t0 + lane + pend_tn - 127;  // lane l holds step l - 64 + tn of that chunk
+      const bool keep = pend_t0 >= 0 && lane >= 64 - pend_tn && jl >= 0 && jl <= A.n2;
+      const unsigned long long w = (unsigned long long)((unsigned)pendM & VMASK) |
+                                   ((unsigned long long)((unsigned)pendG & VMASK) << SW_VAL_BITS) | (put_tag << 54);
+      __hip_atomic_store(below + (keep ? jl : W), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cpb && keep) cpb[jl] = make_int2(pendM, pendG);  // read by later launches only
     };
     for (int t0 = 0; t0 < tend; t0 += SW_CH) {
       unsigned long long cur = nxt;
       const int chB = nxtB;
       // the chunk's words must carry this band's tag (lanes past the chunk
       // or the row were filled with it); reload until they do, bounded
-      for (int spin = 0; __ballot((cur >> 54) != want_tag) != 0; ++spin) {
-        if (spin > (1 << 22)) {  // bounded wait: flag the error, stop waiting
-          if (lane == 0) A.ctrl[1] = 1;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        cur = load_row(t0);
+      if (__ballot((cur >> 54) != want_tag) != 0) {
+        int spin = 0;
+        do {
+          if (++spin > (1 << 22)) {  // bounded wait: flag the error, stop waiting
+            if (lane == 0) A.ctrl[1] = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          cur = load_row(t0);
+        } while (__ballot((cur >> 54) != want_tag) != 0);
+      }
+      // the next chunk's words, in flight while this chunk computes; issued
+      // before this chunk's stores, so that waiting for them (loads and
+      // stores retire in order) leaves the stores in flight
+      if (t0 + SW_CH < tend) {
+        nxt = load_row(t0 + SW_CH);
+        nxtB = load_b(t0 + SW_CH);
       }
       const int chM = (int)((unsigned)cur & VMASK), chG = (int)((unsigned)(cur >> SW_VAL_BITS) & VMASK);
       // feed registers: lanes 0..15 of fM[q] hold chunk columns 16q..16q+15
@@ -160,11 +185,7 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
         fG[q] = __builtin_amdgcn_ds_bpermute(src, chG);
         fB[q] = __builtin_amdgcn_ds_bpermute(src, chB);
       }
-      // the next chunk's words, in flight while this chunk computes
-      if (t0 + SW_CH < tend) {
-        nxt = load_row(t0 + SW_CH);
-        nxtB = load_b(t0 + SW_CH);
-      }
+      flush_prev();
       const int tn = min(SW_CH, tend - t0);  // a multiple of 8
       // every lane's column of every step of the chunk is inside [0, n2):
       // no masks for the columns left of the matrix, past its end, or M(n1,n2)
@@ -193,17 +214,21 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
             bj = sw_shr1(lB, bj);
             // g2 = max(0, M(i,j-1)+open, g2(i,j-1)+ext); extension wins only when strictly better
             const int o2 = max(Mc + open, 0), e2 = G2c + ext;
-            const bool r2 = e2 > o2;
             int g2 = max(o2, e2);
             // g1, from the row above
             const int o1 = max(upM + open, 0), e1 = upG + ext;
-            const bool r1 = e1 > o1;
             const int g1 = max(o1, e1);
             // M: pair first, then g1, then g2 when strictly greater
             const int dg = upPrev + (ai == bj ? match : mismatch);
             const int m1 = max(dg, g1);
-            const bool c1 = g1 > dg, c2 = g2 > m1;
             int M = max(m1, g2);
+            // the code bits: x > y is the sign of y - x (every value here
+            // lies within +-2^30, so the difference does not wrap), shifted
+            // into the word with v_alignbit -- no lane masks kept alive
+            word = __builtin_amdgcn_alignbit(word, (unsigned)(dg - g1), 31);  // g1 > pair
+            word = __builtin_amdgcn_alignbit(word, (unsigned)(m1 - g2), 31);  // g2 > max(pair, g1)
+            word = __builtin_amdgcn_alignbit(word, (unsigned)(o1 - e1), 31);  // g1 extends
+            word = __builtin_amdgcn_alignbit(word, (unsigned)(o2 - e2), 31);  // g2 extends
             upPrev = upM;
             const int j = t - lane;
             if (EDGE) {
@@ -218,17 +243,11 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
             Mc = M;
             G1c = g1;
             G2c = g2;
-            word = word + word + (c1 ? 1u : 0u);
-            word = word + word + (c2 ? 1u : 0u);
-            word = word + word + (r1 ? 1u : 0u);
-            word = word + word + (r2 ? 1u : 0u);
             // collect lane 63's cells: after the chunk's k-th step its value sits in lane 64 - tn + k
             capM = __builtin_amdgcn_update_dpp(M, capM, DPP_WAVE_SHL1, 0xf, 0xf, false);
             capG = __builtin_amdgcn_update_dpp(g1, capG, DPP_WAVE_SHL1, 0xf, 0xf, false);
           }
-          if (CODES) btb[(size_t)((t0 + g * 8) >> 3) * 64 + lane] = word;
-          // the previous chunk's last row, once this chunk's first code word is out
-          if (g == 0) store_pending();
+          wbuf[g] = word;
         }
       };
       if (interior) run(std::false_type{});
@@ -238,7 +257,7 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
       pend_t0 = t0;
       pend_tn = tn;
     }
-    store_pending();
+    flush_prev();
     if (jstar >= 0) A.out[0] = outv;
     // best cell of this band: largest M, then smallest (i, j)
     if (row_ok && best >= 0) {
@@ -256,7 +275,7 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
 // word of an earlier launch or an earlier use of the slot passes as fresh.
 __global__ void sw_reset(SwArgs A, const int2* row0) {
   const int W = A.n2 + 1;
-  const size_t total = (size_t)A.nslots * W;
+  const size_t total = (size_t)A.nslots * (W + 1);  // each slot: columns 0..n2 and a spare word
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
     unsigned long long w = 0;
     if (t < (size_t)W) {
@@ -355,7 +374,7 @@ __global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mru
 
 int sw_launch(const SwArgs& A, const int2* row0, int grid, int chunk, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  const size_t words = (size_t)A.nslots * (A.n2 + 1);
+  const size_t words = (size_t)A.nslots * (A.n2 + 2);
   hipLaunchKernelGGL(sw_reset, dim3((unsigned)std::min<size_t>((words + 255) / 256, 4096)), dim3(256), 0, s, A, row0);
   const bool codes = A.bt != nullptr;
   auto k = chunk == 16 ? (codes ? sw_fill<16, true> : sw_fill<16, false>)
