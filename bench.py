@@ -150,7 +150,7 @@ def serve(ctx, rate, seconds, budget, slots, max_batch, check_every):
     th = threading.Thread(target=produce)
     t_start = dpx.now()
     th.start()
-    check = set(range(0, len(reqs), check_every))
+    check = set(range(0, len(reqs), check_every)) if check_every > 0 else set()
     got = {}
     while th.is_alive() or len(got) < sum(1 for x in ids if x is not None):
         for res in srv.poll(wait_us=5000):
@@ -201,6 +201,36 @@ def serve(ctx, rate, seconds, budget, slots, max_batch, check_every):
         "bitexact_sample": exact, "checked": checked,
         "mix": "Zuker:SW 4:1 (72%:18%), matrix chain 10% of requests",
     }
+
+
+def serve_sweep(ctx, args, budget):
+    """Offered load swept upwards: the achieved GCUPS levels off at the
+    server's capacity (saturation); then p50/p99 at 80% of it (the arrival
+    rate whose offered cells are 80% of the saturation GCUPS)."""
+    from dpx import gen
+    rates = [float(r) for r in args.serve_sweep.split(",")]
+    pts = []
+    for rate in rates:
+        r = serve(ctx, rate, args.serve_sweep_seconds, budget, args.serve_slots, 256, 0)
+        pts.append({"rate": rate, "gcups": r["gcups"], "completed": r["completed"], "requests": r["requests"],
+                    "span_s": r["span_s"], "latency_ms_all": r["latency_ms_all"], "mean_batch": r["mean_batch"]})
+        print("bench: sweep rate %.0f/s: %.3f GCUPS, p50 %.1f ms, p99 %.1f ms, span %.2f s" % (
+            rate, r["gcups"], r["latency_ms_all"]["p50"], r["latency_ms_all"]["p99"], r["span_s"]),
+            file=sys.stderr, flush=True)
+    sat = max(p["gcups"] for p in pts)
+    # mean cells per request of the stream (same generator, long sample)
+    sample = gen.served_stream(1000.0, 20.0, 9)
+    mean_cells = sum((len(x) + 1) * (len(x) + 2) // 2 if k == "zuker" else
+                     (len(x[0]) + 1) * (len(x[1]) + 1) if k == "swat" else (len(x[0]) + 1) * (len(x[0]) + 2) // 2
+                     for _, k, x in sample) / len(sample)
+    rate80 = 0.8 * sat * 1e9 / mean_cells
+    at80 = serve(ctx, rate80, args.serve_sweep_seconds * 2, budget, args.serve_slots, 256, args.serve_check_every)
+    print("bench: at 80%% of saturation (%.0f/s): %.3f GCUPS, p50 %.1f ms, p99 %.1f ms" % (
+        rate80, at80["gcups"], at80["latency_ms_all"]["p50"], at80["latency_ms_all"]["p99"]), file=sys.stderr, flush=True)
+    return {"points": pts, "saturation_gcups": sat, "mean_cells_per_request": round(mean_cells),
+            "rate_at_80pct": round(rate80, 1),
+            "at_80pct": {k: at80[k] for k in ("gcups", "requests", "completed", "failed", "rejected", "latency_ms",
+                                               "latency_ms_all", "mean_batch", "bitexact_sample", "checked", "span_s")}}
 
 
 def single(args):
@@ -361,7 +391,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
-    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-slots", type=int, default=6)
+    ap.add_argument("--serve-sweep", default="300,1000,2000,4000",
+                    help="arrival rates (requests/s) of the saturation sweep; empty to skip")
+    ap.add_argument("--serve-sweep-seconds", type=float, default=3.0)
     ap.add_argument("--serve-check-every", type=int, default=25)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -464,9 +497,11 @@ def main():
 
     served = None
     if not args.no_serve:
-        served = serve(ctx, args.serve_rate, args.serve_seconds, int(args.serve_budget_gib * (1 << 30)),
-                       args.serve_slots, 256, args.serve_check_every)
+        budget = int(args.serve_budget_gib * (1 << 30))
+        served = serve(ctx, args.serve_rate, args.serve_seconds, budget, args.serve_slots, 256, args.serve_check_every)
         print("bench: served %s" % json.dumps(served), file=sys.stderr, flush=True)
+        if args.serve_sweep:
+            served["sweep"] = serve_sweep(ctx, args, budget)
 
     cpu = None
     if not args.no_cpu:

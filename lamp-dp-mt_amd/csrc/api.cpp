@@ -708,7 +708,13 @@ struct Request {
   size_t footprint = 0;
   int64_t cells = 0;
   int size_class = 0;
+  int passed = 0;  // launches started ahead of it while it waited for memory
 };
+
+// A request that does not fit the arena's free memory may be passed over by
+// later requests that do, at most this many times; then it blocks the queue
+// until memory frees up, so a large request cannot starve.
+constexpr int kMaxPassOver = 16;
 
 struct Finished {
   dpx_result r{};
@@ -892,20 +898,26 @@ void harvest_slot(dpx_server* S, Slot& sl, int status) {
   sl.reqs.clear();
 }
 
-// Take the next batch for a free slot: the oldest waiting request decides
-// kind, options and size class; later requests of the same kind, options and
-// a neighbouring size class join in arrival order while the batch fits the
-// largest free range of the arena.  Returns false when nothing can start.
+// Take the next batch for a free slot: the oldest waiting request that fits
+// the largest free range of the arena decides kind, options and size class
+// (requests ahead of it that do not fit are passed over, each at most
+// kMaxPassOver times); later requests of the same kind, options and a
+// neighbouring size class join in arrival order while the batch fits.
+// Returns false when nothing can start.
 bool take_batch(dpx_server* S, Slot& sl) {
   std::lock_guard<std::mutex> lk(S->mu);
   if (S->pending.empty()) return false;
   const size_t room = S->arena.largest();
-  const Request& h = S->pending.front();
-  if (h.footprint > room) return false;  // wait until launches in flight release memory
+  auto hit = S->pending.begin();
+  for (; hit != S->pending.end() && hit->footprint > room; ++hit)
+    if (hit->passed >= kMaxPassOver) return false;  // it waits for memory; nothing passes it any more
+  if (hit == S->pending.end()) return false;  // wait until launches in flight release memory
+  for (auto it = S->pending.begin(); it != hit; ++it) ++it->passed;
+  const Request& h = *hit;
   const int cap = h.kind == DPX_REQ_SWAT ? 1 : S->cfg.max_batch;
   size_t used = 0;
   const int kind = h.kind, trace = h.trace, ml = h.min_loop, cls = h.size_class;
-  for (auto it = S->pending.begin(); it != S->pending.end() && (int)sl.reqs.size() < cap;) {
+  for (auto it = hit; it != S->pending.end() && (int)sl.reqs.size() < cap;) {
     const bool same = it->kind == kind && it->trace == trace && it->min_loop == ml && std::abs(it->size_class - cls) <= 1;
     if (same && used + it->footprint <= room) {
       used += it->footprint;
