@@ -85,14 +85,20 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // ------------------------------------------------------------ middle product
 // acc[t][u] (cells (IT+r0+t, JT+c0+u)) = min/max over k in blocks I<K<J of
-// L(i,k) (+) R(k,j) (+ extra for the matrix chain).
+// L(i,k) (+) R(k,j) (+ extra for the matrix chain).  Per wave, LDS holds the
+// left operand k-major (row r of column k at [k][r ^ mp_swz(k)]: one 128-bit
+// read gives a lane its 4 rows at one k, and the XOR keeps the transposing
+// writes at 2 lanes per bank), the right operand row-major ([k][c]) and, for
+// the chain, rows[k].
+constexpr int MP_STAGE = T * T + T * T + T;
+__device__ __forceinline__ int mp_swz(int k) { return ((k >> 2) & 7) << 2; }
 template <bool MAXP, bool CHAIN>
 __device__ void middle_product(const TriArgs& a, const TriInst& in, int NTv, int LT, int RT, int I, int J,
                                int* stage, int acc[4][4]) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r0 = (lane >> 3) * 4, c0 = (lane & 7) * 4;
-  int* sL = stage + w * (T * (T + 1) + T * T + T);  // [r][k], row stride T+1
-  int* sR = sL + T * (T + 1);                       // [k][c]
+  int* sL = stage + w * MP_STAGE;  // [k][r ^ mp_swz(k)]
+  int* sR = sL + T * T;            // [k][c]
   int* sK = sR + T * T;                             // chain: rows[k]
   uint32_t alpha[4][4];
   if (CHAIN) {
@@ -122,11 +128,11 @@ __device__ void middle_product(const TriArgs& a, const TriInst& in, int NTv, int
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = (q * 64 + lane) * 4;
-      const int r = e / T, k = e % T;
-      sL[r * (T + 1) + k + 0] = vl[q].x;
-      sL[r * (T + 1) + k + 1] = vl[q].y;
-      sL[r * (T + 1) + k + 2] = vl[q].z;
-      sL[r * (T + 1) + k + 3] = vl[q].w;
+      const int r = e / T, k = e % T, rs = r ^ mp_swz(k);  // k % 4 == 0: one swizzle for k..k+3
+      sL[(k + 0) * T + rs] = vl[q].x;
+      sL[(k + 1) * T + rs] = vl[q].y;
+      sL[(k + 2) * T + rs] = vl[q].z;
+      sL[(k + 3) * T + rs] = vl[q].w;
       *(int4*)(sR + e) = vr[q];
     }
     if (K + NWAVE < J) load(K + NWAVE);
@@ -137,8 +143,8 @@ __device__ void middle_product(const TriArgs& a, const TriInst& in, int NTv, int
     wave_lds_sync();
 #pragma unroll 2
     for (int k = 0; k < T; ++k) {
-      int av[4];
-      for (int t = 0; t < 4; ++t) av[t] = sL[(r0 + t) * (T + 1) + k];
+      const int4 a4 = *(const int4*)(sL + k * T + (r0 ^ mp_swz(k)));  // rows r0..r0+3 in order
+      const int av[4] = {a4.x, a4.y, a4.z, a4.w};
       const int4 b = *(const int4*)(sR + k * T + c0);
       const int bv[4] = {b.x, b.y, b.z, b.w};
       uint32_t rk = CHAIN ? (uint32_t)sK[k] : 0u;
@@ -285,7 +291,7 @@ struct GNu {
   template <class S> __device__ static void init(const TriArgs&, S&) {}
   struct Smem {
     union {
-      int stage[NWAVE * (T * (T + 1) + T * T + T)];
+      int stage[NWAVE * MP_STAGE];
       struct {
         int S[T + 1][T + 2];  // origin (IT, JT-1)
         int L[T][T + 1];      // tile (I,I) [r][k]
@@ -381,7 +387,7 @@ struct GMc {
   template <class S> __device__ static void init(const TriArgs&, S&) {}
   struct Smem {
     union {
-      int stage[NWAVE * (T * (T + 1) + T * T + T)];
+      int stage[NWAVE * MP_STAGE];
       struct {
         int S[T][T + 1];      // current tile
         int L[T][T + 1];      // tile (I,I)
@@ -598,7 +604,7 @@ struct GZu {
   static constexpr int G = 5;
   struct Smem {
     union {
-      int stage[NWAVE * (T * (T + 1) + T * T + T)];
+      int stage[NWAVE * MP_STAGE];
       struct {
         int B[BR][BS];         // cl + inner mismatch, origin (IT, JT-32)
         int U[T][T + 1];       // origin (IT, JT-1)
@@ -945,7 +951,7 @@ struct GRf {
   static constexpr int G = 3;
   struct Smem {
     union {
-      int stage[NWAVE * (T * (T + 1) + T * T + T)];
+      int stage[NWAVE * MP_STAGE];
       struct {
         int B[BR][BS];          // Qp + inner mismatch, origin (IT, JT-32)
         int Q[T + 1][T + 1];    // origin (IT, JT-1); row T = frontier row IT+32
