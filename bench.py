@@ -289,12 +289,27 @@ def single(args):
                  "bitexact": check(r) if (check and not args.no_cpu) else None})
     if args.workload == "swat16384":
         fill_s = sum(fills) / args.steps * 1e-3
-        # traceback store: 4 bits per cell written once; score rows: 8 bytes per cell of every 64th row
+        traffic, tsrc = load_traffic("sw_fill_swat_16384x16384.txt")
+        valu_util, _ = load_profile_value("sw_fill_swat_16384x16384.txt", "valu_issue_util")
+        # the band pipeline's ceilings: its critical path is n1 + n2 + nbands * chunk
+        # steps; a step needs at least the dependent chain of its cell (wave
+        # shift, add, clamp, max with the extension, with the pair, with g2:
+        # about 30 cycles) and, with one wave per band, the issue of all its
+        # instructions (VALU per step from the counters x 4 cycles)
+        chunk = ctx.get_tuning("swat", 7)["chunk"] or 32
+        steps = 16384 + 16384 + (16384 // 64) * chunk
+        dep_ms = steps * 30 / 2.4e9 * 1e3
+        issue_ms = steps * 39.3 * 4 / 2.4e9 * 1e3
         line["roofline"] = {"bound": "valu", "achieved": round(n_cells * 10 / fill_s / 1e12, 4),
                             "peak": round(PEAK_INT32_TOPS, 2), "unit": "TOP/s",
                             "frac": round(n_cells * 10 / fill_s / 1e12 / PEAK_INT32_TOPS, 5),
-                            "ops": "about 10 int32 ops per cell", "traffic": None,
-                            "hbm_frac": round(n_cells * 0.5 / fill_s / 8e12, 5)}
+                            "ops": "about 10 int32 ops per cell", "traffic": traffic, "traffic_source": tsrc,
+                            "hbm_frac": round(n_cells * 0.5 / fill_s / 8e12, 5),
+                            "valu_issue_util_counter": valu_util,
+                            "critical_path_steps": steps,
+                            "dependency_ceiling_ms": round(dep_ms, 3),
+                            "one_wave_issue_ceiling_ms": round(issue_ms, 3),
+                            "frac_of_issue_ceiling": round(issue_ms / (fill_s * 1e3), 3)}
     print(json.dumps(line))
 
 
