@@ -92,7 +92,10 @@ struct Workspace {
 
 enum { JOB_TRI = 0, JOB_SW = 1 };
 // device block of a triangular job
-enum { TL_TAB, TL_QUEUE, TL_DEP, TL_CTRL, TL_INST, TL_SEEDS, TL_DATA, TL_SPEC, TL_RES, TL_LENS, TL_TRACES, TL_N };
+enum {
+  TL_TAB, TL_QUEUE, TL_DEP, TL_CTRL, TL_INST, TL_SEEDS, TL_DATA, TL_SPEC, TL_RES, TL_LENS, TL_TRACES,
+  TL_MPDONE, TL_MPBASE, TL_MPPART, TL_N
+};
 // device block of an alignment job
 enum { SL_A, SL_B, SL_ROWS, SL_FLAGS, SL_BEST, SL_BT, SL_CP, SL_TR, SL_N };
 
@@ -104,6 +107,11 @@ struct Job {
   int g = G_ZUKER, count = 0, n_max = 0, min_loop = 0;
   bool want_trace = false;
   int64_t tiles = 0, trace_total = 0;
+  // split middle products (launches of few instances): queue items, partial
+  // slots and each tile's first slot (-1: not split)
+  bool mp_split = false;
+  int64_t items = 0, mp_slots = 0;
+  std::vector<int32_t> mp_base;
   std::vector<TriInst> inst;  // longest first (TriInst::out is the caller's index)
   std::vector<int64_t> trace_off_by_out;
   std::vector<unsigned long long> seeds;
@@ -256,11 +264,13 @@ int tri_grid(dpx_ctx* c, const Job& j) { return tri_grid_for(c, j.g, job_bucket(
 
 // ---------------------------------------------------------------- layouts
 
-// Device block of a triangular launch over instances with the given totals.
-size_t tri_layout(int g, int64_t tiles, int count, size_t data_bytes, int64_t trace_total, size_t* off) {
+// Device block of a triangular launch over instances with the given totals;
+// mp_slots > 0: the middle products are split (queue of `items` slots).
+size_t tri_layout(int g, int64_t tiles, int count, size_t data_bytes, int64_t trace_total, size_t* off,
+                  int64_t items = 0, int64_t mp_slots = 0) {
   size_t sz[TL_N];
   sz[TL_TAB] = (size_t)tiles * nt_of(g) * T * T * sizeof(int32_t);
-  sz[TL_QUEUE] = (size_t)tiles * 8;
+  sz[TL_QUEUE] = (size_t)std::max(tiles, items) * 8;
   sz[TL_DEP] = (size_t)tiles * 4;
   sz[TL_CTRL] = 64;
   sz[TL_INST] = sizeof(TriInst) * (size_t)count;
@@ -271,6 +281,9 @@ size_t tri_layout(int g, int64_t tiles, int count, size_t data_bytes, int64_t tr
   sz[TL_RES] = 4 * (size_t)count;
   sz[TL_LENS] = 4 * (size_t)count;
   sz[TL_TRACES] = sizeof(dpx_trace_t) * (size_t)trace_total;
+  sz[TL_MPDONE] = mp_slots ? (size_t)tiles * 4 : 0;
+  sz[TL_MPBASE] = mp_slots ? (size_t)tiles * 4 : 0;
+  sz[TL_MPPART] = (size_t)mp_slots * T * T * sizeof(int32_t);
   size_t at = 0;
   for (int k = 0; k < TL_N; ++k) {
     if (off) off[k] = at;
@@ -324,10 +337,46 @@ int sw_trace_cap(int n1, int n2) { return 2 * (n1 + n2) + 8; }
 
 // ---------------------------------------------------------------- jobs
 
+// Split middle products pay off while a launch has too few ready tiles to
+// fill the chip: launches of at most this many instances split them, when
+// the partial results fit kMpSplitBytes.
+constexpr int kMpSplitMax = 8;
+constexpr int64_t kMpSplitBytes = (int64_t)4 << 30;
+
+// per-tile partial slots of the split middle products; false if not split
+bool mp_plan(Job& j) {
+  int64_t extra = 0, slots = 0;
+  for (const TriInst& t : j.inst)
+    for (int d = 0; d < t.nb; ++d) {
+      const int P = mp_parts(d);
+      if (P > 1) {
+        extra += (int64_t)(t.nb - d) * (P - 1);
+        slots += (int64_t)(t.nb - d) * P;
+      }
+    }
+  if (!slots || slots * T * T * 4 > kMpSplitBytes || j.tiles + extra >= ((int64_t)1 << 31)) return false;
+  j.mp_base.assign(j.tiles, -1);
+  int32_t at = 0;
+  for (const TriInst& t : j.inst)
+    for (int I = 0; I < t.nb; ++I)
+      for (int J = I; J < t.nb; ++J) {
+        const int P = mp_parts(J - I);
+        if (P > 1) {
+          j.mp_base[t.tile0 + tile_index(t.nb, I, J)] = at;
+          at += P;
+        }
+      }
+  j.mp_split = true;
+  j.items = j.tiles + extra;
+  j.mp_slots = slots;
+  return true;
+}
+
 // Triangular job over `count` instances: lens[q] and the per-instance data
 // (codes for RNA, rows|cols for the chain) at data_off[q] within `data`.
+// split_ok: the launch may split middle products (mp_plan).
 int tri_prepare(std::string& err, Job& j, int g, const std::vector<int>& lens, const std::vector<int64_t>& data_off,
-                std::vector<uint8_t>&& data, int min_loop, bool want_trace) {
+                std::vector<uint8_t>&& data, int min_loop, bool want_trace, bool split_ok = true) {
   const int count = (int)lens.size();
   j.kind = JOB_TRI;
   j.g = g;
@@ -365,7 +414,12 @@ int tri_prepare(std::string& err, Job& j, int g, const std::vector<int>& lens, c
   j.seeds.reserve(j.tiles);
   for (int q = 0; q < count; ++q)
     for (int I = 0; I < j.inst[q].nb; ++I) j.seeds.push_back(pack_item(q, I, I));
-  j.dev_bytes = tri_layout(g, j.tiles, count, j.data.size(), j.trace_total, j.off);
+  j.mp_split = false;
+  j.items = j.tiles;
+  j.mp_slots = 0;
+  j.mp_base.clear();
+  if (split_ok && count <= kMpSplitMax) mp_plan(j);
+  j.dev_bytes = tri_layout(g, j.tiles, count, j.data.size(), j.trace_total, j.off, j.items, j.mp_slots);
   return DPX_OK;
 }
 
@@ -375,6 +429,13 @@ TriArgs tri_args(const dpx_ctx* c, const Job& j, char* dev) {
   a.inst = (const TriInst*)(dev + j.off[TL_INST]);
   a.n_inst = j.count;
   a.total_tiles = (int32_t)j.tiles;
+  a.total_items = (int32_t)(j.mp_split ? j.items : j.tiles);
+  a.mp_split = j.mp_split ? 1 : 0;
+  if (j.mp_split) {
+    a.mp_done = (int32_t*)(dev + j.off[TL_MPDONE]);
+    a.mp_base = (const int32_t*)(dev + j.off[TL_MPBASE]);
+    a.mp_part = (int32_t*)(dev + j.off[TL_MPPART]);
+  }
   a.queue = (unsigned long long*)(dev + j.off[TL_QUEUE]);
   a.ctrl = (int32_t*)(dev + j.off[TL_CTRL]);
   a.dep = (int32_t*)(dev + j.off[TL_DEP]);
@@ -395,7 +456,7 @@ TriArgs tri_args(const dpx_ctx* c, const Job& j, char* dev) {
 
 int tri_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
   const int g = j.g, count = j.count;
-  const size_t in_bytes = sizeof(TriInst) * count + j.seeds.size() * 8 + j.data.size() + 64;
+  const size_t in_bytes = sizeof(TriInst) * count + j.seeds.size() * 8 + j.data.size() + 64 + j.mp_base.size() * 4;
   const size_t out_bytes = 8 * (size_t)count + 64 + (j.want_trace ? sizeof(dpx_trace_t) * (size_t)j.trace_total : 0);
   if (!w.h_in.ensure(in_bytes) || !w.h_out.ensure(out_bytes)) return fail(err, DPX_ENOMEM, "pinned host staging");
   // stage the inputs in pinned memory, then copy them asynchronously
@@ -405,10 +466,14 @@ int tri_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
   std::memcpy(p_seeds, j.seeds.data(), j.seeds.size() * 8);
   char* p_data = p_seeds + j.seeds.size() * 8;
   if (!j.data.empty()) std::memcpy(p_data, j.data.data(), j.data.size());
+  char* p_base = p_data + ((j.data.size() + 63) & ~(size_t)63);
+  if (j.mp_split) std::memcpy(p_base, j.mp_base.data(), j.mp_base.size() * 4);
   hipStream_t s = w.stream;
   HIPCHK(hipMemcpyAsync(dev + j.off[TL_INST], p_inst, sizeof(TriInst) * count, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dev + j.off[TL_SEEDS], p_seeds, j.seeds.size() * 8, hipMemcpyHostToDevice, s));
   if (!j.data.empty()) HIPCHK(hipMemcpyAsync(dev + j.off[TL_DATA], p_data, j.data.size(), hipMemcpyHostToDevice, s));
+  if (j.mp_split)
+    HIPCHK(hipMemcpyAsync(dev + j.off[TL_MPBASE], p_base, j.mp_base.size() * 4, hipMemcpyHostToDevice, s));
   const TriArgs a = tri_args(c, j, dev);
   const int grid = tri_grid(c, j);
   HIPCHK(hipEventRecord(w.ev[0], s));
@@ -844,7 +909,7 @@ int prepare_slot(dpx_server* S, Slot& sl) {
       std::vector<uint8_t> d = chain_data(r.rows.data(), r.cols.data(), lens[q]);
       data.insert(data.end(), d.begin(), d.end());
     }
-    return tri_prepare(sl.err, j, G_MATCHAIN, lens, off, std::move(data), 0, r0.trace != 0);
+    return tri_prepare(sl.err, j, G_MATCHAIN, lens, off, std::move(data), 0, r0.trace != 0, false);
   }
   for (int q = 0; q < count; ++q) {
     const Request& r = sl.reqs[q];
@@ -856,7 +921,8 @@ int prepare_slot(dpx_server* S, Slot& sl) {
     if (rc) return rc;
   }
   const int g = grammar_of(r0.kind);
-  return tri_prepare(sl.err, j, g, lens, off, std::move(data), r0.min_loop, r0.trace != 0);
+  // footprints admitted by the server are those of unsplit launches
+  return tri_prepare(sl.err, j, g, lens, off, std::move(data), r0.min_loop, r0.trace != 0, false);
 }
 
 void harvest_slot(dpx_server* S, Slot& sl, int status) {
@@ -1282,6 +1348,7 @@ int plan_build(dpx_plan* P) {
   HIPCHK(hipMemcpy(dev + j.off[TL_INST], j.inst.data(), sizeof(TriInst) * j.count, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dev + j.off[TL_SEEDS], j.seeds.data(), j.seeds.size() * 8, hipMemcpyHostToDevice));
   if (!j.data.empty()) HIPCHK(hipMemcpy(dev + j.off[TL_DATA], j.data.data(), j.data.size(), hipMemcpyHostToDevice));
+  if (j.mp_split) HIPCHK(hipMemcpy(dev + j.off[TL_MPBASE], j.mp_base.data(), j.mp_base.size() * 4, hipMemcpyHostToDevice));
   const TriArgs a = tri_args(c, j, dev);
   const int grid = tri_grid(c, j);
   hipStream_t s = P->ws.stream;

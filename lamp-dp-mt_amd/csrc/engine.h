@@ -55,7 +55,12 @@ struct TriArgs {
   const TriInst* inst;
   int32_t n_inst;
   int32_t total_tiles;
-  unsigned long long* queue;   // total_tiles slots, ~0 = empty
+  int32_t total_items;         // queue slots: tiles, plus one more per extra part of a split middle product
+  int32_t mp_split;            // 1: the middle products of long tiles are split into part tasks (mp_parts)
+  int32_t* mp_done;            // per tile: part tasks finished
+  const int32_t* mp_base;      // per tile: first partial slot of its part tasks (split tiles)
+  int32_t* mp_part;            // partial results, T*T int32 per slot
+  unsigned long long* queue;   // total_items slots, ~0 = empty
   int32_t* ctrl;               // [0] next slot to claim, [1] next slot to fill
   int32_t* dep;                // completed predecessors per tile
   int32_t* tab;                // tile records
@@ -77,8 +82,17 @@ struct TriArgs {
 
 __host__ __device__ inline int tile_index(int nb, int I, int J) { return I * nb - (I * (I - 1)) / 2 + (J - I); }
 __host__ __device__ inline int64_t tiles_of(int nb) { return (int64_t)nb * (nb + 1) / 2; }
-__host__ __device__ inline unsigned long long pack_item(int inst, int I, int J) {
-  return ((unsigned long long)(unsigned)inst << 32) | ((unsigned long long)(unsigned)I << 16) | (unsigned)J;
+__host__ __device__ inline unsigned long long pack_item(int inst, int I, int J, int part = 0) {
+  // I < 4096 (n <= 30000): bits 12..15 of the I field hold the part index
+  return ((unsigned long long)(unsigned)inst << 32) | ((unsigned long long)(unsigned)((part << 12) | I) << 16) |
+         (unsigned)J;
+}
+// Part tasks of the middle product of a tile J - I blocks off the diagonal
+// when a launch splits them: 1 (not split) below 8 split blocks, else one per
+// 4 blocks, at most 8.
+__host__ __device__ inline int mp_parts(int d) {
+  const int nk = d - 1;
+  return nk < 8 ? 1 : (nk / 4 < 8 ? nk / 4 : 8);
 }
 
 // Smith-Waterman/Gotoh (swat.hip)

@@ -94,7 +94,7 @@ constexpr int MP_STAGE = T * T + T * T + T;
 __device__ __forceinline__ int mp_swz(int k) { return ((k >> 2) & 7) << 2; }
 template <bool MAXP, bool CHAIN>
 __device__ void middle_product(const TriArgs& a, const TriInst& in, int NTv, int LT, int RT, int I, int J,
-                               int* stage, int acc[4][4]) {
+                               int* stage, int acc[4][4], int part = 0, int P = 1) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r0 = (lane >> 3) * 4, c0 = (lane & 7) * 4;
   int* sL = stage + w * MP_STAGE;  // [k][r ^ mp_swz(k)]
@@ -123,8 +123,10 @@ __device__ void middle_product(const TriArgs& a, const TriInst& in, int NTv, int
       vr[q] = *(const int4*)(gR + e);
     }
   };
-  if (I + 1 + w < J) load(I + 1 + w);
-  for (int K = I + 1 + w; K < J; K += NWAVE) {
+  // blocks K = I + 1 + part + P m; wave w takes every NWAVE-th of them
+  const int K0 = I + 1 + part + P * w, KS = P * NWAVE;
+  if (K0 < J) load(K0);
+  for (int K = K0; K < J; K += KS) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = (q * 64 + lane) * 4;
@@ -135,7 +137,7 @@ __device__ void middle_product(const TriArgs& a, const TriInst& in, int NTv, int
       sL[(k + 3) * T + rs] = vl[q].w;
       *(int4*)(sR + e) = vr[q];
     }
-    if (K + NWAVE < J) load(K + NWAVE);
+    if (K + KS < J) load(K + KS);
     if (CHAIN && lane < T) {
       const int kk = K * T + lane;
       sK[lane] = kk < in.n ? a.dims[in.seq_off + kk] : 0;
@@ -160,6 +162,52 @@ __device__ void middle_product(const TriArgs& a, const TriInst& in, int NTv, int
   }
 }
 
+// The middle product of tile (I,J) reduced into X, which the caller has set
+// to MP::NEUTRAL; ends with a workgroup barrier.  PART: the blocks
+// K = I + 1 + part + P m only (part task `part` of P).  Otherwise P == 1:
+// every split block here; P > 1: the P part tasks' partial results, from
+// a.mp_part.  min/max is associative and the candidates the threshold drops
+// (Zuker-like MP::THR) are dropped in either order, so every way gives the
+// same X.
+template <class MP, bool PART>
+__device__ void middle_X(const TriArgs& a, const TriInst& in, int I, int J, int* stage, int (*X)[T + 1], int part,
+                         int P) {
+  const int tid = threadIdx.x;
+  if (J - I < 2) {
+    __syncthreads();
+    return;
+  }
+  if (!PART && P > 1) {
+    __syncthreads();  // X's initial values are in place
+    const int g = (int)(in.tile0 + tile_index(in.nb, I, J));
+    const int32_t* src = a.mp_part + (size_t)a.mp_base[g] * (T * T);
+    for (int e = tid; e < T * T; e += NTHR) {
+      int v = X[e / T][e % T];
+      for (int p = 0; p < P; ++p) {
+        const int c = src[(size_t)p * (T * T) + e];
+        v = MP::MAXP ? imax(v, c) : imin(v, c);
+      }
+      X[e / T][e % T] = v;
+    }
+    __syncthreads();
+    return;
+  }
+  int acc[4][4];
+  for (int t = 0; t < 4; ++t)
+    for (int u = 0; u < 4; ++u) acc[t][u] = MP::NEUTRAL;
+  if (PART) middle_product<MP::MAXP, MP::CHAIN>(a, in, MP::NTv, MP::LT, MP::RT, I, J, stage, acc, part, P);
+  else middle_product<MP::MAXP, MP::CHAIN>(a, in, MP::NTv, MP::LT, MP::RT, I, J, stage, acc);
+  __syncthreads();
+  const int lane = tid & 63, r0 = (lane >> 3) * 4, c0 = (lane & 7) * 4;
+  for (int t = 0; t < 4; ++t)
+    for (int u = 0; u < 4; ++u) {
+      if (MP::THR && acc[t][u] >= INV_THR) continue;
+      if (MP::MAXP) atomicMax(&X[r0 + t][c0 + u], acc[t][u]);
+      else atomicMin(&X[r0 + t][c0 + u], acc[t][u]);
+    }
+  __syncthreads();
+}
+
 // ============================================================ queue driver
 // Wave-uniform 64-bit value (lane 0's).
 __device__ __forceinline__ unsigned long long uniform64(unsigned long long v) {
@@ -176,7 +224,7 @@ __device__ __forceinline__ unsigned long long claim_item(const TriArgs& a) {
   int slot = 0;
   if (threadIdx.x == 0) slot = __hip_atomic_fetch_add(&a.ctrl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   slot = __builtin_amdgcn_readfirstlane(slot);
-  if (slot >= a.total_tiles) return ~0ull;
+  if (slot >= a.total_items) return ~0ull;
   // Poll with relaxed loads.  Bounded wait (a few seconds): a scheduling bug
   // ends the kernel with an error flag instead of a hang.
   unsigned long long it = uniform64(__hip_atomic_load(&a.queue[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -197,10 +245,21 @@ __device__ __forceinline__ unsigned long long claim_item(const TriArgs& a) {
   return it;
 }
 
+// A part task of a split middle product: its blocks into X, X to its partial slot.
 template <class G>
-__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) void tri_fill(TriArgs a) {
+__device__ void mp_part_task(const TriArgs& a, const TriInst& in, int I, int J, int part, int P, typename G::Smem& sm) {
+  for (int e = threadIdx.x; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = G::MP::NEUTRAL;
+  middle_X<typename G::MP, true>(a, in, I, J, sm.u.stage, sm.X, part, P);
+  const int g = (int)(in.tile0 + tile_index(in.nb, I, J));
+  int32_t* dst = a.mp_part + (size_t)(a.mp_base[g] + part) * (T * T);
+  for (int e = threadIdx.x; e < T * T; e += NTHR) dst[e] = sm.X[e / T][e % T];
+}
+
+template <class G>
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(G::WAVES))) void tri_fill(TriArgs a) {
   __shared__ typename G::Smem sm;
   __shared__ unsigned long long s_item;
+  __shared__ int s_fin;
   G::init(a, sm);
   bool live = true;
   while (live) {
@@ -215,9 +274,28 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) void 
       live = false;
     } else {
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      const int q = (int)(it >> 32), I = (int)((it >> 16) & 0xffff), J = (int)(it & 0xffff);
+      const int q = (int)(it >> 32), I = (int)((it >> 16) & 0xfff), part = (int)((it >> 28) & 0xf),
+                J = (int)(it & 0xffff);
       const TriInst in = a.inst[q];
-      G::tile(a, in, I, J, sm);
+      const int P = a.mp_split ? mp_parts(J - I) : 1;
+      if (P > 1) {
+        // a part task; the one that completes the count goes on with the tile
+        mp_part_task<G>(a, in, I, J, part, P, sm);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the partial is written
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          const int g = (int)(in.tile0 + tile_index(in.nb, I, J));
+          const int old = __hip_atomic_fetch_add(&a.mp_done[g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          s_fin = old == P - 1;
+          // the other parts' partials, written back before their counts
+          if (old == P - 1) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+        if (!s_fin) continue;
+      }
+      G::tile(a, in, I, J, sm, 0, P);
       // every wave's stores have completed before the barrier; thread 0's
       // release RMW below then publishes them at device scope
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -233,8 +311,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) void 
           const int g = (int)(in.tile0 + tile_index(in.nb, I2, J2));
           const int old = __hip_atomic_fetch_add(&a.dep[g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if (old == 1) {
-            const int pos = __hip_atomic_fetch_add(&a.ctrl[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&a.queue[pos], pack_item(q, I2, J2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int P2 = a.mp_split ? mp_parts(J2 - I2) : 1;
+            const int pos = __hip_atomic_fetch_add(&a.ctrl[1], P2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int p2 = 0; p2 < P2; ++p2)
+              __hip_atomic_store(&a.queue[pos + p2], pack_item(q, I2, J2, p2), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
           }
         }
       }
@@ -271,9 +352,10 @@ __global__ void tri_spec(TriArgs a) {
 // Queue reset: graph-capturable (no memset nodes).
 __global__ void tri_reset(TriArgs a, const unsigned long long* seeds, int nseed) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < a.total_items) a.queue[t] = t < nseed ? seeds[t] : ~0ull;
   if (t < a.total_tiles) {
-    a.queue[t] = t < nseed ? seeds[t] : ~0ull;
     a.dep[t] = 0;
+    if (a.mp_split) a.mp_done[t] = 0;
   }
   if (t == 0) { a.ctrl[0] = 0; a.ctrl[1] = nseed; a.ctrl[2] = 0; }
 }
@@ -288,6 +370,7 @@ __device__ __forceinline__ bool nu_pair(int x, int y) {
 }
 
 struct GNu {
+  static constexpr int WAVES = 4;  // waves per SIMD the register budget allows
   template <class S> __device__ static void init(const TriArgs&, S&) {}
   struct Smem {
     union {
@@ -301,21 +384,16 @@ struct GNu {
     int X[T][T + 1];
   };
 
-  __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm) {
+  struct MP {
+    static constexpr bool MAXP = true, CHAIN = false, THR = false;
+    static constexpr int NEUTRAL = NEG, NTv = NT_NU, LT = 0, RT = 0;
+  };
+
+  __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm, int part = 0, int P = 1) {
     const int tid = threadIdx.x, IT = I * T, JT = J * T, n = in.n;
     const uint8_t* s = a.codes + in.seq_off;
-    int acc[4][4];
-    for (int t = 0; t < 4; ++t)
-      for (int u = 0; u < 4; ++u) acc[t][u] = NEG;
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = NEG;
-    if (J - I >= 2) middle_product<true, false>(a, in, NT_NU, 0, 0, I, J, sm.u.stage, acc);
-    __syncthreads();
-    {
-      const int lane = tid & 63, r0 = (lane >> 3) * 4, c0 = (lane & 7) * 4;
-      if (J - I >= 2)
-        for (int t = 0; t < 4; ++t)
-          for (int u = 0; u < 4; ++u) atomicMax(&sm.X[r0 + t][c0 + u], acc[t][u]);
-    }
+    middle_X<MP, false>(a, in, I, J, sm.u.stage, sm.X, part, P);
     // stage the frontier
     for (int e = tid; e < (T + 1) * (T + 1); e += NTHR) {
       const int rr = e / (T + 1), cc = e % (T + 1);
@@ -384,6 +462,7 @@ struct GNu {
 // M(i,j) = min_{i<k<j} M(i,k)+M(k,j)+rows[i]*rows[k]*cols[j-1]  (int32 wrap),
 // M(i,i+1) = 0   (Benchmarks.scala:18-27)
 struct GMc {
+  static constexpr int WAVES = 4;  // waves per SIMD the register budget allows
   template <class S> __device__ static void init(const TriArgs&, S&) {}
   struct Smem {
     union {
@@ -398,22 +477,17 @@ struct GMc {
     int X[T][T + 1];
   };
 
-  __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm) {
+  struct MP {
+    static constexpr bool MAXP = false, CHAIN = true, THR = false;
+    static constexpr int NEUTRAL = 0x7fffffff, NTv = NT_MC, LT = 0, RT = 0;
+  };
+
+  __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm, int part = 0, int P = 1) {
     const int tid = threadIdx.x, IT = I * T, JT = J * T, n = in.n;
     const int32_t* rows = a.dims + in.seq_off;
     const int32_t* cols = rows + n;
-    int acc[4][4];
-    for (int t = 0; t < 4; ++t)
-      for (int u = 0; u < 4; ++u) acc[t][u] = 0x7fffffff;
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = 0x7fffffff;
-    if (J - I >= 2) middle_product<false, true>(a, in, NT_MC, 0, 0, I, J, sm.u.stage, acc);
-    __syncthreads();
-    {
-      const int lane = tid & 63, r0 = (lane >> 3) * 4, c0 = (lane & 7) * 4;
-      if (J - I >= 2)
-        for (int t = 0; t < 4; ++t)
-          for (int u = 0; u < 4; ++u) atomicMin(&sm.X[r0 + t][c0 + u], acc[t][u]);
-    }
+    middle_X<MP, false>(a, in, I, J, sm.u.stage, sm.X, part, P);
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.u.w.S[0][0])[e] = 0;
     if (I < J) {
       const int* gL = rec_ptr(a, in, NT_MC, I, I);
@@ -594,6 +668,7 @@ __device__ void load_tab(const TriArgs& a, ZuTab& E) {
 }
 
 struct GZu {
+  static constexpr int WAVES = 3;  // waves per SIMD the register budget allows
   static constexpr int BR = 64, BC = 64;   // B region: rows IT..IT+63, cols JT-32..JT+31
   static constexpr int BS = BC + 1;        // odd row stride: lanes walking a column hit distinct banks
   // The interior-loop window of a cell on diagonal d reads diagonals <= d-6,
@@ -734,14 +809,16 @@ struct GZu {
     return best;
   }
 
-  __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm) {
+  struct MP {
+    static constexpr bool MAXP = false, CHAIN = false, THR = true;
+    static constexpr int NEUTRAL = INV, NTv = NT_ZU, LT = ZT_U, RT = ZT_ML1;
+  };
+
+  __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm, int part = 0, int nparts = 1) {
     const int tid = threadIdx.x, IT = I * T, JT = J * T, n = in.n, nb = in.nb;
     const uint8_t* seq = a.codes + in.seq_off;
     const DevRna* P = (const DevRna*)a.rna;
     const ZuView z{sm.E, P, a.hl_ent, sm.rs, sm.cs, IT - 1, JT - 34, n, a.spec + in.seq_off, a.n_codes};
-    int acc[4][4];
-    for (int t = 0; t < 4; ++t)
-      for (int u = 0; u < 4; ++u) acc[t][u] = INV;
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = INV;
     for (int e = tid; e < G * T; e += NTHR) (&sm.acc[0][0])[e] = INV;
     if (tid == 0) sm.nlist[0] = 0;
@@ -750,15 +827,7 @@ struct GZu {
       sm.rs[tid] = (x >= 0 && x < n) ? seq[x] : 0;
       sm.cs[tid] = (y >= 0 && y < n) ? seq[y] : 0;
     }
-    if (J - I >= 2) middle_product<false, false>(a, in, NT_ZU, ZT_U, ZT_ML1, I, J, sm.u.stage, acc);
-    __syncthreads();
-    {
-      const int lane = tid & 63, r0 = (lane >> 3) * 4, c0 = (lane & 7) * 4;
-      if (J - I >= 2)
-        for (int t = 0; t < 4; ++t)
-          for (int u = 0; u < 4; ++u)
-            if (acc[t][u] < INV_THR) atomicMin(&sm.X[r0 + t][c0 + u], acc[t][u]);
-    }
+    middle_X<MP, false>(a, in, I, J, sm.u.stage, sm.X, part, nparts);
     // ---- stage the frontier (values of finished neighbour tiles) ----
     auto fetch = [&](int tab, int i, int j) -> int {
       if (i < 0 || j < 0 || i > j || j > n) return INV;
@@ -945,6 +1014,7 @@ struct GZu {
 // being its bulges; so the same 32-lane evaluation is used with the stacked
 // pair's energy set to zero and sizes capped at 29.  (0,0) is the stack rule.
 struct GRf {
+  static constexpr int WAVES = 3;  // waves per SIMD the register budget allows
   static constexpr int BR = 64, BS = 65;
   // window candidates of a cell on diagonal d read diagonals <= d-3 (except
   // (0,0), the stack), the split points of QM <= d-3: bands of 3 diagonals
@@ -1044,14 +1114,16 @@ struct GRf {
     }
   }
 
-  __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm) {
+  struct MP {
+    static constexpr bool MAXP = false, CHAIN = false, THR = true;
+    static constexpr int NEUTRAL = INV, NTv = NT_RF, LT = RT_Q, RT = RT_Q;
+  };
+
+  __device__ static void tile(const TriArgs& a, const TriInst& in, int I, int J, Smem& sm, int part = 0, int nparts = 1) {
     const int tid = threadIdx.x, IT = I * T, JT = J * T, n = in.n, nb = in.nb;
     const uint8_t* seq = a.codes + in.seq_off;
     const DevRna* P = (const DevRna*)a.rna;
     const ZuView z{sm.E, P, a.hl_ent, sm.rs, sm.cs, IT - 1, JT - 34, n, a.spec + in.seq_off, a.n_codes};
-    int acc[4][4];
-    for (int t = 0; t < 4; ++t)
-      for (int u = 0; u < 4; ++u) acc[t][u] = INV;
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = INV;
     for (int e = tid; e < G * T; e += NTHR) (&sm.acc[0][0])[e] = INV;
     if (tid == 0) sm.nlist[0] = 0;
@@ -1061,15 +1133,7 @@ struct GRf {
       sm.cs[tid] = (y >= 0 && y < n) ? seq[y] : 0;
     }
     // QM split points in blocks strictly between I and J: Q (x) Q
-    if (J - I >= 2) middle_product<false, false>(a, in, NT_RF, RT_Q, RT_Q, I, J, sm.u.stage, acc);
-    __syncthreads();
-    {
-      const int lane = tid & 63, r0 = (lane >> 3) * 4, c0 = (lane & 7) * 4;
-      if (J - I >= 2)
-        for (int t = 0; t < 4; ++t)
-          for (int u = 0; u < 4; ++u)
-            if (acc[t][u] < INV_THR) atomicMin(&sm.X[r0 + t][c0 + u], acc[t][u]);
-    }
+    middle_X<MP, false>(a, in, I, J, sm.u.stage, sm.X, part, nparts);
     auto fetch = [&](int tab, int i, int j) -> int {
       if (i < 0 || j < 0 || i > j || j > n) return INV;
       const int I2 = i / T, J2 = j / T;
@@ -1223,7 +1287,7 @@ int fill_occupancy(int grammar) {
 
 int launch_fill(int grammar, const TriArgs& a, const unsigned long long* seeds, int nseed, int grid, void* stream) {
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(tri_reset, dim3((a.total_tiles + 255) / 256), dim3(256), 0, st, a, seeds, nseed);
+  hipLaunchKernelGGL(tri_reset, dim3((max(a.total_items, a.total_tiles) + 255) / 256), dim3(256), 0, st, a, seeds, nseed);
   if (a.spec && a.n_codes > 0)
     hipLaunchKernelGGL(tri_spec, dim3((unsigned)((a.n_codes + 255) / 256)), dim3(256), 0, st, a);
   if (grammar == G_NUSSINOV) hipLaunchKernelGGL(tri_fill<GNu>, dim3(grid), dim3(NTHR), 0, st, a);

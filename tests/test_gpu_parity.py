@@ -167,6 +167,31 @@ def test_trace_buffer_too_small(ctx):
     assert score.value == ZU[3]["score"]
 
 
+def test_split_middle_product_equals_unsplit(ctx):
+    """A launch of at most 8 instances splits the middle product of each tile
+    at least 9 blocks off the diagonal into part tasks whose partial results
+    the last one to finish reduces; a launch of 9 or more does not.  The same
+    sequences give the same results either way, and the oracle's."""
+    from dpx import gen
+    longs = [gen.rna_gc(n, 700 + n) for n in (290, 700, 1500)]
+    fill = [gen.rna_gc(40 + k, 800 + k) for k in range(6)]
+    for name, fn in (("zuker", ctx.zuker_batch), ("rnafold", ctx.rnafold_batch),
+                     ("nussinov", lambda s: ctx.nussinov_batch(s, min_loop=3))):
+        print("split vs unsplit: %s" % name, flush=True)
+        split = fn(longs)
+        whole = fn(longs + fill)
+        assert split[0] == whole[0][:3] and split[1] == whole[1][:3], name
+        if name == "zuker":
+            osc, otr = o.zuker(longs[0])
+            assert split[0][0] == osc and _zn(split[1][0]) == _zn(otr)
+        if name == "nussinov":
+            osc, otr = o.nussinov(longs[1], min_loop=3)
+            assert split[0][1] == osc and [t[:4] for t in split[1][1]] == [t[:4] for t in otr]
+    rows, cols = gen.matrix_chain(600, 5)
+    print("split: matrix chain n=600 vs oracle", flush=True)
+    assert ctx.matchain(rows, cols) == o.matchain(rows, cols)
+
+
 def test_plan_graph_replays(ctx):
     """A batch captured as a HIP graph: every replay, with eager work of the
     same context in between, gives the eager batch's results."""
