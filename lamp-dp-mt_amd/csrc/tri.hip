@@ -360,6 +360,19 @@ __global__ void tri_reset(TriArgs a, const unsigned long long* seeds, int nseed)
   if (t == 0) { a.ctrl[0] = 0; a.ctrl[1] = nseed; a.ctrl[2] = 0; }
 }
 
+// max (MAXP) or min over the 8 lanes 8p..8p+7 of a group, valid on lane
+// 8p: two quad permutations and a row shift by 4, as DPP operands (no LDS
+// round trip).  Groups are wholly active or wholly inactive.
+template <bool MAXP>
+__device__ __forceinline__ int group8_reduce(int v) {
+  const int a = __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+  v = MAXP ? imax(v, a) : imin(v, a);
+  const int b = __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false);  // quad_perm [2,3,0,1]
+  v = MAXP ? imax(v, b) : imin(v, b);
+  const int c = __builtin_amdgcn_update_dpp(v, v, 0x104, 0xf, 0xf, false);  // row_shl:4
+  return MAXP ? imax(v, c) : imin(v, c);
+}
+
 // ============================================================== Nussinov
 // s(i,j) = max{0 [i==j]; s(i+1,j); s(i,j-1); s(i+1,j-1)+1 [pair(i,j-1), i+2+m<=j];
 //             max_{i<k<j} s(i,k)+s(k,j)}      (bench2/nu_kern.h:23-45)
@@ -379,6 +392,7 @@ struct GNu {
         int S[T + 1][T + 2];  // origin (IT, JT-1)
         int L[T][T + 1];      // tile (I,I) [r][k]
         int R[T][T + 1];      // tile (J,J) [k][c]
+        int PB[T][T + 1];     // pair rule: 1 if (i, j-1) pairs and the loop is long enough, else NEG
       } w;
     } u;
     int X[T][T + 1];
@@ -414,39 +428,54 @@ struct GNu {
         sm.u.w.R[e / T][e % T] = gR[e];
       }
     }
+    // the sequence-dependent rule of every cell, before the wavefront
+    for (int e = tid; e < T * T; e += NTHR) {
+      const int i = IT + e / T, j = JT + e % T;
+      sm.u.w.PB[e / T][e % T] = (i + 2 + a.min_loop <= j && j <= n && nu_pair(s[i], s[j - 1])) ? 1 : NEG;
+    }
     __syncthreads();
     const int dlo = I == J ? 0 : -(T - 1);
     for (int ld = dlo; ld <= T - 1; ++ld) {
       const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
-      // phase A: split points inside blocks I and J, 8 lanes per cell (a
-      // group of 8 in one wave), each lane a strided share of both ranges
-      // with independent loads, then a max over the group
+      // split points inside blocks I and J, 8 lanes per cell (a group of 8
+      // in one wave), each lane a strided share of both ranges; every load
+      // of the step, the cell's own terms included, is issued before the
+      // first is used: one LDS round trip, then a DPP max over the group
       if (tid < ncell * 8) {
         const int p = tid >> 3, part = tid & 7, r = rmin + p, c = r + ld, i = IT + r, j = JT + c;
+        // (i+1,j), (i,j-1), (i+1,j-1); cells below the diagonal hold NEG
+        const int xv = sm.X[r][c], pb = sm.u.w.PB[r][c];
+        const int dn = sm.u.w.S[r + 1][c + 1], lf = sm.u.w.S[r][c], dg = sm.u.w.S[r + 1][c];
+        // lane `part` takes kk = part + 8m (m < 4) of each range, with
+        // fixed trip counts: every operand address is inside the tile
+        // arrays, out-of-range candidates are masked after the loads
         int best = NEG;
-        if (i < j && j <= n) {
-          int k1lo, k1hi, k2lo, k2hi;
-          if (I == J) { k1lo = i + 1; k1hi = j - 1; k2lo = 1; k2hi = 0; }
-          else { k1lo = i + 1; k1hi = IT + T - 1; k2lo = JT; k2hi = j - 1; }
-          if (I == J) {
-            for (int k = k1lo + part; k <= k1hi; k += 8) best = imax(best, sm.u.w.S[r][k - JT + 1] + sm.u.w.S[k - IT][c + 1]);
-          } else {
-            for (int k = k1lo + part; k <= k1hi; k += 8) best = imax(best, sm.u.w.L[r][k - IT] + sm.u.w.S[k - IT][c + 1]);
-            for (int k = k2lo + part; k <= k2hi; k += 8) best = imax(best, sm.u.w.S[r][k - JT + 1] + sm.u.w.R[k - JT][c]);
+        if (I == J) {
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {  // k = IT + kk, i < k < j
+            const int kk = part + 8 * m, x = sm.u.w.S[r][kk + 1] + sm.u.w.S[kk][c + 1];
+            best = imax(best, kk > r && kk < c ? x : NEG);
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {  // k = IT + kk > i, and k = JT + kk < j
+            const int kk = part + 8 * m;
+            const int x1 = sm.u.w.L[r][kk] + sm.u.w.S[kk][c + 1], x2 = sm.u.w.S[r][kk + 1] + sm.u.w.R[kk][c];
+            best = imax(best, imax(kk > r ? x1 : NEG, kk < c ? x2 : NEG));
           }
         }
-        best = imax(best, __shfl_xor(best, 1));
-        best = imax(best, __shfl_xor(best, 2));
-        best = imax(best, __shfl_xor(best, 4));
-        // phase B on the group's first lane: the cell's other rules, then its
-        // value (the cells of this diagonal read only earlier diagonals)
+        if (!(i < j && j <= n)) best = NEG;
+        best = group8_reduce<true>(best);
+        // the cell's value on the group's first lane (the cells of this
+        // diagonal read only earlier diagonals).  For i < j, s(i+1,j) >= 0
+        // is a candidate, so NEG-based sums never win; for i == j the
+        // neighbours are below the diagonal (NEG)
         if (part == 0) {
           int v = NEG;
           if (i <= j && j <= n) {
-            v = imax(sm.X[r][c], best);
+            v = imax(imax(xv, best), imax(dn, lf));
             if (i == j) v = imax(v, 0);
-            if (i + 1 <= j) { v = imax(v, sm.u.w.S[r + 1][c + 1]); v = imax(v, sm.u.w.S[r][c]); }
-            if (i + 2 + a.min_loop <= j && nu_pair(s[i], s[j - 1])) v = imax(v, sm.u.w.S[r + 1][c] + 1);
+            v = imax(v, dg + pb);
           }
           sm.u.w.S[r][c + 1] = v;
         }
@@ -472,6 +501,7 @@ struct GMc {
         int L[T][T + 1];      // tile (I,I)
         int R[T][T + 1];      // tile (J,J)
         int rows[2 * T];      // rows[IT..IT+T) , rows[JT..JT+T)
+        uint32_t AL[T][T + 1];  // rows[i] * cols[j-1] per cell
       } w;
     } u;
     int X[T][T + 1];
@@ -501,38 +531,46 @@ struct GMc {
       const int k = tid < T ? IT + tid : JT + tid - T;
       sm.u.w.rows[tid] = k < n ? rows[k] : 0;
     }
+    for (int e = tid; e < T * T; e += NTHR) {
+      const int i = IT + e / T, j = JT + e % T;
+      sm.u.w.AL[e / T][e % T] = (i + 1 < j && j <= n) ? (uint32_t)rows[i] * (uint32_t)cols[j - 1] : 0u;
+    }
     __syncthreads();
     const int dlo = I == J ? 0 : -(T - 1);
     for (int ld = dlo; ld <= T - 1; ++ld) {
       const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
       // split points inside blocks I and J: 8 lanes per cell, strided
-      // shares with independent loads, min over the group (int32 wrap: the
-      // minimum of the wrapped candidates, as the sequential loop keeps)
+      // shares with independent loads, DPP min over the group (int32 wrap:
+      // the minimum of the wrapped candidates, as the sequential loop keeps)
       if (tid < ncell * 8) {
         const int p = tid >> 3, part = tid & 7, r = rmin + p, c = r + ld, i = IT + r, j = JT + c;
-        int best = 0x7fffffff;
+        const int xv = sm.X[r][c];
+        const uint32_t alpha = sm.u.w.AL[r][c];
         const bool live = i + 1 < j && j <= n;
-        if (live) {
-          const uint32_t alpha = (uint32_t)rows[i] * (uint32_t)cols[j - 1];
-          if (I == J) {
-            for (int k = i + 1 + part; k <= j - 1; k += 8)
-              best = imin(best, (int)((uint32_t)sm.u.w.S[r][k - JT] + (uint32_t)sm.u.w.S[k - IT][c] +
-                                      alpha * (uint32_t)sm.u.w.rows[k - IT]));
-          } else {
-            for (int k = i + 1 + part; k <= IT + T - 1; k += 8)
-              best = imin(best, (int)((uint32_t)sm.u.w.L[r][k - IT] + (uint32_t)sm.u.w.S[k - IT][c] +
-                                      alpha * (uint32_t)sm.u.w.rows[k - IT]));
-            for (int k = JT + part; k <= j - 1; k += 8)
-              best = imin(best, (int)((uint32_t)sm.u.w.S[r][k - JT] + (uint32_t)sm.u.w.R[k - JT][c] +
-                                      alpha * (uint32_t)sm.u.w.rows[T + k - JT]));
+        // kk = part + 8m (m < 4) per range, fixed trip counts, masked after the loads
+        int best = 0x7fffffff;
+        if (I == J) {
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {  // k = IT + kk, i < k < j
+            const int kk = part + 8 * m;
+            const int x = (int)((uint32_t)sm.u.w.S[r][kk] + (uint32_t)sm.u.w.S[kk][c] + alpha * (uint32_t)sm.u.w.rows[kk]);
+            best = imin(best, kk > r && kk < c ? x : 0x7fffffff);
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {  // k = IT + kk > i, and k = JT + kk < j
+            const int kk = part + 8 * m;
+            const int x1 = (int)((uint32_t)sm.u.w.L[r][kk] + (uint32_t)sm.u.w.S[kk][c] + alpha * (uint32_t)sm.u.w.rows[kk]);
+            const int x2 =
+                (int)((uint32_t)sm.u.w.S[r][kk] + (uint32_t)sm.u.w.R[kk][c] + alpha * (uint32_t)sm.u.w.rows[T + kk]);
+            best = imin(best, imin(kk > r ? x1 : 0x7fffffff, kk < c ? x2 : 0x7fffffff));
           }
         }
-        best = imin(best, __shfl_xor(best, 1));
-        best = imin(best, __shfl_xor(best, 2));
-        best = imin(best, __shfl_xor(best, 4));
+        if (!live) best = 0x7fffffff;
+        best = group8_reduce<false>(best);
         // the cell's value on the group's first lane (this diagonal's cells
         // read only earlier diagonals): M(i,i+1) = 0, else the minimum
-        if (part == 0) sm.u.w.S[r][c] = live ? imin(sm.X[r][c], best) : 0;
+        if (part == 0) sm.u.w.S[r][c] = live ? imin(xv, best) : 0;
       }
       __syncthreads();
     }
