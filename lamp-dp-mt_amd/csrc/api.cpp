@@ -94,7 +94,7 @@ enum { JOB_TRI = 0, JOB_SW = 1 };
 // device block of a triangular job
 enum {
   TL_TAB, TL_QUEUE, TL_DEP, TL_CTRL, TL_INST, TL_SEEDS, TL_DATA, TL_SPEC, TL_RES, TL_LENS, TL_TRACES,
-  TL_MPDONE, TL_MPBASE, TL_MPPART, TL_N
+  TL_MP, TL_N
 };
 // device block of an alignment job
 enum { SL_A, SL_B, SL_ROWS, SL_FLAGS, SL_BEST, SL_BT, SL_CP, SL_TR, SL_N };
@@ -281,9 +281,7 @@ size_t tri_layout(int g, int64_t tiles, int count, size_t data_bytes, int64_t tr
   sz[TL_RES] = 4 * (size_t)count;
   sz[TL_LENS] = 4 * (size_t)count;
   sz[TL_TRACES] = sizeof(dpx_trace_t) * (size_t)trace_total;
-  sz[TL_MPDONE] = mp_slots ? (size_t)tiles * 4 : 0;
-  sz[TL_MPBASE] = mp_slots ? (size_t)tiles * 4 : 0;
-  sz[TL_MPPART] = (size_t)mp_slots * T * T * sizeof(int32_t);
+  sz[TL_MP] = mp_slots ? (size_t)(mp_part_offset(tiles) + mp_slots * T * T) * sizeof(int32_t) : 0;
   size_t at = 0;
   for (int k = 0; k < TL_N; ++k) {
     if (off) off[k] = at;
@@ -431,11 +429,7 @@ TriArgs tri_args(const dpx_ctx* c, const Job& j, char* dev) {
   a.total_tiles = (int32_t)j.tiles;
   a.total_items = (int32_t)(j.mp_split ? j.items : j.tiles);
   a.mp_split = j.mp_split ? 1 : 0;
-  if (j.mp_split) {
-    a.mp_done = (int32_t*)(dev + j.off[TL_MPDONE]);
-    a.mp_base = (const int32_t*)(dev + j.off[TL_MPBASE]);
-    a.mp_part = (int32_t*)(dev + j.off[TL_MPPART]);
-  }
+  if (j.mp_split) a.mp = (int32_t*)(dev + j.off[TL_MP]);
   a.queue = (unsigned long long*)(dev + j.off[TL_QUEUE]);
   a.ctrl = (int32_t*)(dev + j.off[TL_CTRL]);
   a.dep = (int32_t*)(dev + j.off[TL_DEP]);
@@ -473,7 +467,7 @@ int tri_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
   HIPCHK(hipMemcpyAsync(dev + j.off[TL_SEEDS], p_seeds, j.seeds.size() * 8, hipMemcpyHostToDevice, s));
   if (!j.data.empty()) HIPCHK(hipMemcpyAsync(dev + j.off[TL_DATA], p_data, j.data.size(), hipMemcpyHostToDevice, s));
   if (j.mp_split)
-    HIPCHK(hipMemcpyAsync(dev + j.off[TL_MPBASE], p_base, j.mp_base.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dev + j.off[TL_MP] + j.tiles * 4, p_base, j.mp_base.size() * 4, hipMemcpyHostToDevice, s));
   const TriArgs a = tri_args(c, j, dev);
   const int grid = tri_grid(c, j);
   HIPCHK(hipEventRecord(w.ev[0], s));
@@ -1348,7 +1342,8 @@ int plan_build(dpx_plan* P) {
   HIPCHK(hipMemcpy(dev + j.off[TL_INST], j.inst.data(), sizeof(TriInst) * j.count, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dev + j.off[TL_SEEDS], j.seeds.data(), j.seeds.size() * 8, hipMemcpyHostToDevice));
   if (!j.data.empty()) HIPCHK(hipMemcpy(dev + j.off[TL_DATA], j.data.data(), j.data.size(), hipMemcpyHostToDevice));
-  if (j.mp_split) HIPCHK(hipMemcpy(dev + j.off[TL_MPBASE], j.mp_base.data(), j.mp_base.size() * 4, hipMemcpyHostToDevice));
+  if (j.mp_split)
+    HIPCHK(hipMemcpy(dev + j.off[TL_MP] + j.tiles * 4, j.mp_base.data(), j.mp_base.size() * 4, hipMemcpyHostToDevice));
   const TriArgs a = tri_args(c, j, dev);
   const int grid = tri_grid(c, j);
   hipStream_t s = P->ws.stream;

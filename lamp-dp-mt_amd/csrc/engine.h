@@ -57,9 +57,9 @@ struct TriArgs {
   int32_t total_tiles;
   int32_t total_items;         // queue slots: tiles, plus one more per extra part of a split middle product
   int32_t mp_split;            // 1: the middle products of long tiles are split into part tasks (mp_parts)
-  int32_t* mp_done;            // per tile: part tasks finished
-  const int32_t* mp_base;      // per tile: first partial slot of its part tasks (split tiles)
-  int32_t* mp_part;            // partial results, T*T int32 per slot
+  int32_t* mp;                 // split state (mp_split): per tile the part tasks finished [total_tiles], per
+                               // tile its first partial slot [total_tiles], then from mp_part_offset the
+                               // partial results, T*T int32 per slot
   unsigned long long* queue;   // total_items slots, ~0 = empty
   int32_t* ctrl;               // [0] next slot to claim, [1] next slot to fill
   int32_t* dep;                // completed predecessors per tile
@@ -87,6 +87,8 @@ __host__ __device__ inline unsigned long long pack_item(int inst, int I, int J, 
   return ((unsigned long long)(unsigned)inst << 32) | ((unsigned long long)(unsigned)((part << 12) | I) << 16) |
          (unsigned)J;
 }
+// int32 index of the first partial result in TriArgs::mp
+__host__ __device__ inline int64_t mp_part_offset(int64_t tiles) { return (2 * tiles + 63) & ~(int64_t)63; }
 // Part tasks of the middle product of a tile J - I blocks off the diagonal
 // when a launch splits them: 1 (not split) below 8 split blocks, else one per
 // 4 blocks, at most 8.

@@ -111,32 +111,39 @@ __device__ void middle_product(const TriArgs& a, const TriInst& in, int NTv, int
       }
   }
   // operand tiles of block K travel in registers: those of the next block
-  // are loaded while this block's products are computed
-  int4 vl[4], vr[4];
+  // are loaded while this block's products are computed.  Named registers,
+  // not arrays: an array indexed in a loop may be left in scratch memory.
+  int4 vl0, vl1, vl2, vl3, vr0, vr1, vr2, vr3;
   auto load = [&](int K) {
-    const int* gL = rec_ptr(a, in, NTv, I, K) + LT * T * T;
-    const int* gR = rec_ptr(a, in, NTv, K, J) + RT * T * T;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = (q * 64 + lane) * 4;
-      vl[q] = *(const int4*)(gL + e);
-      vr[q] = *(const int4*)(gR + e);
-    }
+    const int4* gL = (const int4*)(rec_ptr(a, in, NTv, I, K) + LT * T * T) + lane;
+    const int4* gR = (const int4*)(rec_ptr(a, in, NTv, K, J) + RT * T * T) + lane;
+    vl0 = gL[0];
+    vr0 = gR[0];
+    vl1 = gL[64];
+    vr1 = gR[64];
+    vl2 = gL[128];
+    vr2 = gR[128];
+    vl3 = gL[192];
+    vr3 = gR[192];
+  };
+  // element e = (q * 64 + lane) * 4 of both operands into LDS
+  auto stage_q = [&](int q, int4 l, int4 rv) {
+    const int e = (q * 64 + lane) * 4;
+    const int r = e / T, k = e % T, rs = r ^ mp_swz(k);  // k % 4 == 0: one swizzle for k..k+3
+    sL[(k + 0) * T + rs] = l.x;
+    sL[(k + 1) * T + rs] = l.y;
+    sL[(k + 2) * T + rs] = l.z;
+    sL[(k + 3) * T + rs] = l.w;
+    *(int4*)(sR + e) = rv;
   };
   // blocks K = I + 1 + part + P m; wave w takes every NWAVE-th of them
   const int K0 = I + 1 + part + P * w, KS = P * NWAVE;
   if (K0 < J) load(K0);
   for (int K = K0; K < J; K += KS) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = (q * 64 + lane) * 4;
-      const int r = e / T, k = e % T, rs = r ^ mp_swz(k);  // k % 4 == 0: one swizzle for k..k+3
-      sL[(k + 0) * T + rs] = vl[q].x;
-      sL[(k + 1) * T + rs] = vl[q].y;
-      sL[(k + 2) * T + rs] = vl[q].z;
-      sL[(k + 3) * T + rs] = vl[q].w;
-      *(int4*)(sR + e) = vr[q];
-    }
+    stage_q(0, vl0, vr0);
+    stage_q(1, vl1, vr1);
+    stage_q(2, vl2, vr2);
+    stage_q(3, vl3, vr3);
     if (K + KS < J) load(K + KS);
     if (CHAIN && lane < T) {
       const int kk = K * T + lane;
@@ -166,7 +173,7 @@ __device__ void middle_product(const TriArgs& a, const TriInst& in, int NTv, int
 // to MP::NEUTRAL; ends with a workgroup barrier.  PART: the blocks
 // K = I + 1 + part + P m only (part task `part` of P).  Otherwise P == 1:
 // every split block here; P > 1: the P part tasks' partial results, from
-// a.mp_part.  min/max is associative and the candidates the threshold drops
+// a.mp.  min/max is associative and the candidates the threshold drops
 // (Zuker-like MP::THR) are dropped in either order, so every way gives the
 // same X.
 template <class MP, bool PART>
@@ -180,7 +187,7 @@ __device__ void middle_X(const TriArgs& a, const TriInst& in, int I, int J, int*
   if (!PART && P > 1) {
     __syncthreads();  // X's initial values are in place
     const int g = (int)(in.tile0 + tile_index(in.nb, I, J));
-    const int32_t* src = a.mp_part + (size_t)a.mp_base[g] * (T * T);
+    const int32_t* src = a.mp + mp_part_offset(a.total_tiles) + (int64_t)a.mp[a.total_tiles + g] * (T * T);
     for (int e = tid; e < T * T; e += NTHR) {
       int v = X[e / T][e % T];
       for (int p = 0; p < P; ++p) {
@@ -251,7 +258,7 @@ __device__ void mp_part_task(const TriArgs& a, const TriInst& in, int I, int J, 
   for (int e = threadIdx.x; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = G::MP::NEUTRAL;
   middle_X<typename G::MP, true>(a, in, I, J, sm.u.stage, sm.X, part, P);
   const int g = (int)(in.tile0 + tile_index(in.nb, I, J));
-  int32_t* dst = a.mp_part + (size_t)(a.mp_base[g] + part) * (T * T);
+  int32_t* dst = a.mp + mp_part_offset(a.total_tiles) + (int64_t)(a.mp[a.total_tiles + g] + part) * (T * T);
   for (int e = threadIdx.x; e < T * T; e += NTHR) dst[e] = sm.X[e / T][e % T];
 }
 
@@ -287,7 +294,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(G::WAVES))
         if (threadIdx.x == 0) {
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
           const int g = (int)(in.tile0 + tile_index(in.nb, I, J));
-          const int old = __hip_atomic_fetch_add(&a.mp_done[g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const int old = __hip_atomic_fetch_add(&a.mp[g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           s_fin = old == P - 1;
           // the other parts' partials, written back before their counts
           if (old == P - 1) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -355,7 +362,7 @@ __global__ void tri_reset(TriArgs a, const unsigned long long* seeds, int nseed)
   if (t < a.total_items) a.queue[t] = t < nseed ? seeds[t] : ~0ull;
   if (t < a.total_tiles) {
     a.dep[t] = 0;
-    if (a.mp_split) a.mp_done[t] = 0;
+    if (a.mp_split) a.mp[t] = 0;  // part tasks finished
   }
   if (t == 0) { a.ctrl[0] = 0; a.ctrl[1] = nseed; a.ctrl[2] = 0; }
 }
@@ -371,6 +378,17 @@ __device__ __forceinline__ int group8_reduce(int v) {
   v = MAXP ? imax(v, b) : imin(v, b);
   const int c = __builtin_amdgcn_update_dpp(v, v, 0x104, 0xf, 0xf, false);  // row_shl:4
   return MAXP ? imax(v, c) : imin(v, c);
+}
+
+// min over the 32 lanes of a half wave, valid on its lanes 16..31: quad
+// permutations and row rotations give every lane its row's minimum, then
+// rows 1 and 3 take the last lane of rows 0 and 2 (row_bcast:15).
+__device__ __forceinline__ int half_wave_min(int v) {
+  v = imin(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
+  v = imin(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
+  v = imin(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xf, 0xf, false));  // row_ror:4
+  v = imin(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xf, 0xf, false));  // row_ror:8
+  return imin(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));  // row_bcast:15 into rows 1, 3
 }
 
 // ============================================================== Nussinov
@@ -765,8 +783,8 @@ struct GZu {
   static constexpr int kMask = 1 << 28;  // masked loop energy: candidates through it stay >= INV_THR
 
   // Rules 2, 3 and 4 of listed cell (i,j) (tile coordinates r, c): lane ap of
-  // the cell's 32 lanes evaluates its share, the result is the minimum over
-  // the 32 lanes.  Lane ap < 30 (left size a = ap+1): the generic interior
+  // the cell's 32 lanes (a half wave) evaluates its share, the result is the
+  // minimum over the 32 lanes, returned on lanes ap >= 16.  Lane ap < 30 (left size a = ap+1): the generic interior
   // loops (a, b = 2..30), the loops (a,1) and (1,a) for a >= 2, and the left
   // and right bulges of size a.  Lane 30: (1,1) and (2,2); lane 31: (2,3) and
   // (3,2).  B(l,k) = cl(l,k) + interior mismatch of (l,k), so
@@ -843,8 +861,7 @@ struct GZu {
         best = imin(best, B32 - E.mmI[icb][Ck][Rl1] + E.mm23I[icb][Ck][Rl1] + out23);
       }
     }
-    for (int m = 16; m >= 1; m >>= 1) best = imin(best, __shfl_xor(best, m));
-    return best;
+    return half_wave_min(best);  // valid on lanes ap >= 16
   }
 
   struct MP {
@@ -873,40 +890,74 @@ struct GZu {
       if (I2 >= nb || J2 >= nb) return INV;
       return rec_ptr(a, in, NT_ZU, I2, J2)[tab * T * T + (i % T) * T + (j % T)];
     };
-    {
-      // B quadrants: tiles (I,J-1) | current tile / (I+1,J-1) | (I+1,J); absent tiles are INV
-      const int* qb[3] = {J - 1 >= I ? rec_ptr(a, in, NT_ZU, I, J - 1) + ZT_B * T * T : nullptr,
-                          I + 1 <= J - 1 ? rec_ptr(a, in, NT_ZU, I + 1, J - 1) + ZT_B * T * T : nullptr,
-                          I + 1 <= J ? rec_ptr(a, in, NT_ZU, I + 1, J) + ZT_B * T * T : nullptr};
-      for (int quad = 0; quad < 3; ++quad) {
-        const int x = tid * 4, rr = x / T + (quad ? T : 0), cc = x % T + (quad == 2 ? T : 0);
-        const int4 v = qb[quad] ? *(const int4*)(qb[quad] + x) : make_int4(INV, INV, INV, INV);
-        int* dst = &sm.u.w.B[rr][cc];
-        dst[0] = v.x;
-        dst[1] = v.y;
-        dst[2] = v.z;
-        dst[3] = v.w;
-      }
-      for (int e = tid; e < T * T; e += NTHR) sm.u.w.B[e / T][T + e % T] = INV;
+    // Every global read of the frontier is issued before the first LDS
+    // write, so the workgroup waits for one memory latency, not one per loop.
+    // B quadrants: tiles (I,J-1) | current tile / (I+1,J-1) | (I+1,J); absent tiles are INV
+    const int* qb[3] = {J - 1 >= I ? rec_ptr(a, in, NT_ZU, I, J - 1) + ZT_B * T * T : nullptr,
+                        I + 1 <= J - 1 ? rec_ptr(a, in, NT_ZU, I + 1, J - 1) + ZT_B * T * T : nullptr,
+                        I + 1 <= J ? rec_ptr(a, in, NT_ZU, I + 1, J) + ZT_B * T * T : nullptr};
+    int4 qv[3];
+#pragma unroll
+    for (int quad = 0; quad < 3; ++quad)
+      qv[quad] = qb[quad] ? *(const int4*)(qb[quad] + tid * 4) : make_int4(INV, INV, INV, INV);
+    const int4 rm = I < J ? *(const int4*)(rec_ptr(a, in, NT_ZU, J, J) + ZT_ML1 * T * T + tid * 4)
+                          : make_int4(INV, INV, INV, INV);  // ml1 tile (J,J)
+    // one boundary cell per thread: [0,32) U and R11 at column JT-1; [32,100)
+    // ML rows IT+32, IT+33; [100,164) ML columns JT-2, JT-1; [164,196) ML1 row IT+32
+    int bv0 = INV, bv1 = INV;
+    if (tid < 32) {
+      bv0 = fetch(ZT_U, IT + tid, JT - 1);
+      bv1 = fetch(ZT_R11, IT + tid, JT - 1);
+    } else if (tid < 100) {
+      const int t = tid - 32;
+      bv0 = fetch(ZT_ML, IT + T + t / (T + 2), JT - 2 + t % (T + 2));
+    } else if (tid < 164) {
+      const int t = tid - 100;
+      bv0 = fetch(ZT_ML, IT + (t >> 1), JT - 2 + (t & 1));
+    } else if (tid < 196) {
+      bv0 = fetch(ZT_ML1, IT + T, JT + tid - 164);
     }
+    // the current tile's cells start empty
+    for (int e = tid; e < T * T; e += NTHR) sm.u.w.B[e / T][T + e % T] = INV;
     for (int e = tid; e < T * (T + 1); e += NTHR) {
       const int rr = e / (T + 1), cc = e % (T + 1);
-      sm.u.w.U[rr][cc] = cc == 0 ? fetch(ZT_U, IT + rr, JT - 1) : INV;
-      sm.u.w.R11[rr][cc] = cc == 0 ? fetch(ZT_R11, IT + rr, JT - 1) : INV;
+      if (cc != 0) sm.u.w.U[rr][cc] = sm.u.w.R11[rr][cc] = INV;
     }
     for (int e = tid; e < (T + 2) * (T + 2); e += NTHR) {
       const int rr = e / (T + 2), cc = e % (T + 2);
-      sm.u.w.ML[rr][cc] = (rr >= T || cc < 2) ? fetch(ZT_ML, IT + rr, JT - 2 + cc) : INV;
+      if (rr < T && cc >= 2) sm.u.w.ML[rr][cc] = INV;
     }
-    for (int e = tid; e < (T + 1) * T; e += NTHR) {
-      const int rr = e / T, cc = e % T;
-      sm.u.w.ML1[rr][cc] = rr == T ? fetch(ZT_ML1, IT + T, JT + cc) : INV;
+    for (int e = tid; e < T * T; e += NTHR) sm.u.w.ML1[e / T][e % T] = INV;
+    // then the values read above
+#pragma unroll
+    for (int quad = 0; quad < 3; ++quad) {
+      const int x = tid * 4, rr = x / T + (quad ? T : 0), cc = x % T + (quad == 2 ? T : 0);
+      int* dst = &sm.u.w.B[rr][cc];
+      dst[0] = qv[quad].x;
+      dst[1] = qv[quad].y;
+      dst[2] = qv[quad].z;
+      dst[3] = qv[quad].w;
+    }
+    if (I < J) {
+      int* dst = &sm.u.w.RM1[(tid * 4) / T][(tid * 4) % T];
+      dst[0] = rm.x;
+      dst[1] = rm.y;
+      dst[2] = rm.z;
+      dst[3] = rm.w;
+    }
+    if (tid < 32) {
+      sm.u.w.U[tid][0] = bv0;
+      sm.u.w.R11[tid][0] = bv1;
+    } else if (tid < 100) {
+      const int t = tid - 32;
+      sm.u.w.ML[T + t / (T + 2)][t % (T + 2)] = bv0;
+    } else if (tid < 164) {
+      const int t = tid - 100;
+      sm.u.w.ML[t >> 1][t & 1] = bv0;
+    } else if (tid < 196) {
+      sm.u.w.ML1[T][tid - 164] = bv0;
     }
     const int* gLU = rec_ptr(a, in, NT_ZU, I, I) + ZT_U * T * T;  // U tile (I,I), read in place
-    if (I < J) {
-      const int* gR = rec_ptr(a, in, NT_ZU, J, J) + ZT_ML1 * T * T;
-      for (int e = tid; e < T * T; e += NTHR) sm.u.w.RM1[e / T][e % T] = gR[e];
-    }
     const int dlo = I == J ? 0 : -(T - 1);
     __syncthreads();  // sequence segments visible to prep_band
     if (tid >= 64) prep_band(z, IT, JT, dlo, sm, tid - 64, 0);
@@ -943,7 +994,7 @@ struct GZu {
           const int qp = sm.list[v], q = qp / T, p = qp % T, ld = ld0 + q;
           const int r = (ld < 0 ? -ld : 0) + p, c = r + ld, i = IT + r, j = JT + c;
           const int best = window_lane(z, sm, sm.pre[buf][q][p], ap, i, j, r, c, fg, f_a1, f_1a);
-          if (ap == 0) sm.acc[q][p] = best;
+          if (ap == 31) sm.acc[q][p] = best;
         } else {
           const int e = (w - wl) >> 1, part = w & 1;
           int q = 0;
@@ -954,29 +1005,26 @@ struct GZu {
             if (I == J) {
               best = lds_minplus8(&sm.u.w.U[r][1], &sm.u.w.ML1[0][c], T + 1);
             } else {
-              // U(I,I) row r from HBM/L2, eight values ahead of their use
+              // U(I,I) row r from HBM/L2: the whole row is requested at
+              // once, so the lane waits for one memory latency, not four
               const int4* g4 = (const int4*)(gLU + r * T);
-              int4 ua = g4[0], ub = g4[1];
+              int4 u4[T / 4];
+#pragma unroll
+              for (int x4 = 0; x4 < T / 4; ++x4) u4[x4] = g4[x4];
 #pragma unroll
               for (int x0 = 0; x0 < T; x0 += 8) {
-                int4 na = ua, nb = ub;
-                if (x0 + 8 < T) {
-                  na = g4[(x0 + 8) / 4];
-                  nb = g4[(x0 + 8) / 4 + 1];
-                }
                 int bv[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) bv[u] = sm.u.w.ML1[x0 + u][c];
+                const int4 ua = u4[x0 / 4], ub = u4[x0 / 4 + 1];
                 best = imin(best, imin(imin(ua.x + bv[0], ua.y + bv[1]), imin(ua.z + bv[2], ua.w + bv[3])));
                 best = imin(best, imin(imin(ub.x + bv[4], ub.y + bv[5]), imin(ub.z + bv[6], ub.w + bv[7])));
-                ua = na;
-                ub = nb;
               }
             }
           } else if (I < J) {
             best = lds_minplus8(&sm.u.w.U[r][1], &sm.u.w.RM1[0][c], T + 1);
           }
-          best = imin(best, __shfl_xor(best, 1));
+          best = imin(best, __builtin_amdgcn_update_dpp(best, best, 0xB1, 0xf, 0xf, false));  // the pair's other lane
           if (part == 0 && best < INV_THR) sm.X[r][c] = imin(sm.X[r][c], best);
         }
       }
@@ -1178,32 +1226,51 @@ struct GRf {
       if (I2 >= nb || J2 >= nb) return INV;
       return rec_ptr(a, in, NT_RF, I2, J2)[tab * T * T + (i % T) * T + (j % T)];
     };
-    {
-      const int* qb[3] = {J - 1 >= I ? rec_ptr(a, in, NT_RF, I, J - 1) + RT_B * T * T : nullptr,
-                          I + 1 <= J - 1 ? rec_ptr(a, in, NT_RF, I + 1, J - 1) + RT_B * T * T : nullptr,
-                          I + 1 <= J ? rec_ptr(a, in, NT_RF, I + 1, J) + RT_B * T * T : nullptr};
-      for (int quad = 0; quad < 3; ++quad) {
-        const int x = tid * 4, rr = x / T + (quad ? T : 0), cc = x % T + (quad == 2 ? T : 0);
-        const int4 v = qb[quad] ? *(const int4*)(qb[quad] + x) : make_int4(INV, INV, INV, INV);
-        int* dst = &sm.u.w.B[rr][cc];
-        dst[0] = v.x;
-        dst[1] = v.y;
-        dst[2] = v.z;
-        dst[3] = v.w;
-      }
-      for (int e = tid; e < T * T; e += NTHR) sm.u.w.B[e / T][T + e % T] = INV;
+    // every global read of the frontier before the first LDS write (one
+    // memory latency): B quadrants as in the Zuker tile, Q of tile (J,J),
+    // and one halo cell of Q and QM per thread below 65 (row IT+32, then
+    // column JT-1)
+    const int* qb[3] = {J - 1 >= I ? rec_ptr(a, in, NT_RF, I, J - 1) + RT_B * T * T : nullptr,
+                        I + 1 <= J - 1 ? rec_ptr(a, in, NT_RF, I + 1, J - 1) + RT_B * T * T : nullptr,
+                        I + 1 <= J ? rec_ptr(a, in, NT_RF, I + 1, J) + RT_B * T * T : nullptr};
+    int4 qv[3];
+#pragma unroll
+    for (int quad = 0; quad < 3; ++quad)
+      qv[quad] = qb[quad] ? *(const int4*)(qb[quad] + tid * 4) : make_int4(INV, INV, INV, INV);
+    const int4 rq = I < J ? *(const int4*)(rec_ptr(a, in, NT_RF, J, J) + RT_Q * T * T + tid * 4)
+                          : make_int4(INV, INV, INV, INV);
+    const int hr = tid < T + 1 ? T : tid - (T + 1), hc = tid < T + 1 ? tid : 0;
+    int hq = INV, hm = INV;
+    if (tid < 2 * T + 1) {
+      hq = fetch(RT_Q, IT + hr, JT - 1 + hc);
+      hm = fetch(RT_QM, IT + hr, JT - 1 + hc);
     }
+    for (int e = tid; e < T * T; e += NTHR) sm.u.w.B[e / T][T + e % T] = INV;
     for (int e = tid; e < (T + 1) * (T + 1); e += NTHR) {
       const int rr = e / (T + 1), cc = e % (T + 1);
-      const bool halo = rr == T || cc == 0;
-      sm.u.w.Q[rr][cc] = halo ? fetch(RT_Q, IT + rr, JT - 1 + cc) : INV;
-      sm.u.w.QM[rr][cc] = halo ? fetch(RT_QM, IT + rr, JT - 1 + cc) : INV;
+      if (rr != T && cc != 0) sm.u.w.Q[rr][cc] = sm.u.w.QM[rr][cc] = INV;
+    }
+#pragma unroll
+    for (int quad = 0; quad < 3; ++quad) {
+      const int x = tid * 4, rr = x / T + (quad ? T : 0), cc = x % T + (quad == 2 ? T : 0);
+      int* dst = &sm.u.w.B[rr][cc];
+      dst[0] = qv[quad].x;
+      dst[1] = qv[quad].y;
+      dst[2] = qv[quad].z;
+      dst[3] = qv[quad].w;
+    }
+    if (I < J) {
+      int* dst = &sm.u.w.RQ[(tid * 4) / T][(tid * 4) % T];
+      dst[0] = rq.x;
+      dst[1] = rq.y;
+      dst[2] = rq.z;
+      dst[3] = rq.w;
+    }
+    if (tid < 2 * T + 1) {
+      sm.u.w.Q[hr][hc] = hq;
+      sm.u.w.QM[hr][hc] = hm;
     }
     const int* gLQ = rec_ptr(a, in, NT_RF, I, I) + RT_Q * T * T;  // Q tile (I,I), read in place
-    if (I < J) {
-      const int* gR = rec_ptr(a, in, NT_RF, J, J) + RT_Q * T * T;
-      for (int e = tid; e < T * T; e += NTHR) sm.u.w.RQ[e / T][e % T] = gR[e];
-    }
     const int dlo = I == J ? 0 : -(T - 1);
     __syncthreads();
     if (tid >= 64) prep_band(z, IT, JT, dlo, sm, tid - 64, 0);
@@ -1234,7 +1301,7 @@ struct GRf {
           const int r = (ld < 0 ? -ld : 0) + p, c = r + ld, i = IT + r, j = JT + c;
           // the Zuker loop window of the cell (i-1, j+1) with sizes <= 29
           const int best = GZu::window_lane(z, sm, sm.pre[buf][q][p], ap, i - 1, j + 1, r - 1, c + 1, fg, f_a1, f_1a, 29);
-          if (ap == 0) sm.acc[q][p] = best;
+          if (ap == 31) sm.acc[q][p] = best;
         } else {
           // QM split points inside blocks I and J (Q(i,k) and Q(k,j) are
           // empty for j-k < 3 and k-i < 3, so all 32 k of a block are taken)
@@ -1248,10 +1315,13 @@ struct GRf {
 #pragma unroll 8
               for (int x = 0; x < T; ++x) best = imin(best, sm.u.w.Q[r][x + 1] + sm.u.w.Q[x][c + 1]);
             } else {
-              const int4* g4 = (const int4*)(gLQ + r * T);
-#pragma unroll 2
+              const int4* g4 = (const int4*)(gLQ + r * T);  // the whole row requested at once
+              int4 u4[T / 4];
+#pragma unroll
+              for (int x4 = 0; x4 < T / 4; ++x4) u4[x4] = g4[x4];
+#pragma unroll
               for (int x4 = 0; x4 < T / 4; ++x4) {
-                const int4 u = g4[x4];
+                const int4 u = u4[x4];
                 const int x = x4 * 4;
                 best = imin(imin(best, u.x + sm.u.w.Q[x][c + 1]), u.y + sm.u.w.Q[x + 1][c + 1]);
                 best = imin(imin(best, u.z + sm.u.w.Q[x + 2][c + 1]), u.w + sm.u.w.Q[x + 3][c + 1]);
@@ -1261,7 +1331,7 @@ struct GRf {
 #pragma unroll 8
             for (int x = 0; x < T; ++x) best = imin(best, sm.u.w.Q[r][x + 1] + sm.u.w.RQ[x][c]);
           }
-          best = imin(best, __shfl_xor(best, 1));
+          best = imin(best, __builtin_amdgcn_update_dpp(best, best, 0xB1, 0xf, 0xf, false));  // the pair's other lane
           if (part == 0 && best < INV_THR) sm.X[r][c] = imin(sm.X[r][c], best);
         }
       }
