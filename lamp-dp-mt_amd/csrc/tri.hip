@@ -402,15 +402,20 @@ __device__ __forceinline__ bool nu_pair(int x, int y) {
 
 struct GNu {
   static constexpr int WAVES = 4;  // waves per SIMD the register budget allows
+  static constexpr int W = T + 4;  // row stride: rows start 16-byte aligned
   template <class S> __device__ static void init(const TriArgs&, S&) {}
-  struct Smem {
+  struct alignas(16) Smem {
     union {
       int stage[NWAVE * MP_STAGE];
       struct {
-        int S[T + 1][T + 2];  // origin (IT, JT-1)
-        int L[T][T + 1];      // tile (I,I) [r][k]
-        int R[T][T + 1];      // tile (J,J) [k][c]
-        int PB[T][T + 1];     // pair rule: 1 if (i, j-1) pairs and the loop is long enough, else NEG
+        // S: origin (IT, JT-4), so cell (IT+r, JT+c) is S[r][c+4] and a run
+        // of 4 columns starting at a multiple of 4 is one 128-bit read; row
+        // T and column 3 hold the frontier (row IT+32, column JT-1)
+        int S[T + 1][W];
+        int ST[T][W];  // the current tile transposed: ST[c][r] = S[r][c+4]
+        int L[T][W];   // tile (I,I) [r][k]
+        int RT[T][W];  // tile (J,J) transposed [c][k]
+        int PB[T][T + 1];  // pair rule: 1 if (i, j-1) pairs and the loop is long enough, else NEG
       } w;
     } u;
     int X[T][T + 1];
@@ -426,24 +431,22 @@ struct GNu {
     const uint8_t* s = a.codes + in.seq_off;
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = NEG;
     middle_X<MP, false>(a, in, I, J, sm.u.stage, sm.X, part, P);
-    // stage the frontier
+    // stage the frontier; the current tile's cells start as NEG
     for (int e = tid; e < (T + 1) * (T + 1); e += NTHR) {
-      const int rr = e / (T + 1), cc = e % (T + 1);
-      const int i = IT + rr, j = JT - 1 + cc;
+      const int rr = e / (T + 1), cc = e % (T + 1) + 3;
+      const int i = IT + rr, j = JT - 4 + cc;
       int v = NEG;
-      const bool cur = rr < T && cc >= 1;
-      if (!cur && i <= j && j <= n && j >= 0) {
-        const int I2 = i / T, J2 = j / T;
-        v = rec_ptr(a, in, NT_NU, I2, J2)[(i % T) * T + (j % T)];
-      }
+      const bool cur = rr < T && cc >= 4;
+      if (!cur && i <= j && j <= n && j >= 0) v = rec_ptr(a, in, NT_NU, i / T, j / T)[(i % T) * T + (j % T)];
       sm.u.w.S[rr][cc] = v;
     }
+    for (int e = tid; e < T * T; e += NTHR) sm.u.w.ST[e / T][e % T] = NEG;
     if (I < J) {
       const int* gL = rec_ptr(a, in, NT_NU, I, I);
       const int* gR = rec_ptr(a, in, NT_NU, J, J);
       for (int e = tid; e < T * T; e += NTHR) {
         sm.u.w.L[e / T][e % T] = gL[e];
-        sm.u.w.R[e / T][e % T] = gR[e];
+        sm.u.w.RT[e % T][e / T] = gR[e];
       }
     }
     // the sequence-dependent rule of every cell, before the wavefront
@@ -456,31 +459,30 @@ struct GNu {
     for (int ld = dlo; ld <= T - 1; ++ld) {
       const int rmin = ld < 0 ? -ld : 0, ncell = T - (ld < 0 ? -ld : ld);
       // split points inside blocks I and J, 8 lanes per cell (a group of 8
-      // in one wave), each lane a strided share of both ranges; every load
-      // of the step, the cell's own terms included, is issued before the
-      // first is used: one LDS round trip, then a DPP max over the group
+      // in one wave), lane `part` taking k = 4 part .. 4 part + 3 of each
+      // block: its operands are four 128-bit reads (rows of L, S and the
+      // transposed ST, RT), issued with the cell's own terms before any is
+      // used; out-of-range candidates are masked; then a DPP max over the group
       if (tid < ncell * 8) {
-        const int p = tid >> 3, part = tid & 7, r = rmin + p, c = r + ld, i = IT + r, j = JT + c;
+        const int p = tid >> 3, part = tid & 7, r = rmin + p, c = r + ld, i = IT + r, j = JT + c, k0 = 4 * part;
         // (i+1,j), (i,j-1), (i+1,j-1); cells below the diagonal hold NEG
         const int xv = sm.X[r][c], pb = sm.u.w.PB[r][c];
-        const int dn = sm.u.w.S[r + 1][c + 1], lf = sm.u.w.S[r][c], dg = sm.u.w.S[r + 1][c];
-        // lane `part` takes kk = part + 8m (m < 4) of each range, with
-        // fixed trip counts: every operand address is inside the tile
-        // arrays, out-of-range candidates are masked after the loads
+        const int dn = sm.u.w.S[r + 1][c + 4], lf = sm.u.w.S[r][c + 3], dg = sm.u.w.S[r + 1][c + 3];
+        const int4 sr = *(const int4*)&sm.u.w.S[r][k0 + 4];  // s(i, JT+k0+m)
+        const int4 sc = *(const int4*)&sm.u.w.ST[c][k0];     // s(IT+k0+m, j)
         int best = NEG;
-        if (I == J) {
+        if (I == J) {  // k = IT + kk, i < k < j
+          const int x[4] = {sr.x + sc.x, sr.y + sc.y, sr.z + sc.z, sr.w + sc.w};
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {  // k = IT + kk, i < k < j
-            const int kk = part + 8 * m, x = sm.u.w.S[r][kk + 1] + sm.u.w.S[kk][c + 1];
-            best = imax(best, kk > r && kk < c ? x : NEG);
-          }
-        } else {
+          for (int m = 0; m < 4; ++m) best = imax(best, k0 + m > r && k0 + m < c ? x[m] : NEG);
+        } else {  // k = IT + kk > i with s(i,k) from L, and k = JT + kk < j with s(k,j) from RT
+          const int4 lv = *(const int4*)&sm.u.w.L[r][k0];
+          const int4 rv = *(const int4*)&sm.u.w.RT[c][k0];
+          const int x1[4] = {lv.x + sc.x, lv.y + sc.y, lv.z + sc.z, lv.w + sc.w};
+          const int x2[4] = {sr.x + rv.x, sr.y + rv.y, sr.z + rv.z, sr.w + rv.w};
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {  // k = IT + kk > i, and k = JT + kk < j
-            const int kk = part + 8 * m;
-            const int x1 = sm.u.w.L[r][kk] + sm.u.w.S[kk][c + 1], x2 = sm.u.w.S[r][kk + 1] + sm.u.w.R[kk][c];
-            best = imax(best, imax(kk > r ? x1 : NEG, kk < c ? x2 : NEG));
-          }
+          for (int m = 0; m < 4; ++m)
+            best = imax(best, imax(k0 + m > r ? x1[m] : NEG, k0 + m < c ? x2[m] : NEG));
         }
         if (!(i < j && j <= n)) best = NEG;
         best = group8_reduce<true>(best);
@@ -495,13 +497,14 @@ struct GNu {
             if (i == j) v = imax(v, 0);
             v = imax(v, dg + pb);
           }
-          sm.u.w.S[r][c + 1] = v;
+          sm.u.w.S[r][c + 4] = v;
+          sm.u.w.ST[c][r] = v;
         }
       }
       __syncthreads();
     }
     int* g = rec_ptr(a, in, NT_NU, I, J);
-    for (int e = tid; e < T * T; e += NTHR) g[e] = sm.u.w.S[e / T][e % T + 1];
+    for (int e = tid; e < T * T; e += NTHR) g[e] = sm.u.w.S[e / T][e % T + 4];
   }
 };
 
