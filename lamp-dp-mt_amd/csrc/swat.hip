@@ -319,48 +319,74 @@ __global__ void sw_pick(SwArgs A, int row0_best, int row0_bestj, int row0_axiom,
 __global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mrule, const uint8_t* row0_g2rule,
                                                int i_base, SwTrace* tr, int cap, int* state) {
   constexpr int NW = 64;
-  __shared__ uint32_t win[NW][64];
+  __shared__ uint32_t win[NW][64];  // words w0 .. w0+NW-1 of band wb, [word][row]
   const int lane = threadIdx.x;
   int wb = -1, w0 = 0;
-  int i = state[0], j = state[1], tab = state[2], n = state[3];
-  bool ok = state[5] != 0, ended = state[4] != 0;
+  // the walk's state is wave-uniform: scalar registers
+  int i = __builtin_amdgcn_readfirstlane(state[0]), j = __builtin_amdgcn_readfirstlane(state[1]);
+  int tab = __builtin_amdgcn_readfirstlane(state[2]), n = __builtin_amdgcn_readfirstlane(state[3]);
+  bool ok = __builtin_amdgcn_readfirstlane(state[5]) != 0, ended = __builtin_amdgcn_readfirstlane(state[4]) != 0;
+  // rule of a cell from its code nibble, 4 bits per nibble value, for the
+  // tables M (g1 beats the pair: 2, g2 beats both: 3, else 1), g1 (5 open,
+  // 6 extend) and g2 (8 open, 9 extend)
+  constexpr unsigned long long RULE_M = [] {
+    unsigned long long k = 0;
+    for (int x = 0; x < 16; ++x) k |= (unsigned long long)((x & 4) ? 3 : (x & 8) ? 2 : 1) << (4 * x);
+    return k;
+  }();
+  constexpr unsigned long long RULE_G1 = [] {
+    unsigned long long k = 0;
+    for (int x = 0; x < 16; ++x) k |= (unsigned long long)(5 + ((x >> 1) & 1)) << (4 * x);
+    return k;
+  }();
+  constexpr unsigned long long RULE_G2 = [] {
+    unsigned long long k = 0;
+    for (int x = 0; x < 16; ++x) k |= (unsigned long long)(8 + (x & 1)) << (4 * x);
+    return k;
+  }();
+  // the move of each rule, 6 bits per rule: bit 0 i decreases, bit 1 j
+  // decreases, bits 2-3 the next table (0 M, 1 g1, 2 g2), bit 4 the walk ends
+  constexpr unsigned long long MOVES = (0x10ull << 0) | (0x03ull << 6) | (0x04ull << 12) | (0x08ull << 18) |
+                                       (0x10ull << 24) | (0x01ull << 30) | (0x05ull << 36) | (0x10ull << 42) |
+                                       (0x02ull << 48) | (0x0Aull << 54);
   while (ok && !ended) {
     if (i_base > 0 && i <= i_base) break;  // the rest of the walk is in an earlier segment
     int rule;
-    if (tab <= 3) {
-      if (i == 0 && j == 0) rule = 0;
-      else if (i == 0) rule = row0_mrule[j];
-      else rule = -1;
-    } else if (tab <= 6) {
-      rule = i == 0 ? 4 : -1;
+    if (i == 0 || (tab > 6 && j == 0)) {  // the first row and column have fixed rules
+      if (tab <= 3) rule = j == 0 ? 0 : row0_mrule[j];
+      else if (tab <= 6) rule = 4;
+      else rule = j == 0 ? 7 : row0_g2rule[j];
+      rule = __builtin_amdgcn_readfirstlane(rule);
     } else {
-      rule = j == 0 ? 7 : (i == 0 ? row0_g2rule[j] : -1);
-    }
-    if (rule < 0) {
-      const int ir = i - i_base;  // row within the codes of A
-      const int b = (ir - 1) / 64, r = (ir - 1) % 64, t = j + r, w = t >> 3;
+      const int ir = i - i_base - 1;  // row within the codes of A, from 0
+      const int b = ir >> 6, r = ir & 63, t = j + r, w = t >> 3;
       if (b != wb || w < w0 || w >= w0 + NW) {
         wb = b;
         w0 = max(0, w - NW + 1);
-        for (int k = 0; k < NW; ++k)
-          if (w0 + k < A.nwords) win[k][lane] = A.bt[((size_t)b * A.nwords + w0 + k) * 64 + lane];
+        // every load issued before the first store (words past the band's
+        // last are clamped to it and never selected): one memory latency
+        const uint32_t* src = A.bt + (size_t)b * A.nwords * 64 + lane;
+        uint32_t v[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) v[k] = src[(size_t)min(w0 + k, A.nwords - 1) * 64];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) win[k][lane] = v[k];
         __syncthreads();
       }
       // the first step of a word is in its top nibble (see sw_fill)
-      const uint32_t nib = (win[w - w0][r] >> (4 * (7 - (t & 7)))) & 15u;
-      rule = tab <= 3 ? ((nib & 4u) ? 3 : (nib & 8u) ? 2 : 1) : tab <= 6 ? 5 + (int)((nib >> 1) & 1u) : 8 + (int)(nib & 1u);
+      const uint32_t word = __builtin_amdgcn_readfirstlane(win[w - w0][r]);
+      const uint32_t nib = (word >> (4 * (7 - (t & 7)))) & 15u;
+      const unsigned long long rk = tab <= 3 ? RULE_M : tab <= 6 ? RULE_G1 : RULE_G2;
+      rule = (int)(rk >> (4 * nib)) & 15;
     }
     if (n >= cap) { ok = false; break; }
     if (lane == 0) tr[n] = SwTrace{i, j, rule, {0, 0}};
     ++n;
-    if (rule == 1) { --i; --j; tab = 0; }
-    else if (rule == 2) { tab = 4; }
-    else if (rule == 3) { tab = 7; }
-    else if (rule == 5) { --i; tab = 0; }
-    else if (rule == 6) { --i; tab = 4; }
-    else if (rule == 8) { --j; tab = 0; }
-    else if (rule == 9) { --j; tab = 7; }
-    else ended = true;
+    const int mv = (int)(MOVES >> (6 * rule)) & 63;
+    i -= mv & 1;
+    j -= (mv >> 1) & 1;
+    ended = (mv & 16) != 0;
+    tab = ended ? tab : (mv & 12) == 0 ? 0 : (mv & 12) == 4 ? 4 : 7;
   }
   if (lane == 0) {
     state[0] = i;
