@@ -87,6 +87,19 @@ __host__ __device__ inline unsigned long long pack_item(int inst, int I, int J, 
   return ((unsigned long long)(unsigned)inst << 32) | ((unsigned long long)(unsigned)((part << 12) | I) << 16) |
          (unsigned)J;
 }
+// min over the 32 lanes of each half wave, valid on its lanes 16..31: quad
+// permutations and row rotations give every lane its row's minimum, then
+// rows 1 and 3 take the last lane of rows 0 and 2 (row_bcast:15).  All DPP:
+// no LDS round trip.  The half wave's lanes must all be active.
+__device__ __forceinline__ int dpp_half_min(int v) {
+  auto mn = [](int x, int y) { return x < y ? x : y; };
+  v = mn(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
+  v = mn(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
+  v = mn(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xf, 0xf, false));  // row_ror:4
+  v = mn(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xf, 0xf, false));  // row_ror:8
+  return mn(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));  // row_bcast:15 into rows 1, 3
+}
+
 // int32 index of the first partial result in TriArgs::mp
 __host__ __device__ inline int64_t mp_part_offset(int64_t tiles) { return (2 * tiles + 63) & ~(int64_t)63; }
 // Part tasks of the middle product of a tile J - I blocks off the diagonal

@@ -313,17 +313,25 @@ __global__ __launch_bounds__(NT) void zu_finalize(TriArgs a) {
           }
         }
       }
-      for (int off = 16; off >= 1; off >>= 1) m = fmin_(m, __shfl_xor(m, off));
-      if (lane == 0) far[r] = m;
+      m = dpp_half_min(m);
+      if (lane == 31) far[r] = m;
     }
-    for (int e = tid; e < T * T; e += NT) {
-      const int r = e / T, c = e % T, i = lo + r, k = lo + c;
-      int v = INV;
-      if (i <= hi && k <= hi && k >= i + 7) {
-        const int U = C.get(ZT_U, i, k);
-        if (U < INV_THR) v = U - mlint;
+    {
+      // every global read of the block before its LDS writes (one latency)
+      constexpr int NQ = (T * T + NT - 1) / NT;
+      int v[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int e = tid + q * NT, r = e / T, c = e % T, i = lo + r, k = lo + c;
+        v[q] = INV;
+        if (e < T * T && i <= hi && k <= hi && k >= i + 7) {
+          const int U = C.get(ZT_U, i, k);
+          if (U < INV_THR) v[q] = U - mlint;
+        }
       }
-      nd[r][c] = v;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (tid + q * NT < T * T) nd[(tid + q * NT) / T][(tid + q * NT) % T] = v[q];
     }
     __syncthreads();
     if (tid < 64) {
@@ -331,8 +339,8 @@ __global__ __launch_bounds__(NT) void zu_finalize(TriArgs a) {
         const int r = i - lo, k = lo + lane;
         int m = INV;
         if (tid < 32 && k >= i + 7 && k <= hi && nd[r][lane] < INV_THR) m = nd[r][lane] + st[k];
-        for (int off = 16; off >= 1; off >>= 1) m = fmin_(m, __shfl_xor(m, off));
-        if (tid == 0) st[i] = fmin_(fmin_(st[i + 1], far[r]), m);
+        m = dpp_half_min(m);  // on lanes 16..31
+        if (tid == 31) st[i] = fmin_(fmin_(st[i + 1], far[r]), m);
         __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): st[i] visible to the wave's next row
         __builtin_amdgcn_wave_barrier();
       }

@@ -380,16 +380,7 @@ __device__ __forceinline__ int group8_reduce(int v) {
   return MAXP ? imax(v, c) : imin(v, c);
 }
 
-// min over the 32 lanes of a half wave, valid on its lanes 16..31: quad
-// permutations and row rotations give every lane its row's minimum, then
-// rows 1 and 3 take the last lane of rows 0 and 2 (row_bcast:15).
-__device__ __forceinline__ int half_wave_min(int v) {
-  v = imin(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
-  v = imin(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
-  v = imin(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xf, 0xf, false));  // row_ror:4
-  v = imin(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xf, 0xf, false));  // row_ror:8
-  return imin(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));  // row_bcast:15 into rows 1, 3
-}
+__device__ __forceinline__ int half_wave_min(int v) { return dpp_half_min(v); }
 
 // ============================================================== Nussinov
 // s(i,j) = max{0 [i==j]; s(i+1,j); s(i,j-1); s(i+1,j-1)+1 [pair(i,j-1), i+2+m<=j];
