@@ -422,7 +422,24 @@ struct GNu {
     const uint8_t* s = a.codes + in.seq_off;
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = NEG;
     middle_X<MP, false>(a, in, I, J, sm.u.stage, sm.X, part, P);
-    // stage the frontier; the current tile's cells start as NEG
+    // Stage the frontier (row IT+32 and column JT-1; the current tile's
+    // cells start as NEG), tiles (I,I) and (J,J), and the pair rule of
+    // every cell.  The code and tile reads are issued before their LDS
+    // writes, so they cost one memory latency.
+    constexpr int NC = T * T / NTHR;
+    uint8_t si[NC], sj[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const int e = tid + q * NTHR, i = IT + e / T, j = JT + e % T;
+      const bool pl = i + 2 + a.min_loop <= j && j <= n;  // the pair rule's bounds
+      si[q] = pl ? s[i] : 0;
+      sj[q] = pl ? s[j - 1] : 0;
+    }
+    int4 l4 = make_int4(NEG, NEG, NEG, NEG), r4 = l4;
+    if (I < J) {
+      l4 = ((const int4*)rec_ptr(a, in, NT_NU, I, I))[tid];
+      r4 = ((const int4*)rec_ptr(a, in, NT_NU, J, J))[tid];
+    }
     for (int e = tid; e < (T + 1) * (T + 1); e += NTHR) {
       const int rr = e / (T + 1), cc = e % (T + 1) + 3;
       const int i = IT + rr, j = JT - 4 + cc;
@@ -431,19 +448,19 @@ struct GNu {
       if (!cur && i <= j && j <= n && j >= 0) v = rec_ptr(a, in, NT_NU, i / T, j / T)[(i % T) * T + (j % T)];
       sm.u.w.S[rr][cc] = v;
     }
-    for (int e = tid; e < T * T; e += NTHR) sm.u.w.ST[e / T][e % T] = NEG;
-    if (I < J) {
-      const int* gL = rec_ptr(a, in, NT_NU, I, I);
-      const int* gR = rec_ptr(a, in, NT_NU, J, J);
-      for (int e = tid; e < T * T; e += NTHR) {
-        sm.u.w.L[e / T][e % T] = gL[e];
-        sm.u.w.RT[e % T][e / T] = gR[e];
-      }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const int e = tid + q * NTHR, i = IT + e / T, j = JT + e % T;
+      sm.u.w.ST[e / T][e % T] = NEG;
+      sm.u.w.PB[e / T][e % T] = (i + 2 + a.min_loop <= j && j <= n && nu_pair(si[q], sj[q])) ? 1 : NEG;
     }
-    // the sequence-dependent rule of every cell, before the wavefront
-    for (int e = tid; e < T * T; e += NTHR) {
-      const int i = IT + e / T, j = JT + e % T;
-      sm.u.w.PB[e / T][e % T] = (i + 2 + a.min_loop <= j && j <= n && nu_pair(s[i], s[j - 1])) ? 1 : NEG;
+    if (I < J) {
+      const int r = (tid * 4) / T, c = (tid * 4) % T;
+      *(int4*)&sm.u.w.L[r][c] = l4;
+      sm.u.w.RT[c][r] = r4.x;
+      sm.u.w.RT[c + 1][r] = r4.y;
+      sm.u.w.RT[c + 2][r] = r4.z;
+      sm.u.w.RT[c + 3][r] = r4.w;
     }
     __syncthreads();
     const int dlo = I == J ? 0 : -(T - 1);
@@ -531,21 +548,37 @@ struct GMc {
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.X[0][0])[e] = 0x7fffffff;
     middle_X<MP, false>(a, in, I, J, sm.u.stage, sm.X, part, P);
     for (int e = tid; e < T * (T + 1); e += NTHR) (&sm.u.w.S[0][0])[e] = 0;
+    // global reads first, then the LDS writes (one memory latency)
+    constexpr int NC = T * T / NTHR;
+    uint32_t ri[NC], cj[NC];
+    int lv[NC], rv[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const int e = tid + q * NTHR, i = IT + e / T, j = JT + e % T;
+      const bool live = i + 1 < j && j <= n;
+      ri[q] = live ? (uint32_t)rows[i] : 0u;
+      cj[q] = live ? (uint32_t)cols[j - 1] : 0u;
+      lv[q] = I < J ? rec_ptr(a, in, NT_MC, I, I)[e] : 0;
+      rv[q] = I < J ? rec_ptr(a, in, NT_MC, J, J)[e] : 0;
+    }
     if (I < J) {
-      const int* gL = rec_ptr(a, in, NT_MC, I, I);
-      const int* gR = rec_ptr(a, in, NT_MC, J, J);
-      for (int e = tid; e < T * T; e += NTHR) {
-        sm.u.w.L[e / T][e % T] = gL[e];
-        sm.u.w.R[e / T][e % T] = gR[e];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        const int e = tid + q * NTHR;
+        sm.u.w.L[e / T][e % T] = lv[q];
+        sm.u.w.R[e / T][e % T] = rv[q];
       }
     }
+    int rk = 0;
     if (tid < 2 * T) {
       const int k = tid < T ? IT + tid : JT + tid - T;
-      sm.u.w.rows[tid] = k < n ? rows[k] : 0;
+      rk = k < n ? rows[k] : 0;
     }
-    for (int e = tid; e < T * T; e += NTHR) {
-      const int i = IT + e / T, j = JT + e % T;
-      sm.u.w.AL[e / T][e % T] = (i + 1 < j && j <= n) ? (uint32_t)rows[i] * (uint32_t)cols[j - 1] : 0u;
+    if (tid < 2 * T) sm.u.w.rows[tid] = rk;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const int e = tid + q * NTHR;
+      sm.u.w.AL[e / T][e % T] = ri[q] * cj[q];
     }
     __syncthreads();
     const int dlo = I == J ? 0 : -(T - 1);
