@@ -51,39 +51,36 @@ struct Cells {
   }
 };
 
-// Smallest c in [0, ncand) whose candidate value equals target, -1 if none:
-// the 64 lanes of a wave test 64 candidates at a time (wave-uniform result).
-template <class Fn>
-__device__ int wave_first_match(int ncand, int target, Fn&& cand) {
-  const int lane = threadIdx.x & 63;
-  for (int base = 0; base < ncand; base += 64) {
-    const int c = base + lane;
-    int v = 0;
-    const bool hit = c < ncand && cand(c, &v) && v == target;
-    const unsigned long long m = __ballot(hit);
-    if (m) return base + (int)__ffsll((unsigned long long)m) - 1;
-  }
-  return -1;
-}
-
 // BFS backtrace driver.  The reference takes its queued records one at a
 // time (zu.c:49-93, CodeGen.scala:233-252); a record's rule depends only on
-// its own cell, so each round here resolves up to NW queued records, one
-// per wave, and then appends their children in record order: the same
-// records, in the same order.  rule_of(rec, &rule, &p0, &p1) runs on one
-// wave and returns false when no candidate matches; kids(rec, push) appends
-// the children of a resolved record.  *len receives the record count, or -1
-// (no match, or more records than cap).
-template <int NW, class RuleOf, class Kids>
+// its own cell, so each round here resolves up to NW queued records at once
+// and then appends their children in record order: the same records, in
+// the same order.  With GROUPS the NW waves are split into groups of
+// g = NW / m (rounded down to a power of two) for the m records of the
+// round, so a lone record is searched by all NW waves (worth it where a
+// record has hundreds of costly candidates, as Zuker's cl, and the
+// workgroup has 16 waves: launches of few instances); otherwise one wave
+// searches one record.
+// rule_of(rec, match, &rule, &p0, &p1) is called by every wave once per
+// round and calls match(ncand, target, cand) exactly once: match returns
+// the smallest candidate index c < ncand with cand(c, &v) true and
+// v == target (-1 if none) for the wave's record; with GROUPS it contains a
+// workgroup barrier.  rule_of returns false when no candidate matches.  kids(rec,
+// push) appends the children of a resolved record.  *len receives the
+// record count, or -1 (no match, or more records than cap).
+template <int NW, bool GROUPS, class RuleOf, class Kids>
 __device__ void bfs_rounds(TraceRec* tr, int cap, TraceRec root, RuleOf&& rule_of, Kids&& kids, int32_t* len) {
+  constexpr int NONE = 0x7fffffff;
   __shared__ int s_rd, s_wr, s_over;
-  __shared__ int s_fail[NW];  // per wave: no candidate matched its record this round
+  __shared__ int s_fail[NW];  // per record: no candidate matched it this round
+  __shared__ int s_hit[NW];   // per record: smallest matching candidate found so far
   __shared__ TraceRec s_rec[NW];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (threadIdx.x == 0) {
     s_rd = 0;
     s_wr = 0;
     s_over = 0;
+    for (int q = 0; q < NW; ++q) s_hit[q] = NONE;
     if (cap > 0) {
       tr[0] = root;
       s_wr = 1;
@@ -96,25 +93,62 @@ __device__ void bfs_rounds(TraceRec* tr, int cap, TraceRec root, RuleOf&& rule_o
     const int rd = s_rd, wr = s_wr;
     if (rd >= wr || s_over) break;
     const int m = wr - rd < NW ? wr - rd : NW;
-    if (w < m) {
-      TraceRec cur = tr[rd + w];
-      int rule = -1, p0 = 0, p1 = 0;
-      const bool ok = rule_of(cur, &rule, &p0, &p1);
-      if (lane == 0) {
-        // s_over is read by every wave at the top of the round, so a
-        // failure is recorded per wave and folded in below, between barriers
-        s_fail[w] = ok ? 0 : 1;
-        cur.rule = rule;
-        cur.pos[0] = p0;
-        cur.pos[1] = p1;
-        tr[rd + w] = cur;
-        s_rec[w] = cur;
+    int g = GROUPS ? NW : 1;
+    while (g > 1 && g * m > NW) g >>= 1;
+    const int rw = w / g, sub = w % g;
+    const bool act = rw < m;
+    // the g waves of a group take the 64-candidate chunks sub, sub + g, ...
+    // of their record; a wave stops at its first hit or once a hit below
+    // its next chunk is known; the smallest hit is the first match
+    auto match = [&](int ncand, int target, auto&& cand) -> int {
+      if (!GROUPS) {
+        if (act)
+          for (int base = 0; base < ncand; base += 64) {
+            const int c = base + lane;
+            int v = 0;
+            const bool hit = c < ncand && cand(c, &v) && v == target;
+            const unsigned long long mk = __ballot(hit);
+            if (mk) return base + (int)__ffsll(mk) - 1;
+          }
+        return -1;
       }
+      if (act) {
+        const volatile int* hp = &s_hit[rw];
+        for (int it = 0;; ++it) {
+          const int base = (it * g + sub) * 64;
+          if (base >= ncand || base > *hp) break;
+          const int c = base + lane;
+          int v = 0;
+          const bool hit = c < ncand && cand(c, &v) && v == target;
+          const unsigned long long mk = __ballot(hit);
+          if (mk) {
+            if (lane == 0) atomicMin(&s_hit[rw], base + (int)__ffsll(mk) - 1);
+            break;
+          }
+        }
+      }
+      __syncthreads();
+      const int h = act ? s_hit[rw] : NONE;
+      return h == NONE ? -1 : h;
+    };
+    TraceRec cur = tr[rd + (act ? rw : 0)];  // waves without a record search nothing
+    int rule = -1, p0 = 0, p1 = 0;
+    const bool ok = rule_of(cur, match, &rule, &p0, &p1);
+    if (act && sub == 0 && lane == 0) {
+      // s_over is read by every wave at the top of the round, so a failure
+      // is recorded per record and folded in below, between barriers
+      s_fail[rw] = ok ? 0 : 1;
+      cur.rule = rule;
+      cur.pos[0] = p0;
+      cur.pos[1] = p1;
+      tr[rd + rw] = cur;
+      s_rec[rw] = cur;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int q = 0; q < m; ++q)
         if (s_fail[q]) s_over = 1;
+      for (int q = 0; q < NW; ++q) s_hit[q] = NONE;
       int wr2 = wr;
       auto push = [&](int pi, int pj, int pr) {
         if (wr2 >= cap) {
@@ -141,7 +175,7 @@ __global__ __launch_bounds__(NT) void nu_finalize(TriArgs a) {
   const int n = in.n;
   if (threadIdx.x == 0) a.results[in.out] = C.get(0, 0, n);
   if (!a.want_trace) return;
-  auto rule_of = [&](const TraceRec& cur, int* rule, int* p0, int* p1) -> bool {
+  auto rule_of = [&](const TraceRec& cur, auto&& match, int* rule, int* p0, int* p1) -> bool {
     const int i = cur.i, j = cur.j;
     const int target = C.get(0, i, j);
     // candidates in nu_kern.h order: nil, left, right, pair, split(k asc)
@@ -159,7 +193,7 @@ __global__ __launch_bounds__(NT) void nu_finalize(TriArgs a) {
       *v = C.get(0, i, k) + C.get(0, k, j);
       return true;
     };
-    const int c = wave_first_match(4 + fmax_(0, j - i - 1), target, cand);
+    const int c = match(4 + fmax_(0, j - i - 1), target, cand);
     *rule = c < 4 ? c : 4;
     *p0 = c >= 4 ? i + 1 + (c - 4) : 0;
     *p1 = 0;
@@ -175,7 +209,7 @@ __global__ __launch_bounds__(NT) void nu_finalize(TriArgs a) {
       default: break;
     }
   };
-  bfs_rounds<NT / 64>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
+  bfs_rounds<NT / 64, false>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
              a.trace_lens + in.out);
 }
 
@@ -189,7 +223,7 @@ __global__ __launch_bounds__(NT) void mc_finalize(TriArgs a) {
   const int n = in.n;
   if (threadIdx.x == 0) a.results[in.out] = C.get(0, 0, n);
   if (!a.want_trace) return;
-  auto rule_of = [&](const TraceRec& cur, int* rule, int* p0, int* p1) -> bool {
+  auto rule_of = [&](const TraceRec& cur, auto&& match, int* rule, int* p0, int* p1) -> bool {
     const int i = cur.i, j = cur.j;
     const int target = C.get(0, i, j);
     auto cand = [&](int c, int* v) -> bool {
@@ -200,7 +234,7 @@ __global__ __launch_bounds__(NT) void mc_finalize(TriArgs a) {
                  (uint32_t)rows[i] * (uint32_t)rows[k] * (uint32_t)cols[j - 1]);
       return true;
     };
-    const int c = wave_first_match(1 + fmax_(0, j - i - 1), target, cand);
+    const int c = match(1 + fmax_(0, j - i - 1), target, cand);
     *rule = c == 0 ? 0 : 1;
     *p0 = c <= 0 ? 0 : i + c;
     *p1 = 0;
@@ -212,7 +246,7 @@ __global__ __launch_bounds__(NT) void mc_finalize(TriArgs a) {
       push(e.pos[0], e.j, 0);
     }
   };
-  bfs_rounds<NT / 64>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
+  bfs_rounds<NT / 64, false>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
              a.trace_lens + in.out);
 }
 
@@ -349,7 +383,7 @@ __global__ __launch_bounds__(NT) void zu_finalize(TriArgs a) {
   }
   if (threadIdx.x == 0) a.results[in.out] = st[0];
   if (!a.want_trace) return;
-  auto rule_of = [&](const TraceRec& cur, int* rule, int* p0, int* p1) -> bool {
+  auto rule_of = [&](const TraceRec& cur, auto&& match, int* rule, int* p0, int* p1) -> bool {
     const int i = cur.i, j = cur.j, tabrule = cur.rule, d = j - i;
     int ncand, target;
     if (tabrule == 12) { target = st[i]; ncand = 2 + fmax_(0, n - (i + 7) + 1); }
@@ -445,7 +479,7 @@ __global__ __launch_bounds__(NT) void zu_finalize(TriArgs a) {
       if (U >= INV_THR) return false;
       *v = U; return true;
     };
-    const int c = wave_first_match(ncand, target, cand);
+    const int c = match(ncand, target, cand);
     if (c < 0) {
       // no candidate: the record is dropped with the whole trace
     } else if (tabrule == 12) {
@@ -487,7 +521,7 @@ __global__ __launch_bounds__(NT) void zu_finalize(TriArgs a) {
       default: break;
     }
   };
-  bfs_rounds<NT / 64>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 12, {0, 0}}, rule_of, kids,
+  bfs_rounds<NT / 64, (NT >= 1024)>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 12, {0, 0}}, rule_of, kids,
              a.trace_lens + in.out);
 }
 
@@ -523,7 +557,7 @@ __global__ __launch_bounds__(NT) void rf_finalize(TriArgs a) {
     if (threadIdx.x == 0) a.trace_lens[in.out] = 0;
     return;
   }
-  auto rule_of = [&](const TraceRec& cur, int* rule, int* p0, int* p1) -> bool {
+  auto rule_of = [&](const TraceRec& cur, auto&& match, int* rule, int* p0, int* p1) -> bool {
     const int i = cur.i, j = cur.j, tabrule = cur.rule, d = j - i;
     const int l0 = fmax_(i + 4, j - 30), nl = fmax_(0, j - 1 - l0 + 1);
     int ncand, target;
@@ -563,7 +597,7 @@ __global__ __launch_bounds__(NT) void rf_finalize(TriArgs a) {
       if (x >= INV_THR) return false;
       *v = x; return true;
     };
-    const int c = wave_first_match(ncand, target, cand);
+    const int c = match(ncand, target, cand);
     if (c < 0) {
       // no candidate: the record is dropped with the whole trace
     } else if (tabrule == 0) {
@@ -598,7 +632,7 @@ __global__ __launch_bounds__(NT) void rf_finalize(TriArgs a) {
       default: break;
     }
   };
-  bfs_rounds<NT / 64>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
+  bfs_rounds<NT / 64, (NT >= 1024)>((TraceRec*)a.traces + in.trace_off, in.trace_cap, TraceRec{0, n, 0, {0, 0}}, rule_of, kids,
              a.trace_lens + in.out);
 }
 
