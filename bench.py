@@ -130,6 +130,9 @@ def serve(ctx, rate, seconds, budget, slots, max_batch, check_every):
     srv.drain(w)
     ids = [None] * len(reqs)
     rejected = []
+    check = set(range(0, len(reqs), check_every)) if check_every > 0 else set()
+    keep = set()  # ids whose traces are fetched (the bit-exact sample); the others are dropped unread
+    known = set()
 
     def produce():
         t0 = dpx.now()
@@ -146,19 +149,36 @@ def serve(ctx, rate, seconds, budget, slots, max_batch, check_every):
                     ids[q] = srv.submit_matchain(x[0], x[1])
             except dpx.DpxError as e:
                 rejected.append((q, e.code))
+                continue
+            if q in check:
+                keep.add(ids[q])
+            known.add(ids[q])  # after keep: an id in known has its keep decision made
 
     th = threading.Thread(target=produce)
     t_start = dpx.now()
     th.start()
-    check = set(range(0, len(reqs), check_every)) if check_every > 0 else set()
     got = {}
-    while th.is_alive() or len(got) < sum(1 for x in ids if x is not None):
+    undecided = []  # finished before the producer recorded its id
+
+    def take(res):
+        tr = None
+        if res.id in keep:
+            tr = srv.trace(res.id, res.trace_len)
+        else:
+            srv.drop_trace(res.id)
+        got[res.id] = (res, tr)
+
+    while th.is_alive() or len(got) + len(undecided) < sum(1 for x in ids if x is not None):
         for res in srv.poll(wait_us=5000):
-            tr = None
-            if res.trace_len > 0:
-                tr = srv.trace(res.id, res.trace_len)
-            got[res.id] = (res, tr)
+            if res.trace_len <= 0:
+                got[res.id] = (res, None)
+            elif res.id in known:
+                take(res)
+            else:
+                undecided.append(res)
     th.join()
+    for res in undecided:
+        take(res)
     st = srv.stats()
     srv.close()
     done = [got[i][0] for i in ids if i is not None]

@@ -90,7 +90,7 @@ struct Workspace {
   }
 };
 
-enum { JOB_TRI = 0, JOB_SW = 1 };
+enum { JOB_TRI = 0, JOB_SW = 1, JOB_SWB = 2 };
 // device block of a triangular job
 enum {
   TL_TAB, TL_QUEUE, TL_DEP, TL_CTRL, TL_INST, TL_SEEDS, TL_DATA, TL_SPEC, TL_RES, TL_LENS, TL_TRACES,
@@ -98,6 +98,22 @@ enum {
 };
 // device block of an alignment job
 enum { SL_A, SL_B, SL_ROWS, SL_FLAGS, SL_BEST, SL_BT, SL_CP, SL_TR, SL_N };
+
+// One alignment of a batched launch (JOB_SWB): its inputs, row 0 and where
+// its parts lie in the job's device block.
+struct SwPairJob {
+  std::string a, b;
+  dpx_swat_scores sc{};
+  int trace_from = 0, cap = 0;
+  std::vector<int2> row0;
+  std::vector<uint8_t> r0m, r0g2;
+  int row0_best = 0, row0_bestj = 0, row0_axiom = 0;
+  int nbands = 0, band0 = 0, nslots = 0, nwords = 0;
+  size_t in_a = 0, in_b = 0, in_row0 = 0;  // inputs (offsets in the block)
+  size_t ring = 0, codes = 0;              // boundary-row ring, traceback codes
+  int64_t tr0 = 0;                         // first trace record in the batch's trace region
+  int64_t cells = 0;
+};
 
 struct Job {
   int kind = JOB_TRI;
@@ -124,6 +140,11 @@ struct Job {
   std::vector<int2> row0;
   std::vector<uint8_t> r0m, r0g2;
   int row0_best = 0, row0_bestj = 0, row0_axiom = 0;
+  // batched Smith-Waterman (JOB_SWB): the alignments and the block's regions
+  std::vector<SwPairJob> swb;
+  int swb_bands = 0;
+  bool swb_trace = false;
+  size_t swb_desc = 0, swb_bandtab = 0, swb_claim = 0, swb_in0 = 0, swb_in1 = 0, swb_out = 0, swb_tr = 0;
   // outputs: views into the workspace's pinned buffer, valid after the event
   const int32_t* res = nullptr;
   const int32_t* lens = nullptr;
@@ -495,31 +516,23 @@ int tri_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
   return DPX_OK;
 }
 
-// seg > 0: keep checkpoint rows every seg bands instead of every band's
-// traceback codes (only with a trace; the synchronous entry point's choice)
-int sw_prepare(std::string& err, Job& j, const char* a, int n1, const char* b, int n2, const dpx_swat_scores& sc,
-               int trace_from, int trace_cap, int seg = 0) {
+// Checks an alignment's size and scores and computes its row 0 (i == 0):
+// g1 = 0 [4]; g2 [7/8/9]; M = g1 [2], g2 when strictly greater [3].
+int sw_row0(std::string& err, int n1, int n2, const dpx_swat_scores& sc, std::vector<int2>& row0,
+            std::vector<uint8_t>& r0m, std::vector<uint8_t>& r0g2, int* best, int* bestj, int* axiom) {
   // the best cell travels as one 64-bit key: a 36-bit cell index and a
   // 28-bit score (bounded by the largest gain per step times the steps)
   if ((int64_t)(n1 + 1) * (n2 + 1) > (int64_t)SW_KEY_MASK) return fail(err, DPX_EINVAL, "matrix too large (over 2^36 cells)");
   const int64_t gain = std::max({0, sc.match, sc.gap_open, sc.gap_ext});
   if (gain * ((int64_t)n1 + n2 + 1) >= ((int64_t)1 << (64 - SW_KEY_BITS - 1)))
     return fail(err, DPX_EINVAL, "scores too large for the best-cell key");
-  j.kind = JOB_SW;
-  j.a.assign(a, n1);
-  j.b.assign(b, n2);
-  j.sc = sc;
-  j.trace_from = trace_from;
-  j.sw_cap = trace_cap;
-  j.sw_seg = trace_cap > 0 ? seg : 0;
   const int W = n2 + 1;
-  // row 0 (i == 0): g1 = 0 [4]; g2 [7/8/9]; M = g1 [2], g2 when strictly greater [3]
-  j.row0.resize(W);
-  j.r0m.resize(W);
-  j.r0g2.resize(W);
+  row0.resize(W);
+  r0m.resize(W);
+  r0g2.resize(W);
   int Mprev = 0, G2prev = 0;
-  j.row0_best = -1;
-  j.row0_bestj = 0;
+  *best = -1;
+  *bestj = 0;
   for (int x = 0; x < W; ++x) {
     int g2, rg2;
     if (x == 0) {
@@ -532,16 +545,123 @@ int sw_prepare(std::string& err, Job& j, const char* a, int n1, const char* b, i
     }
     int M = 0, rm = x == 0 ? 0 : 2;
     if (g2 > M) { M = g2; rm = 3; }
-    j.row0[x] = make_int2(M, 0);
-    j.r0m[x] = (uint8_t)rm;
-    j.r0g2[x] = (uint8_t)rg2;
-    if (M > j.row0_best) { j.row0_best = M; j.row0_bestj = x; }
+    row0[x] = make_int2(M, 0);
+    r0m[x] = (uint8_t)rm;
+    r0g2[x] = (uint8_t)rg2;
+    if (M > *best) { *best = M; *bestj = x; }
     Mprev = M;
     G2prev = g2;
   }
-  j.row0_axiom = j.row0[n2].x;
+  *axiom = row0[n2].x;
+  return DPX_OK;
+}
+
+// seg > 0: keep checkpoint rows every seg bands instead of every band's
+// traceback codes (only with a trace; the synchronous entry point's choice)
+int sw_prepare(std::string& err, Job& j, const char* a, int n1, const char* b, int n2, const dpx_swat_scores& sc,
+               int trace_from, int trace_cap, int seg = 0) {
+  const int r = sw_row0(err, n1, n2, sc, j.row0, j.r0m, j.r0g2, &j.row0_best, &j.row0_bestj, &j.row0_axiom);
+  if (r) return r;
+  j.kind = JOB_SW;
+  j.a.assign(a, n1);
+  j.b.assign(b, n2);
+  j.sc = sc;
+  j.trace_from = trace_from;
+  j.sw_cap = trace_cap;
+  j.sw_seg = trace_cap > 0 ? seg : 0;
   j.cells = (int64_t)(n1 + 1) * (n2 + 1);
   j.dev_bytes = sw_layout(n1, n2, trace_cap, j.sw_seg, j.off);
+  return DPX_OK;
+}
+
+// ---- batched alignments (JOB_SWB) ----
+// Block: descriptors (SwPair per alignment) | band -> alignment table |
+// claim counter | inputs (per alignment a, b, row 0 and its rule bytes) |
+// outputs (per alignment 32 ints: [0..5] pick, [8..13] trace state,
+// [16..19] ctrl, [20..21] best-cell key, [22] M(n1,n2)) | trace records |
+// per alignment its boundary-row ring and traceback codes.  Descriptors,
+// table and inputs are one upload, outputs one download.
+constexpr size_t kSwbOut = 32 * sizeof(int32_t);
+size_t a16(size_t x) { return (x + 15) & ~(size_t)15; }
+size_t swb_in_bytes(int n1, int n2) { return a16(n1) + a16(n2) + a16(sizeof(int2) * (n2 + 1)) + 2 * a16(n2 + 1); }
+size_t swb_work_bytes(int n1, int n2, int cap) {
+  const int nbands = (n1 + 63) / 64;
+  const size_t ring = (size_t)sw_slots(nbands) * (n2 + 2) * 8;
+  const size_t codes = cap > 0 ? (size_t)nbands * sw_code_words(n2) * 64 * 4 : 0;
+  return a256(ring) + a256(codes);
+}
+// upper bound of one alignment's share of a batch's block (the server's footprint)
+size_t swb_pair_bytes(int n1, int n2, int cap) {
+  return a256(sizeof(SwPair)) + a256(4 * (size_t)((n1 + 63) / 64)) + a256(swb_in_bytes(n1, n2)) + a256(kSwbOut) +
+         a256(sizeof(dpx_trace_t) * (size_t)cap) + swb_work_bytes(n1, n2, cap) + 256;
+}
+
+// Lays out a batch whose alignments are in j.swb (sizes, caps set).
+void swb_layout(Job& j) {
+  const int np = (int)j.swb.size();
+  j.swb_bands = 0;
+  int64_t tr = 0;
+  size_t in = 0;
+  for (SwPairJob& q : j.swb) {
+    const int n1 = (int)q.a.size(), n2 = (int)q.b.size();
+    q.nbands = (n1 + 63) / 64;
+    q.band0 = j.swb_bands;
+    j.swb_bands += q.nbands;
+    q.nslots = sw_slots(q.nbands);
+    q.nwords = sw_code_words(n2);
+    q.tr0 = tr;
+    tr += q.cap;
+    q.in_a = in;
+    q.in_b = q.in_a + a16(n1);
+    q.in_row0 = q.in_b + a16(n2);
+    in += swb_in_bytes(n1, n2);
+  }
+  size_t at = 0;
+  j.swb_desc = at;
+  at += a256(sizeof(SwPair) * (size_t)np);
+  j.swb_bandtab = at;
+  at += a256(4 * (size_t)j.swb_bands);
+  j.swb_in0 = 0;  // the upload starts at the descriptors
+  j.swb_claim = at;
+  at += 256;
+  const size_t inputs = at;
+  for (SwPairJob& q : j.swb) {
+    q.in_a += inputs;
+    q.in_b += inputs;
+    q.in_row0 += inputs;
+  }
+  at += a256(in);
+  j.swb_in1 = at;  // end of the upload
+  j.swb_out = at;
+  at += a256(kSwbOut * (size_t)np);
+  j.swb_tr = at;
+  at += a256(sizeof(dpx_trace_t) * (size_t)tr);
+  for (SwPairJob& q : j.swb) {
+    const int n2 = (int)q.b.size();
+    q.ring = at;
+    at += a256((size_t)q.nslots * (n2 + 2) * 8);
+    q.codes = at;
+    at += q.cap > 0 ? a256((size_t)q.nbands * q.nwords * 64 * 4) : 0;
+  }
+  j.dev_bytes = at;
+}
+
+// A batch of alignments (every one traced, or none).
+int swb_add(std::string& err, Job& j, const std::string& a, const std::string& b, const dpx_swat_scores& sc,
+            int trace_from, int cap) {
+  SwPairJob q;
+  const int r = sw_row0(err, (int)a.size(), (int)b.size(), sc, q.row0, q.r0m, q.r0g2, &q.row0_best, &q.row0_bestj,
+                        &q.row0_axiom);
+  if (r) return r;
+  q.a = a;
+  q.b = b;
+  q.sc = sc;
+  q.trace_from = trace_from;
+  q.cap = cap;
+  q.cells = (int64_t)(a.size() + 1) * (b.size() + 1);
+  j.kind = JOB_SWB;
+  j.cells += q.cells;
+  j.swb.push_back(std::move(q));
   return DPX_OK;
 }
 
@@ -662,8 +782,90 @@ int sw_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j)
   return DPX_OK;
 }
 
+// A batch of alignments: one upload, reset + one persistent fill over the
+// bands of all of them, best cell and traceback per alignment, one download.
+int swb_launch_job(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
+  const int np = (int)j.swb.size();
+  int64_t trtot = 0;
+  for (const SwPairJob& q : j.swb) trtot += q.cap;
+  const size_t up = j.swb_in1, outb = a256(kSwbOut * (size_t)np);
+  const size_t trb = j.swb_trace ? sizeof(dpx_trace_t) * (size_t)trtot : 0;
+  if (!w.h_in.ensure(up) || !w.h_out.ensure(outb + trb + 64)) return fail(err, DPX_ENOMEM, "pinned host staging");
+  char* h = (char*)w.h_in.p;
+  SwPair* hp = (SwPair*)(h + j.swb_desc);
+  int32_t* hbt = (int32_t*)(h + j.swb_bandtab);
+  int n_max = 0;
+  size_t max_words = 1;
+  for (int p = 0; p < np; ++p) {
+    const SwPairJob& q = j.swb[p];
+    const int n1 = (int)q.a.size(), n2 = (int)q.b.size(), W = n2 + 1;
+    int* outp = (int*)(dev + j.swb_out + kSwbOut * (size_t)p);
+    SwPair d{};
+    SwArgs& A = d.A;
+    A.n1 = n1;
+    A.n2 = n2;
+    A.match = q.sc.match;
+    A.mismatch = q.sc.mismatch;
+    A.open = q.sc.gap_open;
+    A.ext = q.sc.gap_ext;
+    A.nbands = q.nbands;
+    A.nwords = q.nwords;
+    A.nslots = q.nslots;
+    A.hrow = (unsigned long long*)(dev + q.ring);
+    A.a = (const uint8_t*)(dev + q.in_a);
+    A.b = (const uint8_t*)(dev + q.in_b);
+    A.ctrl = outp + 16;  // [16] (unused claim counter), [17] band-wait time-out
+    A.bt = q.cap > 0 ? (uint32_t*)(dev + q.codes) : nullptr;
+    A.cp = nullptr;
+    A.cp_every = 1;
+    A.best = (unsigned long long*)(outp + 20);
+    A.out = outp + 22;
+    d.band0 = q.band0;
+    d.row0_best = q.row0_best;
+    d.row0_bestj = q.row0_bestj;
+    d.row0_axiom = q.row0_axiom;
+    d.trace_from = q.trace_from;
+    d.cap = q.cap;
+    d.row0 = (const int2*)(dev + q.in_row0);
+    d.r0m = (const uint8_t*)(dev + q.in_row0 + a16(sizeof(int2) * W));
+    d.r0g2 = d.r0m + a16(W);
+    d.pick = outp;
+    d.state = outp + 8;
+    d.tr = q.cap > 0 ? dev + j.swb_tr + sizeof(dpx_trace_t) * (size_t)q.tr0 : nullptr;
+    hp[p] = d;
+    for (int b = 0; b < q.nbands; ++b) hbt[q.band0 + b] = p;
+    std::memcpy(h + q.in_a, q.a.data(), n1);
+    std::memcpy(h + q.in_b, q.b.data(), n2);
+    std::memcpy(h + q.in_row0, q.row0.data(), sizeof(int2) * W);
+    std::memcpy(h + q.in_row0 + a16(sizeof(int2) * W), q.r0m.data(), W);
+    std::memcpy(h + q.in_row0 + a16(sizeof(int2) * W) + a16(W), q.r0g2.data(), W);
+    n_max = std::max(n_max, std::max(n1, n2));
+    max_words = std::max(max_words, (size_t)q.nslots * (n2 + 2));
+  }
+  hipStream_t s = w.stream;
+  HIPCHK(hipMemcpyAsync(dev, h, up, hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(w.ev[0], s));
+  const SwPair* d_pairs = (const SwPair*)(dev + j.swb_desc);
+  if (sw_batch_launch(d_pairs, np, (const int32_t*)(dev + j.swb_bandtab), j.swb_bands, (int*)(dev + j.swb_claim),
+                      max_words, j.swb_trace, j.swb_trace, sw_grid(c, std::max(1, j.swb_bands), n_max),
+                      sw_chunk(c, n_max), s))
+    return fail(err, DPX_EHIP, "batched alignment launch failed");
+  HIPCHK(hipEventRecord(w.ev[1], s));
+  if (sw_batch_finish(d_pairs, np, j.swb_trace, s)) return fail(err, DPX_EHIP, "batched traceback launch failed");
+  HIPCHK(hipEventRecord(w.ev[2], s));
+  char* ho = (char*)w.h_out.p;
+  HIPCHK(hipMemcpyAsync(ho, dev + j.swb_out, kSwbOut * (size_t)np, hipMemcpyDeviceToHost, s));
+  if (trb) HIPCHK(hipMemcpyAsync(ho + outb, dev + j.swb_tr, trb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(w.ev[3], s));
+  j.res = (const int32_t*)ho;
+  j.tr = (const dpx_trace_t*)(ho + outb);
+  return DPX_OK;
+}
+
 int job_launch(dpx_ctx* c, std::string& err, Workspace& w, char* dev, Job& j) {
-  return j.kind == JOB_TRI ? tri_launch(c, err, w, dev, j) : sw_launch_job(c, err, w, dev, j);
+  return j.kind == JOB_TRI  ? tri_launch(c, err, w, dev, j)
+         : j.kind == JOB_SW ? sw_launch_job(c, err, w, dev, j)
+                            : swb_launch_job(c, err, w, dev, j);
 }
 
 // after ev[3] has completed: timings and the kernels' time-out flags
@@ -675,6 +877,13 @@ int job_harvest(std::string& err, Workspace& w, Job& j) {
     int32_t* r = (int32_t*)j.res;
     if (r[7]) return fail(err, DPX_EHIP, "band wait timed out (scheduler error)");
     r[6] = j.sw_cap > 0 ? (r[8 + 5] ? r[8 + 3] : -1) : 0;  // trace length from the trace state
+  }
+  if (j.kind == JOB_SWB) {
+    for (size_t p = 0; p < j.swb.size(); ++p) {
+      int32_t* r = (int32_t*)j.res + 32 * p;
+      if (r[17]) return fail(err, DPX_EHIP, "band wait timed out (scheduler error)");
+      r[6] = j.swb[p].cap > 0 ? (r[8 + 5] ? r[8 + 3] : -1) : 0;
+    }
   }
   return DPX_OK;
 }
@@ -769,6 +978,9 @@ struct Request {
   int size_class = 0;
   int passed = 0;  // launches started ahead of it while it waited for memory
 };
+
+// alignments batched into one launch by the server at most
+constexpr int kSwMaxBatch = 64;
 
 // A request that does not fit the arena's free memory may be passed over by
 // later requests that do, at most this many times; then it blocks the queue
@@ -888,9 +1100,16 @@ int prepare_slot(dpx_server* S, Slot& sl) {
   Job& j = sl.job;
   j = Job();
   const Request& r0 = sl.reqs.front();
-  if (r0.kind == DPX_REQ_SWAT)
-    return sw_prepare(sl.err, j, r0.s1.data(), (int)r0.s1.size(), r0.s2.data(), (int)r0.s2.size(), r0.sc, r0.trace_from,
-                      r0.trace ? sw_trace_cap((int)r0.s1.size(), (int)r0.s2.size()) : 0);
+  if (r0.kind == DPX_REQ_SWAT) {  // one batched launch (a batch shares the trace option)
+    for (const Request& r : sl.reqs) {
+      const int rc = swb_add(sl.err, j, r.s1, r.s2, r.sc, r.trace_from,
+                             r.trace ? sw_trace_cap((int)r.s1.size(), (int)r.s2.size()) : 0);
+      if (rc) return rc;
+    }
+    j.swb_trace = r0.trace != 0;
+    swb_layout(j);
+    return DPX_OK;
+  }
   const int count = (int)sl.reqs.size();
   std::vector<int> lens(count);
   std::vector<int64_t> off(count);
@@ -937,6 +1156,14 @@ void harvest_slot(dpx_server* S, Slot& sl, int status) {
         f.r.best_j = j.res[3];
         tl = j.sw_cap > 0 ? j.res[6] : 0;
         src = j.tr;
+      } else if (j.kind == JOB_SWB) {
+        const int32_t* r = j.res + 32 * q;
+        f.r.score = r[0];
+        f.r.best = r[1];
+        f.r.best_i = r[2];
+        f.r.best_j = r[3];
+        tl = j.swb[q].cap > 0 ? r[6] : 0;
+        src = j.tr + j.swb[q].tr0;
       } else {
         f.r.score = j.res[q];
         tl = j.want_trace ? j.lens[q] : 0;
@@ -974,7 +1201,7 @@ bool take_batch(dpx_server* S, Slot& sl) {
   if (hit == S->pending.end()) return false;  // wait until launches in flight release memory
   for (auto it = S->pending.begin(); it != hit; ++it) ++it->passed;
   const Request& h = *hit;
-  const int cap = h.kind == DPX_REQ_SWAT ? 1 : S->cfg.max_batch;
+  const int cap = h.kind == DPX_REQ_SWAT ? std::min(S->cfg.max_batch, kSwMaxBatch) : S->cfg.max_batch;
   size_t used = 0;
   const int kind = h.kind, trace = h.trace, ml = h.min_loop, cls = h.size_class;
   for (auto it = hit; it != S->pending.end() && (int)sl.reqs.size() < cap;) {
@@ -1758,8 +1985,9 @@ int dpx_server_submit(dpx_server* S, const dpx_request* req, uint64_t* id) {
     q.s1.assign(req->a, n1);
     q.s2.assign(req->b, n2);
     q.sc = req->sc;
-    q.footprint = sw_layout(n1, n2, q.trace ? sw_trace_cap(n1, n2) : 0, 0, nullptr);
+    q.footprint = swb_pair_bytes(n1, n2, q.trace ? sw_trace_cap(n1, n2) : 0);
     q.cells = (int64_t)(n1 + 1) * (n2 + 1);
+    q.size_class = size_class(std::max(n1, n2));
   } else {
     return DPX_EINVAL;
   }

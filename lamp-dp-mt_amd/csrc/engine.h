@@ -150,6 +150,25 @@ int sw_finish_launch(const SwArgs& A, int row0_best, int row0_bestj, int row0_ax
 // its end (i_base = 0, where row 0's rules apply).
 int sw_trace_launch(const SwArgs& A, const uint8_t* r0m, const uint8_t* r0g2, int i_base, void* tr, int cap,
                     int* state, void* stream);
+// Alignments batched into one launch: per alignment (device memory) its
+// arguments, its first band in the launch's claim order and what the reset,
+// pick and traceback kernels read.
+struct SwPair {
+  SwArgs A;
+  int band0;
+  int row0_best, row0_bestj, row0_axiom, trace_from, cap;
+  const int2* row0;                     // row 0 of M and g1
+  const uint8_t* r0m;                   // row 0's M rules
+  const uint8_t* r0g2;                  // row 0's g2 rules
+  int* pick;                            // 6 ints (sw_pick's out)
+  int* state;                           // SW_STATE ints
+  void* tr;                             // cap trace records, or null
+};
+// reset + persistent fill over all bands of the batch (codes: every pair
+// keeps traceback codes); then best cell, start and traceback per pair
+int sw_batch_launch(const SwPair* P, int npairs, const int32_t* band_pair, int total_bands, int* claim, size_t max_words,
+                    bool codes, bool trace, int grid, int chunk, void* stream);
+int sw_batch_finish(const SwPair* P, int npairs, bool trace, void* stream);
 int sw_occupancy();
 
 // launchers (tri.hip, finalize.hip); stream is a hipStream_t

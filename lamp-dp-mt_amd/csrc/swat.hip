@@ -71,17 +71,13 @@ __device__ __forceinline__ int sw_lane0_from(int v) {
 // top nibble): bit 3 = g1 > pair (M rule 2 unless bit 2), bit 2 = g2 > both
 // (M rule 3), bit 1 = g1 extends (rule 6, else 5), bit 0 = g2 extends
 // (rule 9, else 8).
+// One band of alignment A, by the calling wave (band wave-uniform).
 template <int SW_CH, bool CODES>
-__global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
+__device__ __forceinline__ void sw_band(const SwArgs& A, const int band) {
   static_assert(SW_CH % 16 == 0 && SW_CH <= 64, "chunk: whole 16-lane rows of feed, one column per lane");
   constexpr int NQ = SW_CH / 16;
   const int lane = threadIdx.x;
-  for (;;) {
-    // one wave per workgroup: the band claim and every wait are wave-uniform
-    int band = 0;
-    if (lane == 0) band = atomicAdd(&A.ctrl[0], 1);
-    band = __builtin_amdgcn_readfirstlane(band);
-    if (band >= A.nbands) break;
+  {
     const int i = band * 64 + lane + 1;
     const bool row_ok = i <= A.n1;
     const int ai = row_ok ? A.a[i - 1] : -1;
@@ -271,9 +267,50 @@ __global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
   }
 }
 
+// one wave per workgroup: the band claim and every wait are wave-uniform
+template <int SW_CH, bool CODES>
+__global__ __launch_bounds__(64) void sw_fill(SwArgs A) {
+  for (;;) {
+    int band = 0;
+    if (threadIdx.x == 0) band = atomicAdd(&A.ctrl[0], 1);
+    band = __builtin_amdgcn_readfirstlane(band);
+    if (band >= A.nbands) break;
+    sw_band<SW_CH, CODES>(A, band);
+  }
+}
+
+// An alignment's arguments read from device memory into scalar registers
+// (the calling wave's index is uniform).
+__device__ __forceinline__ SwArgs uniform_args(const SwArgs* src) {
+  static_assert(sizeof(SwArgs) % 4 == 0, "whole words");
+  int w[sizeof(SwArgs) / 4];
+  __builtin_memcpy(w, src, sizeof(SwArgs));
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(SwArgs) / 4); ++k) w[k] = __builtin_amdgcn_readfirstlane(w[k]);
+  SwArgs A;
+  __builtin_memcpy(&A, w, sizeof(SwArgs));
+  return A;
+}
+
+// Several alignments in one persistent launch: the bands of all of them are
+// claimed in one order (alignment by alignment, band by band), so a band's
+// predecessor was always claimed before it and is running or done.
+template <int SW_CH, bool CODES>
+__global__ __launch_bounds__(64) void sw_fill_batch(const SwPair* P, const int32_t* band_pair, int total, int* claim) {
+  for (;;) {
+    int g = 0;
+    if (threadIdx.x == 0) g = atomicAdd(claim, 1);
+    g = __builtin_amdgcn_readfirstlane(g);
+    if (g >= total) break;
+    const int p = __builtin_amdgcn_readfirstlane(band_pair[g]);
+    const SwArgs A = uniform_args(&P[p].A);
+    sw_band<SW_CH, CODES>(A, g - __builtin_amdgcn_readfirstlane(P[p].band0));
+  }
+}
+
 // Slot 0 = row 0 tagged for band 0; every other slot cleared (tag 0), so no
 // word of an earlier launch or an earlier use of the slot passes as fresh.
-__global__ void sw_reset(SwArgs A, const int2* row0) {
+__device__ __forceinline__ void sw_reset_body(const SwArgs& A, const int2* row0) {
   const int W = A.n2 + 1;
   const size_t total = (size_t)A.nslots * (W + 1);  // each slot: columns 0..n2 and a spare word
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
@@ -287,6 +324,13 @@ __global__ void sw_reset(SwArgs A, const int2* row0) {
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) { A.ctrl[0] = 0; A.ctrl[1] = 0; *A.best = 0; A.out[0] = 0; }
 }
+__global__ void sw_reset(SwArgs A, const int2* row0) { sw_reset_body(A, row0); }
+// blockIdx.y = alignment; the batch's claim counter is cleared by one thread
+__global__ void sw_reset_batch(const SwPair* P, int* claim) {
+  const SwPair& q = P[blockIdx.y];
+  sw_reset_body(q.A, q.row0);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *claim = 0;
+}
 
 // Traceback: the reference BFS from (i0,j0) (comp1.cu:100-132) is a single
 // chain here; one wave walks it, caching a 64-word window of the band's
@@ -294,8 +338,8 @@ __global__ void sw_reset(SwArgs A, const int2* row0) {
 // Best cell over row 0 (host-computed: value, column) and the device rows
 // (largest M, then smallest (i, j)); start of the trace.  out: [0]=axiom
 // M(n1,n2), [1]=best, [2]=best_i, [3]=best_j, [4]=trace i0, [5]=trace j0.
-__global__ void sw_pick(SwArgs A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out,
-                        int* state) {
+__device__ __forceinline__ void sw_pick_body(const SwArgs& A, int row0_best, int row0_bestj, int row0_axiom,
+                                             int trace_from, int* out, int* state) {
   const unsigned long long key = *A.best;
   const int W = A.n2 + 1;
   int bv = row0_best, bi = 0, bj = row0_bestj;
@@ -315,9 +359,17 @@ __global__ void sw_pick(SwArgs A, int row0_best, int row0_bestj, int row0_axiom,
   state[4] = 0;
   state[5] = 1;
 }
+__global__ void sw_pick(SwArgs A, int row0_best, int row0_bestj, int row0_axiom, int trace_from, int* out,
+                        int* state) {
+  sw_pick_body(A, row0_best, row0_bestj, row0_axiom, trace_from, out, state);
+}
+__global__ void sw_pick_batch(const SwPair* P) {  // one thread per alignment
+  const SwPair& q = P[blockIdx.x];
+  sw_pick_body(q.A, q.row0_best, q.row0_bestj, q.row0_axiom, q.trace_from, q.pick, q.state);
+}
 
-__global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mrule, const uint8_t* row0_g2rule,
-                                               int i_base, SwTrace* tr, int cap, int* state) {
+__device__ __forceinline__ void sw_trace_body(const SwArgs& A, const uint8_t* row0_mrule, const uint8_t* row0_g2rule,
+                                              int i_base, SwTrace* tr, int cap, int* state) {
   constexpr int NW = 64;
   __shared__ uint32_t win[NW][64];  // words w0 .. w0+NW-1 of band wb, [word][row]
   const int lane = threadIdx.x;
@@ -397,6 +449,15 @@ __global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mru
     state[5] = ok ? 1 : 0;
   }
 }
+__global__ __launch_bounds__(64) void sw_trace(SwArgs A, const uint8_t* row0_mrule, const uint8_t* row0_g2rule,
+                                               int i_base, SwTrace* tr, int cap, int* state) {
+  sw_trace_body(A, row0_mrule, row0_g2rule, i_base, tr, cap, state);
+}
+// one wave per alignment (blockIdx.x); alignments without a trace skip
+__global__ __launch_bounds__(64) void sw_trace_batch(const SwPair* P) {
+  const SwPair& q = P[blockIdx.x];
+  if (q.tr && q.A.bt) sw_trace_body(q.A, q.r0m, q.r0g2, 0, (SwTrace*)q.tr, q.cap, q.state);
+}
 
 int sw_launch(const SwArgs& A, const int2* row0, int grid, int chunk, void* stream) {
   hipStream_t s = (hipStream_t)stream;
@@ -422,6 +483,25 @@ int sw_trace_launch(const SwArgs& A, const uint8_t* r0m, const uint8_t* r0g2, in
                     int* state, void* stream) {
   hipLaunchKernelGGL(sw_trace, dim3(1), dim3(64), 0, (hipStream_t)stream, A, r0m, r0g2, i_base, (SwTrace*)tr, cap,
                      state);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sw_batch_launch(const SwPair* P, int npairs, const int32_t* band_pair, int total_bands, int* claim, size_t max_words,
+                    bool codes, bool trace, int grid, int chunk, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(sw_reset_batch, dim3((unsigned)std::min<size_t>((max_words + 255) / 256, 1024), npairs), dim3(256),
+                     0, s, P, claim);
+  auto k = chunk == 16 ? (codes ? sw_fill_batch<16, true> : sw_fill_batch<16, false>)
+         : chunk == 64 ? (codes ? sw_fill_batch<64, true> : sw_fill_batch<64, false>)
+                       : (codes ? sw_fill_batch<32, true> : sw_fill_batch<32, false>);
+  if (total_bands > 0) hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, P, band_pair, total_bands, claim);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sw_batch_finish(const SwPair* P, int npairs, bool trace, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(sw_pick_batch, dim3(npairs), dim3(1), 0, s, P);
+  if (trace) hipLaunchKernelGGL(sw_trace_batch, dim3(npairs), dim3(64), 0, s, P);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

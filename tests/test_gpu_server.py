@@ -132,3 +132,43 @@ def test_submit_errors(ctx):
     finally:
         srv2.close()
     ctx.set_rna_params(dpx.PAR_TURNER2004)  # allowed again once no server exists
+
+
+def test_alignments_batched_in_one_launch(ctx):
+    """Smith-Waterman requests that wait together run as one launch over the
+    bands of all of them (the alignment of each band from a table); every
+    result and trace equals the same pair aligned alone, also with empty
+    sequences, mixed scores and both trace starts; traced and untraced
+    requests never share a launch."""
+    import dpx
+    from dpx import gen
+    rnd = random.Random(77)
+    reqs = []
+    for q in range(40):
+        n1, n2 = rnd.randint(200, 500), rnd.randint(200, 500)
+        if q % 13 == 5:
+            n1 = 0
+        if q % 17 == 3:
+            n2 = 0
+        a, b = gen.dna_pair(n1, n2, 900 + q)
+        sc = (2, -3, -5, -2) if q % 3 else (10, -3, -3, -1)
+        reqs.append((a, b, sc, q % 2, q % 4 != 3))
+    print("batched alignments: %d requests" % len(reqs), flush=True)
+    with dpx.Server(ctx, hbm_budget=4 << 30, slots=1, max_batch=64) as srv:
+        ids = [srv.submit_swat(a, b, *sc, trace_from=tf, trace=tr) for a, b, sc, tf, tr in reqs]
+        got = srv.drain(ids)
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["max_batch_seen"] > 8 and st["launches"] < len(reqs) // 2
+    for (a, b, sc, tf, trace), rid in zip(reqs, ids):
+        res, tr = got[rid]
+        assert res.status == 0
+        s, best, want_tr = ctx.swat(a, b, *sc, trace_from=tf)
+        assert (res.score, (res.best, res.best_i, res.best_j)) == (s, best), (len(a), len(b), sc, tf)
+        assert tr == (want_tr if trace else []), (len(a), len(b), sc, tf)
+    # and two of them against the oracle
+    for (a, b, sc, tf, trace), rid in list(zip(reqs, ids))[:2]:
+        s, best, otr = o.swat(a, b, *sc, from_best=bool(tf))
+        res, tr = got[rid]
+        assert (res.score, (res.best, res.best_i, res.best_j)) == (s, best)
+        assert [t[:3] for t in tr] == ([t[:3] for t in otr] if trace else [])
