@@ -427,7 +427,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
     ap.add_argument("--serve-slots", type=int, default=6)
-    ap.add_argument("--serve-sweep", default="300,1000,2000,4000",
+    ap.add_argument("--serve-sweep", default="1000,4000,8000,12000,16000",
                     help="arrival rates (requests/s) of the saturation sweep; empty to skip")
     ap.add_argument("--serve-sweep-seconds", type=float, default=3.0)
     ap.add_argument("--serve-check-every", type=int, default=25)
