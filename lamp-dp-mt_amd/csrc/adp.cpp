@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <array>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -684,6 +685,51 @@ bool affine(const std::string& e, char v, int& c) {
   return v == 'i' ? ci == 1 && cj == 0 : cj == 1 && ci == 0;
 }
 
+// e == v + c for an index expression of names (i, j, split variables k0,
+// k1, ...), integers, + - and parentheses, where v is one name
+bool lin_is(const std::string& e, const std::string& v, int& c) {
+  std::map<std::string, long> co;
+  long k = 0;
+  size_t p = 0;
+  bool ok = true;
+  std::function<void(int)> expr = [&](int sign) {
+    int sg = sign;
+    for (;;) {
+      if (p >= e.size()) { ok = false; return; }
+      const char ch = e[p];
+      if (ch == '(') {
+        ++p;
+        expr(sg);
+        if (!ok || p >= e.size() || e[p] != ')') { ok = false; return; }
+        ++p;
+      } else if (isalpha((unsigned char)ch) || ch == '_') {
+        size_t q = p;
+        while (q < e.size() && (isalnum((unsigned char)e[q]) || e[q] == '_')) ++q;
+        co[e.substr(p, q - p)] += sg;
+        p = q;
+      } else if (isdigit((unsigned char)ch)) {
+        long x = 0;
+        while (p < e.size() && isdigit((unsigned char)e[p]) && x < (1L << 30)) x = 10 * x + (e[p++] - '0');
+        k += sg * x;
+      } else {
+        ok = false;
+        return;
+      }
+      if (!ok || p >= e.size() || e[p] == ')') return;
+      if (e[p] != '+' && e[p] != '-') { ok = false; return; }
+      sg = e[p] == '+' ? sign : -sign;
+      ++p;
+    }
+  };
+  expr(1);
+  if (!ok || p != e.size()) return false;
+  for (auto& x : co)
+    if (x.second != (x.first == v ? 1 : 0)) return false;
+  if (!co.count(v)) return false;
+  c = (int)k;
+  return true;
+}
+
 struct Loop {
   std::string var;
   std::vector<std::string> lo, hi;  // var from max(lo) to min(hi), ascending
@@ -723,6 +769,55 @@ struct Gen {
   // each step from the lanes' histories and the row above the band
   std::set<std::array<int, 3>>* regs = nullptr;  // (tab, -ci, -cj) referenced
   bool reg_ok = true;
+  // middle product of a one-track tile: the references of the alternative's
+  // split variable v read 32 x 32 blocks staged in LDS -- (i + a, v + b)
+  // the left operand L[row][split], (v + a, j + b) the right operand
+  // R[split][column] -- and (v + a, v + b) cells of finished tiles; inputs at
+  // v + c a staged window of the input.  Any other reference (the cell itself,
+  // (i + a, j + b), offsets beyond MIDM) leaves the alternative to the
+  // in-tile wavefront (mid_ok false).  need_lo / need_hi: how far inside
+  // the row block after the tile's rows and the column block before its
+  // columns the split points must stay for every reference to be a finished
+  // cell of another tile.
+  struct MRef { int t; bool left; int a, b; bool rules; };
+  std::vector<MRef>* mrefs = nullptr;
+  std::set<int>* minputs = nullptr;  // tracks whose input is staged
+  bool mid_ok = true;
+  int need_lo = 0, need_hi = 0;
+  std::string mvar;  // the split variable (first loop of the alternative)
+  static constexpr int MIDM = 16;  // largest offset of a reference from the split variable
+  int reg_mref(int t, bool left, int a, int b, bool rule) {
+    for (size_t x = 0; x < mrefs->size(); ++x) {
+      MRef& r = (*mrefs)[x];
+      if (r.t == t && r.left == left && r.a == a && r.b == b) {
+        r.rules = r.rules || rule;
+        return (int)x;
+      }
+    }
+    mrefs->push_back({t, left, a, b, rule});
+    return (int)mrefs->size() - 1;
+  }
+  std::string mref(int t, const std::string& a, const std::string& b, bool rule) {
+    int ca = 0, cb = 0;
+    const std::string arr = rule ? "R" : "";
+    if (!mvar.empty()) {
+      if (lin_is(a, "i", ca) && lin_is(b, mvar, cb) && ca >= 0 && ca <= MIDM && cb >= -MIDM && cb <= MIDM) {
+        need_hi = std::max(need_hi, cb);  // column split + b before the tile's columns
+        return "sm.mL" + arr + std::to_string(reg_mref(t, true, ca, cb, rule)) + "[_mr * 32 + _mkk]";
+      }
+      if (lin_is(a, mvar, ca) && lin_is(b, "j", cb) && ca >= -MIDM && ca <= MIDM && cb <= 0 && cb >= -MIDM) {
+        need_lo = std::max(need_lo, -ca);  // row split + a below the tile's rows
+        return "sm.mR" + arr + std::to_string(reg_mref(t, false, ca, cb, rule)) + "[_mkk * 32 + _mc]";
+      }
+      if (lin_is(a, mvar, ca) && lin_is(b, mvar, cb) && ca >= -MIDM && cb <= MIDM && ca <= cb) {
+        need_hi = std::max(need_hi, cb);  // rows from the tile's rows on, columns before its columns
+        return rule ? "((const GLOBAL_ short*)T.r[" + std::to_string(t) + "])[sidx_(z," + a + "," + b + ")]"
+                    : "((const GLOBAL_ answer_t*)T.c[" + std::to_string(t) + "])[sidx_(z," + a + "," + b + ")]";
+      }
+    }
+    mid_ok = false;
+    return "0";
+  }
   explicit Gen(const Model& mm, bool in_cell = false) : m(mm), cell(in_cell) {}
   std::string reg(const char* f, int t, const std::string& a, const std::string& b) {
     int ci = 0, cj = 0;
@@ -749,6 +844,7 @@ struct Gen {
   }
   std::string cost(int t, const std::string& a, const std::string& b) {
     if (self(a, b)) return "cv.c" + std::to_string(t);
+    if (mrefs) return mref(t, a, b, false);
     if (regs) return reg("v", t, a, b);
     if (pre) return hoist("answer_t", "((const GLOBAL_ answer_t*)T.c[" + std::to_string(t) + "])[sidx_(z," + a + "," + b + ")]");
     return "((const answer_t*)T.c[" + std::to_string(t) + "])[idx_(z," + a + "," + b + ")]";
@@ -756,6 +852,7 @@ struct Gen {
   std::string valid(int t, const std::string& a, const std::string& b) {
     if (!m.empty.empty()) return "!(" + cost(t, a, b) + "==(answer_t)(" + m.empty + "))";
     if (self(a, b)) return "(cv.r" + std::to_string(t) + "!=-1)";
+    if (mrefs) return "(" + mref(t, a, b, true) + "!=-1)";
     if (regs) return "(" + reg("r", t, a, b) + "!=-1)";
     if (pre) return "(" + hoist("short", "((const GLOBAL_ short*)T.r[" + std::to_string(t) + "])[sidx_(z," + a + "," + b + ")]") + "!=-1)";
     return "(T.r[" + std::to_string(t) + "][idx_(z," + a + "," + b + ")]!=-1)";
@@ -771,6 +868,10 @@ struct Gen {
     int c = 0;
     const std::string in = track == 1 ? "_in1" : "_in2";
     char v = 'i';
+    if (minputs && !mvar.empty() && lin_is(a, mvar, c) && c >= -MIDM && c <= MIDM) {
+      minputs->insert(track);
+      return "sm.mI" + std::to_string(track) + "[_mkk + " + std::to_string(c + MIDM) + "]";
+    }
     if (pre && (affine(a, 'i', c) || (v = 'j', affine(a, 'j', c))))
       return hoist("input_t", in + "[min(max(" + a + ", 0), z.n" + std::to_string(m.tracks == 1 ? 1 : track) + ")]",
                    std::to_string(track) + " " + v + " " + std::to_string(c));
@@ -849,6 +950,7 @@ struct Gen {
             k = add(a, iL);
           } else {
             var = "k" + std::to_string(nvar++);
+            if (mvar.empty()) mvar = var;
             Loop L;
             L.var = var;
             L.lo.push_back(add(a, iL));
@@ -1061,6 +1163,62 @@ bool generate(Model& m, std::string& src, std::string& err) {
       alts[t].push_back(o);
     }
   }
+  // one track: the alternatives whose split candidates strictly inside a
+  // tile's row and column blocks can be taken as a middle product (one split
+  // loop from i + small to j - small, every reference through the split
+  // variable); additive answers only when any order gives the same sum
+  std::vector<std::vector<AltCode>> malts(m.tabs.size());
+  std::vector<std::set<int>> mrule(m.tabs.size());
+  std::vector<Gen::MRef> mrefs;
+  std::set<int> minputs;
+  bool has_mid = false;
+  int mid_lo = 0, mid_hi = 0;  // the middle product's splits: [i0 + 32 + mid_lo, j0 - 1 - mid_hi]
+  for (int t : m.order) {
+    if (m.tracks != 1 || !m.any_order()) break;
+    const Tab& T = m.tabs[t];
+    std::vector<NP> flat;
+    if (!flatten(m, T.p, true, flat, err)) return false;
+    for (int q = 0; q < (int)flat.size(); ++q) {
+      for (int final_pass = 0; final_pass < 2; ++final_pass) {
+        std::vector<Gen::MRef> scratch;
+        std::set<int> scratch_in;
+        Gen g(m, false);
+        g.mrefs = final_pass ? &mrefs : &scratch;
+        g.minputs = final_pass ? &minputs : &scratch_in;
+        AltCode o;
+        o.rule = T.id + q;
+        std::vector<std::string> vals;
+        if (!g.gen(flat[q].get(), "i", "j", o, vals) || vals.size() != 1) break;
+        if (!g.mid_ok || o.loops.size() != 1 || (!final_pass && scratch.empty())) break;
+        const Loop& L = o.loops[0];
+        int cl = 0, ch = 0;
+        if (L.lo.size() != 1 || L.hi.size() != 1 || !lin_is(L.lo[0], "i", cl) || !lin_is(L.hi[0], "j", ch) ||
+            cl > Gen::MIDM + 1 || ch < -(Gen::MIDM + 1))
+          break;
+        if (final_pass) {
+          o.value = vals[0];
+          malts[t].push_back(o);
+          mrule[t].insert(o.rule);
+          has_mid = true;
+          // every cell of the tile has the whole range inside its loop bounds
+          mid_lo = std::max({mid_lo, g.need_lo, cl - 1});
+          mid_hi = std::max({mid_hi, g.need_hi, -ch - 1});
+        }
+      }
+    }
+  }
+  {
+    size_t lds = 0;
+    for (auto& r : mrefs) lds += 1024 * (size_t)m.answer_bytes + (r.rules ? 2048 : 0);
+    lds += minputs.size() * (32 + 2 * Gen::MIDM) * (size_t)m.alphabet_bytes;
+    if (lds > 48 * 1024) {  // too many operands to stage: the wavefront takes every split
+      for (auto& v : malts) v.clear();
+      for (auto& v : mrule) v.clear();
+      mrefs.clear();
+      minputs.clear();
+      has_mid = false;
+    }
+  }
   // third pass: the register form of the band kernel, when every reference
   // allows it
   std::vector<std::vector<AltCode>> alts_r(m.tabs.size());
@@ -1111,9 +1269,17 @@ bool generate(Model& m, std::string& src, std::string& err) {
     << (m.tracks == 1 ? "#define M_W (_x.n1 + 1)\n#define M_H (_x.n1 + 1)\n" : "#define M_H (_x.n1 + 1)\n#define M_W (_x.n2 + 1)\n")
     << (m.rna ? rna_functions() : std::string());
   if (m.tracks == 1)
-    s << "__device__ __forceinline__ long long idx_(const Dims& z, int i, int j) {\n"
-         "  const long long d = j - i;  // diagonal-major: the cells of one diagonal are contiguous\n"
-         "  return d * (z.n1 + 1) - d * (d - 1) / 2 + i;\n}\n";
+    // 32 x 32 tiles, each a contiguous row-major record (a tile row is one
+    // 128-byte line for 4-byte answers), tiles (I, J) numbered by tile
+    // diagonal J - I, then I; windowed grammars keep the tile diagonals that
+    // hold a cell of the window
+    s << "#define TB 32\n"
+         "__device__ __forceinline__ long long tile_ix_(int nb, int I, int J) {\n"
+         "  const long long D = J - I;\n"
+         "  return D * nb - D * (D - 1) / 2 + I;\n}\n"
+         "__device__ __forceinline__ long long idx_(const Dims& z, int i, int j) {\n"
+         "  const int nb = (z.n1 + TB) >> 5;\n"
+         "  return (tile_ix_(nb, i >> 5, j >> 5) << 10) + ((i & 31) << 5) + (j & 31);\n}\n";
   else
     // bands of 64 rows, each stored by anti-diagonal (r + j), lanes (rows)
     // contiguous: one step of the band pipeline writes 64 adjacent cells, and
@@ -1210,9 +1376,14 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "__device__ __forceinline__ void store_cell(const Dims& z, const Tabs& T, int i, int j, const CellV& cv) {\n"
        "  store_at(T, idx_(z, i, j), cv);\n}\n";
   auto emit_tabs = [&](const std::vector<std::vector<AltCode>>& A, const std::vector<std::vector<std::array<std::string, 4>>>& PD,
-                       const std::string& sx, const std::string& extra) {
+                       const std::string& sx, const std::string& extra, bool tile) {
   for (int t : m.order) {
     const Tab& T = m.tabs[t];
+    // tile: the in-tile form of a one-track cell -- alternatives with a
+    // middle product skip the split points [_mlo, _mhi] (the tile's middle
+    // product took them) and the cell's middle-product partial, left in its
+    // table entry, joins the lanes' merge
+    if (!tile) {
     s << "// ---- " << T.name << "[i,j]: rules " << T.id << ".." << T.id + T.alt - 1 << "\n"
       << "struct Pre" << sx << "_" << t << " {";
     for (auto& d : PD[t]) s << " " << d[0] << " " << d[1] << ";";
@@ -1224,8 +1395,9 @@ bool generate(Model& m, std::string& src, std::string& err) {
     s << "  p.pad_ = 0;\n}\n"
       << "__device__ __forceinline__ void pin" << sx << "_" << t << "(const Pre" << sx << "_" << t << "& p) {";
     for (auto& d : PD[t]) s << " pin_(p." << d[1] << ");";
-    s << " (void)p; }\n"
-      << "template <int G, bool DEFER> __device__ __forceinline__ void tab" << sx << "_" << t
+    s << " (void)p; }\n";
+    }
+    s << "template <int G, bool DEFER> __device__ __forceinline__ void tab" << (tile ? "T" : sx) << "_" << t
       << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, int lane, CellV& cv, const Pre" << sx << "_" << t << "& _p" << extra
       << ") {\n"
       << "  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n"
@@ -1269,10 +1441,19 @@ bool generate(Model& m, std::string& src, std::string& err) {
         ++close;
       };
       open_if(0);
+      const bool split_mid = tile && mrule[t].count(o.rule);
       for (int q = 0; q < NL; ++q) {
         const Loop& L = o.loops[q];
         ind += "  ";
-        if (q == 0)
+        if (q == 0 && split_mid) {
+          // two runs: below and above the middle product's split points
+          s << ind << "for (int _sg = 0; _sg < 2; ++_sg) {\n";
+          ++close;
+          ind += "  ";
+          s << ind << "const int _lo = " << bound(L.lo, "max") << ", _hi = " << bound(L.hi, "min") << ";\n"
+            << ind << "for (int " << L.var << " = (_sg ? max(_lo, _mhi + 1) : _lo) + lane, " << L.var
+            << "_u = _sg ? _hi : min(_hi, _mlo - 1); " << L.var << " <= " << L.var << "_u; " << L.var << " += G) {\n";
+        } else if (q == 0)
           s << ind << "for (int " << L.var << " = " << bound(L.lo, "max") << " + lane, " << L.var << "_u = " << bound(L.hi, "min")
             << "; " << L.var << " <= " << L.var << "_u; " << L.var << " += G) {\n";
         else
@@ -1319,8 +1500,41 @@ bool generate(Model& m, std::string& src, std::string& err) {
         << "    else take = r < _r0 || (r == _r0 && (o0 < _o0 || (o0 == _o0 && o1 < _o1)));\n";
     s <<
          "    if (take) { _c0 = c; _r0 = r; _p0 = p0; _p1 = p1; _o0 = o0; _o1 = o1; }\n"
-         "  }\n"
-         "  (void)_o0; (void)_o1;\n"
+         "  }\n";
+    if (tile && !mrule[t].empty()) {
+      // the middle product's partial (value, rule, splits) of this cell,
+      // merged as one more lane; its outer split is the split variable
+      s << "  if (_mid) {\n"
+           "    const long long x = idx_(z, i, j);\n"
+        << "    const answer_t c = ((const answer_t*)T.c[" << t << "])[x];\n"
+        << "    const int r = T.r[" << t << "][x];\n"
+        << "    const int p0 = " << (T.cat > 0 ? "T.p[" + std::to_string(t) + "][0][x]" : std::string("0")) << ";\n"
+        << "    const int p1 = " << (T.cat > 1 ? "T.p[" + std::to_string(t) + "][1][x]" : std::string("0")) << ";\n"
+        << "    int o0 = 0;\n"
+           "    switch (r) {\n";
+      for (auto& o : malts[t])
+        for (size_t q = 0; q < o.var_of_pos.size(); ++q)
+          if (o.var_of_pos[q] == o.loops[0].var) s << "      case " << o.rule << ": o0 = p" << q << "; break;\n";
+      s << "      default: break;\n"
+           "    }\n"
+           "    bool take;\n"
+           "    if (r == -1) take = false;\n"
+           "    else if (_r0 == -1) take = true;\n";
+      if (m.additive())
+        s << "    else {\n"
+             "      _c0 = answer_t(_c0 + c);\n"
+             "      take = r < _r0 || (r == _r0 && (o0 < _o0 || (o0 == _o0 && 0 < _o1)));\n"
+             "      if (take) { _r0 = r; _p0 = p0; _p1 = p1; _o0 = o0; _o1 = 0; }\n"
+             "      take = false;\n"
+             "    }\n";
+      else
+        s << "    else if (" << better("c", "_c0") << ") take = true;\n"
+          << "    else if (" << better("_c0", "c") << ") take = false;\n"
+          << "    else take = r < _r0 || (r == _r0 && (o0 < _o0 || (o0 == _o0 && 0 < _o1)));\n";
+      s << "    if (take) { _c0 = c; _r0 = r; _p0 = p0; _p1 = p1; _o0 = o0; _o1 = 0; }\n"
+           "  }\n";
+    }
+    s << "  (void)_o0; (void)_o1;\n"
       << "  cv.c" << t << " = _c0;\n  cv.r" << t << " = (short)_r0;\n";
     for (int q = 0; q < T.cat; ++q) s << "  cv.p" << t << "_" << q << " = _p" << q << ";\n";
     s << "  if (!DEFER && lane == 0) {\n"
@@ -1334,7 +1548,8 @@ bool generate(Model& m, std::string& src, std::string& err) {
     s << "}\n";
   }
   };
-  emit_tabs(alts, pre_decl, "", "");
+  emit_tabs(alts, pre_decl, "", "", false);
+  if (m.tracks == 1) emit_tabs(alts, pre_decl, "", ", bool _mid, int _mlo, int _mhi", true);
   if (m.reg_band) {
     // what the register band kernel's cells read: R.v<t>_<di>_<dj> (value)
     // and R.r<t>_<di>_<dj> (rule) of cell (i - di, j - dj)
@@ -1342,7 +1557,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
     for (auto& k : rset)
       s << " answer_t v" << k[0] << "_" << k[1] << "_" << k[2] << "; short r" << k[0] << "_" << k[1] << "_" << k[2] << ";";
     s << " int pad_; };\n";
-    emit_tabs(alts_r, pre_decl_r, "R", ", const Regs& R");
+    emit_tabs(alts_r, pre_decl_r, "R", ", const Regs& R", false);
   }
   // fill: one diagonal per launch (d0 == d1 - 1), or a run of small
   // diagonals by one workgroup with a barrier between them
@@ -1378,6 +1593,246 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "  }\n}\n";
   for (int G : {1, 8, 64})
     s << "extern \"C\" __global__ void adp_fill_" << G << "(const Inst* I, int d0, int d1) { fill_<" << G << ">(I, d0, d1); }\n";
+  if (m.tracks == 1) {
+    const int MIDM = Gen::MIDM;
+    const std::string init = m.empty.empty() ? "answer_t{}" : "(answer_t)(" + m.empty + ")";
+    // staged operands of the middle product
+    s << "#define MIDM " << MIDM << "\n"
+         "struct MidSm {";
+    for (size_t x = 0; x < mrefs.size(); ++x) {
+      const std::string side = mrefs[x].left ? "mL" : "mR";
+      s << " answer_t " << side << x << "[1024];";
+      if (mrefs[x].rules) s << " short " << side << "R" << x << "[1024];";
+    }
+    for (int tr : minputs) s << " input_t mI" << tr << "[" << 32 + 2 * MIDM << "];";
+    s << " int pad_; };\n";
+    if (has_mid) {
+      // Middle product of tile (i0, j0): for every cell and alternative with
+      // a middle product, the best candidate over the split points
+      // [KLO, KHI] -- blocks of 32 splits whose left operand rows i0.. and
+      // right operand columns j0.. are staged in LDS by the whole workgroup;
+      // each thread owns CPT cells of one column (a wave: two rows of 32
+      // cells, so a left-operand read is a broadcast and a right-operand
+      // read 32 adjacent words).  Within an alternative the splits run in
+      // increasing order, so "strictly better" keeps the first; the
+      // alternatives then merge in rule order, and the partial goes to the
+      // cell's table entry for the in-tile pass.
+      s << "template <int CPT> __device__ __forceinline__ void middle_(const Ctx& _x, const Dims& z, const Tabs& T, int D, int i0,\n"
+           "                                                        int j0, int KLO, int KHI, MidSm& sm) {\n"
+           "  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n"
+           "  constexpr int RS = 32 / CPT, NT_ = 32 * RS;\n"
+           "  const int tid = threadIdx.x, _mc = tid & 31, rb = tid >> 5;\n"
+           "  bool ok[CPT];\n";
+      for (int t : m.order)
+        for (auto& o : malts[t]) s << "  answer_t bc" << o.rule << "[CPT];\n  int bk" << o.rule << "[CPT];\n";
+      s << "#pragma unroll\n"
+           "  for (int u = 0; u < CPT; ++u) {\n"
+           "    const int i = i0 + rb + RS * u, j = j0 + _mc;\n"
+           "    ok[u] = j <= z.n1 && j - i <= D;\n";
+      for (int t : m.order)
+        for (auto& o : malts[t]) s << "    bc" << o.rule << "[u] = " << init << ";\n    bk" << o.rule << "[u] = -1;\n";
+      s << "  }\n"
+           "  for (int K = KLO >> 5; K <= (KHI >> 5); ++K) {\n"
+           "    __syncthreads();  // the last block's operands are read\n"
+           "    for (int e = tid; e < 1024; e += NT_) {\n"
+           "      const int hi_ = e >> 5, lo_ = e & 31;\n";
+      for (size_t x = 0; x < mrefs.size(); ++x) {
+        const auto& r = mrefs[x];
+        const std::string side = r.left ? "mL" : "mR", t = std::to_string(r.t);
+        const std::string ix = r.left ? "sidx_(z, i0 + hi_ + " + std::to_string(r.a) + ", (K << 5) + lo_ + " + std::to_string(r.b) + ")"
+                                      : "sidx_(z, (K << 5) + hi_ + " + std::to_string(r.a) + ", j0 + lo_ + " + std::to_string(r.b) + ")";
+        s << "      {\n        const long long a_ = " << ix << ";\n"
+          << "        sm." << side << x << "[e] = ((const GLOBAL_ answer_t*)T.c[" << t << "])[a_];\n";
+        if (r.rules) s << "        sm." << side << "R" << x << "[e] = ((const GLOBAL_ short*)T.r[" << t << "])[a_];\n";
+        s << "      }\n";
+      }
+      s << "    }\n";
+      for (int tr : minputs)
+        s << "    if (tid < " << 32 + 2 * MIDM << ") sm.mI" << tr << "[tid] = ((const GLOBAL_ input_t*)_in" << tr
+          << ")[min(max((K << 5) - MIDM + tid, 0), z.n1)];\n";
+      s << "    __syncthreads();\n"
+           "    const int klo = max(KLO, K << 5), khi = min(KHI, (K << 5) + 31);\n"
+           "    for (int k = klo; k <= khi; ++k) {\n"
+           "      const int _mkk = k - (K << 5);\n"
+           "#pragma unroll\n"
+           "      for (int u = 0; u < CPT; ++u) {\n"
+           "        const int _mr = rb + RS * u, i = i0 + _mr, j = j0 + _mc;\n"
+           "        (void)i; (void)j; (void)_mr;\n"
+           "        if (ok[u]) {\n";
+      for (int t : m.order)
+        for (auto& o : malts[t]) {
+          std::vector<std::string> cond = o.arith;
+          for (auto& c : o.other) cond.push_back(c);
+          const std::string R = std::to_string(o.rule);
+          s << "          {  // rule " << o.rule << "\n"
+            << "            const int " << o.loops[0].var << " = k;\n";
+          if (!cond.empty()) s << "            if (" << join(cond, " && ") << ")\n";
+          s << "            {\n";
+          if (m.h.k == A_SUM)
+            s << "              const answer_t _c = " << o.value << ";\n"
+              << "              if (bk" << R << "[u] < 0) { bc" << R << "[u] = answer_t(answer_t(0) + _c); bk" << R << "[u] = k; }\n"
+              << "              else bc" << R << "[u] = answer_t(bc" << R << "[u] + _c);\n";
+          else if (m.h.k == A_COUNT)
+            s << "              if (bk" << R << "[u] < 0) { bc" << R << "[u] = answer_t(1); bk" << R << "[u] = k; }\n"
+              << "              else bc" << R << "[u] = answer_t(bc" << R << "[u] + 1);\n";
+          else
+            s << "              const answer_t _c = " << o.value << ";\n"
+              << "              if (" << better("_c", "bc" + R + "[u]") << (m.empty.empty() ? " || bk" + R + "[u] < 0" : "")
+              << ") { bc" << R << "[u] = _c; bk" << R << "[u] = k; }\n";
+          s << "            }\n"
+               "          }\n";
+        }
+      s << "        }\n"
+           "      }\n"
+           "    }\n"
+           "  }\n"
+           "#pragma unroll\n"
+           "  for (int u = 0; u < CPT; ++u) {\n"
+           "    const int i = i0 + rb + RS * u, j = j0 + _mc;\n"
+           "    if (!ok[u]) continue;\n"
+           "    const long long x = idx_(z, i, j);\n";
+      for (int t : m.order) {
+        if (malts[t].empty()) continue;
+        const Tab& T = m.tabs[t];
+        s << "    {\n      answer_t pc = " << init << ";\n      int pr = -1, pp0 = 0, pp1 = 0;\n";
+        for (auto& o : malts[t]) {
+          const std::string R = std::to_string(o.rule);
+          const std::string setp = "const int " + o.loops[0].var + " = bk" + R + "[u]; pr = " + R + "; pp0 = " +
+                                   (o.pos.size() > 0 ? o.pos[0] : "0") + "; pp1 = " + (o.pos.size() > 1 ? o.pos[1] : "0") + ";";
+          s << "      if (bk" << R << "[u] >= 0) {\n";
+          if (m.additive())
+            s << "        if (pr == -1) { pc = bc" << R << "[u]; " << setp << " } else pc = answer_t(pc + bc" << R << "[u]);\n";
+          else
+            s << "        if (pr == -1 || " << better("bc" + R + "[u]", "pc") << ") { pc = bc" << R << "[u]; " << setp << " }\n";
+          s << "      }\n";
+        }
+        s << "      ((answer_t*)T.c[" << t << "])[x] = pc;\n      T.r[" << t << "][x] = (short)pr;\n";
+        for (int q = 0; q < T.cat; ++q) s << "      T.p[" << t << "][" << q << "][x] = pp" << q << ";\n";
+        s << "      (void)pp0; (void)pp1;\n    }\n";
+      }
+      s << "  }\n}\n";
+    }
+    // One persistent launch fills every tile of every instance.  Tiles wait
+    // in one queue of 64-bit items (instance, I, J): the diagonal tiles are
+    // seeded, and tile (I, J) is appended by whichever of (I, J - 1) and
+    // (I + 1, J) finishes second -- between them they cover every cell a cell
+    // of the tile reads outside the tile.  A workgroup claims the next slot
+    // and waits (bounded) until it holds a tile; a tile's stores are
+    // published by one device-scope release before its successors' counts.
+    s << "struct TInf { long long tile0; int nb, DT; };\n"
+         "__device__ __forceinline__ unsigned long long uni64_(unsigned long long v) {\n"
+         "  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);\n"
+         "  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));\n"
+         "  return ((unsigned long long)hi << 32) | lo;\n}\n"
+         "extern \"C\" __global__ void adp_tile_reset(const int* __restrict__ seed, int count, int ntiles,\n"
+         "                                            unsigned long long* queue, int* ctrl, int* cnt) {\n"
+         "  const int seeds = seed[count];\n"
+         "  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ntiles; s += gridDim.x * blockDim.x) {\n"
+         "    cnt[s] = 0;\n"
+         "    unsigned long long it = ~0ull;\n"
+         "    if (s < seeds) {  // the diagonal tiles, instance by instance\n"
+         "      int lo = 0, hi = count;\n"
+         "      while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (seed[mid] <= s) lo = mid; else hi = mid; }\n"
+         "      const unsigned long long d = (unsigned long long)(s - seed[lo]);\n"
+         "      it = ((unsigned long long)lo << 40) | (d << 20) | d;\n"
+         "    }\n"
+         "    queue[s] = it;\n"
+         "  }\n"
+         "  if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl[0] = 0; ctrl[1] = seeds; ctrl[2] = 0; }\n"
+         "}\n";
+    s << "template <int NT_> __device__ __forceinline__ void tile_(const Inst& q, int TI_, int TJ_, MidSm& sm) {\n"
+         "  const Dims z = q.z;\n  const Tabs T = q.T;\n"
+         "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl};\n"
+      << "  const int D = " << (m.window > 0 ? "min(" + std::to_string(m.window) + ", z.n1)" : std::string("z.n1")) << ";\n"
+      << "  const int i0 = TI_ << 5, j0 = TJ_ << 5;\n"
+      << "  const int KLO = i0 + TB + " << mid_lo << ", KHI = j0 - 1 - " << mid_hi << ";\n"
+      << "  const bool _mid = " << (has_mid ? "TJ_ - TI_ >= 2 && KLO <= KHI" : "false") << ";\n"
+      // no middle product: an empty range far above every split (n < 2^30),
+      // so _mhi + 1 + lane cannot overflow
+      << "  const int _mlo = _mid ? KLO : (1 << 30), _mhi = _mid ? KHI : (1 << 30) - 1;\n";
+    if (has_mid)
+      s << "  if (_mid) {\n"
+           "    middle_<1024 / NT_>(_x, z, T, D, i0, j0, _mlo, _mhi, sm);\n"
+           "    __syncthreads();  // the partials are in the table entries\n"
+           "  }\n";
+    else
+      s << "  (void)sm;\n";
+    // the in-tile wavefront: tile diagonal e holds cells (r, r + e); a cell
+    // reads cells of smaller e only, so one barrier per diagonal
+    s << "  constexpr int G = " << (m.any_order() ? "NT_ / TB" : "1") << ";\n"
+         "  const int grp = threadIdx.x / G, lane = threadIdx.x & (G - 1);\n"
+         "  for (int e = TI_ == TJ_ ? 0 : 1 - TB; e < TB; ++e) {\n"
+         "    const int r = grp + (e < 0 ? -e : 0), c = r + e;\n"
+         "    const int i = i0 + r, j = j0 + c;\n"
+         "    if (r < TB && c < TB && j <= z.n1 && j - i <= D) {\n";
+    s << "      CellV cv;\n";
+    for (int t : m.order) s << "      cv.r" << t << " = -1;\n";
+    for (int t : m.order) s << "      Pre_" << t << " p" << t << ";\n      load_" << t << "(_x, z, T, i, j, p" << t << ");\n";
+    for (int t : m.order) s << "      pin_" << t << "(p" << t << ");\n";
+    for (int t : m.order)
+      s << "      tabT_" << t << "<G, false>(_x, z, T, i, j, lane, cv, p" << t << ", _mid, _mlo, _mhi);\n";
+    s << "    }\n"
+         "    __syncthreads();\n"
+         "  }\n"
+         "}\n";
+    s << "template <int NT_> __device__ __forceinline__ void tile_run_(const Inst* __restrict__ I, const TInf* __restrict__ TI,\n"
+         "                                                          int ntiles, unsigned long long* queue, int* ctrl, int* cnt) {\n"
+         "  __shared__ MidSm sm;\n"
+         "  __shared__ unsigned long long s_item;\n"
+         "  for (;;) {\n"
+         "    if (threadIdx.x < 64) {  // wave 0 claims, with wave-uniform control flow\n"
+         "      int slot = 0;\n"
+         "      if (threadIdx.x == 0) slot = __hip_atomic_fetch_add(&ctrl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "      slot = __builtin_amdgcn_readfirstlane(slot);\n"
+         "      unsigned long long it = ~0ull;\n"
+         "      if (slot < ntiles) {\n"
+         "        it = uni64_(__hip_atomic_load(&queue[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "        for (int spin = 0; it == ~0ull; ++spin) {\n"
+         "          if (spin > (1 << 23)) {  // a scheduling fault ends the launch with a flag, not a hang\n"
+         "            if (threadIdx.x == 0) __hip_atomic_store(&ctrl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "            break;\n"
+         "          }\n"
+         "          __builtin_amdgcn_s_sleep(4);\n"
+         "          it = uni64_(__hip_atomic_load(&queue[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));\n"
+         "        }\n"
+         "        // no copy of another tile's lines cached here, from this launch\n"
+         "        // or an earlier one, may be read in place of what its producer wrote\n"
+         "        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"agent\");\n"
+         "      }\n"
+         "      if (threadIdx.x == 0) s_item = it;\n"
+         "    }\n"
+         "    __syncthreads();\n"
+         "    const unsigned long long it = s_item;\n"
+         "    __syncthreads();\n"
+         "    if (it == ~0ull) break;\n"
+         "    const int q = (int)(it >> 40), TI_ = (int)((it >> 20) & 0xfffff), TJ_ = (int)(it & 0xfffff);\n"
+         "    tile_<NT_>(I[q], TI_, TJ_, sm);\n"
+         "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"workgroup\");\n"
+         "    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are done\n"
+         "    __syncthreads();\n"
+         "    if (threadIdx.x == 0) {\n"
+         "      __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"agent\");\n"
+         "      const TInf f = TI[q];\n"
+         "      for (int s2 = 0; s2 < 2; ++s2) {  // successors (I - 1, J) and (I, J + 1)\n"
+         "        const int I2 = s2 == 0 ? TI_ - 1 : TI_, J2 = s2 == 0 ? TJ_ : TJ_ + 1;\n"
+         "        if (I2 < 0 || J2 >= f.nb || J2 - I2 > f.DT) continue;\n"
+         "        const long long g = f.tile0 + tile_ix_(f.nb, I2, J2);\n"
+         "        const int old = __hip_atomic_fetch_add(&cnt[g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "        if (old == 1) {\n"
+         "          const int pos = __hip_atomic_fetch_add(&ctrl[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "          __hip_atomic_store(&queue[pos], ((unsigned long long)q << 40) | ((unsigned long long)I2 << 20) | (unsigned long long)J2,\n"
+         "                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+         "        }\n"
+         "      }\n"
+         "    }\n"
+         "    __syncthreads();\n"
+         "  }\n"
+         "}\n"
+         "extern \"C\" __global__ __launch_bounds__(256) void adp_tile_256(const Inst* I, const TInf* TI, int ntiles,\n"
+         "    unsigned long long* queue, int* ctrl, int* cnt) { tile_run_<256>(I, TI, ntiles, queue, ctrl, cnt); }\n"
+         "extern \"C\" __global__ __launch_bounds__(1024) void adp_tile_1024(const Inst* I, const TInf* TI, int ntiles,\n"
+         "    unsigned long long* queue, int* ctrl, int* cnt) { tile_run_<1024>(I, TI, ntiles, queue, ctrl, cnt); }\n";
+  }
   if (m.reg_band) {
     // The band pipeline below with the band's own cells in registers: every
     // reference is (i, j - dj) or (i - 1, j - dj) with dj small.  Lane r
@@ -1672,6 +2127,9 @@ struct Grammar {
   hipModule_t mod = nullptr;
   hipFunction_t fill[3] = {nullptr, nullptr, nullptr}, finish = nullptr;  // fill: 1, 8, 64 lanes per cell
   hipFunction_t band = nullptr;  // two tracks without split loops: the band pipeline
+  // one track: the tile queue (reset; persistent fill with 256 or 1024 threads per tile)
+  hipFunction_t tile_reset = nullptr, tile[2] = {nullptr, nullptr};
+  int cus = 0, occ[2] = {0, 0};  // compute units; resident tile workgroups per CU
   void* dev = nullptr;
   size_t dev_cap = 0;
   void* host = nullptr;  // pinned: answer, out[4], trace
@@ -1718,10 +2176,26 @@ Grammar* compile(const char* spec, bool load, std::string& err, int* code) {
         hipModuleGetFunction(&g->fill[1], g->mod, "adp_fill_8") != hipSuccess ||
         hipModuleGetFunction(&g->fill[2], g->mod, "adp_fill_64") != hipSuccess ||
         hipModuleGetFunction(&g->finish, g->mod, "adp_finish") != hipSuccess ||
-        (g->m.tracks == 2 && !g->m.loops && hipModuleGetFunction(&g->band, g->mod, "adp_band") != hipSuccess)) {
+        (g->m.tracks == 2 && !g->m.loops && hipModuleGetFunction(&g->band, g->mod, "adp_band") != hipSuccess) ||
+        (g->m.tracks == 1 && (hipModuleGetFunction(&g->tile_reset, g->mod, "adp_tile_reset") != hipSuccess ||
+                              hipModuleGetFunction(&g->tile[0], g->mod, "adp_tile_256") != hipSuccess ||
+                              hipModuleGetFunction(&g->tile[1], g->mod, "adp_tile_1024") != hipSuccess))) {
       err = "loading the compiled grammar failed";
       if (g->mod) (void)hipModuleUnload(g->mod);
       return nullptr;
+    }
+    if (g->m.tracks == 1) {
+      int dev = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&g->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ[0], g->tile[0], 256, 0) != hipSuccess ||
+          hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ[1], g->tile[1], 1024, 0) != hipSuccess) {
+        err = "querying the tile kernels failed";
+        (void)hipModuleUnload(g->mod);
+        return nullptr;
+      }
+      g->cus = std::max(g->cus, 1);
+      for (int& o : g->occ) o = std::max(o, 1);
     }
     if (hipEventCreate(&g->ev[0]) != hipSuccess || hipEventCreate(&g->ev[1]) != hipSuccess) {
       err = "event creation failed";
@@ -1776,7 +2250,7 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
   const int NT = (int)m.tabs.size();
   const size_t ab = (size_t)m.alphabet_bytes, ans4 = ((size_t)m.answer_bytes + 3) & ~(size_t)3;
   const size_t inst_bytes = 72 + 32 * (size_t)NT;
-  struct Q { int64_t n1, n2, D, cells; size_t o_in1, o_in2, o_tab, o_ans, o_tr; int64_t cap; };
+  struct Q { int64_t n1, n2, D, cells, nb = 0, DT = 0, tiles = 0; size_t o_in1, o_in2, o_tab, o_ans, o_tr; int64_t cap; };
   std::vector<Q> qs(count);
   size_t off = r256(inst_bytes * count);
   int64_t n1max = 0, cmax = 0, Dmax = 0;
@@ -1789,7 +2263,15 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
     if (x.n2 > 0 && !in2) { err = "null input"; return DPX_EINVAL; }
     const int64_t W1 = x.n1 + 1;
     x.D = m.tracks == 1 ? (m.window > 0 ? std::min<int64_t>(m.window, x.n1) : x.n1) : x.n1 + x.n2;
-    x.cells = m.tracks == 1 ? (x.D + 1) * W1 - (x.D + 1) * x.D / 2 : (x.n1 + 64) / 64 * 64 * (x.n2 + 64);
+    (void)W1;
+    if (m.tracks == 1) {  // 32 x 32 tiles over the triangle (and the window)
+      x.nb = (x.n1 + 32) / 32;
+      x.DT = std::min<int64_t>(x.nb - 1, (x.D + 31) / 32);
+      x.tiles = (x.DT + 1) * x.nb - x.DT * (x.DT + 1) / 2;
+      x.cells = x.tiles * 1024;
+    } else {
+      x.cells = (x.n1 + 64) / 64 * 64 * (x.n2 + 64);
+    }
     if (x.cells > ((int64_t)1 << 34)) { err = "grammar tables too large"; return DPX_EINVAL; }
     // one element of padding past each input: hoisted reads clamp their index to n
     x.o_in1 = off; off += r256((size_t)(x.n1 + 1) * ab + 16);
@@ -1831,6 +2313,37 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
       o_band = off;
       off += r256(band_blk.size() * 4);
     }
+  }
+  // tile queue (one track): seeds and per-instance tile records (staged on
+  // the host), then queue, dependency counts and control words (set by the
+  // reset kernel, so the call holds no memset and can sit in a captured graph)
+  std::vector<char> tile_blk;
+  size_t o_tsched = 0, o_tinf = 0, o_queue = 0, o_cnt = 0, o_tctrl = 0;
+  int64_t ntiles = 0;
+  if (m.tracks == 1) {
+    for (auto& x : qs) ntiles += x.tiles;
+    if (ntiles >= ((int64_t)1 << 30) || count >= (1 << 23)) { err = "too many tiles in one call"; return DPX_EINVAL; }
+    const size_t seed_bytes = r256((size_t)(count + 1) * 4);
+    tile_blk.assign(seed_bytes + (size_t)count * 16, 0);
+    int32_t* seeds = (int32_t*)tile_blk.data();
+    int64_t tb = 0, sb = 0;
+    for (int k = 0; k < count; ++k) {
+      const Q& x = qs[order[k]];
+      seeds[k] = (int32_t)sb;
+      sb += x.nb;
+      char* rec = tile_blk.data() + seed_bytes + (size_t)k * 16;
+      const int32_t nd[2] = {(int32_t)x.nb, (int32_t)x.DT};
+      std::memcpy(rec, &tb, 8);
+      std::memcpy(rec + 8, nd, 8);
+      tb += x.tiles;
+    }
+    seeds[count] = (int32_t)sb;
+    o_tsched = off;
+    o_tinf = off + seed_bytes;
+    off += r256(tile_blk.size());
+    o_queue = off; off += r256((size_t)ntiles * 8);
+    o_cnt = off; off += r256((size_t)ntiles * 4);
+    o_tctrl = off; off += 256;
   }
   if (off > g->dev_cap) {
     if (g->dev) (void)hipFree(g->dev);
@@ -1923,7 +2436,30 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
     if (!chk(hipModuleLaunchKernel(g->band, grid, 1, 1, 64, 1, 1, 0, s, bargs, nullptr), "band launch")) return DPX_EHIP;
     if (!chk(hipMemcpyAsync(h_stall, d_ctrl + 1, 4, hipMemcpyDeviceToHost, s), "band status")) return DPX_EHIP;
   }
-  int active = band_blk.empty() ? count : 0;
+  if (m.tracks == 1) {
+    if (!chk(hipMemcpyAsync(dev + o_tsched, tile_blk.data(), tile_blk.size(), hipMemcpyHostToDevice, s), "tile setup")) return DPX_EHIP;
+    const int* d_seed = (const int*)(dev + o_tsched);
+    const void* d_tinf = dev + o_tinf;
+    unsigned long long* d_queue = (unsigned long long*)(dev + o_queue);
+    int* d_cnt = (int*)(dev + o_cnt);
+    int* d_tctrl = (int*)(dev + o_tctrl);
+    int cnt = count, nt = (int)ntiles;
+    void* rargs[] = {(void*)&d_seed, (void*)&cnt, (void*)&nt, (void*)&d_queue, (void*)&d_tctrl, (void*)&d_cnt};
+    const unsigned rgrid = (unsigned)std::min<int64_t>((ntiles + 255) / 256, 4096);
+    if (!chk(hipModuleLaunchKernel(g->tile_reset, rgrid, 1, 1, 256, 1, 1, 0, s, rargs, nullptr), "tile reset")) return DPX_EHIP;
+    // 1024 threads per tile when few tiles can be ready at once (each tile
+    // then runs fastest); 256 when the launch has many more ready tiles than
+    // compute units
+    int64_t ready = 0;
+    for (auto& x : qs) ready += x.nb;
+    int big = ready <= 2 * (int64_t)g->cus ? 1 : 0;
+    if (const char* e = std::getenv("DPX_ADP_TILE_THREADS")) big = std::atoi(e) == 1024 ? 1 : std::atoi(e) == 256 ? 0 : big;
+    const unsigned grid = (unsigned)std::min<int64_t>(ntiles, (int64_t)g->cus * g->occ[big]);
+    void* targs[] = {(void*)&d_inst, (void*)&d_tinf, (void*)&nt, (void*)&d_queue, (void*)&d_tctrl, (void*)&d_cnt};
+    if (!chk(hipModuleLaunchKernel(g->tile[big], grid, 1, 1, big ? 1024 : 256, 1, 1, 0, s, targs, nullptr), "tile fill")) return DPX_EHIP;
+    if (!chk(hipMemcpyAsync(h_stall, d_tctrl + 2, 4, hipMemcpyDeviceToHost, s), "tile status")) return DPX_EHIP;
+  }
+  int active = band_blk.empty() && m.tracks == 2 ? count : 0;
   for (int64_t d = 0; d <= Dmax && active > 0;) {
     while (active > 0 && qs[order[active - 1]].D < d) --active;
     const int64_t c = cnt_bound(d);
@@ -1960,6 +2496,7 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
     if (!chk(hipMemcpyAsync(h_res + (ans4 + 16) * q, dev + qs[q].o_ans, ans4 + 16, hipMemcpyDeviceToHost, s), "result copy")) return DPX_EHIP;
   if (!chk(hipStreamSynchronize(s), "grammar run")) return DPX_EHIP;
   if (!band_blk.empty() && *h_stall) { err = "band pipeline stalled (a band waited too long for the band above)"; return DPX_EHIP; }
+  if (m.tracks == 1 && *h_stall) { err = "tile queue stalled (a workgroup waited too long for a tile)"; return DPX_EHIP; }
   int rc = DPX_OK;
   int64_t tro = 0;
   for (int q = 0; q < count; ++q) {
