@@ -842,20 +842,95 @@ struct Gen {
     if (!fa || !fb) fixed = false;
     return cell && fa && fb && ca == 0 && cb == 0;
   }
+  // in-tile form of a one-track cell: other cells through the accessors of
+  // the tile's LDS region (which fall back to the table in memory)
+  bool tile = false;
+  std::set<int>* tile_rules = nullptr;  // tabulations whose rule entries the region needs
+  // Reads known to land in the region or a strip index LDS directly (no
+  // range tests, no fallback load): set once the region and strips are
+  // decided.  t_reg_h: the region's halo (32 for these paths); t_sl / t_sr:
+  // tabulations with a left / right strip; t_mid: rules with a middle
+  // product and its margins t_ml, t_mh.
+  int t_reg_h = -1;
+  const std::set<int>* t_sl = nullptr;
+  const std::set<int>* t_sr = nullptr;
+  const std::set<int>* t_mid = nullptr;
+  int t_ml = 0, t_mh = 0;
+  AltCode* cur = nullptr;  // the alternative being generated
+  // e == base + [lo, hi] (base "i" or "j"), from e itself or the bounds of
+  // the alternative's loop variable it is an offset of
+  bool span(const std::string& e, const char* base, int& lo, int& hi) const {
+    int c = 0;
+    if (lin_is(e, base, c)) { lo = hi = c; return true; }
+    if (!cur) return false;
+    for (auto& L : cur->loops) {
+      if (!lin_is(e, L.var, c)) continue;
+      bool fl = false, fh = false;
+      int l = 0, h = 0, x = 0;
+      for (auto& it : L.lo)
+        if (lin_is(it, base, x)) { l = fl ? std::max(l, x) : x; fl = true; }
+      for (auto& it : L.hi)
+        if (lin_is(it, base, x)) { h = fh ? std::min(h, x) : x; fh = true; }
+      if (!fl || !fh) return false;
+      lo = l + c;
+      hi = h + c;
+      return true;
+    }
+    return false;
+  }
+  std::string tread(int t, const std::string& a, const std::string& b, bool rule) const {
+    if (t_reg_h != 32) return "";
+    const std::string ts = std::to_string(t), arr = (rule ? "trr_" : "tv_") + ts;
+    int alo = 0, ahi = 0, blo = 0, bhi = 0;
+    // rows i0 .. i0 + 63, columns j0 - 32 .. j0 + 31 for every value of the loops
+    if (span(a, "i", alo, ahi) && span(b, "j", blo, bhi) && alo >= 0 && ahi <= 32 && blo >= -32 && bhi <= 0)
+      return arr + "[((" + a + ") - _ri0) * RW_ + ((" + b + ") - _rj0)]";
+    if (cur && t_mid && t_mid->count(cur->rule) && cur->loops.size() == 1) {
+      // the splits of a rule with a middle product left to the in-tile pass:
+      // below KLO (left operand in the left strip, right operand in the
+      // region) or above KHI (left in the region, right in the right strip)
+      const std::string& v = cur->loops[0].var;
+      int ca = 0, cb = 0;
+      if (t_sl->count(t) && lin_is(a, "i", ca) && ca == 0 && lin_is(b, v, cb) && cb >= 0 && t_ml + cb <= 32 && t_mh - cb <= 32)
+        return arr + "[(" + b + ") >= _rj0 ? (i - _ri0) * RW_ + ((" + b + ") - _rj0) : SLO_" + ts + " + (i - _ri0) * 64 + ((" + b +
+               ") - _ri0)]";
+      if (t_sr->count(t) && lin_is(a, v, ca) && ca >= 0 && lin_is(b, "j", cb) && cb == 0 && t_ml + ca <= 32 && t_mh - ca <= 32)
+        return arr + "[(" + a + ") < _ri0 + RH_ ? ((" + a + ") - _ri0) * RW_ + (j - _rj0) : SRO_" + ts + " + ((" + a +
+               ") - _rj0) * 32 + (j - _rj0 - RHALO_)]";
+    }
+    return "";
+  }
   std::string cost(int t, const std::string& a, const std::string& b) {
     if (self(a, b)) return "cv.c" + std::to_string(t);
     if (mrefs) return mref(t, a, b, false);
+    if (tile) {
+      const std::string f = tread(t, a, b, false);
+      int ca = 0, cb = 0;
+      // a fixed-offset read of the region is in bounds for any cell of the
+      // tile: loaded at the top of the cell with the others
+      if (!f.empty() && pre && lin_is(a, "i", ca) && lin_is(b, "j", cb)) return hoist("answer_t", f);
+      if (!f.empty()) return f;
+      return "rv_" + std::to_string(t) + "(z, T, _ri0, _rj0, " + a + ", " + b + ")";
+    }
     if (regs) return reg("v", t, a, b);
     if (pre) return hoist("answer_t", "((const GLOBAL_ answer_t*)T.c[" + std::to_string(t) + "])[sidx_(z," + a + "," + b + ")]");
-    return "((const answer_t*)T.c[" + std::to_string(t) + "])[idx_(z," + a + "," + b + ")]";
+    return "((const GLOBAL_ answer_t*)T.c[" + std::to_string(t) + "])[idx_(z," + a + "," + b + ")]";
   }
   std::string valid(int t, const std::string& a, const std::string& b) {
     if (!m.empty.empty()) return "!(" + cost(t, a, b) + "==(answer_t)(" + m.empty + "))";
     if (self(a, b)) return "(cv.r" + std::to_string(t) + "!=-1)";
     if (mrefs) return "(" + mref(t, a, b, true) + "!=-1)";
+    if (tile) {
+      tile_rules->insert(t);
+      const std::string f = tread(t, a, b, true);
+      int ca = 0, cb = 0;
+      if (!f.empty() && pre && lin_is(a, "i", ca) && lin_is(b, "j", cb)) return "(" + hoist("short", f) + "!=-1)";
+      if (!f.empty()) return "(" + f + "!=-1)";
+      return "(rr_" + std::to_string(t) + "(z, T, _ri0, _rj0, " + a + ", " + b + ")!=-1)";
+    }
     if (regs) return "(" + reg("r", t, a, b) + "!=-1)";
     if (pre) return "(" + hoist("short", "((const GLOBAL_ short*)T.r[" + std::to_string(t) + "])[sidx_(z," + a + "," + b + ")]") + "!=-1)";
-    return "(T.r[" + std::to_string(t) + "][idx_(z," + a + "," + b + ")]!=-1)";
+    return "(((const GLOBAL_ short*)T.r[" + std::to_string(t) + "])[idx_(z," + a + "," + b + ")]!=-1)";
   }
   void scs(int mn, int mx, const std::string& a, const std::string& b, AltCode& o) {
     if (mn == mx) { o.arith.push_back(add(a, mn) + "==" + b); return; }
@@ -883,6 +958,7 @@ struct Gen {
     return s + ")";
   }
   bool gen(const Node* n, const std::string& a, const std::string& b, AltCode& o, std::vector<std::string>& vals) {
+    cur = &o;
     switch (n->k) {
       case K_TAB: {
         const Tab& T = m.tabs[n->tab];
@@ -1052,7 +1128,7 @@ __device__ __forceinline__ int bp_index(int x, int y) {
   }
   return 7;
 }
-__device__ __forceinline__ int rS_(const Ctx& x, int k) { return x.in1[k]; }
+__device__ __forceinline__ int rS_(const Ctx& x, int k) { return IN1V_(x)[k]; }
 __device__ __forceinline__ int rBP_(const Ctx& x, int a, int b) { return bp_index(rS_(x, a), rS_(x, b)); }
 __device__ __forceinline__ int rTermau_(const Ctx& x, int i, int j) {
   const int p = rS_(x, i), q = rS_(x, j);
@@ -1207,17 +1283,89 @@ bool generate(Model& m, std::string& src, std::string& err) {
       }
     }
   }
+  size_t mid_lds = 0;
   {
-    size_t lds = 0;
-    for (auto& r : mrefs) lds += 1024 * (size_t)m.answer_bytes + (r.rules ? 2048 : 0);
-    lds += minputs.size() * (32 + 2 * Gen::MIDM) * (size_t)m.alphabet_bytes;
-    if (lds > 48 * 1024) {  // too many operands to stage: the wavefront takes every split
+    for (auto& r : mrefs) mid_lds += 1024 * (size_t)m.answer_bytes + (r.rules ? 2048 : 0);
+    mid_lds += minputs.size() * (32 + 2 * Gen::MIDM) * (size_t)m.alphabet_bytes;
+    if (mid_lds > 48 * 1024) {  // too many operands to stage: the wavefront takes every split
       for (auto& v : malts) v.clear();
       for (auto& v : mrule) v.clear();
       mrefs.clear();
       minputs.clear();
       has_mid = false;
+      mid_lds = 0;
     }
+  }
+  // the in-tile form of every alternative (one track), reading other cells
+  // through the LDS region of the tile: rows i0 .. i0 + 31 + H, columns
+  // j0 - H .. j0 + 31 -- the tile and, for H = 32, its neighbours (I, J - 1),
+  // (I + 1, J - 1), (I + 1, J), all finished before the tile starts
+  std::vector<std::vector<AltCode>> alts_tile(m.tabs.size());
+  std::set<int> tile_rules;
+  std::set<int> strip_l, strip_r;  // tabulations with a left / right strip in LDS
+  int reg_h = -1;  // H, or -1: no region (every read from the tables in memory)
+  std::set<int> mid_rule_ids;
+  for (auto& v : mrule) mid_rule_ids.insert(v.begin(), v.end());
+  // the in-tile alternatives: a first pass finds which tabulations' rule
+  // entries the region needs; after the region and strips are sized, the
+  // second pass generates them with the direct LDS reads
+  std::vector<std::map<std::string, std::string>> pre_tile(m.tabs.size());
+  std::vector<std::vector<std::array<std::string, 4>>> pre_decl_tile(m.tabs.size());
+  auto gen_tile = [&](int reg, bool fast) {
+    for (auto& v : alts_tile) v.clear();
+    for (auto& v : pre_tile) v.clear();
+    for (auto& v : pre_decl_tile) v.clear();
+    for (int t : m.order) {
+      const Tab& T = m.tabs[t];
+      std::vector<NP> flat;
+      if (!flatten(m, T.p, true, flat, err)) return false;
+      for (int q = 0; q < (int)flat.size(); ++q) {
+        Gen g(m, true);
+        g.tile = true;
+        g.tile_rules = &tile_rules;
+        if (fast) {
+          g.pre = &pre_tile[t];
+          g.pre_decl = &pre_decl_tile[t];
+          g.t_reg_h = reg;
+          g.t_sl = &strip_l;
+          g.t_sr = &strip_r;
+          g.t_mid = &mid_rule_ids;
+          g.t_ml = mid_lo;
+          g.t_mh = mid_hi;
+        }
+        AltCode o;
+        o.rule = T.id + q;
+        std::vector<std::string> vals;
+        if (!g.gen(flat[q].get(), "i", "j", o, vals) || vals.size() != 1) { err = g.err; return false; }
+        o.value = vals[0];
+        alts_tile[t].push_back(o);
+      }
+    }
+    return true;
+  };
+  if (m.tracks == 1) {
+    if (!gen_tile(-1, false)) return false;
+    size_t per = 0, core = 0;  // region bytes per cell; the tile's rule and split arrays
+    for (int t : m.order) {
+      per += (size_t)m.answer_bytes + (tile_rules.count(t) ? 2 : 0);
+      core += 1024 * (2 + 4 * (size_t)m.tabs[t].cat);
+    }
+    mid_lds += core + 16384 + 256 * (size_t)m.alphabet_bytes;  // and the input copies
+    // strips for the operands of the middle products: their splits left
+    // to the in-tile pass read tiles (I, I), (I, I + 1) (left operands) and
+    // (J - 1, J), (J, J) (right operands)
+    size_t strips = 0;
+    for (auto& r : mrefs) (r.left ? strip_l : strip_r).insert(r.t);
+    for (int t : strip_l) strips += 2048 * ((size_t)m.answer_bytes + (tile_rules.count(t) ? 2 : 0));
+    for (int t : strip_r) strips += 2048 * ((size_t)m.answer_bytes + (tile_rules.count(t) ? 2 : 0));
+    if (64 * 64 * per + mid_lds + strips <= 128 * 1024) reg_h = 32;
+    else {
+      strip_l.clear();
+      strip_r.clear();
+      if (64 * 64 * per + mid_lds <= 96 * 1024) reg_h = 32;
+      else if (32 * 32 * per + mid_lds <= 96 * 1024) reg_h = 0;
+    }
+    if (!gen_tile(reg_h, true)) return false;
   }
   // third pass: the register form of the band kernel, when every reference
   // allows it
@@ -1249,6 +1397,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
   const int NT = (int)m.tabs.size();
   std::ostringstream s;
   s << "// generated by libdpx from an ADP grammar (hipRTC provides the HIP runtime declarations)\n"
+       "#define GLOBAL_ __attribute__((address_space(1)))\n"
        "struct T2ii { int _1, _2; };\n"
     << m.header << "\n"
     << "typedef " << m.alphabet << " input_t;\n"
@@ -1265,8 +1414,33 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "__builtin_offsetof(Inst, T) == 72 && sizeof(Inst) == 72 + " << 32 * NT << ", \"instance layout\");\n"
     << rna_prelude(m)
     << "// what every algebra function sees: the inputs, their sizes, the energy tables\n"
-       "struct Ctx { const input_t* in1; const input_t* in2; int n1, n2; const RnaPar* rna; const int* hl; };\n"
+       "struct Ctx { const input_t* in1; const input_t* in2; int n1, n2; const RnaPar* rna; const int* hl; int wa, wb, wn; };\n"
     << (m.tracks == 1 ? "#define M_W (_x.n1 + 1)\n#define M_H (_x.n1 + 1)\n" : "#define M_H (_x.n1 + 1)\n#define M_W (_x.n2 + 1)\n")
+    // one track: input reads go through a view that serves a tile's two
+    // input windows from LDS (origins wa, wb, stored + 2^30 so that a Ctx
+    // built without them, wa = wb = 0, never matches) and the rest from memory
+    << (m.tracks == 1
+            ? "__shared__ input_t tinA_[128], tinB_[128];\n"
+              "#define INW_CAP_ (16384 / (int)sizeof(input_t))\n"
+              "__shared__ input_t tinW_[INW_CAP_];  // the whole input, when it fits\n"
+              "// the loads that miss the LDS copies sit behind a call, so the code\n"
+              "// around an LDS read never waits for memory loads it does not use\n"
+              "template <class V> __device__ __attribute__((noinline, pure)) V gld_(const V* p, long long x) {\n"
+              "  return ((const GLOBAL_ V*)p)[x];\n}\n"
+              "struct InV {\n"
+              "  const input_t* g; int wa, wb, wn;\n"
+              "  __device__ __forceinline__ input_t operator[](int x) const {\n"
+              "    if (wn) return tinW_[min(max(x, 0), wn - 1)];\n"
+              "    const unsigned a = (unsigned)(x - wa + (1 << 30));\n"
+              "    if (a < 128u) return tinA_[a];\n"
+              "    const unsigned b = (unsigned)(x - wb + (1 << 30));\n"
+              "    if (b < 128u) return tinB_[b];\n"
+              "    return gld_(g, x);\n"
+              "  }\n"
+              "};\n"
+              "#define IN1V_(x) (InV{(x).in1, (x).wa, (x).wb, (x).wn})\n"
+              "#define IN1T_ const InV\n"
+            : "#define IN1V_(x) ((x).in1)\n#define IN1T_ const input_t*\n")
     << (m.rna ? rna_functions() : std::string());
   if (m.tracks == 1)
     // 32 x 32 tiles, each a contiguous row-major record (a tile row is one
@@ -1303,7 +1477,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
     }
     s << "__device__ __forceinline__ auto f_" << f.name << "(const Ctx& _x";
     for (size_t q = 0; q < f.args.size(); ++q) s << ", A" << q << " " << f.args[q];
-    s << ") {\n  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n  " << f.body << "\n}\n";
+    s << ") {\n  IN1T_ _in1 = IN1V_(_x);\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n  " << f.body << "\n}\n";
   }
   auto better = [&](const std::string& c, const std::string& c0) {
     std::string cmp;
@@ -1346,6 +1520,22 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "  int u[(sizeof(V) + 3) / 4] = {};\n"
        "  __builtin_memcpy(u, &v, sizeof(V));\n"
        "  for (int q = 0; q < (int)((sizeof(V) + 3) / 4); ++q) __asm__ volatile(\"\" ::\"v\"(u[q]));\n}\n";
+  s << "template <int CTRL, class V> __device__ __forceinline__ V dpp_(V v) {\n"
+       "  constexpr int N = (sizeof(V) + 3) / 4;\n"
+       "  int u[N] = {};\n"
+       "  __builtin_memcpy(u, &v, sizeof(V));\n"
+       "  for (int q = 0; q < N; ++q) u[q] = __builtin_amdgcn_update_dpp(0, u[q], CTRL, 0xf, 0xf, false);\n"
+       "  __builtin_memcpy(&v, u, sizeof(V));\n"
+       "  return v;\n}\n"
+       "// level lv of a group all-reduce: lane l's partner l ^ 1, l ^ 2, then\n"
+       "// 7 - l within 8 lanes and 15 - l within 16 lanes\n"
+       "template <class V> __device__ __forceinline__ V lvmove_(V v, int lv) {\n"
+       "  switch (lv) {\n"
+       "    case 0: return dpp_<0xB1>(v);   // quad_perm [1,0,3,2]\n"
+       "    case 1: return dpp_<0x4E>(v);   // quad_perm [2,3,0,1]\n"
+       "    case 2: return dpp_<0x141>(v);  // row_half_mirror\n"
+       "    default: return dpp_<0x140>(v); // row_mirror\n"
+       "  }\n}\n";
   s << "template <class V> __device__ __forceinline__ V shfl_xor_(V v, int m, int w) {\n"
        "  constexpr int N = (sizeof(V) + 3) / 4;\n"
        "  int u[N] = {};\n"
@@ -1359,7 +1549,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
   // cell, stored by the last one (DEFER) or each by its own
   // table and input accesses through global pointers (not flat): they count
   // only in vmcnt, so the lane shifts (LDS counter) never wait for them
-  s << "#define GLOBAL_ __attribute__((address_space(1)))\n";
+
   s << "struct CellV {";
   for (int t : m.order) {
     s << " answer_t c" << t << "; short r" << t << ";";
@@ -1398,20 +1588,47 @@ bool generate(Model& m, std::string& src, std::string& err) {
     s << " (void)p; }\n";
     }
     s << "template <int G, bool DEFER> __device__ __forceinline__ void tab" << (tile ? "T" : sx) << "_" << t
-      << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, int lane, CellV& cv, const Pre" << sx << "_" << t << "& _p" << extra
+      << "(const Ctx& _x, const Dims& z, const Tabs& T, int i, int j, int lane, CellV& cv"
+      << (tile ? std::string() : ", const Pre" + sx + "_" + std::to_string(t) + "& _p") << extra
       << ") {\n"
-      << "  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n"
+      << "  IN1T_ _in1 = IN1V_(_x);\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n"
       << "  answer_t _c0" << (m.empty.empty() ? "{}" : " = (answer_t)(" + m.empty + ")") << ";\n"
       << "  int _r0 = -1, _p0 = 0, _p1 = 0, _o0 = 0, _o1 = 0;\n";
-    for (auto& d : PD[t]) s << "  const " << d[0] << " " << d[1] << " = _p." << d[1] << ";\n";
-    s << "  (void)_p;\n";
+    if (tile)  // the cell's fixed-offset reads, issued together
+      for (auto& d : PD[t]) s << "  const " << d[0] << " " << d[1] << " = " << d[2] << ";\n";
+    else
+      for (auto& d : PD[t]) s << "  const " << d[0] << " " << d[1] << " = _p." << d[1] << ";\n";
+    if (!tile) s << "  (void)_p;\n";
+    if (tile && !mrule[t].empty())  // the middle product's partial, loaded ahead of the rules
+    {
+      const std::string ts = std::to_string(t);
+      s << "  answer_t _mq_c{};\n  int _mq_r = -1, _mq_p0 = 0, _mq_p1 = 0;\n"
+           "  if (_mid) {\n";
+      if (reg_h >= 0)
+        s << "    const int e_ = (i - _ri0) * 32 + (j - _rj0 - RHALO_);\n"
+          << "    _mq_c = tv_" << ts << "[(i - _ri0) * RW_ + (j - _rj0)];\n"
+          << "    _mq_r = tcr_" << ts << "[e_];\n"
+          << (T.cat > 0 ? "    _mq_p0 = tcp_" + ts + "_0[e_];\n" : std::string())
+          << (T.cat > 1 ? "    _mq_p1 = tcp_" + ts + "_1[e_];\n" : std::string());
+      else
+        s << "    const long long x = idx_(z, i, j);\n"
+          << "    _mq_c = ((const GLOBAL_ answer_t*)T.c[" << ts << "])[x];\n"
+          << "    _mq_r = ((const GLOBAL_ short*)T.r[" << ts << "])[x];\n"
+          << (T.cat > 0 ? "    _mq_p0 = ((const GLOBAL_ int*)T.p[" + ts + "][0])[x];\n" : std::string())
+          << (T.cat > 1 ? "    _mq_p1 = ((const GLOBAL_ int*)T.p[" + ts + "][1])[x];\n" : std::string());
+      s << "  }\n";
+    }
+    int nl_rule = 0;
     for (auto& o : A[t]) {
       std::string ind = "  ";
       s << ind << "{  // rule " << o.rule << "\n";
       int close = 1;
       if (o.loops.empty()) {
+        // a rule without a split loop on one lane of the group, a different
+        // lane for each such rule (the group merge orders by rule and split,
+        // so the lane holding a candidate does not matter)
         ind += "  ";
-        s << ind << "if (lane == 0) {\n";
+        s << ind << "if (lane == (" << nl_rule++ << " % G)) {\n";
         ++close;
       }
       // each condition at the outermost loop level that binds its names
@@ -1451,6 +1668,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
           ++close;
           ind += "  ";
           s << ind << "const int _lo = " << bound(L.lo, "max") << ", _hi = " << bound(L.hi, "min") << ";\n"
+            << "#pragma unroll 4\n"
             << ind << "for (int " << L.var << " = (_sg ? max(_lo, _mhi + 1) : _lo) + lane, " << L.var
             << "_u = _sg ? _hi : min(_hi, _mlo - 1); " << L.var << " <= " << L.var << "_u; " << L.var << " += G) {\n";
         } else if (q == 0)
@@ -1481,11 +1699,21 @@ bool generate(Model& m, std::string& src, std::string& err) {
         s << ind << "}\n";
       }
     }
-    s << "  for (int off = G / 2; off > 0; off >>= 1) {\n"
-         "    const answer_t c = shfl_xor_(_c0, off, G);\n"
-         "    const int r = __shfl_xor(_r0, off, G), p0 = __shfl_xor(_p0, off, G), p1 = __shfl_xor(_p1, off, G);\n"
-         "    const int o0 = __shfl_xor(_o0, off, G), o1 = __shfl_xor(_o1, off, G);\n"
-         "    bool take;\n"
+    // all-reduce over the group: DPP lane moves within 16-lane rows (xor 1,
+  // xor 2, half-row and row mirrors pair disjoint halves at each level),
+  // lane shuffles across rows
+  s << "  for (int lv = 0; (1 << lv) < G; ++lv) {\n"
+       "    answer_t c;\n"
+       "    int r, p0, p1, o0, o1;\n"
+       "    if (lv < 4) {\n"
+       "      c = lvmove_(_c0, lv); r = lvmove_(_r0, lv); p0 = lvmove_(_p0, lv); p1 = lvmove_(_p1, lv);\n"
+       "      o0 = lvmove_(_o0, lv); o1 = lvmove_(_o1, lv);\n"
+       "    } else {\n"
+       "      const int off = 1 << lv;\n"
+       "      c = shfl_xor_(_c0, off, G); r = __shfl_xor(_r0, off, G); p0 = __shfl_xor(_p0, off, G);\n"
+       "      p1 = __shfl_xor(_p1, off, G); o0 = __shfl_xor(_o0, off, G); o1 = __shfl_xor(_o1, off, G);\n"
+       "    }\n"
+       "    bool take;\n"
          "    if (r == -1) take = false;\n"
          "    else if (_r0 == -1) take = true;\n";
     if (m.additive())  // both lanes hold candidates: add them up, keep the first back-pointer
@@ -1505,12 +1733,9 @@ bool generate(Model& m, std::string& src, std::string& err) {
       // the middle product's partial (value, rule, splits) of this cell,
       // merged as one more lane; its outer split is the split variable
       s << "  if (_mid) {\n"
-           "    const long long x = idx_(z, i, j);\n"
-        << "    const answer_t c = ((const answer_t*)T.c[" << t << "])[x];\n"
-        << "    const int r = T.r[" << t << "][x];\n"
-        << "    const int p0 = " << (T.cat > 0 ? "T.p[" + std::to_string(t) + "][0][x]" : std::string("0")) << ";\n"
-        << "    const int p1 = " << (T.cat > 1 ? "T.p[" + std::to_string(t) + "][1][x]" : std::string("0")) << ";\n"
-        << "    int o0 = 0;\n"
+           "    const answer_t c = _mq_c;\n"
+           "    const int r = _mq_r, p0 = _mq_p0, p1 = _mq_p1;\n"
+           "    int o0 = 0;\n"
            "    switch (r) {\n";
       for (auto& o : malts[t])
         for (size_t q = 0; q < o.var_of_pos.size(); ++q)
@@ -1537,11 +1762,20 @@ bool generate(Model& m, std::string& src, std::string& err) {
     s << "  (void)_o0; (void)_o1;\n"
       << "  cv.c" << t << " = _c0;\n  cv.r" << t << " = (short)_r0;\n";
     for (int q = 0; q < T.cat; ++q) s << "  cv.p" << t << "_" << q << " = _p" << q << ";\n";
-    s << "  if (!DEFER && lane == 0) {\n"
-         "    const long long x = idx_(z, i, j);\n"
-      << "    ((answer_t*)T.c[" << t << "])[x] = _c0;\n"
-      << "    T.r[" << t << "][x] = (short)_r0;\n";
-    for (int q = 0; q < T.cat; ++q) s << "    T.p[" << t << "][" << q << "][x] = _p" << q << ";\n";
+    s << "  if (!DEFER && lane == 0) {\n";
+    if (tile && reg_h >= 0) {
+      // the tile's own cells stay in LDS; the tile goes to memory at its end
+      s << "    const int e_ = (i - _ri0) * RW_ + (j - _rj0), c_ = (i - _ri0) * 32 + (j - _rj0 - RHALO_);\n"
+        << "    tv_" << t << "[e_] = _c0;\n"
+        << "    tcr_" << t << "[c_] = (short)_r0;\n";
+      if (tile_rules.count(t)) s << "    trr_" << t << "[e_] = (short)_r0;\n";
+      for (int q = 0; q < T.cat; ++q) s << "    tcp_" << t << "_" << q << "[c_] = _p" << q << ";\n";
+    } else {
+      s << "    const long long x = idx_(z, i, j);\n"
+        << "    ((GLOBAL_ answer_t*)T.c[" << t << "])[x] = _c0;\n"
+        << "    ((GLOBAL_ short*)T.r[" << t << "])[x] = (short)_r0;\n";
+      for (int q = 0; q < T.cat; ++q) s << "    ((GLOBAL_ int*)T.p[" << t << "][" << q << "])[x] = _p" << q << ";\n";
+    }
     s << "  }\n";
     if (used_tabs > 1 && !m.fixed_refs)  // a later tabulation of this cell may read it from another lane of the wave
       s << "  if (G > 1) { __builtin_amdgcn_s_waitcnt(0x0F70); __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, \"workgroup\"); }\n";
@@ -1549,7 +1783,48 @@ bool generate(Model& m, std::string& src, std::string& err) {
   }
   };
   emit_tabs(alts, pre_decl, "", "", false);
-  if (m.tracks == 1) emit_tabs(alts, pre_decl, "", ", bool _mid, int _mlo, int _mhi", true);
+  if (m.tracks == 1) {
+    if (reg_h >= 0)
+      s << "#define RH_ " << 32 + reg_h << "\n#define RW_ " << 32 + reg_h << "\n#define RHALO_ " << reg_h << "\n";
+    for (int t : m.order) {
+      const std::string ts = std::to_string(t);
+      const bool sl = strip_l.count(t), sr = strip_r.count(t), rl = tile_rules.count(t);
+      // per tabulation one array: the region, then the left strip (rows
+      // i0 .. i0 + 31, columns i0 .. i0 + 63), then the right strip (rows
+      // j0 - 32 .. j0 + 31, columns j0 .. j0 + 31)
+      if (reg_h >= 0) {
+        // the tile's own rule and split entries (its values are in the region)
+        s << "__shared__ short tcr_" << ts << "[1024];\n";
+        for (int q = 0; q < m.tabs[t].cat; ++q) s << "__shared__ int tcp_" << ts << "_" << q << "[1024];\n";
+        s << "#define SLO_" << ts << " (RH_ * RW_)\n"
+          << "#define SRO_" << ts << " (RH_ * RW_ + " << (sl ? 2048 : 0) << ")\n"
+          << "#define TLEN_" << ts << " (RH_ * RW_ + " << (sl ? 2048 : 0) + (sr ? 2048 : 0) << ")\n"
+          << "__shared__ answer_t tv_" << ts << "[TLEN_" << ts << "];\n";
+        if (rl) s << "__shared__ short trr_" << ts << "[TLEN_" << ts << "];\n";
+      }
+      for (int rule = 0; rule < 2; ++rule) {
+        if (rule)
+          s << "__device__ __forceinline__ short rr_" << ts << "(const Dims& z, const Tabs& T, int ri0, int rj0, int a, int b) {\n";
+        else
+          s << "__device__ __forceinline__ answer_t rv_" << ts << "(const Dims& z, const Tabs& T, int ri0, int rj0, int a, int b) {\n";
+        const bool lds = !rule || rl;
+        if (reg_h >= 0 && lds)
+          s << "  const unsigned ra = a - ri0, rb = b - rj0;\n"
+               "  if (ra < RH_ && rb < RW_) return " << (rule ? "trr_" : "tv_") << ts << "[ra * RW_ + rb];\n";
+        if (sl && lds)
+          s << "  if ((unsigned)(a - ri0) < 32u && (unsigned)(b - ri0) < 64u) return " << (rule ? "trr_" : "tv_") << ts
+            << "[SLO_" << ts << " + (a - ri0) * 64 + (b - ri0)];\n";
+        if (sr && lds)
+          s << "  { const int j0 = rj0 + RHALO_;\n"
+               "    if ((unsigned)(a - j0 + 32) < 64u && (unsigned)(b - j0) < 32u) return " << (rule ? "trr_" : "tv_") << ts
+            << "[SRO_" << ts << " + (a - j0 + 32) * 32 + (b - j0)]; }\n";
+        s << "  (void)ri0; (void)rj0;\n"
+          << "  return gld_((const " << (rule ? "short" : "answer_t") << "*)T." << (rule ? "r" : "c") << "[" << ts
+          << "], idx_(z, a, b));\n}\n";
+      }
+    }
+    emit_tabs(alts_tile, pre_decl_tile, "", ", bool _mid, int _mlo, int _mhi, int _ri0, int _rj0", true);
+  }
   if (m.reg_band) {
     // what the register band kernel's cells read: R.v<t>_<di>_<dj> (value)
     // and R.r<t>_<di>_<dj> (rule) of cell (i - di, j - dj)
@@ -1619,7 +1894,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
       // cell's table entry for the in-tile pass.
       s << "template <int CPT> __device__ __forceinline__ void middle_(const Ctx& _x, const Dims& z, const Tabs& T, int D, int i0,\n"
            "                                                        int j0, int KLO, int KHI, MidSm& sm) {\n"
-           "  const input_t* _in1 = _x.in1;\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n"
+           "  IN1T_ _in1 = IN1V_(_x);\n  const input_t* _in2 = _x.in2;\n  (void)_in1; (void)_in2;\n"
            "  constexpr int RS = 32 / CPT, NT_ = 32 * RS;\n"
            "  const int tid = threadIdx.x, _mc = tid & 31, rb = tid >> 5;\n"
            "  bool ok[CPT];\n";
@@ -1631,26 +1906,47 @@ bool generate(Model& m, std::string& src, std::string& err) {
            "    ok[u] = j <= z.n1 && j - i <= D;\n";
       for (int t : m.order)
         for (auto& o : malts[t]) s << "    bc" << o.rule << "[u] = " << init << ";\n    bk" << o.rule << "[u] = -1;\n";
-      s << "  }\n"
-           "  for (int K = KLO >> 5; K <= (KHI >> 5); ++K) {\n"
-           "    __syncthreads();  // the last block's operands are read\n"
-           "    for (int e = tid; e < 1024; e += NT_) {\n"
-           "      const int hi_ = e >> 5, lo_ = e & 31;\n";
+      s << "  }\n";
+      // the operands of block K travel in registers: block K + 1 is fetched
+      // while block K is used
+      for (size_t x = 0; x < mrefs.size(); ++x) {
+        s << "  answer_t pf" << x << "[CPT];\n";
+        if (mrefs[x].rules) s << "  short pfr" << x << "[CPT];\n";
+      }
+      for (int tr : minputs) s << "  input_t pfi" << tr << " = input_t{};\n";
+      std::ostringstream fetch;
+      fetch << "#pragma unroll\n"
+               "    for (int w = 0; w < CPT; ++w) {\n"
+               "      const int e = tid + w * NT_, hi_ = e >> 5, lo_ = e & 31;\n";
       for (size_t x = 0; x < mrefs.size(); ++x) {
         const auto& r = mrefs[x];
-        const std::string side = r.left ? "mL" : "mR", t = std::to_string(r.t);
-        const std::string ix = r.left ? "sidx_(z, i0 + hi_ + " + std::to_string(r.a) + ", (K << 5) + lo_ + " + std::to_string(r.b) + ")"
-                                      : "sidx_(z, (K << 5) + hi_ + " + std::to_string(r.a) + ", j0 + lo_ + " + std::to_string(r.b) + ")";
-        s << "      {\n        const long long a_ = " << ix << ";\n"
-          << "        sm." << side << x << "[e] = ((const GLOBAL_ answer_t*)T.c[" << t << "])[a_];\n";
-        if (r.rules) s << "        sm." << side << "R" << x << "[e] = ((const GLOBAL_ short*)T.r[" << t << "])[a_];\n";
-        s << "      }\n";
+        const std::string t = std::to_string(r.t);
+        const std::string ix = r.left ? "sidx_(z, i0 + hi_ + " + std::to_string(r.a) + ", (KF << 5) + lo_ + " + std::to_string(r.b) + ")"
+                                      : "sidx_(z, (KF << 5) + hi_ + " + std::to_string(r.a) + ", j0 + lo_ + " + std::to_string(r.b) + ")";
+        fetch << "      {\n        const long long a_ = " << ix << ";\n"
+              << "        pf" << x << "[w] = ((const GLOBAL_ answer_t*)T.c[" << t << "])[a_];\n";
+        if (r.rules) fetch << "        pfr" << x << "[w] = ((const GLOBAL_ short*)T.r[" << t << "])[a_];\n";
+        fetch << "      }\n";
+      }
+      fetch << "    }\n";
+      for (int tr : minputs)
+        fetch << "    if (tid < " << 32 + 2 * MIDM << ") pfi" << tr << " = ((const GLOBAL_ input_t*)_x.in" << tr
+              << ")[min(max((KF << 5) - MIDM + tid, 0), z.n1)];\n";
+      s << "  {\n    const int KF = KLO >> 5;\n" << fetch.str() << "  }\n"
+        << "  for (int K = KLO >> 5; K <= (KHI >> 5); ++K) {\n"
+           "    __syncthreads();  // the last block's operands are read\n"
+           "#pragma unroll\n"
+           "    for (int w = 0; w < CPT; ++w) {\n"
+           "      const int e = tid + w * NT_;\n";
+      for (size_t x = 0; x < mrefs.size(); ++x) {
+        const std::string side = mrefs[x].left ? "mL" : "mR";
+        s << "      sm." << side << x << "[e] = pf" << x << "[w];\n";
+        if (mrefs[x].rules) s << "      sm." << side << "R" << x << "[e] = pfr" << x << "[w];\n";
       }
       s << "    }\n";
-      for (int tr : minputs)
-        s << "    if (tid < " << 32 + 2 * MIDM << ") sm.mI" << tr << "[tid] = ((const GLOBAL_ input_t*)_in" << tr
-          << ")[min(max((K << 5) - MIDM + tid, 0), z.n1)];\n";
-      s << "    __syncthreads();\n"
+      for (int tr : minputs) s << "    if (tid < " << 32 + 2 * MIDM << ") sm.mI" << tr << "[tid] = pfi" << tr << ";\n";
+      s << "    if (K < (KHI >> 5)) {\n      const int KF = K + 1;\n" << fetch.str() << "    }\n";
+    s << "    __syncthreads();\n"
            "    const int klo = max(KLO, K << 5), khi = min(KHI, (K << 5) + 31);\n"
            "    for (int k = klo; k <= khi; ++k) {\n"
            "      const int _mkk = k - (K << 5);\n"
@@ -1690,7 +1986,8 @@ bool generate(Model& m, std::string& src, std::string& err) {
            "  for (int u = 0; u < CPT; ++u) {\n"
            "    const int i = i0 + rb + RS * u, j = j0 + _mc;\n"
            "    if (!ok[u]) continue;\n"
-           "    const long long x = idx_(z, i, j);\n";
+           "    const long long x = idx_(z, i, j);\n"
+           "    (void)x;\n";
       for (int t : m.order) {
         if (malts[t].empty()) continue;
         const Tab& T = m.tabs[t];
@@ -1706,8 +2003,13 @@ bool generate(Model& m, std::string& src, std::string& err) {
             s << "        if (pr == -1 || " << better("bc" + R + "[u]", "pc") << ") { pc = bc" << R << "[u]; " << setp << " }\n";
           s << "      }\n";
         }
-        s << "      ((answer_t*)T.c[" << t << "])[x] = pc;\n      T.r[" << t << "][x] = (short)pr;\n";
-        for (int q = 0; q < T.cat; ++q) s << "      T.p[" << t << "][" << q << "][x] = pp" << q << ";\n";
+        if (reg_h >= 0) {
+          s << "      tv_" << t << "[(i - i0) * RW_ + (j - j0 + RHALO_)] = pc;\n      tcr_" << t << "[(i - i0) * 32 + (j - j0)] = (short)pr;\n";
+          for (int q = 0; q < T.cat; ++q) s << "      tcp_" << t << "_" << q << "[(i - i0) * 32 + (j - j0)] = pp" << q << ";\n";
+        } else {
+          s << "      ((GLOBAL_ answer_t*)T.c[" << t << "])[x] = pc;\n      ((GLOBAL_ short*)T.r[" << t << "])[x] = (short)pr;\n";
+          for (int q = 0; q < T.cat; ++q) s << "      ((GLOBAL_ int*)T.p[" << t << "][" << q << "])[x] = pp" << q << ";\n";
+        }
         s << "      (void)pp0; (void)pp1;\n    }\n";
       }
       s << "  }\n}\n";
@@ -1742,9 +2044,21 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "}\n";
     s << "template <int NT_> __device__ __forceinline__ void tile_(const Inst& q, int TI_, int TJ_, MidSm& sm) {\n"
          "  const Dims z = q.z;\n  const Tabs T = q.T;\n"
-         "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl};\n"
+         "  const int i0 = TI_ << 5, j0 = TJ_ << 5;\n"
+         "  // the input in LDS: whole when it fits, else the tile's windows\n"
+         "  // i0 - 32 .. i0 + 95 and j0 - 64 .. j0 + 63\n"
+         "  const bool whole = z.n1 + 1 <= INW_CAP_;\n"
+         "  if (whole) {\n"
+         "    for (int e = threadIdx.x; e <= z.n1; e += NT_) tinW_[e] = ((const GLOBAL_ input_t*)q.in1)[e];\n"
+         "  } else if (threadIdx.x < 256) {\n"
+         "    const int x = threadIdx.x < 128 ? i0 - 32 + threadIdx.x : j0 - 64 + (threadIdx.x - 128);\n"
+         "    const input_t v = ((const GLOBAL_ input_t*)q.in1)[min(max(x, 0), z.n1)];\n"
+         "    if (threadIdx.x < 128) tinA_[threadIdx.x] = v; else tinB_[threadIdx.x - 128] = v;\n"
+         "  }\n"
+         "  __syncthreads();\n"
+         "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl, i0 - 32 + (1 << 30), j0 - 64 + (1 << 30),\n"
+         "               whole ? z.n1 + 1 : 0};\n"
       << "  const int D = " << (m.window > 0 ? "min(" + std::to_string(m.window) + ", z.n1)" : std::string("z.n1")) << ";\n"
-      << "  const int i0 = TI_ << 5, j0 = TJ_ << 5;\n"
       << "  const int KLO = i0 + TB + " << mid_lo << ", KHI = j0 - 1 - " << mid_hi << ";\n"
       << "  const bool _mid = " << (has_mid ? "TJ_ - TI_ >= 2 && KLO <= KHI" : "false") << ";\n"
       // no middle product: an empty range far above every split (n < 2^30),
@@ -1757,9 +2071,53 @@ bool generate(Model& m, std::string& src, std::string& err) {
            "  }\n";
     else
       s << "  (void)sm;\n";
+    if (reg_h >= 0) {
+      // the region's halo (finished neighbour tiles) into LDS; the tile's own
+      // cells enter it as they are computed
+      s << "  const int _ri0 = i0, _rj0 = j0 - RHALO_;\n"
+           "  for (int e = threadIdx.x; e < RH_ * RW_; e += NT_) {\n"
+           "    const int ra = e / RW_, rb = e - ra * RW_;\n"
+           "    if (ra < TB && rb >= RHALO_) continue;\n"
+           "    const long long x = sidx_(z, _ri0 + ra, _rj0 + rb);\n";
+      for (int t : m.order) {
+        s << "    tv_" << t << "[e] = ((const GLOBAL_ answer_t*)T.c[" << t << "])[x];\n";
+        if (tile_rules.count(t)) s << "    trr_" << t << "[e] = ((const GLOBAL_ short*)T.r[" << t << "])[x];\n";
+      }
+      s << "  }\n";
+      if (!strip_l.empty() || !strip_r.empty()) {
+        s << "  for (int e = threadIdx.x; e < 2048; e += NT_) {\n";
+        if (!strip_l.empty()) {
+          s << "    {\n      const long long x = sidx_(z, i0 + (e >> 6), i0 + (e & 63));\n";
+          for (int t : strip_l) {
+            s << "      tv_" << t << "[SLO_" << t << " + e] = ((const GLOBAL_ answer_t*)T.c[" << t << "])[x];\n";
+            if (tile_rules.count(t)) s << "      trr_" << t << "[SLO_" << t << " + e] = ((const GLOBAL_ short*)T.r[" << t << "])[x];\n";
+          }
+          s << "    }\n";
+        }
+        if (!strip_r.empty()) {
+          s << "    {\n      const long long x = sidx_(z, j0 - 32 + (e >> 5), j0 + (e & 31));\n";
+          for (int t : strip_r) {
+            s << "      tv_" << t << "[SRO_" << t << " + e] = ((const GLOBAL_ answer_t*)T.c[" << t << "])[x];\n";
+            if (tile_rules.count(t)) s << "      trr_" << t << "[SRO_" << t << " + e] = ((const GLOBAL_ short*)T.r[" << t << "])[x];\n";
+          }
+          s << "    }\n";
+        }
+        s << "  }\n";
+      }
+      s << "  __syncthreads();\n";
+    } else {
+      s << "  const int _ri0 = i0, _rj0 = j0;\n";
+    }
     // the in-tile wavefront: tile diagonal e holds cells (r, r + e); a cell
     // reads cells of smaller e only, so one barrier per diagonal
-    s << "  constexpr int G = " << (m.any_order() ? "NT_ / TB" : "1") << ";\n"
+    // lanes per cell: 8 (4 waves for a tile diagonal) -- the cell code is
+    // long, so one active wave per SIMD beats more lanes per cell and the
+    // other waves of a 1024-thread tile only meet the barriers; grammars
+    // with nested split loops (interior loops) spread them over NT_ / 32
+    bool nested = false;
+    for (int t : m.order)
+      for (auto& o : alts_tile[t]) nested = nested || o.loops.size() >= 2;
+    s << "  constexpr int G = " << (!m.any_order() ? "1" : nested ? "NT_ / TB" : "8") << ";\n"
          "  const int grp = threadIdx.x / G, lane = threadIdx.x & (G - 1);\n"
          "  for (int e = TI_ == TJ_ ? 0 : 1 - TB; e < TB; ++e) {\n"
          "    const int r = grp + (e < 0 ? -e : 0), c = r + e;\n"
@@ -1767,14 +2125,27 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "    if (r < TB && c < TB && j <= z.n1 && j - i <= D) {\n";
     s << "      CellV cv;\n";
     for (int t : m.order) s << "      cv.r" << t << " = -1;\n";
-    for (int t : m.order) s << "      Pre_" << t << " p" << t << ";\n      load_" << t << "(_x, z, T, i, j, p" << t << ");\n";
-    for (int t : m.order) s << "      pin_" << t << "(p" << t << ");\n";
     for (int t : m.order)
-      s << "      tabT_" << t << "<G, false>(_x, z, T, i, j, lane, cv, p" << t << ", _mid, _mlo, _mhi);\n";
+      s << "      tabT_" << t << "<G, false>(_x, z, T, i, j, lane, cv, _mid, _mlo, _mhi, _ri0, _rj0);\n";
     s << "    }\n"
          "    __syncthreads();\n"
-         "  }\n"
-         "}\n";
+         "  }\n";
+    if (reg_h >= 0) {
+      // the tile's records to memory: each table's 1024 entries contiguous
+      s << "  {\n"
+           "    const long long base = idx_(z, i0, j0);\n"
+           "    for (int e = threadIdx.x; e < 1024; e += NT_) {\n"
+           "      const int ra = e >> 5, rb = e & 31;\n";
+      for (int t : m.order) {
+        s << "      ((GLOBAL_ answer_t*)T.c[" << t << "])[base + e] = tv_" << t << "[ra * RW_ + rb + RHALO_];\n"
+          << "      ((GLOBAL_ short*)T.r[" << t << "])[base + e] = tcr_" << t << "[e];\n";
+        for (int q = 0; q < m.tabs[t].cat; ++q)
+          s << "      ((GLOBAL_ int*)T.p[" << t << "][" << q << "])[base + e] = tcp_" << t << "_" << q << "[e];\n";
+      }
+      s << "    }\n"
+           "  }\n";
+    }
+    s << "}\n";
     s << "template <int NT_> __device__ __forceinline__ void tile_run_(const Inst* __restrict__ I, const TInf* __restrict__ TI,\n"
          "                                                          int ntiles, unsigned long long* queue, int* ctrl, int* cnt) {\n"
          "  __shared__ MidSm sm;\n"
