@@ -198,6 +198,10 @@ struct Node {
   int track = 0;
   int hint[4] = {0, INF, 0, INF};
   Agg agg;
+  // a nested aggregation after flatten: the alternatives of its inner Or
+  // (all of them are aggregated together) and which one this copy stands for
+  std::vector<std::shared_ptr<Node>> sibs;
+  int sib = -1;
 };
 using NP = std::shared_ptr<Node>;
 
@@ -606,12 +610,34 @@ bool analyze(Model& m, std::string& err) {
 bool flatten(const Model& m, const NP& n, bool top, std::vector<NP>& out, std::string& err) {
   switch (n->k) {
     case K_OR: return flatten(m, n->l, top, out, err) && flatten(m, n->r, top, out, err);
-    case K_AGG:
-      if (!top || !(n->agg == m.h)) {
-        err = "nested or differing aggregation is not supported (aggregate below a concatenation or map, or not the signature's h)";
+    case K_AGG: {
+      if (top) {  // the tabulation's own aggregation (CodeGen.scala:31-34)
+        if (!(n->agg == m.h)) { err = "a tabulation's aggregation differs from the signature's h"; return false; }
+        return flatten(m, n->l, top, out, err);
+      }
+      // nested: Aggregate(Or(l, r)) = Or(Aggregate(l), Aggregate(r)), as the
+      // reference's norm (CodeGen.scala:31-36); each keeps its aggregation
+      if (m.additive() || n->agg.k == A_SUM || n->agg.k == A_COUNT) {
+        err = "nested sum or count aggregations are not supported";
         return false;
       }
-      return flatten(m, n->l, top, out, err);
+      // The Scala parsers aggregate the union of the inner alternatives
+      // (BaseParsers.scala:228, Or.apply = left ++ right) and the winner's
+      // rule id carries which alternative won (Concat.apply combines the
+      // sides' ids).  So each distributed copy aggregates all of them and
+      // yields a candidate only when the winner is its own alternative: one
+      // candidate in all, under the rule id the Scala backtrace gives it.
+      std::vector<NP> in;
+      if (!flatten(m, n->l, false, in, err)) return false;
+      for (int q = 0; q < (int)in.size(); ++q) {
+        auto c = std::make_shared<Node>(*n);
+        c->l = in[q];
+        c->sibs = in;
+        c->sib = q;
+        out.push_back(c);
+      }
+      return true;
+    }
     case K_FILTER: case K_MAP: {
       std::vector<NP> in;
       if (!flatten(m, n->l, top && n->k == K_FILTER, in, err)) return false;
@@ -746,6 +772,15 @@ struct AltCode {
   struct Kid { int tab; std::string a, b; };
   std::vector<Kid> kids;
   std::string value;
+  // nested aggregations (an aggregate below a concatenation or a map, one
+  // alternative each after flatten): each is evaluated as a block of its own
+  // loops right before the candidate, into _g<id>c (value), _g<id>r (valid)
+  // and _g<id>p<q> (its splits, which join the alternative's positions)
+  // alts: every alternative of the aggregation's inner Or, in order; sib:
+  // the one this alternative stands for (its candidate counts only when the
+  // aggregation's winner is that one, _g<id>w == sib)
+  struct Nest { std::string id; Agg h; std::vector<std::shared_ptr<AltCode>> alts; int sib = 0; };
+  std::vector<Nest> nested;
 };
 
 struct Gen {
@@ -957,9 +992,44 @@ struct Gen {
     for (auto& x : args) s += "," + x;
     return s + ")";
   }
+  int nagg = 0;  // nested aggregations generated for this alternative
   bool gen(const Node* n, const std::string& a, const std::string& b, AltCode& o, std::vector<std::string>& vals) {
     cur = &o;
     switch (n->k) {
+      case K_AGG: {
+        // a nested aggregation (CodeGen.scala:108-135): its one alternative
+        // (flatten distributed it over Or) aggregated into a local best; the
+        // candidate needs one, and the best one's splits join the
+        // alternative's positions (genUnapp passes through, CodeGen.scala:218)
+        if (mrefs) mid_ok = false;
+        AltCode::Nest nest;
+        nest.id = std::to_string(nagg++);
+        nest.h = n->agg;
+        nest.sib = n->sib < 0 ? 0 : n->sib;
+        const std::vector<NP> one{n->l};
+        for (auto& sn : n->sib < 0 ? one : n->sibs) {
+          auto alt = std::make_shared<AltCode>();
+          alt->rule = o.rule;
+          std::vector<std::string> iv;
+          if (!gen(sn.get(), a, b, *alt, iv)) return false;
+          cur = &o;
+          if (iv.size() != 1) {
+            err = "a nested aggregation's alternative does not yield one answer (map it with a function)";
+            return false;
+          }
+          alt->value = iv[0];
+          nest.alts.push_back(alt);
+        }
+        const AltCode& in = *nest.alts[nest.sib];
+        for (size_t q = 0; q < in.pos.size(); ++q) {
+          o.pos.push_back("_g" + nest.id + "p" + std::to_string(q));
+          o.var_of_pos.push_back(in.var_of_pos[q]);
+        }
+        for (auto& k : in.kids) o.kids.push_back(k);
+        vals.push_back("_g" + nest.id + "c");
+        o.nested.push_back(nest);
+        return true;
+      }
       case K_TAB: {
         const Tab& T = m.tabs[n->tab];
         if (m.tracks == 1) scs(T.minv, T.maxv, a, b, o);
@@ -1234,7 +1304,14 @@ bool generate(Model& m, std::string& src, std::string& err) {
         err = "alternative " + std::to_string(o.rule) + " of '" + T.name + "' does not yield one answer (map the concatenation with a function)";
         return false;
       }
-      if (!o.loops.empty()) m.loops = true;
+      std::function<bool(const AltCode&)> any_loops = [&](const AltCode& x) {
+        if (!x.loops.empty()) return true;
+        for (auto& N : x.nested)
+          for (auto& a : N.alts)
+            if (any_loops(*a)) return true;
+        return false;
+      };
+      if (any_loops(o)) m.loops = true;
       o.value = vals[0];
       alts[t].push_back(o);
     }
@@ -1565,6 +1642,82 @@ bool generate(Model& m, std::string& src, std::string& err) {
   s << "}\n"
        "__device__ __forceinline__ void store_cell(const Dims& z, const Tabs& T, int i, int j, const CellV& cv) {\n"
        "  store_at(T, idx_(z, i, j), cv);\n}\n";
+  // a nested aggregation's block: its accumulator, its alternative's loops
+  // (one lane, in order) and conditions by loop level, the first strictly
+  // better candidate kept (the Scala aggregation's first maximum)
+  auto better_of = [&](const Agg& h, const std::string& c, const std::string& c0) {
+    switch (h.k) {
+      case A_MAX: return c + ">" + c0;
+      case A_MIN: return c + "<" + c0;
+      case A_MAXBY: return "f_" + m.fns[h.fn].name + "(_x," + c + ")>f_" + m.fns[h.fn].name + "(_x," + c0 + ")";
+      case A_MINBY: return "f_" + m.fns[h.fn].name + "(_x," + c + ")<f_" + m.fns[h.fn].name + "(_x," + c0 + ")";
+      default: return std::string("false");
+    }
+  };
+  std::function<void(const AltCode::Nest&, const std::string&)> emit_nest = [&](const AltCode::Nest& N, const std::string& ind0) {
+    const std::string C = "_g" + N.id + "c", R = "_g" + N.id + "r", W = "_g" + N.id + "w";
+    size_t npos = 0;
+    for (auto& a : N.alts) npos = std::max(npos, a->pos.size());
+    s << ind0 << "answer_t " << C << "{};\n" << ind0 << "int " << R << " = -1, " << W << " = -1;\n";
+    for (size_t q = 0; q < npos; ++q) s << ind0 << "int _g" << N.id << "p" << q << " = 0;\n";
+    for (int w = 0; w < (int)N.alts.size(); ++w) {
+    const AltCode& in = *N.alts[w];
+    std::string ind = ind0;
+    s << ind << "{\n";
+    int close = 1;
+    const int NL = (int)in.loops.size();
+    std::vector<std::vector<std::string>> at(NL + 1);
+    auto level = [&](const std::string& c) {
+      int lv = 0;
+      for (int q = 0; q < NL; ++q) {
+        const std::string& v = in.loops[q].var;
+        for (size_t pos = c.find(v); pos != std::string::npos; pos = c.find(v, pos + 1)) {
+          const char nx = pos + v.size() < c.size() ? c[pos + v.size()] : ' ';
+          const char pv = pos ? c[pos - 1] : ' ';
+          if (!isalnum((unsigned char)nx) && nx != '_' && !isalnum((unsigned char)pv) && pv != '_') { lv = q + 1; break; }
+        }
+      }
+      return lv;
+    };
+    for (auto& c : in.arith) at[level(c)].push_back(c);
+    for (size_t q = 0; q < in.other.size(); ++q) at[in.other_free[q] ? level(in.other[q]) : NL].push_back(in.other[q]);
+    auto open_if = [&](int lv) {
+      if (at[lv].empty()) return;
+      ind += "  ";
+      s << ind << "if (" << join(at[lv], " && ") << ") {\n";
+      ++close;
+    };
+    open_if(0);
+    for (int q = 0; q < NL; ++q) {
+      const Loop& L = in.loops[q];
+      ind += "  ";
+      s << ind << "for (int " << L.var << " = " << bound(L.lo, "max") << ", " << L.var << "_u = " << bound(L.hi, "min") << "; "
+        << L.var << " <= " << L.var << "_u; ++" << L.var << ") {\n";
+      ++close;
+      open_if(q + 1);
+    }
+    ind += "  ";
+    std::vector<std::string> post;
+    for (auto& M : in.nested) {
+      emit_nest(M, ind);
+      post.push_back("(_g" + M.id + "w == " + std::to_string(M.sib) + ")");
+    }
+    if (!post.empty()) {
+      s << ind << "if (" << join(post, " && ") << ") {\n";
+      ++close;
+      ind += "  ";
+    }
+    s << ind << "const answer_t _c = " << in.value << ";\n"
+      << ind << "if (" << better_of(N.h, "_c", C) << " || " << R << " == -1) { " << C << " = _c; " << R << " = 1; " << W
+      << " = " << w << ";";
+    for (size_t q = 0; q < in.pos.size(); ++q) s << " _g" << N.id << "p" << q << " = " << in.pos[q] << ";";
+    s << " }\n";
+    for (; close > 0; --close) {
+      ind.resize(ind.size() - 2);
+      s << ind << "}\n";
+    }
+    }
+  };
   auto emit_tabs = [&](const std::vector<std::vector<AltCode>>& A, const std::vector<std::vector<std::array<std::string, 4>>>& PD,
                        const std::string& sx, const std::string& extra, bool tile) {
   for (int t : m.order) {
@@ -1681,6 +1834,16 @@ bool generate(Model& m, std::string& src, std::string& err) {
         open_if(q + 1);
       }
       ind += "  ";
+      if (!o.nested.empty()) {
+        std::vector<std::string> post;
+        for (auto& N : o.nested) {
+          emit_nest(N, ind);
+          post.push_back("(_g" + N.id + "w == " + std::to_string(N.sib) + ")");
+        }
+        s << ind << "if (" << join(post, " && ") << ") {\n";
+        ++close;
+        ind += "  ";
+      }
       const std::string take = "_r0 = " + std::to_string(o.rule) + "; _p0 = " + (o.pos.size() > 0 ? o.pos[0] : "0") +
                                "; _p1 = " + (o.pos.size() > 1 ? o.pos[1] : "0") + "; _o0 = " +
                                (o.loops.size() > 0 ? o.loops[0].var : "0") + "; _o1 = " +

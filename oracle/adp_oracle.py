@@ -18,6 +18,12 @@ Python loops (small inputs only):
   side for p ~ p), -~ splitting track 1, ~- splitting track 2;
 * backtrace (CodeGen.scala:213-252): BFS over (i, j, tabulation id) records;
 * window (CodeGen.scala:325-348, ADPParsers.scala:9-13);
+* nested aggregations (an aggregate below a concatenation or a map,
+  CodeGen.scala:31-36,108-135): distributed over their Or alternatives like
+  the reference's norm; each yields the first best candidate of its
+  alternative (the Scala `aggr` over that list, BaseParsers.scala:228), whose
+  split positions join the alternative's; the backtrace passes through them
+  (genUnapp, CodeGen.scala:218);
 * sum / count (BaseParsers.scala:145-153,385-392, the Scala `aggr`): the
   sum of the candidates' values (count: their number), a cell without a
   candidate has no answer; no backtrace (BaseParsers.scala:150: no candidate
@@ -241,6 +247,8 @@ class Oracle:
             k += self._alt(self.tabs[t]["p"])
         self.n_rules = k
         self.catn = {t: self._cat(self.tabs[t]["p"]) for t in self.used}
+        if max(self.catn.values()) > 2:  # trace records hold two split positions (as the library)
+            raise ValueError("a rule needs more than 2 split positions")
         for t in self.used:
             self.alts[t] = self._flatten(self.tabs[t]["p"])
         self.tab_of_rule = {}
@@ -248,17 +256,23 @@ class Oracle:
             for q in range(len(self.alts[t])):
                 self.tab_of_rule[self.id[t] + q] = (t, q)
 
-    def _flatten(self, n):
+    def _flatten(self, n, top=True):
         if "or" in n:
-            return [a for x in n["or"] for a in self._flatten(x)]
+            return [a for x in n["or"] for a in self._flatten(x, top)]
         if "agg" in n:
-            return self._flatten(n["agg"])
+            if top:  # the tabulation's own aggregation
+                return self._flatten(n["agg"], True)
+            # nested: one copy per inner alternative, numbered like the
+            # reference's norm; each aggregates the union of them (the Scala
+            # parsers' semantics) and counts only when its own alternative wins
+            sibs = self._flatten(n["agg"], False)
+            return [dict(n, agg=a, _sibs=sibs, _k=q) for q, a in enumerate(sibs)]
         if "map" in n:
-            return [dict(n, map=a) for a in self._flatten(n["map"])]
+            return [dict(n, map=a) for a in self._flatten(n["map"], False)]
         if "filter" in n:
-            return [dict(n, filter=a) for a in self._flatten(n["filter"])]
+            return [dict(n, filter=a) for a in self._flatten(n["filter"], top)]
         if "cat" in n:
-            return [dict(n, cat=[a, b]) for a in self._flatten(n["cat"][0]) for b in self._flatten(n["cat"][1])]
+            return [dict(n, cat=[a, b]) for a in self._flatten(n["cat"][0], False) for b in self._flatten(n["cat"][1], False)]
         return [n]
 
     # ---- aggregation order ------------------------------------------------------
@@ -271,12 +285,22 @@ class Oracle:
     def _additive(self):
         return self.h in ("sum", "count")
 
-    def _better(self, c, c0):
-        h = self.h
+    def _better(self, c, c0, h=None):
+        h = self.h if h is None else h
         kind = h if isinstance(h, str) else list(h.keys())[0]
+        key = (lambda v: v) if isinstance(h, str) else self.f[list(h.values())[0]]
         if kind in ("max", "maxBy"):
-            return self._key(c) > self._key(c0)
-        return self._key(c) < self._key(c0)
+            return key(c) > key(c0)
+        return key(c) < key(c0)
+
+    def _first_best(self, cands, h):
+        """The first best of a list of candidates (value, positions, kids)
+        by aggregation h, or None for an empty list."""
+        best = None
+        for c in cands:
+            if best is None or self._better(c[0][0], best[0][0], h):
+                best = c
+        return best
 
     # ---- evaluation -------------------------------------------------------------
     def _cands(self, n, a, b, n1, n2):
@@ -333,6 +357,20 @@ class Oracle:
             if not self.f[n["f"]](a, b):
                 return
             yield from self._cands(n["filter"], a, b, n1, n2)
+            return
+        if "agg" in n:  # a nested aggregation: the first best of its alternatives' union
+            if n["h"] in ("sum", "count"):
+                raise ValueError("nested sum/count aggregations are not supported")
+            sibs, k = n.get("_sibs", [n["agg"]]), n.get("_k", 0)
+            best, won = None, -1
+            for q, alt in enumerate(sibs):
+                for c in self._cands(alt, a, b, n1, n2):
+                    if len(c[0]) != 1:
+                        raise ValueError("nested aggregation without one value")
+                    if best is None or self._better(c[0][0], best[0][0], n["h"]):
+                        best, won = c, q
+            if best is not None and won == k:
+                yield best
             return
         if "map" in n:
             for v, p, k in self._cands(n["map"], a, b, n1, n2):
@@ -486,29 +524,51 @@ class Oracle:
         return out
 
     # ---- independent checks -----------------------------------------------------
-    def enumerate_all(self, in1, in2=None, limit=200000):
+    def enumerate_all(self, in1, in2=None, limit=200000, aggregate_tabs=False):
         """Values of every derivation of the axiom (no aggregation): the
-        reference's Scala `apply` semantics on the original tree."""
+        reference's Scala `apply` semantics on the original tree.  With
+        aggregate_tabs, a tabulation referenced inside a derivation yields
+        its aggregated list (its first best by h), as a Scala Tabulate
+        memoises (BaseParsers.scala:200-212): the semantics a nested
+        aggregation with another h than the tabulations it reads sees."""
         self.in1, self.in2 = list(in1), list(in2 or [])
         n1, n2 = len(self.in1), len(self.in2)
         memo = {}
         count = [0]
 
-        def ap(n, a, b):
+        def ap(n, a, b, top=True):
             if "tab" in n:
                 key = (n["tab"], a, b)
                 if key not in memo:
                     memo[key] = None  # a cycle yields nothing
                     memo[key] = ap(self.tabs[n["tab"]]["p"], a, b)
-                return memo[key] or []
+                vs = memo[key] or []
+                if aggregate_tabs and vs and not top:
+                    best = None
+                    for v in vs:
+                        if best is None or self._better(_value(v), _value(best)):
+                            best = v
+                    vs = [best]
+                return vs
             if "or" in n:
-                return [v for x in n["or"] for v in ap(x, a, b)]
+                return [v for x in n["or"] for v in ap(x, a, b, top)]
             if "agg" in n:
-                return ap(n["agg"], a, b)
+                vs = ap(n["agg"], a, b, top)
+                if top or not vs:  # the tabulation's aggregation: every derivation
+                    return vs
+                if n["h"] in ("sum", "count"):
+                    raise ValueError("nested sum/count aggregations are not supported")
+                # nested (BaseParsers.scala:228): the first best of its list
+                best = None
+                for v in vs:
+                    x = _value(v)
+                    if best is None or self._better(x, _value(best), n["h"]):
+                        best = v
+                return [best]
             if "filter" in n:
-                return ap(n["filter"], a, b) if self.f[n["f"]](a, b) else []
+                return ap(n["filter"], a, b, top) if self.f[n["f"]](a, b) else []
             if "map" in n:
-                return [self.f[n["f"]](*_flat(v)) for v in ap(n["map"], a, b)]
+                return [self.f[n["f"]](*_flat(v)) for v in ap(n["map"], a, b, False)]
             if "term" in n:
                 return [v[0] if v else None for v, _, _ in self._cands(n, a, b, n1, n2)]
             l, r = n["cat"]
@@ -519,17 +579,17 @@ class Oracle:
                 lo = a + iL if (jU == INF or a + iL > b - jU) else b - jU
                 hi = b - jL if (iU == INF or b - jL < a + iU) else a + iU
                 for k in range(lo, hi + 1):
-                    L, Rr = ap(l, a, k), ap(r, k, b)
+                    L, Rr = ap(l, a, k, False), ap(r, k, b, False)
                     out += [_Pair(x, y) for x in L for y in Rr]
             elif tr == 1:
                 lo = 0 if (iU == INF or a - iU < 0) else a - iU
                 for k in range(lo, a - iL + 1):
-                    L, Rr = ap(l, k, a), ap(r, k, b)
+                    L, Rr = ap(l, k, a, False), ap(r, k, b, False)
                     out += [_Pair(x, y) for x in L for y in Rr]
             else:
                 lo = 0 if (jU == INF or b - jU < 0) else b - jU
                 for k in range(lo, b - jL + 1):
-                    L, Rr = ap(l, a, k), ap(r, k, b)
+                    L, Rr = ap(l, a, k, False), ap(r, k, b, False)
                     out += [_Pair(x, y) for x in L for y in Rr]
             count[0] += len(out)
             if count[0] > limit:
@@ -574,6 +634,8 @@ class Oracle:
                 return ev(n["filter"], a, b, base)
             if "map" in n:
                 return [self.f[n["f"]](*ev(n["map"], a, b, base))]
+            if "agg" in n:  # the record's positions name the nested best
+                return ev(n["agg"], a, b, base)
             (la, lb), (ra, rb), nb = self._split(n, a, b, pos, base)
             return ev(n["cat"][0], la, lb, base) + ev(n["cat"][1], ra, rb, nb)
 
@@ -586,6 +648,12 @@ class _Pair:
 
     def __init__(self, x, y):
         self.x, self.y = x, y
+
+
+def _value(v):
+    """The value of a derivation element (a mapped value; a lone leaf)."""
+    f = _flat(v)
+    return f[0] if len(f) == 1 else tuple(f)
 
 
 def _flat(v):

@@ -97,3 +97,26 @@ def levenshtein(a, b):
             cur[y] = min(prev[y] + 1, cur[y - 1] + 1, prev[y - 1] + (a[x - 1] != b[y - 1]))
         prev = cur
     return prev[len(b)]
+
+
+def nested_pairs():
+    """Nussinov with the closed pair's interior as a nested aggregation
+    (s -> empty | el s | (el (s s ^ split aggregate max) el filter bp) ^ pair),
+    the shape of the reference's aggregated loops (Zuker2.scala:111-112,
+    MatrixMult3.scala:41-43)."""
+    import dpx.adp as A
+    g = A.Grammar(tracks=1, h="max")
+    pairs = {("A", "U"), ("U", "A"), ("G", "C"), ("C", "G"), ("G", "U"), ("U", "G")}
+    nil = A.cfun(lambda: 0, "", "return 0;")
+    left = A.cfun(lambda c, a: a, "c,a", "return a;")
+    pair = A.cfun(lambda l, a, r: a + 1, "l,a,r", "return a+1;")
+    split = A.cfun(lambda l, r: l + r, "l,r", "return l+r;")
+    bp = A.cfun(lambda i, j: i + 2 <= j and (chr(g.in1(i)), chr(g.in1(j - 1))) in pairs, "i,j",
+                "if (i+2>j) return false; char a=_in1[i],b=_in1[j-1]; "
+                "return (a=='A'&&b=='U')||(a=='U'&&b=='A')||(a=='G'&&b=='C')||(a=='C'&&b=='G')||"
+                "(a=='G'&&b=='U')||(a=='U'&&b=='G');")
+    s = g.tabulate("s", lambda: ((g.empty ^ nil) | (g.el * s ^ left) |
+                                 ((g.el * (s * s ^ split).aggregate("max") * g.el).filter(bp) ^ pair)).aggregate("max"),
+                   always_valid=True)
+    g.axiom = s
+    return g

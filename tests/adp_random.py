@@ -186,3 +186,53 @@ def two_track(seed, agg=None):
 
 def text(rng, n, alphabet="abc"):
     return "".join(rng.choice(alphabet) for _ in range(n))
+
+
+def nested_track(seed):
+    """One-track grammars with nested aggregations: an aggregate (max or min,
+    over one or two alternatives) below a concatenation, mapped with the
+    monotone algebra functions, so the DP answer is the best derivation under
+    the nested aggregations' semantics (brute force checks it)."""
+    rng = random.Random(seed)
+    h = rng.choice(["max", "min"])
+    g = A.Grammar(tracks=1, h=h, window=rng.choice([0, 0, 5]), header=HEADER)
+    names = ["T0", "T1"][:rng.randint(1, 2)]
+    tabs = {nm: g.tabulate(nm, None) for nm in names}
+
+    def leaf():
+        r = rng.random()
+        if r < 0.5:
+            return tabs[rng.choice(names)]
+        return g.el if r < 0.75 else g.eli
+
+    def nest():
+        def one():
+            k = rng.choice([2, 2, 3])
+            parts = [leaf() for _ in range(k)]
+            p = parts[0]
+            for q in parts[1:]:
+                p = p * q
+            return p ^ _mapfn(rng, k)
+        inner = one() if rng.random() < 0.6 else (one() | one())
+        return inner.aggregate(rng.choice([h, "max", "min"]))
+
+    def alt():
+        parts = [leaf() for _ in range(rng.choice([1, 2, 2]))]
+        parts[rng.randrange(len(parts))] = nest()
+        p = parts[0]
+        for q in parts[1:]:
+            p = p * q
+        return p ^ _mapfn(rng, len(parts))
+
+    for nm in names:
+        alts = [alt() for _ in range(rng.randint(1, 3))]
+        alts.insert(rng.randint(0, len(alts)), g.el ^ _mapfn(rng, 1))
+
+        def thunk(alts=alts):
+            p = alts[0]
+            for a in alts[1:]:
+                p = p | a
+            return p.aggregate(h)
+        tabs[nm].thunk = thunk
+    g.axiom = tabs[names[0]]
+    return g

@@ -418,3 +418,36 @@ def test_band_pipeline_empty_value(ctx):
         assert found and ans == G.levenshtein(a, b), (len(a), len(b))
     got = cg.parse_batch(pairs)
     assert [x[0] for x in got] == [G.levenshtein(a, b) for a, b in pairs]
+
+
+def test_nested_aggregations_vs_oracle(ctx):
+    """Nested aggregations (an aggregate below a concatenation, with the
+    union semantics of the Scala parsers): seeded random grammars and a
+    Nussinov variant with an aggregated pair interior, single parses (answer,
+    found flag, full trace) and one batch against the oracle."""
+    import adp_random as R
+    import dpx
+    from dpx import gen
+    done = 0
+    cases = [(R.nested_track(seed), seed) for seed in range(24)] + [(G.nested_pairs(), 500)]
+    for g, seed in cases:
+        try:
+            G.oracle(g)
+        except ValueError:
+            with pytest.raises(dpx.DpxError):
+                ctx.grammar(g)
+            continue
+        print("nested aggregation grammar seed %d" % seed, flush=True)
+        cg = ctx.grammar(g)
+        rng = random.Random(2000 + seed)
+        ins = []
+        for _ in range(4):
+            s = R.text(rng, rng.randint(0, 20)) if seed < 500 else gen.rna_uniform(rng.randint(0, 60), rng.randint(0, 99))
+            _vs_oracle(cg, g, s)
+            ins.append(s)
+        got = cg.parse_batch(ins)
+        for q, s in enumerate(ins):
+            assert got[q] == cg.parse(s), (seed, q)
+        cg.close()
+        done += 1
+    assert done > 15
