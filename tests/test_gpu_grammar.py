@@ -112,6 +112,29 @@ def test_brackets_window_vs_oracle(ctx):
         _vs_oracle(cg, g, s)
 
 
+def test_long_inputs_past_the_lds_copy(ctx):
+    """One-track tiles keep the whole input in LDS up to 16 KB; longer inputs
+    are read through two windows around the tile and the memory fallback:
+    the windowed Brackets grammar on 17000 characters against the oracle,
+    and MatrixMult2.scala at n = 2100 (8-byte elements: past the copy)
+    against dpx_matchain."""
+    g = G.brackets(8)
+    cg = ctx.grammar(g)
+    rng = random.Random(7)
+    s = "".join(rng.choice("()((12345))6789") for _ in range(17000))
+    print("brackets grammar n=%d" % len(s), flush=True)
+    _vs_oracle(cg, g, s)
+    g = G.matrix_chain()
+    cg = ctx.grammar(g)
+    dims = [rng.randint(1, 100) for _ in range(2101)]
+    mats = [(dims[q], dims[q + 1]) for q in range(2100)]
+    print("chain grammar n=2100", flush=True)
+    ans, found, tr = cg.parse(mats)
+    c, btr = ctx.matchain([m[0] for m in mats], [m[1] for m in mats])
+    assert found and ans == c
+    assert [t[:4] for t in tr] == [t[:4] for t in btr]
+
+
 @pytest.mark.parametrize("alg", ["sw", "nw"])
 def test_seqalign_vs_oracle(ctx, alg):
     from dpx import gen
