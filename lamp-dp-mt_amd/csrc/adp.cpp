@@ -1491,23 +1491,24 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "__builtin_offsetof(Inst, T) == 72 && sizeof(Inst) == 72 + " << 32 * NT << ", \"instance layout\");\n"
     << rna_prelude(m)
     << "// what every algebra function sees: the inputs, their sizes, the energy tables\n"
-       "struct Ctx { const input_t* in1; const input_t* in2; int n1, n2; const RnaPar* rna; const int* hl; int wa, wb, wn; };\n"
+       "#define LDS_ __attribute__((address_space(3)))\n"
+       "struct Ctx { const input_t* in1; const input_t* in2; int n1, n2; const RnaPar* rna; const int* hl; int wa, wb, wn;\n"
+       "             const LDS_ input_t* wp; };\n"
     << (m.tracks == 1 ? "#define M_W (_x.n1 + 1)\n#define M_H (_x.n1 + 1)\n" : "#define M_H (_x.n1 + 1)\n#define M_W (_x.n2 + 1)\n")
     // one track: input reads go through a view that serves a tile's two
     // input windows from LDS (origins wa, wb, stored + 2^30 so that a Ctx
     // built without them, wa = wb = 0, never matches) and the rest from memory
     << (m.tracks == 1
             ? "__shared__ input_t tinA_[128], tinB_[128];\n"
-              "#define INW_CAP_ (16384 / (int)sizeof(input_t))\n"
-              "__shared__ input_t tinW_[INW_CAP_];  // the whole input, when it fits\n"
+
               "// the loads that miss the LDS copies sit behind a call, so the code\n"
               "// around an LDS read never waits for memory loads it does not use\n"
               "template <class V> __device__ __attribute__((noinline, pure)) V gld_(const V* p, long long x) {\n"
               "  return ((const GLOBAL_ V*)p)[x];\n}\n"
               "struct InV {\n"
-              "  const input_t* g; int wa, wb, wn;\n"
+              "  const input_t* g; int wa, wb, wn; const LDS_ input_t* wp;  // wp: the whole input, when it fits\n"
               "  __device__ __forceinline__ input_t operator[](int x) const {\n"
-              "    if (wn) return tinW_[min(max(x, 0), wn - 1)];\n"
+              "    if (wn) return wp[min(max(x, 0), wn - 1)];\n"
               "    const unsigned a = (unsigned)(x - wa + (1 << 30));\n"
               "    if (a < 128u) return tinA_[a];\n"
               "    const unsigned b = (unsigned)(x - wb + (1 << 30));\n"
@@ -1515,7 +1516,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
               "    return gld_(g, x);\n"
               "  }\n"
               "};\n"
-              "#define IN1V_(x) (InV{(x).in1, (x).wa, (x).wb, (x).wn})\n"
+              "#define IN1V_(x) (InV{(x).in1, (x).wa, (x).wb, (x).wn, (x).wp})\n"
               "#define IN1T_ const InV\n"
             : "#define IN1V_(x) ((x).in1)\n#define IN1T_ const input_t*\n")
     << (m.rna ? rna_functions() : std::string());
@@ -2205,14 +2206,14 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  }\n"
          "  if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl[0] = 0; ctrl[1] = seeds; ctrl[2] = 0; }\n"
          "}\n";
-    s << "template <int NT_> __device__ __forceinline__ void tile_(const Inst& q, int TI_, int TJ_, MidSm& sm) {\n"
+    s << "template <int NT_> __device__ __forceinline__ void tile_(const Inst& q, int TI_, int TJ_, MidSm& sm, input_t* tw, int wcap) {\n"
          "  const Dims z = q.z;\n  const Tabs T = q.T;\n"
          "  const int i0 = TI_ << 5, j0 = TJ_ << 5;\n"
          "  // the input in LDS: whole when it fits, else the tile's windows\n"
          "  // i0 - 32 .. i0 + 95 and j0 - 64 .. j0 + 63\n"
-         "  const bool whole = z.n1 + 1 <= INW_CAP_;\n"
+         "  const bool whole = z.n1 + 1 <= wcap;\n"
          "  if (whole) {\n"
-         "    for (int e = threadIdx.x; e <= z.n1; e += NT_) tinW_[e] = ((const GLOBAL_ input_t*)q.in1)[e];\n"
+         "    for (int e = threadIdx.x; e <= z.n1; e += NT_) tw[e] = ((const GLOBAL_ input_t*)q.in1)[e];\n"
          "  } else if (threadIdx.x < 256) {\n"
          "    const int x = threadIdx.x < 128 ? i0 - 32 + threadIdx.x : j0 - 64 + (threadIdx.x - 128);\n"
          "    const input_t v = ((const GLOBAL_ input_t*)q.in1)[min(max(x, 0), z.n1)];\n"
@@ -2220,7 +2221,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  }\n"
          "  __syncthreads();\n"
          "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl, i0 - 32 + (1 << 30), j0 - 64 + (1 << 30),\n"
-         "               whole ? z.n1 + 1 : 0};\n"
+         "               whole ? z.n1 + 1 : 0, (const LDS_ input_t*)tw};\n"
       << "  const int D = " << (m.window > 0 ? "min(" + std::to_string(m.window) + ", z.n1)" : std::string("z.n1")) << ";\n"
       << "  const int KLO = i0 + TB + " << mid_lo << ", KHI = j0 - 1 - " << mid_hi << ";\n"
       << "  const bool _mid = " << (has_mid ? "TJ_ - TI_ >= 2 && KLO <= KHI" : "false") << ";\n"
@@ -2313,6 +2314,10 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "                                                          int ntiles, unsigned long long* queue, int* ctrl, int* cnt) {\n"
          "  __shared__ MidSm sm;\n"
          "  __shared__ unsigned long long s_item;\n"
+         "  // the whole input in LDS when it fits: 16 KB for a 1024-thread tile\n"
+         "  // (few tiles per CU), 4 KB for 256 threads (several tiles per CU)\n"
+         "  constexpr int WCAP = (NT_ == 1024 ? 16384 : 4096) / (int)sizeof(input_t);\n"
+         "  __shared__ input_t tinW[WCAP];\n"
          "  for (;;) {\n"
          "    if (threadIdx.x < 64) {  // wave 0 claims, with wave-uniform control flow\n"
          "      int slot = 0;\n"
@@ -2340,7 +2345,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "    __syncthreads();\n"
          "    if (it == ~0ull) break;\n"
          "    const int q = (int)(it >> 40), TI_ = (int)((it >> 20) & 0xfffff), TJ_ = (int)(it & 0xfffff);\n"
-         "    tile_<NT_>(I[q], TI_, TJ_, sm);\n"
+         "    tile_<NT_>(I[q], TI_, TJ_, sm, tinW, WCAP);\n"
          "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"workgroup\");\n"
          "    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are done\n"
          "    __syncthreads();\n"
