@@ -292,7 +292,12 @@ int dpx_server_get_stats(dpx_server* srv, dpx_server_stats* out);
  * generated code's; trace records hold up to 2 split positions.  "sum" and
  * "count" (BaseParsers.scala:385-392) add up a cell's candidates (count:
  * their number) and have no backtrace: answer and found flag only, trace
- * length 0 (BaseParsers.scala:150). */
+ * length 0 (BaseParsers.scala:150).  An "agg" below a "cat" or "map" is a
+ * nested aggregation (max/min/maxBy/minBy): the best candidate of the union
+ * of its alternatives, as the Scala parsers aggregate it
+ * (BaseParsers.scala:228), under the rule ids of the reference's
+ * normalisation (CodeGen.scala:31-36); the winner's splits join the rule's
+ * positions.  Nested sum/count are rejected. */
 typedef struct dpx_grammar dpx_grammar;
 typedef struct {
   int32_t tracks, n_tabs, n_rules, max_pos, axiom_id, window;
