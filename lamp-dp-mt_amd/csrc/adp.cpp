@@ -1171,7 +1171,24 @@ std::string rna_prelude(const Model& m) {
        "  short int11[8][8][5][5];\n"
        "  short int21[8][8][5][5][5];\n"
        "  short int22[8][8][5][5][5][5];\n"
-       "};\n";
+       "};\n"
+       "// the small tables (all but int21 and int22), as RnaPar starts: a copy\n"
+       "// in LDS serves the energy functions of the tile kernels\n"
+       "struct RnaHead {\n"
+       "  int stack[8][8];\n"
+       "  int mmH[8][5][5], mmI[8][5][5], mm1nI[8][5][5], mm23I[8][5][5], mmM[8][5][5], mmExt[8][5][5];\n"
+       "  int d5[8][5], d3[8][5];\n"
+       "  int hairpin[31], bulge[31];\n"
+       "  int il_ent[61];\n"
+       "  short F[31][31];\n"
+       "  int ninio0, ninio_max, il5, ml_closing, ml_intern, term_au;\n"
+       "  int n_loop[3];\n"
+       "  int loop_key[3][40];\n"
+       "  int loop_e[3][40];\n"
+       "  short int11[8][8][5][5];\n"
+       "};\n"
+       "static_assert(__builtin_offsetof(RnaHead, int11) == __builtin_offsetof(RnaPar, int11) && sizeof(RnaHead) % 4 == 0 &&\n"
+       "              sizeof(RnaHead) <= __builtin_offsetof(RnaPar, int21), \"energy table head\");\n";
   if (m.rna) {
 #define OFF(f) << " && __builtin_offsetof(RnaPar, " #f ") == " << offsetof(DevRna, f)
     s << "static_assert(sizeof(RnaPar) == " << sizeof(DevRna) OFF(mmH) OFF(mmExt) OFF(d5) OFF(hairpin) OFF(bulge) OFF(il_ent)
@@ -1202,9 +1219,9 @@ __device__ __forceinline__ int rS_(const Ctx& x, int k) { return IN1V_(x)[k]; }
 __device__ __forceinline__ int rBP_(const Ctx& x, int a, int b) { return bp_index(rS_(x, a), rS_(x, b)); }
 __device__ __forceinline__ int rTermau_(const Ctx& x, int i, int j) {
   const int p = rS_(x, i), q = rS_(x, j);
-  return ((p == 3 && q == 2) || (p == 2 && q == 3)) ? 0 : x.rna->term_au;
+  return ((p == 3 && q == 2) || (p == 2 && q == 3)) ? 0 : x.rh->term_au;
 }
-__device__ __forceinline__ int rSr_(const Ctx& x, int i, int j) { return x.rna->stack[rBP_(x, i, j)][rBP_(x, j - 1, i + 1)]; }
+__device__ __forceinline__ int rSr_(const Ctx& x, int i, int j) { return x.rh->stack[rBP_(x, i, j)][rBP_(x, j - 1, i + 1)]; }
 __device__ __forceinline__ int rHl_(const Ctx& x, int i, int j) {
   const int size = j - i - 1;
   if (size < 3) return 600;
@@ -1212,27 +1229,28 @@ __device__ __forceinline__ int rHl_(const Ctx& x, int i, int j) {
   if (cls >= 0) {
     int key = 0;
     for (int m = 0; m <= size + 1; ++m) key = key * 8 + rS_(x, i + m);
-    for (int e = 0; e < x.rna->n_loop[cls]; ++e)
-      if (x.rna->loop_key[cls][e] == key) return x.rna->loop_e[cls][e];
+    for (int e = 0; e < x.rh->n_loop[cls]; ++e)
+      if (x.rh->loop_key[cls][e] == key) return x.rh->loop_e[cls][e];
   }
   const int ent = x.hl[size];
   if (size == 3) return ent + rTermau_(x, i, j);
-  return ent + x.rna->mmH[rBP_(x, i, j)][rS_(x, i + 1)][rS_(x, j - 1)];
+  return ent + x.rh->mmH[rBP_(x, i, j)][rS_(x, i + 1)][rS_(x, j - 1)];
 }
 __device__ __forceinline__ int rBl_(const Ctx& x, int i, int k, int l, int j) {
   const int size = l - k + 1;
   if (size < 0 || size > 30) return 10000000;
-  if (size <= 1) return (size ? x.rna->bulge[1] : 0) + x.rna->stack[rBP_(x, i, j)][rBP_(x, j - 1, l + 1)];
-  return x.rna->bulge[size] + rTermau_(x, i, j) + rTermau_(x, j - 1, l + 1);
+  if (size <= 1) return (size ? x.rh->bulge[1] : 0) + x.rh->stack[rBP_(x, i, j)][rBP_(x, j - 1, l + 1)];
+  return x.rh->bulge[size] + rTermau_(x, i, j) + rTermau_(x, j - 1, l + 1);
 }
 __device__ __forceinline__ int rBr_(const Ctx& x, int i, int k, int l, int j) {
   const int size = l - k + 1;
   if (size < 0 || size > 30) return 10000000;
-  if (size <= 1) return (size ? x.rna->bulge[1] : 0) + x.rna->stack[rBP_(x, i, j)][rBP_(x, k - 1, i + 1)];
-  return x.rna->bulge[size] + rTermau_(x, i, j) + rTermau_(x, k - 1, i + 1);
+  if (size <= 1) return (size ? x.rh->bulge[1] : 0) + x.rh->stack[rBP_(x, i, j)][rBP_(x, k - 1, i + 1)];
+  return x.rh->bulge[size] + rTermau_(x, i, j) + rTermau_(x, k - 1, i + 1);
 }
 __device__ __forceinline__ int rIl_(const Ctx& x, int i, int k, int l, int j) {
-  const RnaPar* P = x.rna;
+  const LDS_ RnaHead* P = x.rh;  // the small tables, in LDS
+  const RnaPar* Q = x.rna;        // int21, int22 in memory
   const int sl = k - i - 1, sr = j - l - 1;
   if (sl == 0) return rBr_(x, i, l + 1, j - 1, j);
   if (sr == 0) return rBl_(x, i, i + 1, k - 1, j);
@@ -1241,12 +1259,12 @@ __device__ __forceinline__ int rIl_(const Ctx& x, int i, int k, int l, int j) {
   const int olb = rS_(x, i + 1), orb = rS_(x, j - 1), ilb = rS_(x, l + 1), irb = rS_(x, k - 1);
   if (sl == 1) {
     if (sr == 1) return P->int11[ocb][rBP_(x, j - 2, i + 2)][olb][orb];
-    if (sr == 2) return P->int21[ocb][rBP_(x, j - 3, i + 2)][olb][rS_(x, j - 2)][orb];
+    if (sr == 2) return Q->int21[ocb][rBP_(x, j - 3, i + 2)][olb][rS_(x, j - 2)][orb];
     return P->F[sl][sr] + P->mm1nI[ocb][olb][orb] + P->mm1nI[icb][ilb][irb];
   }
   if (sl == 2) {
-    if (sr == 1) return P->int21[rBP_(x, j - 2, i + 3)][ocb][orb][olb][rS_(x, i + 2)];
-    if (sr == 2) return P->int22[ocb][rBP_(x, j - 3, i + 3)][olb][rS_(x, i + 2)][rS_(x, j - 2)][orb];
+    if (sr == 1) return Q->int21[rBP_(x, j - 2, i + 3)][ocb][orb][olb][rS_(x, i + 2)];
+    if (sr == 2) return Q->int22[ocb][rBP_(x, j - 3, i + 3)][olb][rS_(x, i + 2)][rS_(x, j - 2)][orb];
     if (sr == 3) return P->il5 + P->ninio0 + P->mm23I[ocb][olb][orb] + P->mm23I[icb][ilb][irb];
   } else if (sl == 3 && sr == 2) {
     return P->il5 + P->ninio0 + P->mm23I[ocb][olb][orb] + P->mm23I[icb][ilb][irb];
@@ -1257,12 +1275,12 @@ __device__ __forceinline__ int rIl_(const Ctx& x, int i, int k, int l, int j) {
 }
 __device__ __forceinline__ int rExt_(const Ctx& x, int i, int j) {
   const int b = rBP_(x, i, j);
-  if (i > 0 && j + 1 < x.n1) return x.rna->mmExt[b][rS_(x, i - 1)][rS_(x, j + 1)];
-  if (i > 0) return x.rna->d5[b][rS_(x, i - 1)];
-  if (j + 1 < x.n1) return x.rna->d3[b][rS_(x, j + 1)];
+  if (i > 0 && j + 1 < x.n1) return x.rh->mmExt[b][rS_(x, i - 1)][rS_(x, j + 1)];
+  if (i > 0) return x.rh->d5[b][rS_(x, i - 1)];
+  if (j + 1 < x.n1) return x.rh->d3[b][rS_(x, j + 1)];
   return 0;
 }
-__device__ __forceinline__ int rMlmm_(const Ctx& x, int i, int j) { return x.rna->mmM[rBP_(x, j, i)][rS_(x, j - 1)][rS_(x, i + 1)]; }
+__device__ __forceinline__ int rMlmm_(const Ctx& x, int i, int j) { return x.rh->mmM[rBP_(x, j, i)][rS_(x, j - 1)][rS_(x, i + 1)]; }
 #define hl_energy(i, j) rHl_(_x, (i), (j))
 #define bl_energy(i, k, l, j, xr) rBl_(_x, (i), (k), (l), (j))
 #define br_energy(i, k, l, j, xl) rBr_(_x, (i), (k), (l), (j))
@@ -1271,8 +1289,8 @@ __device__ __forceinline__ int rMlmm_(const Ctx& x, int i, int j) { return x.rna
 #define termau_energy(i, j) rTermau_(_x, (i), (j))
 #define ext_mismatch_energy(i, j) rExt_(_x, (i), (j))
 #define ml_mismatch_energy(i, j) rMlmm_(_x, (i), (j))
-#define ml_energy() (_x.rna->ml_closing)
-#define ul_energy() (_x.rna->ml_intern)
+#define ml_energy() (_x.rh->ml_closing)
+#define ul_energy() (_x.rh->ml_intern)
 )";
 }
 
@@ -1493,7 +1511,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
     << "// what every algebra function sees: the inputs, their sizes, the energy tables\n"
        "#define LDS_ __attribute__((address_space(3)))\n"
        "struct Ctx { const input_t* in1; const input_t* in2; int n1, n2; const RnaPar* rna; const int* hl; int wa, wb, wn;\n"
-       "             const LDS_ input_t* wp; };\n"
+       "             const LDS_ input_t* wp; const LDS_ RnaHead* rh; };\n"
     << (m.tracks == 1 ? "#define M_W (_x.n1 + 1)\n#define M_H (_x.n1 + 1)\n" : "#define M_H (_x.n1 + 1)\n#define M_W (_x.n2 + 1)\n")
     // one track: input reads go through a view that serves a tile's two
     // input windows from LDS (origins wa, wb, stored + 2^30 so that a Ctx
@@ -2003,8 +2021,12 @@ bool generate(Model& m, std::string& src, std::string& err) {
   s << "template <int G> __device__ __forceinline__ void fill_(const Inst* __restrict__ I, int d0, int d1) {\n"
        "  const Inst& q = I[blockIdx.y];  // instances sorted by size, largest first: blockIdx.y < active ones\n"
        "  const Dims z = q.z;\n  const Tabs T = q.T;  // in registers: the stores below cannot change it\n"
-       "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl};\n"
-       "  const int tid = blockIdx.x * blockDim.x + threadIdx.x, lane = tid & (G - 1);\n"
+    << (m.rna ? "  __shared__ RnaHead rhead;\n"
+                "  for (int e = threadIdx.x; e < (int)(sizeof(RnaHead) / 4); e += blockDim.x) ((int*)&rhead)[e] = ((const int*)q.rna)[e];\n"
+                "  __syncthreads();\n"
+                "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl, 0, 0, 0, nullptr, (const LDS_ RnaHead*)&rhead};\n"
+              : "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl};\n")
+    << "  const int tid = blockIdx.x * blockDim.x + threadIdx.x, lane = tid & (G - 1);\n"
        "  for (int d = d0; d < d1; ++d) {\n";
   if (m.tracks == 1)
     s << "    const int D = " << (m.window > 0 ? "min(" + std::to_string(m.window) + ", z.n1)" : std::string("z.n1")) << ";\n"
@@ -2206,7 +2228,8 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  }\n"
          "  if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl[0] = 0; ctrl[1] = seeds; ctrl[2] = 0; }\n"
          "}\n";
-    s << "template <int NT_> __device__ __forceinline__ void tile_(const Inst& q, int TI_, int TJ_, MidSm& sm, input_t* tw, int wcap) {\n"
+    s << "template <int NT_> __device__ __forceinline__ void tile_(const Inst& q, int TI_, int TJ_, MidSm& sm, input_t* tw, int wcap,\n"
+         "                                                      const RnaHead* rhp) {\n"
          "  const Dims z = q.z;\n  const Tabs T = q.T;\n"
          "  const int i0 = TI_ << 5, j0 = TJ_ << 5;\n"
          "  // the input in LDS: whole when it fits, else the tile's windows\n"
@@ -2221,7 +2244,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  }\n"
          "  __syncthreads();\n"
          "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl, i0 - 32 + (1 << 30), j0 - 64 + (1 << 30),\n"
-         "               whole ? z.n1 + 1 : 0, (const LDS_ input_t*)tw};\n"
+         "               whole ? z.n1 + 1 : 0, (const LDS_ input_t*)tw, (const LDS_ RnaHead*)rhp};\n"
       << "  const int D = " << (m.window > 0 ? "min(" + std::to_string(m.window) + ", z.n1)" : std::string("z.n1")) << ";\n"
       << "  const int KLO = i0 + TB + " << mid_lo << ", KHI = j0 - 1 - " << mid_hi << ";\n"
       << "  const bool _mid = " << (has_mid ? "TJ_ - TI_ >= 2 && KLO <= KHI" : "false") << ";\n"
@@ -2318,7 +2341,13 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  // (few tiles per CU), 4 KB for 256 threads (several tiles per CU)\n"
          "  constexpr int WCAP = (NT_ == 1024 ? 16384 : 4096) / (int)sizeof(input_t);\n"
          "  __shared__ input_t tinW[WCAP];\n"
-         "  for (;;) {\n"
+         << (m.rna ? "  // the energy tables' small head, once per workgroup (every instance shares the tables)\n"
+                     "  __shared__ RnaHead rhead;\n"
+                     "  for (int e = threadIdx.x; e < (int)(sizeof(RnaHead) / 4); e += NT_) ((int*)&rhead)[e] = ((const GLOBAL_ int*)I[0].rna)[e];\n"
+                     "  __syncthreads();\n"
+                     "  const RnaHead* rhp = &rhead;\n"
+                   : "  const RnaHead* rhp = nullptr;\n")
+         << "  for (;;) {\n"
          "    if (threadIdx.x < 64) {  // wave 0 claims, with wave-uniform control flow\n"
          "      int slot = 0;\n"
          "      if (threadIdx.x == 0) slot = __hip_atomic_fetch_add(&ctrl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
@@ -2345,7 +2374,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "    __syncthreads();\n"
          "    if (it == ~0ull) break;\n"
          "    const int q = (int)(it >> 40), TI_ = (int)((it >> 20) & 0xfffff), TJ_ = (int)(it & 0xfffff);\n"
-         "    tile_<NT_>(I[q], TI_, TJ_, sm, tinW, WCAP);\n"
+         "    tile_<NT_>(I[q], TI_, TJ_, sm, tinW, WCAP, rhp);\n"
          "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"workgroup\");\n"
          "    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are done\n"
          "    __syncthreads();\n"
