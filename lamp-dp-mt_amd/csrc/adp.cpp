@@ -1206,14 +1206,11 @@ std::string rna_prelude(const Model& m) {
 // (the reference's grammars bound them with seq(1,30)).
 std::string rna_functions() {
   return R"(#define NO_BP 7
+// pair type without branches: three bits per (x - 1) * 5 + y (CG 1, GC 2,
+// GU 3, UG 4, AU 5, UA 6, anything else 7)
 __device__ __forceinline__ int bp_index(int x, int y) {
-  switch (x) {
-    case 1: if (y == 4) return 5; break;
-    case 2: if (y == 3) return 1; break;
-    case 3: if (y == 2) return 2; if (y == 4) return 3; break;
-    case 4: if (y == 3) return 4; if (y == 1) return 6; break;
-  }
-  return 7;
+  const unsigned f = (unsigned)((x - 1) * 5 + y);
+  return (unsigned)(x - 1) < 4u && (unsigned)y < 5u ? (int)((0xf3eefaff9ffdfffull >> (3 * f)) & 7) : 7;
 }
 __device__ __forceinline__ int rS_(const Ctx& x, int k) { return IN1V_(x)[k]; }
 __device__ __forceinline__ int rBP_(const Ctx& x, int a, int b) { return bp_index(rS_(x, a), rS_(x, b)); }
@@ -1252,26 +1249,27 @@ __device__ __forceinline__ int rIl_(const Ctx& x, int i, int k, int l, int j) {
   const LDS_ RnaHead* P = x.rh;  // the small tables, in LDS
   const RnaPar* Q = x.rna;        // int21, int22 in memory
   const int sl = k - i - 1, sr = j - l - 1;
-  if (sl == 0) return rBr_(x, i, l + 1, j - 1, j);
-  if (sr == 0) return rBl_(x, i, i + 1, k - 1, j);
-  if (sl < 0 || sr < 0 || sl > 30 || sr > 30) return 10000000;
+  if (sl <= 0 || sr <= 0 || sl > 30 || sr > 30) {  // bulges, and sizes outside the tables
+    if (sl == 0) return rBr_(x, i, l + 1, j - 1, j);
+    if (sr == 0) return rBl_(x, i, i + 1, k - 1, j);
+    return 10000000;
+  }
   const int ocb = rBP_(x, i, j), icb = rBP_(x, l, k);
   const int olb = rS_(x, i + 1), orb = rS_(x, j - 1), ilb = rS_(x, l + 1), irb = rS_(x, k - 1);
-  if (sl == 1) {
-    if (sr == 1) return P->int11[ocb][rBP_(x, j - 2, i + 2)][olb][orb];
-    if (sr == 2) return Q->int21[ocb][rBP_(x, j - 3, i + 2)][olb][rS_(x, j - 2)][orb];
-    return P->F[sl][sr] + P->mm1nI[ocb][olb][orb] + P->mm1nI[icb][ilb][irb];
-  }
-  if (sl == 2) {
+  if (sl <= 2 && sr <= 2) {  // 1x1, 1x2, 2x1, 2x2: the loop tables
+    if (sl == 1 && sr == 1) return P->int11[ocb][rBP_(x, j - 2, i + 2)][olb][orb];
+    if (sl == 1) return Q->int21[ocb][rBP_(x, j - 3, i + 2)][olb][rS_(x, j - 2)][orb];
     if (sr == 1) return Q->int21[rBP_(x, j - 2, i + 3)][ocb][orb][olb][rS_(x, i + 2)];
-    if (sr == 2) return Q->int22[ocb][rBP_(x, j - 3, i + 3)][olb][rS_(x, i + 2)][rS_(x, j - 2)][orb];
-    if (sr == 3) return P->il5 + P->ninio0 + P->mm23I[ocb][olb][orb] + P->mm23I[icb][ilb][irb];
-  } else if (sl == 3 && sr == 2) {
-    return P->il5 + P->ninio0 + P->mm23I[ocb][olb][orb] + P->mm23I[icb][ilb][irb];
-  } else if (sr == 1) {
-    return P->F[sl][sr] + P->mm1nI[ocb][olb][orb] + P->mm1nI[icb][ilb][irb];
+    return Q->int22[ocb][rBP_(x, j - 3, i + 3)][olb][rS_(x, i + 2)][rS_(x, j - 2)][orb];
   }
-  return P->F[sl][sr] + P->mmI[ocb][olb][orb] + P->mmI[icb][ilb][irb];
+  // every other size takes one path (the lanes of a wave differ in sr): the
+  // size term, and the outer and inner mismatch from one of mmI (generic),
+  // mm1nI (1xn) and mm23I (2x3, with il5 + ninio0 for the size), which lie
+  // one after another
+  const int t = sl == 1 || sr == 1 ? 1 : (sl == 2 && sr == 3) || (sl == 3 && sr == 2) ? 2 : 0;
+  const LDS_ int* mm = (const LDS_ int*)((const LDS_ char*)P + __builtin_offsetof(RnaHead, mmI)) + t * 200;
+  const int size = t == 2 ? P->il5 + P->ninio0 : P->F[sl][sr];
+  return size + mm[ocb * 25 + olb * 5 + orb] + mm[icb * 25 + ilb * 5 + irb];
 }
 __device__ __forceinline__ int rExt_(const Ctx& x, int i, int j) {
   const int b = rBP_(x, i, j);
@@ -1453,11 +1451,19 @@ bool generate(Model& m, std::string& src, std::string& err) {
     for (auto& r : mrefs) (r.left ? strip_l : strip_r).insert(r.t);
     for (int t : strip_l) strips += 2048 * ((size_t)m.answer_bytes + (tile_rules.count(t) ? 2 : 0));
     for (int t : strip_r) strips += 2048 * ((size_t)m.answer_bytes + (tile_rules.count(t) ? 2 : 0));
-    if (64 * 64 * per + mid_lds + strips <= 128 * 1024) reg_h = 32;
+    // a workgroup may hold up to 160 KB of LDS; above 80 KB one workgroup
+    // runs per CU, which the 1024-thread tiles need anyway, so a large
+    // grammar (several tabulations, energy tables) takes the whole region
+    // and one tile per CU rather than reading its halo from memory
+    const size_t big = 64 * 64 * per + mid_lds;
+    const size_t head = m.rna ? offsetof(DevRna, int21) : 0;  // the energy tables' head staged per workgroup
+    if (big + strips <= 128 * 1024) reg_h = 32;
+    else if (big <= 96 * 1024) reg_h = 32, strip_l.clear(), strip_r.clear();
+    else if (big + strips + head + 1024 <= 160 * 1024) reg_h = 32;
     else {
       strip_l.clear();
       strip_r.clear();
-      if (64 * 64 * per + mid_lds <= 96 * 1024) reg_h = 32;
+      if (big + head + 1024 <= 160 * 1024) reg_h = 32;
       else if (32 * 32 * per + mid_lds <= 96 * 1024) reg_h = 0;
     }
     if (!gen_tile(reg_h, true)) return false;
@@ -2016,28 +2022,6 @@ bool generate(Model& m, std::string& src, std::string& err) {
     s << " int pad_; };\n";
     emit_tabs(alts_r, pre_decl_r, "R", ", const Regs& R", false);
   }
-  // fill: one diagonal per launch (d0 == d1 - 1), or a run of small
-  // diagonals by one workgroup with a barrier between them
-  s << "template <int G> __device__ __forceinline__ void fill_(const Inst* __restrict__ I, int d0, int d1) {\n"
-       "  const Inst& q = I[blockIdx.y];  // instances sorted by size, largest first: blockIdx.y < active ones\n"
-       "  const Dims z = q.z;\n  const Tabs T = q.T;  // in registers: the stores below cannot change it\n"
-    << (m.rna ? "  __shared__ RnaHead rhead;\n"
-                "  for (int e = threadIdx.x; e < (int)(sizeof(RnaHead) / 4); e += blockDim.x) ((int*)&rhead)[e] = ((const int*)q.rna)[e];\n"
-                "  __syncthreads();\n"
-                "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl, 0, 0, 0, nullptr, (const LDS_ RnaHead*)&rhead};\n"
-              : "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl};\n")
-    << "  const int tid = blockIdx.x * blockDim.x + threadIdx.x, lane = tid & (G - 1);\n"
-       "  for (int d = d0; d < d1; ++d) {\n";
-  if (m.tracks == 1)
-    s << "    const int D = " << (m.window > 0 ? "min(" + std::to_string(m.window) + ", z.n1)" : std::string("z.n1")) << ";\n"
-      << "    const int cnt = d <= D ? z.n1 + 1 - d : 0;\n";
-  else
-    s << "    const int ilo = d > z.n2 ? d - z.n2 : 0, ihi = d < z.n1 ? d : z.n1, cnt = ihi - ilo + 1;\n";
-  s << "    for (int t = tid / G; t < cnt; t += gridDim.x * blockDim.x / G) {\n";
-  if (m.tracks == 1)
-    s << "      const int i = t, j = t + d;\n";
-  else
-    s << "      const int i = ilo + t, j = d - i;\n";
   // every tabulation's reads of other cells first, all in flight together
   // (pinned here rather than sunk into the rules), then the tabulations
   auto cell_body = [&](const std::string& ind, const std::string& G, const std::string& lane, const std::string& defer) {
@@ -2048,12 +2032,26 @@ bool generate(Model& m, std::string& src, std::string& err) {
     for (int t : m.order)
       s << ind << "tab_" << t << "<" << G << ", " << defer << ">(_x, z, T, i, j, " << lane << ", cv, p" << t << ");\n";
   };
-  cell_body("      ", "G", "lane", "false");
-  s << "    }\n"
-       "    if (d + 1 < d1) __syncthreads();\n"
-       "  }\n}\n";
-  for (int G : {1, 8, 64})
-    s << "extern \"C\" __global__ void adp_fill_" << G << "(const Inst* I, int d0, int d1) { fill_<" << G << ">(I, d0, d1); }\n";
+  // fill: one diagonal per launch (d0 == d1 - 1), or a run of small
+  // diagonals by one workgroup with a barrier between them (two tracks;
+  // one-track grammars run on the tile queue)
+  if (m.tracks == 2) {
+    s << "template <int G> __device__ __forceinline__ void fill_(const Inst* __restrict__ I, int d0, int d1) {\n"
+         "  const Inst& q = I[blockIdx.y];  // instances sorted by size, largest first: blockIdx.y < active ones\n"
+         "  const Dims z = q.z;\n  const Tabs T = q.T;  // in registers: the stores below cannot change it\n"
+         "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl};\n"
+         "  const int tid = blockIdx.x * blockDim.x + threadIdx.x, lane = tid & (G - 1);\n"
+         "  for (int d = d0; d < d1; ++d) {\n"
+         "    const int ilo = d > z.n2 ? d - z.n2 : 0, ihi = d < z.n1 ? d : z.n1, cnt = ihi - ilo + 1;\n"
+         "    for (int t = tid / G; t < cnt; t += gridDim.x * blockDim.x / G) {\n"
+         "      const int i = ilo + t, j = d - i;\n";
+    cell_body("      ", "G", "lane", "false");
+    s << "    }\n"
+         "    if (d + 1 < d1) __syncthreads();\n"
+         "  }\n}\n";
+    for (int G : {1, 8, 64})
+      s << "extern \"C\" __global__ void adp_fill_" << G << "(const Inst* I, int d0, int d1) { fill_<" << G << ">(I, d0, d1); }\n";
+  }
   if (m.tracks == 1) {
     const int MIDM = Gen::MIDM;
     const std::string init = m.empty.empty() ? "answer_t{}" : "(answer_t)(" + m.empty + ")";
@@ -2228,14 +2226,14 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  }\n"
          "  if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl[0] = 0; ctrl[1] = seeds; ctrl[2] = 0; }\n"
          "}\n";
-    s << "template <int NT_> __device__ __forceinline__ void tile_(const Inst& q, int TI_, int TJ_, MidSm& sm, input_t* tw, int wcap,\n"
-         "                                                      const RnaHead* rhp) {\n"
+    s << "// WH_: the whole input fits the tile's LDS copy (every input read is one\n"
+         "// LDS read, no range tests); else the tile's windows i0 - 32 .. i0 + 95 and\n"
+         "// j0 - 64 .. j0 + 63, and memory beyond them\n"
+         "template <int NT_, bool WH_> __device__ __forceinline__ void tile_(const Inst& q, int TI_, int TJ_, MidSm& sm, input_t* tw,\n"
+         "                                                               const RnaHead* rhp) {\n"
          "  const Dims z = q.z;\n  const Tabs T = q.T;\n"
          "  const int i0 = TI_ << 5, j0 = TJ_ << 5;\n"
-         "  // the input in LDS: whole when it fits, else the tile's windows\n"
-         "  // i0 - 32 .. i0 + 95 and j0 - 64 .. j0 + 63\n"
-         "  const bool whole = z.n1 + 1 <= wcap;\n"
-         "  if (whole) {\n"
+         "  if (WH_) {\n"
          "    for (int e = threadIdx.x; e <= z.n1; e += NT_) tw[e] = ((const GLOBAL_ input_t*)q.in1)[e];\n"
          "  } else if (threadIdx.x < 256) {\n"
          "    const int x = threadIdx.x < 128 ? i0 - 32 + threadIdx.x : j0 - 64 + (threadIdx.x - 128);\n"
@@ -2244,7 +2242,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  }\n"
          "  __syncthreads();\n"
          "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl, i0 - 32 + (1 << 30), j0 - 64 + (1 << 30),\n"
-         "               whole ? z.n1 + 1 : 0, (const LDS_ input_t*)tw, (const LDS_ RnaHead*)rhp};\n"
+         "               WH_ ? max(z.n1 + 1, 1) : 0, (const LDS_ input_t*)tw, (const LDS_ RnaHead*)rhp};\n"
       << "  const int D = " << (m.window > 0 ? "min(" + std::to_string(m.window) + ", z.n1)" : std::string("z.n1")) << ";\n"
       << "  const int KLO = i0 + TB + " << mid_lo << ", KHI = j0 - 1 - " << mid_hi << ";\n"
       << "  const bool _mid = " << (has_mid ? "TJ_ - TI_ >= 2 && KLO <= KHI" : "false") << ";\n"
@@ -2374,7 +2372,8 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "    __syncthreads();\n"
          "    if (it == ~0ull) break;\n"
          "    const int q = (int)(it >> 40), TI_ = (int)((it >> 20) & 0xfffff), TJ_ = (int)(it & 0xfffff);\n"
-         "    tile_<NT_>(I[q], TI_, TJ_, sm, tinW, WCAP, rhp);\n"
+         "    if (I[q].z.n1 + 1 <= WCAP) tile_<NT_, true>(I[q], TI_, TJ_, sm, tinW, rhp);\n"
+         "    else tile_<NT_, false>(I[q], TI_, TJ_, sm, tinW, rhp);\n"
          "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"workgroup\");\n"
          "    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are done\n"
          "    __syncthreads();\n"
@@ -2740,9 +2739,9 @@ Grammar* compile(const char* spec, bool load, std::string& err, int* code) {
   if (load) {
     if (code) *code = DPX_EHIP;
     if (hipModuleLoadData(&g->mod, g->code.data()) != hipSuccess ||
-        hipModuleGetFunction(&g->fill[0], g->mod, "adp_fill_1") != hipSuccess ||
-        hipModuleGetFunction(&g->fill[1], g->mod, "adp_fill_8") != hipSuccess ||
-        hipModuleGetFunction(&g->fill[2], g->mod, "adp_fill_64") != hipSuccess ||
+        (g->m.tracks == 2 && (hipModuleGetFunction(&g->fill[0], g->mod, "adp_fill_1") != hipSuccess ||
+                              hipModuleGetFunction(&g->fill[1], g->mod, "adp_fill_8") != hipSuccess ||
+                              hipModuleGetFunction(&g->fill[2], g->mod, "adp_fill_64") != hipSuccess)) ||
         hipModuleGetFunction(&g->finish, g->mod, "adp_finish") != hipSuccess ||
         (g->m.tracks == 2 && !g->m.loops && hipModuleGetFunction(&g->band, g->mod, "adp_band") != hipSuccess) ||
         (g->m.tracks == 1 && (hipModuleGetFunction(&g->tile_reset, g->mod, "adp_tile_reset") != hipSuccess ||
@@ -3016,11 +3015,12 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
     const unsigned rgrid = (unsigned)std::min<int64_t>((ntiles + 255) / 256, 4096);
     if (!chk(hipModuleLaunchKernel(g->tile_reset, rgrid, 1, 1, 256, 1, 1, 0, s, rargs, nullptr), "tile reset")) return DPX_EHIP;
     // 1024 threads per tile when few tiles can be ready at once (each tile
-    // then runs fastest); 256 when the launch has many more ready tiles than
-    // compute units
+    // then runs fastest) or when the tile's LDS leaves room for one
+    // workgroup per CU either way; 256 when the launch has many more ready
+    // tiles than compute units
     int64_t ready = 0;
     for (auto& x : qs) ready += x.nb;
-    int big = ready <= 2 * (int64_t)g->cus ? 1 : 0;
+    int big = ready <= 2 * (int64_t)g->cus || g->occ[0] <= 1 ? 1 : 0;
     if (const char* e = std::getenv("DPX_ADP_TILE_THREADS")) big = std::atoi(e) == 1024 ? 1 : std::atoi(e) == 256 ? 0 : big;
     const unsigned grid = (unsigned)std::min<int64_t>(ntiles, (int64_t)g->cus * g->occ[big]);
     void* targs[] = {(void*)&d_inst, (void*)&d_tinf, (void*)&nt, (void*)&d_queue, (void*)&d_tctrl, (void*)&d_cnt};

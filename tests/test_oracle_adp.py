@@ -134,9 +134,13 @@ def _check(spec):
 @pytest.mark.parametrize("make", [G.nussinov, lambda: G.elmamun("min"), G.brackets, G.seqalign,
                                   lambda: G.seqalign("nw"), G.swat_affine, G.matrix_chain])
 def test_library_generates_and_compiles(make):
-    rc, src, msg = _check(make().spec_json())
+    g = make()
+    rc, src, msg = _check(g.spec_json())
     assert rc == 0, msg
-    assert "adp_fill" in src and "adp_finish" in src
+    # one track: the tile queue; two tracks: diagonal launches (and the band
+    # pipeline where there are no split loops)
+    assert ("adp_tile_1024" in src) == (g.tracks == 1) and ("adp_fill_1" in src) == (g.tracks == 2)
+    assert "adp_finish" in src
 
 
 def test_library_rejects_unsupported():
@@ -324,14 +328,16 @@ def test_sum_count_grammars_compile():
 def test_fill_strategy_of_generated_code():
     """Two-track grammars without split loops get the band-pipeline kernel,
     in its register form when every reference is (i, j - c) or (i - 1, j - c);
-    grammars with split loops keep one launch per diagonal."""
+    two-track grammars with split loops keep one launch per diagonal; one-track
+    grammars run on the tile queue alone."""
     import dpx
     for g, band, regs in [(G.swat_affine(), True, True), (G.seqalign("sw"), False, False),
                           (G.nussinov(), False, False), (G.matrix_chain(), False, False)]:
         rc, src, msg = _check(g.spec_json())
         assert rc == 0, msg
         assert ("void adp_band(" in src) == band and ("struct Regs" in src) == regs, g
-        assert "adp_fill_1" in src and "adp_finish" in src
+        assert ("adp_fill_1" in src) == (g.tracks == 2) and ("adp_tile_1024" in src) == (g.tracks == 1)
+        assert "adp_finish" in src
 
 
 def test_edit_distance_grammar_oracle_vs_textbook():
