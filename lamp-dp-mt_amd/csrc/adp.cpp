@@ -1147,6 +1147,88 @@ std::string join(const std::vector<std::string>& v, const char* sep) {
   return s;
 }
 
+// a left term of the middle product (its generated expression and the split
+// variable it names), computed once per staged block
+struct MidHoist { std::string expr, var; };
+
+// the top-level arguments of a generated call "f_name(a, b, ...)" ((), [] and
+// {} nest); false when e is not one call
+bool split_call(const std::string& e, std::string& head, std::vector<std::string>& args) {
+  if (e.compare(0, 2, "f_") != 0 || e.back() != ')') return false;
+  const size_t lp = e.find('(');
+  if (lp == std::string::npos) return false;
+  head = e.substr(0, lp);
+  for (char c : head)
+    if (!isalnum((unsigned char)c) && c != '_') return false;
+  args.clear();
+  int depth = 0;
+  size_t st = lp + 1;
+  for (size_t q = lp + 1; q < e.size(); ++q) {
+    const char c = e[q];
+    if (c == '(' || c == '[' || c == '{') {
+      ++depth;
+    } else if (c == ')' || c == ']' || c == '}') {
+      if (depth == 0) {
+        if (q != e.size() - 1) return false;
+        args.push_back(e.substr(st, q - st));
+        return true;
+      }
+      --depth;
+    } else if (c == ',' && depth == 0) {
+      args.push_back(e.substr(st, q - st));
+      st = q + 1;
+    }
+  }
+  return false;
+}
+
+// e names only the row i, the split variable, staged left operands and
+// inputs, the context, the algebra functions and constants: no column j, no
+// right operand, no other loop variable
+bool left_only(const std::string& e, const std::string& var) {
+  static const std::set<std::string> ok{"_x", "i", "_mr", "_mkk", "sm", "z", "T", "_in1", "_in2", "T2ii", "answer_t", "input_t",
+                                        "min", "max", "sidx_", "GLOBAL_", "const", "short", "int", "long", "char", "unsigned"};
+  if (e.find("sm.mR") != std::string::npos) return false;
+  for (size_t q = 0; q < e.size();) {
+    const char c = e[q];
+    if (isalpha((unsigned char)c) || c == '_') {
+      size_t r = q;
+      while (r < e.size() && (isalnum((unsigned char)e[r]) || e[r] == '_')) ++r;
+      const std::string id = e.substr(q, r - q);
+      const bool member = q > 0 && e[q - 1] == '.';
+      if (!member && !ok.count(id) && id != var && id.compare(0, 2, "f_") != 0) return false;
+      q = r;
+    } else if (isdigit((unsigned char)c)) {
+      while (q < e.size() && (isalnum((unsigned char)e[q]) || e[q] == '.')) ++q;  // a number and its suffix
+    } else {
+      ++q;
+    }
+  }
+  return true;
+}
+
+// e with each innermost call that takes a staged left operand directly and is
+// a left term replaced by its hoisted value (registered in H)
+std::string hoist_left(const std::string& e, const std::string& var, std::vector<MidHoist>& H) {
+  std::string head;
+  std::vector<std::string> args;
+  if (!split_call(e, head, args)) return e;
+  bool direct = false;
+  for (auto& a : args)
+    if (a.compare(0, 5, "sm.mL") == 0 && a.size() > 5 && isdigit((unsigned char)a[5]) && a.back() == ']' && a.find(',') == std::string::npos)
+      direct = true;
+  if (direct && left_only(e, var)) {
+    size_t h = 0;
+    while (h < H.size() && !(H[h].expr == e && H[h].var == var)) ++h;
+    if (h == H.size()) H.push_back({e, var});
+    const std::string hs = std::to_string(h);
+    return "mu" + hs + "_<MUH" + hs + "_>(_x, z, T, _in1, _in2, i, " + var + ", _mr, _mkk, sm)";
+  }
+  std::string r = head + "(";
+  for (size_t q = 0; q < args.size(); ++q) r += (q ? "," : "") + hoist_left(args[q], var, H);
+  return r + ")";
+}
+
 std::string bound(const std::vector<std::string>& v, const char* f) {
   std::string s = v[0];
   for (size_t q = 1; q < v.size(); ++q) s = std::string(f) + "(" + s + "," + v[q] + ")";
@@ -1376,8 +1458,17 @@ bool generate(Model& m, std::string& src, std::string& err) {
       }
     }
   }
+  // Left terms of the middle product: a call whose arguments are a staged
+  // left operand (i, k) and other terms of i and the split alone (no j, no
+  // right operand) has one value per row and split, shared by the row's 32
+  // cells and by every rule that makes it; it is computed once per staged
+  // block into LDS (mU<h>) when its type is the answer type, and read there
+  std::vector<MidHoist> mhoist;
+  for (int t : m.order)
+    for (auto& o : malts[t]) o.value = hoist_left(o.value, o.loops[0].var, mhoist);
   size_t mid_lds = 0;
   {
+    mid_lds += mhoist.size() * 1024 * (size_t)m.answer_bytes;
     for (auto& r : mrefs) mid_lds += 1024 * (size_t)m.answer_bytes + (r.rules ? 2048 : 0);
     mid_lds += minputs.size() * (32 + 2 * Gen::MIDM) * (size_t)m.alphabet_bytes;
     if (mid_lds > 48 * 1024) {  // too many operands to stage: the wavefront takes every split
@@ -1385,6 +1476,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
       for (auto& v : mrule) v.clear();
       mrefs.clear();
       minputs.clear();
+      mhoist.clear();
       has_mid = false;
       mid_lds = 0;
     }
@@ -1396,6 +1488,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
   std::vector<std::vector<AltCode>> alts_tile(m.tabs.size());
   std::set<int> tile_rules;
   std::set<int> strip_l, strip_r;  // tabulations with a left / right strip in LDS
+  size_t wcap_big = 16384;  // bytes of the whole-input copy of a 1024-thread tile
   int reg_h = -1;  // H, or -1: no region (every read from the tables in memory)
   std::set<int> mid_rule_ids;
   for (auto& v : mrule) mid_rule_ids.insert(v.begin(), v.end());
@@ -1443,7 +1536,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
       per += (size_t)m.answer_bytes + (tile_rules.count(t) ? 2 : 0);
       core += 1024 * (2 + 4 * (size_t)m.tabs[t].cat);
     }
-    mid_lds += core + 16384 + 256 * (size_t)m.alphabet_bytes;  // and the input copies
+    mid_lds += core + 256 * (size_t)m.alphabet_bytes;  // and the input windows
     // strips for the operands of the middle products: their splits left
     // to the in-tile pass read tiles (I, I), (I, I + 1) (left operands) and
     // (J - 1, J), (J, J) (right operands)
@@ -1455,8 +1548,15 @@ bool generate(Model& m, std::string& src, std::string& err) {
     // runs per CU, which the 1024-thread tiles need anyway, so a large
     // grammar (several tabulations, energy tables) takes the whole region
     // and one tile per CU rather than reading its halo from memory
-    const size_t big = 64 * 64 * per + mid_lds;
+    // The whole-input copy of a 1024-thread tile is 16 KB, or 8 KB where
+    // that keeps the strips of a large grammar in LDS
+    size_t big = 64 * 64 * per + mid_lds + 16384;
     const size_t head = m.rna ? offsetof(DevRna, int21) : 0;  // the energy tables' head staged per workgroup
+    if (big + strips > 128 * 1024 && big > 96 * 1024 && big + strips + head + 1024 > 160 * 1024 &&
+        big - 8192 + strips + head + 1024 <= 160 * 1024) {
+      big -= 8192;
+      wcap_big = 8192;
+    }
     if (big + strips <= 128 * 1024) reg_h = 32;
     else if (big <= 96 * 1024) reg_h = 32, strip_l.clear(), strip_r.clear();
     else if (big + strips + head + 1024 <= 160 * 1024) reg_h = 32;
@@ -1464,7 +1564,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
       strip_l.clear();
       strip_r.clear();
       if (big + head + 1024 <= 160 * 1024) reg_h = 32;
-      else if (32 * 32 * per + mid_lds <= 96 * 1024) reg_h = 0;
+      else if (32 * 32 * per + mid_lds + wcap_big <= 96 * 1024) reg_h = 0;
     }
     if (!gen_tile(reg_h, true)) return false;
   }
@@ -2057,7 +2157,7 @@ bool generate(Model& m, std::string& src, std::string& err) {
     const std::string init = m.empty.empty() ? "answer_t{}" : "(answer_t)(" + m.empty + ")";
     // staged operands of the middle product
     s << "#define MIDM " << MIDM << "\n"
-         "struct MidSm {";
+         "struct MidSmB {";
     for (size_t x = 0; x < mrefs.size(); ++x) {
       const std::string side = mrefs[x].left ? "mL" : "mR";
       s << " answer_t " << side << x << "[1024];";
@@ -2065,6 +2165,28 @@ bool generate(Model& m, std::string& src, std::string& err) {
     }
     for (int tr : minputs) s << " input_t mI" << tr << "[" << 32 + 2 * MIDM << "];";
     s << " int pad_; };\n";
+    // the left terms: kept in LDS (MUH<h>_) when their type is the answer type
+    s << "template <class A, class B> struct same_t_ { static constexpr bool v = false; };\n"
+         "template <class A> struct same_t_<A, A> { static constexpr bool v = true; };\n";
+    const std::string mu_params = "(const Ctx& _x, const Dims& z, const Tabs& T, IN1T_& _in1, const input_t* _in2, int i, int ";
+    for (size_t h = 0; h < mhoist.size(); ++h) {
+      const std::string hs = std::to_string(h);
+      s << "__host__ __device__ auto mu_t" << hs << "_" << mu_params << mhoist[h].var << ", int _mr, int _mkk, const MidSmB& sm) -> decltype("
+        << mhoist[h].expr << ");\n"
+        << "constexpr bool MUH" << hs << "_ = same_t_<decltype(mu_t" << hs << "_(*(const Ctx*)nullptr, *(const Dims*)nullptr, "
+        << "*(const Tabs*)nullptr, *(IN1T_*)nullptr, nullptr, 0, 0, 0, 0, *(const MidSmB*)nullptr)), answer_t>::v;\n";
+    }
+    s << "struct MidSm : MidSmB {";
+    for (size_t h = 0; h < mhoist.size(); ++h) s << " answer_t mU" << h << "[MUH" << h << "_ ? 1024 : 1];";
+    s << " int pad2_; };\n";
+    for (size_t h = 0; h < mhoist.size(); ++h) {
+      const std::string hs = std::to_string(h);
+      s << "template <bool H> __device__ __forceinline__ auto mu" << hs << "_" << mu_params << mhoist[h].var
+        << ", int _mr, int _mkk, const MidSm& sm) {\n"
+        << "  (void)_x; (void)z; (void)T; (void)_in1; (void)_in2; (void)i; (void)" << mhoist[h].var << ";\n"
+        << "  if constexpr (H) return sm.mU" << hs << "[_mr * 32 + _mkk];\n"
+        << "  else return " << mhoist[h].expr << ";\n}\n";
+    }
     if (has_mid) {
       // Middle product of tile (i0, j0): for every cell and alternative with
       // a middle product, the best candidate over the split points
@@ -2130,8 +2252,39 @@ bool generate(Model& m, std::string& src, std::string& err) {
       s << "    }\n";
       for (int tr : minputs) s << "    if (tid < " << 32 + 2 * MIDM << ") sm.mI" << tr << "[tid] = pfi" << tr << ";\n";
       s << "    if (K < (KHI >> 5)) {\n      const int KF = K + 1;\n" << fetch.str() << "    }\n";
-    s << "    __syncthreads();\n"
-           "    const int klo = max(KLO, K << 5), khi = min(KHI, (K << 5) + 31);\n"
+    s << "    __syncthreads();\n";
+      if (!mhoist.empty()) {
+        s << "    if (";
+        for (size_t h = 0; h < mhoist.size(); ++h) s << (h ? " || " : "") << "MUH" << h << "_";
+        s << ") {\n"
+             "#pragma unroll\n"
+             "      for (int w = 0; w < CPT; ++w) {\n"
+             "        const int e = tid + w * NT_, _mr = e >> 5, _mkk = e & 31, i = i0 + _mr, k_ = (K << 5) + _mkk;\n"
+             "        (void)i;\n"
+             "        if (k_ >= KLO && k_ <= KHI) {\n";
+        for (size_t h = 0; h < mhoist.size(); ++h) {
+          // evaluated only where its staged left operands are valid cells
+          std::vector<std::string> g;
+          const std::string& ex = mhoist[h].expr;
+          for (size_t q = ex.find("sm.mL"); q != std::string::npos; q = ex.find("sm.mL", q + 1)) {
+            size_t r = q + 5;
+            while (r < ex.size() && isdigit((unsigned char)ex[r])) ++r;
+            if (r == q + 5) continue;
+            const int x = std::stoi(ex.substr(q + 5, r - q - 5));
+            if (!m.empty.empty()) g.push_back("!(sm.mL" + std::to_string(x) + "[e]==(answer_t)(" + m.empty + "))");
+            else if (mrefs[x].rules) g.push_back("sm.mLR" + std::to_string(x) + "[e]!=-1");
+          }
+          s << "          if constexpr (MUH" << h << "_) {\n"
+            << "            const int " << mhoist[h].var << " = k_;\n"
+            << "            if (" << (g.empty() ? std::string("true") : join(g, " && ")) << ") sm.mU" << h << "[e] = " << ex << ";\n"
+            << "          }\n";
+        }
+        s << "        }\n"
+             "      }\n"
+             "      __syncthreads();\n"
+             "    }\n";
+      }
+      s << "    const int klo = max(KLO, K << 5), khi = min(KHI, (K << 5) + 31);\n"
            "    for (int k = klo; k <= khi; ++k) {\n"
            "      const int _mkk = k - (K << 5);\n"
            "#pragma unroll\n"
@@ -2336,8 +2489,9 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  __shared__ MidSm sm;\n"
          "  __shared__ unsigned long long s_item;\n"
          "  // the whole input in LDS when it fits: 16 KB for a 1024-thread tile\n"
-         "  // (few tiles per CU), 4 KB for 256 threads (several tiles per CU)\n"
-         "  constexpr int WCAP = (NT_ == 1024 ? 16384 : 4096) / (int)sizeof(input_t);\n"
+         "  // (few tiles per CU; 8 KB where the tables fill the LDS), 4 KB for\n"
+         "  // 256 threads (several tiles per CU)\n"
+         "  constexpr int WCAP = (NT_ == 1024 ? " << wcap_big << " : 4096) / (int)sizeof(input_t);\n"
          "  __shared__ input_t tinW[WCAP];\n"
          << (m.rna ? "  // the energy tables' small head, once per workgroup (every instance shares the tables)\n"
                      "  __shared__ RnaHead rhead;\n"
