@@ -1990,17 +1990,28 @@ bool generate(Model& m, std::string& src, std::string& err) {
     // all-reduce over the group: DPP lane moves within 16-lane rows (xor 1,
   // xor 2, half-row and row mirrors pair disjoint halves at each level),
   // lane shuffles across rows
+  // only what differs between the lanes moves: the second split when some
+  // rule has one, the order keys when they are not the splits themselves
+  // (one split loop per rule, its variable the first split)
+  bool need_p1 = false, o_is_p = true;
+  for (auto& o : A[t]) {
+    if (o.pos.size() >= 2 || o.loops.size() >= 2 || !o.nested.empty()) need_p1 = true;
+    if (!o.loops.empty() && (o.pos.empty() || o.var_of_pos[0] != o.loops[0].var)) o_is_p = false;
+  }
+  o_is_p = o_is_p && !need_p1;
   s << "  for (int lv = 0; (1 << lv) < G; ++lv) {\n"
        "    answer_t c;\n"
        "    int r, p0, p1, o0, o1;\n"
        "    if (lv < 4) {\n"
-       "      c = lvmove_(_c0, lv); r = lvmove_(_r0, lv); p0 = lvmove_(_p0, lv); p1 = lvmove_(_p1, lv);\n"
-       "      o0 = lvmove_(_o0, lv); o1 = lvmove_(_o1, lv);\n"
-       "    } else {\n"
+       "      c = lvmove_(_c0, lv); r = lvmove_(_r0, lv); p0 = lvmove_(_p0, lv);\n"
+    << (need_p1 ? "      p1 = lvmove_(_p1, lv);\n" : "      p1 = 0;\n")
+    << (o_is_p ? "      o0 = p0; o1 = p1;\n" : "      o0 = lvmove_(_o0, lv); o1 = lvmove_(_o1, lv);\n")
+    << "    } else {\n"
        "      const int off = 1 << lv;\n"
        "      c = shfl_xor_(_c0, off, G); r = __shfl_xor(_r0, off, G); p0 = __shfl_xor(_p0, off, G);\n"
-       "      p1 = __shfl_xor(_p1, off, G); o0 = __shfl_xor(_o0, off, G); o1 = __shfl_xor(_o1, off, G);\n"
-       "    }\n"
+    << (need_p1 ? "      p1 = __shfl_xor(_p1, off, G);\n" : "      p1 = 0;\n")
+    << (o_is_p ? "      o0 = p0; o1 = p1;\n" : "      o0 = __shfl_xor(_o0, off, G); o1 = __shfl_xor(_o1, off, G);\n")
+    << "    }\n"
        "    bool take;\n"
          "    if (r == -1) take = false;\n"
          "    else if (_r0 == -1) take = true;\n";
