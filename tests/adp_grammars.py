@@ -120,3 +120,23 @@ def nested_pairs():
                    always_valid=True)
     g.axiom = s
     return g
+
+
+def left_terms(wide):
+    """A Nussinov-shaped grammar whose split rule maps its left operand
+    before combining it: s -> empty | el s | (s ^ widen) s ^ comb.  widen is
+    a left term of the middle product (a call of the staged left operand
+    alone); it returns a long long (wide) or an int, so the generated code
+    keeps it in LDS only in the int case and evaluates it in place
+    otherwise.  comb(widen(a), b) = a + b + 1: monotone, with many ties."""
+    g = A.Grammar(tracks=1, h="max")
+    nil = A.cfun(lambda: 0, "", "return 0;")
+    left = A.cfun(lambda c, a: a, "c,a", "return a;")
+    widen = A.cfun(lambda a: 3 * a + 1, "a", "return (long long)a * 3 + 1;" if wide else "return a * 3 + 1;")
+    comb = A.cfun(lambda x, y: (x - 1) // 3 + y + 1, "x,y", "return (int)((x - 1) / 3) + y + 1;")
+    # both sides non-empty, as Concat.indices gives s ~ s (BaseParsers.scala:279-283)
+    s = g.tabulate("s", lambda: ((g.empty ^ nil) | (g.el * s ^ left) |
+                                 ((s ^ widen).cat(s, hint=(1, None, 1, None)) ^ comb)).aggregate("max"),
+                   always_valid=True)
+    g.axiom = s
+    return g

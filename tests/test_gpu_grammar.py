@@ -474,3 +474,20 @@ def test_nested_aggregations_vs_oracle(ctx):
         cg.close()
         done += 1
     assert done > 15
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_middle_product_left_terms(ctx, wide):
+    """A left term of the middle product (a call of the staged left operand
+    alone), kept in LDS when its type is the answer type (int) and evaluated
+    in place when it is not (long long): both against the oracle (answer and
+    full trace) at sizes whose tiles have a middle product."""
+    from dpx import gen
+    g = G.left_terms(wide)
+    cg = ctx.grammar(g)
+    rng = random.Random(31 + int(wide))
+    for n in (0, 1, 33, 97, 130, 200):
+        s = gen.rna_uniform(n, rng.randint(0, 99))
+        print("left-term grammar wide=%s n=%d" % (wide, n), flush=True)
+        _vs_oracle(cg, g, s)
+    cg.close()

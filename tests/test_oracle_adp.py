@@ -402,3 +402,17 @@ def test_nested_aggregation_grammars_compile():
     g.axiom = s
     rc, _, msg = _check(g.spec_json())
     assert rc != 0 and "nested" in msg
+
+
+def test_middle_product_left_terms_generated():
+    """Zuker.scala's multiloop term of the closing pair (i, k) is generated as
+    a left term of the middle product (one value per row and split, staged in
+    LDS); a left term of another type than the answer compiles too (it is
+    then evaluated in place)."""
+    rc, src, msg = _check(G.zuker().spec_json())
+    assert rc == 0, msg
+    assert "mu0_<MUH0_>(" in src and "struct MidSm : MidSmB" in src
+    for wide in (False, True):
+        rc, src, msg = _check(G.left_terms(wide).spec_json())
+        assert rc == 0, msg
+        assert "mu0_<MUH0_>(" in src
