@@ -271,6 +271,13 @@ def test_zuker_grammar_vs_reference_vectors_and_builtin(ctx):
     for s, r in zip(seqs, got):
         score, btr = ctx.zuker(s)
         assert r[0] == score and r[2] == btr, len(s)
+    # an input past the tile's 8 KB copy: the windowed input reads of a
+    # grammar that fills the LDS
+    s = gen.rna_gc(8300, 77)
+    print("zuker grammar n=%d (windowed input)" % len(s), flush=True)
+    ans, found, tr = cg.parse(s)
+    score, btr = ctx.zuker(s)
+    assert found and ans == score and tr == btr
 
 
 def test_rnafold_grammar_vs_builtin(ctx):
