@@ -35,7 +35,7 @@ using namespace dpx;
 
 namespace {
 
-constexpr int kMaxZuker = 14000;      // finalize keeps the st column (n+1 int32) and a 32x33 block in 64 KiB of LDS
+constexpr int kMaxZuker = 14000;      // finalize keeps the st column (n+1 int32), the sequence and the energy tables' head in LDS (87 KiB at this n)
 constexpr int kMaxTri = 30000;        // Nussinov / matrix chain
 constexpr int kHairpinTable = 32768;  // hairpin entropy entries kept on the device
 

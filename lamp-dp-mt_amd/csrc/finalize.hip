@@ -251,9 +251,21 @@ __global__ __launch_bounds__(NT) void mc_finalize(TriArgs a) {
 }
 
 // -------------------------------------------------------------------- Zuker
+// the energy tables' small head (every field before int21, 12 KB): the
+// backtrace's candidate energies read it from LDS, staged once per
+// workgroup, instead of through chains of dependent global loads
+constexpr int kRnaHeadWords = (int)(offsetof(DevRna, int21) / sizeof(int32_t));
+static_assert(offsetof(DevRna, int21) % sizeof(int32_t) == 0, "energy table head");
+__device__ __forceinline__ const DevRna* stage_rna_head(const void* g, int32_t* lds) {
+  for (int e = threadIdx.x; e < kRnaHeadWords; e += blockDim.x) lds[e] = ((const int32_t*)g)[e];
+  __syncthreads();
+  return (const DevRna*)lds;  // valid for the fields before int21
+}
+
 struct ZuF {
   const uint8_t* s;
-  const DevRna* P;
+  const DevRna* P;  // the head in LDS (fields before int21)
+  const DevRna* Q;  // the whole tables in memory (int21, int22)
   const int32_t* hl;
   int n;
   __device__ int S(int k) const { return s[k]; }
@@ -282,12 +294,12 @@ struct ZuF {
     const int ocb = bp(i, j), icb = bp(l, k);
     if (sl == 1) {
       if (sr_ == 1) return P->int11[ocb][bp(j - 2, i + 2)][S(i + 1)][S(j - 1)];
-      if (sr_ == 2) return P->int21[ocb][bp(j - 3, i + 2)][S(i + 1)][S(j - 2)][S(j - 1)];
+      if (sr_ == 2) return Q->int21[ocb][bp(j - 3, i + 2)][S(i + 1)][S(j - 2)][S(j - 1)];
       return P->F[sl][sr_] + P->mm1nI[ocb][S(i + 1)][S(j - 1)] + P->mm1nI[icb][S(l + 1)][S(k - 1)];
     }
     if (sl == 2) {
-      if (sr_ == 1) return P->int21[bp(j - 2, i + 3)][ocb][S(j - 1)][S(i + 1)][S(i + 2)];
-      if (sr_ == 2) return P->int22[ocb][bp(j - 3, i + 3)][S(i + 1)][S(i + 2)][S(j - 2)][S(j - 1)];
+      if (sr_ == 1) return Q->int21[bp(j - 2, i + 3)][ocb][S(j - 1)][S(i + 1)][S(i + 2)];
+      if (sr_ == 2) return Q->int22[ocb][bp(j - 3, i + 3)][S(i + 1)][S(i + 2)][S(j - 2)][S(j - 1)];
       if (sr_ == 3) return P->il5 + P->ninio0 + P->mm23I[ocb][S(i + 1)][S(j - 1)] + P->mm23I[icb][S(l + 1)][S(k - 1)];
     } else if (sl == 3 && sr_ == 2) {
       return P->il5 + P->ninio0 + P->mm23I[ocb][S(i + 1)][S(j - 1)] + P->mm23I[icb][S(l + 1)][S(k - 1)];
@@ -314,7 +326,10 @@ __global__ __launch_bounds__(NT) void zu_finalize(TriArgs a) {
   const TriInst in = a.inst[blockIdx.x];
   const Cells C{a.tab, in.nb, NT_ZU, in.tile0};
   const int n = in.n;
-  const ZuF z{a.codes + in.seq_off, (const DevRna*)a.rna, a.hl_ent, n};
+  __shared__ int32_t s_head[kRnaHeadWords];
+  uint8_t* s_seq = (uint8_t*)(st + n + 1);  // the sequence, after the st column
+  for (int e = threadIdx.x; e < n; e += NT) s_seq[e] = a.codes[in.seq_off + e];
+  const ZuF z{s_seq, stage_rna_head(a.rna, s_head), (const DevRna*)a.rna, a.hl_ent, n};
   const int mlint = z.P->ml_intern;
   // ---- st column (zu_kern.h:166-182): rule 12 st(i+1), rule 13 D(i,k)+st(k), rule 14 i==n.
   // Rows are taken in blocks of 32 from the bottom.  For a block [lo, hi] the
@@ -549,7 +564,10 @@ __global__ __launch_bounds__(NT) void rf_finalize(TriArgs a) {
   const TriInst in = a.inst[blockIdx.x];
   const Cells C{a.tab, in.nb, NT_RF, in.tile0};
   const int n = in.n;
-  const ZuF z{a.codes + in.seq_off, (const DevRna*)a.rna, a.hl_ent, n};
+  __shared__ int32_t s_head[kRnaHeadWords];
+  extern __shared__ uint8_t s_seq[];  // the sequence
+  for (int e = threadIdx.x; e < n; e += NT) s_seq[e] = a.codes[in.seq_off + e];
+  const ZuF z{s_seq, stage_rna_head(a.rna, s_head), (const DevRna*)a.rna, a.hl_ent, n};
   const int top = C.get(RT_Q, 0, n);
   if (threadIdx.x == 0) a.results[in.out] = top < INV_THR ? top : ZU_EMPTY;
   if (!a.want_trace) return;
@@ -643,7 +661,9 @@ int launch_finalize(int grammar, const TriArgs& a, int count, int n_max, void* s
   // a few instances: 16 waves each (more backtrace records per round, more
   // st split points in flight); a batch: 4 waves each, several per CU
   const bool big = count <= 32;
-  const size_t lds = grammar == G_ZUKER ? (size_t)(n_max + 1) * sizeof(int) : 0;
+  // Zuker: the st column (n + 1 ints) and the sequence; RNAfold: the sequence
+  const size_t seq_b = ((size_t)n_max + 4) & ~(size_t)3;
+  const size_t lds = grammar == G_ZUKER ? (size_t)(n_max + 1) * sizeof(int) + seq_b : grammar == G_RNAFOLD ? seq_b : 0;
 #define DPX_FIN(K)                                                          \
   do {                                                                      \
     if (big) hipLaunchKernelGGL(K<1024>, dim3(count), dim3(1024), lds, st, a); \
