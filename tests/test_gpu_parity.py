@@ -136,6 +136,29 @@ def test_swat_full_size_vs_oracle(ctx):
     assert [x[:3] for x in tr] == [x[:3] for x in otr]
 
 
+def test_zuker_longest_sequence(ctx):
+    """The longest sequence dpx_zuker accepts (14000 nt: the finalize keeps
+    the st column, the sequence and the energy tables' head in LDS) alone,
+    in a batch beside a short one and through RNAfold; one more base is
+    refused."""
+    import dpx
+    from dpx import gen
+    s = gen.rna_gc(14000, 11)
+    print("zuker n=14000", flush=True)
+    score, tr = ctx.zuker(s)
+    assert tr and all(0 <= t[0] <= t[1] <= len(s) for t in tr)
+    res, trs = ctx.zuker_batch([s, s[:500]])
+    assert res[0] == score and trs[0] == tr
+    alone = ctx.zuker(s[:500])
+    assert res[1] == alone[0] and trs[1] == alone[1]
+    print("rnafold n=14000", flush=True)
+    rs, rtr = ctx.rnafold(s)
+    assert ctx.rnafold_batch([s])[0][0] == rs
+    with pytest.raises(dpx.DpxError) as e:
+        ctx.zuker(s + "A")
+    assert e.value.code == dpx.EINVAL
+
+
 def test_errors(ctx):
     import dpx
     with pytest.raises(dpx.DpxError) as e:
