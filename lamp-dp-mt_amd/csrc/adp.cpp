@@ -2773,9 +2773,11 @@ bool generate(Model& m, std::string& src, std::string& err) {
          "  }\n"
          "}\n";
   }
-  // finish: axiom cell (or best window), answer, backtrace; one thread per instance
+  // finish: axiom cell (or best window), answer, backtrace; one wave per
+  // instance, every lane running the same walk (lane 0 alone writes)
   const Tab& AX = m.tabs[m.axiom];
-  s << "extern \"C\" __global__ void adp_finish(const Inst* I) {\n"
+  s << "extern \"C\" __global__ __launch_bounds__(64) void adp_finish(const Inst* I) {\n"
+       "  const int lane = threadIdx.x;\n"
        "  const Inst& q = I[blockIdx.x];\n"
        "  const Dims z = q.z;\n  const Tabs& T = q.T;\n"
        "  const Ctx _x{q.in1, q.in2, z.n1, z.n2, (const RnaPar*)q.rna, q.hl};\n"
@@ -2792,16 +2794,16 @@ bool generate(Model& m, std::string& src, std::string& err) {
           : m.h.k == A_COUNT ? "    if (found) best = answer_t(best + 1); else { best = answer_t(1); i0 = i; j0 = j; found = true; }\n"
                              : "    if (!found || " + better("c", "best") + ") { best = c; i0 = i; j0 = j; found = true; }\n")
       << "  }\n"
-      << "  *ans = best;  // zero when no window cell parses\n";
+      << "  if (lane == 0) *ans = best;  // zero when no window cell parses\n";
   } else {
     if (m.tracks == 1) s << "  i0 = 0; j0 = z.n1;\n";
     else s << "  i0 = z.n1; j0 = z.n2;\n";
     s << "  found = " << (AX.always_valid ? "true" : Gen(m).valid(m.axiom, "i0", "j0")) << ";\n"
-      << "  *ans = " << Gen(m).cost(m.axiom, "i0", "j0") << ";\n";
+      << "  if (lane == 0) *ans = " << Gen(m).cost(m.axiom, "i0", "j0") << ";\n";
   }
-  s << "  out[0] = found ? 1 : 0; out[2] = i0; out[3] = j0;\n"
-    << "  if (!tr || !found" << (m.additive() ? " || true) { out[1] = 0; return; }  // sums have no backtrace\n"
-                                             : ") { out[1] = 0; return; }\n")
+  s << "  if (lane == 0) { out[0] = found ? 1 : 0; out[2] = i0; out[3] = j0; }\n"
+    << "  if (!tr || !found" << (m.additive() ? " || true) { if (lane == 0) out[1] = 0; return; }  // sums have no backtrace\n"
+                                             : ") { if (lane == 0) out[1] = 0; return; }\n")
     << "  int wr = 0;\n"
        "  bool over = false;\n"
        // the queue's last 1024 records also in LDS: the walk reads them back
@@ -2809,40 +2811,60 @@ bool generate(Model& m, std::string& src, std::string& err) {
        "  __shared__ int qi_[1024], qj_[1024], qt_[1024];\n"
        "  auto push = [&](int i, int j, int rule) {\n"
        "    if (wr < cap) {\n"
-       "      tr[wr].i = i; tr[wr].j = j; tr[wr].rule = rule; tr[wr].pos[0] = 0; tr[wr].pos[1] = 0;\n"
-       "      qi_[wr & 1023] = i; qj_[wr & 1023] = j; qt_[wr & 1023] = rule;\n"
+       "      if (lane == 0) {\n"
+       "        tr[wr].i = i; tr[wr].j = j; tr[wr].rule = rule; tr[wr].pos[0] = 0; tr[wr].pos[1] = 0;\n"
+       "        qi_[wr & 1023] = i; qj_[wr & 1023] = j; qt_[wr & 1023] = rule;\n"
+       "      }\n"
        "      ++wr;\n"
        "    } else over = true;\n"
        "  };\n"
     << "  push(i0, j0, " << AX.id << ");\n"
-    << "  for (int rd = 0; rd < wr; ++rd) {\n"
-       "    const bool near = wr - rd <= 1024;  // not yet overwritten in the ring\n"
-       "    const int i = near ? qi_[rd & 1023] : tr[rd].i, j = near ? qj_[rd & 1023] : tr[rd].j;\n"
-       "    const int tid = near ? qt_[rd & 1023] : tr[rd].rule;\n"
-       "    const long long x = idx_(z, i, j);\n"
-       "    int r = -1, p0 = 0, p1 = 0;\n"
-       "    switch (tid) {\n";
+    // the records queued so far are resolved a batch at a time: lane b
+    // loads record rd + b's rule and splits (all in flight together), then
+    // the walk takes them in queue order from the lanes; records queued
+    // meanwhile wait for the next batch, so the order is the same
+    << "  for (int rd = 0; rd < wr;) {\n"
+       // lane 0's queue entries: the ring in LDS is ordered by the wave's
+       // own LDS operations; entries read back from the trace in memory
+       // wait for lane 0's stores
+       "    if (wr - rd > 1024 - 64) __syncthreads();\n"
+       "    const int nb = min(64, wr - rd);\n"
+       "    int li = 0, lj = 0, lr = -1, lp0 = 0, lp1 = 0;\n"
+       "    if (lane < nb) {\n"
+       "      const int e = rd + lane;\n"
+       "      const bool near = wr - e <= 1024;  // not yet overwritten in the ring\n"
+       "      li = near ? qi_[e & 1023] : tr[e].i;\n"
+       "      lj = near ? qj_[e & 1023] : tr[e].j;\n"
+       "      const int tid = near ? qt_[e & 1023] : tr[e].rule;\n"
+       "      const long long x = idx_(z, li, lj);\n"
+       "      switch (tid) {\n";
   for (int t : m.order) {
     const Tab& T = m.tabs[t];
-    s << "      case " << T.id << ": r = T.r[" << t << "][x];";
-    for (int q = 0; q < T.cat; ++q) s << " p" << q << " = T.p[" << t << "][" << q << "][x];";
+    s << "        case " << T.id << ": lr = T.r[" << t << "][x];";
+    for (int q = 0; q < T.cat; ++q) s << " lp" << q << " = T.p[" << t << "][" << q << "][x];";
     s << " break;\n";
   }
-  s << "    }\n"
-       "    tr[rd].rule = r; tr[rd].pos[0] = p0; tr[rd].pos[1] = p1;\n"
-       "    (void)p0; (void)p1;\n"
-       "    switch (r) {\n";
+  s << "      }\n"
+       "    }\n"
+       "    for (int b = 0; b < nb; ++b, ++rd) {\n"
+       "      const int i = __builtin_amdgcn_readlane(li, b), j = __builtin_amdgcn_readlane(lj, b);\n"
+       "      const int r = __builtin_amdgcn_readlane(lr, b);\n"
+       "      const int p0 = __builtin_amdgcn_readlane(lp0, b), p1 = __builtin_amdgcn_readlane(lp1, b);\n"
+       "      if (lane == 0) { tr[rd].rule = r; tr[rd].pos[0] = p0; tr[rd].pos[1] = p1; }\n"
+       "      (void)i; (void)j; (void)p0; (void)p1;\n"
+       "      switch (r) {\n";
   for (int t : m.order)
     for (auto& o : alts[t]) {
-      s << "      case " << o.rule << ": {";
+      s << "        case " << o.rule << ": {";
       for (size_t q = 0; q < o.var_of_pos.size(); ++q)
         if (!o.var_of_pos[q].empty()) s << " const int " << o.var_of_pos[q] << " = p" << q << ";";
       for (auto& kd : o.kids) s << " push(" << kd.a << ", " << kd.b << ", " << m.tabs[kd.tab].id << ");";
       s << " break; }\n";
     }
-  s << "    }\n"
+  s << "      }\n"
+       "    }\n"
        "  }\n"
-       "  out[1] = over ? -1 : wr;\n"
+       "  if (lane == 0) out[1] = over ? -1 : wr;\n"
        "}\n";
   src = s.str();
   return true;
@@ -3224,7 +3246,7 @@ int parse_batch(Grammar* g, hipStream_t s, int count, const char* in1, const int
   }
   if (!chk(hipEventRecord(g->ev[1], s), "event")) return DPX_EHIP;
   void* fargs[] = {(void*)&d_inst};
-  if (!chk(hipModuleLaunchKernel(g->finish, (unsigned)count, 1, 1, 1, 1, 1, 0, s, fargs, nullptr), "finish launch")) return DPX_EHIP;
+  if (!chk(hipModuleLaunchKernel(g->finish, (unsigned)count, 1, 1, 64, 1, 1, 0, s, fargs, nullptr), "finish launch")) return DPX_EHIP;
   for (int q = 0; q < count; ++q)
     if (!chk(hipMemcpyAsync(h_res + (ans4 + 16) * q, dev + qs[q].o_ans, ans4 + 16, hipMemcpyDeviceToHost, s), "result copy")) return DPX_EHIP;
   if (!chk(hipStreamSynchronize(s), "grammar run")) return DPX_EHIP;
