@@ -385,7 +385,9 @@ class CompiledGrammar:
 
     def parse_batch(self, inputs, trace=True, cap_per=None):
         """[(answer, found, trace)] for a list of inputs (pairs for two-track
-        grammars), all instances in one launch per diagonal."""
+        grammars), all instances in the same launches: one tile-queue launch
+        (one track), one band-pipeline launch (two tracks without split
+        loops) or one launch per anti-diagonal (two tracks with split loops)."""
         g = self.g
         two = g.tracks == 2
         b1 = [g.pack(x[0] if two else x) for x in inputs]
